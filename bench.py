@@ -1,0 +1,157 @@
+"""Benchmark: Ne RHF (30 STOs, 30,200-point grid) SCF on one MI355X.
+
+One step = one full SCF solve on inputs already resident in HBM: STO basis
+tabulation, the fused kernel-generation + contraction F = Kw . P (every
+(r1, r2) grid pair, 1/r12 kernel), and the RHF iterations to convergence.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x3|fp32|fp64]
+
+Prints one JSON line.  `value` is grid-pair contractions per second over the
+whole step (pairs x kernel components / step time); the FP64 drop-in path is
+timed beside it for the speed-up and the energy deviation, and the FP64 CPU
+oracle (OpenMP, this host's cores) is timed on a bounded slice of the same
+contraction for `cpu_baseline`.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SCF wall time + grid-pair contractions/s: mixed-prec MFMA vs FP64 GPU @1e-6 Eh"
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32", "fp64"])
+    p.add_argument("--fp64-steps", type=int, default=2)
+    p.add_argument("--tol", type=float, default=1e-9)
+    p.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU-baseline slice (0 = auto)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; the multi-GPU design is described in DESIGN.md "
+                                   "(r1-row sharding, one all-reduce of nb^2 values per SCF iteration)",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    import tcscf_amd as tc
+    from tcscf_amd import configs
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    cfg = configs.ne_config2()
+    nrad, order, kind = cfg["grid"]
+    basis = cfg["basis"]
+    nb = len(basis)
+    npair = tc.npair(nb)
+    grid = tc.QMCGrid.radial_lebedev(nrad, order, kind, 1.0, 1.0)
+    S, H = tc.one_electron_matrices(basis, cfg["Z"])
+    n = grid.nGrid()
+    pairs = n * n - n  # coincident pairs excluded
+    stream = torch.cuda.current_stream()
+
+    def step(precision, ev=None):
+        grid.setBasisFunctions(basis, False)
+        if ev:
+            ev[0].record(stream)
+        F = tc.computeF(grid, grid, precision=precision)
+        if ev:
+            ev[1].record(stream)
+        hf = tc.RCSHartreeFock(cfg["nup"] + cfg["ndown"], nb)
+        e = hf.compute(S, H, grid, F, True, tol=a.tol, max_iter=100)
+        return e, hf
+
+    def run(precision, steps, warmup):
+        for _ in range(warmup):
+            step(precision)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        out = None
+        for k in range(steps):
+            out = step(precision, evs[k])
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        kern_ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / steps
+        return t, kern_ms, out
+
+    t_mix, k_mix, (e_mix, hf_mix) = run(a.precision, a.steps, a.warmup)
+    if a.precision != "fp64":
+        t64, k64, (e64, hf64) = run("fp64", a.fp64_steps, 1)
+    else:
+        t64, k64, e64, hf64 = t_mix, k_mix, e_mix, hf_mix
+
+    flops = 2.0 * npair * pairs  # algorithmic: one MAC per pair density per grid pair
+    achieved = flops / (k_mix * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[a.precision]
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None,
+                "kernel": "fgemm (fused 1/r12 generation + contraction), computeF wall incl. packing",
+                "kernel_ms": round(k_mix, 4)}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        O.set_threads(os.cpu_count() or 1)
+        threads = O.get_threads()
+        rows = a.cpu_rows or max(8, min(512, 4 * threads))
+        pts = grid.points_host
+        A = grid.basisValues.cpu().numpy()
+        t0 = time.perf_counter()
+        O.compute_F(pts, pts, grid.weights_host, A, rows=(0, rows))
+        tc_cpu = time.perf_counter() - t0
+        cpu = {"value": rows * n / tc_cpu, "unit": "grid-pair contractions/s", "cores": threads, "kind": "port",
+               "sample": f"oracle computeF (FP64, OpenMP) on r1 rows 0..{rows} x all {n} r2 points, nb={nb}: "
+                         f"{tc_cpu:.2f} s; full precompute extrapolates to {tc_cpu * n / rows:.0f} s"}
+
+    res = {
+        "metric": METRIC,
+        "value": pairs / t_mix,
+        "unit": "grid-pair contractions/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_mix * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16x3": "bf16 hi/lo split x3, f32 MFMA accumulate, f64 cross-chunk + f64 SCF",
+                  "fp32": "f32 MFMA, f64 cross-chunk + f64 SCF", "fp64": "f64"}[a.precision],
+        "data": "synthetic (even-tempered STO basis, deterministic radial x Lebedev grid)",
+        "config": {"workload": f"Ne RHF, {nb} STOs (13s,4p,1d even-tempered 0.3*1.8^k), {n}-point grid "
+                               f"({nrad} Gauss-Chebyshev/M4 radial x Lebedev-{order}), 1/r12, tol {a.tol:g}",
+                   "precision": a.precision, "grid_points": n, "nbasis": nb, "npair": npair},
+        "scf_wall_ms": t_mix * 1e3,
+        "scf_iterations": hf_mix.numberOfConvergenceLoops(),
+        "energy": e_mix,
+        "fp64_scf_wall_ms": t64 * 1e3,
+        "fp64_energy": e64,
+        "energy_dev_vs_fp64": e_mix - e64,
+        "speedup_vs_fp64": t64 / t_mix,
+        "computeF_ms": {a.precision: k_mix, "fp64": k64},
+        "sweep_ms_per_scf": hf_mix.t_sweep_ms,
+        "solve_ms_per_scf": hf_mix.t_solve_ms,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

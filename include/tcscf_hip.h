@@ -1,0 +1,157 @@
+/*
+ * tcscf_hip.h -- C ABI of the MI355X (gfx950) grid-integral library.
+ *
+ * The reference (corbett5/tcscf) exposes this path as templated C++ calls
+ * (src/HartreeFock.hpp, src/integration/integrateAll.hpp); lambdas cannot cross
+ * an ABI, so kernels become an enum plus parameters.  Each entry point names the
+ * reference interface it replaces.
+ *
+ * Conventions
+ *   - Return 0 on success, a negative tcscf_status otherwise; tcscf_last_error()
+ *     gives the message of the calling thread's last failure.  No exceptions
+ *     cross the ABI.
+ *   - Pointers documented "dev" are device (HBM) pointers; "host" are host memory.
+ *   - Grid points are SoA 3 x n (x row, y row, z row); a 2-D grid of the
+ *     reference (QMCGrid<REAL,2>, points in the xz-plane) is passed with y == 0
+ *     and scale = 2*pi.  Weights carry every Jacobian (r^2 sin(theta) ...).
+ *   - Basis tables are row-major n x nb; gradients n x nb x 3.
+ *   - Pair-packed tensors: p(j,i) = i*(i+1)/2 + j for j <= i, npair = nb(nb+1)/2.
+ *     F_packed[n1][npair] holds F(r1, j, i) = F(r1, i, j) (real basis).
+ *   - `stream` is a hipStream_t (NULL = default stream).  Entry points are
+ *     asynchronous unless documented otherwise; calls on one stream are ordered.
+ */
+#ifndef TCSCF_HIP_H
+#define TCSCF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum tcscf_status {
+  TCSCF_OK = 0,
+  TCSCF_EINVAL = -1,   /* bad argument / shape (reference: LVARRAY_ERROR_IF_NE abort) */
+  TCSCF_EHIP = -2,     /* HIP runtime error */
+  TCSCF_ENOTCONV = -3, /* SCF did not converge (reference: ConvergenceError) */
+  TCSCF_ESOLVER = -4,  /* eigen-solver failure (reference: LvArray::dense::Error) */
+  TCSCF_ENOMEM = -5
+};
+
+enum tcscf_kernel {
+  TCSCF_K_COULOMB = 0,      /* 1/r12 */
+  TCSCF_K_TC_SCALAR = 1,    /* 2/r12 - |grad1 u(r1,r2)|^2 - |grad1 u(r2,r1)|^2 */
+  TCSCF_K_JASTROW_GRAD = 2  /* grad1 u(r1,r2), 3 components, for c = 1 */
+};
+
+enum tcscf_precision {
+  TCSCF_FP64 = 0,   /* v_mfma_f64_16x16x4_f64, FP64 kernel generation */
+  TCSCF_FP32 = 1,   /* v_mfma_f32_16x16x4_f32, FP64 cross-chunk sums */
+  TCSCF_BF16X3 = 2  /* bf16 hi/lo split, hi*hi + hi*lo + lo*hi on
+                       v_mfma_f32_16x16x32_bf16, FP64 cross-chunk sums */
+};
+
+enum tcscf_dtype { TCSCF_F64 = 0, TCSCF_F32 = 1 };
+
+enum tcscf_radial_map {
+  TCSCF_RADIAL_M4 = 0,        /* Treutler-Ahlrichs M4(a, eps) */
+  TCSCF_RADIAL_BECKE = 1,     /* r = (1+x)/(1-x) */
+  TCSCF_RADIAL_XOVER1MX = 2   /* r = x/(1-x) */
+};
+
+/* One Slater-type orbital N r^(n-1) e^(-zeta r) S_lm (real harmonics, l <= 3). */
+typedef struct tcscf_sto {
+  int n, l, m;
+  double zeta;
+} tcscf_sto;
+
+/* Jastrow u(r12) = c * r12 / (r12 + a12): the r12-only term of the reference's
+ * Ochi form (src/jastrowFunctions.hpp:9-115) with S = {(1,0,0)}. */
+typedef struct tcscf_jastrow {
+  double a12;
+  double c_same;
+  double c_oppo;
+} tcscf_jastrow;
+
+typedef struct tcscf_scf_opts {
+  double tol;     /* |E_new - E_old| < tol ends the loop (reference: 1e-5) */
+  int max_iter;   /* reference: 50 RCS, 100 UOS, 200 TC */
+  int mask;       /* respectOneElectronSymmetry: zero G where H == 0 */
+  int mgs_mode;   /* TC: 0 = reference MGS (Euclidean norm for i >= 1), 1 = metric */
+} tcscf_scf_opts;
+
+typedef struct tcscf_scf_result {
+  double energy;
+  int iterations;   /* numberOfConvergenceLoops() */
+  int status;       /* TCSCF_OK or TCSCF_ENOTCONV / TCSCF_ESOLVER */
+  double t_sweep_ms;  /* time in the per-iteration device sweeps */
+  double t_solve_ms;  /* time in the FP64 eigen-solves and density updates */
+} tcscf_scf_result;
+
+/* ---------------------------------------------------------------- misc */
+const char* tcscf_last_error(void);
+int tcscf_version(void);
+
+/* ---------------------------------------------------------------- host grids */
+/* ChebyshevGauss + createGrid(map): src/integration/ChebyshevGauss.hpp:10-48,
+ * src/integration/changeOfVariables.hpp:100-162, src/integration/quadrature.hpp:76-102 */
+int tcscf_radial_grid(int n, int map, double a, double eps, double* r_host, double* w_host);
+/* getLebedevGrid: src/integration/Lebedev.cpp:23-41 ({theta, phi, w}, sum w = 4 pi) */
+int tcscf_lebedev_npts(int order);
+int tcscf_lebedev(int order, double* theta_host, double* phi_host, double* w_host);
+/* TreutlerAhlrichsLebedev packed into the QMCGrid shape
+ * (src/integration/TreutlerAhlrichsLebedev.hpp:19-73, integrateAll.hpp:41-64);
+ * point k = ia*nrad + ir, weight = w_r * w_ang * r^2.  Returns n or < 0. */
+int tcscf_product_grid(int nrad, int map, double a, double eps, int order, double* pts_host, double* w_host);
+
+/* ---------------------------------------------------------------- host one-electron */
+/* overlap / coreMatrixElement: src/SlaterTypeOrbital.hpp:125-171.  nb x nb row-major. */
+int tcscf_sto_overlap_core(const tcscf_sto* fns, int nb, int Z, double* S_host, double* H_host);
+
+/* ---------------------------------------------------------------- device basis */
+/* QMCGrid::setBasisFunctions: src/integration/integrateAll.hpp:88-137.
+ * values_dev: n x nb; grads_dev: n x nb x 3 or NULL. */
+int tcscf_basis_eval(const double* pts_dev, int64_t n, const tcscf_sto* fns_host, int nb, double* values_dev,
+                     double* grads_dev, void* stream);
+
+/* ---------------------------------------------------------------- device F / V */
+/* precomputeIntegrand + computeF / computeV fused: src/HartreeFock.hpp:23-229.
+ *   F[n1, c, p] = scale * sum_n2 K_c(r1, r2) w2[n2] A2[n2, j] A2[n2, i]
+ * with K generated per tile and never stored; coincident pairs (r12 == 0) give 0.
+ * ncomp = 3 for TCSCF_K_JASTROW_GRAD (layout F[n1][3][npair]), else 1
+ * (layout F[n1][npair]).  For TCSCF_K_TC_SCALAR, same_spin selects c_same or
+ * c_oppo; TCSCF_K_JASTROW_GRAD uses c = 1 (scale by c_same / c_oppo).
+ * out_dtype: TCSCF_F64 or TCSCF_F32.  Allocates and caches a device workspace
+ * for the packed pair densities (see tcscf_F_workspace_bytes). */
+int tcscf_compute_F(const double* pts1_dev, int64_t n1, const double* pts2_dev, const double* w2_dev, int64_t n2,
+                    const double* A2_dev, int nb, int kernel, const tcscf_jastrow* jastrow, int same_spin,
+                    int precision, double scale, void* F_dev, int out_dtype, void* stream);
+size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
+
+/* ---------------------------------------------------------------- SCF drivers */
+/* RCSHartreeFock::compute (src/HartreeFock.cpp:785-886).  S, H host nb x nb
+ * row-major; A1_dev n1 x nb; w1_dev n1; F_dev packed n1 x npair (f_dtype).
+ * Synchronous.  energy_trace (max_iter), D_out (nb x nb), evals_out (nb) may be NULL. */
+int tcscf_rcs_scf(int nelec, int nb, const double* S, const double* H, const double* A1_dev, const double* w1_dev,
+                  int64_t n1, const void* F_dev, int f_dtype, const tcscf_scf_opts* opts, tcscf_scf_result* res,
+                  double* energy_trace, double* D_out, double* evals_out, void* stream);
+
+/* UOSHartreeFock::compute (src/HartreeFock.cpp:890-1003).  D_out 2 x nb x nb,
+ * evals_out 2 x nb. */
+int tcscf_uos_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1_dev,
+                  const double* w1_dev, int64_t n1, const void* F_dev, int f_dtype, const tcscf_scf_opts* opts,
+                  tcscf_scf_result* res, double* energy_trace, double* D_out, double* evals_out, void* stream);
+
+/* ---------------------------------------------------------------- diagnostics */
+/* No reference counterpart: runs one MFMA fragment (which = 0: f64 16x16x4,
+ * 1: f32 16x16x4, 2: bf16 16x16x32) on A_dev (16 x K), B_dev (K x 16), writing
+ * D_dev (16 x 16), all row-major FP64, so tests can pin the fragment layouts
+ * the contraction kernels rely on. */
+int tcscf_selftest_mfma(int which, const double* A_dev, const double* B_dev, double* D_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TCSCF_HIP_H */

@@ -1,0 +1,263 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/tcscf_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py for its CPU-baseline leg.  The product (tcscf_amd)
+never imports this module.
+
+All arrays are numpy float64, C-contiguous.  Grid points are SoA (3, N).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libtcscf_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class ScfOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("mask", ctypes.c_int),
+                ("mgs_mode", ctypes.c_int)]
+
+
+class ScfOut(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("iterations", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def chebyshev_gauss(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_chebyshev_gauss(n, _p(x), _p(w))
+    return x, w
+
+
+RADIAL_M4, RADIAL_BECKE, RADIAL_XOVER1MX = 0, 1, 2
+
+
+def radial_grid(n, kind=RADIAL_BECKE, a=1.0, eps=1.0):
+    r = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_radial_grid(n, kind, ctypes.c_double(a), ctypes.c_double(eps), _p(r), _p(w))
+    return r, w
+
+
+def lebedev(order):
+    n = lib().orc_lebedev_npts(order)
+    if n < 0:
+        raise ValueError(f"Lebedev order {order} not available")
+    t, p, w = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_lebedev(order, _p(t), _p(p), _p(w))
+    return t, p, w
+
+
+def product_grid(nrad, order, kind=RADIAL_BECKE, a=1.0, eps=1.0):
+    n = nrad * lib().orc_lebedev_npts(order)
+    pts = np.zeros((3, n))
+    w = np.zeros(n)
+    rc = lib().orc_product_grid(nrad, kind, ctypes.c_double(a), ctypes.c_double(eps), order, _p(pts), _p(w))
+    if rc < 0:
+        raise ValueError("bad Lebedev order")
+    return pts, w
+
+
+def _fns(fns):
+    return np.ascontiguousarray(np.asarray(fns, dtype=np.float64).reshape(-1, 4))
+
+
+def sto_complex(pts, fns, grads=False):
+    f = _fns(fns)
+    n = pts.shape[1]
+    v = np.zeros((n, len(f), 2))
+    g = np.zeros((n, len(f), 3, 2)) if grads else None
+    lib().orc_sto_complex(_p(np.ascontiguousarray(pts)), ctypes.c_int64(n), _p(f), len(f), _p(v), _p(g))
+    vc = v[..., 0] + 1j * v[..., 1]
+    return (vc, g[..., 0] + 1j * g[..., 1]) if grads else vc
+
+
+def sto_real(pts, fns, grads=False):
+    f = _fns(fns)
+    n = pts.shape[1]
+    v = np.zeros((n, len(f)))
+    g = np.zeros((n, len(f), 3)) if grads else None
+    lib().orc_sto_real(_p(np.ascontiguousarray(pts)), ctypes.c_int64(n), _p(f), len(f), _p(v), _p(g))
+    return (v, g) if grads else v
+
+
+def sto_S_H(fns, Z):
+    f = _fns(fns)
+    nb = len(f)
+    S = np.zeros((nb, nb))
+    H = np.zeros((nb, nb))
+    lib().orc_sto_S_H(_p(f), nb, int(Z), _p(S), _p(H))
+    return S, H
+
+
+def ochi_params(a, a12, terms):
+    """terms: list of (i, j, k, c_oppo, c_same)."""
+    out = [a, a12, len(terms)]
+    for t in terms:
+        out += list(map(float, t))
+    return np.array(out, dtype=np.float64)
+
+
+def ochi_value(prm, r1, r2, r12, same):
+    f = lib().orc_ochi_value
+    f.restype = ctypes.c_double
+    return f(_p(prm), ctypes.c_double(r1), ctypes.c_double(r2), ctypes.c_double(r12), int(same))
+
+
+def ochi_gradient(prm, r1, r2, same):
+    g = np.zeros(3)
+    lib().orc_ochi_gradient(_p(prm), _p(np.asarray(r1, dtype=np.float64)), _p(np.asarray(r2, dtype=np.float64)),
+                            int(same), _p(g))
+    return g
+
+
+K_COULOMB, K_TC_SCALAR, K_JASTROW_GRAD = 0, 1, 2
+
+
+def compute_F(pts1, pts2, w2, A2, kernel=K_COULOMB, prm=None, same=True, scale=1.0, rows=None):
+    """Full F: (n1 rows, nb, nb) or (n1 rows, nb, nb, 3) for the vector kernel."""
+    n1 = pts1.shape[1]
+    n2 = pts2.shape[1]
+    nb = A2.shape[1]
+    r0, r1 = (0, n1) if rows is None else rows
+    ncomp = 3 if kernel == K_JASTROW_GRAD else 1
+    F = np.zeros((r1 - r0, nb, nb, ncomp))
+    if prm is None:
+        prm = ochi_params(1.0, 1.0, [])
+    lib().orc_compute_F(kernel, _p(prm), int(same), _p(np.ascontiguousarray(pts1)), ctypes.c_int64(n1),
+                        ctypes.c_int64(r0), ctypes.c_int64(r1), _p(np.ascontiguousarray(pts2)), _p(w2),
+                        ctypes.c_int64(n2), _p(np.ascontiguousarray(A2)), nb, ctypes.c_double(scale), _p(F))
+    return F if ncomp == 3 else F[..., 0]
+
+
+def compute_F_complex(pts1, pts2, w2, A2c):
+    n1, n2, nb = pts1.shape[1], pts2.shape[1], A2c.shape[1]
+    a = np.ascontiguousarray(np.stack([A2c.real, A2c.imag], axis=-1))
+    F = np.zeros((n1, nb, nb, 2))
+    lib().orc_compute_F_complex(_p(np.ascontiguousarray(pts1)), ctypes.c_int64(n1), _p(np.ascontiguousarray(pts2)),
+                                _p(w2), ctypes.c_int64(n2), _p(a), nb, _p(F))
+    return F[..., 0] + 1j * F[..., 1]
+
+
+def eri_from_F(A1, w1, F):
+    nb = A1.shape[1]
+    out = np.zeros((nb, nb, nb, nb))
+    lib().orc_eri_from_F(_p(np.ascontiguousarray(A1)), _p(w1), ctypes.c_int64(A1.shape[0]),
+                         _p(np.ascontiguousarray(F)), nb, _p(out))
+    return out
+
+
+def three_electron(A1, w1, V):
+    nb = A1.shape[1]
+    Q = np.zeros((nb,) * 6)
+    lib().orc_three_electron(_p(np.ascontiguousarray(A1)), _p(w1), ctypes.c_int64(A1.shape[0]),
+                             _p(np.ascontiguousarray(V)), nb, _p(Q))
+    return Q
+
+
+def gen_sym_eig(F, S):
+    n = F.shape[0]
+    e = np.zeros(n)
+    C = np.zeros((n, n))
+    rc = lib().orc_gen_sym_eig(n, _p(np.ascontiguousarray(F)), _p(np.ascontiguousarray(S)), _p(e), _p(C))
+    if rc:
+        raise RuntimeError("solver failure")
+    return e, C
+
+
+def gen_nonsym_eig(F, S):
+    n = F.shape[0]
+    e = np.zeros((n, 2))
+    X = np.zeros((n, n, 2))
+    rc = lib().orc_gen_nonsym_eig(n, _p(np.ascontiguousarray(F)), _p(np.ascontiguousarray(S)), _p(e), _p(X))
+    if rc:
+        raise RuntimeError("solver failure")
+    return e[:, 0] + 1j * e[:, 1], X[..., 0] + 1j * X[..., 1]
+
+
+def mgs(vecs, S, mode=0):
+    v = np.ascontiguousarray(np.stack([vecs.real, vecs.imag], axis=-1).astype(np.float64))
+    k, n = vecs.shape
+    lib().orc_mgs(k, n, _p(v), _p(np.ascontiguousarray(S)), int(mode))
+    return v[..., 0] + 1j * v[..., 1]
+
+
+def _opts(tol, max_iter, mask, mgs_mode=0):
+    return ScfOpts(tol, max_iter, int(mask), int(mgs_mode))
+
+
+def rcs_scf(nelec, S, H, A1, w1, F, tol=1e-5, max_iter=50, mask=True):
+    nb = S.shape[0]
+    out = ScfOut()
+    trace = np.zeros(max_iter)
+    D = np.zeros((nb, nb))
+    ev = np.zeros(nb)
+    o = _opts(tol, max_iter, mask)
+    lib().orc_rcs_scf(int(nelec), nb, _p(S), _p(H), _p(np.ascontiguousarray(A1)), _p(w1),
+                      ctypes.c_int64(A1.shape[0]), _p(np.ascontiguousarray(F)), ctypes.byref(o), ctypes.byref(out),
+                      _p(trace), _p(D), _p(ev))
+    return dict(energy=out.energy, iterations=out.iterations, status=out.status,
+                trace=trace[:max(out.iterations, 0)], D=D, evals=ev)
+
+
+def uos_scf(nup, ndown, S, H, A1, w1, F, tol=1e-5, max_iter=100, mask=True):
+    nb = S.shape[0]
+    out = ScfOut()
+    trace = np.zeros(max_iter)
+    D = np.zeros((2, nb, nb))
+    ev = np.zeros((2, nb))
+    o = _opts(tol, max_iter, mask)
+    lib().orc_uos_scf(int(nup), int(ndown), nb, _p(S), _p(H), _p(np.ascontiguousarray(A1)), _p(w1),
+                      ctypes.c_int64(A1.shape[0]), _p(np.ascontiguousarray(F)), ctypes.byref(o), ctypes.byref(out),
+                      _p(trace), _p(D), _p(ev))
+    return dict(energy=out.energy, iterations=out.iterations, status=out.status,
+                trace=trace[:max(out.iterations, 0)], D=D, evals=ev)
+
+
+def tc_scf(nup, ndown, S, H, A1, dA1, w1, Fs, Fo, Vs, Vo, tol=1e-5, max_iter=200, mask=True, mgs_mode=0):
+    nb = S.shape[0]
+    out = ScfOut()
+    trace = np.zeros(max_iter)
+    D = np.zeros((2, nb, nb))
+    ps = np.zeros(nup + ndown)
+    o = _opts(tol, max_iter, mask, mgs_mode)
+    c = np.ascontiguousarray
+    lib().orc_tc_scf(int(nup), int(ndown), nb, _p(S), _p(H), _p(c(A1)), _p(c(dA1)), _p(w1),
+                     ctypes.c_int64(A1.shape[0]), _p(c(Fs)), _p(c(Fo)), _p(c(Vs)), _p(c(Vo)), ctypes.byref(o),
+                     ctypes.byref(out), _p(trace), _p(D), _p(ps))
+    return dict(energy=out.energy, iterations=out.iterations, status=out.status,
+                trace=trace[:max(out.iterations, 0)], D=D, pseudo=ps)

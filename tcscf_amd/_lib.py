@@ -1,0 +1,89 @@
+"""ctypes binding of libtcscf_hip.so (the C ABI declared in include/tcscf_hip.h).
+
+The shared library is built in-tree by tcscf_amd.build; importing this module
+never falls back to anything else: a missing library raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtcscf_hip.so")
+
+# enums (include/tcscf_hip.h)
+OK, EINVAL, EHIP, ENOTCONV, ESOLVER, ENOMEM = 0, -1, -2, -3, -4, -5
+K_COULOMB, K_TC_SCALAR, K_JASTROW_GRAD = 0, 1, 2
+FP64, FP32, BF16X3 = 0, 1, 2
+F64, F32 = 0, 1
+RADIAL_M4, RADIAL_BECKE, RADIAL_XOVER1MX = 0, 1, 2
+
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+
+# every symbol include/tcscf_hip.h declares
+EXPORTS = [
+    "tcscf_last_error", "tcscf_version", "tcscf_radial_grid", "tcscf_lebedev_npts", "tcscf_lebedev",
+    "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
+    "tcscf_F_workspace_bytes", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_selftest_mfma",
+]
+
+
+class TcscfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tcscf error {code}: {msg}")
+        self.code = code
+
+
+class ConvergenceError(TcscfError):
+    """Raised when an SCF loop hits its iteration cap (reference: ConvergenceError)."""
+
+
+class SolverError(TcscfError):
+    """Raised when the generalized eigen-solve fails (reference: LvArray::dense::Error)."""
+
+
+class Sto(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("l", ctypes.c_int), ("m", ctypes.c_int), ("zeta", ctypes.c_double)]
+
+
+class Jastrow(ctypes.Structure):
+    _fields_ = [("a12", ctypes.c_double), ("c_same", ctypes.c_double), ("c_oppo", ctypes.c_double)]
+
+
+class ScfOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("mask", ctypes.c_int),
+                ("mgs_mode", ctypes.c_int)]
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("iterations", ctypes.c_int), ("status", ctypes.c_int),
+                ("t_sweep_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `python -m tcscf_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.tcscf_last_error.restype = ctypes.c_char_p
+        L.tcscf_F_workspace_bytes.restype = ctypes.c_size_t
+        L.tcscf_F_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().tcscf_last_error().decode()
+
+
+def check(rc):
+    if rc >= 0:
+        return rc
+    msg = last_error()
+    if rc == ENOTCONV:
+        raise ConvergenceError(rc, msg)
+    if rc == ESOLVER:
+        raise SolverError(rc, msg)
+    raise TcscfError(rc, msg)

@@ -1,0 +1,371 @@
+"""Host-side mirror of the reference's call surface for the grid-integral path.
+
+Names, argument meaning and error behaviour follow corbett5/tcscf:
+  integration::QMCGrid              src/integration/integrateAll.hpp:12-142
+  SlaterTypeOrbital, overlap, ...   src/SlaterTypeOrbital.hpp:15-171
+  computeF / computeV               src/HartreeFock.hpp:88-229
+  RCS/UOS/TCHartreeFock::compute    src/HartreeFock.hpp:233-418, src/HartreeFock.cpp
+Device memory is held in torch tensors (plumbing only); every computation runs
+in libtcscf_hip.so through the C ABI.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("tcscf_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _np(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def npair(nb):
+    return nb * (nb + 1) // 2
+
+
+def pair_index(j, i):
+    """Packed index of (j, i); symmetric in its arguments."""
+    if j > i:
+        j, i = i, j
+    return i * (i + 1) // 2 + j
+
+
+def unpack(Fp, nb):
+    """Packed (..., npair) -> full (..., nb, nb) (numpy or torch)."""
+    idx = np.array([[pair_index(j, i) for i in range(nb)] for j in range(nb)])
+    if isinstance(Fp, np.ndarray):
+        return Fp[..., idx]
+    torch = _torch()
+    return Fp[..., torch.as_tensor(idx, device=Fp.device)]
+
+
+# --------------------------------------------------------------------------- basis
+class SlaterTypeOrbital:
+    """N r^(n-1) e^(-alpha r) S_lm with real solid harmonics S_lm (l <= 3)."""
+
+    def __init__(self, alpha, n, l, m):
+        if n < 1 or n < l or l < abs(m) or l > 3:
+            raise ValueError(f"invalid STO (n={n}, l={l}, m={m})")
+        self.alpha, self.n, self.l, self.m = float(alpha), int(n), int(l), int(m)
+
+    @property
+    def normalization(self):
+        a, n = self.alpha, self.n
+        return (2 * a) ** n * math.sqrt(2 * a / math.factorial(2 * n))
+
+    def as_tuple(self):
+        return (self.n, self.l, self.m, self.alpha)
+
+    def __repr__(self):
+        return f"STO(n={self.n}, l={self.l}, m={self.m}, zeta={self.alpha:g})"
+
+
+def _sto_array(basis):
+    arr = (_lib.Sto * len(basis))()
+    for k, b in enumerate(basis):
+        arr[k].n, arr[k].l, arr[k].m, arr[k].zeta = b.n, b.l, b.m, b.alpha
+    return arr
+
+
+def even_tempered(shells, zeta0, ratio):
+    """Even-tempered STOs: shells = {l: count}; zeta_k = zeta0 * ratio**k per l,
+    n = l + 1, m = -l..l (all m of one exponent adjacent)."""
+    basis = []
+    for l in sorted(shells):
+        for k in range(shells[l]):
+            for m in range(-l, l + 1):
+                basis.append(SlaterTypeOrbital(zeta0 * ratio ** k, l + 1, l, m))
+    return basis
+
+
+def one_electron_matrices(basis, Z):
+    """Analytic overlap S and core Hamiltonian H (src/SlaterTypeOrbital.hpp:125-171)."""
+    nb = len(basis)
+    S = np.zeros((nb, nb))
+    H = np.zeros((nb, nb))
+    check(lib().tcscf_sto_overlap_core(_sto_array(basis), nb, int(Z), S.ctypes.data_as(_dp), H.ctypes.data_as(_dp)))
+    return S, H
+
+
+def overlap(b1, b2):
+    return one_electron_matrices([b1, b2], 0)[0][0, 1]
+
+
+def coreMatrixElement(Z, b1, b2):
+    return one_electron_matrices([b1, b2], Z)[1][0, 1]
+
+
+# --------------------------------------------------------------------------- grids
+def radial_grid(n, radial_map=_lib.RADIAL_BECKE, a=1.0, eps=1.0):
+    r = np.zeros(n)
+    w = np.zeros(n)
+    check(lib().tcscf_radial_grid(n, radial_map, ctypes.c_double(a), ctypes.c_double(eps), r.ctypes.data_as(_dp),
+                                  w.ctypes.data_as(_dp)))
+    return r, w
+
+
+def lebedev(order):
+    n = check(lib().tcscf_lebedev_npts(order))
+    t, p, w = np.zeros(n), np.zeros(n), np.zeros(n)
+    check(lib().tcscf_lebedev(order, t.ctypes.data_as(_dp), p.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
+    return t, p, w
+
+
+def product_grid(nrad, order, radial_map=_lib.RADIAL_BECKE, a=1.0, eps=1.0):
+    """Radial x Lebedev points (3, N) SoA and weights (N,) incl. r^2, on the host."""
+    n = nrad * check(lib().tcscf_lebedev_npts(order))
+    pts = np.zeros((3, n))
+    w = np.zeros(n)
+    check(lib().tcscf_product_grid(nrad, radial_map, ctypes.c_double(a), ctypes.c_double(eps), order,
+                                   pts.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
+    return pts, w
+
+
+class QuadratureGrid:
+    """points (3, N) and weights (N,), device-resident FP64."""
+
+    def __init__(self, points, weights):
+        self.points = points
+        self.weights = weights
+
+
+class QMCGrid:
+    """Mirror of integration::QMCGrid<double, NDIM>: grid points + weights (with
+    every Jacobian) and the basis tabulated on them, all in HBM.
+
+    ndim == 2 reproduces the reference's r2 grid trick (points in the xz-plane,
+    computeF multiplies by 2 pi, src/HartreeFock.hpp:123-124)."""
+
+    def __init__(self, points, weights, ndim=3):
+        torch = _torch()
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        if pts.ndim != 2 or pts.shape[0] != 3:
+            raise ValueError("points must be (3, N)")
+        if ndim == 2 and np.any(pts[1] != 0):
+            raise ValueError("a 2-D grid keeps y == 0")
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if w.shape != (pts.shape[1],):
+            raise ValueError("weights must be (N,)")
+        dev = _dev()
+        self.ndim = ndim
+        self.quadratureGrid = QuadratureGrid(torch.from_numpy(pts).to(dev), torch.from_numpy(w).to(dev))
+        self.points_host = pts
+        self.weights_host = w
+        self.basisValues = None
+        self.basisGradients = None
+        self.basis = None
+
+    @classmethod
+    def radial_lebedev(cls, nrad, order, radial_map=_lib.RADIAL_BECKE, a=1.0, eps=1.0):
+        pts, w = product_grid(nrad, order, radial_map, a, eps)
+        return cls(pts, w, 3)
+
+    def nGrid(self):
+        return self.quadratureGrid.weights.shape[0]
+
+    def nBasis(self):
+        return 0 if self.basisValues is None else self.basisValues.shape[1]
+
+    def gradientsStored(self):
+        return self.basisGradients is not None
+
+    def setBasisFunctions(self, basisFunctions, storeGradients):
+        """QMCGrid::setBasisFunctions (src/integration/integrateAll.hpp:88-137)."""
+        torch = _torch()
+        n, nb = self.nGrid(), len(basisFunctions)
+        dev = self.quadratureGrid.points.device
+        self.basisValues = torch.empty((n, nb), dtype=torch.float64, device=dev)
+        self.basisGradients = torch.empty((n, nb, 3), dtype=torch.float64, device=dev) if storeGradients else None
+        check(lib().tcscf_basis_eval(_ptr(self.quadratureGrid.points), ctypes.c_int64(n), _sto_array(basisFunctions),
+                                     nb, _ptr(self.basisValues),
+                                     _ptr(self.basisGradients) if storeGradients else None, _stream()))
+        self.basis = list(basisFunctions)
+
+
+# --------------------------------------------------------------------------- F / V
+class Jastrow:
+    """u(r12) = c r12 / (r12 + a12) per spin pair: the r12-only Ochi term
+    (src/jastrowFunctions.hpp:9-115, S = {(1,0,0)}).  BASELINE's
+    u = r12 / (2 (1 + b r12)) is a12 = 1/b, c = a12/2."""
+
+    def __init__(self, a12, c_same, c_oppo):
+        self.a12, self.c_same, self.c_oppo = float(a12), float(c_same), float(c_oppo)
+
+    @classmethod
+    def from_b(cls, b, same_over_oppo=0.5):
+        a12 = 1.0 / b
+        return cls(a12, same_over_oppo * a12 / 2, a12 / 2)
+
+    def c(self, same):
+        return self.c_same if same else self.c_oppo
+
+    def _struct(self):
+        return _lib.Jastrow(self.a12, self.c_same, self.c_oppo)
+
+
+def _compute(r1Grid, r2Grid, kernel, jastrow, same_spin, precision, out_dtype):
+    torch = _torch()
+    if r1Grid.nBasis() != r2Grid.nBasis() or r2Grid.basisValues is None:
+        raise ValueError("computeF: both grids need the same tabulated basis")
+    prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    nb = r2Grid.nBasis()
+    n1, n2 = r1Grid.nGrid(), r2Grid.nGrid()
+    ncomp = 3 if kernel == _lib.K_JASTROW_GRAD else 1
+    dtype = torch.float64 if out_dtype in (torch.float64, "fp64", _lib.F64) else torch.float32
+    shape = (n1, npair(nb)) if ncomp == 1 else (n1, 3, npair(nb))
+    F = torch.empty(shape, dtype=dtype, device=r1Grid.quadratureGrid.points.device)
+    scale = 2 * math.pi if r2Grid.ndim == 2 else 1.0
+    jp = jastrow._struct() if jastrow is not None else None
+    check(lib().tcscf_compute_F(_ptr(r1Grid.quadratureGrid.points), ctypes.c_int64(n1),
+                                _ptr(r2Grid.quadratureGrid.points), _ptr(r2Grid.quadratureGrid.weights),
+                                ctypes.c_int64(n2), _ptr(r2Grid.basisValues), nb, kernel,
+                                ctypes.byref(jp) if jp is not None else None, int(bool(same_spin)), prec,
+                                ctypes.c_double(scale), _ptr(F), _lib.F32 if dtype == torch.float32 else _lib.F64,
+                                _stream()))
+    return F
+
+
+def computeF(r1Grid, r2Grid, kernel="coulomb", jastrow=None, same_spin=True, precision="fp64",
+             out_dtype="fp64"):
+    """precomputeIntegrand + computeF fused (src/HartreeFock.hpp:23-131):
+    packed F[n1, p(j,i)] = scale * sum_n2 K(r1, r2) w2 phi_j(r2) phi_i(r2).
+    kernel: 'coulomb' (1/r12) or 'tc' (2/r12 - |grad u12|^2 - |grad u21|^2)."""
+    k = {"coulomb": _lib.K_COULOMB, "tc": _lib.K_TC_SCALAR}[kernel]
+    return _compute(r1Grid, r2Grid, k, jastrow, same_spin, precision, out_dtype)
+
+
+def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64"):
+    """computeV (src/HartreeFock.hpp:185-229) for c = 1: packed V[n1, xyz, p].
+    V_same = c_same * V, V_oppo = c_oppo * V."""
+    return _compute(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, True, precision, out_dtype)
+
+
+# --------------------------------------------------------------------------- SCF
+def _opts(tol, max_iter, mask, mgs_mode=0):
+    return _lib.ScfOpts(float(tol), int(max_iter), int(bool(mask)), int(mgs_mode))
+
+
+def _f_args(r1Grid, F):
+    torch = _torch()
+    if F.dtype not in (torch.float64, torch.float32):
+        raise TypeError("F must be float64 or float32")
+    if F.shape[0] != r1Grid.nGrid():
+        raise ValueError("F rows must match the r1 grid")
+    return _ptr(F), (_lib.F32 if F.dtype == torch.float32 else _lib.F64)
+
+
+class _HFBase:
+    def numberOfConvergenceLoops(self):
+        return self._iters
+
+    def _finish(self, rc, res, trace):
+        self._iters = res.iterations
+        self.energy = res.energy
+        self.energy_trace = trace[:max(res.iterations, 0)].copy()
+        self.t_sweep_ms = res.t_sweep_ms
+        self.t_solve_ms = res.t_solve_ms
+        check(rc)
+        return res.energy
+
+
+class RCSHartreeFock(_HFBase):
+    """RCSHartreeFock (src/HartreeFock.hpp:233-293, src/HartreeFock.cpp:785-886)."""
+
+    def __init__(self, nElectrons, nBasis):
+        if nElectrons % 2:
+            raise ValueError("RCS needs an even number of electrons")
+        self.nElectrons, self.basisSize = nElectrons, nBasis
+        self.density = np.zeros((1, nBasis, nBasis))
+        self.eigenvalues = np.zeros(nBasis)
+        self._iters = 0
+
+    @staticmethod
+    def needsGradients():
+        return False
+
+    def compute(self, overlap, oneElectronTerms, r1Grid, Fji, respectOneElectronSymmetry=True, tol=1e-5,
+                max_iter=50):
+        nb = self.basisSize
+        S, Sp = _np(overlap)
+        H, Hp = _np(oneElectronTerms)
+        if S.shape != (nb, nb) or H.shape != (nb, nb) or r1Grid.nBasis() != nb:
+            raise ValueError("RCS: shape mismatch")
+        fp, fd = _f_args(r1Grid, Fji)
+        res = _lib.ScfResult()
+        trace = np.zeros(max_iter)
+        D = np.zeros((nb, nb))
+        ev = np.zeros(nb)
+        rc = lib().tcscf_rcs_scf(self.nElectrons, nb, Sp, Hp, _ptr(r1Grid.basisValues),
+                                 _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()), fp, fd,
+                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry)), ctypes.byref(res),
+                                 trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp), _stream())
+        self.density[0] = D
+        self.eigenvalues[:] = ev
+        return self._finish(rc, res, trace)
+
+    def highestOccupiedOrbitalEnergy(self):
+        return self.eigenvalues[self.nElectrons // 2 - 1]
+
+
+class UOSHartreeFock(_HFBase):
+    """UOSHartreeFock (src/HartreeFock.hpp:297-349, src/HartreeFock.cpp:890-1003)."""
+
+    def __init__(self, numSpinUp, numSpinDown, nBasis):
+        self.nElectrons = (numSpinUp, numSpinDown)
+        self.basisSize = nBasis
+        self.density = np.zeros((2, nBasis, nBasis))
+        self.eigenvalues = np.zeros((2, nBasis))
+        self._iters = 0
+
+    @staticmethod
+    def needsGradients():
+        return False
+
+    def compute(self, overlap, oneElectronTerms, r1Grid, Fji, respectOneElectronSymmetry=True, tol=1e-5,
+                max_iter=100):
+        nb = self.basisSize
+        S, Sp = _np(overlap)
+        H, Hp = _np(oneElectronTerms)
+        if S.shape != (nb, nb) or H.shape != (nb, nb) or r1Grid.nBasis() != nb:
+            raise ValueError("UOS: shape mismatch")
+        fp, fd = _f_args(r1Grid, Fji)
+        res = _lib.ScfResult()
+        trace = np.zeros(max_iter)
+        D = np.zeros((2, nb, nb))
+        ev = np.zeros((2, nb))
+        rc = lib().tcscf_uos_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
+                                 _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()), fp, fd,
+                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry)), ctypes.byref(res),
+                                 trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp), _stream())
+        self.density[:] = D
+        self.eigenvalues[:] = ev
+        return self._finish(rc, res, trace)
+
+    def highestOccupiedOrbitalEnergy(self):
+        return max(self.eigenvalues[0, self.nElectrons[0] - 1], self.eigenvalues[1, self.nElectrons[1] - 1])

@@ -1,0 +1,785 @@
+// Device kernels of the grid-integral hot path (gfx950):
+//   - STO basis tabulation (values + Cartesian gradients, real solid harmonics)
+//   - pair-density packing P[n2, p(j,i)] = w2 A2[n2,j] A2[n2,i]
+//   - fused kernel generation + contraction F = Kw . P on MFMA, in FP64,
+//     FP32 and BF16 hi/lo split; Kw(r1, r2) is generated per tile in
+//     registers (FP64/FP32) or LDS (BF16) and never stored.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+
+namespace tc {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+constexpr double kFar = 1.0e6;  // padding coordinate: far from every grid point
+
+/* ------------------------------------------------------------------------ */
+/* scratch buffers                                                           */
+/* ------------------------------------------------------------------------ */
+namespace {
+struct ScratchBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_scratch_mu;
+ScratchBuf g_scratch[64][kNumSlots];
+}  // namespace
+
+void* scratch(int slot, size_t bytes, int* rc) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    set_error("scratch: hipGetDevice failed");
+    *rc = TCSCF_EHIP;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  ScratchBuf& b = g_scratch[dev][slot];
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      // Older work queued on any stream may still read the old buffer.
+      if (hipDeviceSynchronize() != hipSuccess || hipFree(b.ptr) != hipSuccess) {
+        set_error("scratch: release failed");
+        *rc = TCSCF_EHIP;
+        return nullptr;
+      }
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+      b.ptr = nullptr;
+      set_error("scratch: hipMalloc of " + std::to_string(want) + " bytes failed");
+      *rc = TCSCF_ENOMEM;
+      return nullptr;
+    }
+    b.bytes = want;
+  }
+  *rc = 0;
+  return b.ptr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* STO basis tabulation                                                      */
+/* ------------------------------------------------------------------------ */
+constexpr int kMaxFnsPerLaunch = 96;
+
+struct StoTable {
+  int nb;
+  int code[kMaxFnsPerLaunch];      // n | l << 8 | (m + 8) << 16
+  double zeta[kMaxFnsPerLaunch];
+  double norm[kMaxFnsPerLaunch];
+};
+
+// Real solid harmonic S_lm(x, y, z) (homogeneous of degree l, unit-normalised
+// on the sphere) and its gradient.  Standard real harmonics without the
+// Condon-Shortley phase: p_{+1} ~ x, p_{-1} ~ y, p_0 ~ z.
+__device__ inline void solid_harmonic(int l, int m, double x, double y, double z, double* S, double* g) {
+  g[0] = g[1] = g[2] = 0;
+  switch (l) {
+    case 0:
+      *S = 0.28209479177387814;
+      return;
+    case 1: {
+      const double c = 0.48860251190291992;
+      if (m == 1) { *S = c * x; g[0] = c; }
+      else if (m == -1) { *S = c * y; g[1] = c; }
+      else { *S = c * z; g[2] = c; }
+      return;
+    }
+    case 2: {
+      const double c = 1.0925484305920792, c0 = 0.31539156525252005, c2 = 0.54627421529603959;
+      switch (m) {
+        case -2: *S = c * x * y; g[0] = c * y; g[1] = c * x; return;
+        case -1: *S = c * y * z; g[1] = c * z; g[2] = c * y; return;
+        case 0: *S = c0 * (2 * z * z - x * x - y * y); g[0] = -2 * c0 * x; g[1] = -2 * c0 * y; g[2] = 4 * c0 * z; return;
+        case 1: *S = c * x * z; g[0] = c * z; g[2] = c * x; return;
+        default: *S = c2 * (x * x - y * y); g[0] = 2 * c2 * x; g[1] = -2 * c2 * y; return;
+      }
+    }
+    default: {
+      const double c3 = 0.59004358992664352, c2 = 2.8906114426405538, c1 = 0.45704579946446572,
+                   c0 = 0.37317633259011539, c2b = 1.4453057213202769;
+      switch (m) {
+        case -3: *S = c3 * (3 * x * x * y - y * y * y); g[0] = 6 * c3 * x * y; g[1] = 3 * c3 * (x * x - y * y); return;
+        case -2: *S = c2 * x * y * z; g[0] = c2 * y * z; g[1] = c2 * x * z; g[2] = c2 * x * y; return;
+        case -1:
+          *S = c1 * y * (4 * z * z - x * x - y * y);
+          g[0] = -2 * c1 * x * y; g[1] = c1 * (4 * z * z - x * x - 3 * y * y); g[2] = 8 * c1 * y * z;
+          return;
+        case 0:
+          *S = c0 * z * (2 * z * z - 3 * x * x - 3 * y * y);
+          g[0] = -6 * c0 * x * z; g[1] = -6 * c0 * y * z; g[2] = c0 * (6 * z * z - 3 * x * x - 3 * y * y);
+          return;
+        case 1:
+          *S = c1 * x * (4 * z * z - x * x - y * y);
+          g[0] = c1 * (4 * z * z - 3 * x * x - y * y); g[1] = -2 * c1 * x * y; g[2] = 8 * c1 * x * z;
+          return;
+        case 2: *S = c2b * z * (x * x - y * y); g[0] = 2 * c2b * x * z; g[1] = -2 * c2b * y * z; g[2] = c2b * (x * x - y * y); return;
+        default: *S = c3 * (x * x * x - 3 * x * y * y); g[0] = 3 * c3 * (x * x - y * y); g[1] = -6 * c3 * x * y; return;
+      }
+    }
+  }
+}
+
+// phi = N r^k e^{-zeta r} S_lm(x,y,z), k = n-1-l;
+// grad phi = N e^{-zeta r} [ (k r^{k-2} - zeta r^{k-1}) S (x,y,z) + r^k grad S ].
+__global__ __launch_bounds__(256) void basis_eval_kernel(const double* __restrict__ pts, int64_t n, StoTable t,
+                                                         int nb_total, int b0, double* __restrict__ values,
+                                                         double* __restrict__ grads) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * t.nb) return;
+  const int64_t k = idx / t.nb;
+  const int b = (int)(idx - k * t.nb);
+  const double x = pts[k], y = pts[n + k], z = pts[2 * n + k];
+  const double r = sqrt(x * x + y * y + z * z);
+  const int code = t.code[b];
+  const int nq = code & 0xff, l = (code >> 8) & 0xff, m = ((code >> 16) & 0xff) - 8;
+  const int kp = nq - 1 - l;
+  const double zeta = t.zeta[b];
+  double S, gS[3];
+  solid_harmonic(l, m, x, y, z, &S, gS);
+  const double e = t.norm[b] * exp(-zeta * r);
+  const double rk = kp == 0 ? 1.0 : pow(r, (double)kp);
+  const int64_t o = k * nb_total + b0 + b;
+  values[o] = e * rk * S;
+  if (grads) {
+    const double rkm1 = kp == 0 ? 1.0 / r : pow(r, (double)(kp - 1));
+    const double rkm2 = kp == 0 ? 0.0 : pow(r, (double)(kp - 2));
+    const double radial = (kp * rkm2 - zeta * rkm1) * S;
+    grads[3 * o + 0] = e * (radial * x + rk * gS[0]);
+    grads[3 * o + 1] = e * (radial * y + rk * gS[1]);
+    grads[3 * o + 2] = e * (radial * z + rk * gS[2]);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pair densities and padded coordinates                                     */
+/* ------------------------------------------------------------------------ */
+__device__ inline void pair_of(int64_t p, int* j, int* i) {
+  int ii = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(ii + 1) * (ii + 2) / 2 <= p) ++ii;
+  while ((int64_t)ii * (ii + 1) / 2 > p) --ii;
+  *i = ii;
+  *j = (int)(p - (int64_t)ii * (ii + 1) / 2);
+}
+
+// Row-major P[k][p] (ld = ldp), rows k < n2p, zero outside (n2, npair).
+template <class T>
+__global__ __launch_bounds__(256) void pack_pairs_rowmajor(const double* __restrict__ A, const double* __restrict__ w,
+                                                           int64_t n2, int nb, int64_t npair, int64_t n2p, int64_t ldp,
+                                                           T* __restrict__ P) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n2p * ldp) return;
+  const int64_t k = idx / ldp, p = idx - k * ldp;
+  double v = 0;
+  if (k < n2 && p < npair) {
+    int j, i;
+    pair_of(p, &j, &i);
+    v = w[k] * A[k * nb + j] * A[k * nb + i];
+  }
+  P[idx] = (T)v;
+}
+
+__device__ inline unsigned short bf16_bits_rne(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_to_float(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+
+// BF16 hi/lo split of P in MFMA B-fragment order for v_mfma_f32_16x16x32_bf16:
+// fragment (kb, cb) = 64 lanes x 8 bf16, lane l = 16 g + r holds
+// P[32 kb + 8 g + e][16 cb + r], e = 0..7.  hi[] then lo[] (each nkb*ncb*512 bf16).
+__global__ __launch_bounds__(256) void pack_pairs_bf16x2(const double* __restrict__ A, const double* __restrict__ w,
+                                                         int64_t n2, int nb, int64_t npair, int64_t nkb, int64_t ncb,
+                                                         unsigned short* __restrict__ hi,
+                                                         unsigned short* __restrict__ lo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one (fragment, lane, e)
+  const int64_t total = nkb * ncb * 512;
+  if (idx >= total) return;
+  const int64_t frag = idx >> 9;
+  const int within = (int)(idx & 511);
+  const int lane = within >> 3, e = within & 7;
+  const int64_t kb = frag / ncb, cb = frag - kb * ncb;
+  const int g = lane >> 4, r = lane & 15;
+  const int64_t k = 32 * kb + 8 * g + e, p = 16 * cb + r;
+  double v = 0;
+  if (k < n2 && p < npair) {
+    int j, i;
+    pair_of(p, &j, &i);
+    v = w[k] * A[k * nb + j] * A[k * nb + i];
+  }
+  const float vf = (float)v;
+  const unsigned short h = bf16_bits_rne(vf);
+  const float rem = (float)(v - (double)bf16_to_float(h));
+  hi[idx] = h;
+  lo[idx] = bf16_bits_rne(rem);
+}
+
+// Padded SoA coordinates (3 x n2p) in FP64 and as float4 {x, y, z, 0}.
+__global__ void pad_coords(const double* __restrict__ pts, int64_t n, int64_t np, double* __restrict__ out,
+                           float4* __restrict__ out4) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  double x = kFar, y = kFar, z = kFar;
+  if (k < n) {
+    x = pts[k];
+    y = pts[n + k];
+    z = pts[2 * n + k];
+  }
+  out[k] = x;
+  out[np + k] = y;
+  out[2 * np + k] = z;
+  out4[k] = make_float4((float)x, (float)y, (float)z, 0.f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kernel functions K(d), d = r1 - r2; coincident points give 0             */
+/* ------------------------------------------------------------------------ */
+__device__ inline double rsqrt_f64(double s) {
+  double y = __builtin_amdgcn_rsq(s);
+  double e = fma(-s * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-s * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  return y;
+}
+__device__ inline double rsqrt_any(double s) { return rsqrt_f64(s); }
+__device__ inline float rsqrt_any(float s) { return __builtin_amdgcn_rsqf(s); }
+
+struct KernCoulomb {
+  static constexpr int ncomp = 1;
+  template <class T>
+  __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
+    const T s = dx * dx + dy * dy + dz * dz;
+    out[0] = s > T(0) ? rsqrt_any(s) : T(0);
+  }
+};
+
+// 2/r12 - 2 c^2 a12^2 / (r12 + a12)^4  (r12-only Ochi term, c = c_same or c_oppo)
+struct KernTCScalar {
+  static constexpr int ncomp = 1;
+  double a12, c2a2;  // c2a2 = 2 c^2 a12^2
+  template <class T>
+  __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
+    const T s = dx * dx + dy * dy + dz * dz;
+    const T y = rsqrt_any(s);
+    const T t = s * y + T(a12);
+    const T t2 = t * t;
+    out[0] = s > T(0) ? T(2) * y - T(c2a2) / (t2 * t2) : T(0);
+  }
+};
+
+// grad_1 u(r1, r2) = c a12 / (r12 (r12 + a12)^2) (r1 - r2), with c = 1.
+struct KernJastrowGrad {
+  static constexpr int ncomp = 3;
+  double a12;
+  template <class T>
+  __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
+    const T s = dx * dx + dy * dy + dz * dz;
+    const T y = rsqrt_any(s);
+    const T t = s * y + T(a12);
+    const T gg = s > T(0) ? T(a12) * y / (t * t) : T(0);
+    out[0] = gg * dx;
+    out[1] = gg * dy;
+    out[2] = gg * dz;
+  }
+};
+
+/* ------------------------------------------------------------------------ */
+/* FP64 contraction: v_mfma_f64_16x16x4_f64                                  */
+/* A (16x4): lane l holds A[l%16][l/16]; B (4x16): B[l/16][l%16];            */
+/* D (16x16): d4 element i of lane l is D[4(l/16)+i][l%16].                  */
+/* Workgroup = 4 waves as 2 (rows) x 2 (cols); wave tile 16*MB x 64.         */
+/* ------------------------------------------------------------------------ */
+template <int MB, class Kern>
+__global__ __launch_bounds__(256) void fgemm_f64_kernel(const double* __restrict__ p1, int64_t n1,
+                                                        const double* __restrict__ x2, int64_t n2p,
+                                                        const double* __restrict__ P, int64_t ldp,
+                                                        void* __restrict__ Fout, int out_f32, int64_t npair,
+                                                        double scale, Kern kern) {
+  constexpr int NC = Kern::ncomp;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int64_t rowW = (int64_t)blockIdx.y * (32 * MB) + (wave >> 1) * (16 * MB);
+  const int64_t colW = (int64_t)blockIdx.x * 128 + (wave & 1) * 64;
+  double ax[MB], ay[MB], az[MB];
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    const int64_t row = rowW + 16 * m + r;
+    const bool ok = row < n1;
+    ax[m] = ok ? p1[row] : -kFar;
+    ay[m] = ok ? p1[n1 + row] : -kFar;
+    az[m] = ok ? p1[2 * n1 + row] : -kFar;
+  }
+  d4 acc[NC][MB][4];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[c][m][n] = d4{0, 0, 0, 0};
+  const double* __restrict__ y2 = x2 + n2p;
+  const double* __restrict__ z2 = x2 + 2 * n2p;
+  const double* __restrict__ Pb = P + colW + r;
+#pragma unroll 2
+  for (int64_t k0 = 0; k0 < n2p; k0 += 4) {
+    const int64_t k = k0 + g;
+    const double bx = x2[k], by = y2[k], bz = z2[k];
+    double b[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) b[n] = Pb[k * ldp + 16 * n];
+    double a[MB][NC];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) kern(ax[m] - bx, ay[m] - by, az[m] - bz, a[m]);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+          acc[c][m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m][c], b[n], acc[c][m][n], 0, 0, 0);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t row = rowW + 16 * m + 4 * g + i, col = colW + 16 * n + r;
+          if (row < n1 && col < npair) {
+            const int64_t o = (row * NC + c) * npair + col;
+            const double v = scale * acc[c][m][n][i];
+            if (out_f32) ((float*)Fout)[o] = (float)v;
+            else ((double*)Fout)[o] = v;
+          }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP32 contraction: v_mfma_f32_16x16x4_f32 (same fragment layout as FP64),  */
+/* FP32 accumulation inside 64-point chunks, FP64 sums across chunks.        */
+/* ------------------------------------------------------------------------ */
+template <int MB, class Kern>
+__global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict__ p1, int64_t n1,
+                                                        const float4* __restrict__ x2, int64_t n2p,
+                                                        const float* __restrict__ P, int64_t ldp,
+                                                        void* __restrict__ Fout, int out_f32, int64_t npair,
+                                                        double scale, Kern kern) {
+  constexpr int NC = Kern::ncomp;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int64_t rowW = (int64_t)blockIdx.y * (32 * MB) + (wave >> 1) * (16 * MB);
+  const int64_t colW = (int64_t)blockIdx.x * 128 + (wave & 1) * 64;
+  float ax[MB], ay[MB], az[MB];
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    const int64_t row = rowW + 16 * m + r;
+    const bool ok = row < n1;
+    ax[m] = ok ? (float)p1[row] : (float)-kFar;
+    ay[m] = ok ? (float)p1[n1 + row] : (float)-kFar;
+    az[m] = ok ? (float)p1[2 * n1 + row] : (float)-kFar;
+  }
+  f4 acc[NC][MB][4];
+  double sum[NC][MB][4][4];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        acc[c][m][n] = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sum[c][m][n][i] = 0;
+      }
+  const float* __restrict__ Pb = P + colW + r;
+  for (int64_t kc = 0; kc < n2p; kc += 64) {
+    const int64_t kend = kc + 64 < n2p ? kc + 64 : n2p;
+#pragma unroll 2
+    for (int64_t k0 = kc; k0 < kend; k0 += 4) {
+      const int64_t k = k0 + g;
+      const float4 q = x2[k];
+      float b[4];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) b[n] = Pb[k * ldp + 16 * n];
+      float a[MB][NC];
+#pragma unroll
+      for (int m = 0; m < MB; ++m) kern(ax[m] - q.x, ay[m] - q.y, az[m] - q.z, a[m]);
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+          for (int n = 0; n < 4; ++n)
+            acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m][c], b[n], acc[c][m][n], 0, 0, 0);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
+          acc[c][m][n] = f4{0, 0, 0, 0};
+        }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t row = rowW + 16 * m + 4 * g + i, col = colW + 16 * n + r;
+          if (row < n1 && col < npair) {
+            const int64_t o = (row * NC + c) * npair + col;
+            const double v = scale * sum[c][m][n][i];
+            if (out_f32) ((float*)Fout)[o] = (float)v;
+            else ((double*)Fout)[o] = v;
+          }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* BF16 split contraction: v_mfma_f32_16x16x32_bf16                          */
+/* A (16x32): lane l = 16g + r holds A[r][8g .. 8g+7]; B (32x16): lane l     */
+/* holds B[8g .. 8g+7][r]; D as for 16x16x4.                                 */
+/* Workgroup = 4 waves sharing 16*MB rows; wave w owns columns               */
+/* [64 w, 64 w + 64) of the workgroup's 256.  Each k-step (32 points) the    */
+/* workgroup generates its K tile once, split hi/lo, into LDS in A-fragment  */
+/* order; products hi*hi + hi*lo + lo*hi; FP64 sums every 256 points.        */
+/* ------------------------------------------------------------------------ */
+template <int MB, class Kern>
+__global__ __launch_bounds__(256) void fgemm_bf16x3_kernel(const double* __restrict__ p1, int64_t n1,
+                                                           const float4* __restrict__ x2, int64_t n2p,
+                                                           const us8* __restrict__ Phi, const us8* __restrict__ Plo,
+                                                           int64_t ncb, void* __restrict__ Fout, int out_f32,
+                                                           int64_t npair, double scale, Kern kern) {
+  constexpr int NC = Kern::ncomp;
+  constexpr int NB = 4;  // 16-column blocks per wave
+  // K tile in LDS: [NC][MB][hi/lo][64 lanes] of 8 bf16, double buffered.
+  __shared__ us8 ktile[2][NC][MB][2][64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int64_t row0 = (int64_t)blockIdx.y * (16 * MB);
+  const int64_t cb0 = (int64_t)blockIdx.x * 16 + wave * NB;  // first 16-col block of this wave
+
+  // Generation work: NC*MB fragments of 64 lanes x 8 values; thread tid
+  // produces element e0..e0+EPT-1 of fragment-lane (gm, gl).
+  constexpr int VALS = MB * 64 * 8;  // values per component per k-step
+  constexpr int EPT = VALS / 256;    // values per thread per component (MB=4 -> 8)
+  static_assert(EPT >= 1 && VALS % 256 == 0, "MB must be a multiple of 1/2");
+  const int gi = tid * EPT;           // flat index of the first value
+  const int gm = gi / 512;            // m-block
+  const int gl = (gi / 8) & 63;       // fragment lane
+  const int ge = gi & 7;              // first element within the lane's 8
+  const int64_t grow = row0 + 16 * gm + (gl & 15);
+  const int gk = 8 * (gl >> 4) + ge;  // k offset within the 32-point step
+  float ax, ay, az;
+  {
+    const bool ok = grow < n1;
+    ax = ok ? (float)p1[grow] : (float)-kFar;
+    ay = ok ? (float)p1[n1 + grow] : (float)-kFar;
+    az = ok ? (float)p1[2 * n1 + grow] : (float)-kFar;
+  }
+
+  auto generate = [&](int buf, int64_t k0) {
+    unsigned short hv[NC][EPT], lv[NC][EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const float4 q = x2[k0 + gk + e];
+      float v[NC];
+      kern(ax - q.x, ay - q.y, az - q.z, v);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const unsigned short h = bf16_bits_rne(v[c]);
+        hv[c][e] = h;
+        lv[c][e] = bf16_bits_rne(v[c] - bf16_to_float(h));
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      unsigned short* dh = (unsigned short*)&ktile[buf][c][gm][0][gl] + ge;
+      unsigned short* dl = (unsigned short*)&ktile[buf][c][gm][1][gl] + ge;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        dh[e] = hv[c][e];
+        dl[e] = lv[c][e];
+      }
+    }
+  };
+
+  f4 acc[NC][MB][NB];
+  double sum[NC][MB][NB][4];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        acc[c][m][n] = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sum[c][m][n][i] = 0;
+      }
+
+  const int64_t nsteps = n2p / 32;
+  generate(0, 0);
+  __syncthreads();
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const int buf = (int)(s & 1);
+    if (s + 1 < nsteps) generate(buf ^ 1, (s + 1) * 32);
+    us8 bh[NB], bl[NB];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      const int64_t frag = s * ncb + cb0 + n;
+      bh[n] = Phi[frag * 64 + lane];
+      bl[n] = Plo[frag * 64 + lane];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        const bf8 ah = __builtin_bit_cast(bf8, ktile[buf][c][m][0][lane]);
+        const bf8 al = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+          const bf8 h = __builtin_bit_cast(bf8, bh[n]);
+          const bf8 l = __builtin_bit_cast(bf8, bl[n]);
+          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, h, acc[c][m][n], 0, 0, 0);
+          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, l, acc[c][m][n], 0, 0, 0);
+          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, h, acc[c][m][n], 0, 0, 0);
+        }
+      }
+    if ((s & 7) == 7 || s + 1 == nsteps) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+          for (int n = 0; n < NB; ++n) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
+            acc[c][m][n] = f4{0, 0, 0, 0};
+          }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < NB; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t row = row0 + 16 * m + 4 * g + i, col = 16 * (cb0 + n) + r;
+          if (row < n1 && col < npair) {
+            const int64_t o = (row * NC + c) * npair + col;
+            const double v = scale * sum[c][m][n][i];
+            if (out_f32) ((float*)Fout)[o] = (float)v;
+            else ((double*)Fout)[o] = v;
+          }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* MFMA fragment self-test: D = A B for one fragment of each instruction.    */
+/* A, B, D given row-major as FP64 on the host side of the test.             */
+/* ------------------------------------------------------------------------ */
+__global__ void mfma_selftest_kernel(int which, const double* A, const double* B, double* D) {
+  const int l = threadIdx.x, g = l >> 4, r = l & 15;
+  if (which == 0) {
+    d4 c = {0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(A[r * 4 + g], B[g * 16 + r], c, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + r] = c[i];
+  } else if (which == 1) {
+    f4 c = {0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[r * 4 + g], (float)B[g * 16 + r], c, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + r] = c[i];
+  } else {
+    us8 a, b;
+    for (int e = 0; e < 8; ++e) {
+      a[e] = bf16_bits_rne((float)A[r * 32 + 8 * g + e]);
+      b[e] = bf16_bits_rne((float)B[(8 * g + e) * 16 + r]);
+    }
+    f4 c = {0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + r] = c[i];
+  }
+}
+
+}  // namespace tc
+
+/* ======================================================================== */
+/* C ABI                                                                      */
+/* ======================================================================== */
+using namespace tc;
+
+extern "C" size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
+
+namespace {
+
+template <int MB, class Kern>
+int launch_f64(const double* p1, int64_t n1, const double* x2, int64_t n2p, const double* P, int64_t ldp, void* F,
+               int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+  dim3 grid((unsigned)(ldp / 128), (unsigned)((n1 + 32 * MB - 1) / (32 * MB)));
+  hipLaunchKernelGGL((fgemm_f64_kernel<MB, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P, ldp, F, out_f32, npair,
+                     scale, kern);
+  TC_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int MB, class Kern>
+int launch_f32(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const float* P, int64_t ldp, void* F,
+               int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+  dim3 grid((unsigned)(ldp / 128), (unsigned)((n1 + 32 * MB - 1) / (32 * MB)));
+  hipLaunchKernelGGL((fgemm_f32_kernel<MB, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P, ldp, F, out_f32, npair,
+                     scale, kern);
+  TC_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int MB, class Kern>
+int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* Phi, const us8* Plo,
+                int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+  dim3 grid((unsigned)(ncb / 16), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
+  hipLaunchKernelGGL((fgemm_bf16x3_kernel<MB, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, Phi, Plo, ncb, F,
+                     out_f32, npair, scale, kern);
+  TC_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class Kern>
+int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, const double* w2, int64_t n2,
+             const double* A2, int nb, double scale, void* F, int out_f32, Kern kern, hipStream_t st) {
+  const int64_t npair = npair_of(nb);
+  int rc = 0;
+  if (precision == TCSCF_BF16X3) {
+    const int64_t n2p = round_up(n2, 32), ncb = round_up(npair, 256) / 16, nkb = n2p / 32;
+    char* ws = (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
+    if (rc) return rc;
+    unsigned short* hi = (unsigned short*)ws;
+    unsigned short* lo = hi + nkb * ncb * 512;
+    double* xd = (double*)(lo + nkb * ncb * 512);
+    float4* x4 = (float4*)(xd + 3 * n2p);
+    const int64_t tot = nkb * ncb * 512;
+    hipLaunchKernelGGL(pack_pairs_bf16x2, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A2, w2, n2, nb, npair,
+                       nkb, ncb, hi, lo);
+    TC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(pad_coords, dim3((unsigned)((n2p + 255) / 256)), dim3(256), 0, st, pts2, n2, n2p, xd, x4);
+    TC_HIP(hipGetLastError());
+    if constexpr (Kern::ncomp == 1) return launch_bf16<4>(pts1, n1, x4, n2p, (const us8*)hi, (const us8*)lo, ncb, F, out_f32, npair, scale, kern, st);
+    else return launch_bf16<2>(pts1, n1, x4, n2p, (const us8*)hi, (const us8*)lo, ncb, F, out_f32, npair, scale, kern, st);
+  }
+  const int64_t n2p = round_up(n2, 16), ldp = round_up(npair, 128);
+  char* ws = (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
+  if (rc) return rc;
+  const size_t el = precision == TCSCF_FP64 ? 8 : 4;
+  double* xd = (double*)(ws + (size_t)n2p * ldp * el);
+  float4* x4 = (float4*)(xd + 3 * n2p);
+  const int64_t tot = n2p * ldp;
+  hipLaunchKernelGGL(pad_coords, dim3((unsigned)((n2p + 255) / 256)), dim3(256), 0, st, pts2, n2, n2p, xd, x4);
+  TC_HIP(hipGetLastError());
+  if (precision == TCSCF_FP64) {
+    hipLaunchKernelGGL(pack_pairs_rowmajor<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A2, w2, n2,
+                       nb, npair, n2p, ldp, (double*)ws);
+    TC_HIP(hipGetLastError());
+    if constexpr (Kern::ncomp == 1) return launch_f64<4>(pts1, n1, xd, n2p, (const double*)ws, ldp, F, out_f32, npair, scale, kern, st);
+    else return launch_f64<2>(pts1, n1, xd, n2p, (const double*)ws, ldp, F, out_f32, npair, scale, kern, st);
+  }
+  hipLaunchKernelGGL(pack_pairs_rowmajor<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A2, w2, n2, nb,
+                     npair, n2p, ldp, (float*)ws);
+  TC_HIP(hipGetLastError());
+  if constexpr (Kern::ncomp == 1) return launch_f32<4>(pts1, n1, x4, n2p, (const float*)ws, ldp, F, out_f32, npair, scale, kern, st);
+  else return launch_f32<1>(pts1, n1, x4, n2p, (const float*)ws, ldp, F, out_f32, npair, scale, kern, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcscf_basis_eval(const double* pts, int64_t n, const tcscf_sto* fns, int nb, double* values, double* grads,
+                     void* stream) {
+  TC_CHECK(pts && fns && values && n > 0 && nb > 0, "tcscf_basis_eval: bad arguments");
+  for (int b0 = 0; b0 < nb; b0 += kMaxFnsPerLaunch) {
+    StoTable t;
+    t.nb = std::min(kMaxFnsPerLaunch, nb - b0);
+    for (int b = 0; b < t.nb; ++b) {
+      const tcscf_sto& f = fns[b0 + b];
+      TC_CHECK(f.n >= 1 && f.n >= f.l && f.l >= std::abs(f.m) && f.l <= 3 && f.zeta > 0,
+               "tcscf_basis_eval: need n >= 1, n >= l, |m| <= l <= 3, zeta > 0");
+      t.code[b] = f.n | (f.l << 8) | ((f.m + 8) << 16);
+      t.zeta[b] = f.zeta;
+      t.norm[b] = std::pow(2 * f.zeta, f.n) * std::sqrt(2 * f.zeta / std::tgamma(2.0 * f.n + 1));
+    }
+    const int64_t total = n * t.nb;
+    hipLaunchKernelGGL(basis_eval_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       pts, n, t, nb, b0, values, grads);
+    TC_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision) {
+  const int64_t npair = npair_of(nb);
+  if (precision == TCSCF_BF16X3) {
+    const int64_t n2p = round_up(n2, 32), ncb = round_up(npair, 256) / 16;
+    return (size_t)(n2p / 32) * ncb * 512 * 2 * 2 + (size_t)n2p * (3 * 8 + 16) + 4096;
+  }
+  const int64_t n2p = round_up(n2, 16), ldp = round_up(npair, 128);
+  const size_t el = precision == TCSCF_FP64 ? 8 : 4;
+  return (size_t)n2p * ldp * el + (size_t)n2p * (3 * 8 + 16) + 4096;
+}
+
+
+int tcscf_compute_F(const double* pts1, int64_t n1, const double* pts2, const double* w2, int64_t n2,
+                    const double* A2, int nb, int kernel, const tcscf_jastrow* jp, int same_spin, int precision,
+                    double scale, void* F, int out_dtype, void* stream) {
+  TC_CHECK(pts1 && pts2 && w2 && A2 && F && n1 > 0 && n2 > 0 && nb > 0, "tcscf_compute_F: bad arguments");
+  TC_CHECK(precision >= 0 && precision <= 2, "tcscf_compute_F: unknown precision");
+  TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F: unknown out_dtype");
+  TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F: grid too large");
+  hipStream_t st = (hipStream_t)stream;
+  const int of = out_dtype == TCSCF_F32;
+  switch (kernel) {
+    case TCSCF_K_COULOMB:
+      return dispatch(precision, pts1, n1, pts2, w2, n2, A2, nb, scale, F, of, KernCoulomb{}, st);
+    case TCSCF_K_TC_SCALAR: {
+      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: TC kernel needs jastrow parameters with a12 > 0");
+      const double c = same_spin ? jp->c_same : jp->c_oppo;
+      return dispatch(precision, pts1, n1, pts2, w2, n2, A2, nb, scale, F, of,
+                      KernTCScalar{jp->a12, 2 * c * c * jp->a12 * jp->a12}, st);
+    }
+    case TCSCF_K_JASTROW_GRAD:
+      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: gradient kernel needs jastrow parameters with a12 > 0");
+      return dispatch(precision, pts1, n1, pts2, w2, n2, A2, nb, scale, F, of, KernJastrowGrad{jp->a12}, st);
+    default:
+      TC_CHECK(false, "tcscf_compute_F: unknown kernel");
+  }
+  return TCSCF_EINVAL;
+}
+
+// Runs one fragment of MFMA `which` (0 f64 16x16x4, 1 f32 16x16x4, 2 bf16
+// 16x16x32) on device buffers: A (16 x K), B (K x 16), D (16 x 16), FP64.
+int tcscf_selftest_mfma(int which, const double* A_dev, const double* B_dev, double* D_dev, void* stream) {
+  TC_CHECK(which >= 0 && which <= 2, "tcscf_selftest_mfma: which");
+  hipLaunchKernelGGL(mfma_selftest_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, which, A_dev, B_dev, D_dev);
+  TC_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

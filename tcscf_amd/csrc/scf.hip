@@ -1,0 +1,301 @@
+// Per-iteration Fock build (HBM-bound sweeps over the stored packed F) and the
+// RCS / UOS SCF drivers.  Each SCF iteration: upload D, one sweep kernel over
+// the r1 grid, a deterministic reduction of per-workgroup partials, download
+// G (nspin x nb x nb), then mask, Fock, energy and the FP64 generalized
+// eigen-solve on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <vector>
+
+#include "common.hpp"
+#include "dense.hpp"
+
+namespace tc {
+
+constexpr int kSweepBlocks = 512;
+constexpr int kGPerThread = 32;  // NSPIN * nb * nb <= 256 * 32
+
+__device__ inline int64_t pidx(int j, int i) { return j <= i ? (int64_t)i * (i + 1) / 2 + j : (int64_t)j * (j + 1) / 2 + i; }
+
+// G_s[j][i] = sum_n w_n A[n,j] (c_self A[n,i] Faa_s[n] + c_other A[n,i] Faa_sbar[n] - Fai_s[n,i])
+//   Faa_s[n]  = sum_{j,i} F[n,j,i] D_s[i,j]                   (calculateIaa, HartreeFock.cpp:183-213)
+//   Fai_s[n,i] = sum_j (sum_k D_s[k,j] A[n,k]) F[n,j,i]       (calculateFai, HartreeFock.cpp:144-179)
+// RCS (NSPIN = 1): c_self = 2, c_other = 0   (HartreeFock.cpp:828-842)
+// UOS (NSPIN = 2): c_self = c_other = 1      (HartreeFock.cpp:940-955)
+// One wave handles one r1 point at a time; the workgroup stages CH points'
+// A rows and Y_s[n,i] = w_n (...) in LDS and folds them into register-held
+// partial G with a rank-CH update.
+template <class TF, int NSPIN>
+__global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F, int64_t npair,
+                                                       const double* __restrict__ A, const double* __restrict__ w,
+                                                       int64_t n1, int nb, int ch, const double* __restrict__ D,
+                                                       double c_self, double c_other, double* __restrict__ Gpart) {
+  extern __shared__ double smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double* Frow = smem;                          // [4][npair]
+  double* Abuf = Frow + 4 * npair;              // [ch][nb]
+  double* Ybuf = Abuf + (int64_t)ch * nb;       // [ch][NSPIN][nb]
+  double* Xbuf = Ybuf + (int64_t)ch * NSPIN * nb;  // [4][NSPIN][nb]
+  const int64_t nb2 = (int64_t)nb * nb;
+  const int64_t per = (n1 + gridDim.x - 1) / gridDim.x;
+  const int64_t p0 = (int64_t)blockIdx.x * per;
+  const int64_t p1 = p0 + per < n1 ? p0 + per : n1;
+  double gacc[kGPerThread];
+#pragma unroll
+  for (int q = 0; q < kGPerThread; ++q) gacc[q] = 0;
+
+  for (int64_t c0 = p0; c0 < p1; c0 += ch) {
+    const int cn = (int)((c0 + ch < p1) ? ch : p1 - c0);
+    for (int pt = wave; pt < cn; pt += 4) {
+      const int64_t n = c0 + pt;
+      double* fr = Frow + wave * npair;
+      for (int64_t p = lane; p < npair; p += 64) fr[p] = (double)F[n * npair + p];
+      double* ab = Abuf + (int64_t)pt * nb;
+      for (int j = lane; j < nb; j += 64) ab[j] = A[n * nb + j];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double* xb = Xbuf + wave * NSPIN * nb;
+      for (int j = lane; j < nb; j += 64)
+        for (int s = 0; s < NSPIN; ++s) {
+          const double* Ds = D + s * nb2;
+          double x = 0;
+          for (int k = 0; k < nb; ++k) x += Ds[k * nb + j] * ab[k];
+          xb[s * nb + j] = x;
+        }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double faa[NSPIN], fai[NSPIN][2];
+      for (int s = 0; s < NSPIN; ++s) {
+        faa[s] = 0;
+        fai[s][0] = fai[s][1] = 0;
+      }
+      for (int t = 0; t < 2; ++t) {
+        const int i = lane + 64 * t;
+        if (i >= nb) continue;
+        for (int j = 0; j < nb; ++j) {
+          const double f = fr[pidx(j, i)];
+          for (int s = 0; s < NSPIN; ++s) {
+            faa[s] += f * D[s * nb2 + (int64_t)j * nb + i];  // D symmetric: D[i][j] == D[j][i]
+            fai[s][t] += xb[s * nb + j] * f;
+          }
+        }
+      }
+      for (int s = 0; s < NSPIN; ++s) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) faa[s] += __shfl_xor(faa[s], off, 64);
+      }
+      const double wn = w[n];
+      double* yb = Ybuf + (int64_t)pt * NSPIN * nb;
+      for (int t = 0; t < 2; ++t) {
+        const int i = lane + 64 * t;
+        if (i >= nb) continue;
+        for (int s = 0; s < NSPIN; ++s) {
+          const double other = NSPIN == 2 ? faa[1 - s] : 0.0;
+          yb[s * nb + i] = wn * (ab[i] * (c_self * faa[s] + c_other * other) - fai[s][t]);
+        }
+      }
+    }
+    __syncthreads();
+    // rank-cn update of the thread's G elements: e = tid + 256 q -> (s, j, i)
+#pragma unroll
+    for (int q = 0; q < kGPerThread; ++q) {
+      const int64_t e = tid + 256 * q;
+      if (e < NSPIN * nb2) {
+        const int s = (int)(e / nb2);
+        const int64_t ji = e - s * nb2;
+        const int j = (int)(ji / nb), i = (int)(ji - (int64_t)j * nb);
+        double acc = 0;
+        for (int pt = 0; pt < cn; ++pt) acc += Abuf[pt * nb + j] * Ybuf[((int64_t)pt * NSPIN + s) * nb + i];
+        gacc[q] += acc;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < kGPerThread; ++q) {
+    const int64_t e = tid + 256 * q;
+    if (e < NSPIN * nb2) Gpart[(int64_t)blockIdx.x * NSPIN * nb2 + e] = gacc[q];
+  }
+}
+
+// Deterministic sum of per-workgroup partials: out[e] = sum_b part[b][e].
+__global__ void reduce_partials(const double* __restrict__ part, int nparts, int64_t len, double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  double s = 0;
+  for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * len + e];
+  out[e] = s;
+}
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Device side of one Fock-build sweep: G (nspin x nb x nb) from D.
+struct Sweep {
+  int nspin, nb;
+  int64_t n1, npair;
+  const void* F;
+  int f32;
+  const double* A;
+  const double* w;
+  hipStream_t st;
+  double* dD = nullptr;
+  double* dPart = nullptr;
+  double* dG = nullptr;
+  int ch = 16;
+  size_t shmem = 0;
+
+  int init() {
+    const int64_t nb2 = (int64_t)nb * nb;
+    TC_CHECK(nspin * nb2 <= 256 * kGPerThread, "SCF sweep: nb too large for the register-held G partials");
+    int rc = 0;
+    char* p = (char*)scratch(kSlotSweep, sizeof(double) * (nspin * nb2 * (kSweepBlocks + 2)), &rc);
+    if (rc) return rc;
+    dPart = (double*)p;
+    dD = dPart + (int64_t)kSweepBlocks * nspin * nb2;
+    dG = dD + nspin * nb2;
+    // LDS: Frow 4*npair + ch*nb*(1+nspin) + 4*nspin*nb doubles, under 64 KiB
+    ch = 32;
+    while (ch > 4 && sizeof(double) * (4 * npair + (int64_t)ch * nb * (1 + nspin) + 4 * nspin * nb) > 64 * 1024) ch /= 2;
+    shmem = sizeof(double) * (4 * npair + (int64_t)ch * nb * (1 + nspin) + 4 * nspin * nb);
+    TC_CHECK(shmem <= 64 * 1024, "SCF sweep: nb too large for the LDS staging");
+    return 0;
+  }
+
+  int run(const double* D_host, double* G_host) {
+    const int64_t nb2 = (int64_t)nb * nb;
+    TC_HIP(hipMemcpyAsync(dD, D_host, sizeof(double) * nspin * nb2, hipMemcpyHostToDevice, st));
+    const double cs = nspin == 1 ? 2.0 : 1.0, co = nspin == 1 ? 0.0 : 1.0;
+    dim3 grid(kSweepBlocks), block(256);
+    if (nspin == 1) {
+      if (f32)
+        hipLaunchKernelGGL((hf_sweep_kernel<float, 1>), grid, block, shmem, st, (const float*)F, npair, A, w, n1, nb, ch,
+                           dD, cs, co, dPart);
+      else
+        hipLaunchKernelGGL((hf_sweep_kernel<double, 1>), grid, block, shmem, st, (const double*)F, npair, A, w, n1, nb,
+                           ch, dD, cs, co, dPart);
+    } else {
+      if (f32)
+        hipLaunchKernelGGL((hf_sweep_kernel<float, 2>), grid, block, shmem, st, (const float*)F, npair, A, w, n1, nb, ch,
+                           dD, cs, co, dPart);
+      else
+        hipLaunchKernelGGL((hf_sweep_kernel<double, 2>), grid, block, shmem, st, (const double*)F, npair, A, w, n1, nb,
+                           ch, dD, cs, co, dPart);
+    }
+    TC_HIP(hipGetLastError());
+    const int64_t len = nspin * nb2;
+    hipLaunchKernelGGL(reduce_partials, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, dPart, kSweepBlocks, len,
+                       dG);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipMemcpyAsync(G_host, dG, sizeof(double) * len, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+};
+
+void mask_by(int nb, double* G, const double* H) {
+  for (int q = 0; q < nb * nb; ++q)
+    if (!(std::fabs(H[q]) > 0)) G[q] = 0;
+}
+
+// D[u][v] = sum_{a < nocc} C[u][a] C[v][a]  (getNewDensity, HartreeFock.cpp:20-44)
+void density_from(int nb, int nocc, const std::vector<double>& C, double* D) {
+  for (int u = 0; u < nb; ++u)
+    for (int v = 0; v < nb; ++v) {
+      double t = 0;
+      for (int a = 0; a < nocc; ++a) t += C[u * nb + a] * C[v * nb + a];
+      D[u * nb + v] = t;
+    }
+}
+
+int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H, const double* A1, const double* w1,
+           int64_t n1, const void* F, int f_dtype, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
+           double* D_out, double* evals_out, hipStream_t st) {
+  TC_CHECK(S && H && A1 && w1 && F && o && res && nb > 0 && n1 > 0, "SCF: bad arguments");
+  TC_CHECK(o->max_iter > 0, "SCF: max_iter must be positive");
+  const int64_t nb2 = (int64_t)nb * nb;
+  Sweep sw{nspin, nb, n1, (int64_t)nb * (nb + 1) / 2, F, f_dtype == TCSCF_F32, A1, w1, st};
+  int rc = sw.init();
+  if (rc) return rc;
+  std::vector<double> D(nspin * nb2, 0.0), G(nspin * nb2), Fock(nspin * nb2), evals(nspin * nb, 0.0), C;
+  double energy = std::numeric_limits<double>::max();
+  res->status = TCSCF_ENOTCONV;
+  res->energy = std::numeric_limits<double>::max();
+  res->iterations = 0;
+  res->t_sweep_ms = res->t_solve_ms = 0;
+  for (int it = 0; it < o->max_iter; ++it) {
+    double t0 = now_ms();
+    rc = sw.run(D.data(), G.data());
+    if (rc) return rc;
+    double t1 = now_ms();
+    res->t_sweep_ms += t1 - t0;
+    if (o->mask)
+      for (int s = 0; s < nspin; ++s) mask_by(nb, G.data() + s * nb2, H);
+    // constructSCFOperator (HartreeFock.cpp:48-98); RCS energy x2 (:850)
+    double e = 0;
+    for (int s = 0; s < nspin; ++s)
+      for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < nb; ++i) {
+          const double g = G[s * nb2 + j * nb + i];
+          Fock[s * nb2 + j * nb + i] = H[j * nb + i] + g;
+          e += D[s * nb2 + i * nb + j] * (H[j * nb + i] + g / 2);
+        }
+    if (nspin == 1) e *= 2;
+    if (trace) trace[it] = e;
+    res->iterations = it + 1;
+    if (std::fabs(e - energy) < o->tol) {
+      res->energy = e;
+      res->status = TCSCF_OK;
+      res->t_solve_ms += now_ms() - t1;
+      break;
+    }
+    energy = e;
+    for (int s = 0; s < nspin; ++s) {
+      std::vector<double> ev;
+      if (!dense::gen_sym_eig(nb, Fock.data() + s * nb2, S, ev, C)) {
+        res->status = TCSCF_ESOLVER;
+        set_error("SCF: generalized eigen-solve failed (overlap not positive definite?)");
+        return TCSCF_ESOLVER;
+      }
+      std::copy(ev.begin(), ev.end(), evals.begin() + s * nb);
+      density_from(nb, nocc[s], C, D.data() + s * nb2);
+    }
+    res->t_solve_ms += now_ms() - t1;
+  }
+  if (D_out) std::copy(D.begin(), D.end(), D_out);
+  if (evals_out) std::copy(evals.begin(), evals.end(), evals_out);
+  if (res->status != TCSCF_OK) set_error("SCF: did not converge");
+  return res->status;
+}
+
+}  // namespace
+}  // namespace tc
+
+extern "C" {
+
+int tcscf_rcs_scf(int nelec, int nb, const double* S, const double* H, const double* A1, const double* w1, int64_t n1,
+                  const void* F, int f_dtype, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
+                  double* D_out, double* evals_out, void* stream) {
+  TC_CHECK(nelec > 0 && nelec % 2 == 0, "RCS: the number of electrons must be even");
+  const int nocc[1] = {nelec / 2};
+  return tc::hf_scf(1, nocc, nb, S, H, A1, w1, n1, F, f_dtype, o, res, trace, D_out, evals_out, (hipStream_t)stream);
+}
+
+int tcscf_uos_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1, const double* w1,
+                  int64_t n1, const void* F, int f_dtype, const tcscf_scf_opts* o, tcscf_scf_result* res,
+                  double* trace, double* D_out, double* evals_out, void* stream) {
+  TC_CHECK(nup >= 0 && ndown >= 0 && nup <= nb && ndown <= nb, "UOS: bad electron counts");
+  const int nocc[2] = {nup, ndown};
+  return tc::hf_scf(2, nocc, nb, S, H, A1, w1, n1, F, f_dtype, o, res, trace, D_out, evals_out, (hipStream_t)stream);
+}
+
+}  // extern "C"

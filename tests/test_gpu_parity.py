@@ -1,0 +1,220 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the
+same seeded inputs.  Tolerances are written per test; FP64 paths are held to
+~1e-13 relative, SCF energies to 1e-10 Eh."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import fns_of, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(tc, nrad, order, kind):
+    return tc.QMCGrid.radial_lebedev(nrad, order, kind, 1.0, 1.0)
+
+
+def _mixed_basis():
+    from tcscf_amd import SlaterTypeOrbital as STO
+    b = [STO(0.9, 1, 0, 0), STO(2.3, 1, 0, 0), STO(1.1, 2, 0, 0)]
+    b += [STO(1.2, 2, 1, m) for m in (-1, 0, 1)] + [STO(0.6, 3, 1, m) for m in (-1, 0, 1)]
+    b += [STO(1.4, 3, 2, m) for m in range(-2, 3)] + [STO(0.9, 4, 3, m) for m in range(-3, 4)]
+    return b
+
+
+def test_mfma_fragment_layouts(tc, gpu):
+    """Pins the lane layouts of the three MFMA instructions the contraction
+    kernels use, against numpy matmul."""
+    torch = gpu
+    import ctypes
+    rng = np.random.default_rng(0)
+    for which, K in ((0, 4), (1, 4), (2, 32)):
+        A = rng.integers(-8, 8, size=(16, K)).astype(np.float64)
+        B = rng.integers(-8, 8, size=(K, 16)).astype(np.float64)
+        dA = torch.from_numpy(A).cuda()
+        dB = torch.from_numpy(B).cuda()
+        dD = torch.zeros((16, 16), dtype=torch.float64, device="cuda")
+        rc = tc.lib().tcscf_selftest_mfma(which, ctypes.c_void_p(dA.data_ptr()), ctypes.c_void_p(dB.data_ptr()),
+                                          ctypes.c_void_p(dD.data_ptr()), None)
+        assert rc == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(dD.cpu().numpy(), A @ B), which
+
+
+def test_basis_values_and_gradients(tc, O, gpu):
+    basis = _mixed_basis()
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(-3, 3, (3, 500))
+    pts[:, :4] = [[0, 0, 0, 0], [0, 0, 0, 0], [0.5, -0.5, 2.0, -1.5]]  # z axis
+    g = tc.QMCGrid(pts, np.ones(500))
+    g.setBasisFunctions(basis, True)
+    v = g.basisValues.cpu().numpy()
+    d = g.basisGradients.cpu().numpy()
+    v2, d2 = O.sto_real(pts, fns_of(basis), grads=True)
+    assert rel_err(v, v2) < 1e-13
+    assert np.max(np.abs(d - d2)) < 1e-12 * np.max(np.abs(d2))
+    # also on a Lebedev product grid (poles included)
+    g = _grid(tc, 30, 17, tc.RADIAL_M4)
+    g.setBasisFunctions(basis, True)
+    v2, d2 = O.sto_real(g.points_host, fns_of(basis), grads=True)
+    assert rel_err(g.basisValues.cpu().numpy(), v2) < 1e-13
+    assert np.max(np.abs(g.basisGradients.cpu().numpy() - d2)) < 1e-11 * np.max(np.abs(d2))
+
+
+def _F_oracle(O, g1, g2, kernel=0, prm=None, same=True, scale=1.0, rows=None):
+    A2 = g2.basisValues.cpu().numpy()
+    return O.compute_F(g1.points_host, g2.points_host, g2.weights_host, A2, kernel=kernel, prm=prm, same=same,
+                       scale=scale, rows=rows)
+
+
+def test_F_fp64_he_config1(tc, O, gpu):
+    from tcscf_amd import configs
+    c = configs.he_config1()
+    g = _grid(tc, *c["grid"])
+    g.setBasisFunctions(c["basis"], False)
+    F = tc.computeF(g, g)
+    ref = _F_oracle(O, g, g)
+    assert rel_err(tc.unpack(F.cpu().numpy(), 5), ref) < 1e-13
+
+
+def test_F_fp64_mixed_l_two_grids(tc, O, gpu):
+    """Different r1 and r2 grids, s..f functions, row count not a multiple of
+    any tile size."""
+    basis = _mixed_basis()
+    g1 = _grid(tc, 13, 11, tc.RADIAL_M4)  # 650 points
+    g2 = _grid(tc, 17, 7, tc.RADIAL_BECKE)  # 442 points
+    g1.setBasisFunctions(basis, False)
+    g2.setBasisFunctions(basis, False)
+    F = tc.computeF(g1, g2)
+    ref = _F_oracle(O, g1, g2)
+    assert rel_err(tc.unpack(F.cpu().numpy(), len(basis)), ref) < 1e-13
+
+
+def test_F_fp64_2d_grid_scale(tc, O, gpu):
+    """The reference's 2-D r2 grid (points in the xz-plane, F x 2 pi)."""
+    from tcscf_amd import SlaterTypeOrbital as STO
+    basis = [STO(0.8, 1, 0, 0), STO(1.7, 1, 0, 0), STO(1.1, 2, 0, 0)]
+    rng = np.random.default_rng(3)
+    p2 = rng.uniform(0, 3, (3, 300))
+    p2[1] = 0
+    g1 = _grid(tc, 10, 11, tc.RADIAL_M4)
+    g2 = tc.QMCGrid(p2, rng.uniform(0.1, 1, 300), ndim=2)
+    g1.setBasisFunctions(basis, False)
+    g2.setBasisFunctions(basis, False)
+    F = tc.computeF(g1, g2)
+    ref = _F_oracle(O, g1, g2, scale=2 * math.pi)
+    assert rel_err(tc.unpack(F.cpu().numpy(), 3), ref) < 1e-13
+
+
+def test_F_fp64_tc_kernels(tc, O, gpu):
+    """TC scalar kernel (same and opposite spin) and the Jastrow-gradient
+    vector kernel against the oracle's general Ochi evaluation."""
+    basis = _mixed_basis()[:9]
+    g1 = _grid(tc, 11, 11, tc.RADIAL_M4)
+    g3 = _grid(tc, 9, 7, tc.RADIAL_M4)
+    for g in (g1, g3):
+        g.setBasisFunctions(basis, True)
+    jas = tc.Jastrow.from_b(1.0)  # a12 = 1, c_oppo = 1/2, c_same = 1/4
+    for same in (True, False):
+        prm = O.ochi_params(1.0, jas.a12, [(1, 0, 0, jas.c_oppo, jas.c_same)])
+        F = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=same)
+        ref = _F_oracle(O, g1, g1, kernel=O.K_TC_SCALAR, prm=prm, same=same)
+        assert rel_err(tc.unpack(F.cpu().numpy(), 9), ref) < 1e-12
+    V = tc.computeV(g1, g3, jas).cpu().numpy()  # (n1, 3, npair), c = 1
+    prm = O.ochi_params(1.0, jas.a12, [(1, 0, 0, 1.0, 1.0)])
+    ref = _F_oracle(O, g1, g3, kernel=O.K_JASTROW_GRAD, prm=prm)  # (n1, nb, nb, 3)
+    full = tc.unpack(V, 9).transpose(0, 2, 3, 1)
+    assert rel_err(full, ref) < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("bf16x3", 3e-5)])
+def test_F_mixed_precision_error(tc, gpu, precision, tol):
+    """Mixed-precision F against the FP64 drop-in on the Ne basis (nb = 30):
+    max element error relative to max |F| (tolerance in the parameters)."""
+    from tcscf_amd import configs
+    c = configs.ne_config2()
+    g = _grid(tc, 40, 17, tc.RADIAL_M4)
+    g.setBasisFunctions(c["basis"], False)
+    F64 = tc.computeF(g, g, precision="fp64").cpu().numpy()
+    F = tc.computeF(g, g, precision=precision).cpu().numpy()
+    err = np.max(np.abs(F - F64)) / np.max(np.abs(F64))
+    assert err < tol, err
+
+
+# ----------------------------------------------------------------- SCF
+def _scf_inputs(tc, cfg, grid=None):
+    g = _grid(tc, *(grid or cfg["grid"]))
+    g.setBasisFunctions(cfg["basis"], False)
+    S, H = tc.one_electron_matrices(cfg["basis"], cfg["Z"])
+    return g, S, H
+
+
+def test_rcs_uos_he_config1_vs_oracle(tc, O, gpu):
+    from tcscf_amd import configs
+    c = configs.he_config1()
+    g, S, H = _scf_inputs(tc, c)
+    F = tc.computeF(g, g)
+    A = g.basisValues.cpu().numpy()
+    w = g.weights_host
+    Fo = _F_oracle(O, g, g)
+    for mask in (True, False):
+        hf = tc.RCSHartreeFock(2, 5)
+        e = hf.compute(S, H, g, F, mask, tol=1e-10)
+        ref = O.rcs_scf(2, S, H, A, w, Fo, tol=1e-10, mask=mask)
+        assert abs(e - ref["energy"]) < 1e-10
+        assert hf.numberOfConvergenceLoops() == ref["iterations"]
+        assert np.max(np.abs(hf.energy_trace - ref["trace"])) < 1e-10
+        uhf = tc.UOSHartreeFock(1, 1, 5)
+        eu = uhf.compute(S, H, g, F, mask, tol=1e-10)
+        refu = O.uos_scf(1, 1, S, H, A, w, Fo, tol=1e-10, mask=mask)
+        assert abs(eu - refu["energy"]) < 1e-10
+        assert abs(eu - e) < 1e-10
+
+
+def test_rcs_ne_small_grid_end_to_end(tc, O, gpu):
+    """Ne basis (30 STOs, s/p/d) on a 40 x 302-point grid: GPU basis + F + SCF
+    against the all-oracle path."""
+    from tcscf_amd import configs
+    c = configs.ne_config2()
+    g, S, H = _scf_inputs(tc, c, (40, 29, tc.RADIAL_M4))
+    F = tc.computeF(g, g)
+    A = O.sto_real(g.points_host, fns_of(c["basis"]))
+    Fo = O.compute_F(g.points_host, g.points_host, g.weights_host, A)
+    hf = tc.RCSHartreeFock(10, 30)
+    e = hf.compute(S, H, g, F, True, tol=1e-10)
+    ref = O.rcs_scf(10, S, H, A, g.weights_host, Fo, tol=1e-10)
+    assert ref["status"] == 0
+    assert abs(e - ref["energy"]) < 1e-10, (e, ref["energy"])
+
+
+def test_rcs_ne_config2_full_size(tc, O, gpu):
+    """Ne config 2 at full size (30,200 points): F rows checked against the
+    oracle on a contiguous slice; the SCF on the device F checked against the
+    oracle SCF on the same F."""
+    from tcscf_amd import configs
+    c = configs.ne_config2()
+    g, S, H = _scf_inputs(tc, c)
+    F = tc.computeF(g, g)
+    Fh = F.cpu().numpy()
+    rows = (15000, 15064)
+    ref_rows = _F_oracle(O, g, g, rows=rows)
+    assert rel_err(tc.unpack(Fh[rows[0]:rows[1]], 30), ref_rows) < 1e-13
+    hf = tc.RCSHartreeFock(10, 30)
+    e = hf.compute(S, H, g, F, True, tol=1e-10)
+    ref = O.rcs_scf(10, S, H, g.basisValues.cpu().numpy(), g.weights_host, tc.unpack(Fh, 30), tol=1e-10)
+    assert abs(e - ref["energy"]) < 1e-10, (e, ref["energy"])
+
+
+def test_scf_error_paths(tc, gpu):
+    from tcscf_amd import configs
+    c = configs.he_config1()
+    g, S, H = _scf_inputs(tc, c)
+    F = tc.computeF(g, g)
+    with pytest.raises(tc.ConvergenceError):
+        tc.UOSHartreeFock(1, 1, 5).compute(S, H, g, F, True, max_iter=1)
+    basis = c["basis"] + [c["basis"][0]]
+    g2, S2, H2 = _scf_inputs(tc, dict(c, basis=basis))
+    F2 = tc.computeF(g2, g2)
+    with pytest.raises(tc.SolverError):
+        tc.RCSHartreeFock(2, 6).compute(S2, H2, g2, F2, True)
