@@ -66,6 +66,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built: run `python -m tcscf_amd.build` (hipcc, gfx950)")
+        # torch ships its own libamdhip64 (SONAME libamdhip64.so.7).  Importing it
+        # first lets the loader satisfy this library's libamdhip64.so.7 with the
+        # copy already mapped, so the process runs one HIP runtime; loading ours
+        # first would map /opt/rocm's copy beside torch's and the two runtimes
+        # then fight over the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         L.tcscf_last_error.restype = ctypes.c_char_p
         L.tcscf_F_workspace_bytes.restype = ctypes.c_size_t

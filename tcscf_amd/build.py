@@ -30,8 +30,9 @@ def build(verbose=False, jobs=None):
     os.makedirs(objdir, exist_ok=True)
     common = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
               "-Wno-unused-lambda-capture"]
-    hip_flags = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                 "-munsafe-fp-atomics"]
+    # code object v5: loadable by the HIP runtime torch bundles as well as /opt/rocm's
+    hip_flags = [f"--offload-arch={ARCH}", "-mcode-object-version=5", "-Xclang", "-target-feature", "-Xclang",
+                 "-packed-fp32-ops", "-munsafe-fp-atomics"]
     objs, procs = [], []
     for s in srcs:
         obj = os.path.join(objdir, os.path.basename(s) + ".o")
@@ -51,7 +52,7 @@ def build(verbose=False, jobs=None):
     if failed:
         raise RuntimeError("hipcc compilation failed")
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-mcode-object-version=5", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

@@ -79,3 +79,11 @@ def test_error_paths(tc):
     H = np.zeros(1)
     dp = ctypes.POINTER(ctypes.c_double)
     assert L.tcscf_sto_overlap_core(bad, 1, 1, S.ctypes.data_as(dp), H.ctypes.data_as(dp)) == _lib.EINVAL
+
+
+def test_single_hip_runtime_in_process(tc):
+    """torch bundles its own libamdhip64; the package loads torch first so this
+    library binds to the same runtime (two runtimes in one process do not see
+    the device consistently)."""
+    paths = {line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64" in line}
+    assert len(paths) == 1, paths
