@@ -145,9 +145,10 @@ int tcscf_uos_scf(int nup, int ndown, int nb, const double* S, const double* H, 
 
 /* ---------------------------------------------------------------- diagnostics */
 /* No reference counterpart: runs one MFMA fragment (which = 0: f64 16x16x4,
- * 1: f32 16x16x4, 2: bf16 16x16x32) on A_dev (16 x K), B_dev (K x 16), writing
- * D_dev (16 x 16), all row-major FP64, so tests can pin the fragment layouts
- * the contraction kernels rely on. */
+ * 1: f32 16x16x4, 2: bf16 16x16x32) on A_dev (16 x K) and B_dev (K x 16),
+ * row-major FP64, with lane l = 16 g + r supplying A[r][k], B[k][r] for its k
+ * slice, and writes the raw accumulator registers to D_dev[4 l + i] (256
+ * doubles), so tests can pin the fragment layouts the contraction kernels use. */
 int tcscf_selftest_mfma(int which, const double* A_dev, const double* B_dev, double* D_dev, void* stream);
 
 #ifdef __cplusplus

@@ -22,6 +22,12 @@ typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 constexpr double kFar = 1.0e6;  // padding coordinate: far from every grid point
 
+// Output layouts of the 16x16 MFMA accumulators: element i of lane l = 16 g + r
+// holds D[row][r] with row = d_row_*(g, i).  (Pinned by tcscf_selftest_mfma.)
+__host__ __device__ constexpr int d_row_f64(int g, int i) { return g + 4 * i; }
+__host__ __device__ constexpr int d_row_f32(int g, int i) { return 4 * g + i; }
+__host__ __device__ constexpr int d_row_bf16(int g, int i) { return 4 * g + i; }
+
 /* ------------------------------------------------------------------------ */
 /* scratch buffers                                                           */
 /* ------------------------------------------------------------------------ */
@@ -298,7 +304,7 @@ struct KernJastrowGrad {
 /* ------------------------------------------------------------------------ */
 /* FP64 contraction: v_mfma_f64_16x16x4_f64                                  */
 /* A (16x4): lane l holds A[l%16][l/16]; B (4x16): B[l/16][l%16];            */
-/* D (16x16): d4 element i of lane l is D[4(l/16)+i][l%16].                  */
+/* D (16x16): d4 element i of lane l is D[l/16 + 4i][l%16].                  */
 /* Workgroup = 4 waves as 2 (rows) x 2 (cols); wave tile 16*MB x 64.         */
 /* ------------------------------------------------------------------------ */
 template <int MB, class Kern>
@@ -357,7 +363,7 @@ __global__ __launch_bounds__(256) void fgemm_f64_kernel(const double* __restrict
       for (int n = 0; n < 4; ++n)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int64_t row = rowW + 16 * m + 4 * g + i, col = colW + 16 * n + r;
+          const int64_t row = rowW + 16 * m + d_row_f64(g, i), col = colW + 16 * n + r;
           if (row < n1 && col < npair) {
             const int64_t o = (row * NC + c) * npair + col;
             const double v = scale * acc[c][m][n][i];
@@ -443,7 +449,7 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
       for (int n = 0; n < 4; ++n)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int64_t row = rowW + 16 * m + 4 * g + i, col = colW + 16 * n + r;
+          const int64_t row = rowW + 16 * m + d_row_f32(g, i), col = colW + 16 * n + r;
           if (row < n1 && col < npair) {
             const int64_t o = (row * NC + c) * npair + col;
             const double v = scale * sum[c][m][n][i];
@@ -586,7 +592,7 @@ __global__ __launch_bounds__(256) void fgemm_bf16x3_kernel(const double* __restr
       for (int n = 0; n < NB; ++n)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int64_t row = row0 + 16 * m + 4 * g + i, col = 16 * (cb0 + n) + r;
+          const int64_t row = row0 + 16 * m + d_row_bf16(g, i), col = 16 * (cb0 + n) + r;
           if (row < n1 && col < npair) {
             const int64_t o = (row * NC + c) * npair + col;
             const double v = scale * sum[c][m][n][i];
@@ -605,11 +611,11 @@ __global__ void mfma_selftest_kernel(int which, const double* A, const double* B
   if (which == 0) {
     d4 c = {0, 0, 0, 0};
     c = __builtin_amdgcn_mfma_f64_16x16x4f64(A[r * 4 + g], B[g * 16 + r], c, 0, 0, 0);
-    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + r] = c[i];
+    for (int i = 0; i < 4; ++i) D[l * 4 + i] = c[i];
   } else if (which == 1) {
     f4 c = {0, 0, 0, 0};
     c = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[r * 4 + g], (float)B[g * 16 + r], c, 0, 0, 0);
-    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + r] = c[i];
+    for (int i = 0; i < 4; ++i) D[l * 4 + i] = c[i];
   } else {
     us8 a, b;
     for (int e = 0; e < 8; ++e) {
@@ -618,7 +624,7 @@ __global__ void mfma_selftest_kernel(int which, const double* A, const double* B
     }
     f4 c = {0, 0, 0, 0};
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
-    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + r] = c[i];
+    for (int i = 0; i < 4; ++i) D[l * 4 + i] = c[i];
   }
 }
 
@@ -774,7 +780,8 @@ int tcscf_compute_F(const double* pts1, int64_t n1, const double* pts2, const do
 }
 
 // Runs one fragment of MFMA `which` (0 f64 16x16x4, 1 f32 16x16x4, 2 bf16
-// 16x16x32) on device buffers: A (16 x K), B (K x 16), D (16 x 16), FP64.
+// 16x16x32) with lane 16 g + r supplying A[r][k] and B[k][r] for its k slice;
+// D receives the raw accumulator registers, D[4 l + i] (64 x 4, FP64).
 int tcscf_selftest_mfma(int which, const double* A_dev, const double* B_dev, double* D_dev, void* stream) {
   TC_CHECK(which >= 0 && which <= 2, "tcscf_selftest_mfma: which");
   hipLaunchKernelGGL(mfma_selftest_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, which, A_dev, B_dev, D_dev);

@@ -23,23 +23,36 @@ def _mixed_basis():
     return b
 
 
+# Output layouts the contraction kernels assume: register i of lane 16 g + r
+# holds D[row(g, i)][r].
+D_ROW = {0: lambda g, i: g + 4 * i, 1: lambda g, i: 4 * g + i, 2: lambda g, i: 4 * g + i}
+
+
 def test_mfma_fragment_layouts(tc, gpu):
     """Pins the lane layouts of the three MFMA instructions the contraction
-    kernels use, against numpy matmul."""
+    kernels use, against numpy matmul (raw accumulator registers)."""
     torch = gpu
     import ctypes
     rng = np.random.default_rng(0)
+    bad = []
     for which, K in ((0, 4), (1, 4), (2, 32)):
         A = rng.integers(-8, 8, size=(16, K)).astype(np.float64)
         B = rng.integers(-8, 8, size=(K, 16)).astype(np.float64)
         dA = torch.from_numpy(A).cuda()
         dB = torch.from_numpy(B).cuda()
-        dD = torch.zeros((16, 16), dtype=torch.float64, device="cuda")
+        dD = torch.zeros(256, dtype=torch.float64, device="cuda")
         rc = tc.lib().tcscf_selftest_mfma(which, ctypes.c_void_p(dA.data_ptr()), ctypes.c_void_p(dB.data_ptr()),
                                           ctypes.c_void_p(dD.data_ptr()), None)
         assert rc == 0
         torch.cuda.synchronize()
-        assert np.array_equal(dD.cpu().numpy(), A @ B), which
+        raw = dD.cpu().numpy().reshape(64, 4)
+        C = A @ B
+        want = np.array([[C[D_ROW[which](l // 16, i)][l % 16] for i in range(4)] for l in range(64)])
+        if not np.array_equal(raw, want):
+            found = [name for name, f in (("4g+i", lambda g, i: 4 * g + i), ("g+4i", lambda g, i: g + 4 * i))
+                     if np.array_equal(raw, [[C[f(l // 16, i)][l % 16] for i in range(4)] for l in range(64)])]
+            bad.append((which, found))
+    assert not bad, bad
 
 
 def test_basis_values_and_gradients(tc, O, gpu):
