@@ -48,8 +48,9 @@ enum tcscf_kernel {
 enum tcscf_precision {
   TCSCF_FP64 = 0,   /* v_mfma_f64_16x16x4_f64, FP64 kernel generation */
   TCSCF_FP32 = 1,   /* v_mfma_f32_16x16x4_f32, FP64 cross-chunk sums */
-  TCSCF_BF16X3 = 2  /* bf16 hi/lo split, hi*hi + hi*lo + lo*hi on
-                       v_mfma_f32_16x16x32_bf16, FP64 cross-chunk sums */
+  TCSCF_BF16X3 = 2, /* v_mfma_f32_16x16x32_bf16: K = Kh + Kl, P = Ph + Pm;
+                       Kh Ph + Kh Pm + Kl Ph, FP64 sums per 32-point step */
+  TCSCF_BF16X5 = 3  /* as BF16X3 with P = Ph + Pm + Pl and + Kh Pl + Kl Pm */
 };
 
 enum tcscf_dtype { TCSCF_F64 = 0, TCSCF_F32 = 1 };
@@ -122,11 +123,16 @@ int tcscf_basis_eval(const double* pts_dev, int64_t n, const tcscf_sto* fns_host
  * ncomp = 3 for TCSCF_K_JASTROW_GRAD (layout F[n1][3][npair]), else 1
  * (layout F[n1][npair]).  For TCSCF_K_TC_SCALAR, same_spin selects c_same or
  * c_oppo; TCSCF_K_JASTROW_GRAD uses c = 1 (scale by c_same / c_oppo).
- * out_dtype: TCSCF_F64 or TCSCF_F32.  Allocates and caches a device workspace
- * for the packed pair densities (see tcscf_F_workspace_bytes). */
-int tcscf_compute_F(const double* pts1_dev, int64_t n1, const double* pts2_dev, const double* w2_dev, int64_t n2,
-                    const double* A2_dev, int nb, int kernel, const tcscf_jastrow* jastrow, int same_spin,
-                    int precision, double scale, void* F_dev, int out_dtype, void* stream);
+ * ord2_dev (n2 int32, or NULL) is the order in which the r2 points are summed;
+ * any permutation gives the same F in exact arithmetic, and summing shell by
+ * shell (points sorted by radius) keeps the dynamic range inside each
+ * 32-point FP32 step small.  out_dtype: TCSCF_F64 or TCSCF_F32.  Allocates and
+ * caches a device workspace for the packed pair densities
+ * (tcscf_F_workspace_bytes). */
+int tcscf_compute_F(const double* pts1_dev, int64_t n1, const double* pts2_dev, const double* w2_dev,
+                    const int32_t* ord2_dev, int64_t n2, const double* A2_dev, int nb, int kernel,
+                    const tcscf_jastrow* jastrow, int same_spin, int precision, double scale, void* F_dev,
+                    int out_dtype, void* stream);
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
 
 /* ---------------------------------------------------------------- SCF drivers */

@@ -175,6 +175,11 @@ class QMCGrid:
         dev = _dev()
         self.ndim = ndim
         self.quadratureGrid = QuadratureGrid(torch.from_numpy(pts).to(dev), torch.from_numpy(w).to(dev))
+        # summation order for contractions over this grid: shell by shell
+        # (by radius), so each 32-point FP32 step of the mixed-precision
+        # kernels sees pair densities of similar magnitude
+        order = np.argsort(np.sqrt((pts * pts).sum(0)), kind="stable").astype(np.int32)
+        self.radial_order = torch.from_numpy(order).to(dev)
         self.points_host = pts
         self.weights_host = w
         self.basisValues = None
@@ -229,7 +234,7 @@ class Jastrow:
         return _lib.Jastrow(self.a12, self.c_same, self.c_oppo)
 
 
-def _compute(r1Grid, r2Grid, kernel, jastrow, same_spin, precision, out_dtype):
+def _compute(r1Grid, r2Grid, kernel, jastrow, same_spin, precision, out_dtype, ordered=True):
     torch = _torch()
     if r1Grid.nBasis() != r2Grid.nBasis() or r2Grid.basisValues is None:
         raise ValueError("computeF: both grids need the same tabulated basis")
@@ -244,7 +249,7 @@ def _compute(r1Grid, r2Grid, kernel, jastrow, same_spin, precision, out_dtype):
     jp = jastrow._struct() if jastrow is not None else None
     check(lib().tcscf_compute_F(_ptr(r1Grid.quadratureGrid.points), ctypes.c_int64(n1),
                                 _ptr(r2Grid.quadratureGrid.points), _ptr(r2Grid.quadratureGrid.weights),
-                                ctypes.c_int64(n2), _ptr(r2Grid.basisValues), nb, kernel,
+                                _ptr(r2Grid.radial_order) if ordered else None, ctypes.c_int64(n2), _ptr(r2Grid.basisValues), nb, kernel,
                                 ctypes.byref(jp) if jp is not None else None, int(bool(same_spin)), prec,
                                 ctypes.c_double(scale), _ptr(F), _lib.F32 if dtype == torch.float32 else _lib.F64,
                                 _stream()))
@@ -252,18 +257,18 @@ def _compute(r1Grid, r2Grid, kernel, jastrow, same_spin, precision, out_dtype):
 
 
 def computeF(r1Grid, r2Grid, kernel="coulomb", jastrow=None, same_spin=True, precision="fp64",
-             out_dtype="fp64"):
+             out_dtype="fp64", ordered=True):
     """precomputeIntegrand + computeF fused (src/HartreeFock.hpp:23-131):
     packed F[n1, p(j,i)] = scale * sum_n2 K(r1, r2) w2 phi_j(r2) phi_i(r2).
     kernel: 'coulomb' (1/r12) or 'tc' (2/r12 - |grad u12|^2 - |grad u21|^2)."""
     k = {"coulomb": _lib.K_COULOMB, "tc": _lib.K_TC_SCALAR}[kernel]
-    return _compute(r1Grid, r2Grid, k, jastrow, same_spin, precision, out_dtype)
+    return _compute(r1Grid, r2Grid, k, jastrow, same_spin, precision, out_dtype, ordered)
 
 
-def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64"):
+def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64", ordered=True):
     """computeV (src/HartreeFock.hpp:185-229) for c = 1: packed V[n1, xyz, p].
     V_same = c_same * V, V_oppo = c_oppo * V."""
-    return _compute(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, True, precision, out_dtype)
+    return _compute(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, True, precision, out_dtype, ordered)
 
 
 # --------------------------------------------------------------------------- SCF

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -178,19 +179,21 @@ __device__ inline void pair_of(int64_t p, int* j, int* i) {
   *j = (int)(p - (int64_t)ii * (ii + 1) / 2);
 }
 
-// Row-major P[k][p] (ld = ldp), rows k < n2p, zero outside (n2, npair).
+// Row-major P[k][p] = P(ord[k], p) (ld = ldp), rows k < n2p, zero outside (n2, npair).
 template <class T>
 __global__ __launch_bounds__(256) void pack_pairs_rowmajor(const double* __restrict__ A, const double* __restrict__ w,
-                                                           int64_t n2, int nb, int64_t npair, int64_t n2p, int64_t ldp,
+                                                           const int* __restrict__ ord, int64_t n2, int nb,
+                                                           int64_t npair, int64_t n2p, int64_t ldp,
                                                            T* __restrict__ P) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n2p * ldp) return;
   const int64_t k = idx / ldp, p = idx - k * ldp;
   double v = 0;
   if (k < n2 && p < npair) {
+    const int64_t ks = ord ? ord[k] : k;
     int j, i;
     pair_of(p, &j, &i);
-    v = w[k] * A[k * nb + j] * A[k * nb + i];
+    v = w[ks] * A[ks * nb + j] * A[ks * nb + i];
   }
   P[idx] = (T)v;
 }
@@ -202,13 +205,15 @@ __device__ inline unsigned short bf16_bits_rne(float f) {
 }
 __device__ inline float bf16_to_float(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
 
-// BF16 hi/lo split of P in MFMA B-fragment order for v_mfma_f32_16x16x32_bf16:
+// BF16 split of P in MFMA B-fragment order for v_mfma_f32_16x16x32_bf16:
 // fragment (kb, cb) = 64 lanes x 8 bf16, lane l = 16 g + r holds
-// P[32 kb + 8 g + e][16 cb + r], e = 0..7.  hi[] then lo[] (each nkb*ncb*512 bf16).
-__global__ __launch_bounds__(256) void pack_pairs_bf16x2(const double* __restrict__ A, const double* __restrict__ w,
-                                                         int64_t n2, int nb, int64_t npair, int64_t nkb, int64_t ncb,
-                                                         unsigned short* __restrict__ hi,
-                                                         unsigned short* __restrict__ lo) {
+// P[ord[32 kb + 8 g + e]][16 cb + r], e = 0..7, where ord is the summation order
+// of the r2 points (NULL = grid order).  NSPLIT terms: P ~ hi + mid (+ lo), each
+// plane nkb*ncb*512 bf16, planes back to back.
+template <int NSPLIT>
+__global__ __launch_bounds__(256) void pack_pairs_bf16(const double* __restrict__ A, const double* __restrict__ w,
+                                                       const int* __restrict__ ord, int64_t n2, int nb, int64_t npair,
+                                                       int64_t nkb, int64_t ncb, unsigned short* __restrict__ planes) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one (fragment, lane, e)
   const int64_t total = nkb * ncb * 512;
   if (idx >= total) return;
@@ -220,27 +225,31 @@ __global__ __launch_bounds__(256) void pack_pairs_bf16x2(const double* __restric
   const int64_t k = 32 * kb + 8 * g + e, p = 16 * cb + r;
   double v = 0;
   if (k < n2 && p < npair) {
+    const int64_t ks = ord ? ord[k] : k;
     int j, i;
     pair_of(p, &j, &i);
-    v = w[k] * A[k * nb + j] * A[k * nb + i];
+    v = w[ks] * A[ks * nb + j] * A[ks * nb + i];
   }
-  const float vf = (float)v;
-  const unsigned short h = bf16_bits_rne(vf);
-  const float rem = (float)(v - (double)bf16_to_float(h));
-  hi[idx] = h;
-  lo[idx] = bf16_bits_rne(rem);
+#pragma unroll
+  for (int t = 0; t < NSPLIT; ++t) {
+    const unsigned short h = bf16_bits_rne((float)v);
+    planes[t * total + idx] = h;
+    v -= (double)bf16_to_float(h);
+  }
 }
 
-// Padded SoA coordinates (3 x n2p) in FP64 and as float4 {x, y, z, 0}.
-__global__ void pad_coords(const double* __restrict__ pts, int64_t n, int64_t np, double* __restrict__ out,
-                           float4* __restrict__ out4) {
+// Padded SoA coordinates (3 x n2p) in FP64 and as float4 {x, y, z, 0}, in the
+// summation order ord (NULL = grid order).
+__global__ void pad_coords(const double* __restrict__ pts, const int* __restrict__ ord, int64_t n, int64_t np,
+                           double* __restrict__ out, float4* __restrict__ out4) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= np) return;
   double x = kFar, y = kFar, z = kFar;
   if (k < n) {
-    x = pts[k];
-    y = pts[n + k];
-    z = pts[2 * n + k];
+    const int64_t ks = ord ? ord[k] : k;
+    x = pts[ks];
+    y = pts[n + ks];
+    z = pts[2 * n + ks];
   }
   out[k] = x;
   out[np + k] = y;
@@ -462,54 +471,64 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 /* ------------------------------------------------------------------------ */
 /* BF16 split contraction: v_mfma_f32_16x16x32_bf16                          */
 /* A (16x32): lane l = 16g + r holds A[r][8g .. 8g+7]; B (32x16): lane l     */
-/* holds B[8g .. 8g+7][r]; D as for 16x16x4.                                 */
+/* holds B[8g .. 8g+7][r]; D rows d_row_bf16(g, i).                          */
+/* K ~ Kh + Kl (bf16), P ~ Ph + Pm (+ Pl) (bf16, pre-split in fragment      */
+/* order).  NPROD = 3: Kh Ph + Kh Pm + Kl Ph (P split in two);               */
+/* NPROD = 5: + Kh Pl + Kl Pm (P split in three).  Products are exact in     */
+/* FP32; the FP32 accumulator covers one 32-point k-step and is folded into  */
+/* FP64 sums after every step (long FP32 runs lose the small terms of the    */
+/* large-dynamic-range pair densities).                                      */
 /* Workgroup = 4 waves sharing 16*MB rows; wave w owns columns               */
-/* [64 w, 64 w + 64) of the workgroup's 256.  Each k-step (32 points) the    */
-/* workgroup generates its K tile once, split hi/lo, into LDS in A-fragment  */
-/* order; products hi*hi + hi*lo + lo*hi; FP64 sums every 256 points.        */
+/* [64 w, 64 w + 64) of the workgroup's 256.  Each k-step the workgroup      */
+/* generates its K tile once, split hi/lo, into LDS in A-fragment order      */
+/* (double buffered); B fragments and the next step's r2 coordinates are     */
+/* prefetched into registers one step ahead.                                 */
 /* ------------------------------------------------------------------------ */
-template <int MB, class Kern>
-__global__ __launch_bounds__(256) void fgemm_bf16x3_kernel(const double* __restrict__ p1, int64_t n1,
-                                                           const float4* __restrict__ x2, int64_t n2p,
-                                                           const us8* __restrict__ Phi, const us8* __restrict__ Plo,
-                                                           int64_t ncb, void* __restrict__ Fout, int out_f32,
-                                                           int64_t npair, double scale, Kern kern) {
+template <int MB, int NPROD, class Kern>
+__global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
+                                                         const float4* __restrict__ x2, int64_t n2p,
+                                                         const us8* __restrict__ P0, int64_t plane,
+                                                         int64_t ncb, void* __restrict__ Fout, int out_f32,
+                                                         int64_t npair, double scale, Kern kern) {
   constexpr int NC = Kern::ncomp;
-  constexpr int NB = 4;  // 16-column blocks per wave
-  // K tile in LDS: [NC][MB][hi/lo][64 lanes] of 8 bf16, double buffered.
+  constexpr int NB = 4;                   // 16-column blocks per wave
+  constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
+  constexpr int VALS = MB * 64 * 8;       // K values per component per k-step
+  constexpr int EPT = VALS / 256;         // per thread
+  static_assert(EPT >= 1 && EPT <= 8 && VALS % 256 == 0, "MB must be 1, 2 or 4");
   __shared__ us8 ktile[2][NC][MB][2][64];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, r = lane & 15;
   const int64_t row0 = (int64_t)blockIdx.y * (16 * MB);
-  const int64_t cb0 = (int64_t)blockIdx.x * 16 + wave * NB;  // first 16-col block of this wave
+  const int64_t cb0 = (int64_t)blockIdx.x * 16 + wave * NB;
 
-  // Generation work: NC*MB fragments of 64 lanes x 8 values; thread tid
-  // produces element e0..e0+EPT-1 of fragment-lane (gm, gl).
-  constexpr int VALS = MB * 64 * 8;  // values per component per k-step
-  constexpr int EPT = VALS / 256;    // values per thread per component (MB=4 -> 8)
-  static_assert(EPT >= 1 && VALS % 256 == 0, "MB must be a multiple of 1/2");
-  const int gi = tid * EPT;           // flat index of the first value
-  const int gm = gi / 512;            // m-block
-  const int gl = (gi / 8) & 63;       // fragment lane
-  const int ge = gi & 7;              // first element within the lane's 8
-  const int64_t grow = row0 + 16 * gm + (gl & 15);
-  const int gk = 8 * (gl >> 4) + ge;  // k offset within the 32-point step
+  // this thread generates elements ge .. ge+EPT-1 of fragment lane gl of m-block gm
+  const int gi = tid * EPT;
+  const int gm = gi / 512;
+  const int gl = (gi / 8) & 63;
+  const int ge = gi & 7;
+  const int gk = 8 * (gl >> 4) + ge;
   float ax, ay, az;
   {
+    const int64_t grow = row0 + 16 * gm + (gl & 15);
     const bool ok = grow < n1;
     ax = ok ? (float)p1[grow] : (float)-kFar;
     ay = ok ? (float)p1[n1 + grow] : (float)-kFar;
     az = ok ? (float)p1[2 * n1 + grow] : (float)-kFar;
   }
 
-  auto generate = [&](int buf, int64_t k0) {
-    unsigned short hv[NC][EPT], lv[NC][EPT];
+  float4 qn[EPT];  // r2 coordinates of the step to generate next
+  auto load_coords = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) qn[e] = x2[k0 + gk + e];
+  };
+  auto generate = [&](int buf) {
+    us8 hv[NC], lv[NC];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const float4 q = x2[k0 + gk + e];
       float v[NC];
-      kern(ax - q.x, ay - q.y, az - q.z, v);
+      kern(ax - qn[e].x, ay - qn[e].y, az - qn[e].z, v);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         const unsigned short h = bf16_bits_rne(v[c]);
@@ -519,42 +538,49 @@ __global__ __launch_bounds__(256) void fgemm_bf16x3_kernel(const double* __restr
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      unsigned short* dh = (unsigned short*)&ktile[buf][c][gm][0][gl] + ge;
-      unsigned short* dl = (unsigned short*)&ktile[buf][c][gm][1][gl] + ge;
+      if constexpr (EPT == 8) {
+        ktile[buf][c][gm][0][gl] = hv[c];
+        ktile[buf][c][gm][1][gl] = lv[c];
+      } else {
+        unsigned short* dh = (unsigned short*)&ktile[buf][c][gm][0][gl] + ge;
+        unsigned short* dl = (unsigned short*)&ktile[buf][c][gm][1][gl] + ge;
 #pragma unroll
-      for (int e = 0; e < EPT; ++e) {
-        dh[e] = hv[c][e];
-        dl[e] = lv[c][e];
+        for (int e = 0; e < EPT; ++e) {
+          dh[e] = hv[c][e];
+          dl[e] = lv[c][e];
+        }
       }
     }
   };
 
-  f4 acc[NC][MB][NB];
+  us8 bcur[NSPLIT][NB], bnext[NSPLIT][NB];
+  auto load_b = [&](us8 (&dst)[NSPLIT][NB], int64_t s) {
+#pragma unroll
+    for (int t = 0; t < NSPLIT; ++t)
+#pragma unroll
+      for (int n = 0; n < NB; ++n) dst[t][n] = P0[t * plane + (s * ncb + cb0 + n) * 64 + lane];
+  };
+
   double sum[NC][MB][NB][4];
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
     for (int m = 0; m < MB; ++m)
 #pragma unroll
-      for (int n = 0; n < NB; ++n) {
-        acc[c][m][n] = f4{0, 0, 0, 0};
+      for (int n = 0; n < NB; ++n)
 #pragma unroll
         for (int i = 0; i < 4; ++i) sum[c][m][n][i] = 0;
-      }
 
   const int64_t nsteps = n2p / 32;
-  generate(0, 0);
+  load_coords(0);
+  generate(0);
+  if (nsteps > 1) load_coords(32);
+  load_b(bcur, 0);
   __syncthreads();
   for (int64_t s = 0; s < nsteps; ++s) {
     const int buf = (int)(s & 1);
-    if (s + 1 < nsteps) generate(buf ^ 1, (s + 1) * 32);
-    us8 bh[NB], bl[NB];
-#pragma unroll
-    for (int n = 0; n < NB; ++n) {
-      const int64_t frag = s * ncb + cb0 + n;
-      bh[n] = Phi[frag * 64 + lane];
-      bl[n] = Plo[frag * 64 + lane];
-    }
+    if (s + 1 < nsteps) load_b(bnext, s + 1);
+    f4 acc[NC][MB][NB];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -563,25 +589,35 @@ __global__ __launch_bounds__(256) void fgemm_bf16x3_kernel(const double* __restr
         const bf8 al = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
-          const bf8 h = __builtin_bit_cast(bf8, bh[n]);
-          const bf8 l = __builtin_bit_cast(bf8, bl[n]);
-          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, h, acc[c][m][n], 0, 0, 0);
-          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, l, acc[c][m][n], 0, 0, 0);
-          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, h, acc[c][m][n], 0, 0, 0);
+          const bf8 ph = __builtin_bit_cast(bf8, bcur[0][n]);
+          const bf8 pm = __builtin_bit_cast(bf8, bcur[1][n]);
+          f4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, f4{0, 0, 0, 0}, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pm, a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, a, 0, 0, 0);
+          if constexpr (NPROD == 5) {
+            const bf8 pl = __builtin_bit_cast(bf8, bcur[NSPLIT - 1][n]);
+            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, a, 0, 0, 0);
+            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, pm, a, 0, 0, 0);
+          }
+          acc[c][m][n] = a;
         }
       }
-    if ((s & 7) == 7 || s + 1 == nsteps) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-          for (int n = 0; n < NB; ++n) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
-            acc[c][m][n] = f4{0, 0, 0, 0};
-          }
+    if (s + 1 < nsteps) {
+      generate(buf ^ 1);
+      if (s + 2 < nsteps) load_coords((s + 2) * 32);
     }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
+#pragma unroll
+    for (int t = 0; t < NSPLIT; ++t)
+#pragma unroll
+      for (int n = 0; n < NB; ++n) bcur[t][n] = bnext[t][n];
     __syncthreads();
   }
 #pragma unroll
@@ -659,37 +695,68 @@ int launch_f32(const double* p1, int64_t n1, const float4* x2, int64_t n2p, cons
   return 0;
 }
 
-template <int MB, class Kern>
-int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* Phi, const us8* Plo,
-                int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+template <int MB, int NPROD, class Kern>
+int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane, int64_t ncb,
+                void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
   dim3 grid((unsigned)(ncb / 16), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
-  hipLaunchKernelGGL((fgemm_bf16x3_kernel<MB, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, Phi, Plo, ncb, F,
+  hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0, plane, ncb, F,
                      out_f32, npair, scale, kern);
   TC_HIP(hipGetLastError());
   return 0;
 }
 
+// Rows per workgroup of the BF16 kernel (16 * MB); TCSCF_BF16_MB overrides for tuning.
+int bf16_mb(int ncomp) {
+  static int env = [] {
+    const char* e = std::getenv("TCSCF_BF16_MB");
+    return e ? std::atoi(e) : 0;
+  }();
+  if (env == 1 || env == 2 || env == 4) return ncomp == 1 ? env : 1;
+  return ncomp == 1 ? 4 : 1;
+}
+
+template <int NPROD, class Kern>
+int launch_bf16_mb(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane,
+                   int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+  if constexpr (Kern::ncomp == 1) {
+    switch (bf16_mb(1)) {
+      case 1: return launch_bf16<1, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      case 2: return launch_bf16<2, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      default: return launch_bf16<4, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+    }
+  } else {
+    return launch_bf16<1, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+  }
+}
+
 template <class Kern>
-int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, const double* w2, int64_t n2,
-             const double* A2, int nb, double scale, void* F, int out_f32, Kern kern, hipStream_t st) {
+int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, const double* w2, const int* ord2,
+             int64_t n2, const double* A2, int nb, double scale, void* F, int out_f32, Kern kern, hipStream_t st) {
   const int64_t npair = npair_of(nb);
   int rc = 0;
-  if (precision == TCSCF_BF16X3) {
+  if (precision == TCSCF_BF16X3 || precision == TCSCF_BF16X5) {
+    const int nsplit = precision == TCSCF_BF16X3 ? 2 : 3;
     const int64_t n2p = round_up(n2, 32), ncb = round_up(npair, 256) / 16, nkb = n2p / 32;
     char* ws = (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
     if (rc) return rc;
-    unsigned short* hi = (unsigned short*)ws;
-    unsigned short* lo = hi + nkb * ncb * 512;
-    double* xd = (double*)(lo + nkb * ncb * 512);
+    unsigned short* planes = (unsigned short*)ws;
+    const int64_t plane = nkb * ncb * 512;  // bf16 per plane
+    double* xd = (double*)(planes + nsplit * plane);
     float4* x4 = (float4*)(xd + 3 * n2p);
-    const int64_t tot = nkb * ncb * 512;
-    hipLaunchKernelGGL(pack_pairs_bf16x2, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A2, w2, n2, nb, npair,
-                       nkb, ncb, hi, lo);
+    const unsigned gp = (unsigned)((plane + 255) / 256);
+    if (nsplit == 2)
+      hipLaunchKernelGGL(pack_pairs_bf16<2>, dim3(gp), dim3(256), 0, st, A2, w2, ord2, n2, nb, npair, nkb, ncb, planes);
+    else
+      hipLaunchKernelGGL(pack_pairs_bf16<3>, dim3(gp), dim3(256), 0, st, A2, w2, ord2, n2, nb, npair, nkb, ncb, planes);
     TC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(pad_coords, dim3((unsigned)((n2p + 255) / 256)), dim3(256), 0, st, pts2, n2, n2p, xd, x4);
+    hipLaunchKernelGGL(pad_coords, dim3((unsigned)((n2p + 255) / 256)), dim3(256), 0, st, pts2, ord2, n2, n2p, xd, x4);
     TC_HIP(hipGetLastError());
-    if constexpr (Kern::ncomp == 1) return launch_bf16<4>(pts1, n1, x4, n2p, (const us8*)hi, (const us8*)lo, ncb, F, out_f32, npair, scale, kern, st);
-    else return launch_bf16<2>(pts1, n1, x4, n2p, (const us8*)hi, (const us8*)lo, ncb, F, out_f32, npair, scale, kern, st);
+    // planes are addressed in 16-byte fragment-lane units
+    const us8* P0 = (const us8*)planes;
+    const int64_t plane8 = plane / 8;
+    if (precision == TCSCF_BF16X3)
+      return launch_bf16_mb<3>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st);
+    return launch_bf16_mb<5>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st);
   }
   const int64_t n2p = round_up(n2, 16), ldp = round_up(npair, 128);
   char* ws = (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
@@ -698,17 +765,17 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
   double* xd = (double*)(ws + (size_t)n2p * ldp * el);
   float4* x4 = (float4*)(xd + 3 * n2p);
   const int64_t tot = n2p * ldp;
-  hipLaunchKernelGGL(pad_coords, dim3((unsigned)((n2p + 255) / 256)), dim3(256), 0, st, pts2, n2, n2p, xd, x4);
+  hipLaunchKernelGGL(pad_coords, dim3((unsigned)((n2p + 255) / 256)), dim3(256), 0, st, pts2, ord2, n2, n2p, xd, x4);
   TC_HIP(hipGetLastError());
   if (precision == TCSCF_FP64) {
-    hipLaunchKernelGGL(pack_pairs_rowmajor<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A2, w2, n2,
-                       nb, npair, n2p, ldp, (double*)ws);
+    hipLaunchKernelGGL(pack_pairs_rowmajor<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A2, w2, ord2,
+                       n2, nb, npair, n2p, ldp, (double*)ws);
     TC_HIP(hipGetLastError());
     if constexpr (Kern::ncomp == 1) return launch_f64<4>(pts1, n1, xd, n2p, (const double*)ws, ldp, F, out_f32, npair, scale, kern, st);
     else return launch_f64<2>(pts1, n1, xd, n2p, (const double*)ws, ldp, F, out_f32, npair, scale, kern, st);
   }
-  hipLaunchKernelGGL(pack_pairs_rowmajor<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A2, w2, n2, nb,
-                     npair, n2p, ldp, (float*)ws);
+  hipLaunchKernelGGL(pack_pairs_rowmajor<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A2, w2, ord2, n2,
+                     nb, npair, n2p, ldp, (float*)ws);
   TC_HIP(hipGetLastError());
   if constexpr (Kern::ncomp == 1) return launch_f32<4>(pts1, n1, x4, n2p, (const float*)ws, ldp, F, out_f32, npair, scale, kern, st);
   else return launch_f32<1>(pts1, n1, x4, n2p, (const float*)ws, ldp, F, out_f32, npair, scale, kern, st);
@@ -742,9 +809,10 @@ int tcscf_basis_eval(const double* pts, int64_t n, const tcscf_sto* fns, int nb,
 
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision) {
   const int64_t npair = npair_of(nb);
-  if (precision == TCSCF_BF16X3) {
+  if (precision == TCSCF_BF16X3 || precision == TCSCF_BF16X5) {
     const int64_t n2p = round_up(n2, 32), ncb = round_up(npair, 256) / 16;
-    return (size_t)(n2p / 32) * ncb * 512 * 2 * 2 + (size_t)n2p * (3 * 8 + 16) + 4096;
+    const size_t nsplit = precision == TCSCF_BF16X3 ? 2 : 3;
+    return (size_t)(n2p / 32) * ncb * 512 * 2 * nsplit + (size_t)n2p * (3 * 8 + 16) + 4096;
   }
   const int64_t n2p = round_up(n2, 16), ldp = round_up(npair, 128);
   const size_t el = precision == TCSCF_FP64 ? 8 : 4;
@@ -752,27 +820,27 @@ size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision) {
 }
 
 
-int tcscf_compute_F(const double* pts1, int64_t n1, const double* pts2, const double* w2, int64_t n2,
-                    const double* A2, int nb, int kernel, const tcscf_jastrow* jp, int same_spin, int precision,
-                    double scale, void* F, int out_dtype, void* stream) {
+int tcscf_compute_F(const double* pts1, int64_t n1, const double* pts2, const double* w2, const int32_t* ord2,
+                    int64_t n2, const double* A2, int nb, int kernel, const tcscf_jastrow* jp, int same_spin,
+                    int precision, double scale, void* F, int out_dtype, void* stream) {
   TC_CHECK(pts1 && pts2 && w2 && A2 && F && n1 > 0 && n2 > 0 && nb > 0, "tcscf_compute_F: bad arguments");
-  TC_CHECK(precision >= 0 && precision <= 2, "tcscf_compute_F: unknown precision");
+  TC_CHECK(precision >= 0 && precision <= 3, "tcscf_compute_F: unknown precision");
   TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F: unknown out_dtype");
   TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F: grid too large");
   hipStream_t st = (hipStream_t)stream;
   const int of = out_dtype == TCSCF_F32;
   switch (kernel) {
     case TCSCF_K_COULOMB:
-      return dispatch(precision, pts1, n1, pts2, w2, n2, A2, nb, scale, F, of, KernCoulomb{}, st);
+      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernCoulomb{}, st);
     case TCSCF_K_TC_SCALAR: {
       TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: TC kernel needs jastrow parameters with a12 > 0");
       const double c = same_spin ? jp->c_same : jp->c_oppo;
-      return dispatch(precision, pts1, n1, pts2, w2, n2, A2, nb, scale, F, of,
+      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of,
                       KernTCScalar{jp->a12, 2 * c * c * jp->a12 * jp->a12}, st);
     }
     case TCSCF_K_JASTROW_GRAD:
       TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: gradient kernel needs jastrow parameters with a12 > 0");
-      return dispatch(precision, pts1, n1, pts2, w2, n2, A2, nb, scale, F, of, KernJastrowGrad{jp->a12}, st);
+      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernJastrowGrad{jp->a12}, st);
     default:
       TC_CHECK(false, "tcscf_compute_F: unknown kernel");
   }

@@ -141,7 +141,7 @@ def test_F_fp64_tc_kernels(tc, O, gpu):
     assert rel_err(full, ref) < 1e-12
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("bf16x3", 3e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("bf16x3", 3e-5), ("bf16x5", 3e-6)])
 def test_F_mixed_precision_error(tc, gpu, precision, tol):
     """Mixed-precision F against the FP64 drop-in on the Ne basis (nb = 30):
     max element error relative to max |F| (tolerance in the parameters)."""
@@ -153,6 +153,32 @@ def test_F_mixed_precision_error(tc, gpu, precision, tol):
     F = tc.computeF(g, g, precision=precision).cpu().numpy()
     err = np.max(np.abs(F - F64)) / np.max(np.abs(F64))
     assert err < tol, err
+
+
+def test_F_summation_order_is_a_permutation(tc, gpu):
+    """Summing r2 in radius order or in grid order gives the same FP64 F."""
+    from tcscf_amd import configs
+    c = configs.ne_config2()
+    g = _grid(tc, 20, 17, tc.RADIAL_M4)
+    g.setBasisFunctions(c["basis"], False)
+    a = tc.computeF(g, g, ordered=True).cpu().numpy()
+    b = tc.computeF(g, g, ordered=False).cpu().numpy()
+    assert rel_err(a, b) < 1e-13
+
+
+def test_mixed_precision_energy_ne(tc, gpu):
+    """Ne RHF on a 50 x 194-point grid: the BF16x5 and FP32 energies lie
+    within 1e-6 Eh of the FP64 drop-in (BF16x3 is reported, not held)."""
+    from tcscf_amd import configs
+    c = configs.ne_config2()
+    g, S, H = _scf_inputs(tc, c, (50, 23, tc.RADIAL_M4))
+    e = {}
+    for prec in ("fp64", "fp32", "bf16x5", "bf16x3"):
+        F = tc.computeF(g, g, precision=prec)
+        e[prec] = tc.RCSHartreeFock(10, 30).compute(S, H, g, F, True, tol=1e-11, max_iter=100)
+    print({k: v - e["fp64"] for k, v in e.items()})
+    assert abs(e["bf16x5"] - e["fp64"]) < 1e-6
+    assert abs(e["fp32"] - e["fp64"]) < 1e-6
 
 
 # ----------------------------------------------------------------- SCF
