@@ -26,111 +26,140 @@ __device__ inline int64_t pidx(int j, int i) { return j <= i ? (int64_t)i * (i +
 //   Fai_s[n,i] = sum_j (sum_k D_s[k,j] A[n,k]) F[n,j,i]       (calculateFai, HartreeFock.cpp:144-179)
 // RCS (NSPIN = 1): c_self = 2, c_other = 0   (HartreeFock.cpp:828-842)
 // UOS (NSPIN = 2): c_self = c_other = 1      (HartreeFock.cpp:940-955)
-// One wave handles one r1 point at a time; the workgroup stages CH points'
-// A rows and Y_s[n,i] = w_n (...) in LDS and folds them into register-held
-// partial G with a rank-CH update.
-template <class TF, int NSPIN>
+// D is symmetric (C C^T), so D[i][j] = D[j][i].
+// Lane layout: a wave works on PPW = 64 / NBP points at once, lane = q * NBP + i
+// (point q of the wave, basis index i < nb <= NBP).  The workgroup stages D in
+// LDS once, walks its r1 range in chunks of CH points, stages each point's
+// packed F row and A row, forms Y_s[n,i] = w_n (...) for the chunk, then folds
+// the chunk into register-held partial G with a rank-CH update.
+template <class TF, int NSPIN, int NBP>
 __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F, int64_t npair,
                                                        const double* __restrict__ A, const double* __restrict__ w,
-                                                       int64_t n1, int nb, int ch, const double* __restrict__ D,
+                                                       int64_t n1, int nb, const double* __restrict__ D,
                                                        double c_self, double c_other, double* __restrict__ Gpart) {
+  constexpr int PPW = 64 / NBP;  // points per wave at a time
+  constexpr int CH = NBP == 64 ? 16 : 32;  // points per chunk
+  constexpr int KG = NSPIN * NBP * NBP / 256;  // G elements per thread
   extern __shared__ double smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double* Frow = smem;                          // [4][npair]
-  double* Abuf = Frow + 4 * npair;              // [ch][nb]
-  double* Ybuf = Abuf + (int64_t)ch * nb;       // [ch][NSPIN][nb]
-  double* Xbuf = Ybuf + (int64_t)ch * NSPIN * nb;  // [4][NSPIN][nb]
+  const int q = lane / NBP, i = lane % NBP;
   const int64_t nb2 = (int64_t)nb * nb;
+  double* Ds = smem;                                   // [NSPIN][nb][nb]
+  double* Frow = Ds + NSPIN * nb2;                     // [4][PPW][npair]
+  double* Abuf = Frow + 4 * PPW * npair;               // [CH][nb]
+  double* Ybuf = Abuf + CH * nb;                       // [CH][NSPIN][nb]
+  double* Xb = Ybuf + CH * NSPIN * nb;                 // [4][PPW][NSPIN][nb]
+  for (int64_t e = tid; e < NSPIN * nb2; e += 256) Ds[e] = D[e];
   const int64_t per = (n1 + gridDim.x - 1) / gridDim.x;
   const int64_t p0 = (int64_t)blockIdx.x * per;
   const int64_t p1 = p0 + per < n1 ? p0 + per : n1;
-  double gacc[kGPerThread];
+  double gacc[KG];
 #pragma unroll
-  for (int q = 0; q < kGPerThread; ++q) gacc[q] = 0;
+  for (int t = 0; t < KG; ++t) gacc[t] = 0;
+  const int tri_i = i * (i + 1) / 2;
+  __syncthreads();
 
-  for (int64_t c0 = p0; c0 < p1; c0 += ch) {
-    const int cn = (int)((c0 + ch < p1) ? ch : p1 - c0);
-    for (int pt = wave; pt < cn; pt += 4) {
-      const int64_t n = c0 + pt;
-      double* fr = Frow + wave * npair;
-      for (int64_t p = lane; p < npair; p += 64) fr[p] = (double)F[n * npair + p];
-      double* ab = Abuf + (int64_t)pt * nb;
-      for (int j = lane; j < nb; j += 64) ab[j] = A[n * nb + j];
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      double* xb = Xbuf + wave * NSPIN * nb;
-      for (int j = lane; j < nb; j += 64)
-        for (int s = 0; s < NSPIN; ++s) {
-          const double* Ds = D + s * nb2;
-          double x = 0;
-          for (int k = 0; k < nb; ++k) x += Ds[k * nb + j] * ab[k];
-          xb[s * nb + j] = x;
+  for (int64_t c0 = p0; c0 < p1; c0 += CH) {
+    const int cn = (int)((c0 + CH < p1) ? CH : p1 - c0);
+    for (int pb = wave * PPW; pb < cn; pb += 4 * PPW) {
+      // stage F rows and A rows of points pb .. pb+PPW-1
+      double* fr = Frow + (int64_t)wave * PPW * npair;
+      for (int qq = 0; qq < PPW; ++qq) {
+        const int pt = pb + qq;
+        if (pt < cn) {
+          const TF* src = F + (c0 + pt) * npair;
+          for (int64_t p = lane; p < npair; p += 64) fr[qq * npair + p] = (double)src[p];
         }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      double faa[NSPIN], fai[NSPIN][2];
-      for (int s = 0; s < NSPIN; ++s) {
-        faa[s] = 0;
-        fai[s][0] = fai[s][1] = 0;
       }
-      for (int t = 0; t < 2; ++t) {
-        const int i = lane + 64 * t;
-        if (i >= nb) continue;
+      const int pt = pb + q;
+      const bool live = pt < cn && i < nb;
+      const int64_t n = c0 + pt;
+      double* ab = Abuf + (int64_t)(live ? pt : 0) * nb;
+      if (live) ab[i] = A[n * nb + i];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      double* xb = Xb + ((int64_t)wave * PPW + q) * NSPIN * nb;
+      if (live) {
+#pragma unroll
+        for (int s = 0; s < NSPIN; ++s) {
+          const double* Dd = Ds + s * nb2;
+          double x = 0;
+          for (int k = 0; k < nb; ++k) x += Dd[k * nb + i] * ab[k];
+          xb[s * nb + i] = x;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      double faa[NSPIN], fai[NSPIN];
+#pragma unroll
+      for (int s = 0; s < NSPIN; ++s) faa[s] = fai[s] = 0;
+      if (live) {
+        const double* f = fr + q * npair;
         for (int j = 0; j < nb; ++j) {
-          const double f = fr[pidx(j, i)];
+          const double fv = f[j <= i ? tri_i + j : j * (j + 1) / 2 + i];
+#pragma unroll
           for (int s = 0; s < NSPIN; ++s) {
-            faa[s] += f * D[s * nb2 + (int64_t)j * nb + i];  // D symmetric: D[i][j] == D[j][i]
-            fai[s][t] += xb[s * nb + j] * f;
+            faa[s] += fv * Ds[s * nb2 + j * nb + i];
+            fai[s] += xb[s * nb + j] * fv;
           }
         }
       }
-      for (int s = 0; s < NSPIN; ++s) {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) faa[s] += __shfl_xor(faa[s], off, 64);
-      }
-      const double wn = w[n];
-      double* yb = Ybuf + (int64_t)pt * NSPIN * nb;
-      for (int t = 0; t < 2; ++t) {
-        const int i = lane + 64 * t;
-        if (i >= nb) continue;
+      for (int s = 0; s < NSPIN; ++s)
+#pragma unroll
+        for (int off = NBP / 2; off > 0; off >>= 1) faa[s] += __shfl_xor(faa[s], off, 64);
+      if (live) {
+        const double wn = w[n];
+        double* yb = Ybuf + (int64_t)pt * NSPIN * nb;
+#pragma unroll
         for (int s = 0; s < NSPIN; ++s) {
           const double other = NSPIN == 2 ? faa[1 - s] : 0.0;
-          yb[s * nb + i] = wn * (ab[i] * (c_self * faa[s] + c_other * other) - fai[s][t]);
+          yb[s * nb + i] = wn * (ab[i] * (c_self * faa[s] + c_other * other) - fai[s]);
         }
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
     __syncthreads();
-    // rank-cn update of the thread's G elements: e = tid + 256 q -> (s, j, i)
 #pragma unroll
-    for (int q = 0; q < kGPerThread; ++q) {
-      const int64_t e = tid + 256 * q;
+    for (int t = 0; t < KG; ++t) {
+      const int64_t e = tid + 256 * t;
       if (e < NSPIN * nb2) {
         const int s = (int)(e / nb2);
         const int64_t ji = e - s * nb2;
-        const int j = (int)(ji / nb), i = (int)(ji - (int64_t)j * nb);
+        const int j = (int)(ji / nb), ii = (int)(ji - (int64_t)j * nb);
         double acc = 0;
-        for (int pt = 0; pt < cn; ++pt) acc += Abuf[pt * nb + j] * Ybuf[((int64_t)pt * NSPIN + s) * nb + i];
-        gacc[q] += acc;
+        for (int pt = 0; pt < cn; ++pt) acc += Abuf[pt * nb + j] * Ybuf[((int64_t)pt * NSPIN + s) * nb + ii];
+        gacc[t] += acc;
       }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < kGPerThread; ++q) {
-    const int64_t e = tid + 256 * q;
-    if (e < NSPIN * nb2) Gpart[(int64_t)blockIdx.x * NSPIN * nb2 + e] = gacc[q];
+  for (int t = 0; t < KG; ++t) {
+    const int64_t e = tid + 256 * t;
+    if (e < NSPIN * nb2) Gpart[(int64_t)blockIdx.x * NSPIN * nb2 + e] = gacc[t];
   }
 }
 
-// Deterministic sum of per-workgroup partials: out[e] = sum_b part[b][e].
-__global__ void reduce_partials(const double* __restrict__ part, int nparts, int64_t len, double* __restrict__ out) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= len) return;
+// Deterministic sum of per-workgroup partials, one workgroup per element:
+// out[e] = sum_b part[b][e] (fixed tree order).
+__global__ __launch_bounds__(256) void reduce_partials(const double* __restrict__ part, int nparts, int64_t len,
+                                                       double* __restrict__ out) {
+  __shared__ double red[256];
+  const int64_t e = blockIdx.x;
   double s = 0;
-  for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * len + e];
-  out[e] = s;
+  for (int b = threadIdx.x; b < nparts; b += 256) s += part[(int64_t)b * len + e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[e] = red[0];
 }
 
 namespace {
@@ -151,11 +180,12 @@ struct Sweep {
   double* dD = nullptr;
   double* dPart = nullptr;
   double* dG = nullptr;
-  int ch = 16;
+  int nbp = 32;
   size_t shmem = 0;
 
   int init() {
     const int64_t nb2 = (int64_t)nb * nb;
+    TC_CHECK(nb <= 64, "SCF sweep: nb > 64 is not supported by the sweep kernel");
     TC_CHECK(nspin * nb2 <= 256 * kGPerThread, "SCF sweep: nb too large for the register-held G partials");
     int rc = 0;
     char* p = (char*)scratch(kSlotSweep, sizeof(double) * (nspin * nb2 * (kSweepBlocks + 2)), &rc);
@@ -163,11 +193,19 @@ struct Sweep {
     dPart = (double*)p;
     dD = dPart + (int64_t)kSweepBlocks * nspin * nb2;
     dG = dD + nspin * nb2;
-    // LDS: Frow 4*npair + ch*nb*(1+nspin) + 4*nspin*nb doubles, under 64 KiB
-    ch = 32;
-    while (ch > 4 && sizeof(double) * (4 * npair + (int64_t)ch * nb * (1 + nspin) + 4 * nspin * nb) > 64 * 1024) ch /= 2;
-    shmem = sizeof(double) * (4 * npair + (int64_t)ch * nb * (1 + nspin) + 4 * nspin * nb);
-    TC_CHECK(shmem <= 64 * 1024, "SCF sweep: nb too large for the LDS staging");
+    nbp = nb <= 32 ? 32 : 64;
+    const int ppw = 64 / nbp, ch = nbp == 64 ? 16 : 32;
+    shmem = sizeof(double) * (nspin * nb2 + 4 * ppw * npair + ch * nb * (1 + nspin) + 4 * ppw * nspin * nb);
+    TC_CHECK(shmem <= 160 * 1024, "SCF sweep: nb too large for the LDS staging");
+    if (shmem > 64 * 1024) {
+      // opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU)
+      const void* fns[8] = {(const void*)hf_sweep_kernel<float, 1, 32>,  (const void*)hf_sweep_kernel<float, 1, 64>,
+                            (const void*)hf_sweep_kernel<double, 1, 32>, (const void*)hf_sweep_kernel<double, 1, 64>,
+                            (const void*)hf_sweep_kernel<float, 2, 32>,  (const void*)hf_sweep_kernel<float, 2, 64>,
+                            (const void*)hf_sweep_kernel<double, 2, 32>, (const void*)hf_sweep_kernel<double, 2, 64>};
+      for (const void* f : fns)
+        TC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    }
     return 0;
   }
 
@@ -176,25 +214,20 @@ struct Sweep {
     TC_HIP(hipMemcpyAsync(dD, D_host, sizeof(double) * nspin * nb2, hipMemcpyHostToDevice, st));
     const double cs = nspin == 1 ? 2.0 : 1.0, co = nspin == 1 ? 0.0 : 1.0;
     dim3 grid(kSweepBlocks), block(256);
+#define TC_SWEEP(TF, NS, NBP)                                                                               \
+  hipLaunchKernelGGL((hf_sweep_kernel<TF, NS, NBP>), grid, block, shmem, st, (const TF*)F, npair, A, w, n1, nb, \
+                     dD, cs, co, dPart)
     if (nspin == 1) {
-      if (f32)
-        hipLaunchKernelGGL((hf_sweep_kernel<float, 1>), grid, block, shmem, st, (const float*)F, npair, A, w, n1, nb, ch,
-                           dD, cs, co, dPart);
-      else
-        hipLaunchKernelGGL((hf_sweep_kernel<double, 1>), grid, block, shmem, st, (const double*)F, npair, A, w, n1, nb,
-                           ch, dD, cs, co, dPart);
+      if (f32) { if (nbp == 32) TC_SWEEP(float, 1, 32); else TC_SWEEP(float, 1, 64); }
+      else { if (nbp == 32) TC_SWEEP(double, 1, 32); else TC_SWEEP(double, 1, 64); }
     } else {
-      if (f32)
-        hipLaunchKernelGGL((hf_sweep_kernel<float, 2>), grid, block, shmem, st, (const float*)F, npair, A, w, n1, nb, ch,
-                           dD, cs, co, dPart);
-      else
-        hipLaunchKernelGGL((hf_sweep_kernel<double, 2>), grid, block, shmem, st, (const double*)F, npair, A, w, n1, nb,
-                           ch, dD, cs, co, dPart);
+      if (f32) { if (nbp == 32) TC_SWEEP(float, 2, 32); else TC_SWEEP(float, 2, 64); }
+      else { if (nbp == 32) TC_SWEEP(double, 2, 32); else TC_SWEEP(double, 2, 64); }
     }
+#undef TC_SWEEP
     TC_HIP(hipGetLastError());
     const int64_t len = nspin * nb2;
-    hipLaunchKernelGGL(reduce_partials, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, dPart, kSweepBlocks, len,
-                       dG);
+    hipLaunchKernelGGL(reduce_partials, dim3((unsigned)len), dim3(256), 0, st, dPart, kSweepBlocks, len, dG);
     TC_HIP(hipGetLastError());
     TC_HIP(hipMemcpyAsync(G_host, dG, sizeof(double) * len, hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));

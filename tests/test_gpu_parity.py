@@ -245,6 +245,29 @@ def test_rcs_ne_config2_full_size(tc, O, gpu):
     assert abs(e - ref["energy"]) < 1e-10, (e, ref["energy"])
 
 
+@pytest.mark.parametrize("shells,Z,nup,ndown", [({0: 10, 1: 6, 2: 2}, 10, 5, 5), ({0: 8, 1: 3}, 3, 2, 1)])
+def test_scf_wide_basis_and_open_shell(tc, O, gpu, shells, Z, nup, ndown):
+    """nb = 38 (> 32: one r1 point per wave in the sweep) RCS + UOS, and an
+    open-shell UOS (Li, 2 up / 1 down) against the oracle."""
+    basis = tc.even_tempered(shells, 0.4, 2.1)
+    g = _grid(tc, 30, 17, tc.RADIAL_M4)
+    g.setBasisFunctions(basis, False)
+    S, H = tc.one_electron_matrices(basis, Z)
+    F = tc.computeF(g, g)
+    A = g.basisValues.cpu().numpy()
+    Fo = tc.unpack(F.cpu().numpy(), len(basis))
+    if nup == ndown:
+        e = tc.RCSHartreeFock(nup + ndown, len(basis)).compute(S, H, g, F, True, tol=1e-10)
+        ref = O.rcs_scf(nup + ndown, S, H, A, g.weights_host, Fo, tol=1e-10)
+        assert abs(e - ref["energy"]) < 1e-10
+    u = tc.UOSHartreeFock(nup, ndown, len(basis))
+    eu = u.compute(S, H, g, F, True, tol=1e-10, max_iter=200)
+    refu = O.uos_scf(nup, ndown, S, H, A, g.weights_host, Fo, tol=1e-10, max_iter=200)
+    assert refu["status"] == 0
+    assert abs(eu - refu["energy"]) < 1e-10
+    assert np.max(np.abs(u.density - refu["D"])) < 1e-8
+
+
 def test_scf_error_paths(tc, gpu):
     from tcscf_amd import configs
     c = configs.he_config1()
