@@ -149,6 +149,28 @@ int tcscf_uos_scf(int nup, int ndown, int nb, const double* S, const double* H, 
                   const double* w1_dev, int64_t n1, const void* F_dev, int f_dtype, const tcscf_scf_opts* opts,
                   tcscf_scf_result* res, double* energy_trace, double* D_out, double* evals_out, void* stream);
 
+/* TCHartreeFock::compute (src/HartreeFock.cpp:1007-1362), real basis.  Fs_dev,
+ * Fo_dev: packed n1 x npair F_same / F_oppo (TCSCF_K_TC_SCALAR, FP64);
+ * V_dev: packed n1 x 3 x npair (TCSCF_K_JASTROW_GRAD, c = 1); A1_dev n1 x nb;
+ * dA1_dev n1 x nb x 3; S, H host.  The Fock build runs on the device, the FP64
+ * generalized non-Hermitian eigen-solve (role of LAPACK ggev, per symmetry
+ * block), the Re(lambda) sort and the MGS of the occupied orbitals on the host.
+ * Synchronous.  D_out 2 x nb x nb; pseudo_out nup + ndown occupied
+ * pseudo-energies. */
+int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1_dev,
+                 const double* dA1_dev, const double* w1_dev, int64_t n1, const double* Fs_dev, const double* Fo_dev,
+                 const double* V_dev, const tcscf_jastrow* jastrow, const tcscf_scf_opts* opts, tcscf_scf_result* res,
+                 double* energy_trace, double* D_out, double* pseudo_out, void* stream);
+
+/* One TC Fock build (the per-iteration sweeps and assembly of
+ * src/HartreeFock.cpp:1118-1224) from D_host (2 x nb x nb) and the occupied
+ * orbitals C_host (2 x nb x max(nup, ndown), column b of spin s used for
+ * b < nocc_s): writes G2 and G3 (2 x nb x nb each, host) before the mask. */
+int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1_dev, const double* dA1_dev, const double* w1_dev,
+                  int64_t n1, const double* Fs_dev, const double* Fo_dev, const double* V_dev,
+                  const tcscf_jastrow* jastrow, const double* D_host, const double* C_host, double* G2_host,
+                  double* G3_host, void* stream);
+
 /* ---------------------------------------------------------------- diagnostics */
 /* No reference counterpart: runs one MFMA fragment (which = 0: f64 16x16x4,
  * 1: f32 16x16x4, 2: bf16 16x16x32) on A_dev (16 x K) and B_dev (K x 16),

@@ -374,3 +374,54 @@ class UOSHartreeFock(_HFBase):
 
     def highestOccupiedOrbitalEnergy(self):
         return max(self.eigenvalues[0, self.nElectrons[0] - 1], self.eigenvalues[1, self.nElectrons[1] - 1])
+
+
+class TCHartreeFock(_HFBase):
+    """TCHartreeFock (src/HartreeFock.hpp:353-418, src/HartreeFock.cpp:1007-1362).
+
+    compute() takes the packed F_same / F_oppo (computeF with kernel='tc') and
+    the packed c = 1 Jastrow-gradient tensor V (computeV); V_same = c_same V and
+    V_oppo = c_oppo V are applied inside the Fock build.  mgs_mode 0 keeps the
+    reference's Gram-Schmidt (Euclidean norm after the first vector), 1 uses
+    the overlap metric throughout."""
+
+    def __init__(self, numSpinUp, numSpinDown, nBasis):
+        self.nElectrons = (numSpinUp, numSpinDown)
+        self.basisSize = nBasis
+        self.density = np.zeros((2, nBasis, nBasis))
+        self.occupiedOrbitalPseudoEnergy = [np.zeros(numSpinUp), np.zeros(numSpinDown)]
+        self._iters = 0
+
+    @staticmethod
+    def needsGradients():
+        return True
+
+    def compute(self, overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo, Vji, jastrow,
+                respectOneElectronSymmetry=True, tol=1e-5, max_iter=200, mgs_mode=0):
+        torch = _torch()
+        nb = self.basisSize
+        S, Sp = _np(overlap)
+        H, Hp = _np(oneElectronTerms)
+        if S.shape != (nb, nb) or H.shape != (nb, nb) or r1Grid.nBasis() != nb or not r1Grid.gradientsStored():
+            raise ValueError("TC: shape mismatch or basis gradients not stored on the r1 grid")
+        n1 = r1Grid.nGrid()
+        for T in (FjiSame, FjiOppo, Vji):
+            if T.dtype != torch.float64 or T.shape[0] != n1:
+                raise ValueError("TC: F/V must be float64 with one row per r1 point")
+        res = _lib.ScfResult()
+        trace = np.zeros(max_iter)
+        D = np.zeros((2, nb, nb))
+        ps = np.zeros(sum(self.nElectrons))
+        jp = jastrow._struct()
+        rc = lib().tcscf_tc_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
+                                _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(n1),
+                                _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), ctypes.byref(jp),
+                                ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, mgs_mode)),
+                                ctypes.byref(res), trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp),
+                                ps.ctypes.data_as(_dp), _stream())
+        self.density[:] = D
+        self.occupiedOrbitalPseudoEnergy = [ps[:self.nElectrons[0]].copy(), ps[self.nElectrons[0]:].copy()]
+        return self._finish(rc, res, trace)
+
+    def highestOccupiedOrbitalEnergy(self):
+        return max(self.occupiedOrbitalPseudoEnergy[0][-1], self.occupiedOrbitalPseudoEnergy[1][-1])

@@ -280,3 +280,55 @@ def test_scf_error_paths(tc, gpu):
     F2 = tc.computeF(g2, g2)
     with pytest.raises(tc.SolverError):
         tc.RCSHartreeFock(2, 6).compute(S2, H2, g2, F2, True)
+
+
+# ----------------------------------------------------------------- TC
+def _tc_case(tc, basis, grid1, grid3, b=1.0):
+    g1 = _grid(tc, *grid1)
+    g3 = _grid(tc, *grid3)
+    for g in (g1, g3):
+        g.setBasisFunctions(basis, True)
+    jas = tc.Jastrow.from_b(b)
+    Fs = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=True)
+    Fo = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=False)
+    V = tc.computeV(g1, g3, jas)
+    return g1, g3, jas, Fs, Fo, V
+
+
+@pytest.mark.parametrize("Z,nup,ndown,mgs_mode", [(4, 2, 2, 0), (4, 2, 2, 1), (3, 2, 1, 1), (2, 1, 1, 0)])
+def test_tc_scf_vs_oracle(tc, O, gpu, Z, nup, ndown, mgs_mode):
+    """TCHartreeFock on the device Fock build against the oracle's restatement,
+    fed the same F_same / F_oppo / V (full per-iteration energy trace)."""
+    basis = tc.even_tempered({0: 5, 1: 2}, 0.6, 2.2)
+    nb = len(basis)
+    g1, g3, jas, Fs, Fo, V = _tc_case(tc, basis, (14, 11, tc.RADIAL_M4), (10, 7, tc.RADIAL_M4))
+    S, H = tc.one_electron_matrices(basis, Z)
+    hf = tc.TCHartreeFock(nup, ndown, nb)
+    e = hf.compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-10, max_iter=200, mgs_mode=mgs_mode)
+    A = g1.basisValues.cpu().numpy()
+    dA = g1.basisGradients.cpu().numpy()
+    Vf = tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1)  # (n1, nb, nb, 3), c = 1
+    ref = O.tc_scf(nup, ndown, S, H, A, dA, g1.weights_host, tc.unpack(Fs.cpu().numpy(), nb),
+                   tc.unpack(Fo.cpu().numpy(), nb), np.ascontiguousarray(jas.c_same * Vf),
+                   np.ascontiguousarray(jas.c_oppo * Vf), tol=1e-10, max_iter=200, mgs_mode=mgs_mode)
+    assert ref["status"] == 0
+    n = min(len(hf.energy_trace), len(ref["trace"]))
+    assert np.max(np.abs(hf.energy_trace[:n] - ref["trace"][:n])) < 1e-9
+    assert abs(e - ref["energy"]) < 1e-9, (e, ref["energy"])
+    assert np.max(np.abs(hf.density - ref["D"])) < 1e-7
+
+
+def test_tc_zero_jastrow_equals_uos(tc, gpu):
+    """c_same = c_oppo = 0 on the device path reproduces UOSHartreeFock."""
+    basis = tc.even_tempered({0: 5, 1: 2}, 0.6, 2.2)
+    nb = len(basis)
+    g1 = _grid(tc, 14, 11, tc.RADIAL_M4)
+    g1.setBasisFunctions(basis, True)
+    jas = tc.Jastrow(1.0, 0.0, 0.0)
+    Fs = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=True)
+    V = tc.computeV(g1, g1, jas)
+    S, H = tc.one_electron_matrices(basis, 4)
+    e = tc.TCHartreeFock(2, 2, nb).compute(S, H, g1, Fs, Fs, V, jas, True, tol=1e-10, mgs_mode=1)
+    F = tc.computeF(g1, g1)
+    eu = tc.UOSHartreeFock(2, 2, nb).compute(S, H, g1, F, True, tol=1e-10)
+    assert abs(e - eu) < 1e-9
