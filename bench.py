@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--tol", type=float, default=1e-9)
     p.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU-baseline slice (0 = auto)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-tc", action="store_true", help="skip the TC-HF Ne side measurement")
+    p.add_argument("--tc-mgs", type=int, default=1, help="TC Gram-Schmidt: 0 reference, 1 metric")
     return p.parse_args()
 
 
@@ -107,6 +109,10 @@ def main():
                 "kernel": "fgemm (fused 1/r12 generation + contraction), computeF wall incl. packing",
                 "kernel_ms": round(k_mix, 4)}
 
+    tc_res = None
+    if not a.no_tc:
+        tc_res = run_tc(a, tc, configs, torch)
+
     cpu = None
     if not a.no_cpu_baseline:
         from oracle import oracle as O
@@ -152,8 +158,49 @@ def main():
         "solve_ms_per_scf": hf_mix.t_solve_ms,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "tc_ne": tc_res,
     }
     print(json.dumps(res))
+
+
+def run_tc(a, tc, configs, torch):
+    """Transcorrelated HF on Ne (30 STOs, b = 1, c_same = c_oppo / 2): r1 = r2 =
+    30,200 points, r3 = 50 x Lebedev-194 = 9,700 points; one mixed-precision
+    and one FP64 solve, precompute (F_same, F_oppo, V) + SCF timed."""
+    cfg = configs.ne_config3()
+    basis = cfg["basis"]
+    nb = len(basis)
+    g1 = tc.QMCGrid.radial_lebedev(*cfg["grid"], 1.0, 1.0)
+    g3 = tc.QMCGrid.radial_lebedev(*cfg["grid3"], 1.0, 1.0)
+    S, H = tc.one_electron_matrices(basis, cfg["Z"])
+    jas = tc.Jastrow.from_b(1.0)
+    out = {}
+    for prec in (a.precision, "fp64"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g1.setBasisFunctions(basis, True)
+        g3.setBasisFunctions(basis, True)
+        Fs = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=True, precision=prec)
+        Fo = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=False, precision=prec)
+        V = tc.computeV(g1, g3, jas, precision=prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hf = tc.TCHartreeFock(cfg["nup"], cfg["ndown"], nb)
+        try:
+            e = hf.compute(S, H, g1, Fs, Fo, V, jas, True, tol=a.tol, max_iter=200, mgs_mode=a.tc_mgs)
+            status = "ok"
+        except tc.TcscfError as err:
+            e, status = float("nan"), str(err)
+        t2 = time.perf_counter()
+        out[prec] = {"energy": e, "status": status, "precompute_ms": (t1 - t0) * 1e3, "scf_ms": (t2 - t1) * 1e3,
+                     "total_ms": (t2 - t0) * 1e3, "iterations": hf.numberOfConvergenceLoops()}
+        del Fs, Fo, V
+    n1, n3 = g1.nGrid(), g3.nGrid()
+    mix, f64 = out[a.precision], out["fp64"]
+    return {"workload": f"TC-HF Ne, {nb} STOs, b=1, r1=r2={n1}, r3={n3}, MGS {'metric' if a.tc_mgs else 'reference'}",
+            "pairs_x_components": 2 * n1 * (n1 - 1) + 3 * n1 * n3,
+            a.precision: mix, "fp64": f64, "energy_dev_vs_fp64": mix["energy"] - f64["energy"],
+            "speedup_vs_fp64": f64["total_ms"] / mix["total_ms"]}
 
 
 if __name__ == "__main__":
