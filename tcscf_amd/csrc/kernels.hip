@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -484,7 +485,7 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 /* (double buffered); B fragments and the next step's r2 coordinates are     */
 /* prefetched into registers one step ahead.                                 */
 /* ------------------------------------------------------------------------ */
-template <int MB, int NPROD, class Kern>
+template <int MB, int NPROD, int FOLD, int PF, class Kern>
 __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
                                                          const float4* __restrict__ x2, int64_t n2p,
                                                          const us8* __restrict__ P0, int64_t plane,
@@ -553,7 +554,8 @@ __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restric
     }
   };
 
-  us8 bcur[NSPLIT][NB], bnext[NSPLIT][NB];
+  // B fragments of the PF steps after the current one, plus the current
+  us8 bq[PF + 1][NSPLIT][NB];
   auto load_b = [&](us8 (&dst)[NSPLIT][NB], int64_t s) {
 #pragma unroll
     for (int t = 0; t < NSPLIT; ++t)
@@ -575,12 +577,20 @@ __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restric
   load_coords(0);
   generate(0);
   if (nsteps > 1) load_coords(32);
-  load_b(bcur, 0);
+#pragma unroll
+  for (int d = 0; d < PF; ++d)
+    if (d < nsteps) load_b(bq[d], d);
   __syncthreads();
+  f4 acc[NC][MB][NB];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < NB; ++n) acc[c][m][n] = f4{0, 0, 0, 0};
   for (int64_t s = 0; s < nsteps; ++s) {
     const int buf = (int)(s & 1);
-    if (s + 1 < nsteps) load_b(bnext, s + 1);
-    f4 acc[NC][MB][NB];
+    if (s + PF < nsteps) load_b(bq[PF], s + PF);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -589,13 +599,13 @@ __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restric
         const bf8 al = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
-          const bf8 ph = __builtin_bit_cast(bf8, bcur[0][n]);
-          const bf8 pm = __builtin_bit_cast(bf8, bcur[1][n]);
-          f4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, f4{0, 0, 0, 0}, 0, 0, 0);
+          const bf8 ph = __builtin_bit_cast(bf8, bq[0][0][n]);
+          const bf8 pm = __builtin_bit_cast(bf8, bq[0][1][n]);
+          f4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, acc[c][m][n], 0, 0, 0);
           a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pm, a, 0, 0, 0);
           a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, a, 0, 0, 0);
           if constexpr (NPROD == 5) {
-            const bf8 pl = __builtin_bit_cast(bf8, bcur[NSPLIT - 1][n]);
+            const bf8 pl = __builtin_bit_cast(bf8, bq[0][NSPLIT - 1][n]);
             a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, a, 0, 0, 0);
             a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, pm, a, 0, 0, 0);
           }
@@ -606,18 +616,24 @@ __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restric
       generate(buf ^ 1);
       if (s + 2 < nsteps) load_coords((s + 2) * 32);
     }
+    if ((s + 1) % FOLD == 0 || s + 1 == nsteps) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
+      for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int m = 0; m < MB; ++m)
+        for (int m = 0; m < MB; ++m)
 #pragma unroll
-        for (int n = 0; n < NB; ++n)
+          for (int n = 0; n < NB; ++n) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
+            for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
+            acc[c][m][n] = f4{0, 0, 0, 0};
+          }
+    }
 #pragma unroll
-    for (int t = 0; t < NSPLIT; ++t)
+    for (int d = 0; d < PF; ++d)
 #pragma unroll
-      for (int n = 0; n < NB; ++n) bcur[t][n] = bnext[t][n];
+      for (int t = 0; t < NSPLIT; ++t)
+#pragma unroll
+        for (int n = 0; n < NB; ++n) bq[d][t][n] = bq[d + 1][t][n];
     __syncthreads();
   }
 #pragma unroll
@@ -695,37 +711,59 @@ int launch_f32(const double* p1, int64_t n1, const float4* x2, int64_t n2p, cons
   return 0;
 }
 
-template <int MB, int NPROD, class Kern>
+template <int MB, int NPROD, int FOLD, int PF, class Kern>
 int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane, int64_t ncb,
                 void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
   dim3 grid((unsigned)(ncb / 16), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
-  hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0, plane, ncb, F,
-                     out_f32, npair, scale, kern);
+  hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, PF, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0, plane,
+                     ncb, F, out_f32, npair, scale, kern);
   TC_HIP(hipGetLastError());
   return 0;
 }
 
-// Rows per workgroup of the BF16 kernel (16 * MB); TCSCF_BF16_MB overrides for tuning.
-int bf16_mb(int ncomp) {
-  static int env = [] {
-    const char* e = std::getenv("TCSCF_BF16_MB");
-    return e ? std::atoi(e) : 0;
+// BF16 kernel variant: rows per workgroup 16*MB, FP64 fold every FOLD steps,
+// B prefetch PF steps ahead.  TCSCF_BF16_CFG="MB,FOLD,PF" selects one for tuning.
+struct Bf16Cfg {
+  int mb, fold, pf;
+};
+Bf16Cfg bf16_cfg(int ncomp) {
+  static Bf16Cfg env = [] {
+    Bf16Cfg c{0, 0, 0};
+    if (const char* e = std::getenv("TCSCF_BF16_CFG")) std::sscanf(e, "%d,%d,%d", &c.mb, &c.fold, &c.pf);
+    return c;
   }();
-  if (env == 1 || env == 2 || env == 4) return ncomp == 1 ? env : 1;
-  return ncomp == 1 ? 4 : 1;
+  Bf16Cfg c = env.mb ? env : Bf16Cfg{2, 1, 1};
+  if (ncomp != 1) c.mb = 1;
+  return c;
+}
+
+template <int NPROD, int MB, class Kern>
+int launch_bf16_fp(Bf16Cfg c, const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0,
+                   int64_t plane, int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern,
+                   hipStream_t st) {
+#define TC_BF16(FOLD, PF) \
+  return launch_bf16<MB, NPROD, FOLD, PF>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st)
+  if (c.pf == 2) {
+    if (c.fold == 2) TC_BF16(2, 2);
+    TC_BF16(1, 2);
+  }
+  if (c.fold == 2) TC_BF16(2, 1);
+  TC_BF16(1, 1);
+#undef TC_BF16
 }
 
 template <int NPROD, class Kern>
 int launch_bf16_mb(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane,
                    int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+  const Bf16Cfg c = bf16_cfg(Kern::ncomp);
   if constexpr (Kern::ncomp == 1) {
-    switch (bf16_mb(1)) {
-      case 1: return launch_bf16<1, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
-      case 2: return launch_bf16<2, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
-      default: return launch_bf16<4, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+    switch (c.mb) {
+      case 1: return launch_bf16_fp<NPROD, 1>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      case 4: return launch_bf16_fp<NPROD, 4>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      default: return launch_bf16_fp<NPROD, 2>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
     }
   } else {
-    return launch_bf16<1, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+    return launch_bf16_fp<NPROD, 1>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
   }
 }
 

@@ -30,6 +30,7 @@ namespace tc {
 namespace tcs {
 
 constexpr int kBlocks = 512;
+constexpr int kMaxOcc = 8;  // occupied orbitals per spin handled by the point kernel
 
 // Row layout per (point, spin): 2 vectors (L, R) of nb doubles each.
 //   0      G2 : (A, w (faa A - FaiS))
@@ -146,14 +147,20 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
       occ[s] = wave_sum(x * Ai);  // calculateOccupiedElectronDensity
       for (int c = 0; c < 3; ++c) sg[s][c] = wave_sum(x * dAi[c]);  // calculateScaledGradients
     }
-    // one pass over j: Faa, FaiS, V . D, Vai, Q (= VTSI / c_s), W
+    // one pass over j: Faa, FaiS, V . D, Vai, Q (= VTSI / c_s), W (registers)
     double faa_p[2] = {0, 0}, fai[2] = {0, 0}, vd[2][3] = {{0, 0, 0}, {0, 0, 0}}, vai1[2] = {0, 0};
     double q[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double wreg[2][kMaxOcc][3];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int b = 0; b < kMaxOcc; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
     if (act) {
       for (int j = 0; j < nb; ++j) {
         const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
         const double f_s = fs[pj], f_o = fo[pj];
         const double v0 = vv[pj], v1 = vv[np + pj], v2 = vv[2 * np + pj];
+#pragma unroll
         for (int s = 0; s < 2; ++s) {
           const double dsj = Ds[s * nb2 + j * nb + i], dbj = Ds[(1 - s) * nb2 + j * nb + i];
           faa_p[s] += f_s * dsj + f_o * dbj;
@@ -167,17 +174,27 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
           q[s][0] += xj * v0;
           q[s][1] += xj * v1;
           q[s][2] += xj * v2;
-          double* wrow = Ws + ((int64_t)(s * nb + i) * nmax) * 3;
           const double* crow = Cs + ((int64_t)s * nb + j) * nmax;
-          for (int b = 0; b < nocc[s]; ++b) {
-            const double cb = crow[b];
-            if (j == 0) wrow[b * 3] = wrow[b * 3 + 1] = wrow[b * 3 + 2] = 0;
-            wrow[b * 3] += cb * v0;
-            wrow[b * 3 + 1] += cb * v1;
-            wrow[b * 3 + 2] += cb * v2;
-          }
+#pragma unroll
+          for (int b = 0; b < kMaxOcc; ++b)
+            if (b < nocc[s]) {
+              const double cb = crow[b];
+              wreg[s][b][0] += cb * v0;
+              wreg[s][b][1] += cb * v1;
+              wreg[s][b][2] += cb * v2;
+            }
         }
       }
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int b = 0; b < kMaxOcc; ++b)
+          if (b < nocc[s]) {
+            double* wrow = Ws + (((int64_t)(s * nb + i) * nmax) + b) * 3;
+            wrow[0] = wreg[s][b][0];
+            wrow[1] = wreg[s][b][1];
+            wrow[2] = wreg[s][b][2];
+          }
     }
     double faa[2], VTv[2][3];
     double vdsum[2][3];
@@ -280,42 +297,54 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
   }
 }
 
-// Pass 2: G_target,s[j][i] += sum_n sum_rows(target) L[j] R[i].  Thread-owned
-// elements e = ((s * 3 + target) * nb + j) * nb + i; rows of one point staged in LDS.
+// Pass 2: G_target,s[j][i] += sum_n sum_rows(target) L[j] R[i].  One
+// workgroup row (blockIdx.y) per (spin, target); thread-owned elements
+// (j, i) = (e / NBP, e % NBP), e = tid + 256 k; the rows of PB points are staged
+// in LDS per step.
+template <int NBP>
 __global__ __launch_bounds__(256) void tc_rank_kernel(const double* __restrict__ rows, int64_t n1, int nb, int nrow,
                                                       double* __restrict__ part) {
-  extern __shared__ double rl[];  // [2][nrow][2][nb] of one point
+  constexpr int KG = NBP * NBP / 256;
+  constexpr int PB = 4;
+  extern __shared__ double rl[];  // [PB][nr][2][nb]
   const int tid = threadIdx.x;
-  const int64_t nb2 = (int64_t)nb * nb, ne = 6 * nb2;
+  const int st = blockIdx.y, s = st / 3, tg = st - 3 * s;
+  const int r0 = tg == 0 ? 0 : (tg == 1 ? 1 : 5), r1 = tg == 0 ? 1 : (tg == 1 ? 5 : nrow);
+  const int nr = r1 - r0;
+  const int64_t nb2 = (int64_t)nb * nb;
   const int64_t per = (n1 + gridDim.x - 1) / gridDim.x;
   const int64_t p0 = (int64_t)blockIdx.x * per;
   const int64_t p1 = p0 + per < n1 ? p0 + per : n1;
-  const int64_t rsz = (int64_t)2 * nrow * 2 * nb;
-  double acc[32];
-  for (int t = 0; t < 32; ++t) acc[t] = 0;
-  for (int64_t n = p0; n < p1; ++n) {
+  const int64_t rpt = (int64_t)2 * nrow * 2 * nb;  // doubles per point in `rows`
+  const int64_t blk = (int64_t)nr * 2 * nb;        // doubles staged per point
+  double acc[KG];
+  int jj[KG], ii[KG];
+#pragma unroll
+  for (int k = 0; k < KG; ++k) {
+    acc[k] = 0;
+    const int e = tid + 256 * k;
+    jj[k] = e / NBP;
+    ii[k] = e % NBP;
+  }
+  for (int64_t n0 = p0; n0 < p1; n0 += PB) {
+    const int pn = (int)((n0 + PB < p1) ? PB : p1 - n0);
     __syncthreads();
-    for (int64_t e = tid; e < rsz; e += 256) rl[e] = rows[n * rsz + e];
+    for (int64_t e = tid; e < pn * blk; e += 256) {
+      const int64_t pt = e / blk, off = e - pt * blk;
+      rl[e] = rows[(n0 + pt) * rpt + ((int64_t)s * nrow + r0) * 2 * nb + off];
+    }
     __syncthreads();
-    for (int t = 0; t < 32; ++t) {
-      const int64_t e = tid + 256 * (int64_t)t;
-      if (e >= ne) break;
-      const int st = (int)(e / nb2);
-      const int s = st / 3, tg = st - 3 * s;
-      const int64_t ji = e - st * nb2;
-      const int j = (int)(ji / nb), ii = (int)(ji - (int64_t)j * nb);
-      const int r0 = tg == 0 ? 0 : (tg == 1 ? 1 : 5), r1 = tg == 0 ? 1 : (tg == 1 ? 5 : nrow);
-      const double* base = rl + (int64_t)s * nrow * 2 * nb;
-      double v = 0;
-      for (int r = r0; r < r1; ++r) v += base[(2 * r) * nb + j] * base[(2 * r + 1) * nb + ii];
-      acc[t] += v;
+    for (int t = 0; t < pn * nr; ++t) {
+      const double* L = rl + (int64_t)t * 2 * nb;
+      const double* R = L + nb;
+#pragma unroll
+      for (int k = 0; k < KG; ++k)
+        if (jj[k] < nb && ii[k] < nb) acc[k] += L[jj[k]] * R[ii[k]];
     }
   }
-  for (int t = 0; t < 32; ++t) {
-    const int64_t e = tid + 256 * (int64_t)t;
-    if (e >= ne) break;
-    part[(int64_t)blockIdx.x * ne + e] = acc[t];
-  }
+#pragma unroll
+  for (int k = 0; k < KG; ++k)
+    if (jj[k] < nb && ii[k] < nb) part[((int64_t)blockIdx.x * 6 + st) * nb2 + (int64_t)jj[k] * nb + ii[k]] = acc[k];
 }
 
 // Pass 3: Rv[s][target][p] = sum_n sum_c h[n][s][target][c] V[n][c][p].
@@ -373,11 +402,11 @@ int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1, const double* dA
   TC_CHECK(A1 && dA1 && w1 && Fs && Fo && V && jp && D_host && C_host && G2_host && G3_host, "tcscf_tc_fock: null");
   TC_CHECK(nb > 0 && nb <= 64, "tcscf_tc_fock: 1 <= nb <= 64");
   TC_CHECK(nup >= 0 && ndown >= 0 && nup <= nb && ndown <= nb, "tcscf_tc_fock: bad electron counts");
+  TC_CHECK(nup <= kMaxOcc && ndown <= kMaxOcc, "tcscf_tc_fock: at most 8 occupied orbitals per spin");
   hipStream_t st = (hipStream_t)stream;
   const int nmax = std::max(1, std::max(nup, ndown));
   const int nrow = kFixedRows + 3 * nmax;
   const int64_t npair = (int64_t)nb * (nb + 1) / 2, nb2 = (int64_t)nb * nb;
-  TC_CHECK(6 * nb2 <= 256 * 32, "tcscf_tc_fock: nb too large for the rank kernel");
   // device scratch: D, C, rows, hvec, partials, results
   const size_t nD = 2 * nb2, nC = 2 * (size_t)nb * nmax, nRows = (size_t)n1 * 2 * nrow * 2 * nb, nH = (size_t)n1 * 12;
   const size_t nPartG = (size_t)kBlocks * 6 * nb2, nPartV = (size_t)kBlocks * 4 * npair, nOut = 6 * nb2 + 4 * npair;
@@ -400,9 +429,12 @@ int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1, const double* dA
   if (sh1 > 64 * 1024) TC_HIP(hipFuncSetAttribute((const void*)tc_point_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh1));
   hipLaunchKernelGGL(tc_point_kernel, dim3(kBlocks), dim3(256), sh1, st, a);
   TC_HIP(hipGetLastError());
-  const size_t sh2 = sizeof(double) * 2 * nrow * 2 * nb;
+  const size_t sh2 = sizeof(double) * 4 * (nrow - 5) * 2 * nb;
   TC_CHECK(sh2 <= 64 * 1024, "tcscf_tc_fock: too many occupied orbitals for the rank kernel");
-  hipLaunchKernelGGL(tc_rank_kernel, dim3(kBlocks), dim3(256), sh2, st, dRows, n1, nb, nrow, dPG);
+  if (nb <= 32)
+    hipLaunchKernelGGL(tc_rank_kernel<32>, dim3(kBlocks, 6), dim3(256), sh2, st, dRows, n1, nb, nrow, dPG);
+  else
+    hipLaunchKernelGGL(tc_rank_kernel<64>, dim3(kBlocks, 6), dim3(256), sh2, st, dRows, n1, nb, nrow, dPG);
   TC_HIP(hipGetLastError());
   hipLaunchKernelGGL(tc_vlin_kernel, dim3(kBlocks), dim3(256), 0, st, V, dH, n1, npair, dPV);
   TC_HIP(hipGetLastError());
