@@ -135,6 +135,14 @@ int tcscf_compute_F(const double* pts1_dev, int64_t n1, const double* pts2_dev, 
                     int out_dtype, void* stream);
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
 
+/* integrateAllR1R2<false> (src/integration/integrateAll.hpp:221-317) from a
+ * computed F: ERI[p(b1,b3)][p(b2,b4)] = sum_n1 w1 A1[n1,b1] A1[n1,b3] F[n1,p(b2,b4)]
+ * (chemist (b1 b3 | b2 b4); the reference's answer(b1,b2,b3,b4)).  F_dev packed
+ * n1 x npair FP64 (the 2 pi of a 2-D r2 grid is already in F); eri_dev
+ * npair x npair.  FP64 MFMA GEMM split over r1. */
+int tcscf_eri(const double* A1_dev, const double* w1_dev, int64_t n1, int nb, const void* F_dev, double* eri_dev,
+              void* stream);
+
 /* ---------------------------------------------------------------- SCF drivers */
 /* RCSHartreeFock::compute (src/HartreeFock.cpp:785-886).  S, H host nb x nb
  * row-major; A1_dev n1 x nb; w1_dev n1; F_dev packed n1 x npair (f_dtype).

@@ -261,3 +261,38 @@ def tc_scf(nup, ndown, S, H, A1, dA1, w1, Fs, Fo, Vs, Vo, tol=1e-5, max_iter=200
                      ctypes.byref(out), _p(trace), _p(D), _p(ps))
     return dict(energy=out.energy, iterations=out.iterations, status=out.status,
                 trace=trace[:max(out.iterations, 0)], D=D, pseudo=ps)
+
+
+def tc_fock(nup, ndown, A1, dA1, w1, Fs, Fo, Vs, Vo, D, C):
+    """One TC Fock build at density D (2, nb, nb) and occupied orbitals C
+    (2, nb, cmax): returns (G2, G3), each (2, nb, nb), before the mask."""
+    nb = A1.shape[1]
+    c = np.ascontiguousarray
+    C = c(C, dtype=np.float64)
+    G2 = np.zeros((2, nb, nb))
+    G3 = np.zeros((2, nb, nb))
+    rc = lib().orc_tc_fock(int(nup), int(ndown), nb, _p(c(A1)), _p(c(dA1)), _p(w1), ctypes.c_int64(A1.shape[0]),
+                           _p(c(Fs)), _p(c(Fo)), _p(c(Vs)), _p(c(Vo)), _p(c(D, dtype=np.float64)), _p(C),
+                           int(C.shape[2]), _p(G2), _p(G3))
+    if rc:
+        raise ValueError("tc_fock: C has fewer columns than occupied orbitals")
+    return G2, G3
+
+
+def compute_Q(A1, w1, Vs, Vo, labelled=False):
+    """computeQ: the three (nb,)*6 tensors QSameSame, QSameOppo, QOppoOppo;
+    labelled=True takes V_same / V_oppo per electron pair from the spins."""
+    nb = A1.shape[1]
+    Q = [np.zeros((nb,) * 6) for _ in range(3)]
+    c = np.ascontiguousarray
+    lib().orc_compute_Q(_p(c(A1)), _p(w1), ctypes.c_int64(A1.shape[0]), _p(c(Vs)), _p(c(Vo)), nb,
+                        int(bool(labelled)), _p(Q[0]), _p(Q[1]), _p(Q[2]))
+    return Q
+
+
+def assemble_G3(QSS, QSO, QOO, D):
+    """assembleG: G3 (2, nb, nb) from the explicit Q tensors."""
+    nb = QSS.shape[0]
+    G3 = np.zeros((2, nb, nb))
+    lib().orc_assemble_G3(nb, _p(QSS), _p(QSO), _p(QOO), _p(np.ascontiguousarray(D, dtype=np.float64)), _p(G3))
+    return G3

@@ -465,7 +465,6 @@ void orc_three_electron(const double* A1, const double* w1, int64_t n1, const do
                     w1[k] * A1[k * nb + p] * dot * A1[k * nb + s];
               }
 }
-
 }  // extern "C"
 
 /* ======================================================================== */
@@ -1017,6 +1016,198 @@ int orc_uos_scf(int nup, int ndown, int nb, const double* S, const double* H, co
   return out->status;
 }
 
+/* One TC Fock build, src/HartreeFock.cpp:1118-1224, real basis: the per-point
+ * intermediates of :1118-1148 (the reference's internal::calculate* sweeps,
+ * :102-609) and the (j, i) assembly of :1150-1224.  D: 2 x nb x nb (density(s,
+ * i, j) at [s][i][j]); Cocc[s]: nb x ne[s] row-major.  G2, G3: 2 x nb x nb,
+ * before the mask. */
+static void tc_fock_build(const int ne[2], int nb, const double* A1, const double* dA1, const double* w1, int64_t n1,
+                          const double* Fs, const double* Fo, const double* Vs, const double* Vo, const double* D,
+                          const double* const Cocc[2], double* G2, double* G3) {
+  int64_t const nb2 = (int64_t)nb * nb;
+  int const nmax = std::max(ne[0], ne[1]);
+  std::vector<double> FaaSPO(2 * n1), FaiS(2 * n1 * nb), VaaSPO(2 * n1 * 3), Vai(2 * n1 * nb), Vja(2 * n1 * nb * 3),
+      sg(2 * n1 * 3), sNVabS(2 * n1), sNVabO(2 * n1), VTSI(2 * n1 * nb * 3), SVbiVT(2 * n1 * nb), occDen(2 * n1),
+      SCVT(2 * n1 * 3);
+  std::vector<double> VjbS(2 * n1 * nb * std::max(nmax, 1) * 3), VjbO(2 * n1 * nb * std::max(nmax, 1) * 3);
+  std::vector<double> VC(2 * nb2);
+  auto V3 = [&](const double* V, int64_t k, int j, int i) { return V + ((k * nb + j) * nb + i) * 3; };
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < n1; ++k) {
+    const double* A = A1 + k * nb;
+    const double* dA = dA1 + k * nb * 3;
+    for (int s = 0; s < 2; ++s) {
+      const double* Ds = D + s * nb2;
+      const double* Db = D + (1 - s) * nb2;
+      /* calculateIaa x2 (F and V), :1120-1126 */
+      double faa = 0, vaa[3] = {0, 0, 0};
+      for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < nb; ++i) {
+          faa += Fs[(k * nb + j) * nb + i] * Ds[i * nb + j];
+          for (int c = 0; c < 3; ++c) vaa[c] += V3(Vs, k, j, i)[c] * Ds[i * nb + j];
+        }
+      for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < nb; ++i) {
+          faa += Fo[(k * nb + j) * nb + i] * Db[i * nb + j];
+          for (int c = 0; c < 3; ++c) vaa[c] += V3(Vo, k, j, i)[c] * Db[i * nb + j];
+        }
+      FaaSPO[s * n1 + k] = faa;
+      for (int c = 0; c < 3; ++c) VaaSPO[(s * n1 + k) * 3 + c] = vaa[c];
+      /* calculateFai, :1123 */
+      double* fai = FaiS.data() + (s * n1 + k) * nb;
+      for (int i = 0; i < nb; ++i) fai[i] = 0;
+      for (int j = 0; j < nb; ++j) {
+        double ks = 0;
+        for (int q = 0; q < nb; ++q) ks += Ds[q * nb + j] * A[q];
+        for (int i = 0; i < nb; ++i) fai[i] += ks * Fs[(k * nb + j) * nb + i];
+      }
+      /* calculateVai (:217-252) */
+      double* vai = Vai.data() + (s * n1 + k) * nb;
+      for (int i = 0; i < nb; ++i) vai[i] = 0;
+      for (int j = 0; j < nb; ++j) {
+        double ks[3] = {0, 0, 0};
+        for (int q = 0; q < nb; ++q)
+          for (int c = 0; c < 3; ++c) ks[c] += Ds[q * nb + j] * dA[q * 3 + c];
+        for (int i = 0; i < nb; ++i) {
+          const double* v = V3(Vs, k, j, i);
+          vai[i] += v[0] * ks[0] + v[1] * ks[1] + v[2] * ks[2];
+        }
+      }
+      /* calculateVja (:256-291) */
+      double* vja = Vja.data() + (s * n1 + k) * nb * 3;
+      for (int q = 0; q < nb * 3; ++q) vja[q] = 0;
+      for (int i = 0; i < nb; ++i) {
+        double ks = 0;
+        for (int q = 0; q < nb; ++q) ks += Ds[i * nb + q] * A[q];
+        for (int j = 0; j < nb; ++j)
+          for (int c = 0; c < 3; ++c) vja[j * 3 + c] += ks * V3(Vs, k, j, i)[c];
+      }
+      /* calculateScaledGradients (:102-140) */
+      double g[3] = {0, 0, 0};
+      for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < nb; ++i)
+          for (int c = 0; c < 3; ++c) g[c] += Ds[i * nb + j] * A[j] * dA[i * 3 + c];
+      for (int c = 0; c < 3; ++c) sg[(s * n1 + k) * 3 + c] = g[c];
+      /* calculateVjb (:295-337), calculateVab (:341-393) with the spin's own orbitals */
+      const double* C = Cocc[s];
+      int const no = ne[s];
+      double* vjbs = VjbS.data() + (s * n1 + k) * nb * nmax * 3;
+      double* vjbo = VjbO.data() + (s * n1 + k) * nb * nmax * 3;
+      for (int j = 0; j < nb; ++j)
+        for (int b = 0; b < no; ++b) {
+          double rs[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
+          for (int i = 0; i < nb; ++i)
+            for (int c = 0; c < 3; ++c) {
+              rs[c] += C[i * no + b] * V3(Vs, k, j, i)[c];
+              ro[c] += C[i * no + b] * V3(Vo, k, j, i)[c];
+            }
+          for (int c = 0; c < 3; ++c) {
+            vjbs[(j * nmax + b) * 3 + c] = rs[c];
+            vjbo[(j * nmax + b) * 3 + c] = ro[c];
+          }
+        }
+      double nS = 0, nO = 0;
+      for (int a = 0; a < no; ++a)
+        for (int b = 0; b < no; ++b) {
+          double rs[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
+          for (int j = 0; j < nb; ++j)
+            for (int c = 0; c < 3; ++c) {
+              rs[c] += C[j * no + a] * vjbs[(j * nmax + b) * 3 + c];
+              ro[c] += C[j * no + a] * vjbo[(j * nmax + b) * 3 + c];
+            }
+          nS += rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2];
+          nO += ro[0] * ro[0] + ro[1] * ro[1] + ro[2] * ro[2];
+        }
+      sNVabS[s * n1 + k] = nS;
+      sNVabO[s * n1 + k] = nO;
+      /* calculateVTildeSubI (:438-480) */
+      double* vtsi = VTSI.data() + (s * n1 + k) * nb * 3;
+      for (int q = 0; q < nb * 3; ++q) vtsi[q] = 0;
+      for (int kk = 0; kk < nb; ++kk) {
+        double es = 0;
+        for (int l = 0; l < nb; ++l) es += Ds[l * nb + kk] * A[l];
+        for (int i = 0; i < nb; ++i)
+          for (int c = 0; c < 3; ++c) vtsi[i * 3 + c] += es * V3(Vs, k, kk, i)[c];
+      }
+      /* calculateOccupiedElectronDensity (:530-567) */
+      double od = 0;
+      for (int kk = 0; kk < nb; ++kk)
+        for (int l = 0; l < nb; ++l) od += Ds[l * nb + kk] * A[kk] * A[l];
+      occDen[s * n1 + k] = od;
+    }
+    for (int s = 0; s < 2; ++s) {
+      const double* Ds = D + s * nb2;
+      const double* vtsi = VTSI.data() + (s * n1 + k) * nb * 3;
+      /* calculateSumOverb_Vbi_dot_VTildeb (:484-526) */
+      double* sv = SVbiVT.data() + (s * n1 + k) * nb;
+      for (int i = 0; i < nb; ++i) sv[i] = 0;
+      for (int kk = 0; kk < nb; ++kk) {
+        double es[3] = {0, 0, 0};
+        for (int l = 0; l < nb; ++l)
+          for (int c = 0; c < 3; ++c) es[c] += Ds[l * nb + kk] * vtsi[l * 3 + c];
+        for (int i = 0; i < nb; ++i) {
+          const double* v = V3(Vs, k, kk, i);
+          sv[i] += v[0] * es[0] + v[1] * es[1] + v[2] * es[2];
+        }
+      }
+      /* calculateSumOverb_conjVTildeb_occb (:571-609) */
+      double cs[3] = {0, 0, 0};
+      for (int kk = 0; kk < nb; ++kk)
+        for (int l = 0; l < nb; ++l)
+          for (int c = 0; c < 3; ++c) cs[c] += Ds[l * nb + kk] * A1[k * nb + l] * vtsi[kk * 3 + c];
+      for (int c = 0; c < 3; ++c) SCVT[(s * n1 + k) * 3 + c] = cs[c];
+    }
+  }
+  /* Fock assembly, :1150-1224 */
+  for (int s = 0; s < 2; ++s) {
+    int const sb = 1 - s;
+#pragma omp parallel for schedule(static)
+    for (int64_t ji = 0; ji < nb2; ++ji) {
+      int const j = (int)(ji / nb), i = (int)(ji % nb);
+      double scalar = 0, vector = 0, scalar3 = 0;
+      for (int64_t k = 0; k < n1; ++k) {
+        const double* A = A1 + k * nb;
+        const double* dA = dA1 + k * nb * 3;
+        auto dot3 = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        const double* vaa = &VaaSPO[(s * n1 + k) * 3];
+        const double* vt = vaa;       /* VTilde == VaaSamePlusOppo */
+        const double* vtb = &VaaSPO[(sb * n1 + k) * 3];
+        const double* vsji = V3(Vs, k, j, i);
+        const double* voji = V3(Vo, k, j, i);
+        double const sa = A[j] * (FaaSPO[s * n1 + k] * A[i] - FaiS[(s * n1 + k) * nb + i]);
+        scalar += w1[k] * sa;
+        double va = A[j] * (dot3(vaa, dA + i * 3) - Vai[(s * n1 + k) * nb + i]);
+        va += dot3(vsji, &sg[(s * n1 + k) * 3]);
+        va += dot3(voji, &sg[(sb * n1 + k) * 3]);
+        va -= dot3(&Vja[((s * n1 + k) * nb + j) * 3], dA + i * 3);
+        vector += w1[k] * va;
+        const double* vtsi_i = &VTSI[((s * n1 + k) * nb + i) * 3];
+        const double* vtsi_j = &VTSI[((s * n1 + k) * nb + j) * 3];
+        double s3 = A[j] * (dot3(vt, vt) - sNVabS[s * n1 + k] - sNVabO[sb * n1 + k]) * A[i];
+        s3 += 2 * A[j] * (SVbiVT[(s * n1 + k) * nb + i] - dot3(vt, vtsi_i));
+        s3 += 2 * (SVbiVT[(s * n1 + k) * nb + j] - dot3(vtsi_j, vt)) * A[i];
+        double sbS = 0, sbO = 0;
+        const double* vjbs = &VjbS[(s * n1 + k) * nb * nmax * 3];
+        const double* vjbo = &VjbO[(s * n1 + k) * nb * nmax * 3];
+        for (int b = 0; b < ne[s]; ++b) {
+          sbS += dot3(vjbs + (j * nmax + b) * 3, vjbs + (i * nmax + b) * 3);
+          sbO += dot3(vjbo + (j * nmax + b) * 3, vjbo + (i * nmax + b) * 3);
+        }
+        s3 += 2 * occDen[s * n1 + k] * (dot3(vsji, vt) - sbS);
+        s3 += 2 * (dot3(vtsi_i, vtsi_j) - dot3(vsji, &SCVT[(s * n1 + k) * 3]));
+        s3 += 2 * occDen[sb * n1 + k] * (dot3(voji, vtb) - sbO);
+        s3 -= 2 * dot3(voji, &SCVT[(sb * n1 + k) * 3]);
+        scalar3 += w1[k] * s3;
+      }
+      G2[s * nb2 + ji] = scalar / 2;
+      VC[s * nb2 + ji] = vector / 2;
+      G3[s * nb2 + ji] = scalar3 / 2;
+    }
+    for (int j = 0; j < nb; ++j)
+      for (int i = 0; i < nb; ++i) G2[s * nb2 + j * nb + i] += VC[s * nb2 + j * nb + i] - VC[s * nb2 + i * nb + j];
+  }
+}
+
 /* TCHartreeFock::compute, src/HartreeFock.cpp:1007-1362, real basis.
  * Fs, Fo: N1 x nb x nb; Vs, Vo: N1 x nb x nb x 3; A1: N1 x nb; dA1: N1 x nb x 3.
  * D_out: 2 x nb x nb, pseudo_out: nup + ndown occupied pseudo-energies. */
@@ -1025,194 +1216,15 @@ static int tc_scf_impl(int nup, int ndown, int nb, const double* S, const double
                const orc_scf_opts* o, orc_scf_out* out, double* energy_trace, double* D_out, double* pseudo_out) {
   int64_t const nb2 = (int64_t)nb * nb;
   int const ne[2] = {nup, ndown};
-  int const nmax = std::max(nup, ndown);
   std::vector<double> D(2 * nb2, 0.0);
   std::vector<double> Cocc[2] = {std::vector<double>((size_t)nb * nup, 0.0), std::vector<double>((size_t)nb * ndown, 0.0)};
   std::vector<double> pseudo(nup + ndown, 0.0);
-  /* per-point intermediates (Appendix of the reference loop, :1118-1148) */
-  std::vector<double> FaaSPO(2 * n1), FaiS(2 * n1 * nb), VaaSPO(2 * n1 * 3), Vai(2 * n1 * nb), Vja(2 * n1 * nb * 3),
-      sg(2 * n1 * 3), sNVabS(2 * n1), sNVabO(2 * n1), VTSI(2 * n1 * nb * 3), SVbiVT(2 * n1 * nb), occDen(2 * n1),
-      SCVT(2 * n1 * 3);
-  std::vector<double> VjbS(2 * n1 * nb * nmax * 3), VjbO(2 * n1 * nb * nmax * 3);
-  std::vector<double> G2(2 * nb2), G3(2 * nb2), VC(2 * nb2), Fock(2 * nb2);
+  std::vector<double> G2(2 * nb2), G3(2 * nb2), Fock(2 * nb2);
   double energy = std::numeric_limits<double>::max();
   out->status = -1;
-  auto V3 = [&](const double* V, int64_t k, int j, int i) { return V + ((k * nb + j) * nb + i) * 3; };
   for (int it = 0; it < o->max_iter; ++it) {
-#pragma omp parallel for schedule(static)
-    for (int64_t k = 0; k < n1; ++k) {
-      const double* A = A1 + k * nb;
-      const double* dA = dA1 + k * nb * 3;
-      for (int s = 0; s < 2; ++s) {
-        const double* Ds = D.data() + s * nb2;
-        const double* Db = D.data() + (1 - s) * nb2;
-        /* calculateIaa x2 (F and V), :1120-1126 */
-        double faa = 0, vaa[3] = {0, 0, 0};
-        for (int j = 0; j < nb; ++j)
-          for (int i = 0; i < nb; ++i) {
-            faa += Fs[(k * nb + j) * nb + i] * Ds[i * nb + j];
-            for (int c = 0; c < 3; ++c) vaa[c] += V3(Vs, k, j, i)[c] * Ds[i * nb + j];
-          }
-        for (int j = 0; j < nb; ++j)
-          for (int i = 0; i < nb; ++i) {
-            faa += Fo[(k * nb + j) * nb + i] * Db[i * nb + j];
-            for (int c = 0; c < 3; ++c) vaa[c] += V3(Vo, k, j, i)[c] * Db[i * nb + j];
-          }
-        FaaSPO[s * n1 + k] = faa;
-        for (int c = 0; c < 3; ++c) VaaSPO[(s * n1 + k) * 3 + c] = vaa[c];
-        /* calculateFai, :1123 */
-        double* fai = FaiS.data() + (s * n1 + k) * nb;
-        for (int i = 0; i < nb; ++i) fai[i] = 0;
-        for (int j = 0; j < nb; ++j) {
-          double ks = 0;
-          for (int q = 0; q < nb; ++q) ks += Ds[q * nb + j] * A[q];
-          for (int i = 0; i < nb; ++i) fai[i] += ks * Fs[(k * nb + j) * nb + i];
-        }
-        /* calculateVai (:217-252) */
-        double* vai = Vai.data() + (s * n1 + k) * nb;
-        for (int i = 0; i < nb; ++i) vai[i] = 0;
-        for (int j = 0; j < nb; ++j) {
-          double ks[3] = {0, 0, 0};
-          for (int q = 0; q < nb; ++q)
-            for (int c = 0; c < 3; ++c) ks[c] += Ds[q * nb + j] * dA[q * 3 + c];
-          for (int i = 0; i < nb; ++i) {
-            const double* v = V3(Vs, k, j, i);
-            vai[i] += v[0] * ks[0] + v[1] * ks[1] + v[2] * ks[2];
-          }
-        }
-        /* calculateVja (:256-291) */
-        double* vja = Vja.data() + (s * n1 + k) * nb * 3;
-        for (int q = 0; q < nb * 3; ++q) vja[q] = 0;
-        for (int i = 0; i < nb; ++i) {
-          double ks = 0;
-          for (int q = 0; q < nb; ++q) ks += Ds[i * nb + q] * A[q];
-          for (int j = 0; j < nb; ++j)
-            for (int c = 0; c < 3; ++c) vja[j * 3 + c] += ks * V3(Vs, k, j, i)[c];
-        }
-        /* calculateScaledGradients (:102-140) */
-        double g[3] = {0, 0, 0};
-        for (int j = 0; j < nb; ++j)
-          for (int i = 0; i < nb; ++i)
-            for (int c = 0; c < 3; ++c) g[c] += Ds[i * nb + j] * A[j] * dA[i * 3 + c];
-        for (int c = 0; c < 3; ++c) sg[(s * n1 + k) * 3 + c] = g[c];
-        /* calculateVjb (:295-337), calculateVab (:341-393) with the spin's own orbitals */
-        const std::vector<double>& C = Cocc[s];
-        int const no = ne[s];
-        double* vjbs = VjbS.data() + (s * n1 + k) * nb * nmax * 3;
-        double* vjbo = VjbO.data() + (s * n1 + k) * nb * nmax * 3;
-        for (int j = 0; j < nb; ++j)
-          for (int b = 0; b < no; ++b) {
-            double rs[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
-            for (int i = 0; i < nb; ++i)
-              for (int c = 0; c < 3; ++c) {
-                rs[c] += C[i * no + b] * V3(Vs, k, j, i)[c];
-                ro[c] += C[i * no + b] * V3(Vo, k, j, i)[c];
-              }
-            for (int c = 0; c < 3; ++c) {
-              vjbs[(j * nmax + b) * 3 + c] = rs[c];
-              vjbo[(j * nmax + b) * 3 + c] = ro[c];
-            }
-          }
-        double nS = 0, nO = 0;
-        for (int a = 0; a < no; ++a)
-          for (int b = 0; b < no; ++b) {
-            double rs[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
-            for (int j = 0; j < nb; ++j)
-              for (int c = 0; c < 3; ++c) {
-                rs[c] += C[j * no + a] * vjbs[(j * nmax + b) * 3 + c];
-                ro[c] += C[j * no + a] * vjbo[(j * nmax + b) * 3 + c];
-              }
-            nS += rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2];
-            nO += ro[0] * ro[0] + ro[1] * ro[1] + ro[2] * ro[2];
-          }
-        sNVabS[s * n1 + k] = nS;
-        sNVabO[s * n1 + k] = nO;
-        /* calculateVTildeSubI (:438-480) */
-        double* vtsi = VTSI.data() + (s * n1 + k) * nb * 3;
-        for (int q = 0; q < nb * 3; ++q) vtsi[q] = 0;
-        for (int kk = 0; kk < nb; ++kk) {
-          double es = 0;
-          for (int l = 0; l < nb; ++l) es += Ds[l * nb + kk] * A[l];
-          for (int i = 0; i < nb; ++i)
-            for (int c = 0; c < 3; ++c) vtsi[i * 3 + c] += es * V3(Vs, k, kk, i)[c];
-        }
-        /* calculateOccupiedElectronDensity (:530-567) */
-        double od = 0;
-        for (int kk = 0; kk < nb; ++kk)
-          for (int l = 0; l < nb; ++l) od += Ds[l * nb + kk] * A[kk] * A[l];
-        occDen[s * n1 + k] = od;
-      }
-      for (int s = 0; s < 2; ++s) {
-        const double* Ds = D.data() + s * nb2;
-        const double* vtsi = VTSI.data() + (s * n1 + k) * nb * 3;
-        /* calculateSumOverb_Vbi_dot_VTildeb (:484-526) */
-        double* sv = SVbiVT.data() + (s * n1 + k) * nb;
-        for (int i = 0; i < nb; ++i) sv[i] = 0;
-        for (int kk = 0; kk < nb; ++kk) {
-          double es[3] = {0, 0, 0};
-          for (int l = 0; l < nb; ++l)
-            for (int c = 0; c < 3; ++c) es[c] += Ds[l * nb + kk] * vtsi[l * 3 + c];
-          for (int i = 0; i < nb; ++i) {
-            const double* v = V3(Vs, k, kk, i);
-            sv[i] += v[0] * es[0] + v[1] * es[1] + v[2] * es[2];
-          }
-        }
-        /* calculateSumOverb_conjVTildeb_occb (:571-609) */
-        double cs[3] = {0, 0, 0};
-        for (int kk = 0; kk < nb; ++kk)
-          for (int l = 0; l < nb; ++l)
-            for (int c = 0; c < 3; ++c) cs[c] += Ds[l * nb + kk] * A1[k * nb + l] * vtsi[kk * 3 + c];
-        for (int c = 0; c < 3; ++c) SCVT[(s * n1 + k) * 3 + c] = cs[c];
-      }
-    }
-    /* Fock assembly, :1150-1224 */
-    for (int s = 0; s < 2; ++s) {
-      int const sb = 1 - s;
-#pragma omp parallel for schedule(static)
-      for (int64_t ji = 0; ji < nb2; ++ji) {
-        int const j = (int)(ji / nb), i = (int)(ji % nb);
-        double scalar = 0, vector = 0, scalar3 = 0;
-        for (int64_t k = 0; k < n1; ++k) {
-          const double* A = A1 + k * nb;
-          const double* dA = dA1 + k * nb * 3;
-          auto dot3 = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-          const double* vaa = &VaaSPO[(s * n1 + k) * 3];
-          const double* vt = vaa;       /* VTilde == VaaSamePlusOppo */
-          const double* vtb = &VaaSPO[(sb * n1 + k) * 3];
-          const double* vsji = V3(Vs, k, j, i);
-          const double* voji = V3(Vo, k, j, i);
-          double const sa = A[j] * (FaaSPO[s * n1 + k] * A[i] - FaiS[(s * n1 + k) * nb + i]);
-          scalar += w1[k] * sa;
-          double va = A[j] * (dot3(vaa, dA + i * 3) - Vai[(s * n1 + k) * nb + i]);
-          va += dot3(vsji, &sg[(s * n1 + k) * 3]);
-          va += dot3(voji, &sg[(sb * n1 + k) * 3]);
-          va -= dot3(&Vja[((s * n1 + k) * nb + j) * 3], dA + i * 3);
-          vector += w1[k] * va;
-          const double* vtsi_i = &VTSI[((s * n1 + k) * nb + i) * 3];
-          const double* vtsi_j = &VTSI[((s * n1 + k) * nb + j) * 3];
-          double s3 = A[j] * (dot3(vt, vt) - sNVabS[s * n1 + k] - sNVabO[sb * n1 + k]) * A[i];
-          s3 += 2 * A[j] * (SVbiVT[(s * n1 + k) * nb + i] - dot3(vt, vtsi_i));
-          s3 += 2 * (SVbiVT[(s * n1 + k) * nb + j] - dot3(vtsi_j, vt)) * A[i];
-          double sbS = 0, sbO = 0;
-          const double* vjbs = &VjbS[(s * n1 + k) * nb * nmax * 3];
-          const double* vjbo = &VjbO[(s * n1 + k) * nb * nmax * 3];
-          for (int b = 0; b < ne[s]; ++b) {
-            sbS += dot3(vjbs + (j * nmax + b) * 3, vjbs + (i * nmax + b) * 3);
-            sbO += dot3(vjbo + (j * nmax + b) * 3, vjbo + (i * nmax + b) * 3);
-          }
-          s3 += 2 * occDen[s * n1 + k] * (dot3(vsji, vt) - sbS);
-          s3 += 2 * (dot3(vtsi_i, vtsi_j) - dot3(vsji, &SCVT[(s * n1 + k) * 3]));
-          s3 += 2 * occDen[sb * n1 + k] * (dot3(voji, vtb) - sbO);
-          s3 -= 2 * dot3(voji, &SCVT[(sb * n1 + k) * 3]);
-          scalar3 += w1[k] * s3;
-        }
-        G2[s * nb2 + ji] = scalar / 2;
-        VC[s * nb2 + ji] = vector / 2;
-        G3[s * nb2 + ji] = scalar3 / 2;
-      }
-      for (int j = 0; j < nb; ++j)
-        for (int i = 0; i < nb; ++i) G2[s * nb2 + j * nb + i] += VC[s * nb2 + j * nb + i] - VC[s * nb2 + i * nb + j];
-    }
+    const double* Cp[2] = {Cocc[0].data(), Cocc[1].data()};
+    tc_fock_build(ne, nb, A1, dA1, w1, n1, Fs, Fo, Vs, Vo, D.data(), Cp, G2.data(), G3.data());
     if (o->mask)
       for (int s = 0; s < 2; ++s) {
         mask_by(nb, G2.data() + s * nb2, H);
@@ -1278,6 +1290,97 @@ int orc_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, con
     out->status = -2;
     return -2;
   }
+}
+
+/* One Fock build of the TC loop (tc_fock_build) for a given density and
+ * occupied orbitals: C: 2 x nb x cmax row-major, column b of spin s used for
+ * b < ne[s]. */
+int orc_tc_fock(int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1, int64_t n1,
+                const double* Fs, const double* Fo, const double* Vs, const double* Vo, const double* D,
+                const double* C, int cmax, double* G2, double* G3) {
+  int const ne[2] = {nup, ndown};
+  if (cmax < std::max(nup, ndown)) return -1;
+  std::vector<double> Cs[2];
+  for (int s = 0; s < 2; ++s) {
+    Cs[s].assign((size_t)nb * std::max(ne[s], 1), 0.0);
+    for (int i = 0; i < nb; ++i)
+      for (int b = 0; b < ne[s]; ++b) Cs[s][i * ne[s] + b] = C[((size_t)s * nb + i) * cmax + b];
+  }
+  const double* Cp[2] = {Cs[0].data(), Cs[1].data()};
+  tc_fock_build(ne, nb, A1, dA1, w1, n1, Fs, Fo, Vs, Vo, D, Cp, G2, G3);
+  return 0;
+}
+
+/* computeQ, src/HartreeFock.cpp:613-682, real basis: the explicit six-index
+ * three-body tensors, Q(j, p, r, i, q, s) = sum_r1 w1 [A_j A_i X12(p,q).X13(r,s)
+ * + A_p A_q X12(j,i).X23(r,s) + A_r A_s X13(j,i).X23(p,q)], electron 1 = (j, i),
+ * 2 = (p, q), 3 = (r, s), X = V_same or V_oppo by the pair's spins.
+ * labelled = 0: the reference's choice (QSS: all same; QSO: X12 = same,
+ * X13 = X23 = oppo except the third term, which reads V_same(j,i).V_oppo(p,q);
+ * QOO: all oppo).  labelled = 1: X per pair from the spins (s,s,s), (s,s,t),
+ * (s,t,t): QSO X12 = same, X13 = X23 = oppo; QOO X12 = X13 = oppo, X23 = same.
+ * Vs, Vo: N1 x nb x nb x 3.  Each Q: nb^6. */
+void orc_compute_Q(const double* A1, const double* w1, int64_t n1, const double* Vs, const double* Vo, int nb,
+                   int labelled, double* QSS, double* QSO, double* QOO) {
+  int64_t const n6 = (int64_t)nb * nb * nb * nb * nb * nb;
+  std::fill(QSS, QSS + n6, 0.0);
+  std::fill(QSO, QSO + n6, 0.0);
+  std::fill(QOO, QOO + n6, 0.0);
+  auto V3 = [&](const double* V, int64_t k, int j, int i) { return V + ((k * nb + j) * nb + i) * 3; };
+  auto dot3 = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  /* pair tensors (X12, X13, X23) per Q */
+  const double* const X[3][3] = {{Vs, Vs, Vs}, {Vs, Vo, Vo}, {Vo, Vo, labelled ? Vs : Vo}};
+  double* const Q[3] = {QSS, QSO, QOO};
+  for (int64_t k = 0; k < n1; ++k) {
+    const double* A = A1 + k * nb;
+    double const wk = w1[k];
+    int64_t idx = 0;
+    for (int j = 0; j < nb; ++j)
+      for (int p = 0; p < nb; ++p)
+        for (int r = 0; r < nb; ++r)
+          for (int i = 0; i < nb; ++i)
+            for (int q = 0; q < nb; ++q)
+              for (int t = 0; t < nb; ++t, ++idx) {
+                /* the reference's loop letters (j, p, r, i, q, s), with s -> t here */
+                double const a123 = A[j] * A[i], a213 = A[p] * A[q], a312 = A[r] * A[t];
+                for (int m = 0; m < 3; ++m) {
+                  const double *x12 = X[m][0], *x13 = X[m][1], *x23 = X[m][2];
+                  /* the reference's so312 term reads V_same(j,i) where the pair (1,3) is opposite */
+                  const double* x13_third = (!labelled && m == 1) ? Vs : x13;
+                  Q[m][idx] += wk * (a123 * dot3(V3(x12, k, p, q), V3(x13, k, r, t)) +
+                                     a213 * dot3(V3(x12, k, j, i), V3(x23, k, r, t)) +
+                                     a312 * dot3(V3(x13_third, k, j, i), V3(x23, k, p, q)));
+                }
+              }
+  }
+}
+
+/* assembleG, src/HartreeFock.cpp:686-750: G3 from the explicit Q tensors and the
+ * density (D[s][i][j] = density(s, i, j)).  G3: 2 x nb x nb. */
+void orc_assemble_G3(int nb, const double* QSS, const double* QSO, const double* QOO, const double* D, double* G3) {
+  int64_t const nb2 = (int64_t)nb * nb;
+  auto Q = [nb](const double* T, int a, int b, int c, int d, int e, int f) {
+    return T[(((((int64_t)a * nb + b) * nb + c) * nb + d) * nb + e) * nb + f];
+  };
+  for (int j = 0; j < nb; ++j)
+    for (int i = 0; i < nb; ++i) {
+      double value[2] = {0, 0};
+      for (int p = 0; p < nb; ++p)
+        for (int q = 0; q < nb; ++q)
+          for (int r = 0; r < nb; ++r)
+            for (int t = 0; t < nb; ++t)
+              for (int s = 0; s < 2; ++s) {
+                const double* Ds = D + s * nb2;
+                const double* Db = D + (1 - s) * nb2;
+                value[s] += Ds[q * nb + p] * Ds[t * nb + r] *
+                            (Q(QSS, j, p, r, i, q, t) - Q(QSS, j, p, r, i, t, q) + 2 * Q(QSS, j, p, r, q, t, i) -
+                             2 * Q(QSS, j, p, r, t, q, i));
+                value[s] += 2 * Ds[q * nb + p] * Db[t * nb + r] * (Q(QSO, j, p, r, i, q, t) - Q(QSO, j, p, r, q, i, t));
+                value[s] += Db[q * nb + p] * Db[t * nb + r] * (Q(QOO, j, p, r, i, q, t) - Q(QOO, j, p, r, i, t, q));
+              }
+      G3[j * nb + i] = value[0] / 2;
+      G3[nb2 + j * nb + i] = value[1] / 2;
+    }
 }
 
 }  // extern "C"

@@ -7,13 +7,13 @@ QMCGrid / computeF / computeV / *HartreeFock call surface.
 from ._lib import (BF16X3, BF16X5, FP32, FP64, K_COULOMB, K_JASTROW_GRAD, K_TC_SCALAR, RADIAL_BECKE, RADIAL_M4,
                    RADIAL_XOVER1MX, ConvergenceError, SolverError, TcscfError, lib)
 from .api import (Jastrow, QMCGrid, QuadratureGrid, RCSHartreeFock, SlaterTypeOrbital, TCHartreeFock, UOSHartreeFock,
-                  coreMatrixElement, computeF, computeV, even_tempered, lebedev, npair, one_electron_matrices,
+                  coreMatrixElement, computeF, computeV, eri_full, even_tempered, integrateAllR1R2, lebedev, npair, one_electron_matrices,
                   overlap, pair_index, product_grid, radial_grid, unpack)
 
 __all__ = [
     "BF16X3", "BF16X5", "FP32", "FP64", "K_COULOMB", "K_JASTROW_GRAD", "K_TC_SCALAR", "RADIAL_BECKE", "RADIAL_M4",
     "RADIAL_XOVER1MX", "ConvergenceError", "SolverError", "TcscfError", "lib", "Jastrow", "QMCGrid",
     "QuadratureGrid", "RCSHartreeFock", "SlaterTypeOrbital", "TCHartreeFock", "UOSHartreeFock", "coreMatrixElement", "computeF",
-    "computeV", "even_tempered", "lebedev", "npair", "one_electron_matrices", "overlap", "pair_index",
+    "computeV", "eri_full", "even_tempered", "integrateAllR1R2", "lebedev", "npair", "one_electron_matrices", "overlap", "pair_index",
     "product_grid", "radial_grid", "unpack",
 ]

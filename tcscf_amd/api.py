@@ -271,6 +271,28 @@ def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64", ordere
     return _compute(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, True, precision, out_dtype, ordered)
 
 
+def integrateAllR1R2(r1Grid, F):
+    """Grid ERI tensor (src/integration/integrateAll.hpp:221-317, USE_GRADIENT =
+    false) from F = computeF(r1Grid, r2Grid, ...): returns the packed
+    (npair, npair) device tensor E with E[p(b1,b3), p(b2,b4)] =
+    answer(b1, b2, b3, b4).  eri_full() expands it to nb^4."""
+    torch = _torch()
+    nb = r1Grid.nBasis()
+    if F.dtype != torch.float64 or F.shape != (r1Grid.nGrid(), npair(nb)):
+        raise ValueError("integrateAllR1R2: F must be packed FP64 (n1, npair)")
+    E = torch.empty((npair(nb), npair(nb)), dtype=torch.float64, device=F.device)
+    check(lib().tcscf_eri(_ptr(r1Grid.basisValues), _ptr(r1Grid.quadratureGrid.weights),
+                          ctypes.c_int64(r1Grid.nGrid()), nb, _ptr(F), _ptr(E), _stream()))
+    return E
+
+
+def eri_full(E, nb):
+    """Packed ERI -> answer(b1, b2, b3, b4) (numpy)."""
+    idx = np.array([[pair_index(j, i) for i in range(nb)] for j in range(nb)])
+    E = E.cpu().numpy() if not isinstance(E, np.ndarray) else E
+    return E[idx[:, None, :, None], idx[None, :, None, :]]
+
+
 # --------------------------------------------------------------------------- SCF
 def _opts(tol, max_iter, mask, mgs_mode=0):
     return _lib.ScfOpts(float(tol), int(max_iter), int(bool(mask)), int(mgs_mode))
@@ -425,3 +447,24 @@ class TCHartreeFock(_HFBase):
 
     def highestOccupiedOrbitalEnergy(self):
         return max(self.occupiedOrbitalPseudoEnergy[0][-1], self.occupiedOrbitalPseudoEnergy[1][-1])
+
+    def fockBuild(self, r1Grid, FjiSame, FjiOppo, Vji, jastrow, density, occupiedOrbitals):
+        """One Fock build of the TC loop (src/HartreeFock.cpp:1118-1224) at a
+        given density (2, nb, nb) and occupied orbitals (2, nb, max nocc):
+        returns the two-electron term G2 and the three-electron term G3, each
+        (2, nb, nb), before the one-electron-symmetry mask."""
+        nb = self.basisSize
+        nmax = max(self.nElectrons)
+        D = np.ascontiguousarray(density, dtype=np.float64)
+        C = np.ascontiguousarray(occupiedOrbitals, dtype=np.float64)
+        if D.shape != (2, nb, nb) or C.shape != (2, nb, nmax) or r1Grid.nBasis() != nb:
+            raise ValueError("fockBuild: density must be (2, nb, nb) and orbitals (2, nb, max nocc)")
+        G2 = np.zeros((2, nb, nb))
+        G3 = np.zeros((2, nb, nb))
+        jp = jastrow._struct()
+        check(lib().tcscf_tc_fock(self.nElectrons[0], self.nElectrons[1], nb, _ptr(r1Grid.basisValues),
+                                  _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights),
+                                  ctypes.c_int64(r1Grid.nGrid()), _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji),
+                                  ctypes.byref(jp), D.ctypes.data_as(_dp), C.ctypes.data_as(_dp),
+                                  G2.ctypes.data_as(_dp), G3.ctypes.data_as(_dp), _stream()))
+        return G2, G3

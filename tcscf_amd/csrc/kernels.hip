@@ -896,3 +896,93 @@ int tcscf_selftest_mfma(int which, const double* A_dev, const double* B_dev, dou
 }
 
 }  // extern "C"
+
+/* ======================================================================== */
+/* Grid ERI: integrateAllR1R2<false> (src/integration/integrateAll.hpp:221-317) */
+/*   ERI[p(b1,b3)][p(b2,b4)] = sum_n1 w1 A1[n1,b1] A1[n1,b3] F[n1, p(b2,b4)]  */
+/* as an FP64 MFMA GEMM C = P1w^T F (M = N = npair, K = n1), split over K.    */
+/* ======================================================================== */
+namespace tc {
+
+// Workgroup = 4 waves (2 x 2) computing a 128 x 128 tile of C over the K range
+// [k_begin, k_end) of blockIdx.z; partial tiles go to part[z][npair][npair].
+__global__ __launch_bounds__(256) void eri_gemm_kernel(const double* __restrict__ P, int64_t ldp,
+                                                       const double* __restrict__ F, int64_t npair, int64_t n1,
+                                                       int64_t kchunk, double* __restrict__ part) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int64_t m0 = (int64_t)blockIdx.y * 128 + (wave >> 1) * 64;
+  const int64_t c0 = (int64_t)blockIdx.x * 128 + (wave & 1) * 64;
+  const int64_t kb = (int64_t)blockIdx.z * kchunk;
+  const int64_t ke = kb + kchunk < n1 ? kb + kchunk : n1;
+  d4 acc[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = d4{0, 0, 0, 0};
+  for (int64_t k0 = kb; k0 < ke; k0 += 4) {
+    const int64_t k = k0 + g;
+    const bool kin = k < ke;
+    double a[4], b[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int64_t row = m0 + 16 * m + r;  // pair index P (ldp-padded, zero beyond npair)
+      a[m] = kin ? P[k * ldp + row] : 0.0;
+    }
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int64_t col = c0 + 16 * n + r;
+      b[n] = (kin && col < npair) ? F[k * npair + col] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[n], acc[m][n], 0, 0, 0);
+  }
+  double* out = part + (int64_t)blockIdx.z * npair * npair;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t row = m0 + 16 * m + d_row_f64(g, i), col = c0 + 16 * n + r;
+        if (row < npair && col < npair) out[row * npair + col] = acc[m][n][i];
+      }
+}
+
+__global__ void eri_reduce_kernel(const double* __restrict__ part, int nz, int64_t len, double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  double s = 0;
+  for (int z = 0; z < nz; ++z) s += part[(int64_t)z * len + e];
+  out[e] = s;
+}
+
+}  // namespace tc
+
+extern "C" int tcscf_eri(const double* A1, const double* w1, int64_t n1, int nb, const void* F, double* eri,
+                         void* stream) {
+  using namespace tc;
+  TC_CHECK(A1 && w1 && F && eri && n1 > 0 && nb > 0, "tcscf_eri: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t npair = npair_of(nb), ldp = round_up(npair, 128), n1p = round_up(n1, 16);
+  const int nz = (int)std::min<int64_t>(64, std::max<int64_t>(1, n1 / 512));
+  const int64_t kchunk = round_up((n1 + nz - 1) / nz, 4);
+  int rc = 0;
+  double* ws = (double*)scratch(kSlotSmall, sizeof(double) * (n1p * ldp + (size_t)nz * npair * npair), &rc);
+  if (rc) return rc;
+  double* P = ws;
+  double* part = ws + n1p * ldp;
+  const int64_t tot = n1p * ldp;
+  hipLaunchKernelGGL(pack_pairs_rowmajor<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A1, w1,
+                     (const int*)nullptr, n1, nb, npair, n1p, ldp, P);
+  TC_HIP(hipGetLastError());
+  dim3 grid((unsigned)(ldp / 128), (unsigned)(ldp / 128), (unsigned)nz);
+  hipLaunchKernelGGL(eri_gemm_kernel, grid, dim3(256), 0, st, P, ldp, (const double*)F, npair, n1, kchunk, part);
+  TC_HIP(hipGetLastError());
+  const int64_t len = npair * npair;
+  hipLaunchKernelGGL(eri_reduce_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nz, len, eri);
+  TC_HIP(hipGetLastError());
+  return 0;
+}

@@ -332,3 +332,60 @@ def test_tc_zero_jastrow_equals_uos(tc, gpu):
     F = tc.computeF(g1, g1)
     eu = tc.UOSHartreeFock(2, 2, nb).compute(S, H, g1, F, True, tol=1e-10)
     assert abs(e - eu) < 1e-9
+
+
+@pytest.mark.parametrize("nup,ndown", [(2, 2), (2, 1), (1, 1)])
+def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown):
+    """One device TC Fock build at a given density and orbitals (different per
+    spin) against the oracle's factorized restatement (G2 and G3), and G3
+    against assembleG over the spin-labelled explicit three-body tensors
+    (src/HartreeFock.cpp:613-750)."""
+    from tcscf_amd import SlaterTypeOrbital as STO
+    basis = [STO(1.3, 1, 0, 0), STO(0.9, 2, 1, 0), STO(0.7, 2, 0, 0)]
+    nb = len(basis)
+    g1, g3, jas, Fs, Fo, V = _tc_case(tc, basis, (6, 7, tc.RADIAL_M4), (8, 7, tc.RADIAL_M4))
+    rng = np.random.default_rng(11)
+    C = np.zeros((2, nb, max(nup, ndown)))
+    D = np.zeros((2, nb, nb))
+    for s, no in enumerate((nup, ndown)):
+        C[s, :, :no] = rng.normal(size=(nb, no)) * 0.5
+        D[s] = C[s, :, :no] @ C[s, :, :no].T
+    G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, V, jas, D, C)
+    A = g1.basisValues.cpu().numpy()
+    dA = g1.basisGradients.cpu().numpy()
+    Vf = tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1)
+    Vs, Vo = np.ascontiguousarray(jas.c_same * Vf), np.ascontiguousarray(jas.c_oppo * Vf)
+    G2o, G3o = O.tc_fock(nup, ndown, A, dA, g1.weights_host, tc.unpack(Fs.cpu().numpy(), nb),
+                         tc.unpack(Fo.cpu().numpy(), nb), Vs, Vo, D, C)
+    assert rel_err(G2, G2o) < 1e-12
+    assert rel_err(G3, G3o) < 1e-12
+    G3q = O.assemble_G3(*O.compute_Q(A, g1.weights_host, Vs, Vo, labelled=True), D)
+    assert rel_err(G3, G3q) < 1e-11
+
+
+def test_grid_eri_vs_oracle(tc, O, gpu):
+    """integrateAllR1R2 (src/integration/integrateAll.hpp:221-317) on the
+    device: against the oracle's contraction of the same F, the 8-fold
+    permutational symmetry of real orbitals, and the 1s closed form."""
+    from tcscf_amd import configs
+    c = configs.he_config1()
+    g = _grid(tc, *c["grid"])
+    g.setBasisFunctions(c["basis"], False)
+    F = tc.computeF(g, g)
+    E = tc.eri_full(tc.integrateAllR1R2(g, F), 5)
+    A = g.basisValues.cpu().numpy()
+    ref = O.eri_from_F(A, g.weights_host, tc.unpack(F.cpu().numpy(), 5))
+    assert rel_err(E, ref) < 1e-13
+    chem = E.transpose(0, 2, 1, 3)  # (b1 b3 | b2 b4)
+    for perm in ((1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1)):
+        assert rel_err(chem.transpose(perm), chem) < 1e-12
+    # several r1 chunks, a different r2 grid, a basis with l > 0
+    basis = _mixed_basis()[:9]
+    g1 = _grid(tc, 23, 11, tc.RADIAL_M4)
+    g2 = _grid(tc, 17, 7, tc.RADIAL_BECKE)
+    g1.setBasisFunctions(basis, False)
+    g2.setBasisFunctions(basis, False)
+    F = tc.computeF(g1, g2)
+    E = tc.eri_full(tc.integrateAllR1R2(g1, F), 9)
+    ref = O.eri_from_F(g1.basisValues.cpu().numpy(), g1.weights_host, tc.unpack(F.cpu().numpy(), 9))
+    assert rel_err(E, ref) < 1e-13

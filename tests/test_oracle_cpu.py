@@ -443,3 +443,75 @@ def test_tc_with_zero_jastrow_equals_uos(O):
     u1 = O.uos_scf(1, 1, S, H, A, w, F, tol=1e-10)
     assert abs(t0["energy"] - t1["energy"]) < 1e-12
     assert abs(t0["energy"] - u1["energy"]) < 1e-9
+
+
+# ------------------------------------------- three-body: factorized vs explicit
+def _tc_small_case(O, nb_fns, npts_r1=(4, 7), npts_r3=(5, 5), seed=7, nocc=(2, 1)):
+    """Small TC inputs: r1 grid, V_same / V_oppo from the Jastrow-gradient
+    kernel on an r3 grid (c_oppo = 2 c_same, as the reference's ratio), random
+    occupied orbitals C and D = C C^T per spin."""
+    pts1, w1 = O.product_grid(*npts_r1)
+    pts3, w3 = O.product_grid(*npts_r3)
+    A1, dA1 = O.sto_real(pts1, nb_fns, grads=True)
+    A3 = O.sto_real(pts3, nb_fns)
+    prm = O.ochi_params(1.0, 1.0, [(1, 0, 0, 0.5, 0.25)])
+    Vs = O.compute_F(pts1, pts3, w3, A3, kernel=O.K_JASTROW_GRAD, prm=prm, same=True)
+    Vo = O.compute_F(pts1, pts3, w3, A3, kernel=O.K_JASTROW_GRAD, prm=prm, same=False)
+    Fs = O.compute_F(pts1, pts1, w1, A1, kernel=O.K_TC_SCALAR, prm=prm, same=True)
+    Fo = O.compute_F(pts1, pts1, w1, A1, kernel=O.K_TC_SCALAR, prm=prm, same=False)
+    nb = len(nb_fns)
+    rng = np.random.default_rng(seed)
+    C = np.zeros((2, nb, max(nocc)))
+    D = np.zeros((2, nb, nb))
+    for s in range(2):
+        C[s, :, :nocc[s]] = rng.normal(size=(nb, nocc[s])) * 0.5
+        D[s] = C[s, :, :nocc[s]] @ C[s, :, :nocc[s]].T
+    return dict(A1=A1, dA1=dA1, w1=w1, Fs=Fs, Fo=Fo, Vs=Vs, Vo=Vo, C=C, D=D, nocc=nocc)
+
+
+@pytest.mark.parametrize("nocc", [(1, 1), (2, 1), (2, 2)])
+def test_factorized_G3_equals_explicit_Q(O, nocc):
+    """The reference's own disabled cross-check (src/HartreeFock.cpp:1098-1116,
+    1227-1247) of the factorized three-body term G3 against assembleG over the
+    explicit six-index tensors of computeQ (src/HartreeFock.cpp:613-750).
+
+    - With the pair tensors taken from the electrons' spins (QSO = (s,s,t),
+      QOO = (s,t,t)), assembleG equals the factorized G3 for arbitrary,
+      independent V_same and V_oppo and different orbitals per spin.
+    - computeQ as the reference writes it labels the pairs differently (its
+      so312 term reads V_same(j,i), QOO is all V_oppo): it agrees only when
+      V_oppo = V_same, and differs at the reference's c_oppo = 2 c_same, which
+      is why that check stays disabled there.
+    - The identities assembleG asserts (2 QSS = QSO, 4 QSS = QOO,
+      Q(j,p,r,i,q,s) = Q(j,r,p,i,s,q)) hold for the reference's computeQ at
+      c_oppo = 2 c_same."""
+    fns = [(1, 0, 0, 1.3), (2, 1, 0, 0.9), (2, 0, 0, 0.7)]
+    c = _tc_small_case(O, fns, nocc=nocc)
+    args = (c["A1"], c["dA1"], c["w1"], c["Fs"], c["Fo"])
+    Vs, Vo, D, C = c["Vs"], c["Vo"], c["D"], c["C"]
+
+    QSS, QSO, QOO = O.compute_Q(c["A1"], c["w1"], Vs, Vo)
+    scale = np.max(np.abs(QSS))
+    assert np.max(np.abs(2 * QSS - QSO)) < 1e-13 * scale
+    assert np.max(np.abs(4 * QSS - QOO)) < 1e-13 * scale
+    assert np.max(np.abs(QSS - QSS.transpose(0, 2, 1, 3, 5, 4))) < 1e-13 * scale
+
+    for Vo_case in (Vo, 0.7 * Vo, Vs):
+        _, G3 = O.tc_fock(*nocc, *args, Vs, Vo_case, D, C)
+        assert np.max(np.abs(G3)) > 1e-7
+        G3q = O.assemble_G3(*O.compute_Q(c["A1"], c["w1"], Vs, Vo_case, labelled=True), D)
+        assert rel_err(G3, G3q) < 1e-12
+        G3r = O.assemble_G3(*O.compute_Q(c["A1"], c["w1"], Vs, Vo_case, labelled=False), D)
+        if Vo_case is Vs:
+            assert rel_err(G3, G3r) < 1e-12
+        else:
+            assert rel_err(G3, G3r) > 1e-2
+
+    # planted mistakes in the factorized inputs are visible
+    G3q = O.assemble_G3(*O.compute_Q(c["A1"], c["w1"], Vs, Vo, labelled=True), D)
+    _, G3x = O.tc_fock(*nocc, *args, Vs, 0 * Vo, D, C)
+    assert rel_err(G3x, G3q) > 1e-2
+    Cx = C.copy()
+    Cx[0] *= -1.1          # orbitals inconsistent with D
+    _, G3x = O.tc_fock(*nocc, *args, Vs, Vo, D, Cx)
+    assert rel_err(G3x, G3q) > 1e-3
