@@ -143,6 +143,26 @@ size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
 int tcscf_eri(const double* A1_dev, const double* w1_dev, int64_t n1, int nb, const void* F_dev, double* eri_dev,
               void* stream);
 
+/* integrateAllR1R2<true> (src/integration/integrateAll.hpp:184-215, 221-317),
+ * real basis, for the vector kernel f = c_same grad_1 u(r1, r2) (kernel must be
+ * TCSCF_K_JASTROW_GRAD):
+ *   answer(b2, b1, b4, b3) = sum_r1 w1 A1_b1 A1_b3 scale sum_r2 w2 A2_b2 f.grad A2_b4
+ * written as eri_dev[p(b1, b3)][b2 nb + b4] (npair x nb^2 FP64).  dA2_dev is
+ * n2 x nb x 3; scale = 2 pi for the reference's 2-D r2 grid (points with y = 0).
+ * The inner integral is one FP64 MFMA GEMM with K = 3 n2 (point x component). */
+int tcscf_eri_gradient(const double* pts1_dev, const double* A1_dev, const double* w1_dev, int64_t n1,
+                       const double* pts2_dev, const double* w2_dev, int64_t n2, const double* A2_dev,
+                       const double* dA2_dev, int nb, int kernel, const tcscf_jastrow* jastrow, double scale,
+                       double* eri_dev, void* stream);
+
+/* threeElectronIntegrals (src/integration/integrateAll.hpp:369-442), real
+ * basis: Q(p,q,r,s,t,u) = sum_r1 w1 A1_p A1_s V(q,t).V(r,u), with V = c V_dev
+ * (V_dev the packed c = 1 n1 x 3 x npair tensor of TCSCF_K_JASTROW_GRAD; the
+ * reference evaluates the same-spin gradient, so c = c_same).  Q_dev: nb^6
+ * FP64, index order (p, q, r, s, t, u).  Validation-sized: nb <= 16. */
+int tcscf_three_electron(const double* A1_dev, const double* w1_dev, int64_t n1, const double* V_dev, int nb, double c,
+                         double* Q_dev, void* stream);
+
 /* ---------------------------------------------------------------- SCF drivers */
 /* RCSHartreeFock::compute (src/HartreeFock.cpp:785-886).  S, H host nb x nb
  * row-major; A1_dev n1 x nb; w1_dev n1; F_dev packed n1 x npair (f_dtype).

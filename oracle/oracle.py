@@ -181,6 +181,17 @@ def eri_from_F(A1, w1, F):
     return out
 
 
+def eri_gradient(prm, same, pts1, A1, w1, pts2, w2, A2, dA2, scale=1.0):
+    """integrateAllR1R2<true>: nb^4 in the reference's (b2, b1, b4, b3) order."""
+    nb = A1.shape[1]
+    out = np.zeros((nb,) * 4)
+    c = np.ascontiguousarray
+    lib().orc_eri_gradient(_p(prm), int(same), _p(c(pts1)), _p(c(A1)), _p(w1), ctypes.c_int64(A1.shape[0]),
+                           _p(c(pts2)), _p(w2), ctypes.c_int64(A2.shape[0]), _p(c(A2)), _p(c(dA2)), nb,
+                           ctypes.c_double(scale), _p(out))
+    return out
+
+
 def three_electron(A1, w1, V):
     nb = A1.shape[1]
     Q = np.zeros((nb,) * 6)

@@ -465,6 +465,49 @@ void orc_three_electron(const double* A1, const double* w1, int64_t n1, const do
                     w1[k] * A1[k * nb + p] * dot * A1[k * nb + s];
               }
 }
+
+/* integrateAllR1R2<true>, src/integration/integrateAll.hpp:184-317, real basis:
+ * answer(b2, b1, b4, b3) = sum_r1 w1 A1_b1 A1_b3 * scale * sum_r2 w2 A2_b2
+ * f(r1, r2).grad A2_b4, with f = grad_1 u (the Ochi gradient of prm, spin
+ * `same`; coincident pairs give 0 as in orc_compute_F).  out: nb^4 in the
+ * reference's (b2, b1, b4, b3) order.  Points SoA 3 x n. */
+int orc_eri_gradient(const double* prm, int same, const double* p1, const double* A1, const double* w1, int64_t n1,
+                     const double* p2, const double* w2, int64_t n2, const double* A2, const double* dA2, int nb,
+                     double scale, double* out) {
+  int64_t const nb2 = (int64_t)nb * nb;
+  std::fill(out, out + nb2 * nb2, 0.0);
+#pragma omp parallel
+  {
+    std::vector<double> loc(nb2 * nb2, 0.0), inner(nb2);
+#pragma omp for schedule(static)
+    for (int64_t k = 0; k < n1; ++k) {
+      double const r1[3] = {p1[k], p1[n1 + k], p1[2 * n1 + k]};
+      std::fill(inner.begin(), inner.end(), 0.0);
+      for (int64_t q = 0; q < n2; ++q) {
+        double const r2[3] = {p2[q], p2[n2 + q], p2[2 * n2 + q]};
+        double const dx = r1[0] - r2[0], dy = r1[1] - r2[1], dz = r1[2] - r2[2];
+        if (dx * dx + dy * dy + dz * dz == 0) continue;
+        double f[3];
+        orc_ochi_gradient(prm, r1, r2, same, f);
+        for (int b4 = 0; b4 < nb; ++b4) {
+          const double* g = dA2 + (q * nb + b4) * 3;
+          double const fg = w2[q] * (f[0] * g[0] + f[1] * g[1] + f[2] * g[2]);
+          for (int b2 = 0; b2 < nb; ++b2) inner[b2 * nb + b4] += A2[q * nb + b2] * fg;
+        }
+      }
+      for (int b2 = 0; b2 < nb; ++b2)
+        for (int b1 = 0; b1 < nb; ++b1)
+          for (int b4 = 0; b4 < nb; ++b4)
+            for (int b3 = 0; b3 < nb; ++b3)
+              loc[((b2 * nb + b1) * nb + b4) * nb + b3] +=
+                  w1[k] * A1[k * nb + b1] * A1[k * nb + b3] * scale * inner[b2 * nb + b4];
+    }
+#pragma omp critical
+    for (int64_t q = 0; q < nb2 * nb2; ++q) out[q] += loc[q];
+  }
+  return 0;
+}
+
 }  // extern "C"
 
 /* ======================================================================== */

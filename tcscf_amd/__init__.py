@@ -7,13 +7,15 @@ QMCGrid / computeF / computeV / *HartreeFock call surface.
 from ._lib import (BF16X3, BF16X5, FP32, FP64, K_COULOMB, K_JASTROW_GRAD, K_TC_SCALAR, RADIAL_BECKE, RADIAL_M4,
                    RADIAL_XOVER1MX, ConvergenceError, SolverError, TcscfError, lib)
 from .api import (Jastrow, QMCGrid, QuadratureGrid, RCSHartreeFock, SlaterTypeOrbital, TCHartreeFock, UOSHartreeFock,
-                  coreMatrixElement, computeF, computeV, eri_full, even_tempered, integrateAllR1R2, lebedev, npair, one_electron_matrices,
-                  overlap, pair_index, product_grid, radial_grid, unpack)
+                  coreMatrixElement, computeF, computeV, eri_full, eri_full_gradient, even_tempered,
+                  integrateAllR1R2, integrateAllR1R2Gradient, lebedev, npair, one_electron_matrices, overlap,
+                  pair_index, product_grid, radial_grid, threeElectronIntegrals, unpack)
 
 __all__ = [
     "BF16X3", "BF16X5", "FP32", "FP64", "K_COULOMB", "K_JASTROW_GRAD", "K_TC_SCALAR", "RADIAL_BECKE", "RADIAL_M4",
     "RADIAL_XOVER1MX", "ConvergenceError", "SolverError", "TcscfError", "lib", "Jastrow", "QMCGrid",
-    "QuadratureGrid", "RCSHartreeFock", "SlaterTypeOrbital", "TCHartreeFock", "UOSHartreeFock", "coreMatrixElement", "computeF",
-    "computeV", "eri_full", "even_tempered", "integrateAllR1R2", "lebedev", "npair", "one_electron_matrices", "overlap", "pair_index",
-    "product_grid", "radial_grid", "unpack",
+    "QuadratureGrid", "RCSHartreeFock", "SlaterTypeOrbital", "TCHartreeFock", "UOSHartreeFock", "coreMatrixElement",
+    "computeF", "computeV", "eri_full", "eri_full_gradient", "even_tempered", "integrateAllR1R2",
+    "integrateAllR1R2Gradient", "lebedev", "npair", "one_electron_matrices", "overlap", "pair_index", "product_grid",
+    "radial_grid", "threeElectronIntegrals", "unpack",
 ]

@@ -22,7 +22,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16x5": BF16X5}
 EXPORTS = [
     "tcscf_last_error", "tcscf_version", "tcscf_radial_grid", "tcscf_lebedev_npts", "tcscf_lebedev",
     "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
-    "tcscf_F_workspace_bytes", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri",
+    "tcscf_F_workspace_bytes", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
     "tcscf_selftest_mfma",
 ]
 

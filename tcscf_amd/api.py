@@ -286,6 +286,55 @@ def integrateAllR1R2(r1Grid, F):
     return E
 
 
+def integrateAllR1R2Gradient(r1Grid, r2Grid, jastrow):
+    """integrateAllR1R2<true> (src/integration/integrateAll.hpp:184-317) with the
+    vector kernel f = c_same grad_1 u(r1, r2): the packed (npair, nb^2) device
+    tensor E with E[p(b1,b3), b2 nb + b4] = answer(b2, b1, b4, b3) =
+    sum_r1 w1 phi_b1 phi_b3 scale sum_r2 w2 phi_b2 f.grad phi_b4 (scale = 2 pi
+    for a 2-D r2 grid).  eri_full_gradient() expands it to nb^4."""
+    torch = _torch()
+    nb = r1Grid.nBasis()
+    if r2Grid.nBasis() != nb or not r2Grid.gradientsStored():
+        raise ValueError("integrateAllR1R2Gradient: the r2 grid needs the same basis with gradients stored")
+    E = torch.empty((npair(nb), nb * nb), dtype=torch.float64, device=r1Grid.basisValues.device)
+    scale = 2 * math.pi if r2Grid.ndim == 2 else 1.0
+    jp = jastrow._struct()
+    check(lib().tcscf_eri_gradient(_ptr(r1Grid.quadratureGrid.points), _ptr(r1Grid.basisValues),
+                                   _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()),
+                                   _ptr(r2Grid.quadratureGrid.points), _ptr(r2Grid.quadratureGrid.weights),
+                                   ctypes.c_int64(r2Grid.nGrid()), _ptr(r2Grid.basisValues),
+                                   _ptr(r2Grid.basisGradients), nb, _lib.K_JASTROW_GRAD, ctypes.byref(jp),
+                                   ctypes.c_double(scale), _ptr(E), _stream()))
+    return E
+
+
+def eri_full_gradient(E, nb):
+    """Packed gradient ERI -> answer(b2, b1, b4, b3) (numpy, index order
+    (b2, b1, b4, b3) as the reference stores it)."""
+    idx = np.array([[pair_index(j, i) for i in range(nb)] for j in range(nb)])
+    E = E.cpu().numpy() if not isinstance(E, np.ndarray) else E
+    b = np.arange(nb)
+    # out[b2, b1, b4, b3] = E[idx[b1, b3], b2 * nb + b4]
+    return E[idx[None, :, None, :], (b[:, None, None, None] * nb + b[None, None, :, None])]
+
+
+def threeElectronIntegrals(r1Grid, r2Grid, jastrow, precision="fp64"):
+    """threeElectronIntegrals (src/integration/integrateAll.hpp:369-442): the
+    (nb,)*6 device tensor Q(p,q,r,s,t,u) = sum_r1 w1 phi_p phi_s(r1)
+    V(r1,q,t).V(r1,r,u), with V(r1,q,t) = sum_r2 w2 phi_q phi_t(r2) grad_1 u(r1, r2)
+    for the same-spin Jastrow.  Validation-sized (nb <= 16)."""
+    torch = _torch()
+    nb = r1Grid.nBasis()
+    if r2Grid.nBasis() != nb:
+        raise ValueError("threeElectronIntegrals: the grids carry different bases")
+    V = computeV(r1Grid, r2Grid, jastrow, precision=precision)
+    Q = torch.empty((nb,) * 6, dtype=torch.float64, device=V.device)
+    check(lib().tcscf_three_electron(_ptr(r1Grid.basisValues), _ptr(r1Grid.quadratureGrid.weights),
+                                     ctypes.c_int64(r1Grid.nGrid()), _ptr(V), nb, ctypes.c_double(jastrow.c_same),
+                                     _ptr(Q), _stream()))
+    return Q
+
+
 def eri_full(E, nb):
     """Packed ERI -> answer(b1, b2, b3, b4) (numpy)."""
     idx = np.array([[pair_index(j, i) for i in range(nb)] for j in range(nb)])

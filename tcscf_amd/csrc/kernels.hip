@@ -898,16 +898,18 @@ int tcscf_selftest_mfma(int which, const double* A_dev, const double* B_dev, dou
 }  // extern "C"
 
 /* ======================================================================== */
-/* Grid ERI: integrateAllR1R2<false> (src/integration/integrateAll.hpp:221-317) */
-/*   ERI[p(b1,b3)][p(b2,b4)] = sum_n1 w1 A1[n1,b1] A1[n1,b3] F[n1, p(b2,b4)]  */
-/* as an FP64 MFMA GEMM C = P1w^T F (M = N = npair, K = n1), split over K.    */
+/* Grid ERI: integrateAllR1R2 (src/integration/integrateAll.hpp:221-317)     */
+/*   E[p(b1,b3)][col] = sum_n1 w1 A1[n1,b1] A1[n1,b3] T[n1, col]              */
+/* as an FP64 MFMA GEMM E = P1w^T T (M = npair, N = ncol, K = n1), split over */
+/* K.  T = F (packed pairs, USE_GRADIENT = false) or the gradient inner       */
+/* integral (nb^2 ordered pairs (b2, b4), USE_GRADIENT = true).              */
 /* ======================================================================== */
 namespace tc {
 
-// Workgroup = 4 waves (2 x 2) computing a 128 x 128 tile of C over the K range
-// [k_begin, k_end) of blockIdx.z; partial tiles go to part[z][npair][npair].
-__global__ __launch_bounds__(256) void eri_gemm_kernel(const double* __restrict__ P, int64_t ldp,
-                                                       const double* __restrict__ F, int64_t npair, int64_t n1,
+// Workgroup = 4 waves (2 x 2) computing a 128 x 128 tile of E over the K range
+// [kb, kb + kchunk) of blockIdx.z; partial tiles go to part[z][M][ncol].
+__global__ __launch_bounds__(256) void eri_gemm_kernel(const double* __restrict__ P, int64_t ldp, int64_t M,
+                                                       const double* __restrict__ T, int64_t ncol, int64_t n1,
                                                        int64_t kchunk, double* __restrict__ part) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
@@ -925,21 +927,18 @@ __global__ __launch_bounds__(256) void eri_gemm_kernel(const double* __restrict_
     const bool kin = k < ke;
     double a[4], b[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int64_t row = m0 + 16 * m + r;  // pair index P (ldp-padded, zero beyond npair)
-      a[m] = kin ? P[k * ldp + row] : 0.0;
-    }
+    for (int m = 0; m < 4; ++m) a[m] = kin ? P[k * ldp + m0 + 16 * m + r] : 0.0;  // zero beyond M
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       const int64_t col = c0 + 16 * n + r;
-      b[n] = (kin && col < npair) ? F[k * npair + col] : 0.0;
+      b[n] = (kin && col < ncol) ? T[k * ncol + col] : 0.0;
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int n = 0; n < 4; ++n) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[n], acc[m][n], 0, 0, 0);
   }
-  double* out = part + (int64_t)blockIdx.z * npair * npair;
+  double* out = part + (int64_t)blockIdx.z * M * ncol;
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -947,7 +946,7 @@ __global__ __launch_bounds__(256) void eri_gemm_kernel(const double* __restrict_
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int64_t row = m0 + 16 * m + d_row_f64(g, i), col = c0 + 16 * n + r;
-        if (row < npair && col < npair) out[row * npair + col] = acc[m][n][i];
+        if (row < M && col < ncol) out[row * ncol + col] = acc[m][n][i];
       }
 }
 
@@ -959,30 +958,225 @@ __global__ void eri_reduce_kernel(const double* __restrict__ part, int nz, int64
   out[e] = s;
 }
 
-}  // namespace tc
+// Ordered gradient pairs, 3 rows per r2 point:
+// Q[3 k + c][b2 nb + b4] = w2[k] A2[k, b2] dA2[k, b4, c]; zero for k >= n2 or col >= nb^2.
+__global__ __launch_bounds__(256) void pack_grad_pairs(const double* __restrict__ A2, const double* __restrict__ dA2,
+                                                       const double* __restrict__ w2, int64_t n2, int nb, int64_t n2p,
+                                                       int64_t ldq, double* __restrict__ Q) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 3 * n2p * ldq) return;
+  const int64_t row = idx / ldq, col = idx - row * ldq;
+  const int64_t k = row / 3;
+  const int c = (int)(row - 3 * k);
+  double v = 0;
+  if (k < n2 && col < (int64_t)nb * nb) {
+    const int b2 = (int)(col / nb), b4 = (int)(col - (int64_t)b2 * nb);
+    v = w2[k] * A2[k * nb + b2] * dA2[(k * nb + b4) * 3 + c];
+  }
+  Q[idx] = v;
+}
 
-extern "C" int tcscf_eri(const double* A1, const double* w1, int64_t n1, int nb, const void* F, double* eri,
-                         void* stream) {
-  using namespace tc;
-  TC_CHECK(A1 && w1 && F && eri && n1 > 0 && nb > 0, "tcscf_eri: bad arguments");
-  hipStream_t st = (hipStream_t)stream;
+// Gradient inner integral T[n1][col] = scale sum_{k, c} f_c(r1 - r2_k) Q[3 k + c][col]
+// (evaluateR2GradientIntegral, src/integration/integrateAll.hpp:184-215) as one
+// FP64 MFMA GEMM with K = 3 n2p (point-major, component-minor).  Same tiling as
+// fgemm_f64_kernel: 4 waves as 2 x 2, wave tile 16 MB x 64.
+template <int MB, class Kern>
+__global__ __launch_bounds__(256) void fgemm_f64_vdot_kernel(const double* __restrict__ p1, int64_t n1,
+                                                             const double* __restrict__ x2, int64_t n2p,
+                                                             const double* __restrict__ Q, int64_t ldq,
+                                                             double* __restrict__ T, int64_t ncol, double scale,
+                                                             Kern kern) {
+  static_assert(Kern::ncomp == 3, "vector kernel expected");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int64_t rowW = (int64_t)blockIdx.y * (32 * MB) + (wave >> 1) * (16 * MB);
+  const int64_t colW = (int64_t)blockIdx.x * 128 + (wave & 1) * 64;
+  double ax[MB], ay[MB], az[MB];
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    const int64_t row = rowW + 16 * m + r;
+    const bool ok = row < n1;
+    ax[m] = ok ? p1[row] : -kFar;
+    ay[m] = ok ? p1[n1 + row] : -kFar;
+    az[m] = ok ? p1[2 * n1 + row] : -kFar;
+  }
+  d4 acc[MB][4];
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = d4{0, 0, 0, 0};
+  const double* __restrict__ Qb = Q + colW + r;
+  for (int64_t k0 = 0; k0 < 3 * n2p; k0 += 4) {
+    const int64_t k = k0 + g, pt = k / 3;
+    const int c = (int)(k - 3 * pt);
+    const double bx = x2[pt], by = x2[n2p + pt], bz = x2[2 * n2p + pt];
+    double b[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) b[n] = Qb[k * ldq + 16 * n];
+    double a[MB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      double v[3];
+      kern(ax[m] - bx, ay[m] - by, az[m] - bz, v);
+      a[m] = c == 0 ? v[0] : (c == 1 ? v[1] : v[2]);
+    }
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[n], acc[m][n], 0, 0, 0);
+  }
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t row = rowW + 16 * m + d_row_f64(g, i), col = colW + 16 * n + r;
+        if (row < n1 && col < ncol) T[row * ncol + col] = scale * acc[m][n][i];
+      }
+}
+
+// E (npair x ncol) = P1w^T T for T n1 x ncol (device); ws holds P1 and the partials.
+int eri_contract(const double* A1, const double* w1, int64_t n1, int nb, const double* T, int64_t ncol, double* E,
+                 double* ws, hipStream_t st) {
   const int64_t npair = npair_of(nb), ldp = round_up(npair, 128), n1p = round_up(n1, 16);
   const int nz = (int)std::min<int64_t>(64, std::max<int64_t>(1, n1 / 512));
   const int64_t kchunk = round_up((n1 + nz - 1) / nz, 4);
-  int rc = 0;
-  double* ws = (double*)scratch(kSlotSmall, sizeof(double) * (n1p * ldp + (size_t)nz * npair * npair), &rc);
-  if (rc) return rc;
   double* P = ws;
   double* part = ws + n1p * ldp;
   const int64_t tot = n1p * ldp;
   hipLaunchKernelGGL(pack_pairs_rowmajor<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A1, w1,
                      (const int*)nullptr, n1, nb, npair, n1p, ldp, P);
   TC_HIP(hipGetLastError());
-  dim3 grid((unsigned)(ldp / 128), (unsigned)(ldp / 128), (unsigned)nz);
-  hipLaunchKernelGGL(eri_gemm_kernel, grid, dim3(256), 0, st, P, ldp, (const double*)F, npair, n1, kchunk, part);
+  dim3 grid((unsigned)(round_up(ncol, 128) / 128), (unsigned)(ldp / 128), (unsigned)nz);
+  hipLaunchKernelGGL(eri_gemm_kernel, grid, dim3(256), 0, st, P, ldp, npair, T, ncol, n1, kchunk, part);
   TC_HIP(hipGetLastError());
-  const int64_t len = npair * npair;
-  hipLaunchKernelGGL(eri_reduce_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nz, len, eri);
+  const int64_t len = npair * ncol;
+  hipLaunchKernelGGL(eri_reduce_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nz, len, E);
+  TC_HIP(hipGetLastError());
+  return 0;
+}
+
+size_t eri_contract_doubles(int64_t n1, int nb, int64_t ncol) {
+  const int64_t npair = npair_of(nb), ldp = round_up(npair, 128), n1p = round_up(n1, 16);
+  const int64_t nz = std::min<int64_t>(64, std::max<int64_t>(1, n1 / 512));
+  return (size_t)(n1p * ldp + nz * npair * ncol);
+}
+
+}  // namespace tc
+
+extern "C" int tcscf_eri(const double* A1, const double* w1, int64_t n1, int nb, const void* F, double* eri,
+                         void* stream) {
+  using namespace tc;
+  TC_CHECK(A1 && w1 && F && eri && n1 > 0 && nb > 0, "tcscf_eri: bad arguments");
+  int rc = 0;
+  double* ws = (double*)scratch(kSlotSmall, sizeof(double) * eri_contract_doubles(n1, nb, npair_of(nb)), &rc);
+  if (rc) return rc;
+  return eri_contract(A1, w1, n1, nb, (const double*)F, npair_of(nb), eri, ws, (hipStream_t)stream);
+}
+
+extern "C" int tcscf_eri_gradient(const double* pts1, const double* A1, const double* w1, int64_t n1,
+                                  const double* pts2, const double* w2, int64_t n2, const double* A2,
+                                  const double* dA2, int nb, int kernel, const tcscf_jastrow* jastrow, double scale,
+                                  double* eri, void* stream) {
+  using namespace tc;
+  TC_CHECK(pts1 && A1 && w1 && pts2 && w2 && A2 && dA2 && eri && n1 > 0 && n2 > 0 && nb > 0,
+           "tcscf_eri_gradient: bad arguments");
+  TC_CHECK(kernel == TCSCF_K_JASTROW_GRAD && jastrow && jastrow->a12 > 0,
+           "tcscf_eri_gradient: needs the vector kernel TCSCF_K_JASTROW_GRAD and a12 > 0");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t ncol = (int64_t)nb * nb, ldq = round_up(ncol, 128), n2p = round_up(n2, 16);
+  const size_t nq = (size_t)3 * n2p * ldq, nx = (size_t)3 * n2p, nt = (size_t)n1 * ncol;
+  int rc = 0;
+  char* ws = (char*)scratch(kSlotSmall, sizeof(float4) * n2p +
+                                            sizeof(double) * (nq + nx + nt + eri_contract_doubles(n1, nb, ncol)), &rc);
+  if (rc) return rc;
+  float4* x4 = (float4*)ws;  // the scratch base is 256-byte aligned
+  double* Q = (double*)(ws + sizeof(float4) * n2p);
+  double* xd = Q + nq;
+  double* T = xd + nx;
+  double* rest = T + nt;
+  hipLaunchKernelGGL(pad_coords, dim3((unsigned)((n2p + 255) / 256)), dim3(256), 0, st, pts2, (const int*)nullptr, n2,
+                     n2p, xd, x4);
+  TC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(pack_grad_pairs, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, A2, dA2, w2, n2, nb, n2p,
+                     ldq, Q);
+  TC_HIP(hipGetLastError());
+  constexpr int MB = 2;
+  const double c = jastrow->c_same;  // f = c_same grad_1 u(r1, r2)
+  KernJastrowGrad kern{jastrow->a12};
+  dim3 grid((unsigned)(ldq / 128), (unsigned)((n1 + 32 * MB - 1) / (32 * MB)));
+  hipLaunchKernelGGL((fgemm_f64_vdot_kernel<MB, KernJastrowGrad>), grid, dim3(256), 0, st, pts1, n1, xd, n2p, Q, ldq,
+                     T, ncol, scale * c, kern);
+  TC_HIP(hipGetLastError());
+  return eri_contract(A1, w1, n1, nb, T, ncol, eri, rest, st);
+}
+/* ======================================================================== */
+/* threeElectronIntegrals (src/integration/integrateAll.hpp:369-442):        */
+/*   Q(p,q,r,s,t,u) = sum_r1 w1 A_p A_s V(q,t).V(r,u)                          */
+/* from the packed c = 1 Jastrow-gradient tensor V (n1 x 3 x npair), scaled   */
+/* by c^2.  Validation-sized (nb <= 16): one thread per output, r1 points     */
+/* staged through LDS in tiles.                                              */
+/* ======================================================================== */
+namespace tc {
+
+constexpr int kQMaxNb = 16;
+constexpr int kQTile = 16;
+
+__global__ __launch_bounds__(256) void three_electron_kernel(const double* __restrict__ A1, const double* __restrict__ w1,
+                                                             int64_t n1, const double* __restrict__ V, int nb,
+                                                             double c2, double* __restrict__ Q) {
+  __shared__ double sA[kQTile][kQMaxNb];
+  __shared__ double sV[kQTile][3][kQMaxNb * (kQMaxNb + 1) / 2];
+  const int64_t npair = (int64_t)nb * (nb + 1) / 2;
+  const int64_t nq = (int64_t)nb * nb * nb * nb * nb * nb;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int u = 0, t = 0, s = 0, r = 0, q = 0, p = 0;
+  if (o < nq) {
+    int64_t x = o;
+    u = (int)(x % nb); x /= nb;
+    t = (int)(x % nb); x /= nb;
+    s = (int)(x % nb); x /= nb;
+    r = (int)(x % nb); x /= nb;
+    q = (int)(x % nb); x /= nb;
+    p = (int)x;
+  }
+  const int pqt = q <= t ? t * (t + 1) / 2 + q : q * (q + 1) / 2 + t;
+  const int pru = r <= u ? u * (u + 1) / 2 + r : r * (r + 1) / 2 + u;
+  double acc = 0;
+  for (int64_t k0 = 0; k0 < n1; k0 += kQTile) {
+    const int kn = (int)(n1 - k0 < kQTile ? n1 - k0 : kQTile);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kn * nb; e += blockDim.x) {
+      const int kk = e / nb, b = e - kk * nb;
+      sA[kk][b] = A1[(k0 + kk) * nb + b];
+    }
+    for (int e = threadIdx.x; e < kn * 3 * (int)npair; e += blockDim.x) {
+      const int kk = e / (3 * (int)npair), rem = e - kk * 3 * (int)npair;
+      const int c = rem / (int)npair, pp = rem - c * (int)npair;
+      sV[kk][c][pp] = V[((k0 + kk) * 3 + c) * npair + pp];
+    }
+    __syncthreads();
+    if (o < nq)
+      for (int kk = 0; kk < kn; ++kk) {
+        const double d = sV[kk][0][pqt] * sV[kk][0][pru] + sV[kk][1][pqt] * sV[kk][1][pru] +
+                         sV[kk][2][pqt] * sV[kk][2][pru];
+        acc += w1[k0 + kk] * sA[kk][p] * sA[kk][s] * d;
+      }
+  }
+  if (o < nq) Q[o] = c2 * acc;
+}
+
+}  // namespace tc
+
+extern "C" int tcscf_three_electron(const double* A1, const double* w1, int64_t n1, const double* V, int nb, double c,
+                                    double* Q, void* stream) {
+  using namespace tc;
+  TC_CHECK(A1 && w1 && V && Q && n1 > 0 && nb > 0, "tcscf_three_electron: bad arguments");
+  TC_CHECK(nb <= kQMaxNb, "tcscf_three_electron: nb > 16 (the nb^6 tensor is a validation tool)");
+  const int64_t nq = (int64_t)nb * nb * nb * nb * nb * nb;
+  hipLaunchKernelGGL(three_electron_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, (hipStream_t)stream, A1,
+                     w1, n1, V, nb, c * c, Q);
   TC_HIP(hipGetLastError());
   return 0;
 }

@@ -389,3 +389,65 @@ def test_grid_eri_vs_oracle(tc, O, gpu):
     E = tc.eri_full(tc.integrateAllR1R2(g1, F), 9)
     ref = O.eri_from_F(g1.basisValues.cpu().numpy(), g1.weights_host, tc.unpack(F.cpu().numpy(), 9))
     assert rel_err(E, ref) < 1e-13
+
+
+def test_three_electron_integrals_vs_oracle(tc, O, gpu):
+    """threeElectronIntegrals on the device against the oracle, from the same
+    device V and end to end from the oracle's own V."""
+    from tcscf_amd import SlaterTypeOrbital as STO
+    basis = [STO(1.3, 1, 0, 0), STO(0.9, 2, 1, 1), STO(0.7, 2, 0, 0), STO(1.1, 2, 1, -1)]
+    nb = len(basis)
+    g1 = _grid(tc, 7, 7, tc.RADIAL_M4)
+    g2 = _grid(tc, 9, 11, tc.RADIAL_M4)
+    for g in (g1, g2):
+        g.setBasisFunctions(basis, False)
+    jas = tc.Jastrow.from_b(1.0)
+    Q = tc.threeElectronIntegrals(g1, g2, jas).cpu().numpy()
+    A1 = g1.basisValues.cpu().numpy()
+    V = tc.computeV(g1, g2, jas)
+    Vf = np.ascontiguousarray(jas.c_same * tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1))
+    ref = O.three_electron(A1, g1.weights_host, Vf)
+    assert np.max(np.abs(ref)) > 1e-8
+    assert rel_err(Q, ref) < 1e-12
+    prm = O.ochi_params(1.0, 1.0, [(1, 0, 0, jas.c_oppo, jas.c_same)])
+    Vo = O.compute_F(g1.points_host, g2.points_host, g2.weights_host, g2.basisValues.cpu().numpy(),
+                     kernel=O.K_JASTROW_GRAD, prm=prm, same=True)
+    assert rel_err(Q, O.three_electron(A1, g1.weights_host, Vo)) < 1e-11
+
+
+def test_gradient_eri_vs_oracle(tc, O, gpu):
+    """integrateAllR1R2<true> on the device against the oracle's brute-force
+    restatement, on a 3-D r2 grid and on the reference's 2-D r2 grid (2 pi);
+    and with synthetic gradients A e_c it equals the plain ERI of V_c."""
+    import torch
+    from tcscf_amd import SlaterTypeOrbital as STO
+    basis = [STO(1.1, 1, 0, 0), STO(0.8, 2, 1, 1), STO(0.6, 2, 0, 0), STO(0.9, 3, 2, -1), STO(1.4, 2, 1, 0)]
+    nb = len(basis)
+    jas = tc.Jastrow.from_b(1.3)
+    prm = O.ochi_params(1.0, jas.a12, [(1, 0, 0, jas.c_oppo, jas.c_same)])
+    rng = np.random.default_rng(5)
+    # scattered points: no parity zeros in any component
+    g1 = tc.QMCGrid(rng.uniform(-2.5, 2.5, (3, 700)), rng.uniform(0.01, 0.05, 700))
+    g1.setBasisFunctions(basis, False)
+    p2d = rng.uniform(0.05, 3, (3, 230))
+    p2d[1] = 0
+    for g2, scale in ((_grid(tc, 11, 7, tc.RADIAL_M4), 1.0),
+                      (tc.QMCGrid(p2d, rng.uniform(0.1, 1, 230), ndim=2), 2 * math.pi)):
+        g2.setBasisFunctions(basis, True)
+        G = tc.eri_full_gradient(tc.integrateAllR1R2Gradient(g1, g2, jas), nb)
+        A2 = g2.basisValues.cpu().numpy()
+        ref = O.eri_gradient(prm, True, g1.points_host, g1.basisValues.cpu().numpy(), g1.weights_host,
+                             g2.points_host, g2.weights_host, A2, g2.basisGradients.cpu().numpy(), scale=scale)
+        assert np.max(np.abs(ref)) > 1e-6
+        assert rel_err(G, ref) < 1e-12
+    g2 = tc.QMCGrid(rng.uniform(-3, 3, (3, 500)), rng.uniform(0.01, 0.05, 500))
+    g2.setBasisFunctions(basis, True)
+    V = tc.computeV(g1, g2, jas)  # c = 1
+    for c in range(3):
+        dA = torch.zeros_like(g2.basisGradients)
+        dA[..., c] = g2.basisValues
+        g2.basisGradients = dA
+        G = tc.eri_full_gradient(tc.integrateAllR1R2Gradient(g1, g2, jas), nb)
+        E = tc.eri_full(tc.integrateAllR1R2(g1, jas.c_same * V[:, c, :].contiguous()), nb)
+        assert np.max(np.abs(E)) > 1e-6
+        assert rel_err(G.transpose(1, 0, 3, 2), E) < 1e-12

@@ -515,3 +515,45 @@ def test_factorized_G3_equals_explicit_Q(O, nocc):
     Cx[0] *= -1.1          # orbitals inconsistent with D
     _, G3x = O.tc_fock(*nocc, *args, Vs, Vo, D, Cx)
     assert rel_err(G3x, G3q) > 1e-3
+
+
+def test_three_electron_integrals_are_the_unsymmetrized_Q(O):
+    """threeElectronIntegrals (src/integration/integrateAll.hpp:369-442) is the
+    one-term kernel T(p,q,r,s,t,u) = sum w A_p A_s V(q,t).V(r,u) whose three
+    electron placements make computeQ's QSameSame (src/HartreeFock.cpp:613-682):
+    QSS(j,p,r,i,q,s) = T(j,p,r,i,q,s) + T(p,j,r,q,i,s) + T(r,j,p,s,i,q)."""
+    fns = [(1, 0, 0, 1.3), (2, 1, 1, 0.9), (2, 0, 0, 0.7)]
+    c = _tc_small_case(O, fns)
+    T = O.three_electron(c["A1"], c["w1"], c["Vs"])
+    QSS, _, _ = O.compute_Q(c["A1"], c["w1"], c["Vs"], c["Vo"])
+    Qt = T + np.einsum("pjrqis->jpriqs", T) + np.einsum("rjpsiq->jpriqs", T)
+    assert np.max(np.abs(QSS)) > 1e-8
+    assert rel_err(Qt, QSS) < 1e-13
+    # T(p,q,r,s,t,u) = T(p,r,q,s,u,t) (the two V factors commute), and a
+    # transposed V index breaks nothing for real orbitals: V(q,t) = V(t,q)
+    assert rel_err(T.transpose(0, 2, 1, 3, 5, 4), T) < 1e-14
+    assert rel_err(T.transpose(0, 4, 2, 3, 1, 5), T) < 1e-14
+
+
+def test_gradient_eri_reduces_to_eri_of_V(O):
+    """integrateAllR1R2<true> (src/integration/integrateAll.hpp:184-317) with
+    'gradients' replaced by the values times a unit vector e_c is the plain ERI
+    contraction of the c-th component of computeV; the result sits at
+    (b2, b1, b4, b3) (src/integration/integrateAll.hpp:302)."""
+    fns = [(1, 0, 0, 1.1), (2, 1, 1, 0.8), (2, 0, 0, 0.6)]
+    rng = np.random.default_rng(2)
+    pts1, w1 = O.product_grid(5, 7)
+    pts2, w2 = O.product_grid(6, 5)
+    pts1 = pts1 + rng.uniform(-0.2, 0.2, pts1.shape)  # no parity zeros
+    pts2 = pts2 + rng.uniform(-0.2, 0.2, pts2.shape)
+    A1 = O.sto_real(pts1, fns)
+    A2 = O.sto_real(pts2, fns)
+    prm = O.ochi_params(1.0, 1.0, [(1, 0, 0, 0.5, 0.25)])
+    V = O.compute_F(pts1, pts2, w2, A2, kernel=O.K_JASTROW_GRAD, prm=prm, same=True)
+    for c in range(3):
+        dA2 = np.zeros(A2.shape + (3,))
+        dA2[..., c] = A2
+        G = O.eri_gradient(prm, True, pts1, A1, w1, pts2, w2, A2, dA2, scale=1.5)
+        E = O.eri_from_F(A1, w1, np.ascontiguousarray(V[..., c]))
+        assert np.max(np.abs(E)) > 1e-6
+        assert rel_err(G.transpose(1, 0, 3, 2), 1.5 * E) < 1e-13
