@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-tc", action="store_true", help="skip the TC-HF Ne side measurement")
     p.add_argument("--tc-mgs", type=int, default=1, help="TC Gram-Schmidt: 0 reference, 1 metric")
+    p.add_argument("--tc-diis", type=int, default=6, help="TC DIIS depth (the TC workload asks for 6; 0 = plain)")
     return p.parse_args()
 
 
@@ -187,7 +188,8 @@ def run_tc(a, tc, configs, torch):
         t1 = time.perf_counter()
         hf = tc.TCHartreeFock(cfg["nup"], cfg["ndown"], nb)
         try:
-            e = hf.compute(S, H, g1, Fs, Fo, V, jas, True, tol=a.tol, max_iter=200, mgs_mode=a.tc_mgs)
+            e = hf.compute(S, H, g1, Fs, Fo, V, jas, True, tol=a.tol, max_iter=200, mgs_mode=a.tc_mgs,
+                           diis=a.tc_diis)
             status = "ok"
         except tc.TcscfError as err:
             e, status = float("nan"), str(err)
@@ -197,7 +199,8 @@ def run_tc(a, tc, configs, torch):
         del Fs, Fo, V
     n1, n3 = g1.nGrid(), g3.nGrid()
     mix, f64 = out[a.precision], out["fp64"]
-    return {"workload": f"TC-HF Ne, {nb} STOs, b=1, r1=r2={n1}, r3={n3}, MGS {'metric' if a.tc_mgs else 'reference'}",
+    return {"workload": f"TC-HF Ne, {nb} STOs, b=1, r1=r2={n1}, r3={n3}, MGS {'metric' if a.tc_mgs else 'reference'}, "
+                        f"DIIS {a.tc_diis}",
             "pairs_x_components": 2 * n1 * (n1 - 1) + 3 * n1 * n3,
             a.precision: mix, "fp64": f64, "energy_dev_vs_fp64": mix["energy"] - f64["energy"],
             "speedup_vs_fp64": f64["total_ms"] / mix["total_ms"]}

@@ -80,6 +80,10 @@ typedef struct tcscf_scf_opts {
   int max_iter;   /* reference: 50 RCS, 100 UOS, 200 TC */
   int mask;       /* respectOneElectronSymmetry: zero G where H == 0 */
   int mgs_mode;   /* TC: 0 = reference MGS (Euclidean norm for i >= 1), 1 = metric */
+  int diis;       /* 0: the reference's plain Roothaan loop; k > 0: Pulay DIIS over the
+                     last k Fock operators (FP64, host), from the second iteration on.
+                     No reference counterpart (the benchmark workloads ask for it);
+                     TC needs mgs_mode = 1 with it. */
 } tcscf_scf_opts;
 
 typedef struct tcscf_scf_result {
@@ -105,6 +109,24 @@ int tcscf_lebedev(int order, double* theta_host, double* phi_host, double* w_hos
  * (src/integration/TreutlerAhlrichsLebedev.hpp:19-73, integrateAll.hpp:41-64);
  * point k = ia*nrad + ir, weight = w_r * w_ang * r^2.  Returns n or < 0. */
 int tcscf_product_grid(int nrad, int map, double a, double eps, int order, double* pts_host, double* w_host);
+
+/* QMCGrid(gridSize) (src/integration/integrateAll.hpp:21-65, QMCCache
+ * src/integration/qmcWrapper.hpp:95-141): nshifts randomly shifted copies of a
+ * rank-1 lattice (n = first prime >= min_points, Korobov generating vector
+ * chosen by the P_2 criterion; the reference takes mppmu/qmc's tables, whose
+ * exact points are not reproducible here), mapped by TreutlerAhlrichsM4(0, 0.9)
+ * x Azimuthal x Linear(2 pi) with weight *= r^2 sin(theta).  ndim 2 stores
+ * (r sin(theta), 0, r cos(theta)).  pts 3 x N SoA; returns N (or < 0). */
+int64_t tcscf_qmc_lattice_npts(int64_t min_points, int nshifts);
+int64_t tcscf_qmc_lattice_grid(int64_t min_points, int ndim, int nshifts, uint64_t seed, double* pts_host,
+                               double* w_host);
+/* No reference counterpart (the QMC workload's point set): Sobol points in
+ * [0, 1)^dim (dim <= 3, Joe-Kuo directions, Gray-code order), optionally
+ * scrambled (linear matrix scrambling + digital shift, seeded); u n x dim. */
+int tcscf_sobol_points(int64_t n, int dim, int scramble, uint64_t seed, double* u_host);
+/* Importance-sampled grid from n scrambled Sobol points: r = -ln(1 - u1)/2
+ * (density 2 e^{-2r}), uniform directions, w = 4 pi r^2 / (2 n e^{-2r}). */
+int tcscf_sobol_grid(int64_t n, uint64_t seed, double* pts_host, double* w_host);
 
 /* ---------------------------------------------------------------- host one-electron */
 /* overlap / coreMatrixElement: src/SlaterTypeOrbital.hpp:125-171.  nb x nb row-major. */

@@ -9,7 +9,8 @@ from ._lib import (BF16X3, BF16X5, FP32, FP64, K_COULOMB, K_JASTROW_GRAD, K_TC_S
 from .api import (Jastrow, QMCGrid, QuadratureGrid, RCSHartreeFock, SlaterTypeOrbital, TCHartreeFock, UOSHartreeFock,
                   coreMatrixElement, computeF, computeV, eri_full, eri_full_gradient, even_tempered,
                   integrateAllR1R2, integrateAllR1R2Gradient, lebedev, npair, one_electron_matrices, overlap,
-                  pair_index, product_grid, radial_grid, threeElectronIntegrals, unpack)
+                  pair_index, product_grid, qmc_lattice_grid, radial_grid, sobol_grid, sobol_points,
+                  threeElectronIntegrals, unpack)
 
 __all__ = [
     "BF16X3", "BF16X5", "FP32", "FP64", "K_COULOMB", "K_JASTROW_GRAD", "K_TC_SCALAR", "RADIAL_BECKE", "RADIAL_M4",
@@ -17,5 +18,5 @@ __all__ = [
     "QuadratureGrid", "RCSHartreeFock", "SlaterTypeOrbital", "TCHartreeFock", "UOSHartreeFock", "coreMatrixElement",
     "computeF", "computeV", "eri_full", "eri_full_gradient", "even_tempered", "integrateAllR1R2",
     "integrateAllR1R2Gradient", "lebedev", "npair", "one_electron_matrices", "overlap", "pair_index", "product_grid",
-    "radial_grid", "threeElectronIntegrals", "unpack",
+    "qmc_lattice_grid", "radial_grid", "sobol_grid", "sobol_points", "threeElectronIntegrals", "unpack",
 ]

@@ -23,6 +23,7 @@ EXPORTS = [
     "tcscf_last_error", "tcscf_version", "tcscf_radial_grid", "tcscf_lebedev_npts", "tcscf_lebedev",
     "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
     "tcscf_F_workspace_bytes", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
+    "tcscf_qmc_lattice_npts", "tcscf_qmc_lattice_grid", "tcscf_sobol_points", "tcscf_sobol_grid",
     "tcscf_selftest_mfma",
 ]
 
@@ -51,7 +52,7 @@ class Jastrow(ctypes.Structure):
 
 class ScfOpts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("mask", ctypes.c_int),
-                ("mgs_mode", ctypes.c_int)]
+                ("mgs_mode", ctypes.c_int), ("diis", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -80,6 +81,13 @@ def lib():
         L.tcscf_last_error.restype = ctypes.c_char_p
         L.tcscf_F_workspace_bytes.restype = ctypes.c_size_t
         L.tcscf_F_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        L.tcscf_qmc_lattice_npts.restype = ctypes.c_int64
+        L.tcscf_qmc_lattice_npts.argtypes = [ctypes.c_int64, ctypes.c_int]
+        L.tcscf_qmc_lattice_grid.restype = ctypes.c_int64
+        L.tcscf_qmc_lattice_grid.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                             ctypes.c_void_p, ctypes.c_void_p]
+        L.tcscf_sobol_points.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p]
+        L.tcscf_sobol_grid.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 

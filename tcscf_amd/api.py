@@ -147,6 +147,34 @@ def product_grid(nrad, order, radial_map=_lib.RADIAL_BECKE, a=1.0, eps=1.0):
     return pts, w
 
 
+def qmc_lattice_grid(gridSize, ndim=3, nshifts=2, seed=0):
+    """The QMCGrid(gridSize) point set (src/integration/integrateAll.hpp:21-65)
+    on the host: points (3, N) and weights (N,), N = nshifts * (first prime >=
+    gridSize)."""
+    n = check(lib().tcscf_qmc_lattice_npts(int(gridSize), int(nshifts)))
+    pts = np.zeros((3, n))
+    w = np.zeros(n)
+    check(lib().tcscf_qmc_lattice_grid(int(gridSize), int(ndim), int(nshifts), int(seed) & (2 ** 64 - 1),
+                                       pts.ctypes.data, w.ctypes.data))
+    return pts, w
+
+
+def sobol_points(n, dim=3, scramble=True, seed=0):
+    """Sobol points (n, dim) in [0, 1), Gray-code order."""
+    u = np.zeros((int(n), int(dim)))
+    check(lib().tcscf_sobol_points(int(n), int(dim), int(bool(scramble)), int(seed) & (2 ** 64 - 1), u.ctypes.data))
+    return u
+
+
+def sobol_grid(n, seed=0):
+    """Importance-sampled grid from n scrambled Sobol points (r density
+    2 e^{-2r}, uniform directions): points (3, n), weights (n,)."""
+    pts = np.zeros((3, int(n)))
+    w = np.zeros(int(n))
+    check(lib().tcscf_sobol_grid(int(n), int(seed) & (2 ** 64 - 1), pts.ctypes.data, w.ctypes.data))
+    return pts, w
+
+
 class QuadratureGrid:
     """points (3, N) and weights (N,), device-resident FP64."""
 
@@ -189,6 +217,18 @@ class QMCGrid:
     @classmethod
     def radial_lebedev(cls, nrad, order, radial_map=_lib.RADIAL_BECKE, a=1.0, eps=1.0):
         pts, w = product_grid(nrad, order, radial_map, a, eps)
+        return cls(pts, w, 3)
+
+    @classmethod
+    def lattice(cls, gridSize, ndim=3, nshifts=2, seed=0):
+        """QMCGrid(gridSize) (src/integration/integrateAll.hpp:21-65)."""
+        pts, w = qmc_lattice_grid(gridSize, ndim, nshifts, seed)
+        return cls(pts, w, ndim)
+
+    @classmethod
+    def sobol(cls, n, seed=0):
+        """Scrambled-Sobol grid, exponential radial importance sampling."""
+        pts, w = sobol_grid(n, seed)
         return cls(pts, w, 3)
 
     def nGrid(self):
@@ -343,8 +383,8 @@ def eri_full(E, nb):
 
 
 # --------------------------------------------------------------------------- SCF
-def _opts(tol, max_iter, mask, mgs_mode=0):
-    return _lib.ScfOpts(float(tol), int(max_iter), int(bool(mask)), int(mgs_mode))
+def _opts(tol, max_iter, mask, mgs_mode=0, diis=0):
+    return _lib.ScfOpts(float(tol), int(max_iter), int(bool(mask)), int(mgs_mode), int(diis))
 
 
 def _f_args(r1Grid, F):
@@ -386,7 +426,7 @@ class RCSHartreeFock(_HFBase):
         return False
 
     def compute(self, overlap, oneElectronTerms, r1Grid, Fji, respectOneElectronSymmetry=True, tol=1e-5,
-                max_iter=50):
+                max_iter=50, diis=0):
         nb = self.basisSize
         S, Sp = _np(overlap)
         H, Hp = _np(oneElectronTerms)
@@ -399,7 +439,7 @@ class RCSHartreeFock(_HFBase):
         ev = np.zeros(nb)
         rc = lib().tcscf_rcs_scf(self.nElectrons, nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                  _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()), fp, fd,
-                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry)), ctypes.byref(res),
+                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, 0, diis)), ctypes.byref(res),
                                  trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp), _stream())
         self.density[0] = D
         self.eigenvalues[:] = ev
@@ -424,7 +464,7 @@ class UOSHartreeFock(_HFBase):
         return False
 
     def compute(self, overlap, oneElectronTerms, r1Grid, Fji, respectOneElectronSymmetry=True, tol=1e-5,
-                max_iter=100):
+                max_iter=100, diis=0):
         nb = self.basisSize
         S, Sp = _np(overlap)
         H, Hp = _np(oneElectronTerms)
@@ -437,7 +477,7 @@ class UOSHartreeFock(_HFBase):
         ev = np.zeros((2, nb))
         rc = lib().tcscf_uos_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                  _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()), fp, fd,
-                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry)), ctypes.byref(res),
+                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, 0, diis)), ctypes.byref(res),
                                  trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp), _stream())
         self.density[:] = D
         self.eigenvalues[:] = ev
@@ -468,7 +508,7 @@ class TCHartreeFock(_HFBase):
         return True
 
     def compute(self, overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo, Vji, jastrow,
-                respectOneElectronSymmetry=True, tol=1e-5, max_iter=200, mgs_mode=0):
+                respectOneElectronSymmetry=True, tol=1e-5, max_iter=200, mgs_mode=0, diis=0):
         torch = _torch()
         nb = self.basisSize
         S, Sp = _np(overlap)
@@ -487,7 +527,7 @@ class TCHartreeFock(_HFBase):
         rc = lib().tcscf_tc_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                 _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(n1),
                                 _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), ctypes.byref(jp),
-                                ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, mgs_mode)),
+                                ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, mgs_mode, diis)),
                                 ctypes.byref(res), trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp),
                                 ps.ctypes.data_as(_dp), _stream())
         self.density[:] = D

@@ -31,3 +31,13 @@ def ne_config3():
     c = ne_config2()
     c["grid3"] = (50, 23, _lib.RADIAL_M4)
     return c
+
+
+def ne_qmc_config4():
+    """Ne, Z=10, TC on scrambled-Sobol grids: 40 STOs (s k=0..14, p k=0..4,
+    d k=0..1, zeta = 0.3 * 1.8**k); 2**16 points per electron coordinate (r1,
+    r2, r3 with seeds 1, 2, 3), r from the inverse CDF of 2 e^{-2r}, uniform
+    directions, importance weights; Jastrow b in {0.5, 1, 2}."""
+    basis = even_tempered({0: 15, 1: 5, 2: 2}, 0.3, 1.8)
+    assert len(basis) == 40
+    return dict(name="Ne", Z=10, nup=5, ndown=5, basis=basis, npts=2 ** 16, seeds=(1, 2, 3), b_values=(0.5, 1.0, 2.0))

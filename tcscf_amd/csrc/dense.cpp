@@ -408,4 +408,98 @@ void mgs(int k, int n, std::vector<cplx>& v, const double* S, bool euclid_after_
   }
 }
 
+void Diis::extrapolate(std::vector<double>& F, const std::vector<double>& e) {
+  f_.push_back(F);
+  e_.push_back(e);
+  while ((int)f_.size() > depth_) {
+    f_.pop_front();
+    e_.pop_front();
+  }
+  while (f_.size() >= 2) {
+    const int m = (int)f_.size(), n = m + 1;
+    // [B 1; 1^T 0] [c; lambda] = [0; 1], B_ij = <e_i, e_j>
+    std::vector<double> M((size_t)n * n, 0.0), x(n, 0.0);
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j <= i; ++j) {
+        double t = 0;
+        for (size_t q = 0; q < e_[i].size(); ++q) t += e_[i][q] * e_[j][q];
+        M[i * n + j] = M[j * n + i] = t;
+      }
+    for (int i = 0; i < m; ++i) M[i * n + m] = M[m * n + i] = 1.0;
+    x[m] = 1.0;
+    // Gaussian elimination with partial pivoting, relative pivot threshold
+    double scale = 0;
+    for (int i = 0; i < m; ++i) scale = std::max(scale, std::fabs(M[i * n + i]));
+    bool ok = true;
+    for (int c = 0; c < n && ok; ++c) {
+      int piv = c;
+      for (int r = c + 1; r < n; ++r)
+        if (std::fabs(M[r * n + c]) > std::fabs(M[piv * n + c])) piv = r;
+      if (!(std::fabs(M[piv * n + c]) > 1e-14 * std::max(scale, 1e-300))) {
+        ok = false;
+        break;
+      }
+      if (piv != c) {
+        for (int k = 0; k < n; ++k) std::swap(M[c * n + k], M[piv * n + k]);
+        std::swap(x[c], x[piv]);
+      }
+      for (int r = c + 1; r < n; ++r) {
+        const double f = M[r * n + c] / M[c * n + c];
+        for (int k = c; k < n; ++k) M[r * n + k] -= f * M[c * n + k];
+        x[r] -= f * x[c];
+      }
+    }
+    if (ok) {
+      for (int c = n - 1; c >= 0; --c) {
+        double t = x[c];
+        for (int k = c + 1; k < n; ++k) t -= M[c * n + k] * x[k];
+        x[c] = t / M[c * n + c];
+      }
+      std::fill(F.begin(), F.end(), 0.0);
+      for (int i = 0; i < m; ++i)
+        for (size_t q = 0; q < F.size(); ++q) F[q] += x[i] * f_[i][q];
+      return;
+    }
+    f_.pop_front();  // singular subspace: drop the oldest pair and retry
+    e_.pop_front();
+  }
+}
+
+namespace {
+// C = A B for n x n row-major
+void matmul(int n, const double* A, const double* B, double* C) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double t = 0;
+      for (int k = 0; k < n; ++k) t += A[i * n + k] * B[k * n + j];
+      C[i * n + j] = t;
+    }
+}
+}  // namespace
+
+void commutator_error(int nspin, int nb, const double* F, const double* D, const double* S, std::vector<double>& e) {
+  const int64_t nb2 = (int64_t)nb * nb;
+  e.assign(nspin * nb2, 0.0);
+  std::vector<double> FD(nb2), FDS(nb2), SD(nb2), SDF(nb2);
+  for (int s = 0; s < nspin; ++s) {
+    matmul(nb, F + s * nb2, D + s * nb2, FD.data());
+    matmul(nb, FD.data(), S, FDS.data());
+    matmul(nb, S, D + s * nb2, SD.data());
+    matmul(nb, SD.data(), F + s * nb2, SDF.data());
+    for (int64_t q = 0; q < nb2; ++q) e[s * nb2 + q] = FDS[q] - SDF[q];
+  }
+}
+
+void right_eigen_error(int nspin, int nb, const double* F, const double* D, const double* S, std::vector<double>& e) {
+  const int64_t nb2 = (int64_t)nb * nb;
+  e.assign(nspin * nb2, 0.0);
+  std::vector<double> FD(nb2), SD(nb2), SDFD(nb2);
+  for (int s = 0; s < nspin; ++s) {
+    matmul(nb, F + s * nb2, D + s * nb2, FD.data());
+    matmul(nb, S, D + s * nb2, SD.data());
+    matmul(nb, SD.data(), FD.data(), SDFD.data());
+    for (int64_t q = 0; q < nb2; ++q) e[s * nb2 + q] = FD[q] - SDFD[q];
+  }
+}
+
 }  // namespace tc::dense

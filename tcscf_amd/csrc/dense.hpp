@@ -2,6 +2,7 @@
 #pragma once
 
 #include <complex>
+#include <deque>
 #include <vector>
 
 namespace tc::dense {
@@ -24,5 +25,28 @@ bool gen_nonsym_eig(int n, const double* F, const double* S, std::vector<cplx>& 
 // euclid_after_first (the reference's behaviour), else the metric norm.
 // vecs: k x n row-major.
 void mgs(int k, int n, std::vector<cplx>& vecs, const double* S, bool euclid_after_first);
+
+// Pulay's direct inversion in the iterative subspace (Pulay, Chem. Phys. Lett.
+// 73, 393 (1980); J. Comput. Chem. 3, 556 (1982)): keeps the last `depth`
+// (Fock, error) pairs and replaces F by sum_i c_i F_i with c minimising
+// |sum_i c_i e_i| subject to sum_i c_i = 1.  Vectors are flat (all spins).
+class Diis {
+ public:
+  explicit Diis(int depth) : depth_(depth) {}
+  // Adds (F, e) and overwrites F with the extrapolation (F is kept as is while
+  // fewer than two pairs are stored or the subspace system is singular).
+  void extrapolate(std::vector<double>& F, const std::vector<double>& e);
+
+ private:
+  int depth_;
+  std::deque<std::vector<double>> f_, e_;
+};
+
+// DIIS error of a Hermitian Fock operator, per spin: F D S - S D F.
+void commutator_error(int nspin, int nb, const double* F, const double* D, const double* S, std::vector<double>& e);
+// DIIS error of a non-Hermitian (TC) Fock operator, per spin: F D - S D F D,
+// which vanishes when the occupied S-orthonormal orbitals are right
+// eigenvectors of F.
+void right_eigen_error(int nspin, int nb, const double* F, const double* D, const double* S, std::vector<double>& e);
 
 }  // namespace tc::dense

@@ -3,6 +3,7 @@
 // Grids are built once on the host in FP64 and uploaded, so every consumer
 // (device kernels, the CPU reference path) integrates over identical points.
 #include <cmath>
+#include <cstdint>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -47,6 +48,119 @@ const LebedevEntry* lebedev_entry(int order) {
   for (int k = 0; k < kLebedevTableSize; ++k)
     if (kLebedevTable[k].order == order) return &kLebedevTable[k];
   return nullptr;
+}
+
+
+// ------------------------------------------------------------ QMC point sets
+// SplitMix64 (Steele, Lea, Flood 2014): seeds the lattice shifts, the
+// generating-vector search and the Sobol scrambling.
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }  // [0, 1)
+};
+
+bool is_prime(int64_t n) {
+  if (n < 2) return false;
+  for (int64_t d = 2; d * d <= n; ++d)
+    if (n % d == 0) return false;
+  return true;
+}
+
+int64_t next_prime(int64_t n) {
+  while (!is_prime(n)) ++n;
+  return n;
+}
+
+// Korobov rank-1 lattice z = (1, a, a^2 mod n): a minimises the worst-case
+// error P_2 = -1 + (1/n) sum_k prod_j (1 + 2 pi^2 B_2({k z_j / n})), B_2(x) =
+// x^2 - x + 1/6 (Sloan & Joe, Lattice Methods for Multiple Integration, 1994),
+// over a seeded sample of candidates.
+std::vector<int64_t> korobov_vector(int64_t n, int dim, uint64_t seed) {
+  SplitMix64 rng{seed ^ 0x6C61747469636521ull};
+  const int ncand = n > 8 ? 256 : 1;
+  double best = 1e300;
+  std::vector<int64_t> z(dim, 1), zb(dim, 1);
+  for (int c = 0; c < ncand; ++c) {
+    const int64_t a = n > 3 ? 2 + (int64_t)(rng.next() % (uint64_t)(n - 3)) : 1;
+    for (int j = 1; j < dim; ++j) z[j] = (int64_t)((__int128)z[j - 1] * a % n);
+    double sum = 0;
+    for (int64_t k = 0; k < n; ++k) {
+      double prod = 1;
+      for (int j = 0; j < dim; ++j) {
+        const double x = (double)((__int128)k * z[j] % n) / (double)n;
+        prod *= 1 + 2 * kPi * kPi * (x * x - x + 1.0 / 6.0);
+      }
+      sum += prod;
+    }
+    const double p2 = sum / (double)n - 1;
+    if (p2 < best) {
+      best = p2;
+      zb = z;
+    }
+  }
+  return zb;
+}
+
+// Sobol direction numbers for dimensions 1-3 (Joe & Kuo 2008, new-joe-kuo
+// table: dim 2 s=1 a=0 m=(1); dim 3 s=2 a=1 m=(1,3)), 32-bit, MSB first.
+void sobol_directions(int dim, uint32_t v[32]) {
+  if (dim == 0) {
+    for (int k = 0; k < 32; ++k) v[k] = 1u << (31 - k);
+    return;
+  }
+  const int s = dim == 1 ? 1 : 2;
+  const unsigned a = dim == 1 ? 0 : 1;
+  uint32_t m[33];
+  m[1] = 1;
+  if (s == 2) m[2] = 3;
+  for (int k = s + 1; k <= 32; ++k) {
+    uint32_t x = m[k - s] ^ (m[k - s] << s);
+    for (int i = 1; i < s; ++i)
+      if ((a >> (s - 1 - i)) & 1u) x ^= m[k - i] << i;
+    m[k] = x;
+  }
+  for (int k = 1; k <= 32; ++k) v[k - 1] = m[k] << (32 - k);
+}
+
+// Sobol points in Gray-code order, x_i = XOR of v_k over the bits of i ^ (i >> 1);
+// scrambled: random lower-triangular (unit diagonal) linear matrix scrambling
+// of the digits plus a random digital shift (Matousek 1998), per dimension.
+void sobol_points(int64_t n, int dim, int scramble, uint64_t seed, double* u) {
+  for (int d = 0; d < dim; ++d) {
+    uint32_t v[32];
+    sobol_directions(d, v);
+    uint32_t shift = 0;
+    if (scramble) {
+      SplitMix64 rng{seed * 0x9E3779B97F4A7C15ull + 0x736F626F6Cull + (uint64_t)d};
+      uint32_t L[32];  // row r: bits c <= r (bit 31 - c), diagonal set
+      for (int r = 0; r < 32; ++r) {
+        uint32_t row = 1u << (31 - r);
+        for (int c = 0; c < r; ++c)
+          if (rng.next() & 1ull) row |= 1u << (31 - c);
+        L[r] = row;
+      }
+      for (int k = 0; k < 32; ++k) {
+        uint32_t out = 0;
+        for (int r = 0; r < 32; ++r)
+          if (__builtin_popcount(L[r] & v[k]) & 1) out |= 1u << (31 - r);
+        v[k] = out;
+      }
+      shift = (uint32_t)(rng.next() >> 32);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      const uint64_t g = (uint64_t)i ^ ((uint64_t)i >> 1);
+      uint32_t x = shift;
+      for (int k = 0; k < 32 && (g >> k); ++k)
+        if ((g >> k) & 1ull) x ^= v[k];
+      u[i * dim + d] = (double)x * 0x1.0p-32;
+    }
+  }
 }
 
 double fact(int n) { return std::tgamma(n + 1.0); }
@@ -124,6 +238,83 @@ int tcscf_product_grid(int nrad, int map, double a, double eps, int order, doubl
     }
   }
   return (int)n;
+}
+
+
+// QMCGrid(gridSize) (src/integration/integrateAll.hpp:21-65) point count:
+// nshifts copies of a rank-1 lattice of n = the first prime >= min_points.
+int64_t tcscf_qmc_lattice_npts(int64_t min_points, int nshifts) {
+  if (min_points < 1 || nshifts < 1) return TCSCF_EINVAL;
+  return (int64_t)nshifts * next_prime(min_points);
+}
+
+// The QMCGrid(gridSize) construction: lattice points x in [0, 1)^ndim (shift
+// s, point i: frac(i z / n + Delta_s), shift-major), mapped by
+// TreutlerAhlrichsM4(a = 0, eps = 0.9) for r, Azimuthal (theta = acos(1 - 2x),
+// dtheta/dx = 1/sqrt(x(1-x))) and Linear(2 pi) for phi
+// (src/integration/changeOfVariables.hpp:46-94, 126-162), weight
+// = jacobian / (nshifts n) * r^2 sin(theta).  ndim = 2 drops phi and stores
+// (r sin(theta), 0, r cos(theta)), as the reference's 2-D grid does.
+// pts: 3 x N SoA, w: N.  Returns N.
+int64_t tcscf_qmc_lattice_grid(int64_t min_points, int ndim, int nshifts, uint64_t seed, double* pts, double* w) {
+  TC_CHECK(min_points >= 1 && nshifts >= 1 && (ndim == 2 || ndim == 3) && pts && w,
+           "tcscf_qmc_lattice_grid: bad arguments");
+  const int64_t n = next_prime(min_points), N = n * nshifts;
+  const std::vector<int64_t> z = korobov_vector(n, ndim, seed);
+  SplitMix64 rng{seed ^ 0x7368696674733131ull};
+  const double c = 0.9 / kLn2, alpha = 0.6;
+  for (int sh = 0; sh < nshifts; ++sh) {
+    double delta[3];
+    for (int j = 0; j < ndim; ++j) delta[j] = rng.uniform();
+    for (int64_t i = 0; i < n; ++i) {
+      double x[3] = {0, 0, 0};
+      for (int j = 0; j < ndim; ++j) {
+        const double t = (double)((__int128)i * z[j] % n) / (double)n + delta[j];
+        x[j] = t - std::floor(t);
+      }
+      const double lg = std::log(1.0 / (1 - x[0]));
+      const double pm1 = std::pow(x[0], alpha - 1);
+      const double r = c * std::pow(x[0], alpha) * lg;
+      const double jr = c * (alpha * pm1 * lg + pm1 * x[0] / (1 - x[0]));
+      const double th = std::acos(1 - 2 * x[1]);
+      const double jt = 1 / std::sqrt(x[1] * (1 - x[1]));
+      const double ph = ndim == 3 ? 2 * kPi * x[2] : 0.0;
+      const double jp = ndim == 3 ? 2 * kPi : 1.0;
+      const int64_t k = (int64_t)sh * n + i;
+      const double st = std::sin(th);
+      pts[k] = r * st * std::cos(ph);
+      pts[N + k] = ndim == 3 ? r * st * std::sin(ph) : 0.0;
+      pts[2 * N + k] = r * std::cos(th);
+      w[k] = jr * jt * jp / (double)N * r * r * st;
+    }
+  }
+  return N;
+}
+
+// Raw Sobol points (dim <= 3, n <= 2^32) in [0, 1), Gray-code order, u: n x dim.
+int tcscf_sobol_points(int64_t n, int dim, int scramble, uint64_t seed, double* u) {
+  TC_CHECK(n >= 1 && n <= (1ll << 32) && dim >= 1 && dim <= 3 && u, "tcscf_sobol_points: bad arguments");
+  sobol_points(n, dim, scramble, seed, u);
+  return 0;
+}
+
+// Importance-sampled 3-D grid from n scrambled Sobol points: r = -ln(1 - u1)/2
+// (density 2 e^{-2r}), theta = acos(1 - 2 u2), phi = 2 pi u3 (uniform on the
+// sphere), w = 4 pi r^2 / (n 2 e^{-2r}).  pts: 3 x n SoA.
+int tcscf_sobol_grid(int64_t n, uint64_t seed, double* pts, double* w) {
+  TC_CHECK(n >= 1 && n <= (1ll << 32) && pts && w, "tcscf_sobol_grid: bad arguments");
+  std::vector<double> u((size_t)n * 3);
+  sobol_points(n, 3, 1, seed, u.data());
+  for (int64_t i = 0; i < n; ++i) {
+    const double r = -std::log1p(-u[3 * i]) / 2;
+    const double ct = 1 - 2 * u[3 * i + 1], st = std::sqrt(std::max(0.0, 1 - ct * ct));
+    const double ph = 2 * kPi * u[3 * i + 2];
+    pts[i] = r * st * std::cos(ph);
+    pts[n + i] = r * st * std::sin(ph);
+    pts[2 * n + i] = r * ct;
+    w[i] = 4 * kPi * r * r / ((double)n * 2 * std::exp(-2 * r));
+  }
+  return 0;
 }
 
 // S_ab = N_a N_b (n_a+n_b)!/(z_a+z_b)^(n_a+n_b+1) delta_{l l'} delta_{m m'};

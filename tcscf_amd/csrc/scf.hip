@@ -259,7 +259,9 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
   Sweep sw{nspin, nb, n1, (int64_t)nb * (nb + 1) / 2, F, f_dtype == TCSCF_F32, A1, w1, st};
   int rc = sw.init();
   if (rc) return rc;
-  std::vector<double> D(nspin * nb2, 0.0), G(nspin * nb2), Fock(nspin * nb2), evals(nspin * nb, 0.0), C;
+  TC_CHECK(o->diis >= 0, "SCF: diis depth must be >= 0");
+  std::vector<double> D(nspin * nb2, 0.0), G(nspin * nb2), Fock(nspin * nb2), evals(nspin * nb, 0.0), C, err;
+  dense::Diis diis(o->diis);
   double energy = std::numeric_limits<double>::max();
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
@@ -292,6 +294,10 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
       break;
     }
     energy = e;
+    if (o->diis > 0 && it > 0) {  // iteration 0 has D = 0, whose error is 0 trivially
+      dense::commutator_error(nspin, nb, Fock.data(), D.data(), S, err);
+      diis.extrapolate(Fock, err);
+    }
     for (int s = 0; s < nspin; ++s) {
       std::vector<double> ev;
       if (!dense::gen_sym_eig(nb, Fock.data() + s * nb2, S, ev, C)) {

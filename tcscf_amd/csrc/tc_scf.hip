@@ -6,13 +6,13 @@
 // V_oppo = c_oppo V); basis values A[n][k] and gradients dA[n][k][xyz].
 //
 // Every term of the reference's G2 / G3 assembly (:1157-1224) is one of
-//   (a) a rank-one product L_j R_i of per-point vectors      -> "rows"
+//   (a) a rank-one product L_j R_i of per-point vectors      -> "vectors"
 //   (b) a per-point 3-vector h dotted into V(n, j, i)        -> "V-linear"
 // Pass 1 (tc_point_kernel, one wave per point) forms every per-point
-// intermediate of the reference (:1118-1148) and writes the rows and the h
-// vectors; pass 2 (tc_rank_kernel) sums L^T R over points and rows per target
-// matrix; pass 3 (tc_vlin_kernel) sums h . V over points.  Partials per
-// workgroup are reduced in a fixed order.
+// intermediate of the reference (:1118-1148) and writes the compact per-point
+// vectors and the h vectors; pass 2 (tc_rank_kernel, FP64 MFMA) sums the
+// rank-one terms over points per target matrix; pass 3 (tc_vlin_kernel) sums
+// h . V over points.  Partials per workgroup are reduced in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,19 +29,23 @@
 namespace tc {
 namespace tcs {
 
-constexpr int kBlocks = 512;
-constexpr int kMaxOcc = 8;  // occupied orbitals per spin handled by the point kernel
+typedef double d4 __attribute__((ext_vector_type(4)));
 
-// Row layout per (point, spin): 2 vectors (L, R) of nb doubles each.
-//   0      G2 : (A, w (faa A - FaiS))
-//   1      VEC: (A, w (VT . dA - c_s Vai1))
-//   2..4   VEC: (-w VTSI_c, dA_c)                  (Vja == VTSI for symmetric V)
-//   5      G3 : (A, w (|VT|^2 - sNS - sNO_other) A + Z)
-//   6      G3 : (Z, A),  Z = 2 w (SVbiVT - VTSI . VT)
-//   7..9   G3 : (2 w VTSI_c, VTSI_c)
-//   10..   G3 : (alpha W_bc, W_bc), b < nocc_s, alpha = -2 w (occ_s c_s^2 + occ_o c_o^2)
-constexpr int kFixedRows = 10;
-__host__ __device__ inline int target_of_row(int r) { return r == 0 ? 0 : (r < 5 ? 1 : 2); }
+constexpr int kBlocks = 512;
+constexpr int kWChunk = 4;  // occupied orbitals per register pass of W = V C
+
+// Compact per-point output, per spin s, vectors of nb doubles (lane i = basis
+// index i):
+//   0        u0 = w (faa A - FaiS)                      G2 : A (x) u0
+//   1        u1 = w (VT . dA - c_s Vai1)                VEC: A (x) u1
+//   2        u2 = w (|VT|^2 - sNS - sNO_other) A + Z    G3 : A (x) u2
+//   3        u3 = Z = 2 w (SVbiVT - VTSI . VT)          G3 : u3 (x) A
+//   4..6     t_c = VTSI_c                               VEC: (-w t_c) (x) dA_c
+//                                                       G3 : (2 w t_c) (x) t_c
+//   7+3b+c   W_bc = sum_j C_s[j][b] V_c(j, .)           G3 : (alpha W_bc) (x) W_bc
+// with alpha_s = -2 w (occ_s c_s^2 + occ_o c_o^2) per (point, spin), and
+// x (x) y the rank-one matrix x_j y_i.  (Vja == VTSI for symmetric V.)
+constexpr int kVecFixed = 7;
 
 struct PointArgs {
   const double* Fs;   // [n1][npair]
@@ -54,8 +58,9 @@ struct PointArgs {
   const double* C;    // [2][nb][nmax]
   double c_same, c_oppo;
   int64_t n1, npair;
-  int nb, nmax, nocc0, nocc1, nrow;
-  double* rows;       // [n1][2][nrow][2][nb]
+  int nb, nmax, nocc0, nocc1, nvec;
+  double* vecs;       // [n1][2][nvec][nb]
+  double* alpha;      // [n1][2]
   double* hvec;       // [n1][2 spins][2 targets (VEC, G3)][3]
 };
 
@@ -65,77 +70,93 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// A wave-uniform value (e.g. a wave_sum result) moved to scalar registers.
+__device__ inline double uniform(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Pass 1: one wave per r1 point, lane i = basis index (nb <= 64).
-__global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
+__host__ __device__ inline int64_t point_wave_doubles(int nb, int nmax) { return 24 * (int64_t)nb + 6 * (int64_t)nb * nmax; }
+
+// Sum of v over the lanes that share a basis index i (lane = jp * nbp + i).
+__device__ inline double part_sum(double v, int nbp) {
+  for (int off = nbp; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Pass 1: one wave per r1 point.  Lane = jp * nbp + i: i is the basis index
+// (nbp = 16, 32 or 64 >= nb) and the 64 / nbp lane groups jp split every loop
+// over a second basis index, so each lane walks nb * nbp / 64 terms.  F and V
+// are read from global memory (each row once from HBM; the symmetric (j, i) /
+// (i, j) re-reads hit the caches), D and C are shared in LDS, the per-point
+// vectors live in a small per-wave LDS area.
+__global__ __launch_bounds__(256, 2) void tc_point_kernel(PointArgs a) {
   extern __shared__ double smem[];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int nb = a.nb, nmax = a.nmax;
+  const int nbp = nb <= 16 ? 16 : (nb <= 32 ? 32 : 64), parts = 64 / nbp;
   const int64_t np = a.npair, nb2 = (int64_t)nb * nb;
-  // workgroup-shared: D [2][nb][nb], C [2][nb][nmax]
-  double* Ds = smem;
-  double* Cs = Ds + 2 * nb2;
-  // per wave: f_s, f_o [np], v [3][np], A [nb], dA [nb][3], X [2][nb], kS [2][nb][3],
-  //           VTSI [2][nb][3], E [2][nb][3], W [2][nb][nmax][3], U scratch
-  const int64_t per_wave = 5 * np + nb + 3 * nb + 2 * nb + 6 * nb + 6 * nb + 6 * nb + 6 * (int64_t)nb * nmax;
-  double* base = Cs + 2 * (int64_t)nb * nmax + wave * per_wave;
-  double* fs = base;
-  double* fo = fs + np;
-  double* vv = fo + np;  // [3][np]
-  double* As = vv + 3 * np;
-  double* dAs = As + nb;
-  double* Xs = dAs + 3 * nb;
-  double* kS = Xs + 2 * nb;
-  double* VT = kS + 6 * nb;  // VTSI [2][nb][3]
-  double* Es = VT + 6 * nb;
-  double* Ws = Es + 6 * nb;  // [2][nb][nmax][3]
-  for (int64_t e = tid; e < 2 * nb2; e += 256) Ds[e] = a.D[e];
-  for (int64_t e = tid; e < 2 * (int64_t)nb * nmax; e += 256) Cs[e] = a.C[e];
+  double* Ds = smem;                       // [2][nb][nb]
+  double* Cs = Ds + 2 * nb2;               // [2][nb][nmax]
+  double* base = Cs + 2 * (int64_t)nb * nmax + wave * point_wave_doubles(nb, nmax);
+  double* As = base;                       // [nb]
+  double* dAs = As + nb;                   // [nb][3]
+  double* Xs = dAs + 3 * nb;               // [2][nb]
+  double* kS = Xs + 2 * nb;                // [2][nb][3]
+  double* VT = kS + 6 * nb;                // VTSI [2][nb][3]
+  double* Es = VT + 6 * nb;                // [2][nb][3]
+  double* Ws = Es + 6 * nb;                // [2][nb][nmax][3]
+  for (int64_t e = tid; e < 2 * nb2; e += blockDim.x) Ds[e] = a.D[e];
+  for (int64_t e = tid; e < 2 * (int64_t)nb * nmax; e += blockDim.x) Cs[e] = a.C[e];
   __syncthreads();
 
-  const int i = lane;
+  const int i = lane & (nbp - 1), jp = lane / nbp;
   const bool act = i < nb;
+  const bool own = act && jp == 0;  // the lane that stores / counts basis index i
   const int tri_i = i * (i + 1) / 2;
   const double cs = a.c_same, co = a.c_oppo;
   const int nocc[2] = {a.nocc0, a.nocc1};
-  const int64_t per = (a.n1 + gridDim.x - 1) / gridDim.x;
-  const int64_t p0 = (int64_t)blockIdx.x * per;
-  const int64_t p1 = p0 + per < a.n1 ? p0 + per : a.n1;
+  const int64_t nvw = (int64_t)gridDim.x * nwaves;
 
-  for (int64_t n = p0 + wave; n < p1; n += 4) {
-    for (int64_t p = lane; p < np; p += 64) {
-      fs[p] = a.Fs[n * np + p];
-      fo[p] = a.Fo[n * np + p];
-      vv[p] = a.V[(n * 3 + 0) * np + p];
-      vv[np + p] = a.V[(n * 3 + 1) * np + p];
-      vv[2 * np + p] = a.V[(n * 3 + 2) * np + p];
-    }
-    if (act) {
+  for (int64_t n = (int64_t)blockIdx.x * nwaves + wave; n < a.n1; n += nvw) {
+    const double* __restrict__ fs = a.Fs + n * np;
+    const double* __restrict__ fo = a.Fo + n * np;
+    const double* __restrict__ vv = a.V + n * 3 * np;
+    if (own) {
       As[i] = a.A[n * nb + i];
       for (int c = 0; c < 3; ++c) dAs[i * 3 + c] = a.dA[(n * nb + i) * 3 + c];
     }
     wave_sync_lds();
     // X_s[i] = sum_k D_s[k][i] A[k];  kS_s[i] = sum_k D_s[k][i] dA[k]
-    if (act)
-      for (int s = 0; s < 2; ++s) {
-        double x = 0, k0 = 0, k1 = 0, k2 = 0;
-        for (int k = 0; k < nb; ++k) {
+    for (int s = 0; s < 2; ++s) {
+      double x = 0, k0 = 0, k1 = 0, k2 = 0;
+      if (act)
+        for (int k = jp; k < nb; k += parts) {
           const double d = Ds[s * nb2 + k * nb + i];
           x += d * As[k];
           k0 += d * dAs[k * 3];
           k1 += d * dAs[k * 3 + 1];
           k2 += d * dAs[k * 3 + 2];
         }
+      x = part_sum(x, nbp);
+      k0 = part_sum(k0, nbp);
+      k1 = part_sum(k1, nbp);
+      k2 = part_sum(k2, nbp);
+      if (own) {
         Xs[s * nb + i] = x;
         kS[(s * nb + i) * 3] = k0;
         kS[(s * nb + i) * 3 + 1] = k1;
         kS[(s * nb + i) * 3 + 2] = k2;
       }
+    }
     wave_sync_lds();
     const double Ai = act ? As[i] : 0.0;
     double dAi[3] = {0, 0, 0};
@@ -143,20 +164,15 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
       for (int c = 0; c < 3; ++c) dAi[c] = dAs[i * 3 + c];
     double occ[2], sg[2][3];
     for (int s = 0; s < 2; ++s) {
-      const double x = act ? Xs[s * nb + i] : 0.0;
-      occ[s] = wave_sum(x * Ai);  // calculateOccupiedElectronDensity
-      for (int c = 0; c < 3; ++c) sg[s][c] = wave_sum(x * dAi[c]);  // calculateScaledGradients
+      const double x = own ? Xs[s * nb + i] : 0.0;
+      occ[s] = uniform(wave_sum(x * Ai));  // calculateOccupiedElectronDensity
+      for (int c = 0; c < 3; ++c) sg[s][c] = uniform(wave_sum(x * dAi[c]));  // calculateScaledGradients
     }
-    // one pass over j: Faa, FaiS, V . D, Vai, Q (= VTSI / c_s), W (registers)
+    // one pass over j: Faa, FaiS, V . D, Vai, Q (= VTSI / c_s)
     double faa_p[2] = {0, 0}, fai[2] = {0, 0}, vd[2][3] = {{0, 0, 0}, {0, 0, 0}}, vai1[2] = {0, 0};
     double q[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    double wreg[2][kMaxOcc][3];
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-      for (int b = 0; b < kMaxOcc; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
     if (act) {
-      for (int j = 0; j < nb; ++j) {
+      for (int j = jp; j < nb; j += parts) {
         const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
         const double f_s = fs[pj], f_o = fo[pj];
         const double v0 = vv[pj], v1 = vv[np + pj], v2 = vv[2 * np + pj];
@@ -174,60 +190,49 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
           q[s][0] += xj * v0;
           q[s][1] += xj * v1;
           q[s][2] += xj * v2;
-          const double* crow = Cs + ((int64_t)s * nb + j) * nmax;
-#pragma unroll
-          for (int b = 0; b < kMaxOcc; ++b)
-            if (b < nocc[s]) {
-              const double cb = crow[b];
-              wreg[s][b][0] += cb * v0;
-              wreg[s][b][1] += cb * v1;
-              wreg[s][b][2] += cb * v2;
-            }
         }
       }
-#pragma unroll
-      for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int b = 0; b < kMaxOcc; ++b)
-          if (b < nocc[s]) {
-            double* wrow = Ws + (((int64_t)(s * nb + i) * nmax) + b) * 3;
-            wrow[0] = wreg[s][b][0];
-            wrow[1] = wreg[s][b][1];
-            wrow[2] = wreg[s][b][2];
-          }
     }
     double faa[2], VTv[2][3];
     double vdsum[2][3];
     for (int s = 0; s < 2; ++s) {
-      faa[s] = wave_sum(faa_p[s]);  // FaaSamePlusOppo
-      for (int c = 0; c < 3; ++c) vdsum[s][c] = wave_sum(vd[s][c]);
+      faa[s] = uniform(wave_sum(faa_p[s]));  // FaaSamePlusOppo
+      for (int c = 0; c < 3; ++c) vdsum[s][c] = uniform(wave_sum(vd[s][c]));
+      fai[s] = part_sum(fai[s], nbp);
+      vai1[s] = part_sum(vai1[s], nbp);
+      for (int c = 0; c < 3; ++c) q[s][c] = part_sum(q[s][c], nbp);
     }
     for (int s = 0; s < 2; ++s)
       for (int c = 0; c < 3; ++c) VTv[s][c] = cs * vdsum[s][c] + co * vdsum[1 - s][c];  // VaaSamePlusOppo == VTilde
-    if (act)
+    if (own)
       for (int s = 0; s < 2; ++s)
         for (int c = 0; c < 3; ++c) VT[(s * nb + i) * 3 + c] = cs * q[s][c];  // VTildeSubI (== Vja)
     wave_sync_lds();
     // E_s[k] = sum_l D_s[l][k] VTSI_s[l]
-    if (act)
-      for (int s = 0; s < 2; ++s) {
-        double e0 = 0, e1 = 0, e2 = 0;
-        for (int l = 0; l < nb; ++l) {
+    for (int s = 0; s < 2; ++s) {
+      double e0 = 0, e1 = 0, e2 = 0;
+      if (act)
+        for (int l = jp; l < nb; l += parts) {
           const double d = Ds[s * nb2 + l * nb + i];
           const double* t = VT + (s * nb + l) * 3;
           e0 += d * t[0];
           e1 += d * t[1];
           e2 += d * t[2];
         }
+      e0 = part_sum(e0, nbp);
+      e1 = part_sum(e1, nbp);
+      e2 = part_sum(e2, nbp);
+      if (own) {
         Es[(s * nb + i) * 3] = e0;
         Es[(s * nb + i) * 3 + 1] = e1;
         Es[(s * nb + i) * 3 + 2] = e2;
       }
+    }
     wave_sync_lds();
     // SumOverb_Vbi_dot_VTildeb_s[i] = c_s sum_k V(k,i) . E_s[k]
     double sv1[2] = {0, 0};
     if (act)
-      for (int k = 0; k < nb; ++k) {
+      for (int k = jp; k < nb; k += parts) {
         const int64_t pk = k <= i ? tri_i + k : k * (k + 1) / 2 + i;
         const double v0 = vv[pk], v1 = vv[np + pk], v2 = vv[2 * np + pk];
         for (int s = 0; s < 2; ++s) {
@@ -235,10 +240,53 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
           sv1[s] += v0 * e[0] + v1 * e[1] + v2 * e[2];
         }
       }
+    sv1[0] = part_sum(sv1[0], nbp);
+    sv1[1] = part_sum(sv1[1], nbp);
+    // W_s[i][b][c] = sum_j C_s[j][b] V_c(j, i), kWChunk orbitals per pass over j
+    double* vout = a.vecs + (int64_t)n * 2 * a.nvec * nb;
+    for (int b0 = 0; b0 < nmax; b0 += kWChunk) {
+      double wreg[2][kWChunk][3];
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int b = 0; b < kWChunk; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
+      if (act) {
+        for (int j = jp; j < nb; j += parts) {
+          const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
+          const double v0 = vv[pj], v1 = vv[np + pj], v2 = vv[2 * np + pj];
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            const double* crow = Cs + ((int64_t)s * nb + j) * nmax + b0;
+#pragma unroll
+            for (int b = 0; b < kWChunk; ++b)
+              if (b0 + b < nocc[s]) {
+                const double cb = crow[b];
+                wreg[s][b][0] += cb * v0;
+                wreg[s][b][1] += cb * v1;
+                wreg[s][b][2] += cb * v2;
+              }
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int b = 0; b < kWChunk; ++b)
+          if (b0 + b < nmax)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const double wv = part_sum(b0 + b < nocc[s] ? wreg[s][b][c] : 0.0, nbp);
+              if (own) {
+                Ws[(((int64_t)(s * nb + i) * nmax) + b0 + b) * 3 + c] = wv;
+                vout[((int64_t)s * a.nvec + kVecFixed + 3 * (b0 + b) + c) * nb + i] = wv;
+              }
+            }
+    }
+    wave_sync_lds();
     double scvt[2][3], nrmU[2];
     for (int s = 0; s < 2; ++s) {
-      const double x = act ? Xs[s * nb + i] : 0.0;
-      for (int c = 0; c < 3; ++c) scvt[s][c] = wave_sum(act ? x * VT[(s * nb + i) * 3 + c] : 0.0);  // SumOverb_conjVTildeb_occb
+      const double x = own ? Xs[s * nb + i] : 0.0;
+      for (int c = 0; c < 3; ++c) scvt[s][c] = uniform(wave_sum(own ? x * VT[(s * nb + i) * 3 + c] : 0.0));  // SumOverb_conjVTildeb_occb
       // sum_ab |U_s[a][b]|^2, U_s[a][b][c] = sum_l C_s[l][a] W_s[l][b][c]  (calculateVab)
       double acc = 0;
       const int nu = nocc[s] * nocc[s] * 3;
@@ -248,20 +296,12 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
         for (int l = 0; l < nb; ++l) u += Cs[((int64_t)s * nb + l) * nmax + aa] * Ws[(((int64_t)s * nb + l) * nmax + bb) * 3 + c];
         acc += u * u;
       }
-      nrmU[s] = wave_sum(acc);
+      nrmU[s] = uniform(wave_sum(acc));
     }
-    // rows and h vectors
     const double wn = a.w[n];
-    double* rbase = a.rows + (int64_t)n * 2 * a.nrow * 2 * nb;
     for (int s = 0; s < 2; ++s) {
       const int sb = 1 - s;
-      double* rs = rbase + (int64_t)s * a.nrow * 2 * nb;
-      auto put = [&](int r, double L, double R) {
-        if (act) {
-          rs[((int64_t)r * 2) * nb + i] = L;
-          rs[((int64_t)r * 2 + 1) * nb + i] = R;
-        }
-      };
+      double* vs = vout + (int64_t)s * a.nvec * nb;
       const double* vts = act ? VT + (s * nb + i) * 3 : nullptr;
       const double vtsi0 = act ? vts[0] : 0, vtsi1 = act ? vts[1] : 0, vtsi2 = act ? vts[2] : 0;
       const double vtdot = VTv[s][0] * vtsi0 + VTv[s][1] * vtsi1 + VTv[s][2] * vtsi2;
@@ -269,22 +309,16 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
       const double svbi = cs * sv1[s];
       const double Z = 2 * wn * (svbi - vtdot);
       const double sNS = cs * cs * nrmU[s], sNOb = co * co * nrmU[sb];
-      put(0, Ai, wn * (faa[s] * Ai - fai[s]));
-      put(1, Ai, wn * (VTv[s][0] * dAi[0] + VTv[s][1] * dAi[1] + VTv[s][2] * dAi[2] - cs * vai1[s]));
-      put(2, -wn * vtsi0, dAi[0]);
-      put(3, -wn * vtsi1, dAi[1]);
-      put(4, -wn * vtsi2, dAi[2]);
-      put(5, Ai, wn * (vtn - sNS - sNOb) * Ai + Z);
-      put(6, Z, Ai);
-      put(7, 2 * wn * vtsi0, vtsi0);
-      put(8, 2 * wn * vtsi1, vtsi1);
-      put(9, 2 * wn * vtsi2, vtsi2);
-      const double alpha = -2 * wn * (occ[s] * cs * cs + occ[sb] * co * co);
-      for (int b = 0; b < nmax; ++b)
-        for (int c = 0; c < 3; ++c) {
-          const double wv = (act && b < nocc[s]) ? Ws[(((int64_t)s * nb + i) * nmax + b) * 3 + c] : 0.0;
-          put(kFixedRows + b * 3 + c, alpha * wv, wv);
-        }
+      if (own) {
+        vs[0 * nb + i] = wn * (faa[s] * Ai - fai[s]);
+        vs[1 * nb + i] = wn * (VTv[s][0] * dAi[0] + VTv[s][1] * dAi[1] + VTv[s][2] * dAi[2] - cs * vai1[s]);
+        vs[2 * nb + i] = wn * (vtn - sNS - sNOb) * Ai + Z;
+        vs[3 * nb + i] = Z;
+        vs[4 * nb + i] = vtsi0;
+        vs[5 * nb + i] = vtsi1;
+        vs[6 * nb + i] = vtsi2;
+      }
+      if (lane == 0) a.alpha[n * 2 + s] = -2 * wn * (occ[s] * cs * cs + occ[sb] * co * co);
       if (lane < 3) {
         const int c = lane;
         double* h = a.hvec + ((int64_t)n * 2 + s) * 6;
@@ -297,54 +331,104 @@ __global__ __launch_bounds__(256) void tc_point_kernel(PointArgs a) {
   }
 }
 
-// Pass 2: G_target,s[j][i] += sum_n sum_rows(target) L[j] R[i].  One
-// workgroup row (blockIdx.y) per (spin, target); thread-owned elements
-// (j, i) = (e / NBP, e % NBP), e = tid + 256 k; the rows of PB points are staged
-// in LDS per step.
-template <int NBP>
-__global__ __launch_bounds__(256) void tc_rank_kernel(const double* __restrict__ rows, int64_t n1, int nb, int nrow,
-                                                      double* __restrict__ part) {
-  constexpr int KG = NBP * NBP / 256;
-  constexpr int PB = 4;
-  extern __shared__ double rl[];  // [PB][nr][2][nb]
-  const int tid = threadIdx.x;
+struct RankArgs {
+  const double* vecs;   // [n1][2][nvec][nb]
+  const double* alpha;  // [n1][2]
+  const double* A;      // [n1][nb]
+  const double* dA;     // [n1][nb][3]
+  const double* w;      // [n1]
+  int64_t n1;
+  int nb, nvec, nocc0, nocc1;
+  double* part;         // [gridDim.x][6][nb][nb]
+};
+
+// Pass 2: G_{s,t}[j][i] = sum over (point, entry) of L[j] R[i] (the rank-one
+// terms listed above) as an FP64 MFMA product, blockIdx.y = 3 s + t.  K runs
+// over (point, entry) pairs of the block's points; the 4 waves take
+// interleaved 4-wide k-steps over all NT x NT output tiles (16x16 each) and
+// are summed in a fixed order through LDS.  v_mfma_f64_16x16x4: A[j][k] from
+// lane 16 g + r = L_{k0+g}[16 m + r], B[k][i] = R_{k0+g}[16 t + r].
+template <int NT>
+__global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
+  __shared__ double red[NT * 16][NT * 16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
   const int st = blockIdx.y, s = st / 3, tg = st - 3 * s;
-  const int r0 = tg == 0 ? 0 : (tg == 1 ? 1 : 5), r1 = tg == 0 ? 1 : (tg == 1 ? 5 : nrow);
-  const int nr = r1 - r0;
-  const int64_t nb2 = (int64_t)nb * nb;
-  const int64_t per = (n1 + gridDim.x - 1) / gridDim.x;
+  const int nb = a.nb;
+  const int nocc_s = s ? a.nocc1 : a.nocc0;
+  const int E = tg == 0 ? 1 : (tg == 1 ? 4 : 5 + 3 * nocc_s);
+  const int64_t per = (a.n1 + gridDim.x - 1) / gridDim.x;
   const int64_t p0 = (int64_t)blockIdx.x * per;
-  const int64_t p1 = p0 + per < n1 ? p0 + per : n1;
-  const int64_t rpt = (int64_t)2 * nrow * 2 * nb;  // doubles per point in `rows`
-  const int64_t blk = (int64_t)nr * 2 * nb;        // doubles staged per point
-  double acc[KG];
-  int jj[KG], ii[KG];
+  const int64_t p1 = p0 + per < a.n1 ? p0 + per : a.n1;
+  const int64_t K = p1 > p0 ? (p1 - p0) * E : 0;
+  d4 acc[NT][NT];
 #pragma unroll
-  for (int k = 0; k < KG; ++k) {
-    acc[k] = 0;
-    const int e = tid + 256 * k;
-    jj[k] = e / NBP;
-    ii[k] = e % NBP;
-  }
-  for (int64_t n0 = p0; n0 < p1; n0 += PB) {
-    const int pn = (int)((n0 + PB < p1) ? PB : p1 - n0);
-    __syncthreads();
-    for (int64_t e = tid; e < pn * blk; e += 256) {
-      const int64_t pt = e / blk, off = e - pt * blk;
-      rl[e] = rows[(n0 + pt) * rpt + ((int64_t)s * nrow + r0) * 2 * nb + off];
+  for (int m = 0; m < NT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = d4{0, 0, 0, 0};
+  for (int64_t k0 = 4 * wave; k0 < K; k0 += 16) {
+    const int64_t k = k0 + g;
+    const bool kin = k < K;
+    const int64_t n = kin ? p0 + k / E : p0;
+    const int e = kin ? (int)(k % E) : 0;
+    const double* vn = a.vecs + (n * 2 + s) * a.nvec * nb;
+    const double* An = a.A + n * nb;
+    const double* L = An;
+    const double* R = An;
+    int sR = 1;
+    double fl = kin ? 1.0 : 0.0;
+    if (tg == 0) {
+      R = vn;  // (A, u0)
+    } else if (tg == 1) {
+      if (e == 0) {
+        R = vn + nb;  // (A, u1)
+      } else {
+        L = vn + (3 + e) * nb;  // (-w t_c, dA_c), c = e - 1
+        R = a.dA + n * nb * 3 + (e - 1);
+        sR = 3;
+        fl *= -a.w[n];
+      }
+    } else {
+      if (e == 0) {
+        R = vn + 2 * nb;  // (A, u2)
+      } else if (e == 1) {
+        L = vn + 3 * nb;  // (u3, A)
+      } else if (e < 5) {
+        L = R = vn + (2 + e) * nb;  // (2 w t_c, t_c), c = e - 2
+        fl *= 2 * a.w[n];
+      } else {
+        L = R = vn + (kVecFixed + e - 5) * nb;  // (alpha W_bc, W_bc)
+        fl *= a.alpha[n * 2 + s];
+      }
     }
-    __syncthreads();
-    for (int t = 0; t < pn * nr; ++t) {
-      const double* L = rl + (int64_t)t * 2 * nb;
-      const double* R = L + nb;
+    double av[NT], bv[NT];
 #pragma unroll
-      for (int k = 0; k < KG; ++k)
-        if (jj[k] < nb && ii[k] < nb) acc[k] += L[jj[k]] * R[ii[k]];
+    for (int m = 0; m < NT; ++m) {
+      const int row = 16 * m + r;
+      av[m] = row < nb ? fl * L[row] : 0.0;
+      bv[m] = (kin && row < nb) ? R[row * sR] : 0.0;
     }
-  }
 #pragma unroll
-  for (int k = 0; k < KG; ++k)
-    if (jj[k] < nb && ii[k] < nb) part[((int64_t)blockIdx.x * 6 + st) * nb2 + (int64_t)jj[k] * nb + ii[k]] = acc[k];
+    for (int m = 0; m < NT; ++m)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[t], acc[m][t], 0, 0, 0);
+  }
+  // fixed-order sum of the 4 waves' tiles
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w)
+#pragma unroll
+      for (int m = 0; m < NT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int ii = 0; ii < 4; ++ii) {
+            const int row = 16 * m + g + 4 * ii, col = 16 * t + r;
+            red[row][col] = (w == 0 ? 0.0 : red[row][col]) + acc[m][t][ii];
+          }
+    __syncthreads();
+  }
+  double* out = a.part + ((int64_t)blockIdx.x * 6 + st) * nb * nb;
+  for (int e2 = threadIdx.x; e2 < nb * nb; e2 += 256) out[e2] = red[e2 / nb][e2 % nb];
 }
 
 // Pass 3: Rv[s][target][p] = sum_n sum_c h[n][s][target][c] V[n][c][p].
@@ -402,39 +486,47 @@ int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1, const double* dA
   TC_CHECK(A1 && dA1 && w1 && Fs && Fo && V && jp && D_host && C_host && G2_host && G3_host, "tcscf_tc_fock: null");
   TC_CHECK(nb > 0 && nb <= 64, "tcscf_tc_fock: 1 <= nb <= 64");
   TC_CHECK(nup >= 0 && ndown >= 0 && nup <= nb && ndown <= nb, "tcscf_tc_fock: bad electron counts");
-  TC_CHECK(nup <= kMaxOcc && ndown <= kMaxOcc, "tcscf_tc_fock: at most 8 occupied orbitals per spin");
   hipStream_t st = (hipStream_t)stream;
   const int nmax = std::max(1, std::max(nup, ndown));
-  const int nrow = kFixedRows + 3 * nmax;
+  const int nvec = kVecFixed + 3 * nmax;
   const int64_t npair = (int64_t)nb * (nb + 1) / 2, nb2 = (int64_t)nb * nb;
-  // device scratch: D, C, rows, hvec, partials, results
-  const size_t nD = 2 * nb2, nC = 2 * (size_t)nb * nmax, nRows = (size_t)n1 * 2 * nrow * 2 * nb, nH = (size_t)n1 * 12;
+  // device scratch: D, C, per-point vectors, alpha, hvec, partials, results
+  const size_t nD = 2 * nb2, nC = 2 * (size_t)nb * nmax, nVec = (size_t)n1 * 2 * nvec * nb, nAl = (size_t)n1 * 2,
+               nH = (size_t)n1 * 12;
   const size_t nPartG = (size_t)kBlocks * 6 * nb2, nPartV = (size_t)kBlocks * 4 * npair, nOut = 6 * nb2 + 4 * npair;
   int rc = 0;
-  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * (nD + nC + nRows + nH + nPartG + nPartV + nOut), &rc);
+  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * (nD + nC + nVec + nAl + nH + nPartG + nPartV + nOut), &rc);
   if (rc) return rc;
   double* dD = ws;
   double* dC = dD + nD;
-  double* dRows = dC + nC;
-  double* dH = dRows + nRows;
+  double* dVec = dC + nC;
+  double* dAl = dVec + nVec;
+  double* dH = dAl + nAl;
   double* dPG = dH + nH;
   double* dPV = dPG + nPartG;
   double* dOut = dPV + nPartV;
   TC_HIP(hipMemcpyAsync(dD, D_host, sizeof(double) * nD, hipMemcpyHostToDevice, st));
   TC_HIP(hipMemcpyAsync(dC, C_host, sizeof(double) * nC, hipMemcpyHostToDevice, st));
-  PointArgs a{Fs, Fo, V, A1, dA1, w1, dD, dC, jp->c_same, jp->c_oppo, n1, npair, nb, nmax, nup, ndown, nrow, dRows, dH};
-  const int64_t per_wave = 5 * npair + nb + 3 * nb + 2 * nb + 6 * nb + 6 * nb + 6 * nb + 6 * (int64_t)nb * nmax;
-  const size_t sh1 = sizeof(double) * (2 * nb2 + 2 * (int64_t)nb * nmax + 4 * per_wave);
-  TC_CHECK(sh1 <= 160 * 1024, "tcscf_tc_fock: nb too large for the LDS staging");
+  PointArgs a{Fs, Fo, V, A1, dA1, w1, dD, dC, jp->c_same, jp->c_oppo, n1, npair, nb, nmax, nup, ndown, nvec,
+              dVec, dAl, dH};
+  // waves per workgroup: as many (up to 4) as the LDS holds next to D and C
+  const size_t shared = sizeof(double) * (2 * nb2 + 2 * (int64_t)nb * nmax);
+  const size_t per_wave = sizeof(double) * point_wave_doubles(nb, nmax);
+  int wpw = 4;
+  while (wpw > 1 && shared + wpw * per_wave > 160 * 1024) wpw /= 2;
+  const size_t sh1 = shared + wpw * per_wave;
+  TC_CHECK(sh1 <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
   if (sh1 > 64 * 1024) TC_HIP(hipFuncSetAttribute((const void*)tc_point_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh1));
-  hipLaunchKernelGGL(tc_point_kernel, dim3(kBlocks), dim3(256), sh1, st, a);
+  const int64_t pblocks = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + wpw - 1) / wpw));
+  hipLaunchKernelGGL(tc_point_kernel, dim3((unsigned)pblocks), dim3(64 * wpw), sh1, st, a);
   TC_HIP(hipGetLastError());
-  const size_t sh2 = sizeof(double) * 4 * (nrow - 5) * 2 * nb;
-  TC_CHECK(sh2 <= 64 * 1024, "tcscf_tc_fock: too many occupied orbitals for the rank kernel");
+  RankArgs ra{dVec, dAl, A1, dA1, w1, n1, nb, nvec, nup, ndown, dPG};
   if (nb <= 32)
-    hipLaunchKernelGGL(tc_rank_kernel<32>, dim3(kBlocks, 6), dim3(256), sh2, st, dRows, n1, nb, nrow, dPG);
+    hipLaunchKernelGGL(tc_rank_kernel<2>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
+  else if (nb <= 48)
+    hipLaunchKernelGGL(tc_rank_kernel<3>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
   else
-    hipLaunchKernelGGL(tc_rank_kernel<64>, dim3(kBlocks, 6), dim3(256), sh2, st, dRows, n1, nb, nrow, dPG);
+    hipLaunchKernelGGL(tc_rank_kernel<4>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
   TC_HIP(hipGetLastError());
   hipLaunchKernelGGL(tc_vlin_kernel, dim3(kBlocks), dim3(256), 0, st, V, dH, n1, npair, dPV);
   TC_HIP(hipGetLastError());
@@ -475,7 +567,11 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
   const int ne[2] = {nup, ndown};
   const int nmax = std::max(1, std::max(nup, ndown));
   std::vector<double> D(2 * nb2, 0.0), C(2 * (size_t)nb * nmax, 0.0), G2(2 * nb2), G3(2 * nb2), Fock(2 * nb2);
-  std::vector<double> pseudo(nup + ndown, 0.0);
+  std::vector<double> pseudo(nup + ndown, 0.0), err;
+  TC_CHECK(o->diis >= 0, "tcscf_tc_scf: diis depth must be >= 0");
+  TC_CHECK(o->diis == 0 || o->mgs_mode == 1,
+           "tcscf_tc_scf: DIIS needs S-orthonormal orbitals (mgs_mode = 1)");
+  dense::Diis diis(o->diis);
   double energy = std::numeric_limits<double>::max();
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
@@ -508,6 +604,10 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
       break;
     }
     energy = e;
+    if (o->diis > 0 && it > 0) {  // iteration 0 has D = 0, whose error is 0 trivially
+      dense::right_eigen_error(2, nb, Fock.data(), D.data(), S, err);
+      diis.extrapolate(Fock, err);
+    }
     for (int s = 0; s < 2; ++s) {
       std::vector<std::complex<double>> ev, X;
       if (!dense::gen_nonsym_eig(nb, Fock.data() + s * nb2, S, ev, X)) {

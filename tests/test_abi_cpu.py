@@ -2,6 +2,7 @@
 declares, and its host-side pieces (grids, one-electron matrices) match the
 oracle.  No device calls."""
 import ctypes
+import math
 import os
 import re
 
@@ -87,3 +88,62 @@ def test_single_hip_runtime_in_process(tc):
     the device consistently)."""
     paths = {line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64" in line}
     assert len(paths) == 1, paths
+
+
+# ------------------------------------------------------------------ QMC grids
+def test_sobol_points_match_scipy_and_keep_the_net_property(tc):
+    """Unscrambled Sobol points (Joe-Kuo directions, Gray-code order) equal
+    scipy's bit for bit; scrambling (linear matrix + digital shift) keeps the
+    (0, m, 1)-net property of every coordinate and the (0, m, 2)-net property
+    of the first two; the seed selects the scramble."""
+    from scipy.stats import qmc
+    u = tc.sobol_points(4096, 3, scramble=False)
+    assert np.array_equal(u, qmc.Sobol(d=3, scramble=False).random(4096))
+    m = 10
+    us = tc.sobol_points(2 ** m, 3, scramble=True, seed=7)
+    assert np.all((us >= 0) & (us < 1))
+    for d in range(3):
+        assert np.all(np.bincount((us[:, d] * 2 ** m).astype(int), minlength=2 ** m) == 1)
+    for a in (3, 5, 7):
+        b = m - a
+        box = (us[:, 0] * 2 ** a).astype(int) * 2 ** b + (us[:, 1] * 2 ** b).astype(int)
+        assert np.all(np.bincount(box, minlength=2 ** m) == 1)
+    assert np.array_equal(us, tc.sobol_points(2 ** m, 3, scramble=True, seed=7))
+    assert not np.array_equal(us, tc.sobol_points(2 ** m, 3, scramble=True, seed=8))
+
+
+def test_sobol_importance_grid_integrates(tc):
+    """r ~ 2 e^{-2r}, uniform directions: the 1s (zeta = 1) density integrates
+    to 1 and e^{-2r} to pi, with QMC accuracy; the weights are the importance
+    ratios 4 pi r^2 / (2 n e^{-2r})."""
+    n = 2 ** 14
+    pts, w = tc.sobol_grid(n, seed=3)
+    r = np.sqrt((pts ** 2).sum(0))
+    assert np.allclose(w, 4 * math.pi * r * r / (2 * n * np.exp(-2 * r)), rtol=1e-13)
+    assert abs((w * np.exp(-2 * r) / math.pi).sum() - 1) < 2e-3
+    assert abs((w * np.exp(-2 * r)).sum() - math.pi) < 2e-3 * math.pi
+    # odd integrand 2 pi r^2 z per unit density (mean |.| ~ 4.7): zero to QMC accuracy
+    assert abs((w * np.exp(-2 * r) * pts[2]).sum()) < 0.05
+
+
+def test_qmc_lattice_grid(tc):
+    """QMCGrid(gridSize) shape (src/integration/integrateAll.hpp:21-65): two
+    shifted copies of a prime-size lattice; M4(0, 0.9) x Azimuthal x Linear maps
+    with weight r^2 sin(theta) integrate a 1s density to 1 and an odd function
+    to 0; the 2-D variant keeps y = 0 and, times 2 pi, integrates axially
+    symmetric functions.  Exact point sets of mppmu/qmc are not reproducible
+    (parity unpinned); these properties are what is checked."""
+    pts, w = tc.qmc_lattice_grid(5000, 3, 2, seed=1)
+    assert len(w) == 2 * 5003  # 5003 is the first prime >= 5000
+    assert np.all(w > 0)
+    r = np.sqrt((pts ** 2).sum(0))
+    assert abs((w * np.exp(-2 * r) / math.pi).sum() - 1) < 1e-6
+    z2 = 1.7
+    assert abs((w * np.exp(-2 * z2 * r) * z2 ** 3 / math.pi).sum() - 1) < 1e-6
+    assert abs((w * np.exp(-2 * r) * pts[0]).sum()) < 1e-4  # int |x| e^{-2r} = 3 pi/8
+    p2, w2 = tc.qmc_lattice_grid(5000, 2, 2, seed=1)
+    assert np.all(p2[1] == 0)
+    r2 = np.sqrt((p2 ** 2).sum(0))
+    assert abs(2 * math.pi * (w2 * np.exp(-2 * r2) / math.pi).sum() - 1) < 1e-6
+    a, _ = tc.qmc_lattice_grid(5000, 3, 2, seed=2)
+    assert not np.array_equal(a, pts)

@@ -225,6 +225,14 @@ def test_rcs_ne_small_grid_end_to_end(tc, O, gpu):
     ref = O.rcs_scf(10, S, H, A, g.weights_host, Fo, tol=1e-10)
     assert ref["status"] == 0
     assert abs(e - ref["energy"]) < 1e-10, (e, ref["energy"])
+    # Pulay DIIS (an option the reference does not have) reaches the same fixed
+    # point, in fewer iterations, for RCS and UOS
+    hd = tc.RCSHartreeFock(10, 30)
+    ed = hd.compute(S, H, g, F, True, tol=1e-10, diis=6)
+    assert abs(ed - ref["energy"]) < 1e-9, (ed, ref["energy"])
+    assert hd.numberOfConvergenceLoops() < hf.numberOfConvergenceLoops()
+    eu = tc.UOSHartreeFock(5, 5, 30).compute(S, H, g, F, True, tol=1e-10, diis=6)
+    assert abs(eu - ref["energy"]) < 1e-9
 
 
 def test_rcs_ne_config2_full_size(tc, O, gpu):
@@ -316,6 +324,14 @@ def test_tc_scf_vs_oracle(tc, O, gpu, Z, nup, ndown, mgs_mode):
     assert np.max(np.abs(hf.energy_trace[:n] - ref["trace"][:n])) < 1e-9
     assert abs(e - ref["energy"]) < 1e-9, (e, ref["energy"])
     assert np.max(np.abs(hf.density - ref["D"])) < 1e-7
+    if mgs_mode == 1:
+        # DIIS on the right-eigenvector error reaches the same TC fixed point
+        hd = tc.TCHartreeFock(nup, ndown, nb)
+        ed = hd.compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-10, max_iter=200, mgs_mode=1, diis=6)
+        assert abs(ed - ref["energy"]) < 1e-8, (ed, ref["energy"])
+    else:
+        with pytest.raises(tc.TcscfError):
+            tc.TCHartreeFock(nup, ndown, nb).compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-10, mgs_mode=0, diis=6)
 
 
 def test_tc_zero_jastrow_equals_uos(tc, gpu):
@@ -451,3 +467,60 @@ def test_gradient_eri_vs_oracle(tc, O, gpu):
         E = tc.eri_full(tc.integrateAllR1R2(g1, jas.c_same * V[:, c, :].contiguous()), nb)
         assert np.max(np.abs(E)) > 1e-6
         assert rel_err(G.transpose(1, 0, 3, 2), E) < 1e-12
+
+
+def test_tc_scf_on_sobol_grids_vs_oracle(tc, O, gpu):
+    """TC SCF on importance-sampled Sobol grids (separate seeds for r1 and r3)
+    against the oracle on the same points, small size."""
+    basis = tc.even_tempered({0: 4, 1: 1}, 0.8, 2.0)
+    nb = len(basis)
+    g1 = tc.QMCGrid.sobol(2 ** 11, seed=1)
+    g3 = tc.QMCGrid.sobol(2 ** 10, seed=3)
+    for g in (g1, g3):
+        g.setBasisFunctions(basis, True)
+    jas = tc.Jastrow.from_b(1.0)
+    Fs = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=True)
+    Fo = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=False)
+    V = tc.computeV(g1, g3, jas)
+    S, H = tc.one_electron_matrices(basis, 4)
+    hf = tc.TCHartreeFock(2, 2, nb)
+    e = hf.compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-10, mgs_mode=1)
+    prm = O.ochi_params(1.0, jas.a12, [(1, 0, 0, jas.c_oppo, jas.c_same)])
+    A = g1.basisValues.cpu().numpy()
+    Fs_o = O.compute_F(g1.points_host, g1.points_host, g1.weights_host, A, kernel=O.K_TC_SCALAR, prm=prm, same=True)
+    Fo_o = O.compute_F(g1.points_host, g1.points_host, g1.weights_host, A, kernel=O.K_TC_SCALAR, prm=prm, same=False)
+    A3 = g3.basisValues.cpu().numpy()
+    Vs = O.compute_F(g1.points_host, g3.points_host, g3.weights_host, A3, kernel=O.K_JASTROW_GRAD, prm=prm, same=True)
+    Vo = O.compute_F(g1.points_host, g3.points_host, g3.weights_host, A3, kernel=O.K_JASTROW_GRAD, prm=prm, same=False)
+    ref = O.tc_scf(2, 2, S, H, A, g1.basisGradients.cpu().numpy(), g1.weights_host, Fs_o, Fo_o, Vs, Vo, tol=1e-10,
+                   mgs_mode=1)
+    assert ref["status"] == 0
+    assert abs(e - ref["energy"]) < 1e-9, (e, ref["energy"])
+
+
+def test_tc_qmc_ne_config4_mixed_vs_fp64(tc, gpu):
+    """The QMC workload at full size (Ne, 40 STOs, 2**16 Sobol points per
+    coordinate, b = 1): the BF16x5 precompute gives the FP64 TC energy to
+    1e-6 Eh (same grids, same SCF).  With this basis the plain Roothaan loop
+    oscillates between two states on these grids (so does plain RHF), so the
+    solve uses DIIS (depth 6)."""
+    from tcscf_amd import configs
+    c = configs.ne_qmc_config4()
+    basis, nb = c["basis"], len(c["basis"])
+    g1 = tc.QMCGrid.sobol(c["npts"], seed=c["seeds"][0])
+    g2 = tc.QMCGrid.sobol(c["npts"], seed=c["seeds"][1])
+    g3 = tc.QMCGrid.sobol(c["npts"], seed=c["seeds"][2])
+    for g in (g1, g2, g3):
+        g.setBasisFunctions(basis, True)
+    S, H = tc.one_electron_matrices(basis, c["Z"])
+    jas = tc.Jastrow.from_b(1.0)
+    E = {}
+    for prec in ("bf16x5", "fp64"):
+        Fs = tc.computeF(g1, g2, kernel="tc", jastrow=jas, same_spin=True, precision=prec)
+        Fo = tc.computeF(g1, g2, kernel="tc", jastrow=jas, same_spin=False, precision=prec)
+        V = tc.computeV(g1, g3, jas, precision=prec)
+        E[prec] = tc.TCHartreeFock(c["nup"], c["ndown"], nb).compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-9,
+                                                                      mgs_mode=1, diis=6)
+        del Fs, Fo, V
+    print(f"QMC Ne TC: E(bf16x5) = {E['bf16x5']:.10f}, E(fp64) = {E['fp64']:.10f}")
+    assert abs(E["bf16x5"] - E["fp64"]) < 1e-6
