@@ -6,6 +6,8 @@
 //     registers (FP64/FP32) or LDS (BF16) and never stored.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -485,8 +487,8 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 /* (double buffered); B fragments and the next step's r2 coordinates are     */
 /* prefetched into registers one step ahead.                                 */
 /* ------------------------------------------------------------------------ */
-template <int MB, int NPROD, int FOLD, int PF, class Kern>
-__global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
+template <int MB, int NPROD, int FOLD, class Kern>
+__global__ __launch_bounds__(256, 2) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
                                                          const float4* __restrict__ x2, int64_t n2p,
                                                          const us8* __restrict__ P0, int64_t plane,
                                                          int64_t ncb, void* __restrict__ Fout, int out_f32,
@@ -496,7 +498,9 @@ __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restric
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
   constexpr int VALS = MB * 64 * 8;       // K values per component per k-step
   constexpr int EPT = VALS / 256;         // per thread
-  static_assert(EPT >= 1 && EPT <= 8 && VALS % 256 == 0, "MB must be 1, 2 or 4");
+  static_assert(EPT >= 2 && EPT <= 8 && VALS % 256 == 0 && EPT % 2 == 0, "MB must be 1, 2 or 4");
+  static_assert(FOLD == 1 || FOLD == 2, "FP64 fold every 1 or 2 steps");
+  typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
   __shared__ us8 ktile[2][NC][MB][2][64];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -524,38 +528,36 @@ __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restric
 #pragma unroll
     for (int e = 0; e < EPT; ++e) qn[e] = x2[k0 + gk + e];
   };
+  // K = hi + lo, both bf16 (round to nearest even, v_cvt_pk_bf16_f32)
   auto generate = [&](int buf) {
-    us8 hv[NC], lv[NC];
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      float v[NC];
-      kern(ax - qn[e].x, ay - qn[e].y, az - qn[e].z, v);
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const unsigned short h = bf16_bits_rne(v[c]);
-        hv[c][e] = h;
-        lv[c][e] = bf16_bits_rne(v[c] - bf16_to_float(h));
-      }
-    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      if constexpr (EPT == 8) {
-        ktile[buf][c][gm][0][gl] = hv[c];
-        ktile[buf][c][gm][1][gl] = lv[c];
-      } else {
-        unsigned short* dh = (unsigned short*)&ktile[buf][c][gm][0][gl] + ge;
-        unsigned short* dl = (unsigned short*)&ktile[buf][c][gm][1][gl] + ge;
+      unsigned hw[EPT / 2], lw[EPT / 2];
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) {
-          dh[e] = hv[c][e];
-          dl[e] = lv[c][e];
-        }
+      for (int e = 0; e < EPT; e += 2) {
+        float v0[NC], v1[NC];
+        kern(ax - qn[e].x, ay - qn[e].y, az - qn[e].z, v0);
+        kern(ax - qn[e + 1].x, ay - qn[e + 1].y, az - qn[e + 1].z, v1);
+        bf2 h;
+        h[0] = (__bf16)v0[c];
+        h[1] = (__bf16)v1[c];
+        const unsigned hb = __builtin_bit_cast(unsigned, h);
+        bf2 l;
+        l[0] = (__bf16)(v0[c] - __uint_as_float(hb << 16));
+        l[1] = (__bf16)(v1[c] - __uint_as_float(hb & 0xffff0000u));
+        hw[e / 2] = hb;
+        lw[e / 2] = __builtin_bit_cast(unsigned, l);
+      }
+      unsigned* dh = (unsigned*)((unsigned short*)&ktile[buf][c][gm][0][gl] + ge);
+      unsigned* dl = (unsigned*)((unsigned short*)&ktile[buf][c][gm][1][gl] + ge);
+#pragma unroll
+      for (int e = 0; e < EPT / 2; ++e) {
+        dh[e] = hw[e];
+        dl[e] = lw[e];
       }
     }
   };
 
-  // B fragments of the PF steps after the current one, plus the current
-  us8 bq[PF + 1][NSPLIT][NB];
   auto load_b = [&](us8 (&dst)[NSPLIT][NB], int64_t s) {
 #pragma unroll
     for (int t = 0; t < NSPLIT; ++t)
@@ -572,25 +574,24 @@ __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restric
       for (int n = 0; n < NB; ++n)
 #pragma unroll
         for (int i = 0; i < 4; ++i) sum[c][m][n][i] = 0;
+  f4 acc[NC][MB][NB];
 
   const int64_t nsteps = n2p / 32;
+  us8 bqA[NSPLIT][NB], bqB[NSPLIT][NB];  // B fragments, ping-pong between steps
   load_coords(0);
   generate(0);
   if (nsteps > 1) load_coords(32);
-#pragma unroll
-  for (int d = 0; d < PF; ++d)
-    if (d < nsteps) load_b(bq[d], d);
+  load_b(bqA, 0);
   __syncthreads();
-  f4 acc[NC][MB][NB];
-#pragma unroll
-  for (int c = 0; c < NC; ++c)
-#pragma unroll
-    for (int m = 0; m < MB; ++m)
-#pragma unroll
-      for (int n = 0; n < NB; ++n) acc[c][m][n] = f4{0, 0, 0, 0};
-  for (int64_t s = 0; s < nsteps; ++s) {
+
+  // One 32-point step: MFMAs on ktile[s & 1] and cur, B of step s + 1 into nxt,
+  // K of step s + 1 into the other tile.  FRESH: the step starts a new FP32
+  // partial (zero accumulator operand); the fold to FP64 follows every FOLD steps.
+  auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto fresh) {
+    constexpr bool FRESH = decltype(fresh)::value;
     const int buf = (int)(s & 1);
-    if (s + PF < nsteps) load_b(bq[PF], s + PF);
+    if (s + 1 < nsteps) load_b(nxt, s + 1);
+    const f4 zero = {0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -599,42 +600,40 @@ __global__ __launch_bounds__(256) void fgemm_bf16_kernel(const double* __restric
         const bf8 al = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
-          const bf8 ph = __builtin_bit_cast(bf8, bq[0][0][n]);
-          const bf8 pm = __builtin_bit_cast(bf8, bq[0][1][n]);
-          f4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, acc[c][m][n], 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pm, a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, a, 0, 0, 0);
+          const bf8 ph = __builtin_bit_cast(bf8, cur[0][n]);
+          const bf8 pm = __builtin_bit_cast(bf8, cur[1][n]);
+          f4 v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, FRESH ? zero : acc[c][m][n], 0, 0, 0);
+          v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pm, v, 0, 0, 0);
+          v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, v, 0, 0, 0);
           if constexpr (NPROD == 5) {
-            const bf8 pl = __builtin_bit_cast(bf8, bq[0][NSPLIT - 1][n]);
-            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, a, 0, 0, 0);
-            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, pm, a, 0, 0, 0);
+            const bf8 pl = __builtin_bit_cast(bf8, cur[NSPLIT - 1][n]);
+            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, v, 0, 0, 0);
+            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, pm, v, 0, 0, 0);
           }
-          acc[c][m][n] = a;
+          acc[c][m][n] = v;
         }
       }
     if (s + 1 < nsteps) {
       generate(buf ^ 1);
       if (s + 2 < nsteps) load_coords((s + 2) * 32);
     }
-    if ((s + 1) % FOLD == 0 || s + 1 == nsteps) {
+    if (!FRESH || FOLD == 1 || s + 1 == nsteps) {
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int m = 0; m < MB; ++m)
 #pragma unroll
-          for (int n = 0; n < NB; ++n) {
+          for (int n = 0; n < NB; ++n)
 #pragma unroll
             for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
-            acc[c][m][n] = f4{0, 0, 0, 0};
-          }
     }
-#pragma unroll
-    for (int d = 0; d < PF; ++d)
-#pragma unroll
-      for (int t = 0; t < NSPLIT; ++t)
-#pragma unroll
-        for (int n = 0; n < NB; ++n) bq[d][t][n] = bq[d + 1][t][n];
     __syncthreads();
+  };
+  using T = std::integral_constant<bool, true>;
+  using Fz = std::integral_constant<bool, FOLD == 1>;
+  for (int64_t s = 0; s < nsteps; s += 2) {
+    step(s, bqA, bqB, T{});
+    if (s + 1 < nsteps) step(s + 1, bqB, bqA, Fz{});
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c)
@@ -711,45 +710,39 @@ int launch_f32(const double* p1, int64_t n1, const float4* x2, int64_t n2p, cons
   return 0;
 }
 
-template <int MB, int NPROD, int FOLD, int PF, class Kern>
+template <int MB, int NPROD, int FOLD, class Kern>
 int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane, int64_t ncb,
                 void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
   dim3 grid((unsigned)(ncb / 16), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
-  hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, PF, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0, plane,
+  hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0, plane,
                      ncb, F, out_f32, npair, scale, kern);
   TC_HIP(hipGetLastError());
   return 0;
 }
 
-// BF16 kernel variant: rows per workgroup 16*MB, FP64 fold every FOLD steps,
-// B prefetch PF steps ahead.  TCSCF_BF16_CFG="MB,FOLD,PF" selects one for tuning.
+// BF16 kernel variant: rows per workgroup 16*MB, FP64 fold every FOLD steps.
+// TCSCF_BF16_CFG="MB,FOLD" selects one for tuning.
 struct Bf16Cfg {
-  int mb, fold, pf;
+  int mb, fold;
 };
 Bf16Cfg bf16_cfg(int ncomp) {
   static Bf16Cfg env = [] {
-    Bf16Cfg c{0, 0, 0};
-    if (const char* e = std::getenv("TCSCF_BF16_CFG")) std::sscanf(e, "%d,%d,%d", &c.mb, &c.fold, &c.pf);
+    Bf16Cfg c{0, 0};
+    if (const char* e = std::getenv("TCSCF_BF16_CFG")) std::sscanf(e, "%d,%d", &c.mb, &c.fold);
     return c;
   }();
-  Bf16Cfg c = env.mb ? env : Bf16Cfg{2, 1, 1};
+  Bf16Cfg c = env.mb ? env : Bf16Cfg{2, 1};
   if (ncomp != 1) c.mb = 1;
   return c;
 }
 
 template <int NPROD, int MB, class Kern>
-int launch_bf16_fp(Bf16Cfg c, const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0,
-                   int64_t plane, int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern,
-                   hipStream_t st) {
-#define TC_BF16(FOLD, PF) \
-  return launch_bf16<MB, NPROD, FOLD, PF>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st)
-  if (c.pf == 2) {
-    if (c.fold == 2) TC_BF16(2, 2);
-    TC_BF16(1, 2);
-  }
-  if (c.fold == 2) TC_BF16(2, 1);
-  TC_BF16(1, 1);
-#undef TC_BF16
+int launch_bf16_fold(Bf16Cfg c, const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0,
+                     int64_t plane, int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern,
+                     hipStream_t st) {
+  if (c.fold == 2)
+    return launch_bf16<MB, NPROD, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+  return launch_bf16<MB, NPROD, 1>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
 }
 
 template <int NPROD, class Kern>
@@ -758,12 +751,11 @@ int launch_bf16_mb(const double* p1, int64_t n1, const float4* x2, int64_t n2p, 
   const Bf16Cfg c = bf16_cfg(Kern::ncomp);
   if constexpr (Kern::ncomp == 1) {
     switch (c.mb) {
-      case 1: return launch_bf16_fp<NPROD, 1>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
-      case 4: return launch_bf16_fp<NPROD, 4>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
-      default: return launch_bf16_fp<NPROD, 2>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      case 4: return launch_bf16_fold<NPROD, 4>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      default: return launch_bf16_fold<NPROD, 2>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
     }
   } else {
-    return launch_bf16_fp<NPROD, 1>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+    return launch_bf16_fold<NPROD, 1>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
   }
 }
 
