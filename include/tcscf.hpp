@@ -1,0 +1,528 @@
+/* C++ host facade: the reference's call surface over the tcscf_hip C ABI.
+ *
+ * Mirrors the names, argument meaning and error behaviour of
+ *   src/SlaterTypeOrbital.hpp      SlaterTypeOrbital, overlap, coreMatrixElement, loadSTO
+ *   src/integration/integrateAll.hpp  integration::QMCGrid, integrateAllR1R2,
+ *                                  threeElectronIntegrals
+ *   src/HartreeFock.hpp            computeF / computeV (fused with precomputeIntegrand),
+ *                                  RCSHartreeFock, UOSHartreeFock, TCHartreeFock
+ * so a driver written against the reference (src/unitTests/HartreeFockTest.cpp)
+ * ports by swapping includes.  Differences, all forced by the boundary:
+ *   - the integrand lambdas f(r1, r2) become kernel descriptors (Coulomb, the TC
+ *     scalar kernel and the Jastrow gradient of the r12 Ochi term), since a
+ *     lambda cannot cross into device code compiled elsewhere;
+ *   - grids, basis tables and F / V live in HBM (DeviceArray); F and V are
+ *     packed over pairs j <= i, V for c = 1 (V_same = c_same V, V_oppo = c_oppo V);
+ *   - the basis is real (real solid harmonics), a unitary change of the
+ *     reference's complex Y_lm that leaves every energy unchanged;
+ *   - a non-converged RCS solve throws ConvergenceError (the reference aborts).
+ * Header-only; link with -ltcscf_hip and the HIP runtime. */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "tcscf_hip.h"
+
+namespace tcscf {
+
+using IndexType = std::ptrdiff_t;
+
+/* Errors: TCSCF_ENOTCONV -> ConvergenceError, TCSCF_ESOLVER -> SolverError
+ * (the reference's LvArray::dense::Error), everything else -> Error. */
+class Error : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
+};
+class ConvergenceError : public Error {
+ public:
+  using Error::Error;
+};
+class SolverError : public Error {
+ public:
+  using Error::Error;
+};
+
+namespace detail {
+inline int check(int rc) {
+  if (rc >= 0) return rc;
+  const std::string msg = tcscf_last_error();
+  if (rc == TCSCF_ENOTCONV) throw ConvergenceError(msg);
+  if (rc == TCSCF_ESOLVER) throw SolverError(msg);
+  throw Error(msg);
+}
+inline void hip(hipError_t e) {
+  if (e != hipSuccess) throw Error(std::string("HIP: ") + hipGetErrorString(e));
+}
+}  // namespace detail
+
+/* Owning device (HBM) array. */
+template <class T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(std::size_t n) { resize(n); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)) {}
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = std::exchange(o.p_, nullptr);
+      n_ = std::exchange(o.n_, 0);
+    }
+    return *this;
+  }
+  ~DeviceArray() { release(); }
+
+  void resize(std::size_t n) {
+    release();
+    if (n) detail::hip(hipMalloc(&p_, n * sizeof(T)));
+    n_ = n;
+  }
+  void upload(const T* h, std::size_t n) {
+    if (n != n_) resize(n);
+    if (n) detail::hip(hipMemcpy(p_, h, n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  std::vector<T> download() const {
+    std::vector<T> h(n_);
+    if (n_) detail::hip(hipMemcpy(h.data(), p_, n_ * sizeof(T), hipMemcpyDeviceToHost));
+    return h;
+  }
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  std::size_t size() const { return n_; }
+  bool empty() const { return n_ == 0; }
+
+ private:
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* p_ = nullptr;
+  std::size_t n_ = 0;
+};
+
+/* ------------------------------------------------------------------ STO basis */
+/* SlaterTypeOrbital (src/SlaterTypeOrbital.hpp:15-113): N r^(n-1) e^(-alpha r) S_lm,
+ * N = (2 alpha)^n sqrt(2 alpha / (2n)!); l <= 3 on the device. */
+template <class REAL>
+struct SlaterTypeOrbital {
+  using Real = REAL;
+  static constexpr bool isBasisReal = true;
+  static constexpr bool isOrthonormal = false;
+
+  SlaterTypeOrbital(Real alphaP, int nP, int lP, int mP) : alpha{alphaP}, n{nP}, l{lP}, m{mP} {
+    if (n < 1 || n < l || l < std::abs(m) || l > 3) throw Error("SlaterTypeOrbital: need n >= 1, n >= l >= |m|, l <= 3");
+    normalization = norm();
+  }
+  void resetOrbitalExponent(Real newAlpha) {
+    alpha = newAlpha;
+    normalization = norm();
+  }
+  tcscf_sto abi() const { return tcscf_sto{n, l, m, static_cast<double>(alpha)}; }
+
+  Real alpha;
+  int n, l, m;
+  Real normalization;
+
+ private:
+  Real norm() const { return std::pow(2 * alpha, n) * std::sqrt(2 * alpha / std::tgamma(2.0 * n + 1)); }
+};
+
+namespace detail {
+template <class B>
+std::vector<tcscf_sto> abi(const std::vector<B>& fns) {
+  std::vector<tcscf_sto> out;
+  out.reserve(fns.size());
+  for (const B& b : fns) out.push_back(b.abi());
+  return out;
+}
+}  // namespace detail
+
+/* overlap / coreMatrixElement (src/SlaterTypeOrbital.hpp:125-171). */
+template <class REAL>
+REAL overlap(const SlaterTypeOrbital<REAL>& b1, const SlaterTypeOrbital<REAL>& b2) {
+  const tcscf_sto f[2] = {b1.abi(), b2.abi()};
+  double S[4], H[4];
+  detail::check(tcscf_sto_overlap_core(f, 2, 0, S, H));
+  return static_cast<REAL>(S[1]);
+}
+template <class REAL>
+REAL coreMatrixElement(int Z, const SlaterTypeOrbital<REAL>& b1, const SlaterTypeOrbital<REAL>& b2) {
+  const tcscf_sto f[2] = {b1.abi(), b2.abi()};
+  double S[4], H[4];
+  detail::check(tcscf_sto_overlap_core(f, 2, Z, S, H));
+  return static_cast<REAL>(H[1]);
+}
+/* Whole matrices at once (row-major nb x nb; S is symmetric, so it is also the
+ * reference's column-major overlap view). */
+template <class B>
+void oneElectronMatrices(const std::vector<B>& fns, int Z, std::vector<double>& S, std::vector<double>& H) {
+  const auto f = detail::abi(fns);
+  S.assign(f.size() * f.size(), 0.0);
+  H.assign(f.size() * f.size(), 0.0);
+  detail::check(tcscf_sto_overlap_core(f.data(), static_cast<int>(f.size()), Z, S.data(), H.data()));
+}
+
+/* loadSTO (src/SlaterTypeOrbital.cpp:24-76). */
+inline std::vector<SlaterTypeOrbital<double>> loadSTO(const std::string& directory, const std::string& basisSet,
+                                                      const std::string& element, int maxL = INT_MAX) {
+  const int n = detail::check(tcscf_load_sto(directory.c_str(), basisSet.c_str(), element.c_str(), maxL, nullptr, 0));
+  std::vector<tcscf_sto> f(static_cast<std::size_t>(n));
+  detail::check(tcscf_load_sto(directory.c_str(), basisSet.c_str(), element.c_str(), maxL, f.data(), n));
+  std::vector<SlaterTypeOrbital<double>> out;
+  out.reserve(f.size());
+  for (const tcscf_sto& s : f) out.emplace_back(s.zeta, s.n, s.l, s.m);
+  return out;
+}
+
+/* --------------------------------------------------------------------- grids */
+namespace integration {
+
+/* QuadratureGrid (src/integration/quadrature.hpp:26-35): points 3 x N (SoA),
+ * weights N, in HBM; host copies kept for set-up and checks. */
+template <class REAL>
+struct QuadratureGrid {
+  DeviceArray<REAL> points;
+  DeviceArray<REAL> weights;
+  std::vector<REAL> hostPoints;
+  std::vector<REAL> hostWeights;
+};
+
+/* QMCGrid (src/integration/integrateAll.hpp:12-142).  NDIM == 2 keeps y = 0
+ * (the reference's axially symmetric r2 grid; computeF multiplies by 2 pi). */
+template <class REAL, int NDIM>
+struct QMCGrid {
+  static_assert(std::is_same_v<REAL, double>, "the device path computes in FP64 / mixed precision from FP64 grids");
+  static_assert(NDIM == 2 || NDIM == 3);
+
+  /* QMCGrid(gridSize): a randomly shifted rank-1 lattice (2 shifts) through the
+   * TreutlerAhlrichsM4(0, 0.9) x Azimuthal x Linear maps, as the reference. */
+  explicit QMCGrid(IndexType gridSize, std::uint64_t seed = 0) {
+    const int64_t n = tcscf_qmc_lattice_npts(gridSize, 2);
+    if (n < 0) detail::check(static_cast<int>(n));
+    std::vector<REAL> p(3 * n), w(n);
+    const int64_t got = tcscf_qmc_lattice_grid(gridSize, NDIM, 2, seed, p.data(), w.data());
+    if (got < 0) detail::check(static_cast<int>(got));
+    setPoints(std::move(p), std::move(w));
+  }
+  /* Explicit points (3 x N SoA) and weights (every Jacobian included). */
+  QMCGrid(std::vector<REAL> points, std::vector<REAL> weights) { setPoints(std::move(points), std::move(weights)); }
+
+  /* Gauss-Chebyshev radial x Lebedev (TreutlerAhlrichsLebedev in QMCGrid shape). */
+  static QMCGrid radialLebedev(int nRadial, int lebedevOrder, int radialMap = TCSCF_RADIAL_M4, double a = 1.0,
+                               double eps = 1.0) {
+    static_assert(NDIM == 3);
+    const int na = detail::check(tcscf_lebedev_npts(lebedevOrder));
+    const std::size_t n = static_cast<std::size_t>(nRadial) * na;
+    std::vector<REAL> p(3 * n), w(n);
+    detail::check(tcscf_product_grid(nRadial, radialMap, a, eps, lebedevOrder, p.data(), w.data()));
+    return QMCGrid(std::move(p), std::move(w));
+  }
+  /* n scrambled-Sobol points, r ~ 2 e^{-2r}, uniform directions, importance weights. */
+  static QMCGrid sobol(IndexType n, std::uint64_t seed) {
+    static_assert(NDIM == 3);
+    std::vector<REAL> p(3 * n), w(n);
+    detail::check(tcscf_sobol_grid(n, seed, p.data(), w.data()));
+    return QMCGrid(std::move(p), std::move(w));
+  }
+
+  /* setBasisFunctions (src/integration/integrateAll.hpp:88-137). */
+  template <class BASIS>
+  void setBasisFunctions(const std::vector<BASIS>& basisFunctions, bool storeGradients) {
+    const auto f = detail::abi(basisFunctions);
+    nb_ = static_cast<IndexType>(f.size());
+    basisValues.resize(static_cast<std::size_t>(nGrid() * nb_));
+    if (storeGradients) basisGradients.resize(static_cast<std::size_t>(nGrid() * nb_ * 3));
+    else basisGradients.resize(0);
+    detail::check(tcscf_basis_eval(quadratureGrid.points.data(), nGrid(), f.data(), static_cast<int>(nb_),
+                                   basisValues.data(), storeGradients ? basisGradients.data() : nullptr, stream));
+    detail::hip(hipStreamSynchronize(stream));
+  }
+
+  IndexType nBasis() const { return nb_; }
+  IndexType nGrid() const { return static_cast<IndexType>(quadratureGrid.hostWeights.size()); }
+  bool gradientsStored() const { return !basisGradients.empty(); }
+
+  QuadratureGrid<REAL> quadratureGrid;
+  DeviceArray<REAL> basisValues;     /* N x nb */
+  DeviceArray<REAL> basisGradients;  /* N x nb x 3 (Cartesian) */
+  DeviceArray<int32_t> radialOrder;  /* summation order of contractions over this grid */
+  hipStream_t stream = nullptr;
+
+ private:
+  void setPoints(std::vector<REAL> p, std::vector<REAL> w) {
+    const std::size_t n = w.size();
+    if (p.size() != 3 * n) throw Error("QMCGrid: points must be 3 x N");
+    std::vector<std::pair<REAL, int32_t>> r(n);
+    for (std::size_t k = 0; k < n; ++k)
+      r[k] = {std::sqrt(p[k] * p[k] + p[n + k] * p[n + k] + p[2 * n + k] * p[2 * n + k]), static_cast<int32_t>(k)};
+    std::stable_sort(r.begin(), r.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<int32_t> ord(n);
+    for (std::size_t k = 0; k < n; ++k) ord[k] = r[k].second;
+    quadratureGrid.points.upload(p.data(), p.size());
+    quadratureGrid.weights.upload(w.data(), w.size());
+    radialOrder.upload(ord.data(), ord.size());
+    quadratureGrid.hostPoints = std::move(p);
+    quadratureGrid.hostWeights = std::move(w);
+  }
+  IndexType nb_ = 0;
+};
+
+}  // namespace integration
+
+/* ------------------------------------------------------------- the integrals */
+/* Jastrow u(r12) = c r12 / (r12 + a12) per spin pair (the r12 Ochi term of
+ * src/unitTests/HartreeFockTest.cpp:351-360); fromB(b) gives r12 / (2 (1 + b r12)). */
+struct Jastrow {
+  double a12, c_same, c_oppo;
+  static Jastrow fromB(double b) { return Jastrow{1.0 / b, 0.25 / b, 0.5 / b}; }
+  tcscf_jastrow abi() const { return tcscf_jastrow{a12, c_same, c_oppo}; }
+};
+
+enum class Precision : int { FP64 = TCSCF_FP64, FP32 = TCSCF_FP32, BF16X3 = TCSCF_BF16X3, BF16X5 = TCSCF_BF16X5 };
+
+namespace detail {
+template <class REAL, int N1, int N2>
+DeviceArray<REAL> contract(const integration::QMCGrid<REAL, N1>& r1, const integration::QMCGrid<REAL, N2>& r2,
+                           int kernel, const tcscf_jastrow* jp, int sameSpin, Precision prec) {
+  if (r1.nBasis() != r2.nBasis() && kernel != TCSCF_K_COULOMB) throw Error("computeF: different bases");
+  const int nb = static_cast<int>(r2.nBasis());
+  if (nb == 0) throw Error("computeF: setBasisFunctions was not called on the inner grid");
+  const IndexType npair = static_cast<IndexType>(nb) * (nb + 1) / 2;
+  const int ncomp = kernel == TCSCF_K_JASTROW_GRAD ? 3 : 1;
+  DeviceArray<REAL> F(static_cast<std::size_t>(r1.nGrid() * ncomp * npair));
+  const double scale = N2 == 2 ? 2 * 3.14159265358979323846 : 1.0;
+  check(tcscf_compute_F(r1.quadratureGrid.points.data(), r1.nGrid(), r2.quadratureGrid.points.data(),
+                        r2.quadratureGrid.weights.data(), r2.radialOrder.data(), r2.nGrid(), r2.basisValues.data(), nb,
+                        kernel, jp, sameSpin, static_cast<int>(prec), scale, F.data(), TCSCF_F64, r1.stream));
+  hip(hipStreamSynchronize(r1.stream));
+  return F;
+}
+}  // namespace detail
+
+/* precomputeIntegrand + computeF (src/HartreeFock.hpp:23-180) for f = 1/r12:
+ * packed F[n1][p(j, i)] = scale sum_n2 f(r1, r2) w2 phi_j(r2) phi_i(r2). */
+template <class REAL, int N1, int N2>
+DeviceArray<REAL> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N2>& r2Grid,
+                           Precision precision = Precision::FP64) {
+  return detail::contract(r1Grid, r2Grid, TCSCF_K_COULOMB, nullptr, 1, precision);
+}
+/* The TC scalar kernel 2/r12 - |grad_1 u12|^2 - |grad_2 u12|^2 of one spin pair
+ * (src/unitTests/HartreeFockTest.cpp:276-296): FjiSame / FjiOppo. */
+template <class REAL, int N1, int N2>
+DeviceArray<REAL> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N2>& r2Grid,
+                           const Jastrow& jastrow, bool sameSpin, Precision precision = Precision::FP64) {
+  const tcscf_jastrow j = jastrow.abi();
+  return detail::contract(r1Grid, r2Grid, TCSCF_K_TC_SCALAR, &j, sameSpin ? 1 : 0, precision);
+}
+/* computeV (src/HartreeFock.hpp:185-229) for c = 1: packed V[n1][xyz][p(j, i)]. */
+template <class REAL, int N1, int N3>
+DeviceArray<REAL> computeV(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N3>& r3Grid,
+                           const Jastrow& jastrow, Precision precision = Precision::FP64) {
+  const tcscf_jastrow j = jastrow.abi();
+  return detail::contract(r1Grid, r3Grid, TCSCF_K_JASTROW_GRAD, &j, 1, precision);
+}
+
+/* integrateAllR1R2<false> (src/integration/integrateAll.hpp:221-317) from F:
+ * answer(b1, b2, b3, b4), nb^4 row-major (host). */
+template <class REAL, int N1>
+std::vector<REAL> integrateAllR1R2(const integration::QMCGrid<REAL, N1>& r1Grid, const DeviceArray<REAL>& F) {
+  const int nb = static_cast<int>(r1Grid.nBasis());
+  const IndexType np = static_cast<IndexType>(nb) * (nb + 1) / 2;
+  DeviceArray<REAL> E(static_cast<std::size_t>(np * np));
+  detail::check(tcscf_eri(r1Grid.basisValues.data(), r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), nb,
+                          F.data(), E.data(), r1Grid.stream));
+  const std::vector<REAL> e = E.download();
+  auto pk = [](int j, int i) { return j <= i ? static_cast<IndexType>(i) * (i + 1) / 2 + j : static_cast<IndexType>(j) * (j + 1) / 2 + i; };
+  std::vector<REAL> out(static_cast<std::size_t>(nb) * nb * nb * nb);
+  for (int b1 = 0; b1 < nb; ++b1)
+    for (int b2 = 0; b2 < nb; ++b2)
+      for (int b3 = 0; b3 < nb; ++b3)
+        for (int b4 = 0; b4 < nb; ++b4)
+          out[((static_cast<std::size_t>(b1) * nb + b2) * nb + b3) * nb + b4] = e[pk(b1, b3) * np + pk(b2, b4)];
+  return out;
+}
+
+/* threeElectronIntegrals (src/integration/integrateAll.hpp:369-442), same-spin
+ * Jastrow: Q(p,q,r,s,t,u), nb^6 row-major (host); nb <= 16. */
+template <class REAL, int N1, int N2>
+std::vector<REAL> threeElectronIntegrals(const integration::QMCGrid<REAL, N1>& r1Grid,
+                                         const integration::QMCGrid<REAL, N2>& r2Grid, const Jastrow& jastrow) {
+  const int nb = static_cast<int>(r1Grid.nBasis());
+  DeviceArray<REAL> V = computeV(r1Grid, r2Grid, jastrow);
+  const std::size_t n6 = static_cast<std::size_t>(nb) * nb * nb * nb * nb * nb;
+  DeviceArray<REAL> Q(n6);
+  detail::check(tcscf_three_electron(r1Grid.basisValues.data(), r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(),
+                                     V.data(), nb, jastrow.c_same, Q.data(), r1Grid.stream));
+  return Q.download();
+}
+
+/* --------------------------------------------------------------- SCF solvers */
+namespace detail {
+inline tcscf_scf_opts opts(double tol, int maxIter, bool mask, int mgsMode, int diis) {
+  return tcscf_scf_opts{tol, maxIter, mask ? 1 : 0, mgsMode, diis};
+}
+inline void checkSquare(const std::vector<double>& M, int nb, const char* what) {
+  if (M.size() != static_cast<std::size_t>(nb) * nb) throw Error(std::string(what) + ": expected nb x nb");
+}
+}  // namespace detail
+
+/* RCSHartreeFock (src/HartreeFock.hpp:233-293, src/HartreeFock.cpp:785-886). */
+template <class T>
+class RCSHartreeFock {
+ public:
+  RCSHartreeFock(int nElectrons, int nBasis) : nElectrons_(nElectrons), basisSize(nBasis) {
+    if (nElectrons % 2 != 0) throw Error("RCSHartreeFock: the number of electrons must be even");
+    density.assign(static_cast<std::size_t>(nBasis) * nBasis, 0);
+    eigenvalues.assign(nBasis, 0);
+  }
+  static constexpr bool needsGradients() { return false; }
+
+  template <int N1>
+  T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
+            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<T>& Fji, bool respectOneElectronSymmetry,
+            double tol = 1e-5, int maxIter = 50, int diis = 0) {
+    detail::checkSquare(overlap, basisSize, "overlap");
+    detail::checkSquare(oneElectronTerms, basisSize, "oneElectronTerms");
+    tcscf_scf_opts o = detail::opts(tol, maxIter, respectOneElectronSymmetry, 0, diis);
+    tcscf_scf_result res{};
+    energyTrace.assign(maxIter, 0);
+    const int rc = tcscf_rcs_scf(nElectrons_, basisSize, overlap.data(), oneElectronTerms.data(), r1Grid.basisValues.data(),
+                                 r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), Fji.data(), TCSCF_F64, &o, &res,
+                                 energyTrace.data(), density.data(), eigenvalues.data(), r1Grid.stream);
+    iter_ = res.iterations;
+    energyTrace.resize(iter_ > 0 ? iter_ : 0);
+    detail::check(rc);
+    return res.energy;
+  }
+  int numberOfConvergenceLoops() const { return iter_; }
+  T highestOccupiedOrbitalEnergy() const { return eigenvalues[nElectrons_ / 2 - 1]; }
+
+  std::vector<T> density;      /* nb x nb (one spin) */
+  std::vector<T> eigenvalues;  /* ascending */
+  std::vector<T> energyTrace;
+
+ private:
+  int nElectrons_;
+
+ public:
+  const int basisSize;
+
+ private:
+  int iter_ = 0;
+};
+
+/* UOSHartreeFock (src/HartreeFock.hpp:296-350, src/HartreeFock.cpp:890-1003). */
+template <class T>
+class UOSHartreeFock {
+ public:
+  UOSHartreeFock(int nUp, int nDown, int nBasis) : nElectrons{nUp, nDown}, basisSize(nBasis) {
+    density.assign(2 * static_cast<std::size_t>(nBasis) * nBasis, 0);
+    eigenvalues.assign(2 * static_cast<std::size_t>(nBasis), 0);
+  }
+  static constexpr bool needsGradients() { return false; }
+
+  template <int N1>
+  T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
+            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<T>& Fji, bool respectOneElectronSymmetry,
+            double tol = 1e-5, int maxIter = 100, int diis = 0) {
+    detail::checkSquare(overlap, basisSize, "overlap");
+    detail::checkSquare(oneElectronTerms, basisSize, "oneElectronTerms");
+    tcscf_scf_opts o = detail::opts(tol, maxIter, respectOneElectronSymmetry, 0, diis);
+    tcscf_scf_result res{};
+    energyTrace.assign(maxIter, 0);
+    const int rc = tcscf_uos_scf(nElectrons[0], nElectrons[1], basisSize, overlap.data(), oneElectronTerms.data(),
+                                 r1Grid.basisValues.data(), r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(),
+                                 Fji.data(), TCSCF_F64, &o, &res, energyTrace.data(), density.data(),
+                                 eigenvalues.data(), r1Grid.stream);
+    iter_ = res.iterations;
+    energyTrace.resize(iter_ > 0 ? iter_ : 0);
+    detail::check(rc);
+    return res.energy;
+  }
+  int numberOfConvergenceLoops() const { return iter_; }
+  T highestOccupiedOrbitalEnergy() const {
+    T e = -INFINITY;
+    for (int s = 0; s < 2; ++s)
+      if (nElectrons[s] > 0) e = std::max(e, eigenvalues[static_cast<std::size_t>(s) * basisSize + nElectrons[s] - 1]);
+    return e;
+  }
+
+  const int nElectrons[2];
+  const int basisSize;
+  std::vector<T> density;      /* 2 x nb x nb */
+  std::vector<T> eigenvalues;  /* 2 x nb, ascending per spin */
+  std::vector<T> energyTrace;
+
+ private:
+  int iter_ = 0;
+};
+
+/* TCHartreeFock (src/HartreeFock.hpp:353-418, src/HartreeFock.cpp:1007-1362).
+ * The reference takes VjiSame and VjiOppo; here one c = 1 tensor V and the
+ * Jastrow's c_same / c_oppo (V_same = c_same V, V_oppo = c_oppo V). */
+template <class T>
+class TCHartreeFock {
+ public:
+  TCHartreeFock(int nUp, int nDown, int nBasis) : nElectrons{nUp, nDown}, basisSize(nBasis) {
+    density.assign(2 * static_cast<std::size_t>(nBasis) * nBasis, 0);
+    occupiedOrbitalPseudoEnergy.assign(static_cast<std::size_t>(nUp + nDown), 0);
+  }
+  static constexpr bool needsGradients() { return true; }
+
+  /* mgsMode 0: the reference's Gram-Schmidt (Euclidean norm after the first
+   * vector, src/orthogonalization.hpp:152); 1: the overlap metric throughout. */
+  template <int N1>
+  T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
+            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<T>& FjiSame, const DeviceArray<T>& FjiOppo,
+            const DeviceArray<T>& Vji, const Jastrow& jastrow, bool respectOneElectronSymmetry, double tol = 1e-5,
+            int maxIter = 200, int mgsMode = 0, int diis = 0) {
+    detail::checkSquare(overlap, basisSize, "overlap");
+    detail::checkSquare(oneElectronTerms, basisSize, "oneElectronTerms");
+    if (!r1Grid.gradientsStored()) throw Error("TCHartreeFock: the r1 grid needs basis gradients");
+    tcscf_scf_opts o = detail::opts(tol, maxIter, respectOneElectronSymmetry, mgsMode, diis);
+    tcscf_scf_result res{};
+    const tcscf_jastrow j = jastrow.abi();
+    energyTrace.assign(maxIter, 0);
+    const int rc = tcscf_tc_scf(nElectrons[0], nElectrons[1], basisSize, overlap.data(), oneElectronTerms.data(),
+                                r1Grid.basisValues.data(), r1Grid.basisGradients.data(),
+                                r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), FjiSame.data(), FjiOppo.data(),
+                                Vji.data(), &j, &o, &res, energyTrace.data(), density.data(),
+                                occupiedOrbitalPseudoEnergy.data(), r1Grid.stream);
+    iter_ = res.iterations;
+    energyTrace.resize(iter_ > 0 ? iter_ : 0);
+    detail::check(rc);
+    return res.energy;
+  }
+  int numberOfConvergenceLoops() const { return iter_; }
+  T highestOccupiedOrbitalEnergy() const {
+    T e = -INFINITY;
+    if (nElectrons[0] > 0) e = std::max(e, occupiedOrbitalPseudoEnergy[nElectrons[0] - 1]);
+    if (nElectrons[1] > 0) e = std::max(e, occupiedOrbitalPseudoEnergy[nElectrons[0] + nElectrons[1] - 1]);
+    return e;
+  }
+
+  const int nElectrons[2];
+  const int basisSize;
+  std::vector<T> density;                      /* 2 x nb x nb */
+  std::vector<T> occupiedOrbitalPseudoEnergy;  /* nUp then nDown, ascending Re(lambda) */
+  std::vector<T> energyTrace;
+
+ private:
+  int iter_ = 0;
+};
+
+}  // namespace tcscf

@@ -128,6 +128,15 @@ int tcscf_sobol_points(int64_t n, int dim, int scramble, uint64_t seed, double* 
  * (density 2 e^{-2r}), uniform directions, w = 4 pi r^2 / (2 n e^{-2r}). */
 int tcscf_sobol_grid(int64_t n, uint64_t seed, double* pts_host, double* w_host);
 
+/* loadSTO (src/SlaterTypeOrbital.cpp:24-76): reads the BASIS block of the ADF
+ * file <directory>/<basis_set>/<element> (basis_set one of TZ3P, QZ6P, ATZ3P,
+ * AQZ6P), expands each shell to m = -l..l and skips l > max_l.  Returns the
+ * function count (writing at most `capacity` to out_host; out_host may be NULL
+ * with capacity 0 to query), or < 0.  A missing element file gives 0 functions,
+ * as in the reference. */
+int tcscf_load_sto(const char* directory, const char* basis_set, const char* element, int max_l, tcscf_sto* out_host,
+                   int capacity);
+
 /* ---------------------------------------------------------------- host one-electron */
 /* overlap / coreMatrixElement: src/SlaterTypeOrbital.hpp:125-171.  nb x nb row-major. */
 int tcscf_sto_overlap_core(const tcscf_sto* fns, int nb, int Z, double* S_host, double* H_host);

@@ -8,7 +8,7 @@ from ._lib import (BF16X3, BF16X5, FP32, FP64, K_COULOMB, K_JASTROW_GRAD, K_TC_S
                    RADIAL_XOVER1MX, ConvergenceError, SolverError, TcscfError, lib)
 from .api import (Jastrow, QMCGrid, QuadratureGrid, RCSHartreeFock, SlaterTypeOrbital, TCHartreeFock, UOSHartreeFock,
                   coreMatrixElement, computeF, computeV, eri_full, eri_full_gradient, even_tempered,
-                  integrateAllR1R2, integrateAllR1R2Gradient, lebedev, npair, one_electron_matrices, overlap,
+                  integrateAllR1R2, integrateAllR1R2Gradient, lebedev, loadSTO, npair, one_electron_matrices, overlap,
                   pair_index, product_grid, qmc_lattice_grid, radial_grid, sobol_grid, sobol_points,
                   threeElectronIntegrals, unpack)
 
@@ -17,6 +17,6 @@ __all__ = [
     "RADIAL_XOVER1MX", "ConvergenceError", "SolverError", "TcscfError", "lib", "Jastrow", "QMCGrid",
     "QuadratureGrid", "RCSHartreeFock", "SlaterTypeOrbital", "TCHartreeFock", "UOSHartreeFock", "coreMatrixElement",
     "computeF", "computeV", "eri_full", "eri_full_gradient", "even_tempered", "integrateAllR1R2",
-    "integrateAllR1R2Gradient", "lebedev", "npair", "one_electron_matrices", "overlap", "pair_index", "product_grid",
+    "integrateAllR1R2Gradient", "lebedev", "loadSTO", "npair", "one_electron_matrices", "overlap", "pair_index", "product_grid",
     "qmc_lattice_grid", "radial_grid", "sobol_grid", "sobol_points", "threeElectronIntegrals", "unpack",
 ]

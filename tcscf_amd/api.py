@@ -93,6 +93,17 @@ def _sto_array(basis):
     return arr
 
 
+def loadSTO(directory, basisSet, element, maxL=2 ** 31 - 1):
+    """loadSTO (src/SlaterTypeOrbital.cpp:24-76): the STOs of an ADF basis file
+    <directory>/<basisSet>/<element> (TZ3P, QZ6P, ATZ3P or AQZ6P), shells
+    expanded to m = -l..l, l > maxL skipped."""
+    args = (str(directory).encode(), str(basisSet).encode(), str(element).encode(), int(min(maxL, 2 ** 31 - 1)))
+    n = check(lib().tcscf_load_sto(*args, None, 0))
+    arr = (_lib.Sto * max(n, 1))()
+    check(lib().tcscf_load_sto(*args, arr, n))
+    return [SlaterTypeOrbital(arr[k].zeta, arr[k].n, arr[k].l, arr[k].m) for k in range(n)]
+
+
 def even_tempered(shells, zeta0, ratio):
     """Even-tempered STOs: shells = {l: count}; zeta_k = zeta0 * ratio**k per l,
     n = l + 1, m = -l..l (all m of one exponent adjacent)."""

@@ -55,7 +55,25 @@ def build(verbose=False, jobs=None):
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-mcode-object-version=5", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_facade_test(verbose)
     return LIB
+
+
+def build_facade_test(verbose=False):
+    """The C++ facade driver (tests/cpp/facade_test.cpp over include/tcscf.hpp),
+    linked against the in-tree library; rpath $ORIGIN/.. finds it there."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "facade_test.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(HERE, "_build", "facade_test")
+    cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), src, "-o", out + ".tmp",
+           "-L", HERE, "-ltcscf_hip", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

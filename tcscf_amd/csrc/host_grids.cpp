@@ -4,7 +4,10 @@
 // (device kernels, the CPU reference path) integrates over identical points.
 #include <cmath>
 #include <cstdint>
+#include <filesystem>
+#include <fstream>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -315,6 +318,45 @@ int tcscf_sobol_grid(int64_t n, uint64_t seed, double* pts, double* w) {
     w[i] = 4 * kPi * r * r / ((double)n * 2 * std::exp(-2 * r));
   }
   return 0;
+}
+
+// loadSTO (src/SlaterTypeOrbital.cpp:24-76): the BASIS ... END block of an ADF
+// basis file <directory>/<basis_set>/<element>, lines "<n><SPDF> <alpha>", each
+// shell expanded to m = -l..l; shells with l > max_l are skipped.  Returns the
+// number of functions (writes at most `capacity` of them to out).
+int tcscf_load_sto(const char* directory, const char* basis_set, const char* element, int max_l, tcscf_sto* out,
+                   int capacity) {
+  TC_CHECK(directory && basis_set && element && capacity >= 0 && (out || capacity == 0), "tcscf_load_sto: bad arguments");
+  const std::string set = basis_set;
+  TC_CHECK(set == "TZ3P" || set == "QZ6P" || set == "ATZ3P" || set == "AQZ6P",
+           "tcscf_load_sto: '" + set + "' is not a valid basis set");
+  const std::filesystem::path dir = std::filesystem::path(directory) / set;
+  TC_CHECK(std::filesystem::is_directory(dir), "tcscf_load_sto: '" + set + "' not found in '" + directory + "'");
+  std::ifstream in(dir / element);
+  int count = 0;
+  bool in_block = false;
+  for (std::string line; std::getline(in, line);) {
+    if (!in_block) {
+      in_block = line.rfind("BASIS", 0) == 0;
+      continue;
+    }
+    if (line.rfind("END", 0) == 0) return count;
+    std::istringstream ls(line);
+    int n;
+    char type;
+    double alpha;
+    if (ls >> n >> type >> alpha) {
+      const int l = type == 'S' ? 0 : type == 'P' ? 1 : type == 'D' ? 2 : type == 'F' ? 3 : -1;
+      TC_CHECK(l >= 0, std::string("tcscf_load_sto: unrecognized type: ") + type);
+      if (l > max_l) continue;
+      for (int m = -l; m <= l; ++m, ++count)
+        if (count < capacity) out[count] = tcscf_sto{n, l, m, alpha};
+    } else {
+      TC_CHECK(line.empty(), "tcscf_load_sto: invalid line in file: " + line);
+    }
+  }
+  TC_CHECK(!in_block, "tcscf_load_sto: BASIS block without END");
+  return count;  // no BASIS block (or no such file): no functions, as the reference returns
 }
 
 // S_ab = N_a N_b (n_a+n_b)!/(z_a+z_b)^(n_a+n_b+1) delta_{l l'} delta_{m m'};

@@ -1,0 +1,145 @@
+// Drives the C++ host facade (include/tcscf.hpp) the way the reference's
+// drivers drive src/HartreeFock.hpp (src/unitTests/HartreeFockTest.cpp:238-447).
+//
+//   facade_test cpu <basis-dir>   host-only checks (no GPU needed)
+//   facade_test gpu <basis-dir>   He / Ne / TC runs on the device; prints one
+//                                 "key value" line per result for the pytest
+//                                 wrapper to compare with the Python path
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "tcscf.hpp"
+
+using tcscf::SlaterTypeOrbital;
+using Grid = tcscf::integration::QMCGrid<double, 3>;
+
+static int failures = 0;
+#define EXPECT(cond)                                                   \
+  do {                                                                 \
+    if (!(cond)) {                                                     \
+      std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);      \
+      ++failures;                                                      \
+    }                                                                  \
+  } while (0)
+
+static std::vector<SlaterTypeOrbital<double>> he_config1() {
+  std::vector<SlaterTypeOrbital<double>> b;
+  for (int k = 0; k < 5; ++k) b.emplace_back(0.5 * std::pow(2.0, k), 1, 0, 0);
+  return b;
+}
+
+static int cpu_checks(const std::string& dir) {
+  // loadSTO: TZ3P Ne = 1S x2, 2S x3, 2P x3, 3D x2, 4F x1 -> 5 + 9 + 10 + 7 = 31
+  const auto ne = tcscf::loadSTO(dir, "TZ3P", "Ne");
+  EXPECT(ne.size() == 31);
+  if (ne.size() == 31) {
+    EXPECT(ne[0].n == 1 && ne[0].l == 0 && std::fabs(ne[0].alpha - 12.45) < 1e-15);
+    EXPECT(ne[5].n == 2 && ne[5].l == 1 && ne[5].m == -1 && std::fabs(ne[5].alpha - 5.50) < 1e-15);
+    EXPECT(ne[30].n == 4 && ne[30].l == 3 && ne[30].m == 3 && std::fabs(ne[30].alpha - 3.00) < 1e-15);
+  }
+  EXPECT(tcscf::loadSTO(dir, "TZ3P", "Ne", 1).size() == 14);
+  EXPECT(tcscf::loadSTO(dir, "QZ6P", "Ne").size() == 8 + 15 + 15 + 21);
+  EXPECT(tcscf::loadSTO(dir, "TZ3P", "NoSuchElement").empty());
+  bool threw = false;
+  try {
+    tcscf::loadSTO(dir, "DZP", "Ne");
+  } catch (const tcscf::Error&) {
+    threw = true;
+  }
+  EXPECT(threw);
+  // overlap / coreMatrixElement closed forms for 1s functions
+  const SlaterTypeOrbital<double> a(0.8, 1, 0, 0), b(1.7, 1, 0, 0);
+  const double Sab = 8 * std::pow(0.8 * 1.7, 1.5) / std::pow(0.8 + 1.7, 3);
+  EXPECT(std::fabs(tcscf::overlap(a, b) - Sab) < 1e-14);
+  EXPECT(std::fabs(tcscf::overlap(a, a) - 1) < 1e-14);
+  // <1s|h|1s> for one exponent zeta, nucleus Z: zeta^2/2 - Z zeta
+  EXPECT(std::fabs(tcscf::coreMatrixElement(2, a, a) - (0.32 - 1.6)) < 1e-14);
+  threw = false;
+  try {
+    SlaterTypeOrbital<double> bad(1.0, 1, 2, 0);  // n < l
+  } catch (const tcscf::Error&) {
+    threw = true;
+  }
+  EXPECT(threw);
+  threw = false;
+  try {
+    tcscf::RCSHartreeFock<double> odd(3, 5);
+  } catch (const tcscf::Error&) {
+    threw = true;
+  }
+  EXPECT(threw);
+  std::printf("cpu checks: %d failure(s)\n", failures);
+  return failures ? 1 : 0;
+}
+
+static int gpu_runs(const std::string& dir) {
+  // He, config 1: RHF and UHF, FP64, mask off, tol 1e-10
+  {
+    const auto basis = he_config1();
+    Grid g = Grid::radialLebedev(25, 11, TCSCF_RADIAL_BECKE);
+    g.setBasisFunctions(basis, false);
+    std::vector<double> S, H;
+    tcscf::oneElectronMatrices(basis, 2, S, H);
+    auto F = tcscf::computeF(g, g);
+    tcscf::RCSHartreeFock<double> rhf(2, 5);
+    const double e = rhf.compute(S, H, g, F, false, 1e-10);
+    tcscf::UOSHartreeFock<double> uhf(1, 1, 5);
+    const double eu = uhf.compute(S, H, g, F, false, 1e-10);
+    std::printf("he_rhf %.15f\nhe_uhf %.15f\nhe_rhf_iters %d\n", e, eu, rhf.numberOfConvergenceLoops());
+    auto eri = tcscf::integrateAllR1R2(g, F);
+    std::printf("he_eri_0000 %.15f\nhe_eri_0123 %.15f\n", eri[0], eri[((0 * 5 + 1) * 5 + 2) * 5 + 3]);
+  }
+  // Ne with the TZ3P STOs (loadSTO, s/p/d/f), 40 x 302 grid: FP64 and BF16x5
+  {
+    const auto basis = tcscf::loadSTO(dir, "TZ3P", "Ne");
+    Grid g = Grid::radialLebedev(40, 29, TCSCF_RADIAL_M4);
+    g.setBasisFunctions(basis, false);
+    std::vector<double> S, H;
+    tcscf::oneElectronMatrices(basis, 10, S, H);
+    const int nb = static_cast<int>(basis.size());
+    for (auto prec : {tcscf::Precision::FP64, tcscf::Precision::BF16X5}) {
+      auto F = tcscf::computeF(g, g, prec);
+      tcscf::RCSHartreeFock<double> rhf(10, nb);
+      const double e = rhf.compute(S, H, g, F, true, 1e-10, 100, 6);
+      std::printf("ne_tz3p_%s %.15f\n", prec == tcscf::Precision::FP64 ? "fp64" : "bf16x5", e);
+    }
+  }
+  // TC Be (2 + 2 electrons) on small grids, b = 1, metric MGS
+  {
+    std::vector<SlaterTypeOrbital<double>> basis;
+    for (int k = 0; k < 5; ++k) basis.emplace_back(0.6 * std::pow(2.2, k), 1, 0, 0);
+    for (int k = 0; k < 2; ++k)
+      for (int m = -1; m <= 1; ++m) basis.emplace_back(0.6 * std::pow(2.2, k), 2, 1, m);
+    Grid g1 = Grid::radialLebedev(14, 11, TCSCF_RADIAL_M4);
+    Grid g3 = Grid::radialLebedev(10, 7, TCSCF_RADIAL_M4);
+    g1.setBasisFunctions(basis, true);
+    g3.setBasisFunctions(basis, true);
+    const auto jas = tcscf::Jastrow::fromB(1.0);
+    auto Fs = tcscf::computeF(g1, g1, jas, true);
+    auto Fo = tcscf::computeF(g1, g1, jas, false);
+    auto V = tcscf::computeV(g1, g3, jas);
+    std::vector<double> S, H;
+    tcscf::oneElectronMatrices(basis, 4, S, H);
+    tcscf::TCHartreeFock<double> tc(2, 2, static_cast<int>(basis.size()));
+    const double e = tc.compute(S, H, g1, Fs, Fo, V, jas, true, 1e-10, 200, 1);
+    std::printf("be_tc %.15f\nbe_tc_iters %d\n", e, tc.numberOfConvergenceLoops());
+  }
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::printf("usage: facade_test cpu|gpu <basis-dir>\n");
+    return 2;
+  }
+  const std::string mode = argv[1], dir = argv[2];
+  try {
+    return mode == "cpu" ? cpu_checks(dir) : gpu_runs(dir);
+  } catch (const std::exception& e) {
+    std::printf("exception: %s\n", e.what());
+    return 1;
+  }
+}

@@ -147,3 +147,40 @@ def test_qmc_lattice_grid(tc):
     assert abs(2 * math.pi * (w2 * np.exp(-2 * r2) / math.pi).sum() - 1) < 1e-6
     a, _ = tc.qmc_lattice_grid(5000, 3, 2, seed=2)
     assert not np.array_equal(a, pts)
+
+
+# ------------------------------------------------------ loadSTO and C++ facade
+BASIS_DIR = os.path.join(ROOT, "tests", "golden", "basisFunctions")
+
+
+def test_load_sto(tc):
+    """loadSTO (src/SlaterTypeOrbital.cpp:24-76) on copies of the reference's
+    ADF basis files: shells expanded to m = -l..l in file order, maxL filter,
+    invalid set names rejected, a missing element gives no functions."""
+    ne = tc.loadSTO(BASIS_DIR, "TZ3P", "Ne")
+    shells = [(1, "S", 12.45), (1, "S", 8.70), (2, "S", 3.65), (2, "S", 2.20), (2, "S", 1.55), (2, "P", 5.50),
+              (2, "P", 2.85), (2, "P", 1.45), (3, "D", 4.00), (3, "D", 2.00), (4, "F", 3.00)]
+    expect = [(n, "SPDF".index(t), m, a) for n, t, a in shells for m in range(-"SPDF".index(t), "SPDF".index(t) + 1)]
+    assert [(b.n, b.l, b.m, b.alpha) for b in ne] == expect
+    assert len(tc.loadSTO(BASIS_DIR, "TZ3P", "Ne", maxL=1)) == 14
+    assert len(tc.loadSTO(BASIS_DIR, "QZ6P", "Ne")) == 59  # indented lines and ".950" parse
+    he = tc.loadSTO(BASIS_DIR, "TZ3P", "He")
+    assert [(b.n, b.l) for b in he][:3] == [(1, 0)] * 3 and len(he) == 3 + 6 + 5
+    assert tc.loadSTO(BASIS_DIR, "TZ3P", "Xx") == []
+    with pytest.raises(tc.TcscfError):
+        tc.loadSTO(BASIS_DIR, "DZP", "Ne")
+
+
+def test_cpp_facade_host_checks(tc):
+    """The C++ host facade (include/tcscf.hpp) compiles against the C ABI and
+    its host-side pieces behave (loadSTO, STO checks, S/H closed forms, RCS
+    electron-count check)."""
+    import subprocess
+    from tcscf_amd import build
+    exe = os.path.join(ROOT, "tcscf_amd", "_build", "facade_test")
+    srcs = [os.path.join(ROOT, "tests", "cpp", "facade_test.cpp"), os.path.join(ROOT, "include", "tcscf.hpp")]
+    if not os.path.exists(exe) or os.path.getmtime(exe) < max(os.path.getmtime(s) for s in srcs):
+        build.build_facade_test()
+    r = subprocess.run([exe, "cpu", BASIS_DIR], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failure(s)" in r.stdout

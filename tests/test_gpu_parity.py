@@ -524,3 +524,47 @@ def test_tc_qmc_ne_config4_mixed_vs_fp64(tc, gpu):
         del Fs, Fo, V
     print(f"QMC Ne TC: E(bf16x5) = {E['bf16x5']:.10f}, E(fp64) = {E['fp64']:.10f}")
     assert abs(E["bf16x5"] - E["fp64"]) < 1e-6
+
+
+def test_cpp_facade_on_device(tc, O, gpu):
+    """The C++ facade (include/tcscf.hpp, driven by tests/cpp/facade_test.cpp)
+    against the oracle and the Python mirror: He config 1 RHF/UHF, the grid
+    ERI, Ne with the TZ3P basis from loadSTO (FP64 and BF16x5) and TC Be."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    from tcscf_amd import SlaterTypeOrbital as STO, configs
+    exe = os.path.join(ROOT, "tcscf_amd", "_build", "facade_test")
+    basis_dir = os.path.join(ROOT, "tests", "golden", "basisFunctions")
+    print("step: facade_test gpu", flush=True)
+    r = subprocess.run([exe, "gpu", basis_dir], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {k: float(v) for k, v in (line.split() for line in r.stdout.splitlines() if len(line.split()) == 2)}
+    c = configs.he_config1()
+    g = _grid(tc, *c["grid"])
+    g.setBasisFunctions(c["basis"], False)
+    S, H = tc.one_electron_matrices(c["basis"], c["Z"])
+    A = O.sto_real(g.points_host, fns_of(c["basis"]))
+    Fo = O.compute_F(g.points_host, g.points_host, g.weights_host, A)
+    ref = O.rcs_scf(2, S, H, A, g.weights_host, Fo, tol=1e-10, mask=False)
+    assert abs(out["he_rhf"] - ref["energy"]) < 1e-10
+    assert abs(out["he_uhf"] - out["he_rhf"]) < 1e-10
+    eri = O.eri_from_F(A, g.weights_host, Fo)
+    assert abs(out["he_eri_0000"] - eri[0, 0, 0, 0]) < 1e-12 * abs(eri[0, 0, 0, 0])
+    assert abs(out["he_eri_0123"] - eri[0, 1, 2, 3]) < 1e-12 * abs(eri[0, 0, 0, 0])
+    # Ne TZ3P through the Python mirror
+    basis = tc.loadSTO(basis_dir, "TZ3P", "Ne")
+    gn = _grid(tc, 40, 29, tc.RADIAL_M4)
+    gn.setBasisFunctions(basis, False)
+    Sn, Hn = tc.one_electron_matrices(basis, 10)
+    e64 = tc.RCSHartreeFock(10, len(basis)).compute(Sn, Hn, gn, tc.computeF(gn, gn), True, tol=1e-10, max_iter=100,
+                                                     diis=6)
+    assert abs(out["ne_tz3p_fp64"] - e64) < 1e-9
+    assert abs(out["ne_tz3p_bf16x5"] - e64) < 1e-6
+    # TC Be
+    bb = [STO(0.6 * 2.2 ** k, 1, 0, 0) for k in range(5)]
+    bb += [STO(0.6 * 2.2 ** k, 2, 1, m) for k in range(2) for m in (-1, 0, 1)]
+    g1, g3, jas, Fs, Fo2, V = _tc_case(tc, bb, (14, 11, tc.RADIAL_M4), (10, 7, tc.RADIAL_M4))
+    Sb, Hb = tc.one_electron_matrices(bb, 4)
+    etc = tc.TCHartreeFock(2, 2, len(bb)).compute(Sb, Hb, g1, Fs, Fo2, V, jas, True, tol=1e-10, mgs_mode=1)
+    assert abs(out["be_tc"] - etc) < 1e-9
