@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 METRIC = "SCF wall time + grid-pair contractions/s: mixed-prec MFMA vs FP64 GPU @1e-6 Eh"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x5": 2500.0}
 PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x5": 5}
+# HBM bytes per launch of the contraction kernel on this workload, from rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --pmc-probe`, each scaled by the
+# reading for a 512 MiB copy (FETCH x2.0, WRITE x1.0); profiles/bf16_fgemm_ne30_traffic.md
+TRAFFIC_BYTES = {"bf16x5": 2.936e9 + 1.145e8, "fp64": 1.016e9 + 1.156e8}
 
 
 def parse():
@@ -42,6 +46,9 @@ def parse():
     p.add_argument("--no-tc", action="store_true", help="skip the TC-HF Ne side measurement")
     p.add_argument("--tc-mgs", type=int, default=1, help="TC Gram-Schmidt: 0 reference, 1 metric")
     p.add_argument("--tc-diis", type=int, default=6, help="TC DIIS depth (the TC workload asks for 6; 0 = plain)")
+    p.add_argument("--pmc-probe", action="store_true",
+                   help="for rocprofv3 --pmc runs: one BF16x5 and one FP64 computeF on the Ne grid, one RHF "
+                        "solve, then a 512 MiB device copy whose counter reading calibrates the byte counters")
     return p.parse_args()
 
 
@@ -60,6 +67,9 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     cfg = configs.ne_config2()
+    if a.pmc_probe:
+        pmc_probe(tc, cfg, torch)
+        return
     nrad, order, kind = cfg["grid"]
     basis = cfg["basis"]
     nb = len(basis)
@@ -105,7 +115,8 @@ def main():
     achieved = flops / (k_mix * 1e-3) / 1e12
     peak = PEAK_TFLOPS[a.precision]
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None,
+                "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get(a.precision),
+                "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                 "issued_frac": round(achieved * PRODUCTS[a.precision] / peak, 4),
                 "kernel": "fgemm (fused 1/r12 generation + contraction), computeF wall incl. packing",
                 "kernel_ms": round(k_mix, 4)}
@@ -117,17 +128,26 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         from oracle import oracle as O
-        O.set_threads(os.cpu_count() or 1)
+        O.set_threads(int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0)) or 1)
         threads = O.get_threads()
-        rows = a.cpu_rows or max(8, min(512, 4 * threads))
         pts = grid.points_host
         A = grid.basisValues.cpu().numpy()
-        t0 = time.perf_counter()
-        O.compute_F(pts, pts, grid.weights_host, A, rows=(0, rows))
-        tc_cpu = time.perf_counter() - t0
+
+        def cpu_rows(rows):
+            t0 = time.perf_counter()
+            O.compute_F(pts, pts, grid.weights_host, A, rows=(0, rows))
+            return time.perf_counter() - t0
+
+        if a.cpu_rows:
+            rows = a.cpu_rows
+        else:  # size the sample to 10-20 s of CPU work
+            probe = max(8, 2 * threads)
+            rows = int(min(n, max(probe, probe * 20.0 / max(cpu_rows(probe), 1e-3))))
+        tc_cpu = cpu_rows(rows)
         cpu = {"value": rows * n / tc_cpu, "unit": "grid-pair contractions/s", "cores": threads, "kind": "port",
-               "sample": f"oracle computeF (FP64, OpenMP) on r1 rows 0..{rows} x all {n} r2 points, nb={nb}: "
-                         f"{tc_cpu:.2f} s; full precompute extrapolates to {tc_cpu * n / rows:.0f} s"}
+               "sample": f"oracle computeF (FP64, OpenMP, {threads} threads) on r1 rows 0..{rows} x all {n} r2 "
+                         f"points, nb={nb}: {tc_cpu:.2f} s; the full precompute extrapolates to "
+                         f"{tc_cpu * n / rows:.0f} s"}
 
     res = {
         "metric": METRIC,
@@ -204,6 +224,26 @@ def run_tc(a, tc, configs, torch):
             "pairs_x_components": 2 * n1 * (n1 - 1) + 3 * n1 * n3,
             a.precision: mix, "fp64": f64, "energy_dev_vs_fp64": mix["energy"] - f64["energy"],
             "speedup_vs_fp64": f64["total_ms"] / mix["total_ms"]}
+
+
+def pmc_probe(tc, cfg, torch):
+    """Kernels to read hardware counters on: the contraction at both
+    precisions, one RHF solve (the HBM-bound sweep) and a plain 512 MiB copy of
+    known traffic (read 512 MiB, write 512 MiB) to scale FETCH_SIZE/WRITE_SIZE."""
+    g = tc.QMCGrid.radial_lebedev(*cfg["grid"], 1.0, 1.0)
+    g.setBasisFunctions(cfg["basis"], False)
+    F = tc.computeF(g, g, precision="bf16x5")
+    torch.cuda.synchronize()
+    F64 = tc.computeF(g, g, precision="fp64")
+    torch.cuda.synchronize()
+    S, H = tc.one_electron_matrices(cfg["basis"], cfg["Z"])
+    e = tc.RCSHartreeFock(10, len(cfg["basis"])).compute(S, H, g, F, True, tol=1e-9)
+    x = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del F64
+    print(json.dumps({"pmc_probe": "done", "energy": e}), flush=True)
 
 
 if __name__ == "__main__":
