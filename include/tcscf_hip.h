@@ -166,6 +166,36 @@ int tcscf_compute_F(const double* pts1_dev, int64_t n1, const double* pts2_dev, 
                     int out_dtype, void* stream);
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
 
+/* One instance of a batched contraction: the arguments of tcscf_compute_F
+ * (FP64 output, packed n1 x ncomp x npair; scale = 2 pi for a 2-D r2 grid). */
+typedef struct tcscf_batch_item {
+  const double* pts1_dev;
+  int64_t n1;
+  const double* pts2_dev;
+  const double* w2_dev;
+  const int32_t* ord2_dev;
+  int64_t n2;
+  const double* A2_dev;
+  int nb;
+  int kernel;
+  tcscf_jastrow jastrow;
+  int same_spin;
+  double scale;
+  double* F_dev;
+} tcscf_batch_item;
+
+/* No single reference counterpart: computeF / computeV (src/HartreeFock.hpp:
+ * 88-229) for many instances of uneven size (atoms, bases, grids, kernels) at
+ * once.  BF16X3 / BF16X5: one persistent launch over a work queue of
+ * (instance, row block, 256-pair block, component) tiles, largest instances
+ * first; the rows per tile (16 or 32) come from an online autotuner that times
+ * both on the head of the queue the first time a precision is used in the
+ * process.  Results equal tcscf_compute_F's bit for bit.  FP64 / FP32: one
+ * launch per instance.  Synchronizes the stream before returning. */
+int tcscf_batch_compute_F(const tcscf_batch_item* items_host, int count, int precision, void* stream);
+/* Rows per tile the autotuner chose (16 or 32; 0 before the first batch). */
+int tcscf_batch_tuned_rows(int precision);
+
 /* integrateAllR1R2<false> (src/integration/integrateAll.hpp:221-317) from a
  * computed F: ERI[p(b1,b3)][p(b2,b4)] = sum_n1 w1 A1[n1,b1] A1[n1,b3] F[n1,p(b2,b4)]
  * (chemist (b1 b3 | b2 b4); the reference's answer(b1,b2,b3,b4)).  F_dev packed

@@ -24,6 +24,7 @@ EXPORTS = [
     "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
     "tcscf_F_workspace_bytes", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
     "tcscf_qmc_lattice_npts", "tcscf_qmc_lattice_grid", "tcscf_sobol_points", "tcscf_sobol_grid", "tcscf_load_sto",
+    "tcscf_batch_compute_F", "tcscf_batch_tuned_rows",
     "tcscf_selftest_mfma",
 ]
 
@@ -48,6 +49,13 @@ class Sto(ctypes.Structure):
 
 class Jastrow(ctypes.Structure):
     _fields_ = [("a12", ctypes.c_double), ("c_same", ctypes.c_double), ("c_oppo", ctypes.c_double)]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("pts1_dev", ctypes.c_void_p), ("n1", ctypes.c_int64), ("pts2_dev", ctypes.c_void_p),
+                ("w2_dev", ctypes.c_void_p), ("ord2_dev", ctypes.c_void_p), ("n2", ctypes.c_int64),
+                ("A2_dev", ctypes.c_void_p), ("nb", ctypes.c_int), ("kernel", ctypes.c_int), ("jastrow", Jastrow),
+                ("same_spin", ctypes.c_int), ("scale", ctypes.c_double), ("F_dev", ctypes.c_void_p)]
 
 
 class ScfOpts(ctypes.Structure):

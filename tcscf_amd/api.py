@@ -322,6 +322,41 @@ def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64", ordere
     return _compute(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, True, precision, out_dtype, ordered)
 
 
+def computeF_batch(specs, precision="bf16x5"):
+    """computeF / computeV for many instances in one call (BF16: one persistent
+    launch over a work queue of tiles of every instance, results equal to the
+    single calls bit for bit).  specs: list of dicts with keys r1, r2, kernel
+    ('coulomb', 'tc' or 'grad'), jastrow (TC kernels) and same_spin.  Returns
+    the packed FP64 tensors in spec order."""
+    torch = _torch()
+    prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    kinds = {"coulomb": _lib.K_COULOMB, "tc": _lib.K_TC_SCALAR, "grad": _lib.K_JASTROW_GRAD}
+    items = (_lib.BatchItem * len(specs))()
+    outs = []
+    for k, sp in enumerate(specs):
+        r1, r2 = sp["r1"], sp["r2"]
+        kind = kinds[sp.get("kernel", "coulomb")]
+        if r1.nBasis() != r2.nBasis() or r2.basisValues is None:
+            raise ValueError("computeF_batch: both grids need the same tabulated basis")
+        nb, n1 = r2.nBasis(), r1.nGrid()
+        shape = (n1, npair(nb)) if kind != _lib.K_JASTROW_GRAD else (n1, 3, npair(nb))
+        F = torch.empty(shape, dtype=torch.float64, device=r1.quadratureGrid.points.device)
+        outs.append(F)
+        it = items[k]
+        it.pts1_dev, it.n1 = r1.quadratureGrid.points.data_ptr(), n1
+        it.pts2_dev, it.w2_dev = r2.quadratureGrid.points.data_ptr(), r2.quadratureGrid.weights.data_ptr()
+        it.ord2_dev, it.n2 = r2.radial_order.data_ptr(), r2.nGrid()
+        it.A2_dev, it.nb, it.kernel = r2.basisValues.data_ptr(), nb, kind
+        jas = sp.get("jastrow")
+        if jas is not None:
+            it.jastrow = jas._struct()
+        it.same_spin = int(bool(sp.get("same_spin", True)))
+        it.scale = 2 * math.pi if r2.ndim == 2 else 1.0
+        it.F_dev = F.data_ptr()
+    check(lib().tcscf_batch_compute_F(items, len(specs), prec, _stream()))
+    return outs
+
+
 def integrateAllR1R2(r1Grid, F):
     """Grid ERI tensor (src/integration/integrateAll.hpp:221-317, USE_GRADIENT =
     false) from F = computeF(r1Grid, r2Grid, ...): returns the packed

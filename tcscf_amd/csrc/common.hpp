@@ -40,6 +40,14 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 // one stream (every entry point documents the slots it uses).
 void* scratch(int slot, size_t bytes, int* rc);
 
-enum ScratchSlot { kSlotPairs = 0, kSlotCoords = 1, kSlotSweep = 2, kSlotSweepOut = 3, kSlotSmall = 4, kNumSlots = 8 };
+enum ScratchSlot {
+  kSlotPairs = 0,
+  kSlotCoords = 1,
+  kSlotSweep = 2,
+  kSlotSweepOut = 3,
+  kSlotSmall = 4,
+  kSlotBatchMeta = 5,  // + 0 / 1: batch descriptors for the two tile heights
+  kNumSlots = 8
+};
 
 }  // namespace tc

@@ -8,11 +8,14 @@
 
 #include <type_traits>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <numeric>
 #include <vector>
 
 #include "common.hpp"
@@ -274,11 +277,32 @@ __device__ inline double rsqrt_f64(double s) {
 __device__ inline double rsqrt_any(double s) { return rsqrt_f64(s); }
 __device__ inline float rsqrt_any(float s) { return __builtin_amdgcn_rsqf(s); }
 
+// The kernel formulas, each with one fixed rounding sequence (contraction off),
+// so every kernel functor -- including the batched one -- produces the same bits.
+template <class T>
+__device__ inline T dist2(T dx, T dy, T dz) {
+#pragma clang fp contract(off)
+  return dx * dx + dy * dy + dz * dz;
+}
+template <class T>
+__device__ inline T k_tc_scalar(T s, T y, double a12, double c2a2) {
+#pragma clang fp contract(off)
+  const T t = s * y + T(a12);
+  const T t2 = t * t;
+  return T(2) * y - T(c2a2) / (t2 * t2);
+}
+template <class T>
+__device__ inline T k_grad_scale(T s, T y, double a12) {
+#pragma clang fp contract(off)
+  const T t = s * y + T(a12);
+  return T(a12) * y / (t * t);
+}
+
 struct KernCoulomb {
   static constexpr int ncomp = 1;
   template <class T>
   __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
-    const T s = dx * dx + dy * dy + dz * dz;
+    const T s = dist2(dx, dy, dz);
     out[0] = s > T(0) ? rsqrt_any(s) : T(0);
   }
 };
@@ -289,11 +313,9 @@ struct KernTCScalar {
   double a12, c2a2;  // c2a2 = 2 c^2 a12^2
   template <class T>
   __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
-    const T s = dx * dx + dy * dy + dz * dz;
+    const T s = dist2(dx, dy, dz);
     const T y = rsqrt_any(s);
-    const T t = s * y + T(a12);
-    const T t2 = t * t;
-    out[0] = s > T(0) ? T(2) * y - T(c2a2) / (t2 * t2) : T(0);
+    out[0] = s > T(0) ? k_tc_scalar(s, y, a12, c2a2) : T(0);
   }
 };
 
@@ -303,10 +325,10 @@ struct KernJastrowGrad {
   double a12;
   template <class T>
   __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
-    const T s = dx * dx + dy * dy + dz * dz;
+#pragma clang fp contract(off)
+    const T s = dist2(dx, dy, dz);
     const T y = rsqrt_any(s);
-    const T t = s * y + T(a12);
-    const T gg = s > T(0) ? T(a12) * y / (t * t) : T(0);
+    const T gg = s > T(0) ? k_grad_scale(s, y, a12) : T(0);
     out[0] = gg * dx;
     out[1] = gg * dy;
     out[2] = gg * dz;
@@ -487,12 +509,16 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 /* (double buffered); B fragments and the next step's r2 coordinates are     */
 /* prefetched into registers one step ahead.                                 */
 /* ------------------------------------------------------------------------ */
+// One workgroup tile of the BF16 contraction: rows row0 .. row0 + 16 MB of r1,
+// 16-column blocks cbw .. cbw + 15 of the pairs, all of r2.  Output element
+// (row, component c, pair col) goes to Fout[row * ld_row + c * npair + col].
+// ktile: the workgroup's K tile (LDS, [2][NC][MB][2][64] fragments).
 template <int MB, int NPROD, int FOLD, class Kern>
-__global__ __launch_bounds__(256, Kern::ncomp == 1 ? 2 : 1) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
-                                                         const float4* __restrict__ x2, int64_t n2p,
-                                                         const us8* __restrict__ P0, int64_t plane,
-                                                         int64_t ncb, void* __restrict__ Fout, int out_f32,
-                                                         int64_t npair, double scale, Kern kern) {
+__device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], const double* __restrict__ p1,
+                                          int64_t n1, const float4* __restrict__ x2, int64_t n2p,
+                                          const us8* __restrict__ P0, int64_t plane, int64_t ncb,
+                                          void* __restrict__ Fout, int out_f32, int64_t npair, int64_t ld_row,
+                                          double scale, Kern kern, int64_t row0, int64_t cbw) {
   constexpr int NC = Kern::ncomp;
   constexpr int NB = 4;                   // 16-column blocks per wave
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
@@ -501,12 +527,10 @@ __global__ __launch_bounds__(256, Kern::ncomp == 1 ? 2 : 1) void fgemm_bf16_kern
   static_assert(EPT >= 2 && EPT <= 8 && VALS % 256 == 0 && EPT % 2 == 0, "MB must be 1, 2 or 4");
   static_assert(FOLD == 1 || FOLD == 2, "FP64 fold every 1 or 2 steps");
   typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-  __shared__ us8 ktile[2][NC][MB][2][64];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, r = lane & 15;
-  const int64_t row0 = (int64_t)blockIdx.y * (16 * MB);
-  const int64_t cb0 = (int64_t)blockIdx.x * 16 + wave * NB;
+  const int64_t cb0 = cbw + wave * NB;
 
   // this thread generates elements ge .. ge+EPT-1 of fragment lane gl of m-block gm
   const int gi = tid * EPT;
@@ -535,6 +559,7 @@ __global__ __launch_bounds__(256, Kern::ncomp == 1 ? 2 : 1) void fgemm_bf16_kern
       unsigned hw[EPT / 2], lw[EPT / 2];
 #pragma unroll
       for (int e = 0; e < EPT; e += 2) {
+#pragma clang fp contract(off)
         float v0[NC], v1[NC];
         kern(ax - qn[e].x, ay - qn[e].y, az - qn[e].z, v0);
         kern(ax - qn[e + 1].x, ay - qn[e + 1].y, az - qn[e + 1].z, v1);
@@ -645,12 +670,24 @@ __global__ __launch_bounds__(256, Kern::ncomp == 1 ? 2 : 1) void fgemm_bf16_kern
         for (int i = 0; i < 4; ++i) {
           const int64_t row = row0 + 16 * m + d_row_bf16(g, i), col = 16 * (cb0 + n) + r;
           if (row < n1 && col < npair) {
-            const int64_t o = (row * NC + c) * npair + col;
+            const int64_t o = row * ld_row + c * npair + col;
             const double v = scale * sum[c][m][n][i];
             if (out_f32) ((float*)Fout)[o] = (float)v;
             else ((double*)Fout)[o] = v;
           }
         }
+}
+
+template <int MB, int NPROD, int FOLD, class Kern>
+__global__ __launch_bounds__(256, Kern::ncomp == 1 ? 2 : 1) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
+                                                         const float4* __restrict__ x2, int64_t n2p,
+                                                         const us8* __restrict__ P0, int64_t plane,
+                                                         int64_t ncb, void* __restrict__ Fout, int out_f32,
+                                                         int64_t npair, double scale, Kern kern) {
+  __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
+  bf16_tile<MB, NPROD, FOLD, Kern>(ktile, p1, n1, x2, n2p, P0, plane, ncb, Fout, out_f32, npair,
+                                   Kern::ncomp * npair, scale, kern, (int64_t)blockIdx.y * (16 * MB),
+                                   (int64_t)blockIdx.x * 16);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1171,4 +1208,254 @@ extern "C" int tcscf_three_electron(const double* A1, const double* w1, int64_t 
                      w1, n1, V, nb, c * c, Q);
   TC_HIP(hipGetLastError());
   return 0;
+}
+
+/* ======================================================================== */
+/* Batched contraction: many (r1 grid, r2 grid, basis, kernel) instances of   */
+/* uneven size in one persistent launch.  The work queue is the list of       */
+/* (instance, 16 MB-row block, 256-column block, component) tiles, instances  */
+/* ordered by descending r2 length; workgroups take tiles from an atomic      */
+/* counter until the queue is empty.  Each tile is the same bf16_tile as the  */
+/* single-instance kernel, so results match it bit for bit.                   */
+/* ======================================================================== */
+namespace tc {
+
+// Any of the three kernels, one scalar component at a time (kind uniform per tile).
+struct KernAny {
+  static constexpr int ncomp = 1;
+  int kind, comp;
+  double a12, c2a2;
+  template <class T>
+  __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
+#pragma clang fp contract(off)
+    const T s = dist2(dx, dy, dz);
+    const T y = rsqrt_any(s);
+    if (kind == TCSCF_K_COULOMB) out[0] = s > T(0) ? y : T(0);
+    else if (kind == TCSCF_K_TC_SCALAR) out[0] = s > T(0) ? k_tc_scalar(s, y, a12, c2a2) : T(0);
+    else out[0] = (s > T(0) ? k_grad_scale(s, y, a12) : T(0)) * (comp == 0 ? dx : (comp == 1 ? dy : dz));
+  }
+};
+
+struct BatchDesc {
+  const double* p1;
+  const float4* x2;
+  const us8* P0;
+  double* F;
+  int64_t n1, n2p, plane, ncb, npair;
+  int64_t rblocks, cblocks, tiles;  // row blocks (16 MB rows), 256-column blocks, all tiles
+  double scale, a12, c2a2;
+  int kind, ncomp;
+};
+
+template <int MB, int NPROD>
+__global__ __launch_bounds__(256, 2) void bf16_batch_kernel(const BatchDesc* __restrict__ desc,
+                                                            const int64_t* __restrict__ start, int ndesc,
+                                                            int64_t t_begin, int64_t t_end,
+                                                            unsigned long long* __restrict__ counter) {
+  __shared__ us8 ktile[2][1][MB][2][64];
+  __shared__ int64_t cur;
+  for (;;) {
+    if (threadIdx.x == 0) cur = t_begin + (int64_t)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const int64_t t = cur;
+    __syncthreads();  // every thread has read cur before it is replaced
+    if (t >= t_end) break;
+    int lo = 0, hi = ndesc - 1;  // last instance with start <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchDesc& d = desc[lo];
+    int64_t local = t - start[lo];
+    const int comp = (int)(local % d.ncomp);
+    local /= d.ncomp;
+    const int64_t cb = local % d.cblocks, rb = local / d.cblocks;
+    const KernAny kern{d.kind, comp, d.a12, d.c2a2};
+    bf16_tile<MB, NPROD, 1, KernAny>(ktile, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, d.F + comp * d.npair, 0,
+                                     d.npair, d.ncomp * d.npair, d.scale, kern, rb * (16 * MB), cb * 16);
+  }
+}
+
+__global__ void zero_counter(unsigned long long* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
+}
+
+// Autotuned row-block height per product count (3 or 5), process-wide.
+std::mutex g_tune_mu;
+std::map<int, int> g_tuned_mb;
+
+}  // namespace tc
+
+extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, int precision, void* stream) {
+  using namespace tc;
+  TC_CHECK(items && count > 0, "tcscf_batch_compute_F: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (precision != TCSCF_BF16X3 && precision != TCSCF_BF16X5) {  // FP64 / FP32: one launch per instance
+    for (int k = 0; k < count; ++k) {
+      const tcscf_batch_item& it = items[k];
+      const int rc = tcscf_compute_F(it.pts1_dev, it.n1, it.pts2_dev, it.w2_dev, it.ord2_dev, it.n2, it.A2_dev, it.nb,
+                                     it.kernel, &it.jastrow, it.same_spin, precision, it.scale, it.F_dev, TCSCF_F64,
+                                     stream);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  const int nsplit = precision == TCSCF_BF16X3 ? 2 : 3;
+  // workspace: per instance pair-density planes + padded coordinates, then the counter
+  std::vector<size_t> off(count + 1, 0);
+  for (int k = 0; k < count; ++k) {
+    const tcscf_batch_item& it = items[k];
+    TC_CHECK(it.pts1_dev && it.pts2_dev && it.w2_dev && it.A2_dev && it.F_dev && it.n1 > 0 && it.n2 > 0 && it.nb > 0,
+             "tcscf_batch_compute_F: bad item");
+    TC_CHECK(it.kernel >= TCSCF_K_COULOMB && it.kernel <= TCSCF_K_JASTROW_GRAD, "tcscf_batch_compute_F: unknown kernel");
+    TC_CHECK(it.kernel == TCSCF_K_COULOMB || it.jastrow.a12 > 0, "tcscf_batch_compute_F: TC kernels need a12 > 0");
+    off[k + 1] = off[k] + round_up(tcscf_F_workspace_bytes(it.n2, it.nb, precision), 256);
+  }
+  int rc = 0;
+  char* ws = (char*)scratch(kSlotPairs, off[count] + 256, &rc);
+  if (rc) return rc;
+  unsigned long long* counter = (unsigned long long*)(ws + off[count]);
+  std::vector<BatchDesc> desc(count);
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  for (int k = 0; k < count; ++k) {
+    const tcscf_batch_item& it = items[k];
+    const int64_t npair = npair_of(it.nb), n2p = round_up(it.n2, 32), ncb = round_up(npair, 256) / 16,
+                  nkb = n2p / 32, plane = nkb * ncb * 512;
+    unsigned short* planes = (unsigned short*)(ws + off[k]);
+    double* xd = (double*)(planes + nsplit * plane);
+    float4* x4 = (float4*)(xd + 3 * n2p);
+    const unsigned gp = (unsigned)((plane + 255) / 256);
+    if (nsplit == 2)
+      hipLaunchKernelGGL(pack_pairs_bf16<2>, dim3(gp), dim3(256), 0, st, it.A2_dev, it.w2_dev, it.ord2_dev, it.n2,
+                         it.nb, npair, nkb, ncb, planes);
+    else
+      hipLaunchKernelGGL(pack_pairs_bf16<3>, dim3(gp), dim3(256), 0, st, it.A2_dev, it.w2_dev, it.ord2_dev, it.n2,
+                         it.nb, npair, nkb, ncb, planes);
+    TC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(pad_coords, dim3((unsigned)((n2p + 255) / 256)), dim3(256), 0, st, it.pts2_dev, it.ord2_dev,
+                       it.n2, n2p, xd, x4);
+    TC_HIP(hipGetLastError());
+    BatchDesc& d = desc[k];
+    d.p1 = it.pts1_dev;
+    d.x2 = x4;
+    d.P0 = (const us8*)planes;
+    d.F = it.F_dev;
+    d.n1 = it.n1;
+    d.n2p = n2p;
+    d.plane = plane / 8;
+    d.ncb = ncb;
+    d.npair = npair;
+    d.cblocks = ncb / 16;
+    d.scale = it.scale;
+    d.kind = it.kernel;
+    d.ncomp = it.kernel == TCSCF_K_JASTROW_GRAD ? 3 : 1;
+    d.a12 = it.jastrow.a12;
+    const double c = it.same_spin ? it.jastrow.c_same : it.jastrow.c_oppo;
+    d.c2a2 = 2 * c * c * it.jastrow.a12 * it.jastrow.a12;
+  }
+  // largest r2 first, so the tail of the queue holds the short tiles
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return desc[a].n2p > desc[b].n2p; });
+  std::vector<BatchDesc> sorted(count);
+  for (int k = 0; k < count; ++k) sorted[k] = desc[order[k]];
+
+  int dev = 0, ncu = 256;
+  TC_HIP(hipGetDevice(&dev));
+  TC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  // descriptors for one row-block height (16 MB rows); uploaded behind the counter
+  auto upload = [&](int mb, int64_t* total, const BatchDesc** d_desc, const int64_t** d_start) -> int {
+    std::vector<int64_t> start(count);
+    int64_t t = 0;
+    for (int k = 0; k < count; ++k) {
+      sorted[k].rblocks = (sorted[k].n1 + 16 * mb - 1) / (16 * mb);
+      sorted[k].tiles = sorted[k].rblocks * sorted[k].cblocks * sorted[k].ncomp;
+      start[k] = t;
+      t += sorted[k].tiles;
+    }
+    *total = t;
+    int r = 0;
+    char* meta = (char*)scratch(kSlotBatchMeta + (mb - 1), sizeof(BatchDesc) * count + sizeof(int64_t) * count + 256, &r);
+    if (r) return r;
+    TC_HIP(hipMemcpyAsync(meta, sorted.data(), sizeof(BatchDesc) * count, hipMemcpyHostToDevice, st));
+    TC_HIP(hipMemcpyAsync(meta + round_up(sizeof(BatchDesc) * count, 256), start.data(), sizeof(int64_t) * count,
+                          hipMemcpyHostToDevice, st));
+    TC_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
+    *d_desc = (const BatchDesc*)meta;
+    *d_start = (const int64_t*)(meta + round_up(sizeof(BatchDesc) * count, 256));
+    return 0;
+  };
+  auto launch = [&](int mb, const BatchDesc* dd, const int64_t* ds, int64_t t0, int64_t t1) -> int {
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, st, counter);
+    TC_HIP(hipGetLastError());
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(t1 - t0, (int64_t)ncu * 2));
+    if (nsplit == 3) {
+      if (mb == 1) hipLaunchKernelGGL((bf16_batch_kernel<1, 5>), dim3(grid), dim3(256), 0, st, dd, ds, count, t0, t1, counter);
+      else hipLaunchKernelGGL((bf16_batch_kernel<2, 5>), dim3(grid), dim3(256), 0, st, dd, ds, count, t0, t1, counter);
+    } else {
+      if (mb == 1) hipLaunchKernelGGL((bf16_batch_kernel<1, 3>), dim3(grid), dim3(256), 0, st, dd, ds, count, t0, t1, counter);
+      else hipLaunchKernelGGL((bf16_batch_kernel<2, 3>), dim3(grid), dim3(256), 0, st, dd, ds, count, t0, t1, counter);
+    }
+    TC_HIP(hipGetLastError());
+    return 0;
+  };
+
+  // Autotuner: the row-block height (16 or 32 rows per tile) is timed on the
+  // head of the queue (the largest tiles) the first time a precision is seen
+  // in this process, and the faster one is kept.
+  const int key = nsplit == 3 ? 5 : 3;
+  int mb_best = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto f = g_tuned_mb.find(key);
+    if (f != g_tuned_mb.end()) mb_best = f->second;
+  }
+  const char* env = std::getenv("TCSCF_BATCH_MB");
+  if (env) mb_best = std::atoi(env) == 1 ? 1 : 2;
+  if (!mb_best) {
+    float best = 1e30f;
+    hipEvent_t e0, e1;
+    TC_HIP(hipEventCreate(&e0));
+    TC_HIP(hipEventCreate(&e1));
+    for (int mb : {1, 2}) {
+      int64_t total;
+      const BatchDesc* dd;
+      const int64_t* ds;
+      if ((rc = upload(mb, &total, &dd, &ds))) return rc;
+      const int64_t probe = std::min<int64_t>(total, std::max<int64_t>(4 * ncu, total / 16));
+      if ((rc = launch(mb, dd, ds, 0, probe))) return rc;  // warm-up
+      TC_HIP(hipEventRecord(e0, st));
+      if ((rc = launch(mb, dd, ds, 0, probe))) return rc;
+      TC_HIP(hipEventRecord(e1, st));
+      TC_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      TC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      // normalise to rows covered: the probe of mb = 1 covers half the rows per tile
+      const float per_row = ms / (float)(probe * mb);
+      if (per_row < best) {
+        best = per_row;
+        mb_best = mb;
+      }
+    }
+    TC_HIP(hipEventDestroy(e0));
+    TC_HIP(hipEventDestroy(e1));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned_mb[key] = mb_best;
+  }
+  int64_t total;
+  const BatchDesc* dd;
+  const int64_t* ds;
+  if ((rc = upload(mb_best, &total, &dd, &ds))) return rc;
+  if ((rc = launch(mb_best, dd, ds, 0, total))) return rc;
+  TC_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+// Rows per tile the autotuner chose for this precision (16 or 32), 0 before the
+// first batch at that precision.
+extern "C" int tcscf_batch_tuned_rows(int precision) {
+  using namespace tc;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto f = g_tuned_mb.find(precision == TCSCF_BF16X3 ? 3 : 5);
+  return f == g_tuned_mb.end() ? 0 : 16 * f->second;
 }

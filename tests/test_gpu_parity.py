@@ -568,3 +568,42 @@ def test_cpp_facade_on_device(tc, O, gpu):
     Sb, Hb = tc.one_electron_matrices(bb, 4)
     etc = tc.TCHartreeFock(2, 2, len(bb)).compute(Sb, Hb, g1, Fs, Fo2, V, jas, True, tol=1e-10, mgs_mode=1)
     assert abs(out["be_tc"] - etc) < 1e-9
+
+
+def test_batched_contraction_matches_single_calls(tc, gpu):
+    """One persistent BF16 launch over uneven instances (different grids,
+    bases, kernels, a 2-D r2 grid, V with 3 components) gives exactly the
+    single-instance results; the autotuner picked a tile height."""
+    import torch
+    from tcscf_amd import SlaterTypeOrbital as STO
+    rng = np.random.default_rng(4)
+    specs, singles = [], []
+    jas = tc.Jastrow.from_b(0.7)
+    for k, (nrad, order, shells) in enumerate([(20, 11, {0: 4}), (31, 17, {0: 6, 1: 2}), (12, 7, {0: 3, 1: 1, 2: 1}),
+                                               (45, 23, {0: 9, 1: 3})]):
+        basis = tc.even_tempered(shells, 0.5, 2.0)
+        g1 = _grid(tc, nrad, order, tc.RADIAL_M4)
+        g1.setBasisFunctions(basis, False)
+        kind = ["coulomb", "tc", "grad", "tc"][k]
+        specs.append(dict(r1=g1, r2=g1, kernel=kind, jastrow=jas, same_spin=k % 2 == 0))
+    p2d = rng.uniform(0.05, 3, (3, 333))
+    p2d[1] = 0
+    g2d = tc.QMCGrid(p2d, rng.uniform(0.1, 1, 333), ndim=2)
+    gq = _grid(tc, 9, 11, tc.RADIAL_M4)
+    b2 = [STO(0.8, 1, 0, 0), STO(1.7, 1, 0, 0), STO(1.1, 2, 0, 0)]
+    for g in (g2d, gq):
+        g.setBasisFunctions(b2, False)
+    specs.append(dict(r1=gq, r2=g2d, kernel="coulomb"))
+    for prec in ("bf16x5", "bf16x3"):
+        outs = tc.computeF_batch(specs, precision=prec)
+        for sp, F in zip(specs, outs):
+            if sp["kernel"] == "grad":
+                ref = tc.computeV(sp["r1"], sp["r2"], sp["jastrow"], precision=prec)
+            else:
+                ref = tc.computeF(sp["r1"], sp["r2"], kernel=sp["kernel"], jastrow=sp.get("jastrow"),
+                                  same_spin=sp.get("same_spin", True), precision=prec)
+            assert torch.equal(F, ref), (prec, sp["kernel"])
+        assert tc.lib().tcscf_batch_tuned_rows(tc.BF16X5 if prec == "bf16x5" else tc.BF16X3) in (16, 32)
+    # FP64 goes instance by instance and equals the single calls too
+    outs = tc.computeF_batch(specs[:2], precision="fp64")
+    assert torch.equal(outs[0], tc.computeF(specs[0]["r1"], specs[0]["r2"]))
