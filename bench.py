@@ -46,6 +46,7 @@ def parse():
     p.add_argument("--no-tc", action="store_true", help="skip the TC-HF Ne side measurement")
     p.add_argument("--tc-mgs", type=int, default=1, help="TC Gram-Schmidt: 0 reference, 1 metric")
     p.add_argument("--tc-diis", type=int, default=6, help="TC DIIS depth (the TC workload asks for 6; 0 = plain)")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched 64-instance side measurement")
     p.add_argument("--pmc-probe", action="store_true",
                    help="for rocprofv3 --pmc runs: one BF16x5 and one FP64 computeF on the Ne grid, one RHF "
                         "solve, then a 512 MiB device copy whose counter reading calibrates the byte counters")
@@ -124,6 +125,9 @@ def main():
     tc_res = None
     if not a.no_tc:
         tc_res = run_tc(a, tc, configs, torch)
+    batch_res = None
+    if not a.no_batch:
+        batch_res = run_batch(a, tc, configs, torch)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -180,6 +184,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "tc_ne": tc_res,
+        "batch64": batch_res,
     }
     print(json.dumps(res))
 
@@ -224,6 +229,39 @@ def run_tc(a, tc, configs, torch):
             "pairs_x_components": 2 * n1 * (n1 - 1) + 3 * n1 * n3,
             a.precision: mix, "fp64": f64, "energy_dev_vs_fp64": mix["energy"] - f64["energy"],
             "speedup_vs_fp64": f64["total_ms"] / mix["total_ms"]}
+
+
+def run_batch(a, tc, configs, torch):
+    """The batched-serving workload: 64 uneven instances (Z 2..18, atoms and
+    cations, 7..53 STOs, 1,250..30,200 points, half HF / half TC with b in
+    [0.5, 2]) from a fixed seed.  One batched contraction call (a persistent
+    launch for BF16) and one batched SCF (a HIP graph per iteration, DIIS 6,
+    tol 1e-8, at most 60 iterations) per precision; the FP64 run gives the
+    per-instance error."""
+    import numpy as np
+    from tcscf_amd import batch
+    inst = configs.batch_instances()
+    prep = batch.prepare(inst)
+    torch.cuda.synchronize()
+    batch.run(prep, a.precision, synchronize=torch.cuda.synchronize)  # warm-up (and the tile autotuner)
+    res_m, t_m = batch.run(prep, a.precision, synchronize=torch.cuda.synchronize)
+    res_d, t_d = batch.run(prep, "fp64", synchronize=torch.cuda.synchronize)
+    both = [(x, m, d) for x, m, d in zip(inst, res_m, res_d) if m["status"] == 0 and d["status"] == 0]
+    err = np.array([abs(m["energy"] - d["energy"]) for _, m, d in both])
+    rel = np.array([abs(m["energy"] - d["energy"]) / abs(d["energy"]) for _, m, d in both])
+    units = batch.pair_units(prep)
+    return {"workload": f"{len(inst)} instances, seed {configs.BATCH_SEED}: Z 2..18 atoms/cations, "
+                        f"{min(x['nb'] for x in inst)}..{max(x['nb'] for x in inst)} STOs, "
+                        f"{min(x['npts'] for x in inst)}..{max(x['npts'] for x in inst)} points, "
+                        f"{sum(x['method'] == 'tc' for x in inst)} TC",
+            "pair_units": units,
+            "value": units / (t_m["total_ms"] * 1e-3), "unit": "grid-pair contractions/s",
+            a.precision: t_m, "fp64": t_d, "speedup_vs_fp64": t_d["total_ms"] / t_m["total_ms"],
+            "converged": {a.precision: sum(r["status"] == 0 for r in res_m), "fp64": sum(r["status"] == 0 for r in res_d)},
+            "energy_err_vs_fp64": {"max": float(err.max()), "median": float(np.median(err)),
+                                   "max_relative": float(rel.max()), "instances": len(both)},
+            "tile_rows_autotuned": tc.lib().tcscf_batch_tuned_rows(tc._lib.PRECISIONS[a.precision])
+            if a.precision in ("bf16x5", "bf16x3") else None}
 
 
 def pmc_probe(tc, cfg, torch):

@@ -260,6 +260,43 @@ int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1_dev, const double
                   const tcscf_jastrow* jastrow, const double* D_host, const double* C_host, double* G2_host,
                   double* G3_host, void* stream);
 
+/* ---------------------------------------------------------------- batched SCF */
+enum tcscf_scf_kind { TCSCF_SCF_RCS = 0, TCSCF_SCF_UOS = 1, TCSCF_SCF_TC = 2 };
+
+/* One instance of a batched solve: the arguments of tcscf_rcs_scf /
+ * tcscf_uos_scf (kind RCS / UOS: F_dev, f_dtype; RCS uses nup + ndown electrons)
+ * or tcscf_tc_scf (kind TC: Fs_dev, Fo_dev, V_dev, dA1_dev, jastrow). */
+typedef struct tcscf_scf_item {
+  int kind;
+  int nup, ndown, nb;
+  const double* S;
+  const double* H;
+  const double* A1_dev;
+  const double* dA1_dev;
+  const double* w1_dev;
+  int64_t n1;
+  const void* F_dev;
+  int f_dtype;
+  const double* Fs_dev;
+  const double* Fo_dev;
+  const double* V_dev;
+  tcscf_jastrow jastrow;
+  double* energy_trace; /* host, opts->max_iter entries, or NULL */
+  double* D_out;        /* host, nspin x nb x nb (RCS 1, else 2), or NULL */
+} tcscf_scf_item;
+
+/* No single reference counterpart: RCS/UOS/TCHartreeFock::compute
+ * (src/HartreeFock.cpp:785-1362) for many instances at once, with the same
+ * options.  Each iteration replays one HIP graph holding every active
+ * instance's Fock build (spread over 4 streams); the FP64 host halves of the
+ * iterations run on a pool of host threads.  The graph is captured again when
+ * an instance converges.  results[k].status is TCSCF_OK / ENOTCONV / ESOLVER per
+ * instance; t_sweep_ms / t_solve_ms hold the batch's device / host totals.
+ * Returns 0 once every instance has finished, < 0 on a set-up error.
+ * graph_builds (may be NULL): number of graph captures. */
+int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcscf_scf_opts* opts, tcscf_scf_result* results,
+                    int* graph_builds, void* stream);
+
 /* ---------------------------------------------------------------- diagnostics */
 /* No reference counterpart: runs one MFMA fragment (which = 0: f64 16x16x4,
  * 1: f32 16x16x4, 2: bf16 16x16x32) on A_dev (16 x K) and B_dev (K x 16),

@@ -24,7 +24,7 @@ EXPORTS = [
     "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
     "tcscf_F_workspace_bytes", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
     "tcscf_qmc_lattice_npts", "tcscf_qmc_lattice_grid", "tcscf_sobol_points", "tcscf_sobol_grid", "tcscf_load_sto",
-    "tcscf_batch_compute_F", "tcscf_batch_tuned_rows",
+    "tcscf_batch_compute_F", "tcscf_batch_tuned_rows", "tcscf_batch_scf",
     "tcscf_selftest_mfma",
 ]
 
@@ -61,6 +61,15 @@ class BatchItem(ctypes.Structure):
 class ScfOpts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("mask", ctypes.c_int),
                 ("mgs_mode", ctypes.c_int), ("diis", ctypes.c_int)]
+
+
+class ScfItem(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("nup", ctypes.c_int), ("ndown", ctypes.c_int), ("nb", ctypes.c_int),
+                ("S", ctypes.c_void_p), ("H", ctypes.c_void_p), ("A1_dev", ctypes.c_void_p),
+                ("dA1_dev", ctypes.c_void_p), ("w1_dev", ctypes.c_void_p), ("n1", ctypes.c_int64),
+                ("F_dev", ctypes.c_void_p), ("f_dtype", ctypes.c_int), ("Fs_dev", ctypes.c_void_p),
+                ("Fo_dev", ctypes.c_void_p), ("V_dev", ctypes.c_void_p), ("jastrow", Jastrow),
+                ("energy_trace", ctypes.c_void_p), ("D_out", ctypes.c_void_p)]
 
 
 class ScfResult(ctypes.Structure):

@@ -421,6 +421,53 @@ def threeElectronIntegrals(r1Grid, r2Grid, jastrow, precision="fp64"):
     return Q
 
 
+def batch_scf(instances, tol=1e-5, max_iter=200, respectOneElectronSymmetry=True, mgs_mode=1, diis=0):
+    """Many SCF solves at once (tcscf_batch_scf): every iteration replays one
+    HIP graph with the Fock builds of all active instances.  instances: dicts
+    with kind ('rcs', 'uos' or 'tc'), nup, ndown, S, H, r1 (QMCGrid with the
+    basis tabulated; gradients for TC) and F (RCS/UOS) or Fs, Fo, V, jastrow
+    (TC).  Returns (list of dicts energy/iterations/status/trace/D, graph builds,
+    device ms, host ms)."""
+    torch = _torch()
+    n = len(instances)
+    items = (_lib.ScfItem * n)()
+    keep = []
+    traces = [np.zeros(max_iter) for _ in range(n)]
+    Ds = []
+    kinds = {"rcs": 0, "uos": 1, "tc": 2}
+    for k, x in enumerate(instances):
+        it = items[k]
+        g = x["r1"]
+        nb = g.nBasis()
+        S, _ = _np(x["S"])
+        H, _ = _np(x["H"])
+        keep += [S, H]
+        it.kind, it.nup, it.ndown, it.nb = kinds[x["kind"]], int(x["nup"]), int(x["ndown"]), nb
+        it.S, it.H = S.ctypes.data, H.ctypes.data
+        it.A1_dev, it.w1_dev, it.n1 = g.basisValues.data_ptr(), g.quadratureGrid.weights.data_ptr(), g.nGrid()
+        if x["kind"] == "tc":
+            if not g.gradientsStored():
+                raise ValueError("batch_scf: TC instances need basis gradients on r1")
+            it.dA1_dev = g.basisGradients.data_ptr()
+            it.Fs_dev, it.Fo_dev, it.V_dev = x["Fs"].data_ptr(), x["Fo"].data_ptr(), x["V"].data_ptr()
+            it.jastrow = x["jastrow"]._struct()
+        else:
+            F = x["F"]
+            it.F_dev = F.data_ptr()
+            it.f_dtype = _lib.F32 if F.dtype == torch.float32 else _lib.F64
+        it.energy_trace = traces[k].ctypes.data
+        D = np.zeros((1 if x["kind"] == "rcs" else 2, nb, nb))
+        Ds.append(D)
+        it.D_out = D.ctypes.data
+    res = (_lib.ScfResult * n)()
+    builds = ctypes.c_int(0)
+    check(lib().tcscf_batch_scf(items, n, ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, mgs_mode,
+                                                              diis)), res, ctypes.byref(builds), _stream()))
+    out = [dict(energy=res[k].energy, iterations=res[k].iterations, status=res[k].status,
+                trace=traces[k][:max(res[k].iterations, 0)].copy(), D=Ds[k]) for k in range(n)]
+    return out, builds.value, (res[0].t_sweep_ms if n else 0.0), (res[0].t_solve_ms if n else 0.0)
+
+
 def eri_full(E, nb):
     """Packed ERI -> answer(b1, b2, b3, b4) (numpy)."""
     idx = np.array([[pair_index(j, i) for i in range(nb)] for j in range(nb)])

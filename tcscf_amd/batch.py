@@ -1,0 +1,71 @@
+"""The batched-serving workload end to end: grids and bases for every instance,
+one batched contraction launch for all F / V tensors, one batched SCF (a HIP
+graph per iteration over all active instances)."""
+import time
+
+from . import api, configs
+
+
+def prepare(instances):
+    """Device grids (basis values, gradients for TC) and S / H per instance."""
+    out = []
+    for x in instances:
+        basis = configs.batch_basis(x)
+        nrad, order = x["grid"]
+        g = api.QMCGrid.radial_lebedev(nrad, order, configs._lib.RADIAL_M4, 1.0, 1.0)
+        g.setBasisFunctions(basis, x["method"] == "tc")
+        S, H = api.one_electron_matrices(basis, x["Z"])
+        out.append(dict(inst=x, grid=g, S=S, H=H, jastrow=api.Jastrow.from_b(x["b"]) if x["b"] else None))
+    return out
+
+
+def contraction_specs(prepared):
+    """One spec per F / V tensor: HF needs F; TC needs F_same, F_oppo and V."""
+    specs = []
+    for p in prepared:
+        g = p["grid"]
+        if p["inst"]["method"] == "tc":
+            j = p["jastrow"]
+            specs += [dict(r1=g, r2=g, kernel="tc", jastrow=j, same_spin=True),
+                      dict(r1=g, r2=g, kernel="tc", jastrow=j, same_spin=False),
+                      dict(r1=g, r2=g, kernel="grad", jastrow=j)]
+        else:
+            specs.append(dict(r1=g, r2=g, kernel="coulomb"))
+    return specs
+
+
+def pair_units(prepared):
+    """Grid-pair contractions (pairs x distinct kernel components, coincident
+    pairs excluded): 1 per pair for HF, 2 (F_same / F_oppo combine linearly) + 3
+    (V) for TC, as the workload's algorithmic count."""
+    total = 0
+    for p in prepared:
+        n = p["grid"].nGrid()
+        total += (n * n - n) * (5 if p["inst"]["method"] == "tc" else 1)
+    return total
+
+
+def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None):
+    """Precompute (one batched call) + batched SCF at one precision.  Returns
+    (results, timings dict)."""
+    t0 = time.perf_counter()
+    tensors = api.computeF_batch(contraction_specs(prepared), precision=precision)
+    if synchronize:
+        synchronize()
+    t1 = time.perf_counter()
+    items, k = [], 0
+    for p in prepared:
+        x = p["inst"]
+        d = dict(nup=x["nup"], ndown=x["ndown"], S=p["S"], H=p["H"], r1=p["grid"])
+        if x["method"] == "tc":
+            d.update(kind="tc", Fs=tensors[k], Fo=tensors[k + 1], V=tensors[k + 2], jastrow=p["jastrow"])
+            k += 3
+        else:
+            d.update(kind="rcs" if x["nup"] == x["ndown"] else "uos", F=tensors[k])
+            k += 1
+        items.append(d)
+    res, builds, dev_ms, host_ms = api.batch_scf(items, tol=tol, max_iter=max_iter, mgs_mode=1, diis=diis)
+    t2 = time.perf_counter()
+    del tensors
+    return res, dict(precompute_ms=(t1 - t0) * 1e3, scf_ms=(t2 - t1) * 1e3, total_ms=(t2 - t0) * 1e3,
+                     graph_builds=builds, scf_device_ms=dev_ms, scf_host_ms=host_ms)

@@ -1,0 +1,261 @@
+// Batched SCF: many RCS / UOS / TC instances of uneven size solved together.
+// Every iteration replays one HIP graph holding the Fock build of every active
+// instance (upload D / C from pinned memory, the sweep kernels, download the
+// result), spread over a few streams so small instances run side by side; the
+// FP64 host halves of the iterations then run on a pool of host threads.  The
+// graph is captured again only when the set of active instances changes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "common.hpp"
+#include "scf_steps.hpp"
+
+namespace tc {
+namespace {
+
+constexpr int kBranches = 4;  // streams the graph's instances are spread over
+
+double wall_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Pinned {
+  double* p = nullptr;
+  explicit Pinned(size_t n) {
+    if (n && hipHostMalloc((void**)&p, n * sizeof(double), hipHostMallocDefault) != hipSuccess) p = nullptr;
+  }
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  Pinned(const Pinned&) = delete;
+  Pinned& operator=(const Pinned&) = delete;
+};
+
+struct Instance {
+  const tcscf_scf_item* item;
+  int nspin = 1;
+  double* ws = nullptr;  // device workspace (hipMalloc), freed at the end
+  HfPlan hf{};
+  TcPlan tcp{};
+  std::unique_ptr<HfIter> hfi;
+  std::unique_ptr<TcIter> tci;
+  std::unique_ptr<Pinned> up, down;  // D (+ C) to the device, G / TC result block back
+  std::vector<double> G2, G3;
+  int it = 0;
+  bool active = true;
+  int rc = 0;
+};
+
+// Runs f(k) for k in [0, n) on up to `threads` host threads.
+template <class F>
+void parallel_for(int n, int threads, F f) {
+  if (n <= 1 || threads <= 1) {
+    for (int k = 0; k < n; ++k) f(k);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < std::min(n, threads); ++t)
+    pool.emplace_back([&] {
+      for (int k; (k = next.fetch_add(1)) < n;) f(k);
+    });
+  for (auto& th : pool) th.join();
+}
+
+}  // namespace
+}  // namespace tc
+
+extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcscf_scf_opts* o,
+                               tcscf_scf_result* results, int* graph_builds, void* stream) {
+  using namespace tc;
+  TC_CHECK(items && count > 0 && o && results && o->max_iter > 0 && o->diis >= 0, "tcscf_batch_scf: bad arguments");
+  (void)stream;
+  std::vector<Instance> inst(count);
+  int rc = 0;
+  auto cleanup = [&] {
+    for (auto& x : inst)
+      if (x.ws) (void)hipFree(x.ws);
+  };
+  for (int k = 0; k < count; ++k) {
+    const tcscf_scf_item& it = items[k];
+    Instance& x = inst[k];
+    x.item = &it;
+    tcscf_scf_result& r = results[k];
+    r.status = TCSCF_ENOTCONV;
+    r.energy = std::numeric_limits<double>::max();
+    r.iterations = 0;
+    r.t_sweep_ms = r.t_solve_ms = 0;
+    const int nb = it.nb;
+    const int64_t nb2 = (int64_t)nb * nb;
+    if (!(it.S && it.H && it.A1_dev && it.w1_dev && it.n1 > 0 && nb > 0)) {
+      cleanup();
+      TC_CHECK(false, "tcscf_batch_scf: item " + std::to_string(k) + ": missing inputs");
+    }
+    if (it.kind == TCSCF_SCF_TC) {
+      if (o->diis > 0 && o->mgs_mode != 1) {
+        cleanup();
+        TC_CHECK(false, "tcscf_batch_scf: DIIS with TC needs mgs_mode = 1");
+      }
+      const size_t n = tc_ws_doubles(it.nup, it.ndown, nb, it.n1);
+      if (hipMalloc((void**)&x.ws, n * sizeof(double)) != hipSuccess) {
+        cleanup();
+        TC_CHECK(false, "tcscf_batch_scf: out of device memory");
+      }
+      if ((rc = tc_prepare(x.tcp, it.nup, it.ndown, nb, it.A1_dev, it.dA1_dev, it.w1_dev, it.n1, it.Fs_dev,
+                           it.Fo_dev, it.V_dev, &it.jastrow, x.ws))) {
+        cleanup();
+        return rc;
+      }
+      x.nspin = 2;
+      x.tci = std::make_unique<TcIter>(it.nup, it.ndown, nb, it.S, it.H, o);
+      x.up = std::make_unique<Pinned>(2 * nb2 + 2 * (size_t)nb * x.tcp.nmax);
+      x.down = std::make_unique<Pinned>(tc_out_doubles(nb));
+      x.G2.assign(2 * nb2, 0.0);
+      x.G3.assign(2 * nb2, 0.0);
+    } else {
+      const bool rcs = it.kind == TCSCF_SCF_RCS;
+      if (rcs && (it.nup + it.ndown) % 2 != 0) {
+        cleanup();
+        TC_CHECK(false, "tcscf_batch_scf: RCS needs an even number of electrons");
+      }
+      x.nspin = rcs ? 1 : 2;
+      const int nocc[2] = {rcs ? (it.nup + it.ndown) / 2 : it.nup, it.ndown};
+      if (hipMalloc((void**)&x.ws, hf_ws_doubles(x.nspin, nb) * sizeof(double)) != hipSuccess) {
+        cleanup();
+        TC_CHECK(false, "tcscf_batch_scf: out of device memory");
+      }
+      if ((rc = hf_prepare(x.hf, x.nspin, nb, it.n1, it.F_dev, it.f_dtype == TCSCF_F32, it.A1_dev, it.w1_dev, x.ws))) {
+        cleanup();
+        return rc;
+      }
+      x.hfi = std::make_unique<HfIter>(x.nspin, nocc, nb, it.S, it.H, o);
+      x.up = std::make_unique<Pinned>(x.nspin * nb2);
+      x.down = std::make_unique<Pinned>(x.nspin * nb2);
+    }
+    if (!x.up->p || !x.down->p) {
+      cleanup();
+      TC_CHECK(false, "tcscf_batch_scf: pinned host allocation failed");
+    }
+  }
+
+  hipStream_t cs = nullptr, br[kBranches] = {};
+  hipEvent_t fork = nullptr, join[kBranches] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  auto release = [&] {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (int b = 0; b < kBranches; ++b) {
+      if (join[b]) (void)hipEventDestroy(join[b]);
+      if (br[b]) (void)hipStreamDestroy(br[b]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (cs) (void)hipStreamDestroy(cs);
+    cleanup();
+  };
+#define TC_BHIP(x)                                         \
+  do {                                                     \
+    hipError_t e_ = (x);                                   \
+    if (e_ != hipSuccess) {                                \
+      set_error(std::string("HIP: ") + hipGetErrorString(e_)); \
+      release();                                           \
+      return TCSCF_EHIP;                                   \
+    }                                                      \
+  } while (0)
+  TC_BHIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  for (int b = 0; b < kBranches; ++b) {
+    TC_BHIP(hipStreamCreateWithFlags(&br[b], hipStreamNonBlocking));
+    TC_BHIP(hipEventCreateWithFlags(&join[b], hipEventDisableTiming));
+  }
+  TC_BHIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+
+  const int threads = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+  std::vector<int> active, built;
+  int builds = 0;
+  double t_dev = 0, t_host = 0;
+  for (;;) {
+    active.clear();
+    for (int k = 0; k < count; ++k)
+      if (inst[k].active) active.push_back(k);
+    if (active.empty()) break;
+    if (active != built) {  // capture the Fock builds of the active set
+      if (exec) TC_BHIP(hipGraphExecDestroy(exec));
+      if (graph) TC_BHIP(hipGraphDestroy(graph));
+      exec = nullptr;
+      graph = nullptr;
+      TC_BHIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      TC_BHIP(hipEventRecord(fork, cs));
+      for (int b = 0; b < kBranches; ++b) TC_BHIP(hipStreamWaitEvent(br[b], fork, 0));
+      int erc = 0;
+      for (size_t q = 0; q < active.size() && !erc; ++q) {
+        Instance& x = inst[active[q]];
+        hipStream_t s = br[q % kBranches];
+        if (x.tci) erc = tc_enqueue(x.tcp, x.up->p, x.up->p + 2 * (int64_t)x.item->nb * x.item->nb, x.down->p, s);
+        else erc = hf_enqueue(x.hf, x.up->p, x.down->p, s);
+      }
+      for (int b = 0; b < kBranches; ++b) {
+        TC_BHIP(hipEventRecord(join[b], br[b]));
+        TC_BHIP(hipStreamWaitEvent(cs, join[b], 0));
+      }
+      const hipError_t ec = hipStreamEndCapture(cs, &graph);
+      if (erc) {
+        release();
+        return erc;
+      }
+      TC_BHIP(ec);
+      TC_BHIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      built = active;
+      ++builds;
+    }
+    // current densities (and TC orbitals) into the pinned upload buffers
+    for (int k : active) {
+      Instance& x = inst[k];
+      if (x.tci) {
+        std::memcpy(x.up->p, x.tci->D.data(), x.tci->D.size() * sizeof(double));
+        std::memcpy(x.up->p + x.tci->D.size(), x.tci->C.data(), x.tci->C.size() * sizeof(double));
+      } else {
+        std::memcpy(x.up->p, x.hfi->D.data(), x.hfi->D.size() * sizeof(double));
+      }
+    }
+    const double t0 = wall_ms();
+    TC_BHIP(hipGraphLaunch(exec, cs));
+    TC_BHIP(hipStreamSynchronize(cs));
+    const double t1 = wall_ms();
+    t_dev += t1 - t0;
+    parallel_for((int)active.size(), threads, [&](int q) {
+      Instance& x = inst[active[q]];
+      tcscf_scf_result& r = results[active[q]];
+      bool done = false;
+      if (x.tci) {
+        tc_assemble(x.tcp, x.down->p, x.G2.data(), x.G3.data());
+        x.rc = x.tci->step(x.G2.data(), x.G3.data(), x.it, &r, x.item->energy_trace, &done);
+      } else {
+        x.rc = x.hfi->step(x.down->p, x.it, &r, x.item->energy_trace, &done);
+      }
+      ++x.it;
+      if (done || x.rc || x.it >= o->max_iter) x.active = false;
+      if (x.rc) r.status = x.rc;
+    });
+    t_host += wall_ms() - t1;
+  }
+  for (int k = 0; k < count; ++k) {
+    Instance& x = inst[k];
+    results[k].t_sweep_ms = t_dev;
+    results[k].t_solve_ms = t_host;
+    if (x.item->D_out) {
+      const std::vector<double>& D = x.tci ? x.tci->D : x.hfi->D;
+      std::copy(D.begin(), D.end(), x.item->D_out);
+    }
+  }
+  if (graph_builds) *graph_builds = builds;
+  release();
+#undef TC_BHIP
+  return 0;
+}

@@ -13,6 +13,7 @@
 
 #include "common.hpp"
 #include "dense.hpp"
+#include "scf_steps.hpp"
 
 namespace tc {
 
@@ -168,73 +169,6 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Device side of one Fock-build sweep: G (nspin x nb x nb) from D.
-struct Sweep {
-  int nspin, nb;
-  int64_t n1, npair;
-  const void* F;
-  int f32;
-  const double* A;
-  const double* w;
-  hipStream_t st;
-  double* dD = nullptr;
-  double* dPart = nullptr;
-  double* dG = nullptr;
-  int nbp = 32;
-  size_t shmem = 0;
-
-  int init() {
-    const int64_t nb2 = (int64_t)nb * nb;
-    TC_CHECK(nb <= 64, "SCF sweep: nb > 64 is not supported by the sweep kernel");
-    TC_CHECK(nspin * nb2 <= 256 * kGPerThread, "SCF sweep: nb too large for the register-held G partials");
-    int rc = 0;
-    char* p = (char*)scratch(kSlotSweep, sizeof(double) * (nspin * nb2 * (kSweepBlocks + 2)), &rc);
-    if (rc) return rc;
-    dPart = (double*)p;
-    dD = dPart + (int64_t)kSweepBlocks * nspin * nb2;
-    dG = dD + nspin * nb2;
-    nbp = nb <= 32 ? 32 : 64;
-    const int ppw = 64 / nbp, ch = nbp == 64 ? 16 : 32;
-    shmem = sizeof(double) * (nspin * nb2 + 4 * ppw * npair + ch * nb * (1 + nspin) + 4 * ppw * nspin * nb);
-    TC_CHECK(shmem <= 160 * 1024, "SCF sweep: nb too large for the LDS staging");
-    if (shmem > 64 * 1024) {
-      // opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU)
-      const void* fns[8] = {(const void*)hf_sweep_kernel<float, 1, 32>,  (const void*)hf_sweep_kernel<float, 1, 64>,
-                            (const void*)hf_sweep_kernel<double, 1, 32>, (const void*)hf_sweep_kernel<double, 1, 64>,
-                            (const void*)hf_sweep_kernel<float, 2, 32>,  (const void*)hf_sweep_kernel<float, 2, 64>,
-                            (const void*)hf_sweep_kernel<double, 2, 32>, (const void*)hf_sweep_kernel<double, 2, 64>};
-      for (const void* f : fns)
-        TC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    }
-    return 0;
-  }
-
-  int run(const double* D_host, double* G_host) {
-    const int64_t nb2 = (int64_t)nb * nb;
-    TC_HIP(hipMemcpyAsync(dD, D_host, sizeof(double) * nspin * nb2, hipMemcpyHostToDevice, st));
-    const double cs = nspin == 1 ? 2.0 : 1.0, co = nspin == 1 ? 0.0 : 1.0;
-    dim3 grid(kSweepBlocks), block(256);
-#define TC_SWEEP(TF, NS, NBP)                                                                               \
-  hipLaunchKernelGGL((hf_sweep_kernel<TF, NS, NBP>), grid, block, shmem, st, (const TF*)F, npair, A, w, n1, nb, \
-                     dD, cs, co, dPart)
-    if (nspin == 1) {
-      if (f32) { if (nbp == 32) TC_SWEEP(float, 1, 32); else TC_SWEEP(float, 1, 64); }
-      else { if (nbp == 32) TC_SWEEP(double, 1, 32); else TC_SWEEP(double, 1, 64); }
-    } else {
-      if (f32) { if (nbp == 32) TC_SWEEP(float, 2, 32); else TC_SWEEP(float, 2, 64); }
-      else { if (nbp == 32) TC_SWEEP(double, 2, 32); else TC_SWEEP(double, 2, 64); }
-    }
-#undef TC_SWEEP
-    TC_HIP(hipGetLastError());
-    const int64_t len = nspin * nb2;
-    hipLaunchKernelGGL(reduce_partials, dim3((unsigned)len), dim3(256), 0, st, dPart, kSweepBlocks, len, dG);
-    TC_HIP(hipGetLastError());
-    TC_HIP(hipMemcpyAsync(G_host, dG, sizeof(double) * len, hipMemcpyDeviceToHost, st));
-    TC_HIP(hipStreamSynchronize(st));
-    return 0;
-  }
-};
-
 void mask_by(int nb, double* G, const double* H) {
   for (int q = 0; q < nb * nb; ++q)
     if (!(std::fabs(H[q]) > 0)) G[q] = 0;
@@ -250,68 +184,144 @@ void density_from(int nb, int nocc, const std::vector<double>& C, double* D) {
     }
 }
 
+}  // namespace
+
+size_t hf_ws_doubles(int nspin, int nb) { return (size_t)nspin * nb * nb * (kSweepBlocks + 2); }
+
+int hf_prepare(HfPlan& p, int nspin, int nb, int64_t n1, const void* F, int f32, const double* A, const double* w,
+               double* ws) {
+  const int64_t nb2 = (int64_t)nb * nb;
+  TC_CHECK(nb <= 64, "SCF sweep: nb > 64 is not supported by the sweep kernel");
+  TC_CHECK(nspin * nb2 <= 256 * kGPerThread, "SCF sweep: nb too large for the register-held G partials");
+  p.nspin = nspin;
+  p.nb = nb;
+  p.n1 = n1;
+  p.npair = (int64_t)nb * (nb + 1) / 2;
+  p.F = F;
+  p.f32 = f32;
+  p.A = A;
+  p.w = w;
+  p.dPart = ws;
+  p.dD = p.dPart + (int64_t)kSweepBlocks * nspin * nb2;
+  p.dG = p.dD + nspin * nb2;
+  p.nbp = nb <= 32 ? 32 : 64;
+  const int ppw = 64 / p.nbp, ch = p.nbp == 64 ? 16 : 32;
+  p.shmem = sizeof(double) * (nspin * nb2 + 4 * ppw * p.npair + ch * nb * (1 + nspin) + 4 * ppw * nspin * nb);
+  TC_CHECK(p.shmem <= 160 * 1024, "SCF sweep: nb too large for the LDS staging");
+  if (p.shmem > 64 * 1024) {
+    // opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU)
+    const void* fns[8] = {(const void*)hf_sweep_kernel<float, 1, 32>,  (const void*)hf_sweep_kernel<float, 1, 64>,
+                          (const void*)hf_sweep_kernel<double, 1, 32>, (const void*)hf_sweep_kernel<double, 1, 64>,
+                          (const void*)hf_sweep_kernel<float, 2, 32>,  (const void*)hf_sweep_kernel<float, 2, 64>,
+                          (const void*)hf_sweep_kernel<double, 2, 32>, (const void*)hf_sweep_kernel<double, 2, 64>};
+    for (const void* f : fns) TC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.shmem));
+  }
+  return 0;
+}
+
+int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st) {
+  const int64_t nb2 = (int64_t)p.nb * p.nb;
+  TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * p.nspin * nb2, hipMemcpyHostToDevice, st));
+  const double cs = p.nspin == 1 ? 2.0 : 1.0, co = p.nspin == 1 ? 0.0 : 1.0;
+  dim3 grid(kSweepBlocks), block(256);
+#define TC_SWEEP(TF, NS, NBP)                                                                                   \
+  hipLaunchKernelGGL((hf_sweep_kernel<TF, NS, NBP>), grid, block, p.shmem, st, (const TF*)p.F, p.npair, p.A, p.w, \
+                     p.n1, p.nb, p.dD, cs, co, p.dPart)
+  if (p.nspin == 1) {
+    if (p.f32) { if (p.nbp == 32) TC_SWEEP(float, 1, 32); else TC_SWEEP(float, 1, 64); }
+    else { if (p.nbp == 32) TC_SWEEP(double, 1, 32); else TC_SWEEP(double, 1, 64); }
+  } else {
+    if (p.f32) { if (p.nbp == 32) TC_SWEEP(float, 2, 32); else TC_SWEEP(float, 2, 64); }
+    else { if (p.nbp == 32) TC_SWEEP(double, 2, 32); else TC_SWEEP(double, 2, 64); }
+  }
+#undef TC_SWEEP
+  TC_HIP(hipGetLastError());
+  const int64_t len = p.nspin * nb2;
+  hipLaunchKernelGGL(reduce_partials, dim3((unsigned)len), dim3(256), 0, st, p.dPart, kSweepBlocks, len, p.dG);
+  TC_HIP(hipGetLastError());
+  TC_HIP(hipMemcpyAsync(G_host, p.dG, sizeof(double) * len, hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+HfIter::HfIter(int nspin_, const int* nocc_, int nb_, const double* S_, const double* H_, const tcscf_scf_opts* o_)
+    : nspin(nspin_), nb(nb_), nocc{nocc_[0], nspin_ > 1 ? nocc_[1] : 0}, S(S_), H(H_), o(o_),
+      D((size_t)nspin_ * nb_ * nb_, 0.0), Fock((size_t)nspin_ * nb_ * nb_), evals((size_t)nspin_ * nb_, 0.0),
+      diis(o_->diis) {}
+
+int HfIter::step(double* G, int it, tcscf_scf_result* res, double* trace, bool* done) {
+  const int64_t nb2 = (int64_t)nb * nb;
+  *done = false;
+  if (o->mask)
+    for (int s = 0; s < nspin; ++s) mask_by(nb, G + s * nb2, H);
+  // constructSCFOperator (HartreeFock.cpp:48-98); RCS energy x2 (:850)
+  double e = 0;
+  for (int s = 0; s < nspin; ++s)
+    for (int j = 0; j < nb; ++j)
+      for (int i = 0; i < nb; ++i) {
+        const double g = G[s * nb2 + j * nb + i];
+        Fock[s * nb2 + j * nb + i] = H[j * nb + i] + g;
+        e += D[s * nb2 + i * nb + j] * (H[j * nb + i] + g / 2);
+      }
+  if (nspin == 1) e *= 2;
+  if (trace) trace[it] = e;
+  res->iterations = it + 1;
+  if (std::fabs(e - energy) < o->tol) {
+    res->energy = e;
+    res->status = TCSCF_OK;
+    *done = true;
+    return 0;
+  }
+  energy = e;
+  if (o->diis > 0 && it > 0) {  // iteration 0 has D = 0, whose error is 0 trivially
+    dense::commutator_error(nspin, nb, Fock.data(), D.data(), S, err);
+    diis.extrapolate(Fock, err);
+  }
+  for (int s = 0; s < nspin; ++s) {
+    std::vector<double> ev;
+    if (!dense::gen_sym_eig(nb, Fock.data() + s * nb2, S, ev, C)) {
+      res->status = TCSCF_ESOLVER;
+      set_error("SCF: generalized eigen-solve failed (overlap not positive definite?)");
+      return TCSCF_ESOLVER;
+    }
+    std::copy(ev.begin(), ev.end(), evals.begin() + s * nb);
+    density_from(nb, nocc[s], C, D.data() + s * nb2);
+  }
+  return 0;
+}
+
+namespace {
+
 int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H, const double* A1, const double* w1,
            int64_t n1, const void* F, int f_dtype, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
            double* D_out, double* evals_out, hipStream_t st) {
   TC_CHECK(S && H && A1 && w1 && F && o && res && nb > 0 && n1 > 0, "SCF: bad arguments");
   TC_CHECK(o->max_iter > 0, "SCF: max_iter must be positive");
-  const int64_t nb2 = (int64_t)nb * nb;
-  Sweep sw{nspin, nb, n1, (int64_t)nb * (nb + 1) / 2, F, f_dtype == TCSCF_F32, A1, w1, st};
-  int rc = sw.init();
-  if (rc) return rc;
   TC_CHECK(o->diis >= 0, "SCF: diis depth must be >= 0");
-  std::vector<double> D(nspin * nb2, 0.0), G(nspin * nb2), Fock(nspin * nb2), evals(nspin * nb, 0.0), C, err;
-  dense::Diis diis(o->diis);
-  double energy = std::numeric_limits<double>::max();
+  const int64_t nb2 = (int64_t)nb * nb;
+  int rc = 0;
+  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * hf_ws_doubles(nspin, nb), &rc);
+  if (rc) return rc;
+  HfPlan plan;
+  if ((rc = hf_prepare(plan, nspin, nb, n1, F, f_dtype == TCSCF_F32, A1, w1, ws))) return rc;
+  HfIter iter(nspin, nocc, nb, S, H, o);
+  std::vector<double> G(nspin * nb2);
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
   res->iterations = 0;
   res->t_sweep_ms = res->t_solve_ms = 0;
   for (int it = 0; it < o->max_iter; ++it) {
     double t0 = now_ms();
-    rc = sw.run(D.data(), G.data());
-    if (rc) return rc;
+    if ((rc = hf_enqueue(plan, iter.D.data(), G.data(), st))) return rc;
+    TC_HIP(hipStreamSynchronize(st));
     double t1 = now_ms();
     res->t_sweep_ms += t1 - t0;
-    if (o->mask)
-      for (int s = 0; s < nspin; ++s) mask_by(nb, G.data() + s * nb2, H);
-    // constructSCFOperator (HartreeFock.cpp:48-98); RCS energy x2 (:850)
-    double e = 0;
-    for (int s = 0; s < nspin; ++s)
-      for (int j = 0; j < nb; ++j)
-        for (int i = 0; i < nb; ++i) {
-          const double g = G[s * nb2 + j * nb + i];
-          Fock[s * nb2 + j * nb + i] = H[j * nb + i] + g;
-          e += D[s * nb2 + i * nb + j] * (H[j * nb + i] + g / 2);
-        }
-    if (nspin == 1) e *= 2;
-    if (trace) trace[it] = e;
-    res->iterations = it + 1;
-    if (std::fabs(e - energy) < o->tol) {
-      res->energy = e;
-      res->status = TCSCF_OK;
-      res->t_solve_ms += now_ms() - t1;
-      break;
-    }
-    energy = e;
-    if (o->diis > 0 && it > 0) {  // iteration 0 has D = 0, whose error is 0 trivially
-      dense::commutator_error(nspin, nb, Fock.data(), D.data(), S, err);
-      diis.extrapolate(Fock, err);
-    }
-    for (int s = 0; s < nspin; ++s) {
-      std::vector<double> ev;
-      if (!dense::gen_sym_eig(nb, Fock.data() + s * nb2, S, ev, C)) {
-        res->status = TCSCF_ESOLVER;
-        set_error("SCF: generalized eigen-solve failed (overlap not positive definite?)");
-        return TCSCF_ESOLVER;
-      }
-      std::copy(ev.begin(), ev.end(), evals.begin() + s * nb);
-      density_from(nb, nocc[s], C, D.data() + s * nb2);
-    }
+    bool done = false;
+    if ((rc = iter.step(G.data(), it, res, trace, &done))) return rc;
     res->t_solve_ms += now_ms() - t1;
+    if (done) break;
   }
-  if (D_out) std::copy(D.begin(), D.end(), D_out);
-  if (evals_out) std::copy(evals.begin(), evals.end(), evals_out);
+  if (D_out) std::copy(iter.D.begin(), iter.D.end(), D_out);
+  if (evals_out) std::copy(iter.evals.begin(), iter.evals.end(), evals_out);
   if (res->status != TCSCF_OK) set_error("SCF: did not converge");
   return res->status;
 }

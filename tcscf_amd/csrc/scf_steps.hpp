@@ -1,0 +1,93 @@
+// The SCF drivers split into reusable pieces: a device plan per instance (its
+// own workspace, asynchronous enqueue of one Fock build between pinned host
+// buffers) and the host half of one iteration.  The single-instance C entry
+// points and the batched driver (one graph of every instance's Fock build per
+// iteration) share them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <limits>
+#include <vector>
+
+#include "common.hpp"
+#include "dense.hpp"
+
+namespace tc {
+
+/* ---------------------------------------------------------------- RCS / UOS */
+struct HfPlan {
+  int nspin, nb;
+  int64_t n1, npair;
+  const void* F;
+  int f32;
+  const double* A;
+  const double* w;
+  double* dD;     // nspin x nb x nb
+  double* dPart;  // per-workgroup partial G
+  double* dG;     // nspin x nb x nb
+  int nbp;
+  size_t shmem;
+};
+size_t hf_ws_doubles(int nspin, int nb);
+// Checks sizes, points the plan at ws (hf_ws_doubles of it) and opts the sweep
+// kernels in to their LDS size.
+int hf_prepare(HfPlan& p, int nspin, int nb, int64_t n1, const void* F, int f32, const double* A, const double* w,
+               double* ws);
+// D (host, nspin x nb x nb) -> G (host), asynchronous on st.
+int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st);
+
+// Host half of one RCS / UOS iteration (src/HartreeFock.cpp:785-1003): mask,
+// Fock, energy from the current density, convergence test, optional DIIS,
+// generalized eigen-solve and new density.
+struct HfIter {
+  int nspin, nb;
+  int nocc[2];
+  const double* S;
+  const double* H;
+  const tcscf_scf_opts* o;
+  std::vector<double> D, Fock, evals, C, err;
+  dense::Diis diis;
+  double energy = std::numeric_limits<double>::max();
+  HfIter(int nspin_, const int* nocc_, int nb_, const double* S_, const double* H_, const tcscf_scf_opts* o_);
+  // G: nspin x nb x nb from the device (modified: the mask is applied in place).
+  // Returns 0, or a negative status; *done is set when converged (res->energy set).
+  int step(double* G, int it, tcscf_scf_result* res, double* trace, bool* done);
+};
+
+/* ------------------------------------------------------------------------ TC */
+struct TcPlan {
+  int nup, ndown, nb, nmax, nvec;
+  int64_t n1, npair;
+  const double *A1, *dA1, *w1, *Fs, *Fo, *V;
+  double c_same, c_oppo;
+  double *dD, *dC, *dVec, *dAl, *dH, *dPG, *dPV, *dOut;
+  int wpw;
+  size_t sh1;
+  int64_t pblocks;
+};
+size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1);
+size_t tc_out_doubles(int nb);  // the per-build result block copied back to the host
+int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1,
+               int64_t n1, const double* Fs, const double* Fo, const double* V, const tcscf_jastrow* jp, double* ws);
+// D (2 x nb x nb) and C (2 x nb x nmax) host -> out (host, tc_out_doubles), async.
+int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, hipStream_t st);
+// out -> G2, G3 (2 x nb x nb each, before the mask): G2 = scalar/2 + (vec/2 - vec^T/2),
+// G3 = scalar3/2 (src/HartreeFock.cpp:1209-1222).
+void tc_assemble(const TcPlan& p, const double* out, double* G2, double* G3);
+
+// Host half of one TC iteration (src/HartreeFock.cpp:1007-1362).
+struct TcIter {
+  int nb, ne[2], nmax;
+  const double* S;
+  const double* H;
+  const tcscf_scf_opts* o;
+  std::vector<double> D, C, Fock, pseudo, err;
+  dense::Diis diis;
+  double energy = std::numeric_limits<double>::max();
+  TcIter(int nup, int ndown, int nb_, const double* S_, const double* H_, const tcscf_scf_opts* o_);
+  int step(double* G2, double* G3, int it, tcscf_scf_result* res, double* trace, bool* done);
+};
+
+}  // namespace tc

@@ -25,6 +25,7 @@
 
 #include "common.hpp"
 #include "dense.hpp"
+#include "scf_steps.hpp"
 
 namespace tc {
 namespace tcs {
@@ -472,6 +473,176 @@ inline int64_t pk(int j, int i) { return j <= i ? (int64_t)i * (i + 1) / 2 + j :
 }  // namespace tcs
 }  // namespace tc
 
+namespace tc {
+using namespace tcs;
+
+size_t tc_out_doubles(int nb) { return 6 * (size_t)nb * nb + 4 * (size_t)nb * (nb + 1) / 2; }
+
+size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1) {
+  const int nmax = std::max(1, std::max(nup, ndown));
+  const size_t nb2 = (size_t)nb * nb, npair = (size_t)nb * (nb + 1) / 2, nvec = kVecFixed + 3 * nmax;
+  return 2 * nb2 + 2 * (size_t)nb * nmax + (size_t)n1 * 2 * nvec * nb + (size_t)n1 * 2 + (size_t)n1 * 12 +
+         (size_t)kBlocks * 6 * nb2 + (size_t)kBlocks * 4 * npair + tc_out_doubles(nb);
+}
+
+int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1,
+               int64_t n1, const double* Fs, const double* Fo, const double* V, const tcscf_jastrow* jp, double* ws) {
+  TC_CHECK(A1 && dA1 && w1 && Fs && Fo && V && jp && ws, "tcscf_tc_fock: null");
+  TC_CHECK(nb > 0 && nb <= 64, "tcscf_tc_fock: 1 <= nb <= 64");
+  TC_CHECK(nup >= 0 && ndown >= 0 && nup <= nb && ndown <= nb, "tcscf_tc_fock: bad electron counts");
+  p.nup = nup;
+  p.ndown = ndown;
+  p.nb = nb;
+  p.nmax = std::max(1, std::max(nup, ndown));
+  p.nvec = kVecFixed + 3 * p.nmax;
+  p.n1 = n1;
+  p.npair = (int64_t)nb * (nb + 1) / 2;
+  p.A1 = A1;
+  p.dA1 = dA1;
+  p.w1 = w1;
+  p.Fs = Fs;
+  p.Fo = Fo;
+  p.V = V;
+  p.c_same = jp->c_same;
+  p.c_oppo = jp->c_oppo;
+  const int64_t nb2 = (int64_t)nb * nb;
+  p.dD = ws;
+  p.dC = p.dD + 2 * nb2;
+  p.dVec = p.dC + 2 * (int64_t)nb * p.nmax;
+  p.dAl = p.dVec + n1 * 2 * p.nvec * nb;
+  p.dH = p.dAl + n1 * 2;
+  p.dPG = p.dH + n1 * 12;
+  p.dPV = p.dPG + (int64_t)kBlocks * 6 * nb2;
+  p.dOut = p.dPV + (int64_t)kBlocks * 4 * p.npair;
+  // waves per workgroup: as many (up to 4) as the LDS holds next to D and C
+  const size_t shared = sizeof(double) * (2 * nb2 + 2 * (int64_t)nb * p.nmax);
+  const size_t per_wave = sizeof(double) * point_wave_doubles(nb, p.nmax);
+  p.wpw = 4;
+  while (p.wpw > 1 && shared + p.wpw * per_wave > 160 * 1024) p.wpw /= 2;
+  p.sh1 = shared + p.wpw * per_wave;
+  TC_CHECK(p.sh1 <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
+  if (p.sh1 > 64 * 1024)
+    TC_HIP(hipFuncSetAttribute((const void*)tc_point_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.sh1));
+  p.pblocks = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + p.wpw - 1) / p.wpw));
+  return 0;
+}
+
+int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, hipStream_t st) {
+  const int64_t nb2 = (int64_t)p.nb * p.nb;
+  TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * 2 * nb2, hipMemcpyHostToDevice, st));
+  TC_HIP(hipMemcpyAsync(p.dC, C_host, sizeof(double) * 2 * p.nb * p.nmax, hipMemcpyHostToDevice, st));
+  PointArgs a{p.Fs, p.Fo, p.V, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb, p.nmax,
+              p.nup, p.ndown, p.nvec, p.dVec, p.dAl, p.dH};
+  hipLaunchKernelGGL(tc_point_kernel, dim3((unsigned)p.pblocks), dim3(64 * p.wpw), p.sh1, st, a);
+  TC_HIP(hipGetLastError());
+  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.dPG};
+  if (p.nb <= 32)
+    hipLaunchKernelGGL(tc_rank_kernel<2>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
+  else if (p.nb <= 48)
+    hipLaunchKernelGGL(tc_rank_kernel<3>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
+  else
+    hipLaunchKernelGGL(tc_rank_kernel<4>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
+  TC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(tc_vlin_kernel, dim3(kBlocks), dim3(256), 0, st, p.V, p.dH, p.n1, p.npair, p.dPV);
+  TC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(6 * nb2)), dim3(256), 0, st, p.dPG, kBlocks, 6 * nb2, p.dOut);
+  TC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(4 * p.npair)), dim3(256), 0, st, p.dPV, kBlocks, 4 * p.npair,
+                     p.dOut + 6 * nb2);
+  TC_HIP(hipGetLastError());
+  TC_HIP(hipMemcpyAsync(out_host, p.dOut, sizeof(double) * tc_out_doubles(p.nb), hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+void tc_assemble(const TcPlan& p, const double* out, double* G2, double* G3) {
+  const int nb = p.nb;
+  const int64_t nb2 = (int64_t)nb * nb;
+  for (int s = 0; s < 2; ++s) {
+    const double* Sc = out + (s * 3 + 0) * nb2;
+    const double* Vc = out + (s * 3 + 1) * nb2;
+    const double* S3 = out + (s * 3 + 2) * nb2;
+    const double* Rvec = out + 6 * nb2 + (s * 2 + 0) * p.npair;
+    const double* Rg3 = out + 6 * nb2 + (s * 2 + 1) * p.npair;
+    for (int j = 0; j < nb; ++j)
+      for (int i = 0; i < nb; ++i) {
+        const double vji = Vc[j * nb + i] + Rvec[pk(j, i)];
+        const double vij = Vc[i * nb + j] + Rvec[pk(i, j)];
+        G2[s * nb2 + j * nb + i] = Sc[j * nb + i] / 2 + (vji / 2 - vij / 2);
+        G3[s * nb2 + j * nb + i] = (S3[j * nb + i] + Rg3[pk(j, i)]) / 2;
+      }
+  }
+}
+
+TcIter::TcIter(int nup, int ndown, int nb_, const double* S_, const double* H_, const tcscf_scf_opts* o_)
+    : nb(nb_), ne{nup, ndown}, nmax(std::max(1, std::max(nup, ndown))), S(S_), H(H_), o(o_),
+      D(2 * (size_t)nb_ * nb_, 0.0), C(2 * (size_t)nb_ * std::max(1, std::max(nup, ndown)), 0.0),
+      Fock(2 * (size_t)nb_ * nb_), pseudo(nup + ndown, 0.0), diis(o_->diis) {}
+
+int TcIter::step(double* G2, double* G3, int it, tcscf_scf_result* res, double* trace, bool* done) {
+  const int64_t nb2 = (int64_t)nb * nb;
+  *done = false;
+  if (o->mask)
+    for (int q = 0; q < 2 * nb2; ++q)
+      if (!(std::fabs(H[q % nb2]) > 0)) G2[q] = G3[q] = 0;
+  double e = 0;
+  for (int s = 0; s < 2; ++s)
+    for (int j = 0; j < nb; ++j)
+      for (int i = 0; i < nb; ++i) {
+        const double g2 = G2[s * nb2 + j * nb + i], g3 = G3[s * nb2 + j * nb + i];
+        Fock[s * nb2 + j * nb + i] = H[j * nb + i] + g2 - g3;
+        e += D[s * nb2 + i * nb + j] * (H[j * nb + i] + g2 / 2 - g3 / 3);
+      }
+  if (trace) trace[it] = e;
+  res->iterations = it + 1;
+  if (std::fabs(e - energy) < o->tol) {
+    res->energy = e;
+    res->status = TCSCF_OK;
+    *done = true;
+    return 0;
+  }
+  energy = e;
+  if (o->diis > 0 && it > 0) {  // iteration 0 has D = 0, whose error is 0 trivially
+    dense::right_eigen_error(2, nb, Fock.data(), D.data(), S, err);
+    diis.extrapolate(Fock, err);
+  }
+  for (int s = 0; s < 2; ++s) {
+    std::vector<std::complex<double>> ev, X;
+    if (!dense::gen_nonsym_eig(nb, Fock.data() + s * nb2, S, ev, X)) {
+      res->status = TCSCF_ESOLVER;
+      set_error("TC SCF: generalized eigen-solve failed");
+      return TCSCF_ESOLVER;
+    }
+    std::vector<int> idx(nb);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a].real() < ev[b].real(); });
+    const int no = ne[s];
+    std::vector<std::complex<double>> vec((size_t)no * nb);
+    for (int b = 0; b < no; ++b) {
+      pseudo[s * ne[0] + b] = ev[idx[b]].real();
+      for (int j = 0; j < nb; ++j) vec[b * nb + j] = X[j * nb + idx[b]];
+    }
+    dense::mgs(no, nb, vec, S, o->mgs_mode == 0);
+    // real basis: remove each orbital's global phase (density unchanged)
+    for (int b = 0; b < no; ++b) {
+      int jm = 0;
+      for (int j = 1; j < nb; ++j)
+        if (std::abs(vec[b * nb + j]) > std::abs(vec[b * nb + jm])) jm = j;
+      const std::complex<double> ph =
+          std::abs(vec[b * nb + jm]) > 0 ? std::conj(vec[b * nb + jm]) / std::abs(vec[b * nb + jm]) : 1.0;
+      for (int j = 0; j < nb; ++j) C[((size_t)s * nb + j) * nmax + b] = (vec[b * nb + j] * ph).real();
+    }
+    for (int u = 0; u < nb; ++u)
+      for (int v = 0; v < nb; ++v) {
+        double t = 0;
+        for (int b = 0; b < no; ++b) t += C[((size_t)s * nb + u) * nmax + b] * C[((size_t)s * nb + v) * nmax + b];
+        D[s * nb2 + u * nb + v] = t;
+      }
+  }
+  return 0;
+}
+
+}  // namespace tc
+
 using namespace tc;
 using namespace tc::tcs;
 
@@ -483,75 +654,17 @@ extern "C" {
 int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1, int64_t n1,
                   const double* Fs, const double* Fo, const double* V, const tcscf_jastrow* jp, const double* D_host,
                   const double* C_host, double* G2_host, double* G3_host, void* stream) {
-  TC_CHECK(A1 && dA1 && w1 && Fs && Fo && V && jp && D_host && C_host && G2_host && G3_host, "tcscf_tc_fock: null");
-  TC_CHECK(nb > 0 && nb <= 64, "tcscf_tc_fock: 1 <= nb <= 64");
-  TC_CHECK(nup >= 0 && ndown >= 0 && nup <= nb && ndown <= nb, "tcscf_tc_fock: bad electron counts");
+  TC_CHECK(D_host && C_host && G2_host && G3_host, "tcscf_tc_fock: null");
   hipStream_t st = (hipStream_t)stream;
-  const int nmax = std::max(1, std::max(nup, ndown));
-  const int nvec = kVecFixed + 3 * nmax;
-  const int64_t npair = (int64_t)nb * (nb + 1) / 2, nb2 = (int64_t)nb * nb;
-  // device scratch: D, C, per-point vectors, alpha, hvec, partials, results
-  const size_t nD = 2 * nb2, nC = 2 * (size_t)nb * nmax, nVec = (size_t)n1 * 2 * nvec * nb, nAl = (size_t)n1 * 2,
-               nH = (size_t)n1 * 12;
-  const size_t nPartG = (size_t)kBlocks * 6 * nb2, nPartV = (size_t)kBlocks * 4 * npair, nOut = 6 * nb2 + 4 * npair;
   int rc = 0;
-  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * (nD + nC + nVec + nAl + nH + nPartG + nPartV + nOut), &rc);
+  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1), &rc);
   if (rc) return rc;
-  double* dD = ws;
-  double* dC = dD + nD;
-  double* dVec = dC + nC;
-  double* dAl = dVec + nVec;
-  double* dH = dAl + nAl;
-  double* dPG = dH + nH;
-  double* dPV = dPG + nPartG;
-  double* dOut = dPV + nPartV;
-  TC_HIP(hipMemcpyAsync(dD, D_host, sizeof(double) * nD, hipMemcpyHostToDevice, st));
-  TC_HIP(hipMemcpyAsync(dC, C_host, sizeof(double) * nC, hipMemcpyHostToDevice, st));
-  PointArgs a{Fs, Fo, V, A1, dA1, w1, dD, dC, jp->c_same, jp->c_oppo, n1, npair, nb, nmax, nup, ndown, nvec,
-              dVec, dAl, dH};
-  // waves per workgroup: as many (up to 4) as the LDS holds next to D and C
-  const size_t shared = sizeof(double) * (2 * nb2 + 2 * (int64_t)nb * nmax);
-  const size_t per_wave = sizeof(double) * point_wave_doubles(nb, nmax);
-  int wpw = 4;
-  while (wpw > 1 && shared + wpw * per_wave > 160 * 1024) wpw /= 2;
-  const size_t sh1 = shared + wpw * per_wave;
-  TC_CHECK(sh1 <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
-  if (sh1 > 64 * 1024) TC_HIP(hipFuncSetAttribute((const void*)tc_point_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh1));
-  const int64_t pblocks = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + wpw - 1) / wpw));
-  hipLaunchKernelGGL(tc_point_kernel, dim3((unsigned)pblocks), dim3(64 * wpw), sh1, st, a);
-  TC_HIP(hipGetLastError());
-  RankArgs ra{dVec, dAl, A1, dA1, w1, n1, nb, nvec, nup, ndown, dPG};
-  if (nb <= 32)
-    hipLaunchKernelGGL(tc_rank_kernel<2>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
-  else if (nb <= 48)
-    hipLaunchKernelGGL(tc_rank_kernel<3>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
-  else
-    hipLaunchKernelGGL(tc_rank_kernel<4>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
-  TC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tc_vlin_kernel, dim3(kBlocks), dim3(256), 0, st, V, dH, n1, npair, dPV);
-  TC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(6 * nb2)), dim3(256), 0, st, dPG, kBlocks, 6 * nb2, dOut);
-  TC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(4 * npair)), dim3(256), 0, st, dPV, kBlocks, 4 * npair, dOut + 6 * nb2);
-  TC_HIP(hipGetLastError());
-  std::vector<double> out(nOut);
-  TC_HIP(hipMemcpyAsync(out.data(), dOut, sizeof(double) * nOut, hipMemcpyDeviceToHost, st));
+  TcPlan plan;
+  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, jp, ws))) return rc;
+  std::vector<double> out(tc_out_doubles(nb));
+  if ((rc = tc_enqueue(plan, D_host, C_host, out.data(), st))) return rc;
   TC_HIP(hipStreamSynchronize(st));
-  // G2 = scalar/2 + (vec/2 - vec^T/2), G3 = scalar3/2 (HartreeFock.cpp:1209-1222)
-  for (int s = 0; s < 2; ++s) {
-    const double* Sc = out.data() + (s * 3 + 0) * nb2;
-    const double* Vc = out.data() + (s * 3 + 1) * nb2;
-    const double* S3 = out.data() + (s * 3 + 2) * nb2;
-    const double* Rvec = out.data() + 6 * nb2 + (s * 2 + 0) * npair;
-    const double* Rg3 = out.data() + 6 * nb2 + (s * 2 + 1) * npair;
-    for (int j = 0; j < nb; ++j)
-      for (int i = 0; i < nb; ++i) {
-        const double vji = Vc[j * nb + i] + Rvec[pk(j, i)];
-        const double vij = Vc[i * nb + j] + Rvec[pk(i, j)];
-        G2_host[s * nb2 + j * nb + i] = Sc[j * nb + i] / 2 + (vji / 2 - vij / 2);
-        G3_host[s * nb2 + j * nb + i] = (S3[j * nb + i] + Rg3[pk(j, i)]) / 2;
-      }
-  }
+  tc_assemble(plan, out.data(), G2_host, G3_host);
   return 0;
 }
 
@@ -563,88 +676,35 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
                  const tcscf_jastrow* jp, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
                  double* D_out, double* pseudo_out, void* stream) {
   TC_CHECK(S && H && o && res && o->max_iter > 0, "tcscf_tc_scf: bad arguments");
-  const int64_t nb2 = (int64_t)nb * nb;
-  const int ne[2] = {nup, ndown};
-  const int nmax = std::max(1, std::max(nup, ndown));
-  std::vector<double> D(2 * nb2, 0.0), C(2 * (size_t)nb * nmax, 0.0), G2(2 * nb2), G3(2 * nb2), Fock(2 * nb2);
-  std::vector<double> pseudo(nup + ndown, 0.0), err;
   TC_CHECK(o->diis >= 0, "tcscf_tc_scf: diis depth must be >= 0");
-  TC_CHECK(o->diis == 0 || o->mgs_mode == 1,
-           "tcscf_tc_scf: DIIS needs S-orthonormal orbitals (mgs_mode = 1)");
-  dense::Diis diis(o->diis);
-  double energy = std::numeric_limits<double>::max();
+  TC_CHECK(o->diis == 0 || o->mgs_mode == 1, "tcscf_tc_scf: DIIS needs S-orthonormal orbitals (mgs_mode = 1)");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nb2 = (int64_t)nb * nb;
+  int rc = 0;
+  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1), &rc);
+  if (rc) return rc;
+  TcPlan plan;
+  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, jp, ws))) return rc;
+  TcIter iter(nup, ndown, nb, S, H, o);
+  std::vector<double> out(tc_out_doubles(nb)), G2(2 * nb2), G3(2 * nb2);
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
   res->iterations = 0;
   res->t_sweep_ms = res->t_solve_ms = 0;
   for (int it = 0; it < o->max_iter; ++it) {
     double t0 = now_ms();
-    int rc = tcscf_tc_fock(nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, jp, D.data(), C.data(), G2.data(), G3.data(),
-                           stream);
-    if (rc) return rc;
+    if ((rc = tc_enqueue(plan, iter.D.data(), iter.C.data(), out.data(), st))) return rc;
+    TC_HIP(hipStreamSynchronize(st));
+    tc_assemble(plan, out.data(), G2.data(), G3.data());
     double t1 = now_ms();
     res->t_sweep_ms += t1 - t0;
-    if (o->mask)
-      for (int q = 0; q < 2 * nb2; ++q)
-        if (!(std::fabs(H[q % nb2]) > 0)) G2[q] = G3[q] = 0;
-    double e = 0;
-    for (int s = 0; s < 2; ++s)
-      for (int j = 0; j < nb; ++j)
-        for (int i = 0; i < nb; ++i) {
-          const double g2 = G2[s * nb2 + j * nb + i], g3 = G3[s * nb2 + j * nb + i];
-          Fock[s * nb2 + j * nb + i] = H[j * nb + i] + g2 - g3;
-          e += D[s * nb2 + i * nb + j] * (H[j * nb + i] + g2 / 2 - g3 / 3);
-        }
-    if (trace) trace[it] = e;
-    res->iterations = it + 1;
-    if (std::fabs(e - energy) < o->tol) {
-      res->energy = e;
-      res->status = TCSCF_OK;
-      res->t_solve_ms += now_ms() - t1;
-      break;
-    }
-    energy = e;
-    if (o->diis > 0 && it > 0) {  // iteration 0 has D = 0, whose error is 0 trivially
-      dense::right_eigen_error(2, nb, Fock.data(), D.data(), S, err);
-      diis.extrapolate(Fock, err);
-    }
-    for (int s = 0; s < 2; ++s) {
-      std::vector<std::complex<double>> ev, X;
-      if (!dense::gen_nonsym_eig(nb, Fock.data() + s * nb2, S, ev, X)) {
-        res->status = TCSCF_ESOLVER;
-        set_error("TC SCF: generalized eigen-solve failed");
-        return TCSCF_ESOLVER;
-      }
-      std::vector<int> idx(nb);
-      std::iota(idx.begin(), idx.end(), 0);
-      std::sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a].real() < ev[b].real(); });
-      const int no = ne[s];
-      std::vector<std::complex<double>> vec((size_t)no * nb);
-      for (int b = 0; b < no; ++b) {
-        pseudo[s * nup + b] = ev[idx[b]].real();
-        for (int j = 0; j < nb; ++j) vec[b * nb + j] = X[j * nb + idx[b]];
-      }
-      dense::mgs(no, nb, vec, S, o->mgs_mode == 0);
-      // real basis: remove each orbital's global phase (density unchanged)
-      for (int b = 0; b < no; ++b) {
-        int jm = 0;
-        for (int j = 1; j < nb; ++j)
-          if (std::abs(vec[b * nb + j]) > std::abs(vec[b * nb + jm])) jm = j;
-        const std::complex<double> ph =
-            std::abs(vec[b * nb + jm]) > 0 ? std::conj(vec[b * nb + jm]) / std::abs(vec[b * nb + jm]) : 1.0;
-        for (int j = 0; j < nb; ++j) C[((size_t)s * nb + j) * nmax + b] = (vec[b * nb + j] * ph).real();
-      }
-      for (int u = 0; u < nb; ++u)
-        for (int v = 0; v < nb; ++v) {
-          double t = 0;
-          for (int b = 0; b < no; ++b) t += C[((size_t)s * nb + u) * nmax + b] * C[((size_t)s * nb + v) * nmax + b];
-          D[s * nb2 + u * nb + v] = t;
-        }
-    }
+    bool done = false;
+    if ((rc = iter.step(G2.data(), G3.data(), it, res, trace, &done))) return rc;
     res->t_solve_ms += now_ms() - t1;
+    if (done) break;
   }
-  if (D_out) std::copy(D.begin(), D.end(), D_out);
-  if (pseudo_out) std::copy(pseudo.begin(), pseudo.end(), pseudo_out);
+  if (D_out) std::copy(iter.D.begin(), iter.D.end(), D_out);
+  if (pseudo_out) std::copy(iter.pseudo.begin(), iter.pseudo.end(), pseudo_out);
   if (res->status != TCSCF_OK) set_error("TC SCF: did not converge");
   return res->status;
 }

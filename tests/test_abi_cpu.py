@@ -184,3 +184,20 @@ def test_cpp_facade_host_checks(tc):
     r = subprocess.run([exe, "cpu", BASIS_DIR], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failure(s)" in r.stdout
+
+
+def test_batch_workload_generator_is_pinned(tc):
+    """The batched-serving instance list is reproducible from its seed and
+    matches the committed fixture; it spans the workload's ranges."""
+    import json
+    from tcscf_amd import configs
+    with open(os.path.join(ROOT, "tests", "golden", "batch64_instances.json")) as f:
+        fixture = json.load(f)
+    drawn = json.loads(json.dumps(configs.batch_instances()))
+    assert drawn == fixture
+    assert len(drawn) == 64
+    assert {x["Z"] for x in drawn} <= set(range(2, 19))
+    assert min(x["npts"] for x in drawn) == 1250 and max(x["npts"] for x in drawn) == 30200
+    assert sum(x["method"] == "tc" for x in drawn) == 32
+    assert all(x["nb"] == len(configs.batch_basis(x)) and x["nb"] >= x["nup"] + 2 for x in drawn)
+    assert all(0.5 <= x["b"] <= 2 for x in drawn if x["method"] == "tc")

@@ -607,3 +607,47 @@ def test_batched_contraction_matches_single_calls(tc, gpu):
     # FP64 goes instance by instance and equals the single calls too
     outs = tc.computeF_batch(specs[:2], precision="fp64")
     assert torch.equal(outs[0], tc.computeF(specs[0]["r1"], specs[0]["r2"]))
+
+
+def test_batched_scf_matches_single_solves(tc, gpu):
+    """tcscf_batch_scf (a HIP graph of every active instance's Fock build per
+    iteration) gives the single-instance drivers' energies and iteration counts
+    for a mix of RCS, UOS and TC instances of different sizes, and recaptures
+    its graph as instances converge."""
+    from tcscf_amd import SlaterTypeOrbital as STO
+    cases = []
+    for Z, nup, ndown, kind, shells, grid in [(2, 1, 1, "rcs", {0: 5}, (20, 11)), (4, 2, 2, "rcs", {0: 6, 1: 1}, (30, 17)),
+                                              (3, 2, 1, "uos", {0: 6, 1: 1}, (25, 11)), (4, 2, 2, "tc", {0: 5, 1: 2}, (14, 11)),
+                                              (5, 3, 2, "tc", {0: 7, 1: 2}, (20, 17))]:
+        basis = tc.even_tempered(shells, 0.5, 2.0)
+        g = _grid(tc, *grid, tc.RADIAL_M4)
+        g.setBasisFunctions(basis, kind == "tc")
+        S, H = tc.one_electron_matrices(basis, Z)
+        d = dict(kind=kind, nup=nup, ndown=ndown, S=S, H=H, r1=g)
+        if kind == "tc":
+            jas = tc.Jastrow.from_b(1.0)
+            d.update(Fs=tc.computeF(g, g, kernel="tc", jastrow=jas, same_spin=True),
+                     Fo=tc.computeF(g, g, kernel="tc", jastrow=jas, same_spin=False), V=tc.computeV(g, g, jas),
+                     jastrow=jas)
+        else:
+            d.update(F=tc.computeF(g, g))
+        cases.append(d)
+    for diis in (0, 6):
+        res, builds, _, _ = tc.batch_scf(cases, tol=1e-10, max_iter=200, mgs_mode=1, diis=diis)
+        assert builds >= 2
+        for d, r in zip(cases, res):
+            assert r["status"] == 0, (d["kind"], r)
+            nb = d["r1"].nBasis()
+            if d["kind"] == "tc":
+                hf = tc.TCHartreeFock(d["nup"], d["ndown"], nb)
+                e = hf.compute(d["S"], d["H"], d["r1"], d["Fs"], d["Fo"], d["V"], d["jastrow"], True, tol=1e-10,
+                               max_iter=200, mgs_mode=1, diis=diis)
+            elif d["kind"] == "rcs":
+                hf = tc.RCSHartreeFock(d["nup"] + d["ndown"], nb)
+                e = hf.compute(d["S"], d["H"], d["r1"], d["F"], True, tol=1e-10, max_iter=200, diis=diis)
+            else:
+                hf = tc.UOSHartreeFock(d["nup"], d["ndown"], nb)
+                e = hf.compute(d["S"], d["H"], d["r1"], d["F"], True, tol=1e-10, max_iter=200, diis=diis)
+            assert r["energy"] == e, (d["kind"], r["energy"], e)
+            assert r["iterations"] == hf.numberOfConvergenceLoops()
+            assert np.array_equal(r["trace"], hf.energy_trace)
