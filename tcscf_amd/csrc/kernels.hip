@@ -512,9 +512,12 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 // One workgroup tile of the BF16 contraction: rows row0 .. row0 + 16 MB of r1,
 // 16-column blocks cbw .. cbw + 15 of the pairs, all of r2.  Output element
 // (row, component c, pair col) goes to Fout[row * ld_row + c * npair + col].
-// ktile: the workgroup's K tile (LDS, [2][NC][MB][2][64] fragments).
+// ktile: the workgroup's K tile (LDS, [2][NC][MB][2][64] fragments); crd: a
+// two-slot LDS ring of r2 coordinates (32 points per step), filled two steps
+// ahead by direct-to-LDS loads of wave 0 so no register waits on them.
 template <int MB, int NPROD, int FOLD, class Kern>
-__device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], const double* __restrict__ p1,
+__device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], float4 (*crd)[32],
+                                          const double* __restrict__ p1,
                                           int64_t n1, const float4* __restrict__ x2, int64_t n2p,
                                           const us8* __restrict__ P0, int64_t plane, int64_t ncb,
                                           void* __restrict__ Fout, int out_f32, int64_t npair, int64_t ld_row,
@@ -547,13 +550,22 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
     az = ok ? (float)p1[2 * n1 + grow] : (float)-kFar;
   }
 
-  float4 qn[EPT];  // r2 coordinates of the step to generate next
-  auto load_coords = [&](int64_t k0) {
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) qn[e] = x2[k0 + gk + e];
+  // coordinates of step t -> crd[t & 1] (lanes 0..31 of wave 0, 16 bytes each)
+  auto fetch_coords = [&](int64_t t) {
+    if (wave == 0 && lane < 32)
+      __builtin_amdgcn_global_load_lds((const void*)(x2 + t * 32 + lane),
+                                       (__attribute__((address_space(3))) void*)&crd[t & 1][0], 16, 0, 0);
   };
-  // K = hi + lo, both bf16 (round to nearest even, v_cvt_pk_bf16_f32)
+  // wave 0 waits for its direct-to-LDS loads before the barrier that publishes them
+  auto coords_landed = [&] {
+    if (wave == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  };
+  // K = hi + lo, both bf16 (round to nearest even, v_cvt_pk_bf16_f32), for the
+  // step whose coordinates sit in crd[buf]
   auto generate = [&](int buf) {
+    float4 qn[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) qn[e] = crd[buf][gk + e];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       unsigned hw[EPT / 2], lw[EPT / 2];
@@ -603,9 +615,11 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
 
   const int64_t nsteps = n2p / 32;
   us8 bqA[NSPLIT][NB], bqB[NSPLIT][NB];  // B fragments, ping-pong between steps
-  load_coords(0);
+  fetch_coords(0);
+  if (nsteps > 1) fetch_coords(1);
+  coords_landed();
+  __syncthreads();
   generate(0);
-  if (nsteps > 1) load_coords(32);
   load_b(bqA, 0);
   __syncthreads();
 
@@ -615,6 +629,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
   auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto fresh) {
     constexpr bool FRESH = decltype(fresh)::value;
     const int buf = (int)(s & 1);
+    if (s + 2 < nsteps) fetch_coords(s + 2);  // into crd[s & 1], read last by generate(s) one step ago
     if (s + 1 < nsteps) load_b(nxt, s + 1);
     const f4 zero = {0, 0, 0, 0};
 #pragma unroll
@@ -638,10 +653,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
           acc[c][m][n] = v;
         }
       }
-    if (s + 1 < nsteps) {
-      generate(buf ^ 1);
-      if (s + 2 < nsteps) load_coords((s + 2) * 32);
-    }
+    if (s + 1 < nsteps) generate(buf ^ 1);
     if (!FRESH || FOLD == 1 || s + 1 == nsteps) {
 #pragma unroll
       for (int c = 0; c < NC; ++c)
@@ -652,6 +664,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
 #pragma unroll
             for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
     }
+    coords_landed();
     __syncthreads();
   };
   using T = std::integral_constant<bool, true>;
@@ -685,7 +698,8 @@ __global__ __launch_bounds__(256, Kern::ncomp == 1 ? 2 : 1) void fgemm_bf16_kern
                                                          int64_t ncb, void* __restrict__ Fout, int out_f32,
                                                          int64_t npair, double scale, Kern kern) {
   __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
-  bf16_tile<MB, NPROD, FOLD, Kern>(ktile, p1, n1, x2, n2p, P0, plane, ncb, Fout, out_f32, npair,
+  __shared__ float4 crd[2][32];
+  bf16_tile<MB, NPROD, FOLD, Kern>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, out_f32, npair,
                                    Kern::ncomp * npair, scale, kern, (int64_t)blockIdx.y * (16 * MB),
                                    (int64_t)blockIdx.x * 16);
 }
@@ -1253,6 +1267,7 @@ __global__ __launch_bounds__(256, 2) void bf16_batch_kernel(const BatchDesc* __r
                                                             int64_t t_begin, int64_t t_end,
                                                             unsigned long long* __restrict__ counter) {
   __shared__ us8 ktile[2][1][MB][2][64];
+  __shared__ float4 crd[2][32];
   __shared__ int64_t cur;
   for (;;) {
     if (threadIdx.x == 0) cur = t_begin + (int64_t)atomicAdd(counter, 1ull);
@@ -1272,7 +1287,7 @@ __global__ __launch_bounds__(256, 2) void bf16_batch_kernel(const BatchDesc* __r
     local /= d.ncomp;
     const int64_t cb = local % d.cblocks, rb = local / d.cblocks;
     const KernAny kern{d.kind, comp, d.a12, d.c2a2};
-    bf16_tile<MB, NPROD, 1, KernAny>(ktile, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, d.F + comp * d.npair, 0,
+    bf16_tile<MB, NPROD, 1, KernAny>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, d.F + comp * d.npair, 0,
                                      d.npair, d.ncomp * d.npair, d.scale, kern, rb * (16 * MB), cb * 16);
   }
 }
