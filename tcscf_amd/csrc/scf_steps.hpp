@@ -63,7 +63,7 @@ struct TcPlan {
   const double *A1, *dA1, *w1, *Fs, *Fo, *V;
   double c_same, c_oppo;
   double *dD, *dC, *dVec, *dAl, *dH, *dPG, *dPV, *dOut;
-  int wpw;
+  int wpw, stage;
   size_t sh1;
   int64_t pblocks;
 };

@@ -33,7 +33,8 @@ namespace tcs {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlocks = 512;
-constexpr int kWChunk = 4;  // occupied orbitals per register pass of W = V C
+constexpr int kWMain = 5;   // occupied orbitals of W = V C per pass over j (the first in the main pass)
+constexpr int kWChunk = 2;  // orbitals of W per LDS round of |U|^2
 
 // Compact per-point output, per spin s, vectors of nb doubles (lane i = basis
 // index i):
@@ -65,10 +66,43 @@ struct PointArgs {
   double* hvec;       // [n1][2 spins][2 targets (VEC, G3)][3]
 };
 
+// Cross-lane sums in registers: DPP moves within a row of 16 lanes (pairs at
+// lane distance 1 and 2, then the half-row and row mirrors), the gfx950
+// permlane swaps across rows (16 and 32 lanes apart).  Each step adds a lane's
+// value to its partner's; fp addition is commutative, so both partners hold
+// the same bits and every lane of the group ends with the identical sum.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// v + v of the lane 16 (ROWS = 1) or 32 (ROWS = 2) lanes away.
+template <int ROWS>
+__device__ inline double swap_sum(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+  unsigned __attribute__((ext_vector_type(2))) l, h;
+  if (ROWS == 1) {
+    l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  } else {
+    l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  }
+  const double x = __longlong_as_double((long long)(((unsigned long long)h[0] << 32) | l[0]));
+  const double y = __longlong_as_double((long long)(((unsigned long long)h[1] << 32) | l[1]));
+  return x + y;
+}
+
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror
+  return swap_sum<2>(swap_sum<1>(v));
 }
 
 // A wave-uniform value (e.g. a wave_sum result) moved to scalar registers.
@@ -85,21 +119,32 @@ __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-__host__ __device__ inline int64_t point_wave_doubles(int nb, int nmax) { return 24 * (int64_t)nb + 6 * (int64_t)nb * nmax; }
+__host__ __device__ inline int64_t point_wave_doubles(int nb) { return 24 * (int64_t)nb; }
 
 // Sum of v over the lanes that share a basis index i (lane = jp * nbp + i).
 __device__ inline double part_sum(double v, int nbp) {
-  for (int off = nbp; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+  if (nbp == 16) v = swap_sum<1>(v);
+  if (nbp <= 32) v = swap_sum<2>(v);
   return v;
 }
 
+// Doubles of staged V rows per wave ([3][npair] when staged).
+__host__ __device__ inline int64_t staged_rows(bool stage, int nb) { return stage ? 3 * ((int64_t)nb * (nb + 1) / 2) : 0; }
+
 // Pass 1: one wave per r1 point.  Lane = jp * nbp + i: i is the basis index
 // (nbp = 16, 32 or 64 >= nb) and the 64 / nbp lane groups jp split every loop
-// over a second basis index, so each lane walks nb * nbp / 64 terms.  F and V
-// are read from global memory (each row once from HBM; the symmetric (j, i) /
-// (i, j) re-reads hit the caches), D and C are shared in LDS, the per-point
-// vectors live in a small per-wave LDS area.
-__global__ __launch_bounds__(256, 2) void tc_point_kernel(PointArgs a) {
+// over a second basis index, so each lane walks nb * nbp / 64 terms.  The
+// point's packed F_same / F_oppo / V rows are read at triangular (j, i)
+// addresses, a gather that touches a separate cache line for almost every
+// lane.  One pass over j forms every F- and V-linear intermediate (F / V
+// against D, X and kS, and W = V C for the first kWMain orbitals), a second
+// pass V . E once E is known.  STAGE copies the V rows (read in both passes)
+// into the wave's LDS area by direct-to-LDS loads first, so those gathers hit
+// LDS; it pays when 8 waves of a workgroup fit in LDS and hide the latency of
+// the dependent passes.  D and C are shared in LDS, the per-point vectors
+// live in a small per-wave LDS area.
+template <bool STAGE>
+__global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
   extern __shared__ double smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int nb = a.nb, nmax = a.nmax;
@@ -107,14 +152,15 @@ __global__ __launch_bounds__(256, 2) void tc_point_kernel(PointArgs a) {
   const int64_t np = a.npair, nb2 = (int64_t)nb * nb;
   double* Ds = smem;                       // [2][nb][nb]
   double* Cs = Ds + 2 * nb2;               // [2][nb][nmax]
-  double* base = Cs + 2 * (int64_t)nb * nmax + wave * point_wave_doubles(nb, nmax);
+  double* base = Cs + 2 * (int64_t)nb * nmax + wave * (point_wave_doubles(nb) + staged_rows(STAGE, nb));
   double* As = base;                       // [nb]
   double* dAs = As + nb;                   // [nb][3]
   double* Xs = dAs + 3 * nb;               // [2][nb]
-  double* kS = Xs + 2 * nb;                // [2][nb][3]
-  double* VT = kS + 6 * nb;                // VTSI [2][nb][3]
-  double* Es = VT + 6 * nb;                // [2][nb][3]
-  double* Ws = Es + 6 * nb;                // [2][nb][nmax][3]
+  double* VT = Xs + 2 * nb;                // VTSI [2][nb][3]
+  double* kS = VT + 6 * nb;                // [2][nb][3]
+  double* Es = kS + 6 * nb;                // [2][nb][3]
+  double* Wc = kS;                         // [2][nb][kWChunk][3], once kS and Es are consumed
+  double* Sv = Es + 6 * nb;                // STAGE: the point's V rows [3][npair]
   for (int64_t e = tid; e < 2 * nb2; e += blockDim.x) Ds[e] = a.D[e];
   for (int64_t e = tid; e < 2 * (int64_t)nb * nmax; e += blockDim.x) Cs[e] = a.C[e];
   __syncthreads();
@@ -126,11 +172,46 @@ __global__ __launch_bounds__(256, 2) void tc_point_kernel(PointArgs a) {
   const double cs = a.c_same, co = a.c_oppo;
   const int nocc[2] = {a.nocc0, a.nocc1};
   const int64_t nvw = (int64_t)gridDim.x * nwaves;
+  // body(j, f_s, f_o, v0, v1, v2) for every j of the lane
+  auto for_j = [&](const double* fs, const double* fo, const double* vv, auto&& body) {
+    if (act)
+      for (int j = jp; j < nb; j += parts) {
+        const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
+        body(j, fs ? fs[pj] : 0.0, fo ? fo[pj] : 0.0, vv[pj], vv[np + pj], vv[2 * np + pj]);
+      }
+  };
+
+  // wreg[s][b][c] += C_s[j][b0 + b] V_c(j, i) for the occupied orbitals b0 + b
+  auto w_accumulate = [&](double (&wreg)[2][kWMain][3], int b0, int j, double v0, double v1, double v2) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const double* crow = Cs + ((int64_t)s * nb + j) * nmax + b0;
+#pragma unroll
+      for (int b = 0; b < kWMain; ++b)
+        if (b0 + b < nocc[s]) {
+          const double cb = crow[b];
+          wreg[s][b][0] += cb * v0;
+          wreg[s][b][1] += cb * v1;
+          wreg[s][b][2] += cb * v2;
+        }
+    }
+  };
 
   for (int64_t n = (int64_t)blockIdx.x * nwaves + wave; n < a.n1; n += nvw) {
     const double* __restrict__ fs = a.Fs + n * np;
     const double* __restrict__ fo = a.Fo + n * np;
     const double* __restrict__ vv = a.V + n * 3 * np;
+    if (STAGE) {
+      // the V rows copied by direct-to-LDS loads (lane l of a load writes
+      // dword k + l of the run), all loads in flight together, one wait
+      const unsigned* g = reinterpret_cast<const unsigned*>(vv);
+      unsigned* l = reinterpret_cast<unsigned*>(Sv);
+      for (int64_t k = 0; k < 6 * np; k += 64)
+        if (k + lane < 6 * np)
+          __builtin_amdgcn_global_load_lds(g + k + lane, (__attribute__((address_space(3))) void*)(l + k), 4, 0, 0);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      vv = Sv;
+    }
     if (own) {
       As[i] = a.A[n * nb + i];
       for (int c = 0; c < 3; ++c) dAs[i * 3 + c] = a.dA[(n * nb + i) * 3 + c];
@@ -169,29 +250,79 @@ __global__ __launch_bounds__(256, 2) void tc_point_kernel(PointArgs a) {
       occ[s] = uniform(wave_sum(x * Ai));  // calculateOccupiedElectronDensity
       for (int c = 0; c < 3; ++c) sg[s][c] = uniform(wave_sum(x * dAi[c]));  // calculateScaledGradients
     }
-    // one pass over j: Faa, FaiS, V . D, Vai, Q (= VTSI / c_s)
+    // one pass over j: Faa, FaiS, V . D, Vai, Q (= VTSI / c_s) and
+    // W_s[i][b][c] = sum_j C_s[j][b] V_c(j, i) for the first kWMain orbitals
     double faa_p[2] = {0, 0}, fai[2] = {0, 0}, vd[2][3] = {{0, 0, 0}, {0, 0, 0}}, vai1[2] = {0, 0};
     double q[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    if (act) {
-      for (int j = jp; j < nb; j += parts) {
-        const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
-        const double f_s = fs[pj], f_o = fo[pj];
-        const double v0 = vv[pj], v1 = vv[np + pj], v2 = vv[2 * np + pj];
+    double wreg[2][kWMain][3];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int b = 0; b < kWMain; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
+    for_j(fs, fo, vv, [&](int j, double f_s, double f_o, double v0, double v1, double v2) {
+      w_accumulate(wreg, 0, j, v0, v1, v2);
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const double dsj = Ds[s * nb2 + j * nb + i], dbj = Ds[(1 - s) * nb2 + j * nb + i];
+        faa_p[s] += f_s * dsj + f_o * dbj;
+        const double xj = Xs[s * nb + j];
+        fai[s] += xj * f_s;
+        vd[s][0] += v0 * dsj;
+        vd[s][1] += v1 * dsj;
+        vd[s][2] += v2 * dsj;
+        const double* ks = kS + (s * nb + j) * 3;
+        vai1[s] += v0 * ks[0] + v1 * ks[1] + v2 * ks[2];
+        q[s][0] += xj * v0;
+        q[s][1] += xj * v1;
+        q[s][2] += xj * v2;
+      }
+    });
+    // W blocks of kWMain orbitals: the first from the main pass, the rest
+    // from passes of their own; each block goes to vout and, kWChunk orbitals
+    // at a time through LDS (over kS and Es, consumed / not yet written), adds
+    // its part of sum_ab |U_s[a][b]|^2, U_s[a][b][c] = sum_l C_s[l][a] W_s[l][b][c]
+    // (calculateVab)
+    double* vout = a.vecs + (int64_t)n * 2 * a.nvec * nb;
+    double nacc[2] = {0, 0};
+    for (int b0 = 0; b0 < nmax; b0 += kWMain) {
+      if (b0 > 0) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int b = 0; b < kWMain; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
+        for_j(nullptr, nullptr, vv, [&](int j, double, double, double v0, double v1, double v2) {
+          w_accumulate(wreg, b0, j, v0, v1, v2);
+        });
+      }
+#pragma unroll
+      for (int c0 = 0; c0 < kWMain; c0 += kWChunk) {
+        if (b0 + c0 >= nmax) break;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int b = c0; b < c0 + kWChunk && b < kWMain; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const double wv = part_sum(b0 + b < nocc[s] ? wreg[s][b][c] : 0.0, nbp);
+              if (own) {
+                Wc[((s * nb + i) * kWChunk + b - c0) * 3 + c] = wv;
+                if (b0 + b < nmax) vout[((int64_t)s * a.nvec + kVecFixed + 3 * (b0 + b) + c) * nb + i] = wv;
+              }
+            }
+        wave_sync_lds();
+        const int lim = std::min(nmax - b0 - c0, kWMain - c0), nbc = lim < kWChunk ? lim : kWChunk;
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-          const double dsj = Ds[s * nb2 + j * nb + i], dbj = Ds[(1 - s) * nb2 + j * nb + i];
-          faa_p[s] += f_s * dsj + f_o * dbj;
-          const double xj = Xs[s * nb + j];
-          fai[s] += xj * f_s;
-          vd[s][0] += v0 * dsj;
-          vd[s][1] += v1 * dsj;
-          vd[s][2] += v2 * dsj;
-          const double* ks = kS + (s * nb + j) * 3;
-          vai1[s] += v0 * ks[0] + v1 * ks[1] + v2 * ks[2];
-          q[s][0] += xj * v0;
-          q[s][1] += xj * v1;
-          q[s][2] += xj * v2;
+          const int nu = nocc[s] * nbc * 3;  // (a, b in the chunk, c)
+          for (int t = lane; t < nu; t += 64) {
+            const int aa = t / (nbc * 3), rem = t - aa * nbc * 3, bb = rem / 3, c = rem - bb * 3;
+            double u = 0;
+            for (int l = 0; l < nb; ++l)
+              u += Cs[((int64_t)s * nb + l) * nmax + aa] * Wc[((s * nb + l) * kWChunk + bb) * 3 + c];
+            nacc[s] += u * u;
+          }
         }
+        wave_sync_lds();
       }
     }
     double faa[2], VTv[2][3];
@@ -232,72 +363,20 @@ __global__ __launch_bounds__(256, 2) void tc_point_kernel(PointArgs a) {
     wave_sync_lds();
     // SumOverb_Vbi_dot_VTildeb_s[i] = c_s sum_k V(k,i) . E_s[k]
     double sv1[2] = {0, 0};
-    if (act)
-      for (int k = jp; k < nb; k += parts) {
-        const int64_t pk = k <= i ? tri_i + k : k * (k + 1) / 2 + i;
-        const double v0 = vv[pk], v1 = vv[np + pk], v2 = vv[2 * np + pk];
-        for (int s = 0; s < 2; ++s) {
-          const double* e = Es + (s * nb + k) * 3;
-          sv1[s] += v0 * e[0] + v1 * e[1] + v2 * e[2];
-        }
+    for_j(nullptr, nullptr, vv, [&](int k, double, double, double v0, double v1, double v2) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const double* e = Es + (s * nb + k) * 3;
+        sv1[s] += v0 * e[0] + v1 * e[1] + v2 * e[2];
       }
+    });
     sv1[0] = part_sum(sv1[0], nbp);
     sv1[1] = part_sum(sv1[1], nbp);
-    // W_s[i][b][c] = sum_j C_s[j][b] V_c(j, i), kWChunk orbitals per pass over j
-    double* vout = a.vecs + (int64_t)n * 2 * a.nvec * nb;
-    for (int b0 = 0; b0 < nmax; b0 += kWChunk) {
-      double wreg[2][kWChunk][3];
-#pragma unroll
-      for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int b = 0; b < kWChunk; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
-      if (act) {
-        for (int j = jp; j < nb; j += parts) {
-          const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
-          const double v0 = vv[pj], v1 = vv[np + pj], v2 = vv[2 * np + pj];
-#pragma unroll
-          for (int s = 0; s < 2; ++s) {
-            const double* crow = Cs + ((int64_t)s * nb + j) * nmax + b0;
-#pragma unroll
-            for (int b = 0; b < kWChunk; ++b)
-              if (b0 + b < nocc[s]) {
-                const double cb = crow[b];
-                wreg[s][b][0] += cb * v0;
-                wreg[s][b][1] += cb * v1;
-                wreg[s][b][2] += cb * v2;
-              }
-          }
-        }
-      }
-#pragma unroll
-      for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int b = 0; b < kWChunk; ++b)
-          if (b0 + b < nmax)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              const double wv = part_sum(b0 + b < nocc[s] ? wreg[s][b][c] : 0.0, nbp);
-              if (own) {
-                Ws[(((int64_t)(s * nb + i) * nmax) + b0 + b) * 3 + c] = wv;
-                vout[((int64_t)s * a.nvec + kVecFixed + 3 * (b0 + b) + c) * nb + i] = wv;
-              }
-            }
-    }
-    wave_sync_lds();
     double scvt[2][3], nrmU[2];
     for (int s = 0; s < 2; ++s) {
       const double x = own ? Xs[s * nb + i] : 0.0;
       for (int c = 0; c < 3; ++c) scvt[s][c] = uniform(wave_sum(own ? x * VT[(s * nb + i) * 3 + c] : 0.0));  // SumOverb_conjVTildeb_occb
-      // sum_ab |U_s[a][b]|^2, U_s[a][b][c] = sum_l C_s[l][a] W_s[l][b][c]  (calculateVab)
-      double acc = 0;
-      const int nu = nocc[s] * nocc[s] * 3;
-      for (int t = lane; t < nu; t += 64) {
-        const int aa = t / (nocc[s] * 3), rem = t - aa * nocc[s] * 3, bb = rem / 3, c = rem - bb * 3;
-        double u = 0;
-        for (int l = 0; l < nb; ++l) u += Cs[((int64_t)s * nb + l) * nmax + aa] * Ws[(((int64_t)s * nb + l) * nmax + bb) * 3 + c];
-        acc += u * u;
-      }
-      nrmU[s] = uniform(wave_sum(acc));
+      nrmU[s] = uniform(wave_sum(nacc[s]));
     }
     const double wn = a.w[n];
     for (int s = 0; s < 2; ++s) {
@@ -516,13 +595,19 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   p.dOut = p.dPV + (int64_t)kBlocks * 4 * p.npair;
   // waves per workgroup: as many (up to 4) as the LDS holds next to D and C
   const size_t shared = sizeof(double) * (2 * nb2 + 2 * (int64_t)nb * p.nmax);
-  const size_t per_wave = sizeof(double) * point_wave_doubles(nb, p.nmax);
-  p.wpw = 4;
+  // the V rows staged in LDS when 8 waves of a workgroup hold them (the gathers
+  // then come from LDS and 8 waves hide their latency), else gathers from
+  // memory with up to 4 waves per workgroup
+  const size_t staged_wave = sizeof(double) * (point_wave_doubles(nb) + staged_rows(1, nb));
+  p.stage = shared + 8 * staged_wave <= 160 * 1024 ? 1 : 0;
+  const size_t per_wave = p.stage ? staged_wave : sizeof(double) * point_wave_doubles(nb);
+  p.wpw = p.stage ? 8 : 4;
   while (p.wpw > 1 && shared + p.wpw * per_wave > 160 * 1024) p.wpw /= 2;
   p.sh1 = shared + p.wpw * per_wave;
   TC_CHECK(p.sh1 <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
   if (p.sh1 > 64 * 1024)
-    TC_HIP(hipFuncSetAttribute((const void*)tc_point_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.sh1));
+    TC_HIP(hipFuncSetAttribute(p.stage ? (const void*)tc_point_kernel<true> : (const void*)tc_point_kernel<false>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.sh1));
   p.pblocks = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + p.wpw - 1) / p.wpw));
   return 0;
 }
@@ -533,7 +618,10 @@ int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, doub
   TC_HIP(hipMemcpyAsync(p.dC, C_host, sizeof(double) * 2 * p.nb * p.nmax, hipMemcpyHostToDevice, st));
   PointArgs a{p.Fs, p.Fo, p.V, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb, p.nmax,
               p.nup, p.ndown, p.nvec, p.dVec, p.dAl, p.dH};
-  hipLaunchKernelGGL(tc_point_kernel, dim3((unsigned)p.pblocks), dim3(64 * p.wpw), p.sh1, st, a);
+  if (p.stage)
+    hipLaunchKernelGGL(tc_point_kernel<true>, dim3((unsigned)p.pblocks), dim3(64 * p.wpw), p.sh1, st, a);
+  else
+    hipLaunchKernelGGL(tc_point_kernel<false>, dim3((unsigned)p.pblocks), dim3(64 * p.wpw), p.sh1, st, a);
   TC_HIP(hipGetLastError());
   RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.dPG};
   if (p.nb <= 32)

@@ -379,6 +379,32 @@ def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown):
     assert rel_err(G3, G3q) < 1e-11
 
 
+@pytest.mark.parametrize("nbasis,nup,ndown", [(12, 7, 6), (20, 6, 7), (36, 9, 8)])
+def test_tc_fock_build_many_orbitals_vs_oracle(tc, O, gpu, nbasis, nup, ndown):
+    """The device TC Fock build with more occupied orbitals than one register
+    block of W = V C holds (several blocks and LDS chunks for |U|^2), for each
+    lane split of the basis (16, 32 and 64 lanes per basis row), against the
+    oracle's factorized restatement."""
+    basis = (_mixed_basis() * 3)[:nbasis]
+    nb = len(basis)
+    g1, g3, jas, Fs, Fo, V = _tc_case(tc, basis, (5, 7, tc.RADIAL_M4), (6, 7, tc.RADIAL_M4))
+    rng = np.random.default_rng(nbasis)
+    C = np.zeros((2, nb, max(nup, ndown)))
+    D = np.zeros((2, nb, nb))
+    for s, no in enumerate((nup, ndown)):
+        C[s, :, :no] = rng.normal(size=(nb, no)) * 0.3
+        D[s] = C[s, :, :no] @ C[s, :, :no].T
+    G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, V, jas, D, C)
+    A = g1.basisValues.cpu().numpy()
+    dA = g1.basisGradients.cpu().numpy()
+    Vf = tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1)
+    Vs, Vo = np.ascontiguousarray(jas.c_same * Vf), np.ascontiguousarray(jas.c_oppo * Vf)
+    G2o, G3o = O.tc_fock(nup, ndown, A, dA, g1.weights_host, tc.unpack(Fs.cpu().numpy(), nb),
+                         tc.unpack(Fo.cpu().numpy(), nb), Vs, Vo, D, C)
+    assert rel_err(G2, G2o) < 1e-12
+    assert rel_err(G3, G3o) < 1e-12
+
+
 def test_grid_eri_vs_oracle(tc, O, gpu):
     """integrateAllR1R2 (src/integration/integrateAll.hpp:221-317) on the
     device: against the oracle's contraction of the same F, the 8-fold
