@@ -295,45 +295,54 @@ struct Jastrow {
 enum class Precision : int { FP64 = TCSCF_FP64, FP32 = TCSCF_FP32, BF16X3 = TCSCF_BF16X3, BF16X5 = TCSCF_BF16X5 };
 
 namespace detail {
-template <class REAL, int N1, int N2>
-DeviceArray<REAL> contract(const integration::QMCGrid<REAL, N1>& r1, const integration::QMCGrid<REAL, N2>& r2,
-                           int kernel, const tcscf_jastrow* jp, int sameSpin, Precision prec) {
+/* tcscf_dtype of a device array's element type (FP64, or FP32 storage of the
+ * mixed-precision path). */
+template <class FT>
+constexpr int dtype() {
+  static_assert(std::is_same_v<FT, double> || std::is_same_v<FT, float>, "F / V elements are double or float");
+  return std::is_same_v<FT, float> ? TCSCF_F32 : TCSCF_F64;
+}
+
+template <class FT, class REAL, int N1, int N2>
+DeviceArray<FT> contract(const integration::QMCGrid<REAL, N1>& r1, const integration::QMCGrid<REAL, N2>& r2,
+                         int kernel, const tcscf_jastrow* jp, int sameSpin, Precision prec) {
   if (r1.nBasis() != r2.nBasis() && kernel != TCSCF_K_COULOMB) throw Error("computeF: different bases");
   const int nb = static_cast<int>(r2.nBasis());
   if (nb == 0) throw Error("computeF: setBasisFunctions was not called on the inner grid");
   const IndexType npair = static_cast<IndexType>(nb) * (nb + 1) / 2;
   const int ncomp = kernel == TCSCF_K_JASTROW_GRAD ? 3 : 1;
-  DeviceArray<REAL> F(static_cast<std::size_t>(r1.nGrid() * ncomp * npair));
+  DeviceArray<FT> F(static_cast<std::size_t>(r1.nGrid() * ncomp * npair));
   const double scale = N2 == 2 ? 2 * 3.14159265358979323846 : 1.0;
   check(tcscf_compute_F(r1.quadratureGrid.points.data(), r1.nGrid(), r2.quadratureGrid.points.data(),
                         r2.quadratureGrid.weights.data(), r2.radialOrder.data(), r2.nGrid(), r2.basisValues.data(), nb,
-                        kernel, jp, sameSpin, static_cast<int>(prec), scale, F.data(), TCSCF_F64, r1.stream));
+                        kernel, jp, sameSpin, static_cast<int>(prec), scale, F.data(), dtype<FT>(), r1.stream));
   hip(hipStreamSynchronize(r1.stream));
   return F;
 }
 }  // namespace detail
 
 /* precomputeIntegrand + computeF (src/HartreeFock.hpp:23-180) for f = 1/r12:
- * packed F[n1][p(j, i)] = scale sum_n2 f(r1, r2) w2 phi_j(r2) phi_i(r2). */
-template <class REAL, int N1, int N2>
-DeviceArray<REAL> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N2>& r2Grid,
-                           Precision precision = Precision::FP64) {
-  return detail::contract(r1Grid, r2Grid, TCSCF_K_COULOMB, nullptr, 1, precision);
+ * packed F[n1][p(j, i)] = scale sum_n2 f(r1, r2) w2 phi_j(r2) phi_i(r2).  FT:
+ * the stored element type (float: the FP32 storage of the mixed-precision path). */
+template <class FT = double, class REAL, int N1, int N2>
+DeviceArray<FT> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N2>& r2Grid,
+                         Precision precision = Precision::FP64) {
+  return detail::contract<FT>(r1Grid, r2Grid, TCSCF_K_COULOMB, nullptr, 1, precision);
 }
 /* The TC scalar kernel 2/r12 - |grad_1 u12|^2 - |grad_2 u12|^2 of one spin pair
  * (src/unitTests/HartreeFockTest.cpp:276-296): FjiSame / FjiOppo. */
-template <class REAL, int N1, int N2>
-DeviceArray<REAL> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N2>& r2Grid,
-                           const Jastrow& jastrow, bool sameSpin, Precision precision = Precision::FP64) {
+template <class FT = double, class REAL, int N1, int N2>
+DeviceArray<FT> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N2>& r2Grid,
+                         const Jastrow& jastrow, bool sameSpin, Precision precision = Precision::FP64) {
   const tcscf_jastrow j = jastrow.abi();
-  return detail::contract(r1Grid, r2Grid, TCSCF_K_TC_SCALAR, &j, sameSpin ? 1 : 0, precision);
+  return detail::contract<FT>(r1Grid, r2Grid, TCSCF_K_TC_SCALAR, &j, sameSpin ? 1 : 0, precision);
 }
 /* computeV (src/HartreeFock.hpp:185-229) for c = 1: packed V[n1][xyz][p(j, i)]. */
-template <class REAL, int N1, int N3>
-DeviceArray<REAL> computeV(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N3>& r3Grid,
-                           const Jastrow& jastrow, Precision precision = Precision::FP64) {
+template <class FT = double, class REAL, int N1, int N3>
+DeviceArray<FT> computeV(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N3>& r3Grid,
+                         const Jastrow& jastrow, Precision precision = Precision::FP64) {
   const tcscf_jastrow j = jastrow.abi();
-  return detail::contract(r1Grid, r3Grid, TCSCF_K_JASTROW_GRAD, &j, 1, precision);
+  return detail::contract<FT>(r1Grid, r3Grid, TCSCF_K_JASTROW_GRAD, &j, 1, precision);
 }
 
 /* integrateAllR1R2<false> (src/integration/integrateAll.hpp:221-317) from F:
@@ -391,9 +400,9 @@ class RCSHartreeFock {
   }
   static constexpr bool needsGradients() { return false; }
 
-  template <int N1>
+  template <int N1, class FT>
   T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
-            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<T>& Fji, bool respectOneElectronSymmetry,
+            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<FT>& Fji, bool respectOneElectronSymmetry,
             double tol = 1e-5, int maxIter = 50, int diis = 0) {
     detail::checkSquare(overlap, basisSize, "overlap");
     detail::checkSquare(oneElectronTerms, basisSize, "oneElectronTerms");
@@ -401,7 +410,7 @@ class RCSHartreeFock {
     tcscf_scf_result res{};
     energyTrace.assign(maxIter, 0);
     const int rc = tcscf_rcs_scf(nElectrons_, basisSize, overlap.data(), oneElectronTerms.data(), r1Grid.basisValues.data(),
-                                 r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), Fji.data(), TCSCF_F64, &o, &res,
+                                 r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), Fji.data(), detail::dtype<FT>(), &o, &res,
                                  energyTrace.data(), density.data(), eigenvalues.data(), r1Grid.stream);
     iter_ = res.iterations;
     energyTrace.resize(iter_ > 0 ? iter_ : 0);
@@ -435,9 +444,9 @@ class UOSHartreeFock {
   }
   static constexpr bool needsGradients() { return false; }
 
-  template <int N1>
+  template <int N1, class FT>
   T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
-            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<T>& Fji, bool respectOneElectronSymmetry,
+            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<FT>& Fji, bool respectOneElectronSymmetry,
             double tol = 1e-5, int maxIter = 100, int diis = 0) {
     detail::checkSquare(overlap, basisSize, "overlap");
     detail::checkSquare(oneElectronTerms, basisSize, "oneElectronTerms");
@@ -446,7 +455,7 @@ class UOSHartreeFock {
     energyTrace.assign(maxIter, 0);
     const int rc = tcscf_uos_scf(nElectrons[0], nElectrons[1], basisSize, overlap.data(), oneElectronTerms.data(),
                                  r1Grid.basisValues.data(), r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(),
-                                 Fji.data(), TCSCF_F64, &o, &res, energyTrace.data(), density.data(),
+                                 Fji.data(), detail::dtype<FT>(), &o, &res, energyTrace.data(), density.data(),
                                  eigenvalues.data(), r1Grid.stream);
     iter_ = res.iterations;
     energyTrace.resize(iter_ > 0 ? iter_ : 0);
@@ -485,10 +494,11 @@ class TCHartreeFock {
 
   /* mgsMode 0: the reference's Gram-Schmidt (Euclidean norm after the first
    * vector, src/orthogonalization.hpp:152); 1: the overlap metric throughout. */
-  template <int N1>
+  /* FT = float: the mixed-precision Fock build (per-point terms in FP32). */
+  template <int N1, class FT>
   T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
-            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<T>& FjiSame, const DeviceArray<T>& FjiOppo,
-            const DeviceArray<T>& Vji, const Jastrow& jastrow, bool respectOneElectronSymmetry, double tol = 1e-5,
+            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<FT>& FjiSame, const DeviceArray<FT>& FjiOppo,
+            const DeviceArray<FT>& Vji, const Jastrow& jastrow, bool respectOneElectronSymmetry, double tol = 1e-5,
             int maxIter = 200, int mgsMode = 0, int diis = 0) {
     detail::checkSquare(overlap, basisSize, "overlap");
     detail::checkSquare(oneElectronTerms, basisSize, "oneElectronTerms");
@@ -500,7 +510,7 @@ class TCHartreeFock {
     const int rc = tcscf_tc_scf(nElectrons[0], nElectrons[1], basisSize, overlap.data(), oneElectronTerms.data(),
                                 r1Grid.basisValues.data(), r1Grid.basisGradients.data(),
                                 r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), FjiSame.data(), FjiOppo.data(),
-                                Vji.data(), &j, &o, &res, energyTrace.data(), density.data(),
+                                Vji.data(), detail::dtype<FT>(), &j, &o, &res, energyTrace.data(), density.data(),
                                 occupiedOrbitalPseudoEnergy.data(), r1Grid.stream);
     iter_ = res.iterations;
     energyTrace.resize(iter_ > 0 ? iter_ : 0);

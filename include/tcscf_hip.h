@@ -239,24 +239,26 @@ int tcscf_uos_scf(int nup, int ndown, int nb, const double* S, const double* H, 
                   tcscf_scf_result* res, double* energy_trace, double* D_out, double* evals_out, void* stream);
 
 /* TCHartreeFock::compute (src/HartreeFock.cpp:1007-1362), real basis.  Fs_dev,
- * Fo_dev: packed n1 x npair F_same / F_oppo (TCSCF_K_TC_SCALAR, FP64);
- * V_dev: packed n1 x 3 x npair (TCSCF_K_JASTROW_GRAD, c = 1); A1_dev n1 x nb;
- * dA1_dev n1 x nb x 3; S, H host.  The Fock build runs on the device, the FP64
- * generalized non-Hermitian eigen-solve (role of LAPACK ggev, per symmetry
- * block), the Re(lambda) sort and the MGS of the occupied orbitals on the host.
- * Synchronous.  D_out 2 x nb x nb; pseudo_out nup + ndown occupied
- * pseudo-energies. */
+ * Fo_dev: packed n1 x npair F_same / F_oppo (TCSCF_K_TC_SCALAR); V_dev: packed
+ * n1 x 3 x npair (TCSCF_K_JASTROW_GRAD, c = 1); all three of f_dtype:
+ * TCSCF_F64 runs the Fock build in FP64, TCSCF_F32 (mixed precision) forms
+ * its per-point terms in FP32 and sums them over points in FP64.  A1_dev
+ * n1 x nb; dA1_dev n1 x nb x 3; S, H host.  The Fock build runs on the device,
+ * the FP64 generalized non-Hermitian eigen-solve (role of LAPACK ggev, per
+ * symmetry block), the Re(lambda) sort and the MGS of the occupied orbitals
+ * on the host.  Synchronous.  D_out 2 x nb x nb; pseudo_out nup + ndown
+ * occupied pseudo-energies. */
 int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1_dev,
-                 const double* dA1_dev, const double* w1_dev, int64_t n1, const double* Fs_dev, const double* Fo_dev,
-                 const double* V_dev, const tcscf_jastrow* jastrow, const tcscf_scf_opts* opts, tcscf_scf_result* res,
-                 double* energy_trace, double* D_out, double* pseudo_out, void* stream);
+                 const double* dA1_dev, const double* w1_dev, int64_t n1, const void* Fs_dev, const void* Fo_dev,
+                 const void* V_dev, int f_dtype, const tcscf_jastrow* jastrow, const tcscf_scf_opts* opts,
+                 tcscf_scf_result* res, double* energy_trace, double* D_out, double* pseudo_out, void* stream);
 
 /* One TC Fock build (the per-iteration sweeps and assembly of
  * src/HartreeFock.cpp:1118-1224) from D_host (2 x nb x nb) and the occupied
  * orbitals C_host (2 x nb x max(nup, ndown), column b of spin s used for
  * b < nocc_s): writes G2 and G3 (2 x nb x nb each, host) before the mask. */
 int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1_dev, const double* dA1_dev, const double* w1_dev,
-                  int64_t n1, const double* Fs_dev, const double* Fo_dev, const double* V_dev,
+                  int64_t n1, const void* Fs_dev, const void* Fo_dev, const void* V_dev, int f_dtype,
                   const tcscf_jastrow* jastrow, const double* D_host, const double* C_host, double* G2_host,
                   double* G3_host, void* stream);
 
@@ -265,7 +267,7 @@ enum tcscf_scf_kind { TCSCF_SCF_RCS = 0, TCSCF_SCF_UOS = 1, TCSCF_SCF_TC = 2 };
 
 /* One instance of a batched solve: the arguments of tcscf_rcs_scf /
  * tcscf_uos_scf (kind RCS / UOS: F_dev, f_dtype; RCS uses nup + ndown electrons)
- * or tcscf_tc_scf (kind TC: Fs_dev, Fo_dev, V_dev, dA1_dev, jastrow). */
+ * or tcscf_tc_scf (kind TC: Fs_dev, Fo_dev, V_dev, f_dtype, dA1_dev, jastrow). */
 typedef struct tcscf_scf_item {
   int kind;
   int nup, ndown, nb;
@@ -276,10 +278,10 @@ typedef struct tcscf_scf_item {
   const double* w1_dev;
   int64_t n1;
   const void* F_dev;
-  int f_dtype;
-  const double* Fs_dev;
-  const double* Fo_dev;
-  const double* V_dev;
+  int f_dtype; /* of F_dev, or of Fs_dev, Fo_dev and V_dev */
+  const void* Fs_dev;
+  const void* Fo_dev;
+  const void* V_dev;
   tcscf_jastrow jastrow;
   double* energy_trace; /* host, opts->max_iter entries, or NULL */
   double* D_out;        /* host, nspin x nb x nb (RCS 1, else 2), or NULL */

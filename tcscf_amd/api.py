@@ -322,6 +322,16 @@ def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64", ordere
     return _compute(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, True, precision, out_dtype, ordered)
 
 
+def _tc_dtype(Fs, Fo, V, n1):
+    """The f_dtype of a TC tensor triple: all float64 (FP64 Fock build) or all
+    float32 (mixed-precision Fock build), one row per r1 point."""
+    torch = _torch()
+    dts = {T.dtype for T in (Fs, Fo, V)}
+    if len(dts) != 1 or dts.pop() not in (torch.float64, torch.float32) or any(T.shape[0] != n1 for T in (Fs, Fo, V)):
+        raise ValueError("TC: F_same, F_oppo and V must all be float64 or all float32, one row per r1 point")
+    return _lib.F32 if Fs.dtype == torch.float32 else _lib.F64
+
+
 def computeF_batch(specs, precision="bf16x5"):
     """computeF / computeV for many instances in one call (BF16: one persistent
     launch over a work queue of tiles of every instance, results equal to the
@@ -449,6 +459,7 @@ def batch_scf(instances, tol=1e-5, max_iter=200, respectOneElectronSymmetry=True
             if not g.gradientsStored():
                 raise ValueError("batch_scf: TC instances need basis gradients on r1")
             it.dA1_dev = g.basisGradients.data_ptr()
+            it.f_dtype = _tc_dtype(x["Fs"], x["Fo"], x["V"], g.nGrid())
             it.Fs_dev, it.Fo_dev, it.V_dev = x["Fs"].data_ptr(), x["Fo"].data_ptr(), x["V"].data_ptr()
             it.jastrow = x["jastrow"]._struct()
         else:
@@ -609,9 +620,7 @@ class TCHartreeFock(_HFBase):
         if S.shape != (nb, nb) or H.shape != (nb, nb) or r1Grid.nBasis() != nb or not r1Grid.gradientsStored():
             raise ValueError("TC: shape mismatch or basis gradients not stored on the r1 grid")
         n1 = r1Grid.nGrid()
-        for T in (FjiSame, FjiOppo, Vji):
-            if T.dtype != torch.float64 or T.shape[0] != n1:
-                raise ValueError("TC: F/V must be float64 with one row per r1 point")
+        fdt = _tc_dtype(FjiSame, FjiOppo, Vji, n1)
         res = _lib.ScfResult()
         trace = np.zeros(max_iter)
         D = np.zeros((2, nb, nb))
@@ -619,7 +628,7 @@ class TCHartreeFock(_HFBase):
         jp = jastrow._struct()
         rc = lib().tcscf_tc_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                 _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(n1),
-                                _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), ctypes.byref(jp),
+                                _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), fdt, ctypes.byref(jp),
                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, mgs_mode, diis)),
                                 ctypes.byref(res), trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp),
                                 ps.ctypes.data_as(_dp), _stream())
@@ -641,12 +650,13 @@ class TCHartreeFock(_HFBase):
         C = np.ascontiguousarray(occupiedOrbitals, dtype=np.float64)
         if D.shape != (2, nb, nb) or C.shape != (2, nb, nmax) or r1Grid.nBasis() != nb:
             raise ValueError("fockBuild: density must be (2, nb, nb) and orbitals (2, nb, max nocc)")
+        fdt = _tc_dtype(FjiSame, FjiOppo, Vji, r1Grid.nGrid())
         G2 = np.zeros((2, nb, nb))
         G3 = np.zeros((2, nb, nb))
         jp = jastrow._struct()
         check(lib().tcscf_tc_fock(self.nElectrons[0], self.nElectrons[1], nb, _ptr(r1Grid.basisValues),
                                   _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights),
-                                  ctypes.c_int64(r1Grid.nGrid()), _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji),
+                                  ctypes.c_int64(r1Grid.nGrid()), _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), fdt,
                                   ctypes.byref(jp), D.ctypes.data_as(_dp), C.ctypes.data_as(_dp),
                                   G2.ctypes.data_as(_dp), G3.ctypes.data_as(_dp), _stream()))
         return G2, G3

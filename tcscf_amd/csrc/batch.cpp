@@ -50,6 +50,7 @@ struct Instance {
   std::vector<double> G2, G3;
   int it = 0;
   bool active = true;
+  bool restricted = false;  // TC: closed shell with identical spins, as captured in the graph
   int rc = 0;
 };
 
@@ -109,7 +110,7 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
         TC_CHECK(false, "tcscf_batch_scf: out of device memory");
       }
       if ((rc = tc_prepare(x.tcp, it.nup, it.ndown, nb, it.A1_dev, it.dA1_dev, it.w1_dev, it.n1, it.Fs_dev,
-                           it.Fo_dev, it.V_dev, &it.jastrow, x.ws))) {
+                           it.Fo_dev, it.V_dev, it.f_dtype, &it.jastrow, x.ws))) {
         cleanup();
         return rc;
       }
@@ -185,7 +186,18 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
     for (int k = 0; k < count; ++k)
       if (inst[k].active) active.push_back(k);
     if (active.empty()) break;
-    if (active != built) {  // capture the Fock builds of the active set
+    // TC instances whose spins became (or stopped being) identical change kernels
+    bool recapture = active != built;
+    for (int k : active) {
+      Instance& x = inst[k];
+      if (!x.tci) continue;
+      const bool r = tc_restricted(x.tcp, x.tci->D.data(), x.tci->C.data());
+      if (r != x.restricted) {
+        x.restricted = r;
+        recapture = true;
+      }
+    }
+    if (recapture) {  // capture the Fock builds of the active set
       if (exec) TC_BHIP(hipGraphExecDestroy(exec));
       if (graph) TC_BHIP(hipGraphDestroy(graph));
       exec = nullptr;
@@ -197,7 +209,8 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
       for (size_t q = 0; q < active.size() && !erc; ++q) {
         Instance& x = inst[active[q]];
         hipStream_t s = br[q % kBranches];
-        if (x.tci) erc = tc_enqueue(x.tcp, x.up->p, x.up->p + 2 * (int64_t)x.item->nb * x.item->nb, x.down->p, s);
+        if (x.tci)
+          erc = tc_enqueue(x.tcp, x.up->p, x.up->p + 2 * (int64_t)x.item->nb * x.item->nb, x.down->p, x.restricted, s);
         else erc = hf_enqueue(x.hf, x.up->p, x.down->p, s);
       }
       for (int b = 0; b < kBranches; ++b) {
@@ -234,7 +247,7 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
       tcscf_scf_result& r = results[active[q]];
       bool done = false;
       if (x.tci) {
-        tc_assemble(x.tcp, x.down->p, x.G2.data(), x.G3.data());
+        tc_assemble(x.tcp, x.down->p, x.restricted, x.G2.data(), x.G3.data());
         x.rc = x.tci->step(x.G2.data(), x.G3.data(), x.it, &r, x.item->energy_trace, &done);
       } else {
         x.rc = x.hfi->step(x.down->p, x.it, &r, x.item->energy_trace, &done);

@@ -60,22 +60,31 @@ struct HfIter {
 struct TcPlan {
   int nup, ndown, nb, nmax, nvec;
   int64_t n1, npair;
-  const double *A1, *dA1, *w1, *Fs, *Fo, *V;
+  const double *A1, *dA1, *w1;
+  const void *Fs, *Fo, *V;  // element type: FP32 when f32, else FP64
+  int f32;                  // FP32 Fock build (per-point terms in FP32, sums over points in FP64)
   double c_same, c_oppo;
   double *dD, *dC, *dVec, *dAl, *dH, *dPG, *dPV, *dOut;
   int wpw, stage;
+  int ustart[7];  // rank workgroups per target (prefix offsets)
   size_t sh1;
   int64_t pblocks;
 };
 size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1);
 size_t tc_out_doubles(int nb);  // the per-build result block copied back to the host
 int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1,
-               int64_t n1, const double* Fs, const double* Fo, const double* V, const tcscf_jastrow* jp, double* ws);
-// D (2 x nb x nb) and C (2 x nb x nmax) host -> out (host, tc_out_doubles), async.
-int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, hipStream_t st);
+               int64_t n1, const void* Fs, const void* Fo, const void* V, int f_dtype, const tcscf_jastrow* jp,
+               double* ws);
+// Closed shell whose two spins hold bitwise identical densities and orbitals:
+// the spin-1 Fock terms are then the spin-0 numbers, formed once.
+bool tc_restricted(const TcPlan& p, const double* D, const double* C);
+// D (2 x nb x nb) and C (2 x nb x nmax) host -> out (host, tc_out_doubles), async;
+// restricted (tc_restricted of D and C) forms the spin-0 blocks only.
+int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, bool restricted,
+               hipStream_t st);
 // out -> G2, G3 (2 x nb x nb each, before the mask): G2 = scalar/2 + (vec/2 - vec^T/2),
-// G3 = scalar3/2 (src/HartreeFock.cpp:1209-1222).
-void tc_assemble(const TcPlan& p, const double* out, double* G2, double* G3);
+// G3 = scalar3/2 (src/HartreeFock.cpp:1209-1222); restricted copies spin 0 to spin 1.
+void tc_assemble(const TcPlan& p, const double* out, bool restricted, double* G2, double* G3);
 
 // Host half of one TC iteration (src/HartreeFock.cpp:1007-1362).
 struct TcIter {

@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <limits>
 #include <numeric>
 #include <vector>
@@ -33,6 +34,7 @@ namespace tcs {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlocks = 512;
+constexpr int kRankChunk = 512;  // (point, entry) terms per rank workgroup
 constexpr int kWMain = 5;   // occupied orbitals of W = V C per pass over j (the first in the main pass)
 constexpr int kWChunk = 2;  // orbitals of W per LDS round of |U|^2
 
@@ -49,10 +51,12 @@ constexpr int kWChunk = 2;  // orbitals of W per LDS round of |U|^2
 // x (x) y the rank-one matrix x_j y_i.  (Vja == VTSI for symmetric V.)
 constexpr int kVecFixed = 7;
 
+// Fs, Fo and V hold the kernel's storage type ST (double, or float for the
+// mixed-precision build); everything else, and all arithmetic, is FP64.
 struct PointArgs {
-  const double* Fs;   // [n1][npair]
-  const double* Fo;   // [n1][npair]
-  const double* V;    // [n1][3][npair]
+  const void* Fs;     // [n1][npair]
+  const void* Fo;     // [n1][npair]
+  const void* V;      // [n1][3][npair]
   const double* A;    // [n1][nb]
   const double* dA;   // [n1][nb][3]
   const double* w;    // [n1]
@@ -119,7 +123,7 @@ __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-__host__ __device__ inline int64_t point_wave_doubles(int nb) { return 24 * (int64_t)nb; }
+__host__ __device__ inline int64_t point_wave_elems(int nb) { return 24 * (int64_t)nb; }
 
 // Sum of v over the lanes that share a basis index i (lane = jp * nbp + i).
 __device__ inline double part_sum(double v, int nbp) {
@@ -128,8 +132,12 @@ __device__ inline double part_sum(double v, int nbp) {
   return v;
 }
 
-// Doubles of staged V rows per wave ([3][npair] when staged).
-__host__ __device__ inline int64_t staged_rows(bool stage, int nb) { return stage ? 3 * ((int64_t)nb * (nb + 1) / 2) : 0; }
+// Doubles of LDS that the staged V rows ([3][npair] of element type ST) of a
+// wave take, 0 when not staged.
+template <class ST>
+__host__ __device__ inline int64_t staged_doubles(bool stage, int nb) {
+  return stage ? (3 * ((int64_t)nb * (nb + 1) / 2) * (int64_t)sizeof(ST) + 7) / 8 : 0;
+}
 
 // Pass 1: one wave per r1 point.  Lane = jp * nbp + i: i is the basis index
 // (nbp = 16, 32 or 64 >= nb) and the 64 / nbp lane groups jp split every loop
@@ -142,17 +150,19 @@ __host__ __device__ inline int64_t staged_rows(bool stage, int nb) { return stag
 // into the wave's LDS area by direct-to-LDS loads first, so those gathers hit
 // LDS; it pays when 8 waves of a workgroup fit in LDS and hide the latency of
 // the dependent passes.  D and C are shared in LDS, the per-point vectors
-// live in a small per-wave LDS area.
-template <bool STAGE>
+// live in a small per-wave LDS area.  NS = 1 (closed shell with identical
+// spin densities and orbitals) forms spin 0 only, the spin-1 terms being the
+// same numbers.
+template <class ST, bool STAGE, int NS>
 __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
   extern __shared__ double smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int nb = a.nb, nmax = a.nmax;
   const int nbp = nb <= 16 ? 16 : (nb <= 32 ? 32 : 64), parts = 64 / nbp;
   const int64_t np = a.npair, nb2 = (int64_t)nb * nb;
-  double* Ds = smem;                       // [2][nb][nb]
+  double* Ds = smem;                            // [2][nb][nb]
   double* Cs = Ds + 2 * nb2;               // [2][nb][nmax]
-  double* base = Cs + 2 * (int64_t)nb * nmax + wave * (point_wave_doubles(nb) + staged_rows(STAGE, nb));
+  double* base = Cs + 2 * (int64_t)nb * nmax + wave * (point_wave_elems(nb) + staged_doubles<ST>(STAGE, nb));
   double* As = base;                       // [nb]
   double* dAs = As + nb;                   // [nb][3]
   double* Xs = dAs + 3 * nb;               // [2][nb]
@@ -160,7 +170,7 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
   double* kS = VT + 6 * nb;                // [2][nb][3]
   double* Es = kS + 6 * nb;                // [2][nb][3]
   double* Wc = kS;                         // [2][nb][kWChunk][3], once kS and Es are consumed
-  double* Sv = Es + 6 * nb;                // STAGE: the point's V rows [3][npair]
+  ST* Sv = reinterpret_cast<ST*>(Es + 6 * nb);  // STAGE: the point's V rows [3][npair]
   for (int64_t e = tid; e < 2 * nb2; e += blockDim.x) Ds[e] = a.D[e];
   for (int64_t e = tid; e < 2 * (int64_t)nb * nmax; e += blockDim.x) Cs[e] = a.C[e];
   __syncthreads();
@@ -173,18 +183,18 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
   const int nocc[2] = {a.nocc0, a.nocc1};
   const int64_t nvw = (int64_t)gridDim.x * nwaves;
   // body(j, f_s, f_o, v0, v1, v2) for every j of the lane
-  auto for_j = [&](const double* fs, const double* fo, const double* vv, auto&& body) {
+  auto for_j = [&](const ST* fs, const ST* fo, const ST* vv, auto&& body) {
     if (act)
       for (int j = jp; j < nb; j += parts) {
         const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
-        body(j, fs ? fs[pj] : 0.0, fo ? fo[pj] : 0.0, vv[pj], vv[np + pj], vv[2 * np + pj]);
+        body(j, fs ? double(fs[pj]) : 0.0, fo ? double(fo[pj]) : 0.0, double(vv[pj]), double(vv[np + pj]), double(vv[2 * np + pj]));
       }
   };
 
   // wreg[s][b][c] += C_s[j][b0 + b] V_c(j, i) for the occupied orbitals b0 + b
   auto w_accumulate = [&](double (&wreg)[2][kWMain][3], int b0, int j, double v0, double v1, double v2) {
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const double* crow = Cs + ((int64_t)s * nb + j) * nmax + b0;
 #pragma unroll
       for (int b = 0; b < kWMain; ++b)
@@ -198,16 +208,17 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
   };
 
   for (int64_t n = (int64_t)blockIdx.x * nwaves + wave; n < a.n1; n += nvw) {
-    const double* __restrict__ fs = a.Fs + n * np;
-    const double* __restrict__ fo = a.Fo + n * np;
-    const double* __restrict__ vv = a.V + n * 3 * np;
+    const ST* __restrict__ fs = (const ST*)a.Fs + n * np;
+    const ST* __restrict__ fo = (const ST*)a.Fo + n * np;
+    const ST* __restrict__ vv = (const ST*)a.V + n * 3 * np;
     if (STAGE) {
       // the V rows copied by direct-to-LDS loads (lane l of a load writes
       // dword k + l of the run), all loads in flight together, one wait
       const unsigned* g = reinterpret_cast<const unsigned*>(vv);
       unsigned* l = reinterpret_cast<unsigned*>(Sv);
-      for (int64_t k = 0; k < 6 * np; k += 64)
-        if (k + lane < 6 * np)
+      const int64_t ndw = 3 * np * (int64_t)(sizeof(ST) / 4);
+      for (int64_t k = 0; k < ndw; k += 64)
+        if (k + lane < ndw)
           __builtin_amdgcn_global_load_lds(g + k + lane, (__attribute__((address_space(3))) void*)(l + k), 4, 0, 0);
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       vv = Sv;
@@ -218,7 +229,7 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
     }
     wave_sync_lds();
     // X_s[i] = sum_k D_s[k][i] A[k];  kS_s[i] = sum_k D_s[k][i] dA[k]
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NS; ++s) {
       double x = 0, k0 = 0, k1 = 0, k2 = 0;
       if (act)
         for (int k = jp; k < nb; k += parts) {
@@ -245,7 +256,7 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
     if (act)
       for (int c = 0; c < 3; ++c) dAi[c] = dAs[i * 3 + c];
     double occ[2], sg[2][3];
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const double x = own ? Xs[s * nb + i] : 0.0;
       occ[s] = uniform(wave_sum(x * Ai));  // calculateOccupiedElectronDensity
       for (int c = 0; c < 3; ++c) sg[s][c] = uniform(wave_sum(x * dAi[c]));  // calculateScaledGradients
@@ -256,14 +267,14 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
     double q[2][3] = {{0, 0, 0}, {0, 0, 0}};
     double wreg[2][kWMain][3];
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int b = 0; b < kWMain; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
     for_j(fs, fo, vv, [&](int j, double f_s, double f_o, double v0, double v1, double v2) {
       w_accumulate(wreg, 0, j, v0, v1, v2);
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const double dsj = Ds[s * nb2 + j * nb + i], dbj = Ds[(1 - s) * nb2 + j * nb + i];
+      for (int s = 0; s < NS; ++s) {
+        const double dsj = Ds[s * nb2 + j * nb + i], dbj = Ds[(NS == 1 ? s : 1 - s) * nb2 + j * nb + i];
         faa_p[s] += f_s * dsj + f_o * dbj;
         const double xj = Xs[s * nb + j];
         fai[s] += xj * f_s;
@@ -287,7 +298,7 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
     for (int b0 = 0; b0 < nmax; b0 += kWMain) {
       if (b0 > 0) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+        for (int s = 0; s < NS; ++s)
 #pragma unroll
           for (int b = 0; b < kWMain; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
         for_j(nullptr, nullptr, vv, [&](int j, double, double, double v0, double v1, double v2) {
@@ -298,7 +309,7 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
       for (int c0 = 0; c0 < kWMain; c0 += kWChunk) {
         if (b0 + c0 >= nmax) break;
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+        for (int s = 0; s < NS; ++s)
 #pragma unroll
           for (int b = c0; b < c0 + kWChunk && b < kWMain; ++b)
 #pragma unroll
@@ -312,7 +323,7 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
         wave_sync_lds();
         const int lim = std::min(nmax - b0 - c0, kWMain - c0), nbc = lim < kWChunk ? lim : kWChunk;
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < NS; ++s) {
           const int nu = nocc[s] * nbc * 3;  // (a, b in the chunk, c)
           for (int t = lane; t < nu; t += 64) {
             const int aa = t / (nbc * 3), rem = t - aa * nbc * 3, bb = rem / 3, c = rem - bb * 3;
@@ -327,21 +338,21 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
     }
     double faa[2], VTv[2][3];
     double vdsum[2][3];
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NS; ++s) {
       faa[s] = uniform(wave_sum(faa_p[s]));  // FaaSamePlusOppo
       for (int c = 0; c < 3; ++c) vdsum[s][c] = uniform(wave_sum(vd[s][c]));
       fai[s] = part_sum(fai[s], nbp);
       vai1[s] = part_sum(vai1[s], nbp);
       for (int c = 0; c < 3; ++c) q[s][c] = part_sum(q[s][c], nbp);
     }
-    for (int s = 0; s < 2; ++s)
-      for (int c = 0; c < 3; ++c) VTv[s][c] = cs * vdsum[s][c] + co * vdsum[1 - s][c];  // VaaSamePlusOppo == VTilde
+    for (int s = 0; s < NS; ++s)
+      for (int c = 0; c < 3; ++c) VTv[s][c] = cs * vdsum[s][c] + co * vdsum[NS == 1 ? s : 1 - s][c];  // VaaSamePlusOppo == VTilde
     if (own)
-      for (int s = 0; s < 2; ++s)
+      for (int s = 0; s < NS; ++s)
         for (int c = 0; c < 3; ++c) VT[(s * nb + i) * 3 + c] = cs * q[s][c];  // VTildeSubI (== Vja)
     wave_sync_lds();
     // E_s[k] = sum_l D_s[l][k] VTSI_s[l]
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NS; ++s) {
       double e0 = 0, e1 = 0, e2 = 0;
       if (act)
         for (int l = jp; l < nb; l += parts) {
@@ -365,7 +376,7 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
     double sv1[2] = {0, 0};
     for_j(nullptr, nullptr, vv, [&](int k, double, double, double v0, double v1, double v2) {
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
+      for (int s = 0; s < NS; ++s) {
         const double* e = Es + (s * nb + k) * 3;
         sv1[s] += v0 * e[0] + v1 * e[1] + v2 * e[2];
       }
@@ -373,14 +384,14 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
     sv1[0] = part_sum(sv1[0], nbp);
     sv1[1] = part_sum(sv1[1], nbp);
     double scvt[2][3], nrmU[2];
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const double x = own ? Xs[s * nb + i] : 0.0;
       for (int c = 0; c < 3; ++c) scvt[s][c] = uniform(wave_sum(own ? x * VT[(s * nb + i) * 3 + c] : 0.0));  // SumOverb_conjVTildeb_occb
       nrmU[s] = uniform(wave_sum(nacc[s]));
     }
     const double wn = a.w[n];
-    for (int s = 0; s < 2; ++s) {
-      const int sb = 1 - s;
+    for (int s = 0; s < NS; ++s) {
+      const int sb = NS == 1 ? s : 1 - s;
       double* vs = vout + (int64_t)s * a.nvec * nb;
       const double* vts = act ? VT + (s * nb + i) * 3 : nullptr;
       const double vtsi0 = act ? vts[0] : 0, vtsi1 = act ? vts[1] : 0, vtsi2 = act ? vts[2] : 0;
@@ -419,8 +430,15 @@ struct RankArgs {
   const double* w;      // [n1]
   int64_t n1;
   int nb, nvec, nocc0, nocc1;
-  double* part;         // [gridDim.x][6][nb][nb]
+  int ustart[7];        // workgroups of target 3 s + t: [ustart[st], ustart[st + 1])
+  double* part;         // [gridDim.x][nb][nb]
 };
+
+// (point, entry) terms per point of target st = 3 s + t: 1, 4, 5 + 3 nocc_s.
+__host__ __device__ inline int rank_entries(int st, int nocc0, int nocc1) {
+  const int s = st / 3, tg = st - 3 * s;
+  return tg == 0 ? 1 : (tg == 1 ? 4 : 5 + 3 * (s ? nocc1 : nocc0));
+}
 
 // Pass 2: G_{s,t}[j][i] = sum over (point, entry) of L[j] R[i] (the rank-one
 // terms listed above) as an FP64 MFMA product, blockIdx.y = 3 s + t.  K runs
@@ -433,24 +451,32 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
   __shared__ double red[NT * 16][NT * 16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
-  const int st = blockIdx.y, s = st / 3, tg = st - 3 * s;
+  const int u = blockIdx.x;
+  int st = 0;
+  while (st < 5 && u >= a.ustart[st + 1]) ++st;
+  const int s = st / 3, tg = st - 3 * s;
   const int nb = a.nb;
-  const int nocc_s = s ? a.nocc1 : a.nocc0;
-  const int E = tg == 0 ? 1 : (tg == 1 ? 4 : 5 + 3 * nocc_s);
-  const int64_t per = (a.n1 + gridDim.x - 1) / gridDim.x;
-  const int64_t p0 = (int64_t)blockIdx.x * per;
-  const int64_t p1 = p0 + per < a.n1 ? p0 + per : a.n1;
-  const int64_t K = p1 > p0 ? (p1 - p0) * E : 0;
-  d4 acc[NT][NT];
-#pragma unroll
-  for (int m = 0; m < NT; ++m)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[m][t] = d4{0, 0, 0, 0};
-  for (int64_t k0 = 4 * wave; k0 < K; k0 += 16) {
-    const int64_t k = k0 + g;
+  const int E = rank_entries(st, a.nocc0, a.nocc1);
+  // this workgroup's terms: k = kb + [0, K) of the target's n1 E (point, entry) terms
+  const int64_t kb = (int64_t)(u - a.ustart[st]) * kRankChunk;
+  const int K = (int)(a.n1 * E - kb < kRankChunk ? a.n1 * E - kb : kRankChunk);
+  const int64_t p0 = kb / E;
+  const int e0 = (int)(kb - p0 * E);
+  const float invE = 1.0f / (float)E;
+  // operands of the k-step at k0: lane (g, r) holds L_{k0+g}[16 m + r] and R_{k0+g}[16 m + r]
+  auto fetch = [&](int k0, double (&av)[NT], double (&bv)[NT]) {
+    const int k = k0 + g, kk = e0 + k;
     const bool kin = k < K;
-    const int64_t n = kin ? p0 + k / E : p0;
-    const int e = kin ? (int)(k % E) : 0;
+    int q = (int)(((float)kk + 0.5f) * invE), e = kk - q * E;  // kk = q E + e
+    if (e < 0) {
+      --q;
+      e += E;
+    } else if (e >= E) {
+      ++q;
+      e -= E;
+    }
+    const int64_t n = kin ? p0 + q : p0;
+    if (!kin) e = 0;
     const double* vn = a.vecs + (n * 2 + s) * a.nvec * nb;
     const double* An = a.A + n * nb;
     const double* L = An;
@@ -481,17 +507,32 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
         fl *= a.alpha[n * 2 + s];
       }
     }
-    double av[NT], bv[NT];
 #pragma unroll
     for (int m = 0; m < NT; ++m) {
       const int row = 16 * m + r;
       av[m] = row < nb ? fl * L[row] : 0.0;
       bv[m] = (kin && row < nb) ? R[row * sR] : 0.0;
     }
+  };
+  d4 acc[NT][NT];
+#pragma unroll
+  for (int m = 0; m < NT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = d4{0, 0, 0, 0};
+  // the loads of the next k-step are issued before the MFMAs of this one
+  double av[NT] = {}, bv[NT] = {}, an[NT] = {}, bn[NT] = {};
+  if (4 * wave < K) fetch(4 * wave, av, bv);
+  for (int k0 = 4 * wave; k0 < K; k0 += 16) {
+    if (k0 + 16 < K) fetch(k0 + 16, an, bn);
 #pragma unroll
     for (int m = 0; m < NT; ++m)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[t], acc[m][t], 0, 0, 0);
+#pragma unroll
+    for (int m = 0; m < NT; ++m) {
+      av[m] = an[m];
+      bv[m] = bn[m];
+    }
   }
   // fixed-order sum of the 4 waves' tiles
   for (int w = 0; w < 4; ++w) {
@@ -507,13 +548,15 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
           }
     __syncthreads();
   }
-  double* out = a.part + ((int64_t)blockIdx.x * 6 + st) * nb * nb;
+  double* out = a.part + (int64_t)blockIdx.x * nb * nb;
   for (int e2 = threadIdx.x; e2 < nb * nb; e2 += 256) out[e2] = red[e2 / nb][e2 % nb];
 }
 
-// Pass 3: Rv[s][target][p] = sum_n sum_c h[n][s][target][c] V[n][c][p].
-__global__ __launch_bounds__(256) void tc_vlin_kernel(const double* __restrict__ V, const double* __restrict__ hvec,
-                                                      int64_t n1, int64_t npair, double* __restrict__ part) {
+// Pass 3: Rv[s][target][p] = sum_n sum_c h[n][s][target][c] V[n][c][p], for the
+// nk = 2 x spins (s, target) pairs.
+template <class ST>
+__global__ __launch_bounds__(256) void tc_vlin_kernel(const ST* __restrict__ V, const double* __restrict__ hvec,
+                                                      int64_t n1, int64_t npair, int nk, double* __restrict__ part) {
   const int64_t per = (n1 + gridDim.x - 1) / gridDim.x;
   const int64_t p0 = (int64_t)blockIdx.x * per;
   const int64_t p1 = p0 + per < n1 ? p0 + per : n1;
@@ -522,10 +565,30 @@ __global__ __launch_bounds__(256) void tc_vlin_kernel(const double* __restrict__
     for (int64_t n = p0; n < p1; ++n) {
       const double v0 = V[(n * 3) * npair + p], v1 = V[(n * 3 + 1) * npair + p], v2 = V[(n * 3 + 2) * npair + p];
       const double* h = hvec + n * 12;
-      for (int k = 0; k < 4; ++k) acc[k] += v0 * h[3 * k] + v1 * h[3 * k + 1] + v2 * h[3 * k + 2];
+      for (int k = 0; k < nk; ++k) acc[k] += v0 * h[3 * k] + v1 * h[3 * k + 1] + v2 * h[3 * k + 2];
     }
-    for (int k = 0; k < 4; ++k) part[((int64_t)blockIdx.x * 4 + k) * npair + p] = acc[k];
+    for (int k = 0; k < nk; ++k) part[((int64_t)blockIdx.x * nk + k) * npair + p] = acc[k];
   }
+}
+
+// out[st][e] = sum over the rank workgroups u of target st of part[u][e], e < len.
+struct RankRanges {
+  int ustart[7];
+};
+__global__ __launch_bounds__(256) void tc_reduce_ranges(const double* __restrict__ part, RankRanges r, int64_t len,
+                                                        double* __restrict__ out) {
+  __shared__ double red[256];
+  const int st = (int)(blockIdx.x / len);
+  const int64_t e = blockIdx.x - (int64_t)st * len;
+  double s = 0;
+  for (int u = r.ustart[st] + threadIdx.x; u < r.ustart[st + 1]; u += 256) s += part[(int64_t)u * len + e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 __global__ __launch_bounds__(256) void tc_reduce(const double* __restrict__ part, int nparts, int64_t len,
@@ -549,6 +612,16 @@ double now_ms() {
 
 inline int64_t pk(int j, int i) { return j <= i ? (int64_t)i * (i + 1) / 2 + j : (int64_t)j * (j + 1) / 2 + i; }
 
+// The point-kernel instance for a storage type, staging choice and spin count.
+template <int NS>
+inline const void* point_kernel_ns(int f32, int stage) {
+  if (f32) return stage ? (const void*)tc_point_kernel<float, true, NS> : (const void*)tc_point_kernel<float, false, NS>;
+  return stage ? (const void*)tc_point_kernel<double, true, NS> : (const void*)tc_point_kernel<double, false, NS>;
+}
+inline const void* point_kernel(int f32, int stage, bool restricted) {
+  return restricted ? point_kernel_ns<1>(f32, stage) : point_kernel_ns<2>(f32, stage);
+}
+
 }  // namespace tcs
 }  // namespace tc
 
@@ -557,16 +630,29 @@ using namespace tcs;
 
 size_t tc_out_doubles(int nb) { return 6 * (size_t)nb * nb + 4 * (size_t)nb * (nb + 1) / 2; }
 
+// Rank workgroups per target st = 3 s + t (kRankChunk terms each), as prefix
+// offsets ustart[0..6].
+static void rank_units(int nup, int ndown, int64_t n1, int* ustart) {
+  ustart[0] = 0;
+  for (int st = 0; st < 6; ++st)
+    ustart[st + 1] = ustart[st] + (int)((n1 * rank_entries(st, nup, ndown) + kRankChunk - 1) / kRankChunk);
+}
+
 size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1) {
   const int nmax = std::max(1, std::max(nup, ndown));
   const size_t nb2 = (size_t)nb * nb, npair = (size_t)nb * (nb + 1) / 2, nvec = kVecFixed + 3 * nmax;
+  int us[7];
+  rank_units(nup, ndown, n1, us);
   return 2 * nb2 + 2 * (size_t)nb * nmax + (size_t)n1 * 2 * nvec * nb + (size_t)n1 * 2 + (size_t)n1 * 12 +
-         (size_t)kBlocks * 6 * nb2 + (size_t)kBlocks * 4 * npair + tc_out_doubles(nb);
+         (size_t)us[6] * nb2 + (size_t)kBlocks * 4 * npair + tc_out_doubles(nb);
 }
 
 int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1,
-               int64_t n1, const double* Fs, const double* Fo, const double* V, const tcscf_jastrow* jp, double* ws) {
+               int64_t n1, const void* Fs, const void* Fo, const void* V, int f_dtype, const tcscf_jastrow* jp,
+               double* ws) {
   TC_CHECK(A1 && dA1 && w1 && Fs && Fo && V && jp && ws, "tcscf_tc_fock: null");
+  TC_CHECK(f_dtype == TCSCF_F64 || f_dtype == TCSCF_F32, "tcscf_tc_fock: f_dtype must be TCSCF_F64 or TCSCF_F32");
+  p.f32 = f_dtype == TCSCF_F32;
   TC_CHECK(nb > 0 && nb <= 64, "tcscf_tc_fock: 1 <= nb <= 64");
   TC_CHECK(nup >= 0 && ndown >= 0 && nup <= nb && ndown <= nb, "tcscf_tc_fock: bad electron counts");
   p.nup = nup;
@@ -591,66 +677,84 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   p.dAl = p.dVec + n1 * 2 * p.nvec * nb;
   p.dH = p.dAl + n1 * 2;
   p.dPG = p.dH + n1 * 12;
-  p.dPV = p.dPG + (int64_t)kBlocks * 6 * nb2;
+  rank_units(nup, ndown, n1, p.ustart);
+  p.dPV = p.dPG + (int64_t)p.ustart[6] * nb2;
   p.dOut = p.dPV + (int64_t)kBlocks * 4 * p.npair;
   // waves per workgroup: as many (up to 4) as the LDS holds next to D and C
   const size_t shared = sizeof(double) * (2 * nb2 + 2 * (int64_t)nb * p.nmax);
   // the V rows staged in LDS when 8 waves of a workgroup hold them (the gathers
   // then come from LDS and 8 waves hide their latency), else gathers from
   // memory with up to 4 waves per workgroup
-  const size_t staged_wave = sizeof(double) * (point_wave_doubles(nb) + staged_rows(1, nb));
+  const size_t staged_wave =
+      sizeof(double) * (point_wave_elems(nb) + (p.f32 ? staged_doubles<float>(true, nb) : staged_doubles<double>(true, nb)));
   p.stage = shared + 8 * staged_wave <= 160 * 1024 ? 1 : 0;
-  const size_t per_wave = p.stage ? staged_wave : sizeof(double) * point_wave_doubles(nb);
+  const size_t per_wave = p.stage ? staged_wave : sizeof(double) * point_wave_elems(nb);
   p.wpw = p.stage ? 8 : 4;
   while (p.wpw > 1 && shared + p.wpw * per_wave > 160 * 1024) p.wpw /= 2;
   p.sh1 = shared + p.wpw * per_wave;
   TC_CHECK(p.sh1 <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
   if (p.sh1 > 64 * 1024)
-    TC_HIP(hipFuncSetAttribute(p.stage ? (const void*)tc_point_kernel<true> : (const void*)tc_point_kernel<false>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.sh1));
+    for (bool r : {false, true})
+      TC_HIP(hipFuncSetAttribute(point_kernel(p.f32, p.stage, r), hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.sh1));
   p.pblocks = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + p.wpw - 1) / p.wpw));
   return 0;
 }
 
-int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, hipStream_t st) {
+bool tc_restricted(const TcPlan& p, const double* D, const double* C) {
+  const size_t nb2 = (size_t)p.nb * p.nb, nc = (size_t)p.nb * p.nmax;
+  return p.nup == p.ndown && std::memcmp(D, D + nb2, nb2 * sizeof(double)) == 0 &&
+         std::memcmp(C, C + nc, nc * sizeof(double)) == 0;
+}
+
+int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, bool restricted,
+               hipStream_t st) {
   const int64_t nb2 = (int64_t)p.nb * p.nb;
+  const int nst = restricted ? 3 : 6;  // rank targets (spin, term) formed
   TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * 2 * nb2, hipMemcpyHostToDevice, st));
   TC_HIP(hipMemcpyAsync(p.dC, C_host, sizeof(double) * 2 * p.nb * p.nmax, hipMemcpyHostToDevice, st));
   PointArgs a{p.Fs, p.Fo, p.V, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb, p.nmax,
               p.nup, p.ndown, p.nvec, p.dVec, p.dAl, p.dH};
-  if (p.stage)
-    hipLaunchKernelGGL(tc_point_kernel<true>, dim3((unsigned)p.pblocks), dim3(64 * p.wpw), p.sh1, st, a);
-  else
-    hipLaunchKernelGGL(tc_point_kernel<false>, dim3((unsigned)p.pblocks), dim3(64 * p.wpw), p.sh1, st, a);
-  TC_HIP(hipGetLastError());
-  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.dPG};
+  void* pargs[] = {&a};
+  TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage, restricted), dim3((unsigned)p.pblocks), dim3(64 * p.wpw), pargs,
+                         p.sh1, st));
+  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, {}, p.dPG};
+  RankRanges rr{};
+  for (int q = 0; q < 7; ++q) ra.ustart[q] = rr.ustart[q] = p.ustart[q];
+  const dim3 rgrid((unsigned)p.ustart[nst]);
   if (p.nb <= 32)
-    hipLaunchKernelGGL(tc_rank_kernel<2>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
+    hipLaunchKernelGGL(tc_rank_kernel<2>, rgrid, dim3(256), 0, st, ra);
   else if (p.nb <= 48)
-    hipLaunchKernelGGL(tc_rank_kernel<3>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
+    hipLaunchKernelGGL(tc_rank_kernel<3>, rgrid, dim3(256), 0, st, ra);
   else
-    hipLaunchKernelGGL(tc_rank_kernel<4>, dim3(kBlocks, 6), dim3(256), 0, st, ra);
+    hipLaunchKernelGGL(tc_rank_kernel<4>, rgrid, dim3(256), 0, st, ra);
   TC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tc_vlin_kernel, dim3(kBlocks), dim3(256), 0, st, p.V, p.dH, p.n1, p.npair, p.dPV);
+  const int nk = restricted ? 2 : 4;  // (spin, target) pairs of the V-linear sums
+  if (p.f32)
+    hipLaunchKernelGGL(tc_vlin_kernel<float>, dim3(kBlocks), dim3(256), 0, st, (const float*)p.V, p.dH, p.n1,
+                       p.npair, nk, p.dPV);
+  else
+    hipLaunchKernelGGL(tc_vlin_kernel<double>, dim3(kBlocks), dim3(256), 0, st, (const double*)p.V, p.dH, p.n1,
+                       p.npair, nk, p.dPV);
   TC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(6 * nb2)), dim3(256), 0, st, p.dPG, kBlocks, 6 * nb2, p.dOut);
+  hipLaunchKernelGGL(tc_reduce_ranges, dim3((unsigned)(nst * nb2)), dim3(256), 0, st, p.dPG, rr, nb2, p.dOut);
   TC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(4 * p.npair)), dim3(256), 0, st, p.dPV, kBlocks, 4 * p.npair,
+  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(nk * p.npair)), dim3(256), 0, st, p.dPV, kBlocks, nk * p.npair,
                      p.dOut + 6 * nb2);
   TC_HIP(hipGetLastError());
   TC_HIP(hipMemcpyAsync(out_host, p.dOut, sizeof(double) * tc_out_doubles(p.nb), hipMemcpyDeviceToHost, st));
   return 0;
 }
 
-void tc_assemble(const TcPlan& p, const double* out, double* G2, double* G3) {
+void tc_assemble(const TcPlan& p, const double* out, bool restricted, double* G2, double* G3) {
   const int nb = p.nb;
   const int64_t nb2 = (int64_t)nb * nb;
   for (int s = 0; s < 2; ++s) {
-    const double* Sc = out + (s * 3 + 0) * nb2;
-    const double* Vc = out + (s * 3 + 1) * nb2;
-    const double* S3 = out + (s * 3 + 2) * nb2;
-    const double* Rvec = out + 6 * nb2 + (s * 2 + 0) * p.npair;
-    const double* Rg3 = out + 6 * nb2 + (s * 2 + 1) * p.npair;
+    const int so = restricted ? 0 : s;  // the spin whose blocks were formed
+    const double* Sc = out + (so * 3 + 0) * nb2;
+    const double* Vc = out + (so * 3 + 1) * nb2;
+    const double* S3 = out + (so * 3 + 2) * nb2;
+    const double* Rvec = out + 6 * nb2 + (so * 2 + 0) * p.npair;
+    const double* Rg3 = out + 6 * nb2 + (so * 2 + 1) * p.npair;
     for (int j = 0; j < nb; ++j)
       for (int i = 0; i < nb; ++i) {
         const double vji = Vc[j * nb + i] + Rvec[pk(j, i)];
@@ -740,27 +844,29 @@ extern "C" {
  * orbitals C (2 x nb x nmax, column b of spin s valid for b < nocc_s) to
  * G2 and G3 (each 2 x nb x nb, host), before the one-electron mask. */
 int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1, int64_t n1,
-                  const double* Fs, const double* Fo, const double* V, const tcscf_jastrow* jp, const double* D_host,
-                  const double* C_host, double* G2_host, double* G3_host, void* stream) {
+                  const void* Fs, const void* Fo, const void* V, int f_dtype, const tcscf_jastrow* jp,
+                  const double* D_host, const double* C_host, double* G2_host, double* G3_host, void* stream) {
   TC_CHECK(D_host && C_host && G2_host && G3_host, "tcscf_tc_fock: null");
   hipStream_t st = (hipStream_t)stream;
   int rc = 0;
   double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1), &rc);
   if (rc) return rc;
   TcPlan plan;
-  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, jp, ws))) return rc;
+  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, f_dtype, jp, ws))) return rc;
   std::vector<double> out(tc_out_doubles(nb));
-  if ((rc = tc_enqueue(plan, D_host, C_host, out.data(), st))) return rc;
+  const bool restricted = tc_restricted(plan, D_host, C_host);
+  if ((rc = tc_enqueue(plan, D_host, C_host, out.data(), restricted, st))) return rc;
   TC_HIP(hipStreamSynchronize(st));
-  tc_assemble(plan, out.data(), G2_host, G3_host);
+  tc_assemble(plan, out.data(), restricted, G2_host, G3_host);
   return 0;
 }
 
 /* TCHartreeFock::compute (src/HartreeFock.cpp:1007-1362), real basis.
- * Fs, Fo: packed n1 x npair (FP64); V: packed n1 x 3 x npair for c = 1;
+ * Fs, Fo: packed n1 x npair; V: packed n1 x 3 x npair for c = 1 (all three
+ * f_dtype: TCSCF_F32 runs the per-point terms of the Fock build in FP32);
  * A1: n1 x nb; dA1: n1 x nb x 3.  D_out 2 x nb x nb, pseudo_out nup + ndown. */
 int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1, const double* dA1,
-                 const double* w1, int64_t n1, const double* Fs, const double* Fo, const double* V,
+                 const double* w1, int64_t n1, const void* Fs, const void* Fo, const void* V, int f_dtype,
                  const tcscf_jastrow* jp, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
                  double* D_out, double* pseudo_out, void* stream) {
   TC_CHECK(S && H && o && res && o->max_iter > 0, "tcscf_tc_scf: bad arguments");
@@ -772,7 +878,7 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
   double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1), &rc);
   if (rc) return rc;
   TcPlan plan;
-  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, jp, ws))) return rc;
+  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, f_dtype, jp, ws))) return rc;
   TcIter iter(nup, ndown, nb, S, H, o);
   std::vector<double> out(tc_out_doubles(nb)), G2(2 * nb2), G3(2 * nb2);
   res->status = TCSCF_ENOTCONV;
@@ -781,9 +887,10 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
   res->t_sweep_ms = res->t_solve_ms = 0;
   for (int it = 0; it < o->max_iter; ++it) {
     double t0 = now_ms();
-    if ((rc = tc_enqueue(plan, iter.D.data(), iter.C.data(), out.data(), st))) return rc;
+    const bool restricted = tc_restricted(plan, iter.D.data(), iter.C.data());
+    if ((rc = tc_enqueue(plan, iter.D.data(), iter.C.data(), out.data(), restricted, st))) return rc;
     TC_HIP(hipStreamSynchronize(st));
-    tc_assemble(plan, out.data(), G2.data(), G3.data());
+    tc_assemble(plan, out.data(), restricted, G2.data(), G3.data());
     double t1 = now_ms();
     res->t_sweep_ms += t1 - t0;
     bool done = false;

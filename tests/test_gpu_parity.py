@@ -379,22 +379,28 @@ def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown):
     assert rel_err(G3, G3q) < 1e-11
 
 
-@pytest.mark.parametrize("nbasis,nup,ndown", [(12, 7, 6), (20, 6, 7), (36, 9, 8)])
-def test_tc_fock_build_many_orbitals_vs_oracle(tc, O, gpu, nbasis, nup, ndown):
+@pytest.mark.parametrize("nbasis,nup,ndown,same", [(12, 7, 6, False), (20, 6, 7, False), (36, 9, 8, False),
+                                                    (20, 6, 6, True), (36, 7, 7, True)])
+def test_tc_fock_build_many_orbitals_vs_oracle(tc, O, gpu, nbasis, nup, ndown, same):
     """The device TC Fock build with more occupied orbitals than one register
     block of W = V C holds (several blocks and LDS chunks for |U|^2), for each
     lane split of the basis (16, 32 and 64 lanes per basis row), against the
-    oracle's factorized restatement."""
+    oracle's factorized restatement.  same: a closed shell with identical spin
+    densities and orbitals, which forms spin 0 once; both spin blocks must then
+    be bitwise equal."""
     basis = (_mixed_basis() * 3)[:nbasis]
     nb = len(basis)
     g1, g3, jas, Fs, Fo, V = _tc_case(tc, basis, (5, 7, tc.RADIAL_M4), (6, 7, tc.RADIAL_M4))
     rng = np.random.default_rng(nbasis)
     C = np.zeros((2, nb, max(nup, ndown)))
     D = np.zeros((2, nb, nb))
+    C0 = rng.normal(size=(nb, max(nup, ndown))) * 0.3
     for s, no in enumerate((nup, ndown)):
-        C[s, :, :no] = rng.normal(size=(nb, no)) * 0.3
+        C[s, :, :no] = C0[:, :no] if same else rng.normal(size=(nb, no)) * 0.3
         D[s] = C[s, :, :no] @ C[s, :, :no].T
     G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, V, jas, D, C)
+    if same:
+        assert np.array_equal(G2[0], G2[1]) and np.array_equal(G3[0], G3[1])
     A = g1.basisValues.cpu().numpy()
     dA = g1.basisGradients.cpu().numpy()
     Vf = tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1)
@@ -594,6 +600,7 @@ def test_cpp_facade_on_device(tc, O, gpu):
     Sb, Hb = tc.one_electron_matrices(bb, 4)
     etc = tc.TCHartreeFock(2, 2, len(bb)).compute(Sb, Hb, g1, Fs, Fo2, V, jas, True, tol=1e-10, mgs_mode=1)
     assert abs(out["be_tc"] - etc) < 1e-9
+    assert abs(out["be_tc_mixed"] - etc) < 1e-6
 
 
 def test_batched_contraction_matches_single_calls(tc, gpu):
