@@ -46,6 +46,10 @@ def parse():
     p.add_argument("--no-tc", action="store_true", help="skip the TC-HF Ne side measurement")
     p.add_argument("--tc-mgs", type=int, default=1, help="TC Gram-Schmidt: 0 reference, 1 metric")
     p.add_argument("--tc-diis", type=int, default=6, help="TC DIIS depth (the TC workload asks for 6; 0 = plain)")
+    p.add_argument("--tc-smooth-precision", default="bf16x3", choices=["bf16x5", "bf16x3", "fp32", "fp64"],
+                   help="mixed TC: precision of the smooth 1/(r12 + a12)^4 part of F_same / F_oppo")
+    p.add_argument("--tc-v-precision", default="bf16x3", choices=["bf16x5", "bf16x3", "fp32", "fp64"],
+                   help="mixed TC: precision of the Jastrow-gradient tensor V")
     p.add_argument("--no-batch", action="store_true", help="skip the batched 64-instance side measurement")
     p.add_argument("--pmc-probe", action="store_true",
                    help="for rocprofv3 --pmc runs: one BF16x5 and one FP64 computeF on the Ne grid, one RHF "
@@ -192,7 +196,12 @@ def main():
 def run_tc(a, tc, configs, torch):
     """Transcorrelated HF on Ne (30 STOs, b = 1, c_same = c_oppo / 2): r1 = r2 =
     30,200 points, r3 = 50 x Lebedev-194 = 9,700 points; one mixed-precision
-    and one FP64 solve, precompute (F_same, F_oppo, V) + SCF timed."""
+    and one FP64 solve, precompute (F_same, F_oppo, V) + SCF timed.  The FP64
+    path computes F_same and F_oppo with two computeF calls as the reference's
+    driver does; the mixed path computes them together from the 1/r12
+    contraction (at --precision) and the contraction of the smooth part
+    1/(r12 + a12)^4 (at --tc-smooth-precision), V at --tc-v-precision, all
+    three stored in FP32 (the Fock build sums over points in FP64)."""
     cfg = configs.ne_config3()
     basis = cfg["basis"]
     nb = len(basis)
@@ -200,15 +209,32 @@ def run_tc(a, tc, configs, torch):
     g3 = tc.QMCGrid.radial_lebedev(*cfg["grid3"], 1.0, 1.0)
     S, H = tc.one_electron_matrices(basis, cfg["Z"])
     jas = tc.Jastrow.from_b(1.0)
+    mixed = a.precision != "fp64"
+
+    def tensors(prec):
+        """(F_same, F_oppo, V, ms spent on F_same + F_oppo)."""
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        if prec == "fp64":
+            Fs = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=True, precision="fp64")
+            Fo = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=False, precision="fp64")
+        else:
+            Fs, Fo = tc.computeF_tc_pair(g1, g1, jas, precision=prec, smooth_precision=a.tc_smooth_precision,
+                                         out_dtype="fp32")
+        torch.cuda.synchronize()
+        tf = (time.perf_counter() - t) * 1e3
+        V = tc.computeV(g1, g3, jas, precision="fp64" if prec == "fp64" else a.tc_v_precision,
+                        out_dtype="fp64" if prec == "fp64" else "fp32")
+        return Fs, Fo, V, tf
+
     out = {}
-    for prec in (a.precision, "fp64"):
+    legs = (a.precision, "fp64") if mixed else ("fp64",)
+    for prec in legs + legs:  # the first pass over the legs warms up (workspaces, code objects); the second is timed
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         g1.setBasisFunctions(basis, True)
         g3.setBasisFunctions(basis, True)
-        Fs = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=True, precision=prec)
-        Fo = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=False, precision=prec)
-        V = tc.computeV(g1, g3, jas, precision=prec)
+        Fs, Fo, V, tf = tensors(prec)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         hf = tc.TCHartreeFock(cfg["nup"], cfg["ndown"], nb)
@@ -220,15 +246,29 @@ def run_tc(a, tc, configs, torch):
             e, status = float("nan"), str(err)
         t2 = time.perf_counter()
         out[prec] = {"energy": e, "status": status, "precompute_ms": (t1 - t0) * 1e3, "scf_ms": (t2 - t1) * 1e3,
-                     "total_ms": (t2 - t0) * 1e3, "iterations": hf.numberOfConvergenceLoops()}
+                     "total_ms": (t2 - t0) * 1e3, "F_ms": tf, "iterations": hf.numberOfConvergenceLoops(),
+                     "fock_build_ms": hf.t_sweep_ms / max(1, hf.numberOfConvergenceLoops())}
         del Fs, Fo, V
     n1, n3 = g1.nGrid(), g3.nGrid()
-    mix, f64 = out[a.precision], out["fp64"]
-    return {"workload": f"TC-HF Ne, {nb} STOs, b=1, r1=r2={n1}, r3={n3}, MGS {'metric' if a.tc_mgs else 'reference'}, "
-                        f"DIIS {a.tc_diis}",
-            "pairs_x_components": 2 * n1 * (n1 - 1) + 3 * n1 * n3,
-            a.precision: mix, "fp64": f64, "energy_dev_vs_fp64": mix["energy"] - f64["energy"],
-            "speedup_vs_fp64": f64["total_ms"] / mix["total_ms"]}
+    res = {"workload": f"TC-HF Ne, {nb} STOs, b=1, r1=r2={n1}, r3={n3}, MGS {'metric' if a.tc_mgs else 'reference'}, "
+                       f"DIIS {a.tc_diis}",
+           "pairs_x_components": 2 * n1 * (n1 - 1) + 3 * n1 * n3, "fp64": out["fp64"]}
+    if mixed:
+        mix, f64 = out[a.precision], out["fp64"]
+        # the FP64 path with F_same / F_oppo from the pair form as well (F only, for the second ratio)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Fs, Fo = tc.computeF_tc_pair(g1, g1, jas, precision="fp64")
+        torch.cuda.synchronize()
+        pair64 = (time.perf_counter() - t0) * 1e3
+        del Fs, Fo
+        res.update({a.precision: mix, "mixed_tensors": {"F_coulomb": a.precision, "F_smooth": a.tc_smooth_precision,
+                                                        "V": a.tc_v_precision, "storage": "fp32"},
+                    "energy_dev_vs_fp64": mix["energy"] - f64["energy"],
+                    "speedup_vs_fp64": f64["total_ms"] / mix["total_ms"],
+                    "fp64_pair_F_ms": pair64,
+                    "speedup_vs_fp64_with_pair_F": (f64["total_ms"] - f64["F_ms"] + pair64) / mix["total_ms"]})
+    return res
 
 
 def run_batch(a, tc, configs, torch):

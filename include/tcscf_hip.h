@@ -42,7 +42,8 @@ enum tcscf_status {
 enum tcscf_kernel {
   TCSCF_K_COULOMB = 0,      /* 1/r12 */
   TCSCF_K_TC_SCALAR = 1,    /* 2/r12 - |grad1 u(r1,r2)|^2 - |grad1 u(r2,r1)|^2 */
-  TCSCF_K_JASTROW_GRAD = 2  /* grad1 u(r1,r2), 3 components, for c = 1 */
+  TCSCF_K_JASTROW_GRAD = 2, /* grad1 u(r1,r2), 3 components, for c = 1 */
+  TCSCF_K_TC_SMOOTH = 3     /* 1/(r12 + a12)^4, the smooth part of TCSCF_K_TC_SCALAR */
 };
 
 enum tcscf_precision {
@@ -165,6 +166,18 @@ int tcscf_compute_F(const double* pts1_dev, int64_t n1, const double* pts2_dev, 
                     const tcscf_jastrow* jastrow, int same_spin, int precision, double scale, void* F_dev,
                     int out_dtype, void* stream);
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
+
+/* F_same and F_oppo of TCSCF_K_TC_SCALAR together (the two computeF calls of
+ * src/unitTests/HartreeFockTest.cpp:276-296): the kernel is
+ * 2/r12 - 2 c^2 a12^2 / (r12 + a12)^4, so F_spin = 2 F_C - 2 c_spin^2 a12^2 G
+ * with F_C the 1/r12 contraction (at precision) and G the contraction of the
+ * smooth, bounded 1/(r12 + a12)^4 (at smooth_precision, which may be a cheaper
+ * split than the one the 1/r12 cusp needs).  Fs_dev, Fo_dev: n1 x npair each,
+ * out_dtype; other arguments as tcscf_compute_F. */
+int tcscf_compute_F_tc_pair(const double* pts1_dev, int64_t n1, const double* pts2_dev, const double* w2_dev,
+                            const int32_t* ord2_dev, int64_t n2, const double* A2_dev, int nb,
+                            const tcscf_jastrow* jastrow, int precision, int smooth_precision, double scale,
+                            void* Fs_dev, void* Fo_dev, int out_dtype, void* stream);
 
 /* One instance of a batched contraction: the arguments of tcscf_compute_F
  * (FP64 output, packed n1 x ncomp x npair; scale = 2 pi for a 2-D r2 grid). */

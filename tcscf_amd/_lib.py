@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libtcscf_hip.so")
 
 # enums (include/tcscf_hip.h)
 OK, EINVAL, EHIP, ENOTCONV, ESOLVER, ENOMEM = 0, -1, -2, -3, -4, -5
-K_COULOMB, K_TC_SCALAR, K_JASTROW_GRAD = 0, 1, 2
+K_COULOMB, K_TC_SCALAR, K_JASTROW_GRAD, K_TC_SMOOTH = 0, 1, 2, 3
 FP64, FP32, BF16X3, BF16X5 = 0, 1, 2, 3
 F64, F32 = 0, 1
 RADIAL_M4, RADIAL_BECKE, RADIAL_XOVER1MX = 0, 1, 2
@@ -22,7 +22,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16x5": BF16X5}
 EXPORTS = [
     "tcscf_last_error", "tcscf_version", "tcscf_radial_grid", "tcscf_lebedev_npts", "tcscf_lebedev",
     "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
-    "tcscf_F_workspace_bytes", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
+    "tcscf_F_workspace_bytes", "tcscf_compute_F_tc_pair", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
     "tcscf_qmc_lattice_npts", "tcscf_qmc_lattice_grid", "tcscf_sobol_points", "tcscf_sobol_grid", "tcscf_load_sto",
     "tcscf_batch_compute_F", "tcscf_batch_tuned_rows", "tcscf_batch_scf",
     "tcscf_selftest_mfma",

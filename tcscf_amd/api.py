@@ -316,6 +316,36 @@ def computeF(r1Grid, r2Grid, kernel="coulomb", jastrow=None, same_spin=True, pre
     return _compute(r1Grid, r2Grid, k, jastrow, same_spin, precision, out_dtype, ordered)
 
 
+def computeF_tc_pair(r1Grid, r2Grid, jastrow, precision="fp64", smooth_precision=None, out_dtype="fp64",
+                     ordered=True):
+    """F_same and F_oppo of the TC scalar kernel together (the two computeF
+    calls of src/unitTests/HartreeFockTest.cpp:276-296).  The kernel is
+    2/r12 - 2 c^2 a12^2 / (r12 + a12)^4, so F_spin = 2 F_C - 2 c_spin^2 a12^2 G:
+    F_C (1/r12) at `precision`, G (the smooth part) at `smooth_precision`
+    (default: `precision`).  Returns (FjiSame, FjiOppo)."""
+    torch = _torch()
+    if r1Grid.nBasis() != r2Grid.nBasis() or r2Grid.basisValues is None:
+        raise ValueError("computeF_tc_pair: both grids need the same tabulated basis")
+    prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    sp = prec if smooth_precision is None else (
+        _lib.PRECISIONS[smooth_precision] if isinstance(smooth_precision, str) else int(smooth_precision))
+    nb = r2Grid.nBasis()
+    n1, n2 = r1Grid.nGrid(), r2Grid.nGrid()
+    dtype = torch.float64 if out_dtype in (torch.float64, "fp64", _lib.F64) else torch.float32
+    dev = r1Grid.quadratureGrid.points.device
+    Fs = torch.empty((n1, npair(nb)), dtype=dtype, device=dev)
+    Fo = torch.empty((n1, npair(nb)), dtype=dtype, device=dev)
+    scale = 2 * math.pi if r2Grid.ndim == 2 else 1.0
+    jp = jastrow._struct()
+    check(lib().tcscf_compute_F_tc_pair(_ptr(r1Grid.quadratureGrid.points), ctypes.c_int64(n1),
+                                        _ptr(r2Grid.quadratureGrid.points), _ptr(r2Grid.quadratureGrid.weights),
+                                        _ptr(r2Grid.radial_order) if ordered else None, ctypes.c_int64(n2),
+                                        _ptr(r2Grid.basisValues), nb, ctypes.byref(jp), prec, sp,
+                                        ctypes.c_double(scale), _ptr(Fs), _ptr(Fo),
+                                        _lib.F32 if dtype == torch.float32 else _lib.F64, _stream()))
+    return Fs, Fo
+
+
 def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64", ordered=True):
     """computeV (src/HartreeFock.hpp:185-229) for c = 1: packed V[n1, xyz, p].
     V_same = c_same * V, V_oppo = c_oppo * V."""

@@ -292,6 +292,13 @@ __device__ inline T k_tc_scalar(T s, T y, double a12, double c2a2) {
   return T(2) * y - T(c2a2) / (t2 * t2);
 }
 template <class T>
+__device__ inline T k_tc_smooth(T s, T y, double a12) {
+#pragma clang fp contract(off)
+  const T t = s * y + T(a12);
+  const T t2 = t * t;
+  return T(1) / (t2 * t2);
+}
+template <class T>
 __device__ inline T k_grad_scale(T s, T y, double a12) {
 #pragma clang fp contract(off)
   const T t = s * y + T(a12);
@@ -316,6 +323,19 @@ struct KernTCScalar {
     const T s = dist2(dx, dy, dz);
     const T y = rsqrt_any(s);
     out[0] = s > T(0) ? k_tc_scalar(s, y, a12, c2a2) : T(0);
+  }
+};
+
+// 1 / (r12 + a12)^4: the smooth part of the TC scalar kernel, which is
+// 2/r12 - 2 c^2 a12^2 / (r12 + a12)^4 (coincident pairs excluded).
+struct KernTCSmooth {
+  static constexpr int ncomp = 1;
+  double a12;
+  template <class T>
+  __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
+    const T s = dist2(dx, dy, dz);
+    const T y = rsqrt_any(s);
+    out[0] = s > T(0) ? k_tc_smooth(s, y, a12) : T(0);
   }
 };
 
@@ -810,6 +830,18 @@ int launch_bf16_mb(const double* p1, int64_t n1, const float4* x2, int64_t n2p, 
   }
 }
 
+// (Fs, Fo) <- (2 Fs - cs Fo, 2 Fs - co Fo) elementwise.
+template <class T>
+__global__ __launch_bounds__(256) void tc_pair_combine(T* __restrict__ Fs, T* __restrict__ Fo, int64_t len, double cs,
+                                                       double co) {
+#pragma clang fp contract(off)
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < len; e += (int64_t)gridDim.x * 256) {
+    const double fc = Fs[e], g = Fo[e];
+    Fs[e] = (T)(2 * fc - cs * g);
+    Fo[e] = (T)(2 * fc - co * g);
+  }
+}
+
 template <class Kern>
 int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, const double* w2, const int* ord2,
              int64_t n2, const double* A2, int nb, double scale, void* F, int out_f32, Kern kern, hipStream_t st) {
@@ -922,10 +954,41 @@ int tcscf_compute_F(const double* pts1, int64_t n1, const double* pts2, const do
     case TCSCF_K_JASTROW_GRAD:
       TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: gradient kernel needs jastrow parameters with a12 > 0");
       return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernJastrowGrad{jp->a12}, st);
+    case TCSCF_K_TC_SMOOTH:
+      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: TC kernel needs jastrow parameters with a12 > 0");
+      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernTCSmooth{jp->a12}, st);
     default:
       TC_CHECK(false, "tcscf_compute_F: unknown kernel");
   }
   return TCSCF_EINVAL;
+}
+
+// F_same and F_oppo of the TC scalar kernel from the Coulomb contraction F_C
+// (in Fs, at `precision`) and the contraction G of its smooth part (in Fo, at
+// `smooth_precision`): F_spin = 2 F_C - 2 c_spin^2 a12^2 G, in place.
+int tcscf_compute_F_tc_pair(const double* pts1, int64_t n1, const double* pts2, const double* w2,
+                            const int32_t* ord2, int64_t n2, const double* A2, int nb, const tcscf_jastrow* jp,
+                            int precision, int smooth_precision, double scale, void* Fs, void* Fo, int out_dtype,
+                            void* stream) {
+  TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F_tc_pair: jastrow parameters with a12 > 0 needed");
+  TC_CHECK(Fs && Fo && Fs != Fo, "tcscf_compute_F_tc_pair: two distinct outputs needed");
+  int rc = tcscf_compute_F(pts1, n1, pts2, w2, ord2, n2, A2, nb, TCSCF_K_COULOMB, nullptr, 1, precision, scale, Fs,
+                           out_dtype, stream);
+  if (rc) return rc;
+  if ((rc = tcscf_compute_F(pts1, n1, pts2, w2, ord2, n2, A2, nb, TCSCF_K_TC_SMOOTH, jp, 1, smooth_precision, scale,
+                            Fo, out_dtype, stream)))
+    return rc;
+  const double a2 = jp->a12 * jp->a12;
+  const double cs = 2 * jp->c_same * jp->c_same * a2, co = 2 * jp->c_oppo * jp->c_oppo * a2;
+  const int64_t len = n1 * npair_of(nb);
+  const unsigned grid = (unsigned)std::min<int64_t>((len + 255) / 256, 65536);
+  hipStream_t st = (hipStream_t)stream;
+  if (out_dtype == TCSCF_F32)
+    hipLaunchKernelGGL(tc_pair_combine<float>, dim3(grid), dim3(256), 0, st, (float*)Fs, (float*)Fo, len, cs, co);
+  else
+    hipLaunchKernelGGL(tc_pair_combine<double>, dim3(grid), dim3(256), 0, st, (double*)Fs, (double*)Fo, len, cs, co);
+  TC_HIP(hipGetLastError());
+  return 0;
 }
 
 // Runs one fragment of MFMA `which` (0 f64 16x16x4, 1 f32 16x16x4, 2 bf16

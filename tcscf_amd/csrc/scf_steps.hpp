@@ -65,10 +65,12 @@ struct TcPlan {
   int f32;                  // FP32 Fock build (per-point terms in FP32, sums over points in FP64)
   double c_same, c_oppo;
   double *dD, *dC, *dVec, *dAl, *dH, *dPG, *dPV, *dOut;
-  int wpw, stage;
+  // point-kernel launch, index 0 for two spins, 1 for the restricted build:
+  // waves per workgroup, V rows staged in LDS, LDS bytes, workgroups
+  int wpw[2], stage[2];
+  size_t sh1[2];
+  int64_t pblocks[2];
   int ustart[7];  // rank workgroups per target (prefix offsets)
-  size_t sh1;
-  int64_t pblocks;
 };
 size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1);
 size_t tc_out_doubles(int nb);  // the per-build result block copied back to the host

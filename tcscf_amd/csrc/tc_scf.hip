@@ -123,7 +123,9 @@ __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-__host__ __device__ inline int64_t point_wave_elems(int nb) { return 24 * (int64_t)nb; }
+// Per-wave LDS doubles of the point vectors for NS spins: A, dA, then X, VTSI,
+// kS and E per spin.
+__host__ __device__ inline int64_t point_wave_elems(int nb, int ns) { return (4 + 10 * (int64_t)ns) * nb; }
 
 // Sum of v over the lanes that share a basis index i (lane = jp * nbp + i).
 __device__ inline double part_sum(double v, int nbp) {
@@ -160,19 +162,19 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
   const int nb = a.nb, nmax = a.nmax;
   const int nbp = nb <= 16 ? 16 : (nb <= 32 ? 32 : 64), parts = 64 / nbp;
   const int64_t np = a.npair, nb2 = (int64_t)nb * nb;
-  double* Ds = smem;                            // [2][nb][nb]
-  double* Cs = Ds + 2 * nb2;               // [2][nb][nmax]
-  double* base = Cs + 2 * (int64_t)nb * nmax + wave * (point_wave_elems(nb) + staged_doubles<ST>(STAGE, nb));
+  double* Ds = smem;                       // [NS][nb][nb]
+  double* Cs = Ds + NS * nb2;              // [NS][nb][nmax]
+  double* base = Cs + NS * (int64_t)nb * nmax + wave * (point_wave_elems(nb, NS) + staged_doubles<ST>(STAGE, nb));
   double* As = base;                       // [nb]
   double* dAs = As + nb;                   // [nb][3]
-  double* Xs = dAs + 3 * nb;               // [2][nb]
-  double* VT = Xs + 2 * nb;                // VTSI [2][nb][3]
-  double* kS = VT + 6 * nb;                // [2][nb][3]
-  double* Es = kS + 6 * nb;                // [2][nb][3]
-  double* Wc = kS;                         // [2][nb][kWChunk][3], once kS and Es are consumed
-  ST* Sv = reinterpret_cast<ST*>(Es + 6 * nb);  // STAGE: the point's V rows [3][npair]
-  for (int64_t e = tid; e < 2 * nb2; e += blockDim.x) Ds[e] = a.D[e];
-  for (int64_t e = tid; e < 2 * (int64_t)nb * nmax; e += blockDim.x) Cs[e] = a.C[e];
+  double* Xs = dAs + 3 * nb;               // [NS][nb]
+  double* VT = Xs + NS * nb;               // VTSI [NS][nb][3]
+  double* kS = VT + 3 * NS * nb;           // [NS][nb][3]
+  double* Es = kS + 3 * NS * nb;           // [NS][nb][3]
+  double* Wc = kS;                         // [NS][nb][kWChunk][3], once kS and Es are consumed
+  ST* Sv = reinterpret_cast<ST*>(Es + 3 * NS * nb);  // STAGE: the point's V rows [3][npair]
+  for (int64_t e = tid; e < NS * nb2; e += blockDim.x) Ds[e] = a.D[e];
+  for (int64_t e = tid; e < NS * (int64_t)nb * nmax; e += blockDim.x) Cs[e] = a.C[e];
   __syncthreads();
 
   const int i = lane & (nbp - 1), jp = lane / nbp;
@@ -187,7 +189,8 @@ __global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
     if (act)
       for (int j = jp; j < nb; j += parts) {
         const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
-        body(j, fs ? double(fs[pj]) : 0.0, fo ? double(fo[pj]) : 0.0, double(vv[pj]), double(vv[np + pj]), double(vv[2 * np + pj]));
+        body(j, fs ? double(fs[pj]) : 0.0, fo ? double(fo[pj]) : 0.0, double(vv[pj]), double(vv[np + pj]),
+             double(vv[2 * np + pj]));
       }
   };
 
@@ -463,8 +466,16 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
   const int64_t p0 = kb / E;
   const int e0 = (int)(kb - p0 * E);
   const float invE = 1.0f / (float)E;
-  // operands of the k-step at k0: lane (g, r) holds L_{k0+g}[16 m + r] and R_{k0+g}[16 m + r]
-  auto fetch = [&](int k0, double (&av)[NT], double (&bv)[NT]) {
+  // Operands of the k-step at k0 (lane (g, r): L_{k0+g}[16 m + r] and
+  // R_{k0+g}[16 m + r]) in two halves: fetch issues the raw loads into a ring
+  // slot (no arithmetic on loaded values, so nothing waits for them there);
+  // use scales / masks them just before the MFMAs, three steps later.
+  struct Slot {
+    double l[NT], r[NT];
+    double w, al;
+    int sk;  // scale: 0 none, 1 -w, 2 2w, 3 alpha, 4 zero (past K)
+  };
+  auto fetch = [&](int k0, Slot& o) {
     const int k = k0 + g, kk = e0 + k;
     const bool kin = k < K;
     int q = (int)(((float)kk + 0.5f) * invE), e = kk - q * E;  // kk = q E + e
@@ -477,41 +488,38 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
     }
     const int64_t n = kin ? p0 + q : p0;
     if (!kin) e = 0;
+    // entry e of target tg: L / R source (0 the point's vectors, 1 A, 2 dA),
+    // offset and stride, and the scale
+    int ls = 1, lo = 0, rs = 0, ro = 0, rst = 1, sk = 0;
+    if (tg == 1) {
+      const bool z = e == 0;
+      ls = z ? 1 : 0;
+      lo = z ? 0 : (3 + e) * nb;  // (A, u1) | (-w t_c, dA_c), c = e - 1
+      rs = z ? 0 : 2;
+      ro = z ? nb : e - 1;
+      rst = z ? 1 : 3;
+      sk = z ? 0 : 1;
+    } else if (tg == 2) {
+      const bool z = e == 0, one = e == 1, tc3 = e >= 2 && e < 5;
+      ls = z ? 1 : 0;  // (A, u2) | (u3, A) | (2 w t_c, t_c) | (alpha W_bc, W_bc)
+      lo = z ? 0 : (one ? 3 * nb : (tc3 ? (2 + e) * nb : (kVecFixed + e - 5) * nb));
+      rs = one ? 1 : 0;
+      ro = z ? 2 * nb : (one ? 0 : lo);
+      sk = z || one ? 0 : (tc3 ? 2 : 3);
+    }  // tg == 0: (A, u0)
     const double* vn = a.vecs + (n * 2 + s) * a.nvec * nb;
     const double* An = a.A + n * nb;
-    const double* L = An;
-    const double* R = An;
-    int sR = 1;
-    double fl = kin ? 1.0 : 0.0;
-    if (tg == 0) {
-      R = vn;  // (A, u0)
-    } else if (tg == 1) {
-      if (e == 0) {
-        R = vn + nb;  // (A, u1)
-      } else {
-        L = vn + (3 + e) * nb;  // (-w t_c, dA_c), c = e - 1
-        R = a.dA + n * nb * 3 + (e - 1);
-        sR = 3;
-        fl *= -a.w[n];
-      }
-    } else {
-      if (e == 0) {
-        R = vn + 2 * nb;  // (A, u2)
-      } else if (e == 1) {
-        L = vn + 3 * nb;  // (u3, A)
-      } else if (e < 5) {
-        L = R = vn + (2 + e) * nb;  // (2 w t_c, t_c), c = e - 2
-        fl *= 2 * a.w[n];
-      } else {
-        L = R = vn + (kVecFixed + e - 5) * nb;  // (alpha W_bc, W_bc)
-        fl *= a.alpha[n * 2 + s];
-      }
-    }
+    const double* dAn = a.dA + n * nb * 3;
+    const double* L = (ls == 0 ? vn : An) + lo;
+    const double* R = (rs == 0 ? vn : (rs == 1 ? An : dAn)) + ro;
+    o.w = a.w[n];
+    o.al = a.alpha[n * 2 + s];
+    o.sk = kin ? sk : 4;
 #pragma unroll
     for (int m = 0; m < NT; ++m) {
-      const int row = 16 * m + r;
-      av[m] = row < nb ? fl * L[row] : 0.0;
-      bv[m] = (kin && row < nb) ? R[row * sR] : 0.0;
+      const int row = 16 * m + r < nb ? 16 * m + r : nb - 1;  // in range; masked in use
+      o.l[m] = L[row];
+      o.r[m] = R[row * rst];
     }
   };
   d4 acc[NT][NT];
@@ -519,20 +527,33 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
   for (int m = 0; m < NT; ++m)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[m][t] = d4{0, 0, 0, 0};
-  // the loads of the next k-step are issued before the MFMAs of this one
-  double av[NT] = {}, bv[NT] = {}, an[NT] = {}, bn[NT] = {};
-  if (4 * wave < K) fetch(4 * wave, av, bv);
-  for (int k0 = 4 * wave; k0 < K; k0 += 16) {
-    if (k0 + 16 < K) fetch(k0 + 16, an, bn);
+  auto use = [&](const Slot& o) {
+    const double fl = o.sk == 0 ? 1.0 : (o.sk == 1 ? -o.w : (o.sk == 2 ? 2 * o.w : (o.sk == 3 ? o.al : 0.0)));
+    double av[NT], bv[NT];
+#pragma unroll
+    for (int m = 0; m < NT; ++m) {
+      const bool in = 16 * m + r < nb;
+      av[m] = in ? fl * o.l[m] : 0.0;
+      bv[m] = in && o.sk != 4 ? o.r[m] : 0.0;
+    }
 #pragma unroll
     for (int m = 0; m < NT; ++m)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[t], acc[m][t], 0, 0, 0);
-#pragma unroll
-    for (int m = 0; m < NT; ++m) {
-      av[m] = an[m];
-      bv[m] = bn[m];
-    }
+  };
+  // a ring of three slots: loads run three k-steps ahead of their MFMAs (steps
+  // past K load in-range data that use() zeroes)
+  Slot s0, s1, s2;
+  const int kw = 4 * wave;
+  fetch(kw, s0);
+  fetch(kw + 16, s1);
+  for (int k0 = kw; k0 < K; k0 += 48) {
+    fetch(k0 + 32, s2);
+    use(s0);
+    fetch(k0 + 48, s0);
+    use(s1);
+    fetch(k0 + 64, s1);
+    use(s2);
   }
   // fixed-order sum of the 4 waves' tiles
   for (int w = 0; w < 4; ++w) {
@@ -553,21 +574,23 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
 }
 
 // Pass 3: Rv[s][target][p] = sum_n sum_c h[n][s][target][c] V[n][c][p], for the
-// nk = 2 x spins (s, target) pairs.
-template <class ST>
+// NK = 2 x spins (s, target) pairs.
+template <class ST, int NK>
 __global__ __launch_bounds__(256) void tc_vlin_kernel(const ST* __restrict__ V, const double* __restrict__ hvec,
-                                                      int64_t n1, int64_t npair, int nk, double* __restrict__ part) {
+                                                      int64_t n1, int64_t npair, double* __restrict__ part) {
   const int64_t per = (n1 + gridDim.x - 1) / gridDim.x;
   const int64_t p0 = (int64_t)blockIdx.x * per;
   const int64_t p1 = p0 + per < n1 ? p0 + per : n1;
   for (int64_t p = threadIdx.x; p < npair; p += 256) {
-    double acc[4] = {0, 0, 0, 0};
+    double acc[NK] = {};
     for (int64_t n = p0; n < p1; ++n) {
       const double v0 = V[(n * 3) * npair + p], v1 = V[(n * 3 + 1) * npair + p], v2 = V[(n * 3 + 2) * npair + p];
       const double* h = hvec + n * 12;
-      for (int k = 0; k < nk; ++k) acc[k] += v0 * h[3 * k] + v1 * h[3 * k + 1] + v2 * h[3 * k + 2];
+#pragma unroll
+      for (int k = 0; k < NK; ++k) acc[k] += v0 * h[3 * k] + v1 * h[3 * k + 1] + v2 * h[3 * k + 2];
     }
-    for (int k = 0; k < nk; ++k) part[((int64_t)blockIdx.x * nk + k) * npair + p] = acc[k];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) part[((int64_t)blockIdx.x * NK + k) * npair + p] = acc[k];
   }
 }
 
@@ -680,23 +703,26 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   rank_units(nup, ndown, n1, p.ustart);
   p.dPV = p.dPG + (int64_t)p.ustart[6] * nb2;
   p.dOut = p.dPV + (int64_t)kBlocks * 4 * p.npair;
-  // waves per workgroup: as many (up to 4) as the LDS holds next to D and C
-  const size_t shared = sizeof(double) * (2 * nb2 + 2 * (int64_t)nb * p.nmax);
-  // the V rows staged in LDS when 8 waves of a workgroup hold them (the gathers
-  // then come from LDS and 8 waves hide their latency), else gathers from
-  // memory with up to 4 waves per workgroup
-  const size_t staged_wave =
-      sizeof(double) * (point_wave_elems(nb) + (p.f32 ? staged_doubles<float>(true, nb) : staged_doubles<double>(true, nb)));
-  p.stage = shared + 8 * staged_wave <= 160 * 1024 ? 1 : 0;
-  const size_t per_wave = p.stage ? staged_wave : sizeof(double) * point_wave_elems(nb);
-  p.wpw = p.stage ? 8 : 4;
-  while (p.wpw > 1 && shared + p.wpw * per_wave > 160 * 1024) p.wpw /= 2;
-  p.sh1 = shared + p.wpw * per_wave;
-  TC_CHECK(p.sh1 <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
-  if (p.sh1 > 64 * 1024)
-    for (bool r : {false, true})
-      TC_HIP(hipFuncSetAttribute(point_kernel(p.f32, p.stage, r), hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.sh1));
-  p.pblocks = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + p.wpw - 1) / p.wpw));
+  // Point-kernel launch per spin count (index 0: two spins, 1: restricted).
+  // The V rows are staged in LDS when 8 waves of a workgroup hold them (the
+  // gathers then come from LDS and 8 waves hide their latency), else the
+  // gathers go to memory with up to 4 waves per workgroup.
+  for (int r = 0; r < 2; ++r) {
+    const int ns = r ? 1 : 2;
+    const size_t shared = sizeof(double) * ns * (nb2 + (int64_t)nb * p.nmax);  // D and C
+    const size_t staged_wave =
+        sizeof(double) * (point_wave_elems(nb, ns) + (p.f32 ? staged_doubles<float>(true, nb) : staged_doubles<double>(true, nb)));
+    p.stage[r] = shared + 8 * staged_wave <= 160 * 1024 ? 1 : 0;
+    const size_t per_wave = p.stage[r] ? staged_wave : sizeof(double) * point_wave_elems(nb, ns);
+    p.wpw[r] = p.stage[r] ? 8 : 4;
+    while (p.wpw[r] > 1 && shared + p.wpw[r] * per_wave > 160 * 1024) p.wpw[r] /= 2;
+    p.sh1[r] = shared + p.wpw[r] * per_wave;
+    TC_CHECK(p.sh1[r] <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
+    if (p.sh1[r] > 64 * 1024)
+      TC_HIP(hipFuncSetAttribute(point_kernel(p.f32, p.stage[r], r), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)p.sh1[r]));
+    p.pblocks[r] = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + p.wpw[r] - 1) / p.wpw[r]));
+  }
   return 0;
 }
 
@@ -715,8 +741,9 @@ int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, doub
   PointArgs a{p.Fs, p.Fo, p.V, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb, p.nmax,
               p.nup, p.ndown, p.nvec, p.dVec, p.dAl, p.dH};
   void* pargs[] = {&a};
-  TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage, restricted), dim3((unsigned)p.pblocks), dim3(64 * p.wpw), pargs,
-                         p.sh1, st));
+  const int r = restricted ? 1 : 0;
+  TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage[r], restricted), dim3((unsigned)p.pblocks[r]), dim3(64 * p.wpw[r]),
+                         pargs, p.sh1[r], st));
   RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, {}, p.dPG};
   RankRanges rr{};
   for (int q = 0; q < 7; ++q) ra.ustart[q] = rr.ustart[q] = p.ustart[q];
@@ -729,12 +756,15 @@ int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, doub
     hipLaunchKernelGGL(tc_rank_kernel<4>, rgrid, dim3(256), 0, st, ra);
   TC_HIP(hipGetLastError());
   const int nk = restricted ? 2 : 4;  // (spin, target) pairs of the V-linear sums
-  if (p.f32)
-    hipLaunchKernelGGL(tc_vlin_kernel<float>, dim3(kBlocks), dim3(256), 0, st, (const float*)p.V, p.dH, p.n1,
-                       p.npair, nk, p.dPV);
+  const dim3 vg(kBlocks), vb(256);
+  if (p.f32 && restricted)
+    hipLaunchKernelGGL((tc_vlin_kernel<float, 2>), vg, vb, 0, st, (const float*)p.V, p.dH, p.n1, p.npair, p.dPV);
+  else if (p.f32)
+    hipLaunchKernelGGL((tc_vlin_kernel<float, 4>), vg, vb, 0, st, (const float*)p.V, p.dH, p.n1, p.npair, p.dPV);
+  else if (restricted)
+    hipLaunchKernelGGL((tc_vlin_kernel<double, 2>), vg, vb, 0, st, (const double*)p.V, p.dH, p.n1, p.npair, p.dPV);
   else
-    hipLaunchKernelGGL(tc_vlin_kernel<double>, dim3(kBlocks), dim3(256), 0, st, (const double*)p.V, p.dH, p.n1,
-                       p.npair, nk, p.dPV);
+    hipLaunchKernelGGL((tc_vlin_kernel<double, 4>), vg, vb, 0, st, (const double*)p.V, p.dH, p.n1, p.npair, p.dPV);
   TC_HIP(hipGetLastError());
   hipLaunchKernelGGL(tc_reduce_ranges, dim3((unsigned)(nst * nb2)), dim3(256), 0, st, p.dPG, rr, nb2, p.dOut);
   TC_HIP(hipGetLastError());

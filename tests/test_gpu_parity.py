@@ -141,6 +141,27 @@ def test_F_fp64_tc_kernels(tc, O, gpu):
     assert rel_err(full, ref) < 1e-12
 
 
+def test_F_tc_pair_vs_oracle(tc, O, gpu):
+    """computeF_tc_pair (F_C and the smooth-part contraction G combined) at
+    FP64 against the oracle's TC scalar F from the general Ochi gradients, for
+    two Jastrow parameter sets; and the mixed form (BF16x5 F_C, BF16x3 G,
+    FP32 storage) against the FP64 pair (tolerance in the assertion)."""
+    basis = _mixed_basis()[:9]
+    g1 = _grid(tc, 11, 11, tc.RADIAL_M4)
+    g1.setBasisFunctions(basis, True)
+    for b in (1.0, 0.6):
+        jas = tc.Jastrow.from_b(b)
+        Fs, Fo = tc.computeF_tc_pair(g1, g1, jas)
+        for same, F in ((True, Fs), (False, Fo)):
+            prm = O.ochi_params(1.0, jas.a12, [(1, 0, 0, jas.c_oppo, jas.c_same)])
+            ref = _F_oracle(O, g1, g1, kernel=O.K_TC_SCALAR, prm=prm, same=same)
+            assert rel_err(tc.unpack(F.cpu().numpy(), 9), ref) < 1e-12
+        Ms, Mo = tc.computeF_tc_pair(g1, g1, jas, precision="bf16x5", smooth_precision="bf16x3", out_dtype="fp32")
+        assert Ms.dtype == gpu.float32
+        for M, F in ((Ms, Fs), (Mo, Fo)):
+            assert rel_err(M.double().cpu().numpy(), F.cpu().numpy()) < 3e-6
+
+
 @pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("bf16x3", 3e-5), ("bf16x5", 3e-6)])
 def test_F_mixed_precision_error(tc, gpu, precision, tol):
     """Mixed-precision F against the FP64 drop-in on the Ne basis (nb = 30):
