@@ -30,7 +30,7 @@ PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x5": 5}
 # HBM bytes per launch of the contraction kernel on this workload, from rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --pmc-probe`, each scaled by the
 # reading for a 512 MiB copy (FETCH x2.0, WRITE x1.0); profiles/bf16_fgemm_ne30_traffic.md
-TRAFFIC_BYTES = {"bf16x5": 2.936e9 + 1.145e8, "fp64": 1.016e9 + 1.156e8}
+TRAFFIC_BYTES = {"bf16x5": 1.502e9 + 1.163e8, "fp64": 1.016e9 + 1.156e8}
 
 
 def parse():

@@ -337,6 +337,29 @@ DeviceArray<FT> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const int
   const tcscf_jastrow j = jastrow.abi();
   return detail::contract<FT>(r1Grid, r2Grid, TCSCF_K_TC_SCALAR, &j, sameSpin ? 1 : 0, precision);
 }
+/* FjiSame and FjiOppo together: the TC scalar kernel is 2/r12 - 2 c^2 a12^2 / (r12 + a12)^4,
+ * so both follow from the 1/r12 contraction (at precision) and the contraction of the smooth
+ * part (at smoothPrecision). */
+template <class FT = double, class REAL, int N1, int N2>
+std::pair<DeviceArray<FT>, DeviceArray<FT>> computeFPair(const integration::QMCGrid<REAL, N1>& r1Grid,
+                                                         const integration::QMCGrid<REAL, N2>& r2Grid,
+                                                         const Jastrow& jastrow, Precision precision = Precision::FP64,
+                                                         Precision smoothPrecision = Precision::FP64) {
+  if (r1Grid.nBasis() != r2Grid.nBasis()) throw Error("computeFPair: different bases");
+  const int nb = static_cast<int>(r2Grid.nBasis());
+  if (nb == 0) throw Error("computeFPair: setBasisFunctions was not called on the inner grid");
+  const std::size_t len = static_cast<std::size_t>(r1Grid.nGrid()) * (static_cast<std::size_t>(nb) * (nb + 1) / 2);
+  DeviceArray<FT> Fs(len), Fo(len);
+  const tcscf_jastrow j = jastrow.abi();
+  const double scale = N2 == 2 ? 2 * 3.14159265358979323846 : 1.0;
+  detail::check(tcscf_compute_F_tc_pair(r1Grid.quadratureGrid.points.data(), r1Grid.nGrid(),
+                                        r2Grid.quadratureGrid.points.data(), r2Grid.quadratureGrid.weights.data(),
+                                        r2Grid.radialOrder.data(), r2Grid.nGrid(), r2Grid.basisValues.data(), nb, &j,
+                                        static_cast<int>(precision), static_cast<int>(smoothPrecision), scale,
+                                        Fs.data(), Fo.data(), detail::dtype<FT>(), r1Grid.stream));
+  detail::hip(hipStreamSynchronize(r1Grid.stream));
+  return {std::move(Fs), std::move(Fo)};
+}
 /* computeV (src/HartreeFock.hpp:185-229) for c = 1: packed V[n1][xyz][p(j, i)]. */
 template <class FT = double, class REAL, int N1, int N3>
 DeviceArray<FT> computeV(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N3>& r3Grid,

@@ -126,10 +126,11 @@ static int gpu_runs(const std::string& dir) {
     tcscf::TCHartreeFock<double> tc(2, 2, static_cast<int>(basis.size()));
     const double e = tc.compute(S, H, g1, Fs, Fo, V, jas, true, 1e-10, 200, 1);
     std::printf("be_tc %.15f\nbe_tc_iters %d\n", e, tc.numberOfConvergenceLoops());
-    // mixed precision: BF16x5 contractions stored in FP32, FP32 per-point Fock terms
-    auto Fs32 = tcscf::computeF<float>(g1, g1, jas, true, tcscf::Precision::BF16X5);
-    auto Fo32 = tcscf::computeF<float>(g1, g1, jas, false, tcscf::Precision::BF16X5);
-    auto V32 = tcscf::computeV<float>(g1, g3, jas, tcscf::Precision::BF16X5);
+    // mixed precision: F pair (1/r12 at BF16x5, the smooth part at BF16x3), V at BF16x3, FP32 storage
+    auto F32 = tcscf::computeFPair<float>(g1, g1, jas, tcscf::Precision::BF16X5, tcscf::Precision::BF16X3);
+    auto& Fs32 = F32.first;
+    auto& Fo32 = F32.second;
+    auto V32 = tcscf::computeV<float>(g1, g3, jas, tcscf::Precision::BF16X3);
     tcscf::TCHartreeFock<double> tcm(2, 2, static_cast<int>(basis.size()));
     std::printf("be_tc_mixed %.15f\n", tcm.compute(S, H, g1, Fs32, Fo32, V32, jas, true, 1e-10, 200, 1, 6));
   }
