@@ -632,6 +632,16 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
 #pragma unroll
         for (int i = 0; i < 4; ++i) sum[c][m][n][i] = 0;
   f4 acc[NC][MB][NB];
+  auto fold_all = [&] {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
+  };
 
   const int64_t nsteps = n2p / 32;
   us8 bqA[NSPLIT][NB], bqB[NSPLIT][NB];  // B fragments, ping-pong between steps
@@ -645,12 +655,23 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
 
   // One 32-point step: MFMAs on ktile[s & 1] and cur, B of step s + 1 into nxt,
   // K of step s + 1 into the other tile.  FRESH: the step starts a new FP32
-  // partial (zero accumulator operand); the fold to FP64 follows every FOLD steps.
+  // partial (zero accumulator operand); the fold to FP64 follows every FOLD
+  // steps.  With several kernel components (ONEBLOCK) the step between the
+  // coordinate fetch and the barrier is one basic block (the B load past the
+  // last step re-reads the last step, and the K generated after the last step
+  // is not used), so the scheduler interleaves each wave's MFMAs with its
+  // generation and fold; with one component the MFMAs issue as one run and
+  // the other wave of the SIMD fills the VALU meanwhile (measured faster for
+  // one component, slower for three).
+  constexpr bool ONEBLOCK = NC > 1;
   auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto fresh) {
     constexpr bool FRESH = decltype(fresh)::value;
     const int buf = (int)(s & 1);
     if (s + 2 < nsteps) fetch_coords(s + 2);  // into crd[s & 1], read last by generate(s) one step ago
-    if (s + 1 < nsteps) load_b(nxt, s + 1);
+    if (ONEBLOCK)
+      load_b(nxt, s + 1 < nsteps ? s + 1 : s);
+    else if (s + 1 < nsteps)
+      load_b(nxt, s + 1);
     const f4 zero = {0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -673,17 +694,8 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
           acc[c][m][n] = v;
         }
       }
-    if (s + 1 < nsteps) generate(buf ^ 1);
-    if (!FRESH || FOLD == 1 || s + 1 == nsteps) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-          for (int n = 0; n < NB; ++n)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
-    }
+    if (ONEBLOCK || s + 1 < nsteps) generate(buf ^ 1);
+    if constexpr (!FRESH || FOLD == 1) fold_all();
     coords_landed();
     __syncthreads();
   };
@@ -693,6 +705,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
     step(s, bqA, bqB, T{});
     if (s + 1 < nsteps) step(s + 1, bqB, bqA, Fz{});
   }
+  if (FOLD == 2 && (nsteps & 1)) fold_all();  // the last, unpaired step
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
