@@ -92,6 +92,7 @@ def main():
         F = tc.computeF(grid, grid, precision=precision)
         if ev:
             ev[1].record(stream)
+        torch.cuda.synchronize()  # the SCF's own sweep timers then see the sweeps only
         hf = tc.RCSHartreeFock(cfg["nup"] + cfg["ndown"], nb)
         e = hf.compute(S, H, grid, F, True, tol=a.tol, max_iter=100)
         return e, hf
@@ -153,6 +154,7 @@ def main():
             rows = int(min(n, max(probe, probe * 20.0 / max(cpu_rows(probe), 1e-3))))
         tc_cpu = cpu_rows(rows)
         cpu = {"value": rows * n / tc_cpu, "unit": "grid-pair contractions/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(),
                "sample": f"oracle computeF (FP64, OpenMP, {threads} threads) on r1 rows 0..{rows} x all {n} r2 "
                          f"points, nb={nb}: {tc_cpu:.2f} s; the full precompute extrapolates to "
                          f"{tc_cpu * n / rows:.0f} s"}
@@ -185,12 +187,33 @@ def main():
         "computeF_ms": {a.precision: k_mix, "fp64": k64},
         "sweep_ms_per_scf": hf_mix.t_sweep_ms,
         "solve_ms_per_scf": hf_mix.t_solve_ms,
+        "sweep_roofline": hbm_roofline(n * npair * 8 + n * nb * 8, hf_mix.t_sweep_ms / max(1, hf_mix.numberOfConvergenceLoops()),
+                                       "hf_sweep_kernel: one read of the packed FP64 F and of A per iteration; time "
+                                       "per iteration as the SCF measures it (D upload, sweep, G download, sync)"),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "tc_ne": tc_res,
         "batch64": batch_res,
     }
     print(json.dumps(res))
+
+
+def hbm_roofline(nbytes, ms, note):
+    """Achieved HBM bandwidth of a per-iteration pass: algorithmic bytes / time."""
+    gbps = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "algorithmic_bytes": nbytes, "ms": round(ms, 4), "achieved": round(gbps, 1),
+            "peak": 8000.0, "unit": "GB/s", "frac": round(gbps / 8000.0, 4), "note": note}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def run_tc(a, tc, configs, torch):
@@ -245,9 +268,16 @@ def run_tc(a, tc, configs, torch):
         except tc.TcscfError as err:
             e, status = float("nan"), str(err)
         t2 = time.perf_counter()
+        fb_ms = hf.t_sweep_ms / max(1, hf.numberOfConvergenceLoops())
+        esz = 8 if prec == "fp64" else 4
+        n1 = g1.nGrid()
+        fb_bytes = 5 * n1 * tc.npair(nb) * esz + 4 * n1 * nb * 8  # F_same, F_oppo, V (3) + A, dA (3), once each
         out[prec] = {"energy": e, "status": status, "precompute_ms": (t1 - t0) * 1e3, "scf_ms": (t2 - t1) * 1e3,
                      "total_ms": (t2 - t0) * 1e3, "F_ms": tf, "iterations": hf.numberOfConvergenceLoops(),
-                     "fock_build_ms": hf.t_sweep_ms / max(1, hf.numberOfConvergenceLoops())}
+                     "fock_build_ms": fb_ms,
+                     "fock_build_roofline": hbm_roofline(fb_bytes, fb_ms, "TC Fock build (point, rank, V-linear "
+                                                         "passes): one read of F_same, F_oppo, V, A and dA per "
+                                                         "iteration; time per iteration as the SCF measures it")}
         del Fs, Fo, V
     n1, n3 = g1.nGrid(), g3.nGrid()
     res = {"workload": f"TC-HF Ne, {nb} STOs, b=1, r1=r2={n1}, r3={n3}, MGS {'metric' if a.tc_mgs else 'reference'}, "

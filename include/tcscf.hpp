@@ -2,6 +2,8 @@
  *
  * Mirrors the names, argument meaning and error behaviour of
  *   src/SlaterTypeOrbital.hpp      SlaterTypeOrbital, overlap, coreMatrixElement, loadSTO
+ *   src/OchiBasis.hpp, src/HydrogenLikeBasis.hpp
+ *                                  OchiBasisFunction, HydrogenLikeBasisFunction
  *   src/integration/integrateAll.hpp  integration::QMCGrid, integrateAllR1R2,
  *                                  threeElectronIntegrals
  *   src/HartreeFock.hpp            computeF / computeV (fused with precomputeIntegrand),
@@ -131,7 +133,7 @@ struct SlaterTypeOrbital {
     alpha = newAlpha;
     normalization = norm();
   }
-  tcscf_sto abi() const { return tcscf_sto{n, l, m, static_cast<double>(alpha)}; }
+  tcscf_basis_fn abi() const { return tcscf_basis_fn{TCSCF_BASIS_STO, n, l, m, static_cast<double>(alpha)}; }
 
   Real alpha;
   int n, l, m;
@@ -141,30 +143,80 @@ struct SlaterTypeOrbital {
   Real norm() const { return std::pow(2 * alpha, n) * std::sqrt(2 * alpha / std::tgamma(2.0 * n + 1)); }
 };
 
+/* OchiBasisFunction (src/OchiBasis.hpp:15-155): N r^l L_n^(2l+2)(2 alpha r)
+ * e^(-alpha r) S_lm, N = (2 alpha)^(l + 3/2) / sqrt((n + 2l + 2)! / n!).  The
+ * functions of one alpha are orthonormal; l <= 3 on the device. */
+template <class REAL>
+struct OchiBasisFunction {
+  using Real = REAL;
+  static constexpr bool isBasisReal = true;
+  static constexpr bool isOrthonormal = true;
+
+  OchiBasisFunction(Real alphaP, int nP, int lP, int mP) : alpha{alphaP}, n{nP}, l{lP}, m{mP} {
+    if (n < 0 || l < std::abs(m) || l > 3 || !(alpha > 0))
+      throw Error("OchiBasisFunction: need n >= 0, l >= |m|, l <= 3, alpha > 0");
+  }
+  tcscf_basis_fn abi() const { return tcscf_basis_fn{TCSCF_BASIS_OCHI, n, l, m, static_cast<double>(alpha)}; }
+
+  Real alpha;
+  int n, l, m;
+};
+
+/* HydrogenLikeBasisFunction (src/HydrogenLikeBasis.hpp:12-57): the bound
+ * eigenfunctions of -1/2 lap - Z/r, N r^l L_(n-l-1)^(2l+1)(2 beta r) e^(-beta r)
+ * S_lm with beta = Z / n and the reference's negative normalization.  The
+ * device also tabulates their gradients (the reference leaves them out). */
+template <class REAL>
+struct HydrogenLikeBasisFunction {
+  using Real = REAL;
+  static constexpr bool isBasisReal = true;
+  static constexpr bool isOrthonormal = true;
+
+  HydrogenLikeBasisFunction(int Z, int nP, int lP, int mP) : n{nP}, l{lP}, m{mP}, beta{Z / Real(nP)}, Z_{Z} {
+    if (Z <= 0 || n <= 0 || l >= n || l < std::abs(m) || l > 3)
+      throw Error("HydrogenLikeBasisFunction: need Z > 0, n > l >= |m|, l <= 3");
+  }
+  tcscf_basis_fn abi() const { return tcscf_basis_fn{TCSCF_BASIS_HYDROGEN, n, l, m, static_cast<double>(Z_)}; }
+
+  int n, l, m;
+  Real beta;
+
+ private:
+  int Z_;
+};
+
 namespace detail {
 template <class B>
-std::vector<tcscf_sto> abi(const std::vector<B>& fns) {
-  std::vector<tcscf_sto> out;
+std::vector<tcscf_basis_fn> abi(const std::vector<B>& fns) {
+  std::vector<tcscf_basis_fn> out;
   out.reserve(fns.size());
   for (const B& b : fns) out.push_back(b.abi());
   return out;
 }
+template <class B>
+void oneElectronPair(int Z, const B& b1, const B& b2, double& S, double& H) {
+  const tcscf_basis_fn f[2] = {b1.abi(), b2.abi()};
+  double s[4], h[4];
+  detail::check(tcscf_basis_one_electron(f, 2, Z, s, h));
+  S = s[1];
+  H = h[1];
+}
 }  // namespace detail
 
-/* overlap / coreMatrixElement (src/SlaterTypeOrbital.hpp:125-171). */
-template <class REAL>
-REAL overlap(const SlaterTypeOrbital<REAL>& b1, const SlaterTypeOrbital<REAL>& b2) {
-  const tcscf_sto f[2] = {b1.abi(), b2.abi()};
-  double S[4], H[4];
-  detail::check(tcscf_sto_overlap_core(f, 2, 0, S, H));
-  return static_cast<REAL>(S[1]);
+/* overlap / coreMatrixElement: src/SlaterTypeOrbital.hpp:125-171 (closed
+ * forms), src/OchiBasis.hpp:118-153 (the reference's coefficient times the
+ * radial integral, here in closed form), src/HydrogenLikeBasis.hpp:67-80. */
+template <class B>
+typename B::Real overlap(const B& b1, const B& b2) {
+  double S, H;
+  detail::oneElectronPair(0, b1, b2, S, H);
+  return static_cast<typename B::Real>(S);
 }
-template <class REAL>
-REAL coreMatrixElement(int Z, const SlaterTypeOrbital<REAL>& b1, const SlaterTypeOrbital<REAL>& b2) {
-  const tcscf_sto f[2] = {b1.abi(), b2.abi()};
-  double S[4], H[4];
-  detail::check(tcscf_sto_overlap_core(f, 2, Z, S, H));
-  return static_cast<REAL>(H[1]);
+template <class B>
+typename B::Real coreMatrixElement(int Z, const B& b1, const B& b2) {
+  double S, H;
+  detail::oneElectronPair(Z, b1, b2, S, H);
+  return static_cast<typename B::Real>(H);
 }
 /* Whole matrices at once (row-major nb x nb; S is symmetric, so it is also the
  * reference's column-major overlap view). */
@@ -173,7 +225,7 @@ void oneElectronMatrices(const std::vector<B>& fns, int Z, std::vector<double>& 
   const auto f = detail::abi(fns);
   S.assign(f.size() * f.size(), 0.0);
   H.assign(f.size() * f.size(), 0.0);
-  detail::check(tcscf_sto_overlap_core(f.data(), static_cast<int>(f.size()), Z, S.data(), H.data()));
+  detail::check(tcscf_basis_one_electron(f.data(), static_cast<int>(f.size()), Z, S.data(), H.data()));
 }
 
 /* loadSTO (src/SlaterTypeOrbital.cpp:24-76). */
@@ -247,8 +299,8 @@ struct QMCGrid {
     basisValues.resize(static_cast<std::size_t>(nGrid() * nb_));
     if (storeGradients) basisGradients.resize(static_cast<std::size_t>(nGrid() * nb_ * 3));
     else basisGradients.resize(0);
-    detail::check(tcscf_basis_eval(quadratureGrid.points.data(), nGrid(), f.data(), static_cast<int>(nb_),
-                                   basisValues.data(), storeGradients ? basisGradients.data() : nullptr, stream));
+    detail::check(tcscf_basis_eval_fns(quadratureGrid.points.data(), nGrid(), f.data(), static_cast<int>(nb_),
+                                       basisValues.data(), storeGradients ? basisGradients.data() : nullptr, stream));
     detail::hip(hipStreamSynchronize(stream));
   }
 

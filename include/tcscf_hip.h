@@ -68,6 +68,21 @@ typedef struct tcscf_sto {
   double zeta;
 } tcscf_sto;
 
+/* One function of the reference's atomic bases: g(r) S_lm(x, y, z), with S_lm
+ * = r^l Y_lm the real solid harmonic (l <= 3) and g
+ *   TCSCF_BASIS_STO       N r^(n-1-l) e^(-alpha r), n >= 1           (SlaterTypeOrbital.hpp:15-113)
+ *   TCSCF_BASIS_OCHI      N L_n^(2l+2)(2 alpha r) e^(-alpha r), n >= 0 (OchiBasis.hpp:15-110)
+ *   TCSCF_BASIS_HYDROGEN  N L_(n-l-1)^(2l+1)(2 b r) e^(-b r), b = alpha / n, with alpha the
+ *                         function's nuclear charge Z, n > l             (HydrogenLikeBasis.hpp:12-61)
+ * (L the associated Laguerre polynomials) and the reference's normalizations N
+ * (the hydrogen-like one keeps its negative sign). */
+enum tcscf_basis_kind { TCSCF_BASIS_STO = 0, TCSCF_BASIS_OCHI = 1, TCSCF_BASIS_HYDROGEN = 2 };
+typedef struct tcscf_basis_fn {
+  int kind;
+  int n, l, m;
+  double alpha; /* STO: zeta; Ochi: alpha; hydrogen-like: its Z */
+} tcscf_basis_fn;
+
 /* Jastrow u(r12) = c * r12 / (r12 + a12): the r12-only term of the reference's
  * Ochi form (src/jastrowFunctions.hpp:9-115) with S = {(1,0,0)}. */
 typedef struct tcscf_jastrow {
@@ -142,9 +157,22 @@ int tcscf_load_sto(const char* directory, const char* basis_set, const char* ele
 /* overlap / coreMatrixElement: src/SlaterTypeOrbital.hpp:125-171.  nb x nb row-major. */
 int tcscf_sto_overlap_core(const tcscf_sto* fns, int nb, int Z, double* S_host, double* H_host);
 
+/* overlap / coreMatrixElement for one kind of basis (all functions of one
+ * kind): STO as tcscf_sto_overlap_core; Ochi (OchiBasis.hpp:117-155, all
+ * functions of one alpha): S = delta, H = c I - delta_nn' alpha^2 / 2 with
+ * c = alpha (l+1)(1 + 2 min(n, n') / (2l+3)) - Z and I = int fnl fnl' / r dr,
+ * evaluated exactly from the Laguerre coefficients (the reference integrates
+ * it on a 1000-point radial grid); hydrogen-like (HydrogenLikeBasis.hpp:67-80):
+ * S = delta, H = -Z^2 / (2 n^2) delta.  nb x nb row-major. */
+int tcscf_basis_one_electron(const tcscf_basis_fn* fns, int nb, int Z, double* S_host, double* H_host);
+
 /* ---------------------------------------------------------------- device basis */
-/* QMCGrid::setBasisFunctions: src/integration/integrateAll.hpp:88-137.
+/* QMCGrid::setBasisFunctions: src/integration/integrateAll.hpp:88-137, for
+ * functions of any tcscf_basis_fn kinds (mixed kinds allowed).
  * values_dev: n x nb; grads_dev: n x nb x 3 or NULL. */
+int tcscf_basis_eval_fns(const double* pts_dev, int64_t n, const tcscf_basis_fn* fns_host, int nb, double* values_dev,
+                         double* grads_dev, void* stream);
+/* The same for a list of STOs. */
 int tcscf_basis_eval(const double* pts_dev, int64_t n, const tcscf_sto* fns_host, int nb, double* values_dev,
                      double* grads_dev, void* stream);
 

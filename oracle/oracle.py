@@ -7,6 +7,7 @@ never imports this module.
 All arrays are numpy float64, C-contiguous.  Grid points are SoA (3, N).
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -113,6 +114,58 @@ def sto_real(pts, fns, grads=False):
     g = np.zeros((n, len(f), 3)) if grads else None
     lib().orc_sto_real(_p(np.ascontiguousarray(pts)), ctypes.c_int64(n), _p(f), len(f), _p(v), _p(g))
     return (v, g) if grads else v
+
+
+def _laguerre_coeffs(q, k):
+    """L_q^(k)(x) = sum_j c_j x^j with c_j = (-1)^j C(q+k, q-j) / j! (the explicit
+    series; the device code uses the three-term recurrence)."""
+    return [(-1) ** j * math.comb(q + k, q - j) / math.factorial(j) for j in range(q + 1)]
+
+
+def atomic_as_stos(fn):
+    """An Ochi (src/OchiBasis.hpp:29-59) or hydrogen-like
+    (src/HydrogenLikeBasis.hpp:24-50) function as an exact linear combination of
+    normalized STOs of one exponent: N r^l L_q^(k)(2 a r) e^(-a r) Y_lm =
+    sum_j N c_j (2a)^j / N_sto(l+1+j, a) STO(l+1+j, a).  fn: (kind, n, l, m, alpha);
+    kind 1 Ochi (q = n, k = 2l+2, a = alpha), 2 hydrogen-like (q = n-l-1,
+    k = 2l+1, a = Z/n, alpha = Z).  Returns [(weight, (n', l, m, a)), ...]."""
+    kind, n, l, m, alpha = fn
+    if kind == 1:
+        q, k, a = n, 2 * l + 2, alpha
+        N = (2 * a) ** (l + 1.5) / math.sqrt(math.factorial(n + 2 * l + 2) / math.factorial(n))
+    else:
+        q, k, a = n - l - 1, 2 * l + 1, alpha / n
+        N = -(2 * a) ** (l + 1.5) / math.sqrt(2 * n * math.factorial(n + l) / math.factorial(n - l - 1))
+    out = []
+    for j, c in enumerate(_laguerre_coeffs(q, k)):
+        ns = l + 1 + j
+        nsto = (2 * a) ** ns * math.sqrt(2 * a / math.factorial(2 * ns))
+        out.append((N * c * (2 * a) ** j / nsto, (ns, l, m, a)))
+    return out
+
+
+def atomic_real(pts, fns, grads=False):
+    """Real-harmonic values (and gradients) of (kind, n, l, m, alpha) functions:
+    kind 0 STO (sto_real), 1 Ochi, 2 hydrogen-like (via atomic_as_stos)."""
+    terms, owner = [], []
+    for b, f in enumerate(fns):
+        if f[0] == 0:
+            terms.append((1.0, (f[1], f[2], f[3], f[4])))
+            owner.append(b)
+        else:
+            for w, sto in atomic_as_stos(f):
+                terms.append((w, sto))
+                owner.append(b)
+    r = sto_real(pts, [t[1] for t in terms], grads)
+    v, g = (r if grads else (r, None))
+    n = pts.shape[1]
+    vo = np.zeros((n, len(fns)))
+    go = np.zeros((n, len(fns), 3)) if grads else None
+    for t, (w, _) in enumerate(terms):
+        vo[:, owner[t]] += w * v[:, t]
+        if grads:
+            go[:, owner[t]] += w * g[:, t]
+    return (vo, go) if grads else vo
 
 
 def sto_S_H(fns, Z):

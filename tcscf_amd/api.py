@@ -68,6 +68,7 @@ def unpack(Fp, nb):
 # --------------------------------------------------------------------------- basis
 class SlaterTypeOrbital:
     """N r^(n-1) e^(-alpha r) S_lm with real solid harmonics S_lm (l <= 3)."""
+    kind = _lib.BASIS_STO
 
     def __init__(self, alpha, n, l, m):
         if n < 1 or n < l or l < abs(m) or l > 3:
@@ -84,6 +85,53 @@ class SlaterTypeOrbital:
 
     def __repr__(self):
         return f"STO(n={self.n}, l={self.l}, m={self.m}, zeta={self.alpha:g})"
+
+
+class OchiBasisFunction:
+    """OchiBasisFunction (src/OchiBasis.hpp:15-110): N r^l L_n^(2l+2)(2 alpha r)
+    e^(-alpha r) Y_lm, n >= 0, N = (2 alpha)^(l+3/2) / sqrt((n+2l+2)! / n!);
+    orthonormal for one alpha.  Real harmonics here (l <= 3)."""
+    kind = _lib.BASIS_OCHI
+
+    def __init__(self, alpha, n, l, m):
+        if n < 0 or l < abs(m) or l > 3 or alpha <= 0:
+            raise ValueError(f"invalid Ochi function (n={n}, l={l}, m={m}, alpha={alpha})")
+        self.alpha, self.n, self.l, self.m = float(alpha), int(n), int(l), int(m)
+
+    @property
+    def normalization(self):
+        n, l = self.n, self.l
+        return (2 * self.alpha) ** (l + 1.5) / math.sqrt(math.factorial(n + 2 * l + 2) / math.factorial(n))
+
+    def __repr__(self):
+        return f"Ochi(n={self.n}, l={self.l}, m={self.m}, alpha={self.alpha:g})"
+
+
+class HydrogenLikeBasisFunction:
+    """HydrogenLikeBasisFunction (src/HydrogenLikeBasis.hpp:12-61): the bound
+    eigenfunction (n, l, m) of charge Z, N r^l L_(n-l-1)^(2l+1)(2 b r) e^(-b r)
+    Y_lm with b = Z / n and the reference's (negative) normalization."""
+    kind = _lib.BASIS_HYDROGEN
+
+    def __init__(self, Z, n, l, m):
+        if Z <= 0 or n <= l or l < abs(m) or l > 3:
+            raise ValueError(f"invalid hydrogen-like function (Z={Z}, n={n}, l={l}, m={m})")
+        self.Z, self.n, self.l, self.m = int(Z), int(n), int(l), int(m)
+        self.alpha = float(Z)
+
+    @property
+    def beta(self):
+        return self.Z / self.n
+
+    def __repr__(self):
+        return f"HydrogenLike(Z={self.Z}, n={self.n}, l={self.l}, m={self.m})"
+
+
+def _fn_array(basis):
+    arr = (_lib.BasisFn * len(basis))()
+    for k, b in enumerate(basis):
+        arr[k].kind, arr[k].n, arr[k].l, arr[k].m, arr[k].alpha = b.kind, b.n, b.l, b.m, b.alpha
+    return arr
 
 
 def _sto_array(basis):
@@ -116,11 +164,13 @@ def even_tempered(shells, zeta0, ratio):
 
 
 def one_electron_matrices(basis, Z):
-    """Analytic overlap S and core Hamiltonian H (src/SlaterTypeOrbital.hpp:125-171)."""
+    """Overlap S and core Hamiltonian H of one kind of basis: STOs
+    (src/SlaterTypeOrbital.hpp:125-171), Ochi functions (src/OchiBasis.hpp:117-155)
+    or hydrogen-like functions (src/HydrogenLikeBasis.hpp:67-80)."""
     nb = len(basis)
     S = np.zeros((nb, nb))
     H = np.zeros((nb, nb))
-    check(lib().tcscf_sto_overlap_core(_sto_array(basis), nb, int(Z), S.ctypes.data_as(_dp), H.ctypes.data_as(_dp)))
+    check(lib().tcscf_basis_one_electron(_fn_array(basis), nb, int(Z), S.ctypes.data_as(_dp), H.ctypes.data_as(_dp)))
     return S, H
 
 
@@ -258,9 +308,9 @@ class QMCGrid:
         dev = self.quadratureGrid.points.device
         self.basisValues = torch.empty((n, nb), dtype=torch.float64, device=dev)
         self.basisGradients = torch.empty((n, nb, 3), dtype=torch.float64, device=dev) if storeGradients else None
-        check(lib().tcscf_basis_eval(_ptr(self.quadratureGrid.points), ctypes.c_int64(n), _sto_array(basisFunctions),
-                                     nb, _ptr(self.basisValues),
-                                     _ptr(self.basisGradients) if storeGradients else None, _stream()))
+        check(lib().tcscf_basis_eval_fns(_ptr(self.quadratureGrid.points), ctypes.c_int64(n), _fn_array(basisFunctions),
+                                         nb, _ptr(self.basisValues),
+                                         _ptr(self.basisGradients) if storeGradients else None, _stream()))
         self.basis = list(basisFunctions)
 
 

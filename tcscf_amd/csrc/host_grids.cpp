@@ -386,4 +386,44 @@ int tcscf_sto_overlap_core(const tcscf_sto* f, int nb, int Z, double* S, double*
   return 0;
 }
 
+int tcscf_basis_one_electron(const tcscf_basis_fn* f, int nb, int Z, double* S, double* H) {
+  TC_CHECK(nb > 0 && f && S && H, "tcscf_basis_one_electron: bad arguments");
+  const int kind = f[0].kind;
+  for (int a = 0; a < nb; ++a) {
+    TC_CHECK(f[a].kind == kind, "tcscf_basis_one_electron: all functions must be of one kind");
+    TC_CHECK(f[a].l >= std::abs(f[a].m) && f[a].l <= 3 && f[a].alpha > 0, "tcscf_basis_one_electron: invalid function");
+  }
+  if (kind == TCSCF_BASIS_STO) {
+    std::vector<tcscf_sto> sto(nb);
+    for (int a = 0; a < nb; ++a) sto[a] = tcscf_sto{f[a].n, f[a].l, f[a].m, f[a].alpha};
+    return tcscf_sto_overlap_core(sto.data(), nb, Z, S, H);
+  }
+  TC_CHECK(kind == TCSCF_BASIS_OCHI || kind == TCSCF_BASIS_HYDROGEN, "tcscf_basis_one_electron: unknown basis kind");
+  for (int a = 0; a < nb; ++a)
+    for (int b = 0; b < nb; ++b) {
+      const bool lm = f[a].l == f[b].l && f[a].m == f[b].m;
+      S[a * nb + b] = lm && f[a].n == f[b].n ? 1.0 : 0.0;  // orthonormal sets
+      H[a * nb + b] = 0;
+      if (!lm) continue;
+      if (kind == TCSCF_BASIS_HYDROGEN) {
+        TC_CHECK(f[a].n > f[a].l, "tcscf_basis_one_electron: hydrogen-like function needs n > l");
+        if (f[a].n == f[b].n) H[a * nb + b] = -double(Z) * Z / (2.0 * f[a].n * f[a].n);
+        continue;
+      }
+      TC_CHECK(f[a].alpha == f[b].alpha, "tcscf_basis_one_electron: Ochi functions of one alpha only");
+      TC_CHECK(f[a].n >= 0, "tcscf_basis_one_electron: Ochi function needs n >= 0");
+      const int l = f[a].l, n1 = f[a].n, n2 = f[b].n, mn = std::min(n1, n2), k = 2 * l + 2;
+      const double alpha = f[a].alpha;
+      // I = int fnl_1 fnl_2 / r dr with fnl = N r^(l+1) L_n^(k)(2 alpha r) e^(-alpha r); with
+      // x = 2 alpha r and int e^-x x^(k-1) L_m^(k) L_n^(k) dx = Gamma(k + 1 + min) / (k min!):
+      const double lnI = std::lgamma(k + 1.0 + mn) - std::lgamma(mn + 1.0) - std::log((double)k) -
+                         0.5 * (std::lgamma(n1 + k + 1.0) - std::lgamma(n1 + 1.0) + std::lgamma(n2 + k + 1.0) -
+                                std::lgamma(n2 + 1.0));
+      const double I = 2 * alpha * std::exp(lnI);
+      const double c = alpha * (l + 1) * (1 + 2.0 * mn / (2 * l + 3)) - Z;
+      H[a * nb + b] = c * I - (n1 == n2 ? alpha * alpha / 2 : 0.0);
+    }
+  return 0;
+}
+
 }  // extern "C"

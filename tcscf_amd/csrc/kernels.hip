@@ -87,10 +87,24 @@ constexpr int kMaxFnsPerLaunch = 96;
 
 struct StoTable {
   int nb;
-  int code[kMaxFnsPerLaunch];      // n | l << 8 | (m + 8) << 16
-  double zeta[kMaxFnsPerLaunch];
+  int code[kMaxFnsPerLaunch];      // n | l << 8 | (m + 8) << 16 | kind << 24
+  double zeta[kMaxFnsPerLaunch];   // exponent of e^{-zeta r} (alpha, or b = Z / n)
   double norm[kMaxFnsPerLaunch];
 };
+
+// Associated Laguerre polynomial L_n^(k)(x) by the three-term recurrence
+// (m + 1) L_{m+1} = (2m + 1 + k - x) L_m - (m + k) L_{m-1}; 0 for n < 0.
+__device__ inline double laguerre(int n, int k, double x) {
+  if (n < 0) return 0.0;
+  double lm1 = 1.0, l = 1.0 + k - x;
+  if (n == 0) return lm1;
+  for (int m = 1; m < n; ++m) {
+    const double lp = ((2 * m + 1 + k - x) * l - (m + k) * lm1) / (m + 1);
+    lm1 = l;
+    l = lp;
+  }
+  return l;
+}
 
 // Real solid harmonic S_lm(x, y, z) (homogeneous of degree l, unit-normalised
 // on the sphere) and its gradient.  Standard real harmonics without the
@@ -143,8 +157,12 @@ __device__ inline void solid_harmonic(int l, int m, double x, double y, double z
   }
 }
 
-// phi = N r^k e^{-zeta r} S_lm(x,y,z), k = n-1-l;
-// grad phi = N e^{-zeta r} [ (k r^{k-2} - zeta r^{k-1}) S (x,y,z) + r^k grad S ].
+// phi = g(r) S_lm(x,y,z), grad phi = g'(r) (x,y,z)/r S + g(r) grad S, with
+//   STO:       g = N r^k e^{-zeta r}, k = n-1-l
+//              (grad written as N e^{-zeta r} [ (k r^{k-2} - zeta r^{k-1}) S (x,y,z) + r^k grad S ]);
+//   Ochi:      g = N L_n^(2l+2)(2 zeta r) e^{-zeta r};
+//   hydrogen:  g = N L_(n-l-1)^(2l+1)(2 zeta r) e^{-zeta r}, zeta = Z / n;
+// and d/dx L_q^(k)(x) = -L_(q-1)^(k+1)(x).
 __global__ __launch_bounds__(256) void basis_eval_kernel(const double* __restrict__ pts, int64_t n, StoTable t,
                                                          int nb_total, int b0, double* __restrict__ values,
                                                          double* __restrict__ grads) {
@@ -155,22 +173,38 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(const double* __restric
   const double x = pts[k], y = pts[n + k], z = pts[2 * n + k];
   const double r = sqrt(x * x + y * y + z * z);
   const int code = t.code[b];
-  const int nq = code & 0xff, l = (code >> 8) & 0xff, m = ((code >> 16) & 0xff) - 8;
-  const int kp = nq - 1 - l;
+  const int nq = code & 0xff, l = (code >> 8) & 0xff, m = ((code >> 16) & 0xff) - 8, kind = code >> 24;
   const double zeta = t.zeta[b];
   double S, gS[3];
   solid_harmonic(l, m, x, y, z, &S, gS);
   const double e = t.norm[b] * exp(-zeta * r);
-  const double rk = kp == 0 ? 1.0 : pow(r, (double)kp);
   const int64_t o = k * nb_total + b0 + b;
-  values[o] = e * rk * S;
+  if (kind == TCSCF_BASIS_STO) {
+    const int kp = nq - 1 - l;
+    const double rk = kp == 0 ? 1.0 : pow(r, (double)kp);
+    values[o] = e * rk * S;
+    if (grads) {
+      const double rkm1 = kp == 0 ? 1.0 / r : pow(r, (double)(kp - 1));
+      const double rkm2 = kp == 0 ? 0.0 : pow(r, (double)(kp - 2));
+      const double radial = (kp * rkm2 - zeta * rkm1) * S;
+      grads[3 * o + 0] = e * (radial * x + rk * gS[0]);
+      grads[3 * o + 1] = e * (radial * y + rk * gS[1]);
+      grads[3 * o + 2] = e * (radial * z + rk * gS[2]);
+    }
+    return;
+  }
+  // Laguerre degree q and order kl of the radial polynomial
+  const int q = kind == TCSCF_BASIS_OCHI ? nq : nq - l - 1;
+  const int kl = kind == TCSCF_BASIS_OCHI ? 2 * l + 2 : 2 * l + 1;
+  const double xl = 2 * zeta * r;
+  const double L = laguerre(q, kl, xl);
+  values[o] = e * L * S;
   if (grads) {
-    const double rkm1 = kp == 0 ? 1.0 / r : pow(r, (double)(kp - 1));
-    const double rkm2 = kp == 0 ? 0.0 : pow(r, (double)(kp - 2));
-    const double radial = (kp * rkm2 - zeta * rkm1) * S;
-    grads[3 * o + 0] = e * (radial * x + rk * gS[0]);
-    grads[3 * o + 1] = e * (radial * y + rk * gS[1]);
-    grads[3 * o + 2] = e * (radial * z + rk * gS[2]);
+    const double dg = e * (-2 * zeta * laguerre(q - 1, kl + 1, xl) - zeta * L);  // g'(r)
+    const double radial = dg / r * S;
+    grads[3 * o + 0] = radial * x + e * L * gS[0];
+    grads[3 * o + 1] = radial * y + e * L * gS[1];
+    grads[3 * o + 2] = radial * z + e * L * gS[2];
   }
 }
 
@@ -911,19 +945,35 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
 
 extern "C" {
 
-int tcscf_basis_eval(const double* pts, int64_t n, const tcscf_sto* fns, int nb, double* values, double* grads,
-                     void* stream) {
+int tcscf_basis_eval_fns(const double* pts, int64_t n, const tcscf_basis_fn* fns, int nb, double* values,
+                         double* grads, void* stream) {
   TC_CHECK(pts && fns && values && n > 0 && nb > 0, "tcscf_basis_eval: bad arguments");
   for (int b0 = 0; b0 < nb; b0 += kMaxFnsPerLaunch) {
     StoTable t;
     t.nb = std::min(kMaxFnsPerLaunch, nb - b0);
     for (int b = 0; b < t.nb; ++b) {
-      const tcscf_sto& f = fns[b0 + b];
-      TC_CHECK(f.n >= 1 && f.n >= f.l && f.l >= std::abs(f.m) && f.l <= 3 && f.zeta > 0,
-               "tcscf_basis_eval: need n >= 1, n >= l, |m| <= l <= 3, zeta > 0");
-      t.code[b] = f.n | (f.l << 8) | ((f.m + 8) << 16);
-      t.zeta[b] = f.zeta;
-      t.norm[b] = std::pow(2 * f.zeta, f.n) * std::sqrt(2 * f.zeta / std::tgamma(2.0 * f.n + 1));
+      const tcscf_basis_fn& f = fns[b0 + b];
+      TC_CHECK(f.l >= std::abs(f.m) && f.l <= 3 && f.alpha > 0, "tcscf_basis_eval: need |m| <= l <= 3, alpha > 0");
+      double zeta = f.alpha, norm = 0;
+      if (f.kind == TCSCF_BASIS_STO) {
+        TC_CHECK(f.n >= 1 && f.n >= f.l, "tcscf_basis_eval: STO needs n >= 1, n >= l");
+        norm = std::pow(2 * f.alpha, f.n) * std::sqrt(2 * f.alpha / std::tgamma(2.0 * f.n + 1));
+      } else if (f.kind == TCSCF_BASIS_OCHI) {
+        TC_CHECK(f.n >= 0, "tcscf_basis_eval: Ochi function needs n >= 0");
+        // (2 alpha)^(l + 3/2) / sqrt((n + 2l + 2)! / n!)
+        norm = std::pow(2 * f.alpha, f.l + 1.5) / std::sqrt(std::exp(std::lgamma(f.n + 2.0 * f.l + 3) - std::lgamma(f.n + 1.0)));
+      } else if (f.kind == TCSCF_BASIS_HYDROGEN) {
+        TC_CHECK(f.n > f.l, "tcscf_basis_eval: hydrogen-like function needs n > l");
+        zeta = f.alpha / f.n;  // b = Z / n
+        // -(2b)^(l + 3/2) / sqrt(2 n (n + l)! / (n - l - 1)!)
+        norm = -std::pow(2 * zeta, f.l + 1.5) /
+               std::sqrt(2.0 * f.n * std::exp(std::lgamma(f.n + f.l + 1.0) - std::lgamma(f.n - f.l + 0.0)));
+      } else {
+        TC_CHECK(false, "tcscf_basis_eval: unknown basis kind");
+      }
+      t.code[b] = f.n | (f.l << 8) | ((f.m + 8) << 16) | (f.kind << 24);
+      t.zeta[b] = zeta;
+      t.norm[b] = norm;
     }
     const int64_t total = n * t.nb;
     hipLaunchKernelGGL(basis_eval_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
@@ -931,6 +981,14 @@ int tcscf_basis_eval(const double* pts, int64_t n, const tcscf_sto* fns, int nb,
     TC_HIP(hipGetLastError());
   }
   return 0;
+}
+
+int tcscf_basis_eval(const double* pts, int64_t n, const tcscf_sto* fns, int nb, double* values, double* grads,
+                     void* stream) {
+  TC_CHECK(fns && nb > 0, "tcscf_basis_eval: bad arguments");
+  std::vector<tcscf_basis_fn> f(nb);
+  for (int b = 0; b < nb; ++b) f[b] = tcscf_basis_fn{TCSCF_BASIS_STO, fns[b].n, fns[b].l, fns[b].m, fns[b].zeta};
+  return tcscf_basis_eval_fns(pts, n, f.data(), nb, values, grads, stream);
 }
 
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision) {

@@ -57,6 +57,22 @@ static int cpu_checks(const std::string& dir) {
   EXPECT(std::fabs(tcscf::overlap(a, a) - 1) < 1e-14);
   // <1s|h|1s> for one exponent zeta, nucleus Z: zeta^2/2 - Z zeta
   EXPECT(std::fabs(tcscf::coreMatrixElement(2, a, a) - (0.32 - 1.6)) < 1e-14);
+  // Ochi n = 0, l = 0 is the 1s STO of zeta = alpha; the set is orthonormal
+  const tcscf::OchiBasisFunction<double> o0(1.34, 0, 0, 0), o1(1.34, 1, 0, 0), p0(1.34, 0, 1, 0);
+  EXPECT(tcscf::overlap(o0, o0) == 1.0 && tcscf::overlap(o0, o1) == 0.0 && tcscf::overlap(o0, p0) == 0.0);
+  EXPECT(std::fabs(tcscf::coreMatrixElement(2, o0, o0) - (1.34 * 1.34 / 2 - 2 * 1.34)) < 1e-14);
+  EXPECT(tcscf::coreMatrixElement(2, o0, p0) == 0.0);
+  // hydrogen-like functions of the nucleus: -Z^2 / (2 n^2)
+  const tcscf::HydrogenLikeBasisFunction<double> h2p(3, 2, 1, 0), h3s(3, 3, 0, 0);
+  EXPECT(std::fabs(tcscf::coreMatrixElement(3, h2p, h2p) + 9.0 / 8) < 1e-15);
+  EXPECT(tcscf::overlap(h2p, h3s) == 0.0 && tcscf::coreMatrixElement(3, h2p, h3s) == 0.0);
+  threw = false;
+  try {
+    tcscf::HydrogenLikeBasisFunction<double> bad(2, 1, 1, 0);  // l >= n
+  } catch (const tcscf::Error&) {
+    threw = true;
+  }
+  EXPECT(threw);
   threw = false;
   try {
     SlaterTypeOrbital<double> bad(1.0, 1, 2, 0);  // n < l

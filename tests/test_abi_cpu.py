@@ -201,3 +201,53 @@ def test_batch_workload_generator_is_pinned(tc):
     assert sum(x["method"] == "tc" for x in drawn) == 32
     assert all(x["nb"] == len(configs.batch_basis(x)) and x["nb"] >= x["nup"] + 2 for x in drawn)
     assert all(0.5 <= x["b"] <= 2 for x in drawn if x["method"] == "tc")
+
+
+def _one_electron_numeric(O, fns, Z, pts, w):
+    """S_ab = sum w phi_a phi_b and H_ab = sum w (1/2 grad phi_a . grad phi_b
+    - Z phi_a phi_b / r) from the oracle's values and gradients."""
+    v, g = O.atomic_real(pts, fns, grads=True)
+    r = np.sqrt((pts ** 2).sum(0))
+    S = (v * w[:, None]).T @ v
+    T = 0.5 * np.einsum("n,nac,nbc->ab", w, g, g)
+    V = -Z * (v * (w / r)[:, None]).T @ v
+    return S, T + V
+
+
+def test_ochi_and_hydrogen_bases_one_electron(tc, O):
+    """Ochi functions (src/OchiBasis.hpp) and hydrogen-like functions
+    (src/HydrogenLikeBasis.hpp): the oracle's exact STO expansion against
+    scipy's Laguerre polynomials; orthonormality on a fine grid; the hydrogen-like
+    core matrix -Z^2/(2 n^2) as the expectation value of -1/2 lap - Z/r; and the
+    library's closed-form Ochi H (the reference's coefficient times the exact
+    Laguerre integral) against the numerically integrated <a| -1/2 lap - Z/r |b>."""
+    from scipy.special import eval_genlaguerre
+    pts, w = tc.product_grid(240, 17, tc.RADIAL_M4, 1.0, 1.0)
+    r = np.sqrt((pts ** 2).sum(0))
+    # s functions: the expansion against the closed form (Y_00 = 1/sqrt(4 pi))
+    for kind, n, alpha in ((1, 0, 1.3), (1, 3, 1.3), (1, 6, 0.9), (2, 1, 2.0), (2, 3, 2.0), (2, 4, 3.0)):
+        v = O.atomic_real(pts, [(kind, n, 0, 0, alpha)])[:, 0]
+        if kind == 1:
+            N = (2 * alpha) ** 1.5 / math.sqrt(math.factorial(n + 2) / math.factorial(n))
+            ref = N * eval_genlaguerre(n, 2, 2 * alpha * r) * np.exp(-alpha * r)
+        else:
+            b = alpha / n
+            N = -(2 * b) ** 1.5 / math.sqrt(2 * n * math.factorial(n) / math.factorial(n - 1))
+            ref = N * eval_genlaguerre(n - 1, 1, 2 * b * r) * np.exp(-b * r)
+        assert np.max(np.abs(v - ref / math.sqrt(4 * math.pi))) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    # Ochi: s and p functions of one alpha are orthonormal; H from the library vs quadrature
+    ochi = [tc.OchiBasisFunction(1.4, n, l, m) for n in range(4) for l in (0, 1) for m in range(-l, l + 1)]
+    fns = [(1, b.n, b.l, b.m, b.alpha) for b in ochi]
+    Sn, Hn = _one_electron_numeric(O, fns, 2, pts, w)
+    S, H = tc.one_electron_matrices(ochi, 2)
+    assert np.max(np.abs(Sn - np.eye(len(ochi)))) < 1e-9
+    assert np.array_equal(S, np.eye(len(ochi)))
+    assert np.max(np.abs(H - Hn)) < 1e-8
+    # hydrogen-like functions of the atom's Z: H = -Z^2 / (2 n^2) on the diagonal
+    hyd = [tc.HydrogenLikeBasisFunction(3, n, l, 0) for n in (1, 2, 3) for l in range(n) if l <= 1]
+    fns = [(2, b.n, b.l, b.m, b.alpha) for b in hyd]
+    Sn, Hn = _one_electron_numeric(O, fns, 3, pts, w)
+    S, H = tc.one_electron_matrices(hyd, 3)
+    assert np.max(np.abs(Sn - np.eye(len(hyd)))) < 1e-9
+    assert np.max(np.abs(Hn - H)) < 1e-8
+    assert np.allclose(np.diag(H), [-9 / (2 * b.n ** 2) for b in hyd], rtol=0, atol=1e-15)

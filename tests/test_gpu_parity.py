@@ -2,6 +2,7 @@
 same seeded inputs.  Tolerances are written per test; FP64 paths are held to
 ~1e-13 relative, SCF energies to 1e-10 Eh."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -705,3 +706,41 @@ def test_batched_scf_matches_single_solves(tc, gpu):
             assert r["energy"] == e, (d["kind"], r["energy"], e)
             assert r["iterations"] == hf.numberOfConvergenceLoops()
             assert np.array_equal(r["trace"], hf.energy_trace)
+
+
+def test_ochi_basis_on_device_and_reference_he_runs(tc, O, gpu):
+    """Ochi (src/OchiBasis.hpp) and hydrogen-like (src/HydrogenLikeBasis.hpp)
+    functions tabulated on the device against the oracle's STO expansion (values
+    and gradients); then He RCS with the orthonormal Ochi s basis n = 0..nBasis-1
+    at the exponents the reference's recorded runs ended at
+    (tests/golden/he_ochi_convergence.json, from its convergence.txt/data.txt).
+    As in those runs, r1 and r2 are distinct QMC lattice grids and the energy is
+    the mean over grids (here 8 pairs of 2x20011 / 2x10007 points).  The mean
+    must lie within 4 standard errors + 2e-6 Eh of the grid-free RHF energy of
+    the same basis (exact_energy) and within 2 combined standard deviations of
+    the reference's recorded energy."""
+    import json
+    g = _grid(tc, 30, 11, tc.RADIAL_M4)
+    mixed = [tc.OchiBasisFunction(1.2, n, l, m) for n in (0, 2, 5) for l in (0, 1, 2) for m in range(-l, l + 1)]
+    hyd = [tc.HydrogenLikeBasisFunction(4, n, l, m) for n in (1, 2, 3) for l in range(min(n, 3)) for m in (-l, 0, l)]
+    for basis in (mixed, hyd):
+        g.setBasisFunctions(basis, True)
+        v, gr = O.atomic_real(g.points_host, [(b.kind, b.n, b.l, b.m, b.alpha) for b in basis], grads=True)
+        assert rel_err(g.basisValues.cpu().numpy(), v) < 1e-12
+        assert rel_err(g.basisGradients.cpu().numpy(), gr) < 1e-12
+    data = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "he_ochi_convergence.json")))
+    grids = [(tc.QMCGrid.lattice(20000, 3, 2, s), tc.QMCGrid.lattice(10000, 3, 2, 1000 + s)) for s in range(8)]
+    for row in data["rows"]:
+        basis = [tc.OchiBasisFunction(row["alpha"], n, 0, 0) for n in range(row["nBasis"])]
+        S, H = tc.one_electron_matrices(basis, 2)
+        es = []
+        for g1, g2 in grids:
+            g1.setBasisFunctions(basis, False)
+            g2.setBasisFunctions(basis, False)
+            es.append(tc.RCSHartreeFock(2, len(basis)).compute(S, H, g1, tc.computeF(g1, g2), True, tol=1e-11,
+                                                                max_iter=200))
+        mean, sem = np.mean(es), np.std(es, ddof=1) / np.sqrt(len(es))
+        print(f"nBasis {row['nBasis']}: E = {mean:.9f} +/- {sem:.1e}, exact {row['exact_energy']:.9f}, "
+              f"reference {row['energy']:.9f} +/- {row['stddev']:.1e}")
+        assert abs(mean - row["exact_energy"]) < 4 * sem + 2e-6
+        assert abs(mean - row["energy"]) < 2 * np.hypot(row["stddev"], sem)

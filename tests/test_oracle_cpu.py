@@ -380,6 +380,24 @@ def test_he_single_zeta_closed_form(O):
     assert errs[1] < errs[0] < 0.03
 
 
+def test_he_ochi_fixture_exact_energies():
+    """The grid-free RHF energies of He in the Ochi s basis (tests/golden/
+    make_he_ochi.py) against the reference's recorded QMC-grid runs
+    (convergence.txt): each within 2 of that run's standard deviations; and
+    variational: decreasing with the basis size and above the HF limit."""
+    import json
+    import os
+    data = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "he_ochi_convergence.json")))
+    rows = data["rows"]
+    assert [r["nBasis"] for r in rows] == list(range(2, 11))
+    for r in rows:
+        assert abs(r["energy"] - r["exact_energy"]) < 2 * r["stddev"], r
+        assert r["exact_energy"] > data["hf_limit"] - 1e-7
+    e = [r["exact_energy"] for r in rows]
+    assert all(a > b for a, b in zip(e, e[1:]))
+    assert e[-1] - data["hf_limit"] < 1e-6
+
+
 def _he_config1(O):
     fns = [(1, 0, 0, 0.5 * 2 ** k) for k in range(5)]
     pts, w = O.product_grid(25, 11)
