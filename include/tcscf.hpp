@@ -32,6 +32,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <array>
 #include <vector>
 
 #include "tcscf_hip.h"
@@ -344,6 +345,50 @@ struct Jastrow {
   tcscf_jastrow abi() const { return tcscf_jastrow{a12, c_same, c_oppo}; }
 };
 
+/* jastrowFunctions::Ochi{a, a12, c, S} (src/jastrowFunctions.hpp:9-115): terms
+ * S[t] = (i, j, k) of (r12/(r12+a12))^i (r1/(r1+a))^j (r2/(r2+a))^k with
+ * coefficients c[t] = {c(t, false), c(t, true)} = {opposite spin, same spin}. */
+struct OchiJastrow {
+  double a, a12;
+  std::vector<std::array<double, 2>> c;
+  std::vector<std::array<int, 3>> S;
+
+  tcscf_ochi_jastrow abi() const {
+    if (c.size() != S.size() || S.size() > TCSCF_OCHI_MAX_TERMS)
+      throw Error("OchiJastrow: c and S need one entry per term, at most 8 terms");
+    tcscf_ochi_jastrow u{};
+    u.a = a;
+    u.a12 = a12;
+    u.nterms = static_cast<int>(S.size());
+    for (std::size_t t = 0; t < S.size(); ++t) {
+      u.i[t] = S[t][0];
+      u.j[t] = S[t][1];
+      u.k[t] = S[t][2];
+      u.c_oppo[t] = c[t][0];
+      u.c_same[t] = c[t][1];
+    }
+    return u;
+  }
+  /* The TC Fock build takes one V with V_same = c_same V and V_oppo = c_oppo V.
+   * With V = VjiSame that is {a12, 1, rho} when c_oppo = rho c_same term by term. */
+  Jastrow tcScales() const {
+    double rho = 0;
+    bool any = false;
+    for (const auto& ct : c) {
+      if (ct[1] == 0) {
+        if (ct[0] != 0) throw Error("OchiJastrow: TC needs c_oppo = rho * c_same term by term");
+        continue;
+      }
+      const double r = ct[0] / ct[1];
+      if (any && std::fabs(r - rho) > 1e-14 * std::fabs(rho)) throw Error("OchiJastrow: TC needs c_oppo = rho * c_same term by term");
+      rho = r;
+      any = true;
+    }
+    if (!any) throw Error("OchiJastrow: TC needs a nonzero same-spin coefficient");
+    return Jastrow{a12, 1.0, rho};
+  }
+};
+
 enum class Precision : int { FP64 = TCSCF_FP64, FP32 = TCSCF_FP32, BF16X3 = TCSCF_BF16X3, BF16X5 = TCSCF_BF16X5 };
 
 namespace detail {
@@ -371,6 +416,25 @@ DeviceArray<FT> contract(const integration::QMCGrid<REAL, N1>& r1, const integra
   hip(hipStreamSynchronize(r1.stream));
   return F;
 }
+
+template <class FT, class REAL, int N1, int N2>
+DeviceArray<FT> contractOchi(const integration::QMCGrid<REAL, N1>& r1, const integration::QMCGrid<REAL, N2>& r2,
+                             int kernel, const OchiJastrow& jastrow, bool sameSpin, Precision prec) {
+  if (r1.nBasis() != r2.nBasis()) throw Error("computeF: different bases");
+  const int nb = static_cast<int>(r2.nBasis());
+  if (nb == 0) throw Error("computeF: setBasisFunctions was not called on the inner grid");
+  const IndexType npair = static_cast<IndexType>(nb) * (nb + 1) / 2;
+  const int ncomp = kernel == TCSCF_K_JASTROW_GRAD ? 3 : 1;
+  DeviceArray<FT> F(static_cast<std::size_t>(r1.nGrid() * ncomp * npair));
+  const double scale = N2 == 2 ? 2 * 3.14159265358979323846 : 1.0;
+  const tcscf_ochi_jastrow u = jastrow.abi();
+  check(tcscf_compute_F_ochi(r1.quadratureGrid.points.data(), r1.nGrid(), r2.quadratureGrid.points.data(),
+                             r2.quadratureGrid.weights.data(), r2.radialOrder.data(), r2.nGrid(),
+                             r2.basisValues.data(), nb, kernel, &u, sameSpin ? 1 : 0, static_cast<int>(prec), scale,
+                             F.data(), dtype<FT>(), r1.stream));
+  hip(hipStreamSynchronize(r1.stream));
+  return F;
+}
 }  // namespace detail
 
 /* precomputeIntegrand + computeF (src/HartreeFock.hpp:23-180) for f = 1/r12:
@@ -388,6 +452,12 @@ DeviceArray<FT> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const int
                          const Jastrow& jastrow, bool sameSpin, Precision precision = Precision::FP64) {
   const tcscf_jastrow j = jastrow.abi();
   return detail::contract<FT>(r1Grid, r2Grid, TCSCF_K_TC_SCALAR, &j, sameSpin ? 1 : 0, precision);
+}
+/* The same for the general Ochi Jastrow (r12, r1 and r2 terms). */
+template <class FT = double, class REAL, int N1, int N2>
+DeviceArray<FT> computeF(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N2>& r2Grid,
+                         const OchiJastrow& jastrow, bool sameSpin, Precision precision = Precision::FP64) {
+  return detail::contractOchi<FT>(r1Grid, r2Grid, TCSCF_K_TC_SCALAR, jastrow, sameSpin, precision);
 }
 /* FjiSame and FjiOppo together: the TC scalar kernel is 2/r12 - 2 c^2 a12^2 / (r12 + a12)^4,
  * so both follow from the 1/r12 contraction (at precision) and the contraction of the smooth
@@ -418,6 +488,13 @@ DeviceArray<FT> computeV(const integration::QMCGrid<REAL, N1>& r1Grid, const int
                          const Jastrow& jastrow, Precision precision = Precision::FP64) {
   const tcscf_jastrow j = jastrow.abi();
   return detail::contract<FT>(r1Grid, r3Grid, TCSCF_K_JASTROW_GRAD, &j, 1, precision);
+}
+/* computeV for the general Ochi Jastrow: the reference's VjiSame (sameSpin) or
+ * VjiOppo, packed V[n1][xyz][p(j, i)]. */
+template <class FT = double, class REAL, int N1, int N3>
+DeviceArray<FT> computeV(const integration::QMCGrid<REAL, N1>& r1Grid, const integration::QMCGrid<REAL, N3>& r3Grid,
+                         const OchiJastrow& jastrow, bool sameSpin, Precision precision = Precision::FP64) {
+  return detail::contractOchi<FT>(r1Grid, r3Grid, TCSCF_K_JASTROW_GRAD, jastrow, sameSpin, precision);
 }
 
 /* integrateAllR1R2<false> (src/integration/integrateAll.hpp:221-317) from F:
@@ -591,6 +668,16 @@ class TCHartreeFock {
     energyTrace.resize(iter_ > 0 ? iter_ : 0);
     detail::check(rc);
     return res.energy;
+  }
+  /* With the general Ochi Jastrow: Vji is VjiSame, and VjiOppo = rho VjiSame
+   * (OchiJastrow::tcScales). */
+  template <int N1, class FT>
+  T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
+            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<FT>& FjiSame, const DeviceArray<FT>& FjiOppo,
+            const DeviceArray<FT>& VjiSame, const OchiJastrow& jastrow, bool respectOneElectronSymmetry,
+            double tol = 1e-5, int maxIter = 200, int mgsMode = 0, int diis = 0) {
+    return compute(overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo, VjiSame, jastrow.tcScales(),
+                   respectOneElectronSymmetry, tol, maxIter, mgsMode, diis);
   }
   int numberOfConvergenceLoops() const { return iter_; }
   T highestOccupiedOrbitalEnergy() const {

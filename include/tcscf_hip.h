@@ -91,6 +91,20 @@ typedef struct tcscf_jastrow {
   double c_oppo;
 } tcscf_jastrow;
 
+/* The reference's general Ochi Jastrow, jastrowFunctions::Ochi{a, a12, c, S}
+ * (src/jastrowFunctions.hpp:9-115):
+ *   u(r1, r2) = sum_t c_t (r12/(r12+a12))^i_t (r1/(r1+a))^j_t (r2/(r2+a))^k_t
+ * with S(t, :) = (i_t, j_t, k_t) and c(t, sameSpin) = c_same[t] or c_oppo[t]. */
+#define TCSCF_OCHI_MAX_TERMS 8
+typedef struct tcscf_ochi_jastrow {
+  double a;
+  double a12;
+  int nterms; /* 0 .. TCSCF_OCHI_MAX_TERMS */
+  int i[TCSCF_OCHI_MAX_TERMS], j[TCSCF_OCHI_MAX_TERMS], k[TCSCF_OCHI_MAX_TERMS]; /* 0 .. 16 */
+  double c_same[TCSCF_OCHI_MAX_TERMS];
+  double c_oppo[TCSCF_OCHI_MAX_TERMS];
+} tcscf_ochi_jastrow;
+
 typedef struct tcscf_scf_opts {
   double tol;     /* |E_new - E_old| < tol ends the loop (reference: 1e-5) */
   int max_iter;   /* reference: 50 RCS, 100 UOS, 200 TC */
@@ -194,6 +208,19 @@ int tcscf_compute_F(const double* pts1_dev, int64_t n1, const double* pts2_dev, 
                     const tcscf_jastrow* jastrow, int same_spin, int precision, double scale, void* F_dev,
                     int out_dtype, void* stream);
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
+
+/* tcscf_compute_F for the general Ochi Jastrow (the integrand lambdas of
+ * src/unitTests/HartreeFockTest.cpp:276-316 with any jastrowFunctions::Ochi):
+ *   kernel TCSCF_K_TC_SCALAR:   2/r12 - |grad1 u(r1,r2)|^2 - |grad1 u(r2,r1)|^2
+ *   kernel TCSCF_K_JASTROW_GRAD: grad1 u(r1,r2), 3 components
+ * with the coefficients of same_spin (the reference's VjiSame / VjiOppo, not
+ * the c = 1 tensor of the r12-only form).  Coincident pairs give 0; at r = 0
+ * the r1 / r2 terms of the gradient contribute 0.  Other arguments and layouts
+ * as tcscf_compute_F. */
+int tcscf_compute_F_ochi(const double* pts1_dev, int64_t n1, const double* pts2_dev, const double* w2_dev,
+                         const int32_t* ord2_dev, int64_t n2, const double* A2_dev, int nb, int kernel,
+                         const tcscf_ochi_jastrow* jastrow, int same_spin, int precision, double scale, void* F_dev,
+                         int out_dtype, void* stream);
 
 /* F_same and F_oppo of TCSCF_K_TC_SCALAR together (the two computeF calls of
  * src/unitTests/HartreeFockTest.cpp:276-296): the kernel is

@@ -6,7 +6,7 @@ QMCGrid / computeF / computeV / *HartreeFock call surface.
 """
 from ._lib import (BF16X3, BF16X5, FP32, FP64, K_COULOMB, K_JASTROW_GRAD, K_TC_SCALAR, RADIAL_BECKE, RADIAL_M4,
                    RADIAL_XOVER1MX, ConvergenceError, SolverError, TcscfError, lib)
-from .api import (HydrogenLikeBasisFunction, Jastrow, OchiBasisFunction, batch_scf, QMCGrid, QuadratureGrid, RCSHartreeFock,
+from .api import (HydrogenLikeBasisFunction, Jastrow, OchiBasisFunction, OchiJastrow, batch_scf, QMCGrid, QuadratureGrid, RCSHartreeFock,
                   SlaterTypeOrbital, TCHartreeFock, UOSHartreeFock,
                   coreMatrixElement, computeF, computeF_batch, computeF_tc_pair, computeV, eri_full, eri_full_gradient, even_tempered,
                   integrateAllR1R2, integrateAllR1R2Gradient, lebedev, loadSTO, npair, one_electron_matrices, overlap,
@@ -16,7 +16,7 @@ from .api import (HydrogenLikeBasisFunction, Jastrow, OchiBasisFunction, batch_s
 __all__ = [
     "BF16X3", "BF16X5", "FP32", "FP64", "K_COULOMB", "K_JASTROW_GRAD", "K_TC_SCALAR", "RADIAL_BECKE", "RADIAL_M4",
     "RADIAL_XOVER1MX", "ConvergenceError", "SolverError", "TcscfError", "lib", "HydrogenLikeBasisFunction", "Jastrow",
-    "OchiBasisFunction", "QMCGrid",
+    "OchiBasisFunction", "OchiJastrow", "QMCGrid",
     "QuadratureGrid", "batch_scf", "RCSHartreeFock", "SlaterTypeOrbital", "TCHartreeFock", "UOSHartreeFock", "coreMatrixElement",
     "computeF", "computeF_batch", "computeF_tc_pair", "computeV", "eri_full", "eri_full_gradient", "even_tempered", "integrateAllR1R2",
     "integrateAllR1R2Gradient", "lebedev", "loadSTO", "npair", "one_electron_matrices", "overlap", "pair_index", "product_grid",

@@ -22,7 +22,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16x5": BF16X5}
 EXPORTS = [
     "tcscf_last_error", "tcscf_version", "tcscf_radial_grid", "tcscf_lebedev_npts", "tcscf_lebedev",
     "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
-    "tcscf_F_workspace_bytes", "tcscf_compute_F_tc_pair", "tcscf_basis_eval_fns", "tcscf_basis_one_electron", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
+    "tcscf_F_workspace_bytes", "tcscf_compute_F_tc_pair", "tcscf_compute_F_ochi", "tcscf_basis_eval_fns", "tcscf_basis_one_electron", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
     "tcscf_qmc_lattice_npts", "tcscf_qmc_lattice_grid", "tcscf_sobol_points", "tcscf_sobol_grid", "tcscf_load_sto",
     "tcscf_batch_compute_F", "tcscf_batch_tuned_rows", "tcscf_batch_scf",
     "tcscf_selftest_mfma",
@@ -49,6 +49,16 @@ class Sto(ctypes.Structure):
 
 class Jastrow(ctypes.Structure):
     _fields_ = [("a12", ctypes.c_double), ("c_same", ctypes.c_double), ("c_oppo", ctypes.c_double)]
+
+
+OCHI_MAX_TERMS = 8
+
+
+class OchiJastrow(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_double), ("a12", ctypes.c_double), ("nterms", ctypes.c_int),
+                ("i", ctypes.c_int * OCHI_MAX_TERMS), ("j", ctypes.c_int * OCHI_MAX_TERMS),
+                ("k", ctypes.c_int * OCHI_MAX_TERMS), ("c_same", ctypes.c_double * OCHI_MAX_TERMS),
+                ("c_oppo", ctypes.c_double * OCHI_MAX_TERMS)]
 
 
 BASIS_STO, BASIS_OCHI, BASIS_HYDROGEN = 0, 1, 2

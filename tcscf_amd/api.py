@@ -334,6 +334,79 @@ class Jastrow:
     def _struct(self):
         return _lib.Jastrow(self.a12, self.c_same, self.c_oppo)
 
+    def _tc_struct(self):
+        return self._struct()
+
+
+class OchiJastrow:
+    """The reference's general Ochi Jastrow, jastrowFunctions::Ochi{a, a12, c, S}
+    (src/jastrowFunctions.hpp:9-115): u = sum_t c(t, sameSpin) (r12/(r12+a12))^i_t
+    (r1/(r1+a))^j_t (r2/(r2+a))^k_t with S = [(i, j, k), ...] and
+    c = [(c_oppo, c_same), ...] (the reference's c(t, false), c(t, true)); at
+    most 8 terms, exponents 0..16."""
+
+    def __init__(self, a, a12, c, S):
+        c = np.asarray(c, dtype=np.float64).reshape(-1, 2)
+        S = np.asarray(S, dtype=np.int64).reshape(-1, 3)
+        if len(c) != len(S) or len(S) > _lib.OCHI_MAX_TERMS:
+            raise ValueError("OchiJastrow: c and S need one row per term, at most 8 terms")
+        if a <= 0 or a12 <= 0 or (S < 0).any() or (S > 16).any():
+            raise ValueError("OchiJastrow: need a > 0, a12 > 0 and exponents in 0..16")
+        self.a, self.a12, self.c, self.S = float(a), float(a12), c, S
+
+    @classmethod
+    def from_jastrow(cls, jastrow, a=1.0):
+        """The r12-only Jastrow as an Ochi parameter set with S = [(1, 0, 0)]."""
+        return cls(a, jastrow.a12, [(jastrow.c_oppo, jastrow.c_same)], [(1, 0, 0)])
+
+    def _struct(self):
+        u = _lib.OchiJastrow()
+        u.a, u.a12, u.nterms = self.a, self.a12, len(self.S)
+        for t, ((i, j, k), (co, cs)) in enumerate(zip(self.S, self.c)):
+            u.i[t], u.j[t], u.k[t] = int(i), int(j), int(k)
+            u.c_same[t], u.c_oppo[t] = float(cs), float(co)
+        return u
+
+    def spin_ratio(self):
+        """rho with c_oppo = rho c_same for every term, or None if there is none
+        (a same-spin coefficient that is zero must have a zero partner)."""
+        cs, co = self.c[:, 1], self.c[:, 0]
+        nz = cs != 0
+        if not nz.any() or (co[~nz] != 0).any():
+            return None
+        rho = co[nz] / cs[nz]
+        return float(rho[0]) if np.allclose(rho, rho[0], rtol=1e-14, atol=0) else None
+
+    def _tc_struct(self):
+        """The TC Fock build takes one V and (c_same, c_oppo) with V_same =
+        c_same V and V_oppo = c_oppo V: for V = VjiSame that is (1, rho)."""
+        rho = self.spin_ratio()
+        if rho is None:
+            raise ValueError("TC with an Ochi Jastrow needs c_oppo = rho * c_same term by term (one V tensor); "
+                             "pass VjiSame as Vji")
+        return _lib.Jastrow(self.a12, 1.0, rho)
+
+
+def _compute_ochi(r1Grid, r2Grid, kernel, jastrow, same_spin, precision, out_dtype, ordered=True):
+    torch = _torch()
+    if r1Grid.nBasis() != r2Grid.nBasis() or r2Grid.basisValues is None:
+        raise ValueError("computeF: both grids need the same tabulated basis")
+    prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    nb = r2Grid.nBasis()
+    n1, n2 = r1Grid.nGrid(), r2Grid.nGrid()
+    dtype = torch.float64 if out_dtype in (torch.float64, "fp64", _lib.F64) else torch.float32
+    shape = (n1, npair(nb)) if kernel == _lib.K_TC_SCALAR else (n1, 3, npair(nb))
+    F = torch.empty(shape, dtype=dtype, device=r1Grid.quadratureGrid.points.device)
+    scale = 2 * math.pi if r2Grid.ndim == 2 else 1.0
+    u = jastrow._struct()
+    check(lib().tcscf_compute_F_ochi(_ptr(r1Grid.quadratureGrid.points), ctypes.c_int64(n1),
+                                     _ptr(r2Grid.quadratureGrid.points), _ptr(r2Grid.quadratureGrid.weights),
+                                     _ptr(r2Grid.radial_order) if ordered else None, ctypes.c_int64(n2),
+                                     _ptr(r2Grid.basisValues), nb, kernel, ctypes.byref(u), int(bool(same_spin)), prec,
+                                     ctypes.c_double(scale), _ptr(F), _lib.F32 if dtype == torch.float32 else _lib.F64,
+                                     _stream()))
+    return F
+
 
 def _compute(r1Grid, r2Grid, kernel, jastrow, same_spin, precision, out_dtype, ordered=True):
     torch = _torch()
@@ -361,8 +434,13 @@ def computeF(r1Grid, r2Grid, kernel="coulomb", jastrow=None, same_spin=True, pre
              out_dtype="fp64", ordered=True):
     """precomputeIntegrand + computeF fused (src/HartreeFock.hpp:23-131):
     packed F[n1, p(j,i)] = scale * sum_n2 K(r1, r2) w2 phi_j(r2) phi_i(r2).
-    kernel: 'coulomb' (1/r12) or 'tc' (2/r12 - |grad u12|^2 - |grad u21|^2)."""
+    kernel: 'coulomb' (1/r12) or 'tc' (2/r12 - |grad u12|^2 - |grad u21|^2,
+    with a Jastrow or an OchiJastrow)."""
     k = {"coulomb": _lib.K_COULOMB, "tc": _lib.K_TC_SCALAR}[kernel]
+    if isinstance(jastrow, OchiJastrow):
+        if k != _lib.K_TC_SCALAR:
+            raise ValueError("computeF: an OchiJastrow goes with kernel='tc'")
+        return _compute_ochi(r1Grid, r2Grid, k, jastrow, same_spin, precision, out_dtype, ordered)
     return _compute(r1Grid, r2Grid, k, jastrow, same_spin, precision, out_dtype, ordered)
 
 
@@ -374,6 +452,8 @@ def computeF_tc_pair(r1Grid, r2Grid, jastrow, precision="fp64", smooth_precision
     F_C (1/r12) at `precision`, G (the smooth part) at `smooth_precision`
     (default: `precision`).  Returns (FjiSame, FjiOppo)."""
     torch = _torch()
+    if not isinstance(jastrow, Jastrow):
+        raise TypeError("computeF_tc_pair: needs the r12-only Jastrow (its kernel splits into 1/r12 and a smooth part)")
     if r1Grid.nBasis() != r2Grid.nBasis() or r2Grid.basisValues is None:
         raise ValueError("computeF_tc_pair: both grids need the same tabulated basis")
     prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
@@ -396,9 +476,12 @@ def computeF_tc_pair(r1Grid, r2Grid, jastrow, precision="fp64", smooth_precision
     return Fs, Fo
 
 
-def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64", ordered=True):
-    """computeV (src/HartreeFock.hpp:185-229) for c = 1: packed V[n1, xyz, p].
-    V_same = c_same * V, V_oppo = c_oppo * V."""
+def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64", ordered=True, same_spin=True):
+    """computeV (src/HartreeFock.hpp:185-229): packed V[n1, xyz, p].  For the
+    r12-only Jastrow the c = 1 tensor (V_same = c_same V, V_oppo = c_oppo V);
+    for an OchiJastrow the reference's VjiSame (same_spin) or VjiOppo."""
+    if isinstance(jastrow, OchiJastrow):
+        return _compute_ochi(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, same_spin, precision, out_dtype, ordered)
     return _compute(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, True, precision, out_dtype, ordered)
 
 
@@ -705,7 +788,7 @@ class TCHartreeFock(_HFBase):
         trace = np.zeros(max_iter)
         D = np.zeros((2, nb, nb))
         ps = np.zeros(sum(self.nElectrons))
-        jp = jastrow._struct()
+        jp = jastrow._tc_struct()
         rc = lib().tcscf_tc_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                 _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(n1),
                                 _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), fdt, ctypes.byref(jp),
@@ -733,7 +816,7 @@ class TCHartreeFock(_HFBase):
         fdt = _tc_dtype(FjiSame, FjiOppo, Vji, r1Grid.nGrid())
         G2 = np.zeros((2, nb, nb))
         G3 = np.zeros((2, nb, nb))
-        jp = jastrow._struct()
+        jp = jastrow._tc_struct()
         check(lib().tcscf_tc_fock(self.nElectrons[0], self.nElectrons[1], nb, _ptr(r1Grid.basisValues),
                                   _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights),
                                   ctypes.c_int64(r1Grid.nGrid()), _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), fdt,

@@ -28,6 +28,7 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 constexpr double kFar = 1.0e6;  // padding coordinate: far from every grid point
+constexpr int kOchiMaxTerms = TCSCF_OCHI_MAX_TERMS;
 
 // Output layouts of the 16x16 MFMA accumulators: element i of lane l = 16 g + r
 // holds D[row][r] with row = d_row_*(g, i).  (Pinned by tcscf_selftest_mfma.)
@@ -341,6 +342,7 @@ __device__ inline T k_grad_scale(T s, T y, double a12) {
 
 struct KernCoulomb {
   static constexpr int ncomp = 1;
+  static constexpr bool absolute = false;
   template <class T>
   __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
     const T s = dist2(dx, dy, dz);
@@ -351,6 +353,7 @@ struct KernCoulomb {
 // 2/r12 - 2 c^2 a12^2 / (r12 + a12)^4  (r12-only Ochi term, c = c_same or c_oppo)
 struct KernTCScalar {
   static constexpr int ncomp = 1;
+  static constexpr bool absolute = false;
   double a12, c2a2;  // c2a2 = 2 c^2 a12^2
   template <class T>
   __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
@@ -364,6 +367,7 @@ struct KernTCScalar {
 // 2/r12 - 2 c^2 a12^2 / (r12 + a12)^4 (coincident pairs excluded).
 struct KernTCSmooth {
   static constexpr int ncomp = 1;
+  static constexpr bool absolute = false;
   double a12;
   template <class T>
   __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
@@ -373,9 +377,96 @@ struct KernTCSmooth {
   }
 };
 
+// The reference's general Ochi Jastrow (src/jastrowFunctions.hpp:9-115),
+//   u(r1, r2) = sum_t c_t (r12/(r12+a12))^i_t (r1/(r1+a))^j_t (r2/(r2+a))^k_t,
+// with the coefficients c_t of one spin case already selected.
+struct OchiTerms {
+  double a, a12;
+  int nt;
+  int i[kOchiMaxTerms], j[kOchiMaxTerms], k[kOchiMaxTerms];
+  double c[kOchiMaxTerms];
+};
+
+template <class T>
+__device__ inline T sqrt_any(T s);
+template <>
+__device__ inline double sqrt_any(double s) { return __builtin_sqrt(s); }
+template <>
+__device__ inline float sqrt_any(float s) { return __builtin_sqrtf(s); }
+
+template <class T>
+__device__ inline T ipow(T x, int n) {
+  T r = T(1);
+  for (int e = 0; e < n; ++e) r = r * x;
+  return r;
+}
+
+// grad_p u(p, q) = r12L (p - q) + rpL p (Ochi::gradient(p, q), jastrowFunctions.hpp:72-108),
+// given d = p - q, s = |d|^2 > 0 and r12 = |d|.  At p = 0 the r_p terms have no
+// defined direction; they contribute 0 there.
+template <class T>
+__device__ inline void ochi_gradient(const OchiTerms& u, T px, T py, T pz, T qx, T qy, T qz, T dx, T dy, T dz, T s,
+                                     T r12, T* g) {
+#pragma clang fp contract(off)
+  const T sp = dist2(px, py, pz), rp = sqrt_any(sp), rq = sqrt_any(dist2(qx, qy, qz));
+  const T pT = rp / (rp + T(u.a)), qT = rq / (rq + T(u.a)), tT = r12 / (r12 + T(u.a12));
+  const T pS = sp > T(0) ? T(u.a) / (sp * (rp + T(u.a))) : T(0);
+  const T tS = T(u.a12) / (s * (r12 + T(u.a12)));
+  T pL = T(0), tL = T(0);
+  for (int t = 0; t < u.nt; ++t) {
+    const T f = T(u.c[t]) * ipow(tT, u.i[t]) * ipow(pT, u.j[t]) * ipow(qT, u.k[t]);
+    pL = pL + f * T(u.j[t]) * pS;
+    tL = tL + f * T(u.i[t]) * tS;
+  }
+  g[0] = tL * dx + pL * px;
+  g[1] = tL * dy + pL * py;
+  g[2] = tL * dz + pL * pz;
+}
+
+// GRAD = false: 2/r12 - |grad_1 u(r1,r2)|^2 - |grad_1 u(r2,r1)|^2 (the TC scalar
+// integrand of src/unitTests/HartreeFockTest.cpp:276-296); GRAD = true:
+// grad_1 u(r1, r2) (:300-316).  Needs r1 and r2 themselves, not only r1 - r2.
+template <bool GRAD>
+struct KernOchi {
+  static constexpr int ncomp = GRAD ? 3 : 1;
+  static constexpr bool absolute = true;
+  OchiTerms u;
+  template <class T>
+  __device__ inline void eval(T ax, T ay, T az, T bx, T by, T bz, T* out) const {
+#pragma clang fp contract(off)
+    const T dx = ax - bx, dy = ay - by, dz = az - bz;
+    const T s = dist2(dx, dy, dz);
+    if (!(s > T(0))) {
+      for (int c = 0; c < ncomp; ++c) out[c] = T(0);
+      return;
+    }
+    const T r12 = sqrt_any(s);
+    T g[3];
+    ochi_gradient(u, ax, ay, az, bx, by, bz, dx, dy, dz, s, r12, g);
+    if constexpr (GRAD) {
+      out[0] = g[0];
+      out[1] = g[1];
+      out[2] = g[2];
+    } else {
+      T h[3];
+      ochi_gradient(u, bx, by, bz, ax, ay, az, T(0) - dx, T(0) - dy, T(0) - dz, s, r12, h);
+      out[0] = T(2) / r12 - (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) - (h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    }
+  }
+};
+
+// K at r1 = a, r2 = b for any kernel functor: functors marked `absolute` see
+// both points, the others only d = a - b.
+template <class Kern, class T>
+__device__ inline void kernel_eval(const Kern& kern, T ax, T ay, T az, T bx, T by, T bz, T* out) {
+  if constexpr (Kern::absolute) kern.eval(ax, ay, az, bx, by, bz, out);
+  else kern(ax - bx, ay - by, az - bz, out);
+}
+
 // grad_1 u(r1, r2) = c a12 / (r12 (r12 + a12)^2) (r1 - r2), with c = 1.
 struct KernJastrowGrad {
   static constexpr int ncomp = 3;
+  static constexpr bool absolute = false;
   double a12;
   template <class T>
   __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
@@ -434,7 +525,7 @@ __global__ __launch_bounds__(256) void fgemm_f64_kernel(const double* __restrict
     for (int n = 0; n < 4; ++n) b[n] = Pb[k * ldp + 16 * n];
     double a[MB][NC];
 #pragma unroll
-    for (int m = 0; m < MB; ++m) kern(ax[m] - bx, ay[m] - by, az[m] - bz, a[m]);
+    for (int m = 0; m < MB; ++m) kernel_eval(kern, ax[m], ay[m], az[m], bx, by, bz, a[m]);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -509,7 +600,7 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
       for (int n = 0; n < 4; ++n) b[n] = Pb[k * ldp + 16 * n];
       float a[MB][NC];
 #pragma unroll
-      for (int m = 0; m < MB; ++m) kern(ax[m] - q.x, ay[m] - q.y, az[m] - q.z, a[m]);
+      for (int m = 0; m < MB; ++m) kernel_eval(kern, ax[m], ay[m], az[m], q.x, q.y, q.z, a[m]);
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -627,8 +718,8 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
       for (int e = 0; e < EPT; e += 2) {
 #pragma clang fp contract(off)
         float v0[NC], v1[NC];
-        kern(ax - qn[e].x, ay - qn[e].y, az - qn[e].z, v0);
-        kern(ax - qn[e + 1].x, ay - qn[e + 1].y, az - qn[e + 1].z, v1);
+        kernel_eval(kern, ax, ay, az, qn[e].x, qn[e].y, qn[e].z, v0);
+        kernel_eval(kern, ax, ay, az, qn[e + 1].x, qn[e + 1].y, qn[e + 1].z, v1);
         bf2 h;
         h[0] = (__bf16)v0[c];
         h[1] = (__bf16)v1[c];
@@ -1034,6 +1125,35 @@ int tcscf_compute_F(const double* pts1, int64_t n1, const double* pts2, const do
   return TCSCF_EINVAL;
 }
 
+int tcscf_compute_F_ochi(const double* pts1, int64_t n1, const double* pts2, const double* w2, const int32_t* ord2,
+                         int64_t n2, const double* A2, int nb, int kernel, const tcscf_ochi_jastrow* u, int same_spin,
+                         int precision, double scale, void* F, int out_dtype, void* stream) {
+  TC_CHECK(pts1 && pts2 && w2 && A2 && F && n1 > 0 && n2 > 0 && nb > 0 && u, "tcscf_compute_F_ochi: bad arguments");
+  TC_CHECK(precision >= 0 && precision <= 3, "tcscf_compute_F_ochi: unknown precision");
+  TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F_ochi: unknown out_dtype");
+  TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F_ochi: grid too large");
+  TC_CHECK(u->a > 0 && u->a12 > 0, "tcscf_compute_F_ochi: need a > 0 and a12 > 0");
+  TC_CHECK(u->nterms >= 0 && u->nterms <= TCSCF_OCHI_MAX_TERMS, "tcscf_compute_F_ochi: 0 .. 8 terms");
+  OchiTerms t{};
+  t.a = u->a;
+  t.a12 = u->a12;
+  t.nt = u->nterms;
+  for (int q = 0; q < u->nterms; ++q) {
+    TC_CHECK(u->i[q] >= 0 && u->i[q] <= 16 && u->j[q] >= 0 && u->j[q] <= 16 && u->k[q] >= 0 && u->k[q] <= 16,
+             "tcscf_compute_F_ochi: exponents must lie in 0 .. 16");
+    t.i[q] = u->i[q];
+    t.j[q] = u->j[q];
+    t.k[q] = u->k[q];
+    t.c[q] = same_spin ? u->c_same[q] : u->c_oppo[q];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int of = out_dtype == TCSCF_F32;
+  if (kernel == TCSCF_K_TC_SCALAR)
+    return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernOchi<false>{t}, st);
+  TC_CHECK(kernel == TCSCF_K_JASTROW_GRAD, "tcscf_compute_F_ochi: kernel must be TCSCF_K_TC_SCALAR or TCSCF_K_JASTROW_GRAD");
+  return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernOchi<true>{t}, st);
+}
+
 // F_same and F_oppo of the TC scalar kernel from the Coulomb contraction F_C
 // (in Fs, at `precision`) and the contraction G of its smooth part (in Fo, at
 // `smooth_precision`): F_spin = 2 F_C - 2 c_spin^2 a12^2 G, in place.
@@ -1194,7 +1314,7 @@ __global__ __launch_bounds__(256) void fgemm_f64_vdot_kernel(const double* __res
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
       double v[3];
-      kern(ax[m] - bx, ay[m] - by, az[m] - bz, v);
+      kernel_eval(kern, ax[m], ay[m], az[m], bx, by, bz, v);
       a[m] = c == 0 ? v[0] : (c == 1 ? v[1] : v[2]);
     }
 #pragma unroll
@@ -1371,6 +1491,7 @@ namespace tc {
 // Any of the three kernels, one scalar component at a time (kind uniform per tile).
 struct KernAny {
   static constexpr int ncomp = 1;
+  static constexpr bool absolute = false;
   int kind, comp;
   double a12, c2a2;
   template <class T>

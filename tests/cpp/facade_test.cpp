@@ -66,6 +66,17 @@ static int cpu_checks(const std::string& dir) {
   const tcscf::HydrogenLikeBasisFunction<double> h2p(3, 2, 1, 0), h3s(3, 3, 0, 0);
   EXPECT(std::fabs(tcscf::coreMatrixElement(3, h2p, h2p) + 9.0 / 8) < 1e-15);
   EXPECT(tcscf::overlap(h2p, h3s) == 0.0 && tcscf::coreMatrixElement(3, h2p, h3s) == 0.0);
+  // OchiJastrow: the one-V TC form needs c_oppo = rho c_same term by term
+  const tcscf::OchiJastrow prop{1.0, 1.0, {{0.5, 0.25}, {-0.2, -0.1}}, {{1, 0, 0}, {2, 1, 0}}};
+  EXPECT(prop.tcScales().c_same == 1.0 && prop.tcScales().c_oppo == 2.0);
+  threw = false;
+  try {
+    const tcscf::OchiJastrow np{1.0, 1.0, {{0.5, 0.25}, {0.3, 0.1}}, {{1, 0, 0}, {2, 1, 0}}};
+    (void)np.tcScales();
+  } catch (const tcscf::Error&) {
+    threw = true;
+  }
+  EXPECT(threw);
   threw = false;
   try {
     tcscf::HydrogenLikeBasisFunction<double> bad(2, 1, 1, 0);  // l >= n
@@ -149,6 +160,13 @@ static int gpu_runs(const std::string& dir) {
     auto V32 = tcscf::computeV<float>(g1, g3, jas, tcscf::Precision::BF16X3);
     tcscf::TCHartreeFock<double> tcm(2, 2, static_cast<int>(basis.size()));
     std::printf("be_tc_mixed %.15f\n", tcm.compute(S, H, g1, Fs32, Fo32, V32, jas, true, 1e-10, 200, 1, 6));
+    // the same Jastrow as a general Ochi parameter set (S = {(1, 0, 0)}, c = {c_oppo, c_same})
+    const tcscf::OchiJastrow u{1.0, jas.a12, {{jas.c_oppo, jas.c_same}}, {{1, 0, 0}}};
+    auto Fsu = tcscf::computeF(g1, g1, u, true);
+    auto Fou = tcscf::computeF(g1, g1, u, false);
+    auto Vsu = tcscf::computeV(g1, g3, u, true);
+    tcscf::TCHartreeFock<double> tcu(2, 2, static_cast<int>(basis.size()));
+    std::printf("be_tc_ochi %.15f\n", tcu.compute(S, H, g1, Fsu, Fou, Vsu, u, true, 1e-10, 200, 1));
   }
   return 0;
 }

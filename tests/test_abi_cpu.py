@@ -251,3 +251,30 @@ def test_ochi_and_hydrogen_bases_one_electron(tc, O):
     assert np.max(np.abs(Sn - np.eye(len(hyd)))) < 1e-9
     assert np.max(np.abs(Hn - H)) < 1e-8
     assert np.allclose(np.diag(H), [-9 / (2 * b.n ** 2) for b in hyd], rtol=0, atol=1e-15)
+
+
+def test_ochi_jastrow_host_logic(tc):
+    """OchiJastrow (src/jastrowFunctions.hpp:9-115 parameters): the C struct it
+    passes, the one-V TC scales (c_oppo = rho c_same term by term, else an
+    error) and argument checks."""
+    u = tc.OchiJastrow(1.2, 0.9, [(0.5, 0.25), (-0.2, -0.1), (0.0, 0.0)], [(1, 0, 0), (2, 1, 0), (0, 3, 2)])
+    s = u._struct()
+    assert (s.a, s.a12, s.nterms) == (1.2, 0.9, 3)
+    assert list(s.i)[:3] == [1, 2, 0] and list(s.j)[:3] == [0, 1, 3] and list(s.k)[:3] == [0, 0, 2]
+    assert list(s.c_oppo)[:3] == [0.5, -0.2, 0.0] and list(s.c_same)[:3] == [0.25, -0.1, 0.0]
+    assert u.spin_ratio() == 2.0
+    t = u._tc_struct()
+    assert (t.a12, t.c_same, t.c_oppo) == (0.9, 1.0, 2.0)
+    for c in ([(0.5, 0.25), (0.3, 0.1)], [(0.5, 0.25), (0.1, 0.0)], [(0.5, 0.0), (0.1, 0.0)]):
+        v = tc.OchiJastrow(1.0, 1.0, c, [(1, 0, 0), (0, 1, 0)])
+        assert v.spin_ratio() is None
+        with pytest.raises(ValueError):
+            v._tc_struct()
+    j = tc.Jastrow.from_b(2.0)
+    w = tc.OchiJastrow.from_jastrow(j)
+    assert w.S.tolist() == [[1, 0, 0]] and w.c.tolist() == [[j.c_oppo, j.c_same]] and w.spin_ratio() == 2.0
+    for bad in (dict(a=0.0), dict(S=[(1, 0, 17)]), dict(c=[(1, 1)] * 9, S=[(1, 0, 0)] * 9)):
+        kw = dict(a=1.0, a12=1.0, c=[(1, 1)], S=[(1, 0, 0)])
+        kw.update(bad)
+        with pytest.raises(ValueError):
+            tc.OchiJastrow(**kw)

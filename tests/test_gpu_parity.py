@@ -356,6 +356,82 @@ def test_tc_scf_vs_oracle(tc, O, gpu, Z, nup, ndown, mgs_mode):
             tc.TCHartreeFock(nup, ndown, nb).compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-10, mgs_mode=0, diis=6)
 
 
+_OCHI_TERMS = [(1, 0, 0), (2, 1, 0), (0, 2, 1), (1, 1, 1)]
+
+
+def test_F_general_ochi_vs_oracle(tc, O, gpu):
+    """The general Ochi Jastrow (src/jastrowFunctions.hpp:9-115: r12, r1 and r2
+    powers, coefficients per spin that are not proportional) in the TC scalar
+    kernel and computeV, both spins, at FP64 against the oracle's general
+    evaluation (1e-12); the FP32 and BF16x5 contractions against FP64
+    (tolerances in the assertions); and S = [(1, 0, 0)] against the r12-only
+    kernels."""
+    basis = _mixed_basis()[:9]
+    g1 = _grid(tc, 11, 11, tc.RADIAL_M4)
+    g3 = _grid(tc, 9, 7, tc.RADIAL_M4)
+    for g in (g1, g3):
+        g.setBasisFunctions(basis, True)
+    c = [(0.5, 0.25), (-0.3, 0.2), (0.15, -0.4), (0.1, 0.05)]
+    u = tc.OchiJastrow(1.3, 0.8, c, _OCHI_TERMS)
+    prm = O.ochi_params(1.3, 0.8, [(*S, co, cs) for S, (co, cs) in zip(_OCHI_TERMS, c)])
+    for same in (True, False):
+        F = tc.computeF(g1, g1, kernel="tc", jastrow=u, same_spin=same)
+        ref = _F_oracle(O, g1, g1, kernel=O.K_TC_SCALAR, prm=prm, same=same)
+        assert rel_err(tc.unpack(F.cpu().numpy(), 9), ref) < 1e-12
+        V = tc.computeV(g1, g3, u, same_spin=same).cpu().numpy()
+        ref = _F_oracle(O, g1, g3, kernel=O.K_JASTROW_GRAD, prm=prm, same=same)
+        assert rel_err(tc.unpack(V, 9).transpose(0, 2, 3, 1), ref) < 1e-12
+        for prec, tol in (("fp32", 3e-6), ("bf16x5", 3e-6)):
+            M = tc.computeF(g1, g1, kernel="tc", jastrow=u, same_spin=same, precision=prec)
+            assert rel_err(M.cpu().numpy(), F.cpu().numpy()) < tol, prec
+    jas = tc.Jastrow.from_b(1.3)
+    u1 = tc.OchiJastrow.from_jastrow(jas)
+    for same in (True, False):
+        a = tc.computeF(g1, g1, kernel="tc", jastrow=u1, same_spin=same).cpu().numpy()
+        b = tc.computeF(g1, g1, kernel="tc", jastrow=jas, same_spin=same).cpu().numpy()
+        assert rel_err(a, b) < 1e-13
+    V1 = tc.computeV(g1, g3, u1, same_spin=True).cpu().numpy()
+    assert rel_err(V1, jas.c_same * tc.computeV(g1, g3, jas).cpu().numpy()) < 1e-13
+    with pytest.raises(TypeError):
+        tc.computeF_tc_pair(g1, g1, u)
+
+
+def test_tc_scf_general_ochi_vs_oracle(tc, O, gpu):
+    """TCHartreeFock with a general Ochi Jastrow whose opposite-spin
+    coefficients are twice the same-spin ones term by term (as in the
+    reference's r12-only driver, HartreeFockTest.cpp:351-360) against the
+    oracle fed VjiSame and VjiOppo separately; a Jastrow without that
+    proportionality is rejected by the device Fock build's one-V interface."""
+    basis = tc.even_tempered({0: 5, 1: 2}, 0.6, 2.2)
+    nb = len(basis)
+    g1 = _grid(tc, 14, 11, tc.RADIAL_M4)
+    g3 = _grid(tc, 10, 7, tc.RADIAL_M4)
+    for g in (g1, g3):
+        g.setBasisFunctions(basis, True)
+    cs = [0.25, -0.1, 0.05, 0.08]
+    u = tc.OchiJastrow(1.1, 1.0, [(2 * x, x) for x in cs], _OCHI_TERMS)
+    assert u.spin_ratio() == 2.0
+    Fs = tc.computeF(g1, g1, kernel="tc", jastrow=u, same_spin=True)
+    Fo = tc.computeF(g1, g1, kernel="tc", jastrow=u, same_spin=False)
+    Vs = tc.computeV(g1, g3, u, same_spin=True)
+    Vo = tc.computeV(g1, g3, u, same_spin=False)
+    assert rel_err(Vo.cpu().numpy(), 2 * Vs.cpu().numpy()) < 1e-14
+    S, H = tc.one_electron_matrices(basis, 4)
+    hf = tc.TCHartreeFock(2, 2, nb)
+    e = hf.compute(S, H, g1, Fs, Fo, Vs, u, True, tol=1e-10, max_iter=200, mgs_mode=1)
+    A = g1.basisValues.cpu().numpy()
+    dA = g1.basisGradients.cpu().numpy()
+    full = lambda V: np.ascontiguousarray(tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1))
+    ref = O.tc_scf(2, 2, S, H, A, dA, g1.weights_host, tc.unpack(Fs.cpu().numpy(), nb),
+                   tc.unpack(Fo.cpu().numpy(), nb), full(Vs), full(Vo), tol=1e-10, max_iter=200, mgs_mode=1)
+    assert ref["status"] == 0
+    assert abs(e - ref["energy"]) < 1e-9, (e, ref["energy"])
+    assert np.max(np.abs(hf.density - ref["D"])) < 1e-7
+    bad = tc.OchiJastrow(1.1, 1.0, [(0.5, 0.25), (0.3, 0.1)], _OCHI_TERMS[:2])
+    with pytest.raises(ValueError):
+        tc.TCHartreeFock(2, 2, nb).compute(S, H, g1, Fs, Fo, Vs, bad, True, tol=1e-10)
+
+
 def test_tc_zero_jastrow_equals_uos(tc, gpu):
     """c_same = c_oppo = 0 on the device path reproduces UOSHartreeFock."""
     basis = tc.even_tempered({0: 5, 1: 2}, 0.6, 2.2)
@@ -623,6 +699,8 @@ def test_cpp_facade_on_device(tc, O, gpu):
     etc = tc.TCHartreeFock(2, 2, len(bb)).compute(Sb, Hb, g1, Fs, Fo2, V, jas, True, tol=1e-10, mgs_mode=1)
     assert abs(out["be_tc"] - etc) < 1e-9
     assert abs(out["be_tc_mixed"] - etc) < 1e-6
+    # the general Ochi path with S = {(1, 0, 0)}: the same kernel up to rounding
+    assert abs(out["be_tc_ochi"] - etc) < 1e-9
 
 
 def test_batched_contraction_matches_single_calls(tc, gpu):
