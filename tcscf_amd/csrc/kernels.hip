@@ -850,7 +850,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
 }
 
 template <int MB, int NPROD, int FOLD, class Kern>
-__global__ __launch_bounds__(256, Kern::ncomp == 1 ? 2 : 1) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
+__global__ __launch_bounds__(256, (Kern::ncomp == 1 || NPROD == 3) ? 2 : 1) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
                                                          const float4* __restrict__ x2, int64_t n2p,
                                                          const us8* __restrict__ P0, int64_t plane,
                                                          int64_t ncb, void* __restrict__ Fout, int out_f32,
