@@ -319,25 +319,31 @@ __device__ inline T dist2(T dx, T dy, T dz) {
 #pragma clang fp contract(off)
   return dx * dx + dy * dy + dz * dz;
 }
+// a / b: exact division in FP64; in FP32 (the BF16 and FP32 contractions,
+// whose K is rounded to bf16 pieces or accumulated in FP32 anyway) the
+// hardware reciprocal (1 ulp) in place of the IEEE division sequence.
+__device__ inline double quot(double a, double b) { return a / b; }
+__device__ inline float quot(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 template <class T>
 __device__ inline T k_tc_scalar(T s, T y, double a12, double c2a2) {
 #pragma clang fp contract(off)
   const T t = s * y + T(a12);
   const T t2 = t * t;
-  return T(2) * y - T(c2a2) / (t2 * t2);
+  return T(2) * y - quot(T(c2a2), t2 * t2);
 }
 template <class T>
 __device__ inline T k_tc_smooth(T s, T y, double a12) {
 #pragma clang fp contract(off)
   const T t = s * y + T(a12);
   const T t2 = t * t;
-  return T(1) / (t2 * t2);
+  return quot(T(1), t2 * t2);
 }
 template <class T>
 __device__ inline T k_grad_scale(T s, T y, double a12) {
 #pragma clang fp contract(off)
   const T t = s * y + T(a12);
-  return T(a12) * y / (t * t);
+  return quot(T(a12) * y, t * t);
 }
 
 struct KernCoulomb {
