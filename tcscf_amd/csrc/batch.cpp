@@ -77,7 +77,9 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
                                tcscf_scf_result* results, int* graph_builds, void* stream) {
   using namespace tc;
   TC_CHECK(items && count > 0 && o && results && o->max_iter > 0 && o->diis >= 0, "tcscf_batch_scf: bad arguments");
-  (void)stream;
+  // The batch runs on non-blocking streams of its own, which do not wait for
+  // the caller's stream: its inputs (F, V, basis tables) must be complete first.
+  TC_HIP(hipStreamSynchronize((hipStream_t)stream));
   std::vector<Instance> inst(count);
   int rc = 0;
   auto cleanup = [&] {

@@ -276,6 +276,12 @@ int HfIter::step(double* G, int it, tcscf_scf_result* res, double* trace, bool* 
     dense::commutator_error(nspin, nb, Fock.data(), D.data(), S, err);
     diis.extrapolate(Fock, err);
   }
+  for (double f : Fock)
+    if (!std::isfinite(f)) {
+      res->status = TCSCF_ESOLVER;
+      set_error("SCF: non-finite Fock matrix (F input not finite?)");
+      return TCSCF_ESOLVER;
+    }
   for (int s = 0; s < nspin; ++s) {
     std::vector<double> ev;
     if (!dense::gen_sym_eig(nb, Fock.data() + s * nb2, S, ev, C)) {

@@ -827,6 +827,12 @@ int TcIter::step(double* G2, double* G3, int it, tcscf_scf_result* res, double* 
     dense::right_eigen_error(2, nb, Fock.data(), D.data(), S, err);
     diis.extrapolate(Fock, err);
   }
+  for (double f : Fock)
+    if (!std::isfinite(f)) {
+      res->status = TCSCF_ESOLVER;
+      set_error("TC SCF: non-finite Fock matrix (F / V inputs not finite?)");
+      return TCSCF_ESOLVER;
+    }
   for (int s = 0; s < 2; ++s) {
     std::vector<std::complex<double>> ev, X;
     if (!dense::gen_nonsym_eig(nb, Fock.data() + s * nb2, S, ev, X)) {
@@ -834,6 +840,13 @@ int TcIter::step(double* G2, double* G3, int it, tcscf_scf_result* res, double* 
       set_error("TC SCF: generalized eigen-solve failed");
       return TCSCF_ESOLVER;
     }
+    // the ordering below needs finite real parts (a strict weak order)
+    for (const auto& x : ev)
+      if (!std::isfinite(x.real())) {
+        res->status = TCSCF_ESOLVER;
+        set_error("TC SCF: non-finite eigenvalue");
+        return TCSCF_ESOLVER;
+      }
     std::vector<int> idx(nb);
     std::iota(idx.begin(), idx.end(), 0);
     std::sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a].real() < ev[b].real(); });

@@ -310,6 +310,20 @@ def test_scf_error_paths(tc, gpu):
     F2 = tc.computeF(g2, g2)
     with pytest.raises(tc.SolverError):
         tc.RCSHartreeFock(2, 6).compute(S2, H2, g2, F2, True)
+    # a non-finite F reaches the Fock matrix from the first density on: a
+    # solver error, for RCS and for TC (whose eigenvalue ordering needs finite
+    # values; nb = 19 takes the sort past its insertion-sort cutoff)
+    Fn = F.clone()
+    Fn[3, 2] = float("nan")
+    with pytest.raises(tc.SolverError):
+        tc.RCSHartreeFock(2, 5).compute(S, H, g, Fn, True, tol=1e-10)
+    basis = tc.even_tempered({0: 7, 1: 4}, 0.5, 2.0)
+    g1, g3, jas, Fs, Fo, V = _tc_case(tc, basis, (12, 11, tc.RADIAL_M4), (8, 7, tc.RADIAL_M4))
+    St, Ht = tc.one_electron_matrices(basis, 5)
+    Vn = V.clone()
+    Vn[5, 1, 7] = float("inf")
+    with pytest.raises(tc.SolverError):
+        tc.TCHartreeFock(3, 2, len(basis)).compute(St, Ht, g1, Fs, Fo, Vn, jas, True, tol=1e-10, mgs_mode=1)
 
 
 # ----------------------------------------------------------------- TC
