@@ -461,6 +461,41 @@ struct KernOchi {
   }
 };
 
+// The TC scalar kernel of both spins in one contraction: component 0 is
+// 1/r12 (5 products), component 1 the smooth 1/(r12 + a12)^4 (3 products),
+// and the epilogue writes F_spin = 2 F_C - c2a2_spin G (c2a2 = 2 c^2 a12^2)
+// for both spins from the FP64 sums.
+struct KernTCPair {
+  static constexpr int ncomp = 2;
+  static constexpr int ncomp5 = 1;  // components that take the five-product split
+  static constexpr int nout = 2;
+  static constexpr bool absolute = false;
+  double a12, c2a2_same, c2a2_oppo;
+  template <class T>
+  __device__ inline void operator()(T dx, T dy, T dz, T* out) const {
+    const T s = dist2(dx, dy, dz);
+    const T y = rsqrt_any(s);
+    out[0] = s > T(0) ? y : T(0);
+    out[1] = s > T(0) ? k_tc_smooth(s, y, a12) : T(0);
+  }
+  __device__ inline void finish(const double* v, double* o) const {
+#pragma clang fp contract(off)
+    o[0] = 2 * v[0] - c2a2_same * v[1];
+    o[1] = 2 * v[0] - c2a2_oppo * v[1];
+  }
+};
+
+// Components that take the five-product split when NPROD = 5 (default: all);
+// functors with `nout` combine their components in the epilogue.
+template <class K, class = void>
+struct Comp5 : std::integral_constant<int, K::ncomp> {};
+template <class K>
+struct Comp5<K, std::void_t<decltype(K::ncomp5)>> : std::integral_constant<int, K::ncomp5> {};
+template <class K, class = void>
+struct Combines : std::false_type {};
+template <class K>
+struct Combines<K, std::void_t<decltype(K::nout)>> : std::true_type {};
+
 // K at r1 = a, r2 = b for any kernel functor: functors marked `absolute` see
 // both points, the others only d = a - b.
 template <class Kern, class T>
@@ -671,8 +706,9 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
                                           const double* __restrict__ p1,
                                           int64_t n1, const float4* __restrict__ x2, int64_t n2p,
                                           const us8* __restrict__ P0, int64_t plane, int64_t ncb,
-                                          void* __restrict__ Fout, int out_f32, int64_t npair, int64_t ld_row,
-                                          double scale, Kern kern, int64_t row0, int64_t cbw) {
+                                          void* __restrict__ Fout, void* __restrict__ Fout2, int out_f32,
+                                          int64_t npair, int64_t ld_row, double scale, Kern kern, int64_t row0,
+                                          int64_t cbw) {
   constexpr int NC = Kern::ncomp;
   constexpr int NB = 4;                   // 16-column blocks per wave
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
@@ -818,9 +854,11 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
           v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pm, v, 0, 0, 0);
           v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, v, 0, 0, 0);
           if constexpr (NPROD == 5) {
-            const bf8 pl = __builtin_bit_cast(bf8, cur[NSPLIT - 1][n]);
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, v, 0, 0, 0);
-            v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, pm, v, 0, 0, 0);
+            if (c < Comp5<Kern>::value) {  // folded at compile time (c is unrolled)
+              const bf8 pl = __builtin_bit_cast(bf8, cur[NSPLIT - 1][n]);
+              v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, v, 0, 0, 0);
+              v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, pm, v, 0, 0, 0);
+            }
           }
           acc[c][m][n] = v;
         }
@@ -837,8 +875,9 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
     if (s + 1 < nsteps) step(s + 1, bqB, bqA, Fz{});
   }
   if (FOLD == 2 && (nsteps & 1)) fold_all();  // the last, unpaired step
-#pragma unroll
-  for (int c = 0; c < NC; ++c)
+  if constexpr (Combines<Kern>::value) {
+    // outputs o = 0, 1 of kern.finish to Fout / Fout2, row-major n1 x npair each
+    static_assert(Kern::nout == 2, "two outputs");
 #pragma unroll
     for (int m = 0; m < MB; ++m)
 #pragma unroll
@@ -847,25 +886,57 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
         for (int i = 0; i < 4; ++i) {
           const int64_t row = row0 + 16 * m + d_row_bf16(g, i), col = 16 * (cb0 + n) + r;
           if (row < n1 && col < npair) {
-            const int64_t o = row * ld_row + c * npair + col;
-            const double v = scale * sum[c][m][n][i];
-            if (out_f32) ((float*)Fout)[o] = (float)v;
-            else ((double*)Fout)[o] = v;
+            double v[NC], o[2];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] = scale * sum[c][m][n][i];
+            kern.finish(v, o);
+            const int64_t e = row * ld_row + col;
+            if (out_f32) {
+              ((float*)Fout)[e] = (float)o[0];
+              ((float*)Fout2)[e] = (float)o[1];
+            } else {
+              ((double*)Fout)[e] = o[0];
+              ((double*)Fout2)[e] = o[1];
+            }
           }
         }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int64_t row = row0 + 16 * m + d_row_bf16(g, i), col = 16 * (cb0 + n) + r;
+            if (row < n1 && col < npair) {
+              const int64_t o = row * ld_row + c * npair + col;
+              const double v = scale * sum[c][m][n][i];
+              if (out_f32) ((float*)Fout)[o] = (float)v;
+              else ((double*)Fout)[o] = v;
+            }
+          }
+  }
+}
+
+// Two workgroups per CU (a 256-VGPR budget) where the kernel fits it without
+// spills: one component, the three-product split, or a mixed split.
+template <class Kern, int NPROD>
+constexpr int bf16_min_blocks() {
+  return (Kern::ncomp == 1 || NPROD == 3 || Comp5<Kern>::value < Kern::ncomp) ? 2 : 1;
 }
 
 template <int MB, int NPROD, int FOLD, class Kern>
-__global__ __launch_bounds__(256, (Kern::ncomp == 1 || NPROD == 3) ? 2 : 1) void fgemm_bf16_kernel(const double* __restrict__ p1, int64_t n1,
-                                                         const float4* __restrict__ x2, int64_t n2p,
-                                                         const us8* __restrict__ P0, int64_t plane,
-                                                         int64_t ncb, void* __restrict__ Fout, int out_f32,
-                                                         int64_t npair, double scale, Kern kern) {
+__global__ __launch_bounds__(256, (bf16_min_blocks<Kern, NPROD>())) void fgemm_bf16_kernel(
+    const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
+    const us8* __restrict__ P0, int64_t plane, int64_t ncb, void* __restrict__ Fout, void* __restrict__ Fout2,
+    int out_f32, int64_t npair, double scale, Kern kern) {
   __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
   __shared__ float4 crd[2][32];
-  bf16_tile<MB, NPROD, FOLD, Kern>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, out_f32, npair,
-                                   Kern::ncomp * npair, scale, kern, (int64_t)blockIdx.y * (16 * MB),
-                                   (int64_t)blockIdx.x * 16);
+  bf16_tile<MB, NPROD, FOLD, Kern>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32, npair,
+                                   Combines<Kern>::value ? npair : Kern::ncomp * npair, scale, kern,
+                                   (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * 16);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -927,10 +998,10 @@ int launch_f32(const double* p1, int64_t n1, const float4* x2, int64_t n2p, cons
 
 template <int MB, int NPROD, int FOLD, class Kern>
 int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane, int64_t ncb,
-                void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+                void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st, void* F2) {
   dim3 grid((unsigned)(ncb / 16), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
   hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0, plane,
-                     ncb, F, out_f32, npair, scale, kern);
+                     ncb, F, F2, out_f32, npair, scale, kern);
   TC_HIP(hipGetLastError());
   return 0;
 }
@@ -954,23 +1025,26 @@ Bf16Cfg bf16_cfg(int ncomp) {
 template <int NPROD, int MB, class Kern>
 int launch_bf16_fold(Bf16Cfg c, const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0,
                      int64_t plane, int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern,
-                     hipStream_t st) {
+                     hipStream_t st, void* F2) {
   if (c.fold == 2)
-    return launch_bf16<MB, NPROD, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
-  return launch_bf16<MB, NPROD, 1>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+    return launch_bf16<MB, NPROD, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+  return launch_bf16<MB, NPROD, 1>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
 }
 
 template <int NPROD, class Kern>
 int launch_bf16_mb(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane,
-                   int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+                   int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st,
+                   void* F2 = nullptr) {
   const Bf16Cfg c = bf16_cfg(Kern::ncomp);
   if constexpr (Kern::ncomp == 1) {
     switch (c.mb) {
-      case 4: return launch_bf16_fold<NPROD, 4>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
-      default: return launch_bf16_fold<NPROD, 2>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      case 4:
+        return launch_bf16_fold<NPROD, 4>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+      default:
+        return launch_bf16_fold<NPROD, 2>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
     }
   } else {
-    return launch_bf16_fold<NPROD, 1>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+    return launch_bf16_fold<NPROD, 1>(c, p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
   }
 }
 
@@ -988,7 +1062,8 @@ __global__ __launch_bounds__(256) void tc_pair_combine(T* __restrict__ Fs, T* __
 
 template <class Kern>
 int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, const double* w2, const int* ord2,
-             int64_t n2, const double* A2, int nb, double scale, void* F, int out_f32, Kern kern, hipStream_t st) {
+             int64_t n2, const double* A2, int nb, double scale, void* F, int out_f32, Kern kern, hipStream_t st,
+             void* F2 = nullptr) {
   const int64_t npair = npair_of(nb);
   int rc = 0;
   if (precision == TCSCF_BF16X3 || precision == TCSCF_BF16X5) {
@@ -1012,9 +1087,12 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
     const us8* P0 = (const us8*)planes;
     const int64_t plane8 = plane / 8;
     if (precision == TCSCF_BF16X3)
-      return launch_bf16_mb<3>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st);
-    return launch_bf16_mb<5>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st);
+      return launch_bf16_mb<3>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st, F2);
+    return launch_bf16_mb<5>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st, F2);
   }
+  if constexpr (Combines<Kern>::value) {
+    TC_CHECK(false, "combined-output kernels run on the BF16 path only");
+  } else {
   const int64_t n2p = round_up(n2, 16), ldp = round_up(npair, 128);
   char* ws = (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
   if (rc) return rc;
@@ -1036,6 +1114,8 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
   TC_HIP(hipGetLastError());
   if constexpr (Kern::ncomp == 1) return launch_f32<4>(pts1, n1, x4, n2p, (const float*)ws, ldp, F, out_f32, npair, scale, kern, st);
   else return launch_f32<1>(pts1, n1, x4, n2p, (const float*)ws, ldp, F, out_f32, npair, scale, kern, st);
+  }
+  return TCSCF_EINVAL;
 }
 
 }  // namespace
@@ -1169,14 +1249,23 @@ int tcscf_compute_F_tc_pair(const double* pts1, int64_t n1, const double* pts2, 
                             void* stream) {
   TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F_tc_pair: jastrow parameters with a12 > 0 needed");
   TC_CHECK(Fs && Fo && Fs != Fo, "tcscf_compute_F_tc_pair: two distinct outputs needed");
+  const double a2 = jp->a12 * jp->a12;
+  const double cs = 2 * jp->c_same * jp->c_same * a2, co = 2 * jp->c_oppo * jp->c_oppo * a2;
+  if (precision == TCSCF_BF16X5 && smooth_precision == TCSCF_BF16X3) {
+    // one launch: 1/r12 at five products and the smooth part at three share
+    // the three-way split pair densities and r12; the epilogue combines
+    TC_CHECK(pts1 && pts2 && w2 && A2 && n1 > 0 && n2 > 0 && nb > 0, "tcscf_compute_F_tc_pair: bad arguments");
+    TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F_tc_pair: unknown out_dtype");
+    TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F_tc_pair: grid too large");
+    return dispatch(TCSCF_BF16X5, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, Fs, out_dtype == TCSCF_F32,
+                    KernTCPair{jp->a12, cs, co}, (hipStream_t)stream, Fo);
+  }
   int rc = tcscf_compute_F(pts1, n1, pts2, w2, ord2, n2, A2, nb, TCSCF_K_COULOMB, nullptr, 1, precision, scale, Fs,
                            out_dtype, stream);
   if (rc) return rc;
   if ((rc = tcscf_compute_F(pts1, n1, pts2, w2, ord2, n2, A2, nb, TCSCF_K_TC_SMOOTH, jp, 1, smooth_precision, scale,
                             Fo, out_dtype, stream)))
     return rc;
-  const double a2 = jp->a12 * jp->a12;
-  const double cs = 2 * jp->c_same * jp->c_same * a2, co = 2 * jp->c_oppo * jp->c_oppo * a2;
   const int64_t len = n1 * npair_of(nb);
   const unsigned grid = (unsigned)std::min<int64_t>((len + 255) / 256, 65536);
   hipStream_t st = (hipStream_t)stream;
@@ -1548,7 +1637,8 @@ __global__ __launch_bounds__(256, 2) void bf16_batch_kernel(const BatchDesc* __r
     local /= d.ncomp;
     const int64_t cb = local % d.cblocks, rb = local / d.cblocks;
     const KernAny kern{d.kind, comp, d.a12, d.c2a2};
-    bf16_tile<MB, NPROD, 1, KernAny>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, d.F + comp * d.npair, 0,
+    bf16_tile<MB, NPROD, 1, KernAny>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, d.F + comp * d.npair,
+                                     nullptr, 0,
                                      d.npair, d.ncomp * d.npair, d.scale, kern, rb * (16 * MB), cb * 16);
   }
 }
