@@ -156,7 +156,7 @@ __host__ __device__ inline int64_t staged_doubles(bool stage, int nb) {
 // spin densities and orbitals) forms spin 0 only, the spin-1 terms being the
 // same numbers.
 template <class ST, bool STAGE, int NS>
-__global__ __launch_bounds__(512, 1) void tc_point_kernel(PointArgs a) {
+__global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void tc_point_kernel(PointArgs a) {
   extern __shared__ double smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int nb = a.nb, nmax = a.nmax;
@@ -714,7 +714,9 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
         sizeof(double) * (point_wave_elems(nb, ns) + (p.f32 ? staged_doubles<float>(true, nb) : staged_doubles<double>(true, nb)));
     p.stage[r] = shared + 8 * staged_wave <= 160 * 1024 ? 1 : 0;
     const size_t per_wave = p.stage[r] ? staged_wave : sizeof(double) * point_wave_elems(nb, ns);
-    p.wpw[r] = p.stage[r] ? 8 : 4;
+    // staged: 8 waves per workgroup, 12 for the one-spin kernel on FP32 rows
+    // (three waves per SIMD fit its registers) when LDS holds them
+    p.wpw[r] = p.stage[r] ? (ns == 1 && p.f32 && shared + 12 * per_wave <= 160 * 1024 ? 12 : 8) : 4;
     while (p.wpw[r] > 1 && shared + p.wpw[r] * per_wave > 160 * 1024) p.wpw[r] /= 2;
     p.sh1[r] = shared + p.wpw[r] * per_wave;
     TC_CHECK(p.sh1[r] <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
