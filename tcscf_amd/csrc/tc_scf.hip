@@ -583,7 +583,8 @@ __global__ __launch_bounds__(256) void tc_vlin_kernel(const ST* __restrict__ V, 
   const int64_t p1 = p0 + per < n1 ? p0 + per : n1;
   for (int64_t p = threadIdx.x; p < npair; p += 256) {
     double acc[NK] = {};
-    for (int64_t n = p0; n < p1; ++n) {
+#pragma unroll 4
+    for (int64_t n = p0; n < p1; ++n) {  // unrolled: several points' loads in flight
       const double v0 = V[(n * 3) * npair + p], v1 = V[(n * 3 + 1) * npair + p], v2 = V[(n * 3 + 2) * npair + p];
       const double* h = hvec + n * 12;
 #pragma unroll
