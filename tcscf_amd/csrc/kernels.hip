@@ -840,29 +840,37 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
     else if (s + 1 < nsteps)
       load_b(nxt, s + 1);
     const f4 zero = {0, 0, 0, 0};
+    // Product-major: every accumulator of a component takes one product before
+    // any takes the next, so consecutive MFMAs are independent and each
+    // accumulator's next product issues MB * NB MFMAs later (a chain per
+    // accumulator stalls on its own result).  The per-element order of the
+    // products (hh, hm, lh, hl, lm) is unchanged.
+    auto mfma_all = [&](int c, const bf8 (&a)[MB], int sp, bool first) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], __builtin_bit_cast(bf8, cur[sp][n]),
+                                                                 first ? zero : acc[c][m][n], 0, 0, 0);
+    };
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      bf8 ah[MB], al[MB];
 #pragma unroll
       for (int m = 0; m < MB; ++m) {
-        const bf8 ah = __builtin_bit_cast(bf8, ktile[buf][c][m][0][lane]);
-        const bf8 al = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
-#pragma unroll
-        for (int n = 0; n < NB; ++n) {
-          const bf8 ph = __builtin_bit_cast(bf8, cur[0][n]);
-          const bf8 pm = __builtin_bit_cast(bf8, cur[1][n]);
-          f4 v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, FRESH ? zero : acc[c][m][n], 0, 0, 0);
-          v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pm, v, 0, 0, 0);
-          v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, v, 0, 0, 0);
-          if constexpr (NPROD == 5) {
-            if (c < Comp5<Kern>::value) {  // folded at compile time (c is unrolled)
-              const bf8 pl = __builtin_bit_cast(bf8, cur[NSPLIT - 1][n]);
-              v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, v, 0, 0, 0);
-              v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, pm, v, 0, 0, 0);
-            }
-          }
-          acc[c][m][n] = v;
+        ah[m] = __builtin_bit_cast(bf8, ktile[buf][c][m][0][lane]);
+        al[m] = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
+      }
+      mfma_all(c, ah, 0, FRESH);
+      mfma_all(c, ah, 1, false);
+      mfma_all(c, al, 0, false);
+      if constexpr (NPROD == 5) {
+        if (c < Comp5<Kern>::value) {  // folded at compile time (c is unrolled)
+          mfma_all(c, ah, NSPLIT - 1, false);
+          mfma_all(c, al, 1, false);
         }
       }
+    }
     if (ONEBLOCK || s + 1 < nsteps) generate(buf ^ 1);
     if constexpr (!FRESH || FOLD == 1) fold_all();
     coords_landed();
