@@ -249,7 +249,8 @@ typedef struct tcscf_batch_item {
   tcscf_jastrow jastrow;
   int same_spin;
   double scale;
-  double* F_dev;
+  void* F_dev;   /* output, tcscf_compute_F layout, of out_dtype */
+  int out_dtype; /* TCSCF_F64 or TCSCF_F32 (FP32 storage of the mixed-precision path) */
 } tcscf_batch_item;
 
 /* No single reference counterpart: computeF / computeV (src/HartreeFock.hpp:

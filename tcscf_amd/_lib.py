@@ -73,7 +73,8 @@ class BatchItem(ctypes.Structure):
     _fields_ = [("pts1_dev", ctypes.c_void_p), ("n1", ctypes.c_int64), ("pts2_dev", ctypes.c_void_p),
                 ("w2_dev", ctypes.c_void_p), ("ord2_dev", ctypes.c_void_p), ("n2", ctypes.c_int64),
                 ("A2_dev", ctypes.c_void_p), ("nb", ctypes.c_int), ("kernel", ctypes.c_int), ("jastrow", Jastrow),
-                ("same_spin", ctypes.c_int), ("scale", ctypes.c_double), ("F_dev", ctypes.c_void_p)]
+                ("same_spin", ctypes.c_int), ("scale", ctypes.c_double), ("F_dev", ctypes.c_void_p),
+                ("out_dtype", ctypes.c_int)]
 
 
 class ScfOpts(ctypes.Structure):

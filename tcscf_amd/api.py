@@ -495,14 +495,15 @@ def _tc_dtype(Fs, Fo, V, n1):
     return _lib.F32 if Fs.dtype == torch.float32 else _lib.F64
 
 
-def computeF_batch(specs, precision="bf16x5"):
+def computeF_batch(specs, precision="bf16x5", out_dtype="fp64"):
     """computeF / computeV for many instances in one call (BF16: one persistent
     launch over a work queue of tiles of every instance, results equal to the
     single calls bit for bit).  specs: list of dicts with keys r1, r2, kernel
     ('coulomb', 'tc' or 'grad'), jastrow (TC kernels) and same_spin.  Returns
-    the packed FP64 tensors in spec order."""
+    the packed tensors (out_dtype 'fp64' or 'fp32' storage) in spec order."""
     torch = _torch()
     prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    f32 = out_dtype in (torch.float32, "fp32", _lib.F32)
     kinds = {"coulomb": _lib.K_COULOMB, "tc": _lib.K_TC_SCALAR, "grad": _lib.K_JASTROW_GRAD}
     items = (_lib.BatchItem * len(specs))()
     outs = []
@@ -513,7 +514,7 @@ def computeF_batch(specs, precision="bf16x5"):
             raise ValueError("computeF_batch: both grids need the same tabulated basis")
         nb, n1 = r2.nBasis(), r1.nGrid()
         shape = (n1, npair(nb)) if kind != _lib.K_JASTROW_GRAD else (n1, 3, npair(nb))
-        F = torch.empty(shape, dtype=torch.float64, device=r1.quadratureGrid.points.device)
+        F = torch.empty(shape, dtype=torch.float32 if f32 else torch.float64, device=r1.quadratureGrid.points.device)
         outs.append(F)
         it = items[k]
         it.pts1_dev, it.n1 = r1.quadratureGrid.points.data_ptr(), n1
@@ -526,6 +527,7 @@ def computeF_batch(specs, precision="bf16x5"):
         it.same_spin = int(bool(sp.get("same_spin", True)))
         it.scale = 2 * math.pi if r2.ndim == 2 else 1.0
         it.F_dev = F.data_ptr()
+        it.out_dtype = _lib.F32 if f32 else _lib.F64
     check(lib().tcscf_batch_compute_F(items, len(specs), prec, _stream()))
     return outs
 

@@ -45,11 +45,15 @@ def pair_units(prepared):
     return total
 
 
-def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None):
-    """Precompute (one batched call) + batched SCF at one precision.  Returns
-    (results, timings dict)."""
+def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None, out_dtype=None):
+    """Precompute (one batched call) + batched SCF at one precision.  F / V
+    storage: FP32 for the BF16 contractions (the Fock builds keep FP64
+    arithmetic), FP64 otherwise, unless out_dtype says.  Returns (results,
+    timings dict)."""
+    if out_dtype is None:
+        out_dtype = "fp32" if precision in ("bf16x3", "bf16x5") else "fp64"
     t0 = time.perf_counter()
-    tensors = api.computeF_batch(contraction_specs(prepared), precision=precision)
+    tensors = api.computeF_batch(contraction_specs(prepared), precision=precision, out_dtype=out_dtype)
     if synchronize:
         synchronize()
     t1 = time.perf_counter()

@@ -1612,11 +1612,11 @@ struct BatchDesc {
   const double* p1;
   const float4* x2;
   const us8* P0;
-  double* F;
+  void* F;  // FP64, or FP32 when out_f32
   int64_t n1, n2p, plane, ncb, npair;
   int64_t rblocks, cblocks, tiles;  // row blocks (16 MB rows), 256-column blocks, all tiles
   double scale, a12, c2a2;
-  int kind, ncomp;
+  int kind, ncomp, out_f32;
 };
 
 template <int MB, int NPROD>
@@ -1645,8 +1645,8 @@ __global__ __launch_bounds__(256, 2) void bf16_batch_kernel(const BatchDesc* __r
     local /= d.ncomp;
     const int64_t cb = local % d.cblocks, rb = local / d.cblocks;
     const KernAny kern{d.kind, comp, d.a12, d.c2a2};
-    bf16_tile<MB, NPROD, 1, KernAny>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, d.F + comp * d.npair,
-                                     nullptr, 0,
+    void* Fc = d.out_f32 ? (void*)((float*)d.F + comp * d.npair) : (void*)((double*)d.F + comp * d.npair);
+    bf16_tile<MB, NPROD, 1, KernAny>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, Fc, nullptr, d.out_f32,
                                      d.npair, d.ncomp * d.npair, d.scale, kern, rb * (16 * MB), cb * 16);
   }
 }
@@ -1664,13 +1664,16 @@ std::map<int, int> g_tuned_mb;
 extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, int precision, void* stream) {
   using namespace tc;
   TC_CHECK(items && count > 0, "tcscf_batch_compute_F: bad arguments");
+  for (int k = 0; k < count; ++k)
+    TC_CHECK(items[k].out_dtype == TCSCF_F64 || items[k].out_dtype == TCSCF_F32,
+             "tcscf_batch_compute_F: item " + std::to_string(k) + ": unknown out_dtype");
   hipStream_t st = (hipStream_t)stream;
   if (precision != TCSCF_BF16X3 && precision != TCSCF_BF16X5) {  // FP64 / FP32: one launch per instance
     for (int k = 0; k < count; ++k) {
       const tcscf_batch_item& it = items[k];
       const int rc = tcscf_compute_F(it.pts1_dev, it.n1, it.pts2_dev, it.w2_dev, it.ord2_dev, it.n2, it.A2_dev, it.nb,
-                                     it.kernel, &it.jastrow, it.same_spin, precision, it.scale, it.F_dev, TCSCF_F64,
-                                     stream);
+                                     it.kernel, &it.jastrow, it.same_spin, precision, it.scale, it.F_dev,
+                                     it.out_dtype, stream);
       if (rc) return rc;
     }
     return 0;
@@ -1716,6 +1719,7 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
     d.x2 = x4;
     d.P0 = (const us8*)planes;
     d.F = it.F_dev;
+    d.out_f32 = it.out_dtype == TCSCF_F32;
     d.n1 = it.n1;
     d.n2p = n2p;
     d.plane = plane / 8;

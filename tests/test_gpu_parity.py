@@ -751,6 +751,16 @@ def test_batched_contraction_matches_single_calls(tc, gpu):
                                   same_spin=sp.get("same_spin", True), precision=prec)
             assert torch.equal(F, ref), (prec, sp["kernel"])
         assert tc.lib().tcscf_batch_tuned_rows(tc.BF16X5 if prec == "bf16x5" else tc.BF16X3) in (16, 32)
+    # FP32 storage: the same rounding of the same sums as the single calls
+    outs = tc.computeF_batch(specs, precision="bf16x5", out_dtype="fp32")
+    for sp, F in zip(specs, outs):
+        assert F.dtype == torch.float32
+        if sp["kernel"] == "grad":
+            ref = tc.computeV(sp["r1"], sp["r2"], sp["jastrow"], precision="bf16x5", out_dtype="fp32")
+        else:
+            ref = tc.computeF(sp["r1"], sp["r2"], kernel=sp["kernel"], jastrow=sp.get("jastrow"),
+                              same_spin=sp.get("same_spin", True), precision="bf16x5", out_dtype="fp32")
+        assert torch.equal(F, ref), ("fp32", sp["kernel"])
     # FP64 goes instance by instance and equals the single calls too
     outs = tc.computeF_batch(specs[:2], precision="fp64")
     assert torch.equal(outs[0], tc.computeF(specs[0]["r1"], specs[0]["r2"]))
