@@ -150,6 +150,15 @@ int tcscf_product_grid(int nrad, int map, double a, double eps, int order, doubl
 int64_t tcscf_qmc_lattice_npts(int64_t min_points, int nshifts);
 int64_t tcscf_qmc_lattice_grid(int64_t min_points, int ndim, int nshifts, uint64_t seed, double* pts_host,
                                double* w_host);
+/* The point sets of the reference's per-element QMC integrals,
+ * sphericalCoordinates{3,5,6}DIntegral (src/integration/qmcWrapper.hpp:146-276):
+ * a randomly shifted Korobov lattice in [0,1)^ndim mapped to r = x/(1-x),
+ * theta = pi x, phi = 2 pi x per electron (ndim 5: phi2 = 0), weights with the
+ * Jacobians, the prefactor (2 pi^2; 4 pi^4 for ndim 5, 6) and 1/N.  pts1 (and
+ * pts2 for ndim 5, 6): 3 x N Cartesian SoA on the host; N = nshifts * (first
+ * prime >= min_points), returned. */
+int64_t tcscf_qmc_spherical_points(int64_t min_points, int ndim, int nshifts, uint64_t seed, double* pts1_host,
+                                   double* pts2_host, double* w_host);
 /* No reference counterpart (the QMC workload's point set): Sobol points in
  * [0, 1)^dim (dim <= 3, Joe-Kuo directions, Gray-code order), optionally
  * scrambled (linear matrix scrambling + digital shift, seeded); u n x dim. */
@@ -272,6 +281,15 @@ int tcscf_batch_tuned_rows(int precision);
  * npair x npair.  FP64 MFMA GEMM split over r1. */
 int tcscf_eri(const double* A1_dev, const double* w1_dev, int64_t n1, int nb, const void* F_dev, double* eri_dev,
               void* stream);
+
+/* fillR12Array (src/AtomicBasis.hpp:167-177) with r12MatrixElement (:30-48),
+ * the per-element 6-D QMC route, in the real basis: for the point pairs
+ * (r1_k, r2_k) and weights w_k of tcscf_qmc_spherical_points(ndim = 6),
+ *   eri[p(a,c)][p(b,d)] = sum_k w_k phi_a(r1_k) phi_c(r1_k) phi_b(r2_k) phi_d(r2_k) / r12_k
+ * (coincident pairs give 0), the layout of tcscf_eri.  A1_dev / A2_dev: the basis
+ * values at the r1 / r2 points (n x nb). */
+int tcscf_eri_qmc6d(const double* pts1_dev, const double* pts2_dev, const double* w_dev, int64_t n,
+                    const double* A1_dev, const double* A2_dev, int nb, double* eri_dev, void* stream);
 
 /* integrateAllR1R2<true> (src/integration/integrateAll.hpp:184-215, 221-317),
  * real basis, for the vector kernel f = c_same grad_1 u(r1, r2) (kernel must be

@@ -144,6 +144,17 @@ def atomic_as_stos(fn):
     return out
 
 
+def eri_qmc6d(pts1, pts2, w, A1, A2):
+    """fillR12Array with r12MatrixElement (src/AtomicBasis.hpp:30-48, 167-177)
+    on given point pairs (r1_k, r2_k) with weights w_k (the 6-D QMC route of
+    src/integration/qmcWrapper.hpp:249-276), real basis:
+    answer[a, b, c, d] = sum_k w_k A1[k, a] A2[k, b] A1[k, c] A2[k, d] / r12_k,
+    coincident pairs excluded."""
+    r12 = np.sqrt(((np.asarray(pts1) - np.asarray(pts2)) ** 2).sum(0))
+    inv = np.where(r12 > 0, 1.0 / np.where(r12 > 0, r12, 1.0), 0.0)
+    return np.einsum("k,ka,kb,kc,kd->abcd", np.asarray(w) * inv, A1, A2, A1, A2, optimize=True)
+
+
 def atomic_real(pts, fns, grads=False):
     """Real-harmonic values (and gradients) of (kind, n, l, m, alpha) functions:
     kind 0 STO (sto_real), 1 Ochi, 2 hydrogen-like (via atomic_as_stos)."""

@@ -220,6 +220,20 @@ def qmc_lattice_grid(gridSize, ndim=3, nshifts=2, seed=0):
     return pts, w
 
 
+def qmc_spherical_points(min_points, ndim=6, nshifts=2, seed=0):
+    """The point sets of sphericalCoordinates{3,5,6}DIntegral
+    (src/integration/qmcWrapper.hpp:146-276) on a shifted Korobov lattice:
+    (pts1 (3, N), pts2 (3, N) or None for ndim 3, weights (N,)), weights with
+    the Jacobians, the prefactor and 1/N."""
+    n = check(lib().tcscf_qmc_lattice_npts(int(min_points), int(nshifts)))
+    p1 = np.zeros((3, n))
+    p2 = np.zeros((3, n)) if ndim != 3 else None
+    w = np.zeros(n)
+    check(lib().tcscf_qmc_spherical_points(int(min_points), int(ndim), int(nshifts), int(seed) & (2 ** 64 - 1),
+                                           p1.ctypes.data, p2.ctypes.data if p2 is not None else None, w.ctypes.data))
+    return p1, p2, w
+
+
 def sobol_points(n, dim=3, scramble=True, seed=0):
     """Sobol points (n, dim) in [0, 1), Gray-code order."""
     u = np.zeros((int(n), int(dim)))
@@ -530,6 +544,34 @@ def computeF_batch(specs, precision="bf16x5", out_dtype="fp64"):
         it.out_dtype = _lib.F32 if f32 else _lib.F64
     check(lib().tcscf_batch_compute_F(items, len(specs), prec, _stream()))
     return outs
+
+
+def fillR12Array(basisFunctions, min_points, nshifts=2, seed=0):
+    """fillR12Array (src/AtomicBasis.hpp:167-177) with r12MatrixElement
+    (:30-48): every (ab|cd) as one 6-D QMC integral over (r1, r2) on the point
+    set of qmc_spherical_points(ndim=6), in the real basis.  Returns the full
+    (nb, nb, nb, nb) host array answer[a, b, c, d]."""
+    torch = _torch()
+    p1, p2, w = qmc_spherical_points(min_points, 6, nshifts, seed)
+    g1, g2 = QMCGrid(p1, w), QMCGrid(p2, w)
+    g1.setBasisFunctions(basisFunctions, False)
+    g2.setBasisFunctions(basisFunctions, False)
+    nb, n = g1.nBasis(), g1.nGrid()
+    E = torch.empty((npair(nb), npair(nb)), dtype=torch.float64, device=g1.basisValues.device)
+    check(lib().tcscf_eri_qmc6d(_ptr(g1.quadratureGrid.points), _ptr(g2.quadratureGrid.points),
+                                _ptr(g1.quadratureGrid.weights), ctypes.c_int64(n), _ptr(g1.basisValues),
+                                _ptr(g2.basisValues), nb, _ptr(E), _stream()))
+    return eri_full(E, nb)
+
+
+def qmc_overlap(basisFunctions, min_points, nshifts=2, seed=0):
+    """overlap (src/AtomicBasis.hpp:15-26) for every pair as a 3-D QMC integral
+    on the point set of qmc_spherical_points(ndim=3), real basis: (nb, nb)."""
+    p1, _, w = qmc_spherical_points(min_points, 3, nshifts, seed)
+    g = QMCGrid(p1, w)
+    g.setBasisFunctions(basisFunctions, False)
+    A = g.basisValues
+    return (A.t() @ (A * g.quadratureGrid.weights[:, None])).cpu().numpy()
 
 
 def integrateAllR1R2(r1Grid, F):

@@ -47,6 +47,7 @@ enum ScratchSlot {
   kSlotSweepOut = 3,
   kSlotSmall = 4,
   kSlotBatchMeta = 5,  // + 0 / 1: batch descriptors for the two tile heights
+  kSlotQmc = 7,        // tcscf_eri_qmc6d: the per-point pair densities / r12 and the contraction
   kNumSlots = 8
 };
 

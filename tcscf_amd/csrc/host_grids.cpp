@@ -294,6 +294,50 @@ int64_t tcscf_qmc_lattice_grid(int64_t min_points, int ndim, int nshifts, uint64
   return N;
 }
 
+// The point sets of sphericalCoordinates{3,5,6}DIntegral (src/integration/
+// qmcWrapper.hpp:146-276) on a randomly shifted Korobov lattice in [0, 1)^ndim:
+// r = x0 / (1 - x0), theta = pi x1, phi = 2 pi x2 for electron 1 and the same
+// from x3, x4, x5 for electron 2 (ndim 5 fixes phi2 = 0, its integral being
+// the prefactor's 2 pi).  Weight = prefactor / N * prod_e r_e^2 sin(theta_e) /
+// (1 - x_{3e})^2, prefactor 2 pi^2 (ndim 3) or 4 pi^4 (ndim 5, 6).  pts1 (and
+// pts2 for ndim 5, 6): 3 x N Cartesian SoA; w: N.  Returns N.
+int64_t tcscf_qmc_spherical_points(int64_t min_points, int ndim, int nshifts, uint64_t seed, double* pts1,
+                                   double* pts2, double* w) {
+  TC_CHECK(min_points >= 1 && nshifts >= 1 && (ndim == 3 || ndim == 5 || ndim == 6) && pts1 && w &&
+               (ndim == 3 || pts2),
+           "tcscf_qmc_spherical_points: bad arguments");
+  const int64_t n = next_prime(min_points), N = n * nshifts;
+  const std::vector<int64_t> z = korobov_vector(n, ndim, seed);
+  SplitMix64 rng{seed ^ 0x7370686572696361ull};
+  const double pref = ndim == 3 ? 2 * kPi * kPi : 4 * kPi * kPi * kPi * kPi;
+  for (int sh = 0; sh < nshifts; ++sh) {
+    double delta[6];
+    for (int j = 0; j < ndim; ++j) delta[j] = rng.uniform();
+    for (int64_t i = 0; i < n; ++i) {
+      double x[6] = {0, 0, 0, 0, 0, 0};
+      for (int j = 0; j < ndim; ++j) {
+        const double t = (double)((__int128)i * z[j] % n) / (double)n + delta[j];
+        x[j] = t - std::floor(t);
+      }
+      const int64_t k = (int64_t)sh * n + i;
+      double wk = pref / (double)N;
+      for (int e = 0; e < (ndim == 3 ? 1 : 2); ++e) {
+        const double* xe = x + 3 * e;
+        const double r = xe[0] / (1 - xe[0]), th = kPi * xe[1];
+        const double ph = (e == 1 && ndim == 5) ? 0.0 : 2 * kPi * xe[2];
+        const double st = std::sin(th);
+        double* p = e == 0 ? pts1 : pts2;
+        p[k] = r * st * std::cos(ph);
+        p[N + k] = r * st * std::sin(ph);
+        p[2 * N + k] = r * std::cos(th);
+        wk *= r * r * st / ((1 - xe[0]) * (1 - xe[0]));
+      }
+      w[k] = wk;
+    }
+  }
+  return N;
+}
+
 // Raw Sobol points (dim <= 3, n <= 2^32) in [0, 1), Gray-code order, u: n x dim.
 int tcscf_sobol_points(int64_t n, int dim, int scramble, uint64_t seed, double* u) {
   TC_CHECK(n >= 1 && n <= (1ll << 32) && dim >= 1 && dim <= 3 && u, "tcscf_sobol_points: bad arguments");

@@ -1475,6 +1475,35 @@ extern "C" int tcscf_eri(const double* A1, const double* w1, int64_t n1, int nb,
   return eri_contract(A1, w1, n1, nb, (const double*)F, npair_of(nb), eri, ws, (hipStream_t)stream);
 }
 
+// F[k][p(j, i)] = A2[k][j] A2[k][i] / |r1_k - r2_k| (0 for coincident points).
+__global__ __launch_bounds__(256) void qmc_pair_F(const double* __restrict__ p1, const double* __restrict__ p2,
+                                                  const double* __restrict__ A2, int64_t n, int nb, int64_t npair,
+                                                  double* __restrict__ F) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * npair) return;
+  const int64_t k = e / npair, p = e - k * npair;
+  int j, i;
+  pair_of(p, &j, &i);
+  const double dx = p1[k] - p2[k], dy = p1[n + k] - p2[n + k], dz = p1[2 * n + k] - p2[2 * n + k];
+  const double s = dx * dx + dy * dy + dz * dz;
+  F[e] = s > 0 ? A2[k * nb + j] * A2[k * nb + i] / __builtin_sqrt(s) : 0.0;
+}
+
+extern "C" int tcscf_eri_qmc6d(const double* pts1, const double* pts2, const double* w, int64_t n, const double* A1,
+                               const double* A2, int nb, double* eri, void* stream) {
+  using namespace tc;
+  TC_CHECK(pts1 && pts2 && w && A1 && A2 && eri && n > 0 && nb > 0, "tcscf_eri_qmc6d: bad arguments");
+  const int64_t npair = npair_of(nb);
+  int rc = 0;
+  double* ws = (double*)scratch(kSlotQmc, sizeof(double) * ((size_t)n * npair + eri_contract_doubles(n, nb, npair)), &rc);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(qmc_pair_F, dim3((unsigned)((n * npair + 255) / 256)), dim3(256), 0, st, pts1, pts2, A2, n, nb, npair,
+                     ws);
+  TC_HIP(hipGetLastError());
+  return eri_contract(A1, w, n, nb, ws, npair, eri, ws + (size_t)n * npair, st);
+}
+
 extern "C" int tcscf_eri_gradient(const double* pts1, const double* A1, const double* w1, int64_t n1,
                                   const double* pts2, const double* w2, int64_t n2, const double* A2,
                                   const double* dA2, int nb, int kernel, const tcscf_jastrow* jastrow, double scale,

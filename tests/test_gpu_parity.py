@@ -846,3 +846,26 @@ def test_ochi_basis_on_device_and_reference_he_runs(tc, O, gpu):
               f"reference {row['energy']:.9f} +/- {row['stddev']:.1e}")
         assert abs(mean - row["exact_energy"]) < 4 * sem + 2e-6
         assert abs(mean - row["energy"]) < 2 * np.hypot(row["stddev"], sem)
+
+
+def test_qmc6d_r12_array_and_overlap_vs_oracle(tc, O, gpu):
+    """The per-element QMC route (src/AtomicBasis.hpp): fillR12Array with
+    r12MatrixElement (:30-48, 167-177) on the device, over the 6-D point pairs
+    of qmc_spherical_points, against the oracle's sum over the same pairs
+    (1e-12, s, p and d functions); and overlap (:15-26) by the 3-D QMC route
+    against the analytic STO overlaps (QMC error, tolerance in the assertion)."""
+    from tcscf_amd import SlaterTypeOrbital as STO
+    basis = [STO(0.8, 1, 0, 0), STO(1.7, 1, 0, 0), STO(0.9, 2, 0, 0), STO(1.1, 2, 1, 1), STO(1.1, 2, 1, 0),
+             STO(1.3, 3, 2, -2)]
+    E = tc.fillR12Array(basis, 30000, 2, 3)
+    p1, p2, w = tc.qmc_spherical_points(30000, 6, 2, 3)
+    fns = fns_of(basis)
+    ref = O.eri_qmc6d(p1, p2, w, O.sto_real(p1, fns), O.sto_real(p2, fns))
+    assert rel_err(E, ref) < 1e-12
+    # answer(a, b, c, d) = answer(c, b, a, d) holds per point pair (the electron
+    # exchange answer(b, a, d, c) only up to the QMC error, which the oracle's
+    # closed-form test bounds)
+    assert np.allclose(E, E.transpose(2, 1, 0, 3), rtol=0, atol=1e-14 * np.abs(E).max())
+    S = tc.qmc_overlap(basis, 30000, 2, 5)
+    Sx, _ = tc.one_electron_matrices(basis, 1)
+    assert np.max(np.abs(S - Sx)) < 2e-4

@@ -575,3 +575,32 @@ def test_gradient_eri_reduces_to_eri_of_V(O):
         E = O.eri_from_F(A1, w1, np.ascontiguousarray(V[..., c]))
         assert np.max(np.abs(E)) > 1e-6
         assert rel_err(G.transpose(1, 0, 3, 2), 1.5 * E) < 1e-13
+
+
+def test_qmc_spherical_points_and_eri_qmc6d(O):
+    """The per-element QMC route (src/AtomicBasis.hpp, src/integration/
+    qmcWrapper.hpp:146-276): the point maps integrate e^{-2r} to pi (3-D) and
+    e^{-2 r1 - 2 r2} to pi^2 (5-D, 6-D); the oracle's 6-D r12 matrix elements
+    of 1s STOs approach the closed form (ac|bd) (QMC error, tolerance in the
+    assertion); the 3-D overlap approaches the analytic one."""
+    import tcscf_amd as tc
+    p1, _, w = tc.qmc_spherical_points(20000, 3, 2, 7)
+    assert abs((w * np.exp(-2 * np.sqrt((p1 ** 2).sum(0)))).sum() - math.pi) < 1e-6
+    for nd in (5, 6):
+        p1, p2, w = tc.qmc_spherical_points(20000, nd, 2, 7)
+        r1, r2 = np.sqrt((p1 ** 2).sum(0)), np.sqrt((p2 ** 2).sum(0))
+        assert abs((w * np.exp(-2 * r1 - 2 * r2)).sum() - math.pi ** 2) < 1e-3
+        if nd == 5:
+            assert np.all(p2[1] == 0)  # phi2 = 0
+    z = [0.8, 1.7]
+    fns = [(1, 0, 0, zz) for zz in z]
+    p1, p2, w = tc.qmc_spherical_points(20000, 6, 2, 3)
+    E = O.eri_qmc6d(p1, p2, w, O.sto_real(p1, fns), O.sto_real(p2, fns))
+    ex = np.array([[[[_eri_1s(z[a], z[c], z[b], z[d]) for d in range(2)] for c in range(2)] for b in range(2)]
+                   for a in range(2)])
+    assert np.max(np.abs(E - ex) / np.abs(ex)) < 3e-3
+    p1, _, w = tc.qmc_spherical_points(20000, 3, 2, 5)
+    A = O.sto_real(p1, fns)
+    S = (A * w[:, None]).T @ A
+    Sx, _ = O.sto_S_H([(1, 0, 0, zz) for zz in z], 1)
+    assert np.max(np.abs(S - Sx)) < 1e-5
