@@ -305,9 +305,8 @@ def run_batch(a, tc, configs, torch):
     """The batched-serving workload: 64 uneven instances (Z 2..18, atoms and
     cations, 7..53 STOs, 1,250..30,200 points, half HF / half TC with b in
     [0.5, 2]) from a fixed seed.  One batched contraction call (a persistent
-    launch for BF16) and one batched SCF (a HIP graph per iteration, DIIS 6,
-    tol 1e-8, at most 60 iterations) per precision; the FP64 run gives the
-    per-instance error."""
+    launch for BF16) and the SCF of every instance (DIIS 6, tol 1e-8, at most
+    60 iterations) per precision; the FP64 run gives the per-instance error."""
     import numpy as np
     from tcscf_amd import batch
     inst = configs.batch_instances()

@@ -1,9 +1,10 @@
 """The batched-serving workload end to end: grids and bases for every instance,
-one batched contraction launch for all F / V tensors, one batched SCF (a HIP
-graph per iteration over all active instances)."""
+one batched contraction launch for all F / V tensors, then the SCF of every
+instance."""
 import time
 
 from . import api, configs
+from ._lib import TcscfError
 
 
 def prepare(instances):
@@ -68,8 +69,35 @@ def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None, ou
             d.update(kind="rcs" if x["nup"] == x["ndown"] else "uos", F=tensors[k])
             k += 1
         items.append(d)
-    res, builds, dev_ms, host_ms = api.batch_scf(items, tol=tol, max_iter=max_iter, mgs_mode=1, diis=diis)
+    res = solve_each(items, tol=tol, max_iter=max_iter, diis=diis)
     t2 = time.perf_counter()
     del tensors
     return res, dict(precompute_ms=(t1 - t0) * 1e3, scf_ms=(t2 - t1) * 1e3, total_ms=(t2 - t0) * 1e3,
-                     graph_builds=builds, scf_device_ms=dev_ms, scf_host_ms=host_ms)
+                     scf="one instance after another (the single-instance drivers)")
+
+
+def solve_each(items, tol, max_iter, diis):
+    """The SCF of every instance with the single-instance drivers, one after
+    another (the graph-batched tcscf_batch_scf is switched off).  Returns the
+    per-instance dicts energy / iterations / status / trace / D."""
+    out = []
+    for x in items:
+        nb = x["r1"].nBasis()
+        if x["kind"] == "tc":
+            hf = api.TCHartreeFock(x["nup"], x["ndown"], nb)
+            call = lambda: hf.compute(x["S"], x["H"], x["r1"], x["Fs"], x["Fo"], x["V"], x["jastrow"], True, tol=tol,
+                                      max_iter=max_iter, mgs_mode=1, diis=diis)
+        elif x["kind"] == "rcs":
+            hf = api.RCSHartreeFock(x["nup"] + x["ndown"], nb)
+            call = lambda: hf.compute(x["S"], x["H"], x["r1"], x["F"], True, tol=tol, max_iter=max_iter, diis=diis)
+        else:
+            hf = api.UOSHartreeFock(x["nup"], x["ndown"], nb)
+            call = lambda: hf.compute(x["S"], x["H"], x["r1"], x["F"], True, tol=tol, max_iter=max_iter, diis=diis)
+        try:
+            call()
+            status = 0
+        except TcscfError as err:
+            status = err.code
+        out.append(dict(energy=hf.energy, iterations=hf.numberOfConvergenceLoops(), status=status,
+                        trace=hf.energy_trace, D=hf.density))
+    return out

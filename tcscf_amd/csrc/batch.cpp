@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <thread>
@@ -76,6 +77,14 @@ void parallel_for(int n, int threads, F f) {
 extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcscf_scf_opts* o,
                                tcscf_scf_result* results, int* graph_builds, void* stream) {
   using namespace tc;
+  // Switched off by default: an intermittent segmentation fault on the calling
+  // thread inside this function has been seen and its cause is not found yet.
+  {
+    const char* on = std::getenv("TCSCF_BATCH_SCF");
+    TC_CHECK(on && std::atoi(on) == 1,
+             "tcscf_batch_scf is switched off (an intermittent fault inside it is not explained yet): solve the "
+             "instances with tcscf_rcs_scf / tcscf_uos_scf / tcscf_tc_scf, or set TCSCF_BATCH_SCF=1");
+  }
   TC_CHECK(items && count > 0 && o && results && o->max_iter > 0 && o->diis >= 0, "tcscf_batch_scf: bad arguments");
   // The batch runs on non-blocking streams of its own, which do not wait for
   // the caller's stream: its inputs (F, V, basis tables) must be complete first.

@@ -766,6 +766,8 @@ def test_batched_contraction_matches_single_calls(tc, gpu):
     assert torch.equal(outs[0], tc.computeF(specs[0]["r1"], specs[0]["r2"]))
 
 
+@pytest.mark.skip(reason="tcscf_batch_scf is switched off: an intermittent segmentation fault on the calling "
+                         "thread inside it was seen in two runs of this test and its cause is not found yet")
 def test_batched_scf_matches_single_solves(tc, gpu):
     """tcscf_batch_scf (a HIP graph of every active instance's Fock build per
     iteration) gives the single-instance drivers' energies and iteration counts
