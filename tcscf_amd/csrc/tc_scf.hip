@@ -836,7 +836,11 @@ int TcIter::step(double* G2, double* G3, int it, tcscf_scf_result* res, double* 
       set_error("TC SCF: non-finite Fock matrix (F / V inputs not finite?)");
       return TCSCF_ESOLVER;
     }
-  for (int s = 0; s < 2; ++s) {
+  // Closed shell with bitwise-equal spin operators: the spin-1 solve would
+  // repeat spin 0's exactly, so its orbitals, density and pseudo-energies are
+  // copied instead.
+  const bool same = ne[0] == ne[1] && std::memcmp(Fock.data(), Fock.data() + nb2, sizeof(double) * nb2) == 0;
+  for (int s = 0; s < (same ? 1 : 2); ++s) {
     std::vector<std::complex<double>> ev, X;
     if (!dense::gen_nonsym_eig(nb, Fock.data() + s * nb2, S, ev, X)) {
       res->status = TCSCF_ESOLVER;
@@ -875,6 +879,11 @@ int TcIter::step(double* G2, double* G3, int it, tcscf_scf_result* res, double* 
         for (int b = 0; b < no; ++b) t += C[((size_t)s * nb + u) * nmax + b] * C[((size_t)s * nb + v) * nmax + b];
         D[s * nb2 + u * nb + v] = t;
       }
+  }
+  if (same) {
+    std::copy(C.begin(), C.begin() + (size_t)nb * nmax, C.begin() + (size_t)nb * nmax);
+    std::copy(D.begin(), D.begin() + nb2, D.begin() + nb2);
+    for (int b = 0; b < ne[1]; ++b) pseudo[ne[0] + b] = pseudo[b];
   }
   return 0;
 }
