@@ -186,6 +186,57 @@ def test_cpp_facade_host_checks(tc):
     assert "0 failure(s)" in r.stdout
 
 
+@pytest.mark.parametrize("sanitizer", ["address", "thread"])
+def test_batch_scf_host_logic_under_sanitizers(tmp_path, sanitizer):
+    """tcscf_batch_scf's host side on the CPU (tests/cpp/batch_host_emu.cpp):
+    the real batch driver and the real host halves of the RCS / UOS / TC
+    iterations, linked against host stand-ins for the HIP runtime that replay
+    each captured graph with a CPU model of the Fock builds, under
+    AddressSanitizer and ThreadSanitizer.  Every instance converges, the graph
+    is captured again as instances finish, and two runs of one batch agree
+    bitwise (energies, iteration counts, traces)."""
+    import glob
+    import re
+    import shutil
+    import subprocess
+    clang = "/opt/rocm/llvm/bin/clang++"
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not (os.path.exists(clang) and os.path.exists(hipcc)):
+        pytest.skip("ROCm clang / hipcc not found")
+    csrc = os.path.join(ROOT, "tcscf_amd", "csrc")
+    inc = ["-I", csrc, "-I", os.path.join(ROOT, "include")]
+    flags = ["-O1", "-g", "-std=c++17", "-fno-omit-frame-pointer"]
+    objs = []
+    for f in ("scf.hip", "tc_scf.hip"):  # host halves only; kernels are never launched here
+        o = str(tmp_path / (f + ".o"))
+        subprocess.run([hipcc, "-x", "hip", "--offload-host-only", "-fno-gpu-sanitize", "-Xarch_host",
+                        "-fsanitize=" + sanitizer, *flags, *inc, "-c", os.path.join(csrc, f), "-o", o],
+                       check=True, capture_output=True)
+        objs.append(o)
+    for f in (os.path.join(csrc, "batch.cpp"), os.path.join(csrc, "dense.cpp"),
+              os.path.join(ROOT, "tests", "cpp", "batch_host_emu.cpp")):
+        o = str(tmp_path / (os.path.basename(f) + ".o"))
+        subprocess.run([clang, "-fsanitize=" + sanitizer, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", *flags,
+                        *inc, "-c", f, "-o", o], check=True, capture_output=True)
+        objs.append(o)
+    # the host-only objects still name their (absent) device code objects
+    names = set()
+    for o in objs[:2]:
+        nm = subprocess.run(["nm", "-u", o], check=True, capture_output=True, text=True).stdout
+        names |= set(re.findall(r"__hip_fatbin_[0-9a-f]+", nm))
+    stub = tmp_path / "fatbin_stubs.cpp"
+    stub.write_text("".join('extern "C" char %s[64]; char %s[64];\n' % (n, n) for n in sorted(names)))
+    subprocess.run([clang, "-c", str(stub), "-o", str(stub) + ".o"], check=True)
+    exe = str(tmp_path / "batch_host_emu")
+    subprocess.run([clang, "-no-pie", "-fsanitize=" + sanitizer, "-o", exe, *objs, str(stub) + ".o", "-lpthread"],
+                   check=True, capture_output=True)
+    r = subprocess.run([exe, "2"], capture_output=True, text=True, timeout=600)
+    out = r.stdout + r.stderr
+    assert r.returncode == 0, out[-4000:]
+    assert "0 failure(s)" in r.stdout and "Sanitizer" not in out, out[-4000:]
+    assert len(glob.glob(str(tmp_path / "*.o"))) == 6
+
+
 def test_batch_workload_generator_is_pinned(tc):
     """The batched-serving instance list is reproducible from its seed and
     matches the committed fixture; it spans the workload's ranges."""
