@@ -291,6 +291,24 @@ int tcscf_eri(const double* A1_dev, const double* w1_dev, int64_t n1, int nb, co
 int tcscf_eri_qmc6d(const double* pts1_dev, const double* pts2_dev, const double* w_dev, int64_t n,
                     const double* A1_dev, const double* A2_dev, int nb, double* eri_dev, void* stream);
 
+/* atomicCoulombOperator and atomicExchangeOperator (src/AtomicBasis.hpp:59-140),
+ * the per-element 5-D QMC route, for every quadruple at once, with the
+ * reference's complex functions g = R(r) sph_legendre(l, |m|, theta) e^{i m phi}:
+ * for the point pairs and weights of tcscf_qmc_spherical_points(ndim = 5) (r2 on
+ * the phi = 0 half-plane, the phi2 integral in the weights),
+ *   eri[2e], eri[2e+1] = Re, Im of sum_k w_k conj(g_a(r1_k)) g_c(r1_k) g_b(r2_k) g_d(r2_k) / r12_k,
+ *   e = ((a nb + b) nb + c) nb + d  (complex nb^4, device),
+ * which is the Coulomb integrand where (l, m) of a, c and of b, d agree and the
+ * exchange integrand (phase e^{i (m_c - m_a) phi1}) where a, d and b, c agree.
+ * g is formed on the device from the real table A1_dev / A2_dev (n x nb, as
+ * tcscf_basis_eval_fns writes it): g_{+u} = (-1)^u (S_{+u} + i S_{-u}) / sqrt2,
+ * g_{-u} = (-1)^u (S_{+u} - i S_{-u}) / sqrt2.  m_host[a] is the m of column a and
+ * partner_host[a] the column with the same function and -m (a itself for
+ * m = 0); the basis must hold both.  nb <= 32. */
+int tcscf_eri_qmc5d_atomic(const double* pts1_dev, const double* pts2_dev, const double* w_dev, int64_t n,
+                           const double* A1_dev, const double* A2_dev, int nb, const int* m_host,
+                           const int* partner_host, double* eri_dev, void* stream);
+
 /* integrateAllR1R2<true> (src/integration/integrateAll.hpp:184-215, 221-317),
  * real basis, for the vector kernel f = c_same grad_1 u(r1, r2) (kernel must be
  * TCSCF_K_JASTROW_GRAD):

@@ -179,6 +179,68 @@ def atomic_real(pts, fns, grads=False):
     return (vo, go) if grads else vo
 
 
+def atomic_complex(pts, fns):
+    """The reference's complex functions R(r) sph_legendre(l, |m|, theta) e^{i m phi}
+    (src/SlaterTypeOrbital.hpp, src/OchiBasis.hpp, src/HydrogenLikeBasis.hpp over
+    src/mathFunctions.hpp:126-139) of (kind, n, l, m, alpha) functions: (n, nb)."""
+    terms, owner = [], []
+    for b, f in enumerate(fns):
+        for w, sto in ([(1.0, (f[1], f[2], f[3], f[4]))] if f[0] == 0 else atomic_as_stos(f)):
+            terms.append((w, sto))
+            owner.append(b)
+    v = sto_complex(pts, [t[1] for t in terms])
+    vo = np.zeros((pts.shape[1], len(fns)), complex)
+    for t, (w, _) in enumerate(terms):
+        vo[:, owner[t]] += w * v[:, t]
+    return vo
+
+
+def eri_qmc5d_atomic(pts1, pts2, w, fns):
+    """atomicCoulombOperator / atomicExchangeOperator (src/AtomicBasis.hpp:59-140)
+    for every quadruple on the 5-D point set (r2 on the phi = 0 half-plane, the
+    phi2 integral in the weights), the reference's complex functions g:
+    answer[a, b, c, d] = sum_k w_k conj(g_a(r1_k) g_b(r2_k)) g_c(r1_k) g_d(r2_k) / r12_k.
+    At phi2 = 0 the r2 factors are the reference's sph_legendre magnitudes, so
+    the Coulomb entries (a, c and b, d of equal l, m) are its Coulomb integrand
+    and the exchange entries (a, d and b, c equal) its exchange integrand with
+    the phase e^{i (m_c - m_a) phi1}."""
+    g1, g2 = atomic_complex(pts1, fns), atomic_complex(pts2, fns)
+    r12 = np.sqrt(((np.asarray(pts1) - np.asarray(pts2)) ** 2).sum(0))
+    inv = np.where(r12 > 0, 1.0 / np.where(r12 > 0, r12, 1.0), 0.0)
+    return np.einsum("k,ka,kb,kc,kd->abcd", np.asarray(w) * inv, g1.conj(), g2.conj(), g1, g2, optimize=True)
+
+
+def fill_two_electron(N, real, value):
+    """fillTwoElectronSymmetricHermitianArray (src/mathFunctions.hpp:367-433):
+    value(a, b, c, d) on the canonical quadruples only (a <= b, a <= c, and
+    d >= b for real functions, else d >= a), each copied to its images under the
+    electron swap, Hermitian conjugation and, for real functions, the b <-> d
+    swap; the array accumulates (+=).  Pure Python, for small N."""
+    A = np.zeros((N,) * 4, complex)
+    for a in range(N):
+        for b in range(a, N):
+            for c in range(a, N):
+                for d in range(b if real else a, N):
+                    v = complex(value(a, b, c, d))
+                    A[a, b, c, d] += v
+                    sym, herm, rl = a != b, a != c and a != d, real and d != b
+                    if sym:
+                        A[b, a, d, c] += v
+                    if herm or (real and a != c):
+                        A[c, d, a, b] += v.conjugate()
+                    if herm and sym:
+                        A[d, c, b, a] += v.conjugate()
+                    if rl:
+                        A[a, d, c, b] += v
+                    if rl and sym:
+                        A[d, a, b, c] += v
+                    if rl and herm:
+                        A[c, b, a, d] += v
+                    if rl and herm and sym:
+                        A[b, c, d, a] += v
+    return A
+
+
 def sto_S_H(fns, Z):
     f = _fns(fns)
     nb = len(f)

@@ -8,7 +8,7 @@ from ._lib import (BF16X3, BF16X5, FP32, FP64, K_COULOMB, K_JASTROW_GRAD, K_TC_S
                    RADIAL_XOVER1MX, ConvergenceError, SolverError, TcscfError, lib)
 from .api import (HydrogenLikeBasisFunction, Jastrow, OchiBasisFunction, OchiJastrow, batch_scf, QMCGrid, QuadratureGrid, RCSHartreeFock,
                   SlaterTypeOrbital, TCHartreeFock, UOSHartreeFock,
-                  coreMatrixElement, computeF, computeF_batch, fillR12Array, qmc_overlap, qmc_spherical_points, computeF_tc_pair, computeV, eri_full, eri_full_gradient, even_tempered,
+                  coreMatrixElement, computeF, computeF_batch, fillR12Array, fillAtomicR12Array, atomicCoulombOperator, atomicExchangeOperator, qmc_overlap, qmc_spherical_points, computeF_tc_pair, computeV, eri_full, eri_full_gradient, even_tempered,
                   integrateAllR1R2, integrateAllR1R2Gradient, lebedev, loadSTO, npair, one_electron_matrices, overlap,
                   pair_index, product_grid, qmc_lattice_grid, radial_grid, sobol_grid, sobol_points,
                   threeElectronIntegrals, unpack)
@@ -18,7 +18,7 @@ __all__ = [
     "RADIAL_XOVER1MX", "ConvergenceError", "SolverError", "TcscfError", "lib", "HydrogenLikeBasisFunction", "Jastrow",
     "OchiBasisFunction", "OchiJastrow", "QMCGrid",
     "QuadratureGrid", "batch_scf", "RCSHartreeFock", "SlaterTypeOrbital", "TCHartreeFock", "UOSHartreeFock", "coreMatrixElement",
-    "computeF", "computeF_batch", "fillR12Array", "qmc_overlap", "qmc_spherical_points", "computeF_tc_pair", "computeV", "eri_full", "eri_full_gradient", "even_tempered", "integrateAllR1R2",
+    "computeF", "computeF_batch", "fillR12Array", "fillAtomicR12Array", "atomicCoulombOperator", "atomicExchangeOperator", "qmc_overlap", "qmc_spherical_points", "computeF_tc_pair", "computeV", "eri_full", "eri_full_gradient", "even_tempered", "integrateAllR1R2",
     "integrateAllR1R2Gradient", "lebedev", "loadSTO", "npair", "one_electron_matrices", "overlap", "pair_index", "product_grid",
     "qmc_lattice_grid", "radial_grid", "sobol_grid", "sobol_points", "threeElectronIntegrals", "unpack",
 ]

@@ -22,7 +22,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16x5": BF16X5}
 EXPORTS = [
     "tcscf_last_error", "tcscf_version", "tcscf_radial_grid", "tcscf_lebedev_npts", "tcscf_lebedev",
     "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
-    "tcscf_F_workspace_bytes", "tcscf_compute_F_tc_pair", "tcscf_compute_F_ochi", "tcscf_eri_qmc6d", "tcscf_qmc_spherical_points", "tcscf_basis_eval_fns", "tcscf_basis_one_electron", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
+    "tcscf_F_workspace_bytes", "tcscf_compute_F_tc_pair", "tcscf_compute_F_ochi", "tcscf_eri_qmc6d", "tcscf_eri_qmc5d_atomic", "tcscf_qmc_spherical_points", "tcscf_basis_eval_fns", "tcscf_basis_one_electron", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
     "tcscf_qmc_lattice_npts", "tcscf_qmc_lattice_grid", "tcscf_sobol_points", "tcscf_sobol_grid", "tcscf_load_sto",
     "tcscf_batch_compute_F", "tcscf_batch_tuned_rows", "tcscf_batch_scf",
     "tcscf_selftest_mfma",
@@ -128,6 +128,9 @@ def lib():
                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.tcscf_eri_qmc6d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.tcscf_eri_qmc5d_atomic.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.tcscf_sobol_grid.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib

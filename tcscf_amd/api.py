@@ -549,8 +549,12 @@ def computeF_batch(specs, precision="bf16x5", out_dtype="fp64"):
 def fillR12Array(basisFunctions, min_points, nshifts=2, seed=0):
     """fillR12Array (src/AtomicBasis.hpp:167-177) with r12MatrixElement
     (:30-48): every (ab|cd) as one 6-D QMC integral over (r1, r2) on the point
-    set of qmc_spherical_points(ndim=6), in the real basis.  Returns the full
-    (nb, nb, nb, nb) host array answer[a, b, c, d]."""
+    set of qmc_spherical_points(ndim=6), in the real basis.  Every quadruple
+    comes from one device pass; as the reference fills the array
+    (fillTwoElectronSymmetricHermitianArray with real functions), the canonical
+    quadruples' values are copied to their symmetry images, so entries that
+    differ only by the electron swap are equal.  Returns the (nb, nb, nb, nb)
+    host array answer[a, b, c, d]."""
     torch = _torch()
     p1, p2, w = qmc_spherical_points(min_points, 6, nshifts, seed)
     g1, g2 = QMCGrid(p1, w), QMCGrid(p2, w)
@@ -561,7 +565,8 @@ def fillR12Array(basisFunctions, min_points, nshifts=2, seed=0):
     check(lib().tcscf_eri_qmc6d(_ptr(g1.quadratureGrid.points), _ptr(g2.quadratureGrid.points),
                                 _ptr(g1.quadratureGrid.weights), ctypes.c_int64(n), _ptr(g1.basisValues),
                                 _ptr(g2.basisValues), nb, _ptr(E), _stream()))
-    return eri_full(E, nb)
+    full = eri_full(E, nb)
+    return _fill_two_electron(nb, True, lambda a, b, c, d: full[a, b, c, d]).real
 
 
 def qmc_overlap(basisFunctions, min_points, nshifts=2, seed=0):
@@ -572,6 +577,118 @@ def qmc_overlap(basisFunctions, min_points, nshifts=2, seed=0):
     g.setBasisFunctions(basisFunctions, False)
     A = g.basisValues
     return (A.t() @ (A * g.quadratureGrid.weights[:, None])).cpu().numpy()
+
+
+def _fill_two_electron(N, real, value):
+    """fillTwoElectronSymmetricHermitianArray (src/mathFunctions.hpp:367-433):
+    value(a, b, c, d) is asked for the canonical quadruples only (a <= b,
+    a <= c, d >= b for real functions, else d >= a) and copied to their images
+    under the electron swap (b, a, d, c), Hermitian conjugation (c, d, a, b),
+    (d, c, b, a) and, for real functions, b <-> d; entries accumulate."""
+    A = np.zeros((N,) * 4, complex)
+    for a in range(N):
+        for b in range(a, N):
+            for c in range(a, N):
+                for d in range(b if real else a, N):
+                    v = complex(value(a, b, c, d))
+                    if v == 0:
+                        continue
+                    vc = v.conjugate()
+                    sym, herm, rl = a != b, a != c and a != d, real and d != b
+                    A[a, b, c, d] += v
+                    if sym:
+                        A[b, a, d, c] += v
+                    if herm or (real and a != c):
+                        A[c, d, a, b] += vc
+                    if herm and sym:
+                        A[d, c, b, a] += vc
+                    if rl:
+                        A[a, d, c, b] += v
+                        if sym:
+                            A[d, a, b, c] += v
+                        if herm:
+                            A[c, b, a, d] += v
+                        if herm and sym:
+                            A[b, c, d, a] += v
+    return A
+
+
+def _atomic_eri5d(basisFunctions, min_points, nshifts, seed):
+    """Every quadruple of tcscf_eri_qmc5d_atomic for the reference's complex
+    functions (l, m) of basisFunctions: complex (nb,)*4 numpy array.  Missing -m
+    partners are tabulated alongside (the complex functions are formed from
+    real pairs) and dropped from the result."""
+    import copy
+    torch = _torch()
+    basis = list(basisFunctions)
+    key = lambda b: (type(b).__name__, b.kind, b.n, b.l, b.m, b.alpha)
+    index = {key(b): k for k, b in enumerate(basis)}
+    for b in list(basis):
+        if b.m != 0:
+            q = copy.copy(b)
+            q.m = -b.m
+            if key(q) not in index:
+                index[key(q)] = len(basis)
+                basis.append(q)
+    nb = len(basis)
+    if nb > 32:
+        raise ValueError("atomic QMC integrals: at most 32 functions (with their -m partners)")
+    partner = np.array([k if b.m == 0 else index[(type(b).__name__, b.kind, b.n, b.l, -b.m, b.alpha)]
+                        for k, b in enumerate(basis)], dtype=np.int32)
+    ms = np.array([b.m for b in basis], dtype=np.int32)
+    p1, p2, w = qmc_spherical_points(min_points, 5, nshifts, seed)
+    g1, g2 = QMCGrid(p1, w), QMCGrid(p2, w)
+    g1.setBasisFunctions(basis, False)
+    g2.setBasisFunctions(basis, False)
+    E = torch.empty((nb, nb, nb, nb, 2), dtype=torch.float64, device=g1.basisValues.device)
+    check(lib().tcscf_eri_qmc5d_atomic(_ptr(g1.quadratureGrid.points), _ptr(g2.quadratureGrid.points),
+                                       _ptr(g1.quadratureGrid.weights), ctypes.c_int64(g1.nGrid()),
+                                       _ptr(g1.basisValues), _ptr(g2.basisValues), nb, ms.ctypes.data,
+                                       partner.ctypes.data, _ptr(E), _stream()))
+    E = E.cpu().numpy()
+    n0 = len(basisFunctions)
+    return (E[..., 0] + 1j * E[..., 1])[:n0, :n0, :n0, :n0]
+
+
+def atomicCoulombOperator(b1, b2, b3, b4, min_points, nshifts=2, seed=0):
+    """atomicCoulombOperator (src/AtomicBasis.hpp:59-98): the 5-D QMC integral
+    of |Y_l1m1(theta1)|^2 |Y_l2m2(theta2)|^2 R1 R3(r1) R2 R4(r2) / r12 on the
+    point set of qmc_spherical_points(ndim=5); b1, b3 and b2, b4 must share
+    (l, m).  Returns a float."""
+    if (b1.l, b1.m) != (b3.l, b3.m) or (b2.l, b2.m) != (b4.l, b4.m):
+        raise ValueError("atomicCoulombOperator: b1, b3 and b2, b4 need equal (l, m)")
+    E = _atomic_eri5d([b1, b2, b3, b4], min_points, nshifts, seed)
+    return float(E[0, 1, 2, 3].real)
+
+
+def atomicExchangeOperator(b1, b2, b3, b4, min_points, nshifts=2, seed=0):
+    """atomicExchangeOperator (src/AtomicBasis.hpp:103-140): the 5-D QMC
+    integral of R1 P_l1m1(theta1) R2 P_l2m2(theta2) R3 P_l2m2(theta1)
+    R4 P_l1m1(theta2) e^{i (m2 - m1) phi1} / r12; b1, b4 and b2, b3 must share
+    (l, m).  Returns a complex."""
+    if (b1.l, b1.m) != (b4.l, b4.m) or (b2.l, b2.m) != (b3.l, b3.m):
+        raise ValueError("atomicExchangeOperator: b1, b4 and b2, b3 need equal (l, m)")
+    E = _atomic_eri5d([b1, b2, b3, b4], min_points, nshifts, seed)
+    return complex(E[0, 1, 2, 3])
+
+
+def fillAtomicR12Array(basisFunctions, min_points, nshifts=2, seed=0):
+    """fillAtomicR12Array (src/AtomicBasis.hpp:197-248): the (nb,)*4 complex
+    array of the reference's complex functions holding the Coulomb integrals
+    ((l, m) of a, c and of b, d equal; real-function symmetry) and, elsewhere,
+    the exchange integrals ((l, m) of a, d and of b, c equal; Hermitian
+    symmetry), 0 otherwise.  Every integral comes from one device pass over all
+    quadruples on one 5-D point set; the reference's canonical quadruples and
+    their symmetry copies are kept as it fills them."""
+    E = _atomic_eri5d(basisFunctions, min_points, nshifts, seed)
+    lm = [(b.l, b.m) for b in basisFunctions]
+    N = len(lm)
+    coul = lambda a, b, c, d: lm[a] == lm[c] and lm[b] == lm[d]
+    exch = lambda a, b, c, d: lm[a] == lm[d] and lm[b] == lm[c]
+    A = _fill_two_electron(N, True, lambda a, b, c, d: E[a, b, c, d].real if coul(a, b, c, d) else 0)
+    A += _fill_two_electron(N, False, lambda a, b, c, d: E[a, b, c, d] if exch(a, b, c, d) and not coul(a, b, c, d)
+                            else 0)
+    return A
 
 
 def integrateAllR1R2(r1Grid, F):

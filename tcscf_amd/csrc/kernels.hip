@@ -1504,6 +1504,88 @@ extern "C" int tcscf_eri_qmc6d(const double* pts1, const double* pts2, const dou
   return eri_contract(A1, w, n, nb, ws, npair, eri, ws + (size_t)n * npair, st);
 }
 
+namespace tc {
+struct ComplexCols {
+  int m[64];        // m of each function
+  int partner[64];  // column of the real function with -m (itself for m = 0)
+};
+
+// The reference's complex functions g = R sph_legendre(l, |m|, theta) e^{i m phi}
+// from the real table S (no Condon-Shortley phase in S):
+//   g_{+u} = (-1)^u (S_{+u} + i S_{-u}) / sqrt2,  g_{-u} = (-1)^u (S_{+u} - i S_{-u}) / sqrt2.
+// G1 (n x 2nb): [Re g | Im g] at the r1 points; G2 (n x nb): Re g at the r2
+// points, which lie on the phi = 0 half-plane where g is real.
+__global__ __launch_bounds__(256) void complex_columns(const double* __restrict__ A1, const double* __restrict__ A2,
+                                                       int64_t n, int nb, ComplexCols cc, double* __restrict__ G1,
+                                                       double* __restrict__ G2) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * nb) return;
+  const int64_t k = e / nb;
+  const int a = (int)(e - k * nb), m = cc.m[a], q = cc.partner[a];
+  double re1 = A1[e], im1 = 0.0, re2 = A2[e];
+  if (m != 0) {
+    const double s = ((m < 0 ? -m : m) & 1) ? -M_SQRT1_2 : M_SQRT1_2;
+    const double plus1 = m > 0 ? A1[e] : A1[k * nb + q], minus1 = m > 0 ? A1[k * nb + q] : A1[e];
+    re1 = s * plus1;
+    im1 = m > 0 ? s * minus1 : -s * minus1;
+    re2 = s * (m > 0 ? A2[e] : A2[k * nb + q]);
+  }
+  G1[k * 2 * nb + a] = re1;
+  G1[k * 2 * nb + nb + a] = im1;
+  G2[e] = re2;
+}
+
+// eri[2 e], eri[2 e + 1] (e = ((a nb + b) nb + c) nb + d): the complex integral
+// from Eh = [Re g | Im g]-pair rows x (b, d) columns.
+__global__ __launch_bounds__(256) void complex_eri_assemble(const double* __restrict__ Eh, int nb, int64_t ncol,
+                                                            double* __restrict__ eri) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t nb4 = (int64_t)nb * nb * nb * nb;
+  if (e >= nb4) return;
+  const int d = (int)(e % nb), c = (int)((e / nb) % nb), b = (int)((e / nb / nb) % nb), a = (int)(e / nb / nb / nb);
+  auto p = [](int j, int i) { return j <= i ? (int64_t)i * (i + 1) / 2 + j : (int64_t)j * (j + 1) / 2 + i; };
+  const int64_t col = p(b, d);
+  const double re = Eh[p(a, c) * ncol + col] + Eh[p(nb + a, nb + c) * ncol + col];
+  const double im = Eh[p(a, nb + c) * ncol + col] - Eh[p(c, nb + a) * ncol + col];
+  eri[2 * e] = re;
+  eri[2 * e + 1] = im;
+}
+}  // namespace tc
+
+extern "C" int tcscf_eri_qmc5d_atomic(const double* pts1, const double* pts2, const double* w, int64_t n,
+                                      const double* A1, const double* A2, int nb, const int* m_host,
+                                      const int* partner_host, double* eri, void* stream) {
+  using namespace tc;
+  TC_CHECK(pts1 && pts2 && w && A1 && A2 && eri && m_host && partner_host && n > 0 && nb > 0,
+           "tcscf_eri_qmc5d_atomic: bad arguments");
+  TC_CHECK(nb <= 32, "tcscf_eri_qmc5d_atomic: at most 32 functions");
+  ComplexCols cc{};
+  for (int a = 0; a < nb; ++a) {
+    const int q = partner_host[a];
+    TC_CHECK(q >= 0 && q < nb && m_host[q] == -m_host[a] && (m_host[a] != 0 || q == a),
+             "tcscf_eri_qmc5d_atomic: function " + std::to_string(a) + " needs its -m partner in the basis");
+    cc.m[a] = m_host[a];
+    cc.partner[a] = q;
+  }
+  const int64_t npair = npair_of(nb), nrow = npair_of(2 * nb);
+  const size_t nG = (size_t)n * 3 * nb, nF = (size_t)n * npair, nE = (size_t)nrow * npair;
+  int rc = 0;
+  double* ws = (double*)scratch(kSlotQmc, sizeof(double) * (nG + nF + nE + eri_contract_doubles(n, 2 * nb, npair)), &rc);
+  if (rc) return rc;
+  double *G1 = ws, *G2 = ws + (size_t)n * 2 * nb, *F = ws + nG, *Eh = F + nF, *cws = Eh + nE;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(complex_columns, dim3((unsigned)((n * nb + 255) / 256)), dim3(256), 0, st, A1, A2, n, nb, cc, G1, G2);
+  TC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(qmc_pair_F, dim3((unsigned)((n * npair + 255) / 256)), dim3(256), 0, st, pts1, pts2, G2, n, nb,
+                     npair, F);
+  TC_HIP(hipGetLastError());
+  if ((rc = eri_contract(G1, w, n, 2 * nb, F, npair, Eh, cws, st))) return rc;
+  const int64_t nb4 = (int64_t)nb * nb * nb * nb;
+  hipLaunchKernelGGL(complex_eri_assemble, dim3((unsigned)((nb4 + 255) / 256)), dim3(256), 0, st, Eh, nb, npair, eri);
+  TC_HIP(hipGetLastError());
+  return 0;
+}
+
 extern "C" int tcscf_eri_gradient(const double* pts1, const double* A1, const double* w1, int64_t n1,
                                   const double* pts2, const double* w2, int64_t n2, const double* A2,
                                   const double* dA2, int nb, int kernel, const tcscf_jastrow* jastrow, double scale,

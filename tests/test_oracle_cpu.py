@@ -577,6 +577,65 @@ def test_gradient_eri_reduces_to_eri_of_V(O):
         assert rel_err(G.transpose(1, 0, 3, 2), 1.5 * E) < 1e-13
 
 
+def test_atomic_qmc5d_hydrogenic_closed_forms(O):
+    """atomicCoulombOperator / atomicExchangeOperator (src/AtomicBasis.hpp:59-140)
+    as the oracle restates them, on the 5-D point set, for hydrogen-like
+    functions of Z = 1, against the textbook Slater-Condon values:
+    F0(1s,1s) = 5/8, J(1s,2s) = 17/81, K(1s,2s) = 16/729, and with
+    F0(2p,2p) = 93/512, F2(2p,2p) = 45/512: J(2p0,2p0) = F0 + 4/25 F2,
+    J(2p0,2p1) = F0 - 2/25 F2, K(2p0,2p1) = 3/25 F2 (its imaginary part
+    integrates to 0).  The reference's own test prints J(2p0,2p0) = 0.196207 +-
+    0.0018 and J(2p0,2p1) = 0.174825 +- 0.000862
+    (src/unitTests/IntegrationTest.cpp:397-398, 444-446).  QMC error: the
+    tolerances are in the assertions."""
+    import tcscf_amd as tc
+    fns = [(2, 1, 0, 0, 1.0), (2, 2, 0, 0, 1.0), (2, 2, 1, 0, 1.0), (2, 2, 1, 1, 1.0)]
+    p1, p2, w = tc.qmc_spherical_points(100000, 5, 2, 0)
+    E = O.eri_qmc5d_atomic(p1, p2, w, fns)
+    F0, F2 = 93 / 512, 45 / 512
+    checks = [((0, 0, 0, 0), 5 / 8, 1e-3), ((0, 1, 0, 1), 17 / 81, 5e-4), ((0, 1, 1, 0), 16 / 729, 2e-4),
+              ((2, 2, 2, 2), F0 + 4 / 25 * F2, 1e-3), ((2, 3, 2, 3), F0 - 2 / 25 * F2, 5e-4),
+              ((2, 3, 3, 2), 3 / 25 * F2, 3e-4)]
+    for q, exact, tol in checks:
+        assert abs(E[q].real - exact) < tol, (q, E[q], exact)
+        assert abs(E[q].imag) < 1e-4, (q, E[q])
+    assert abs(E[2, 2, 2, 2].real - 0.196207) < 0.0018 and abs(E[2, 3, 2, 3].real - 0.174825) < 0.000862
+    # conj(g_a) g_c at r1: answer(c, b, a, d) = conj(answer(a, b, c, d)) on every point set
+    assert np.allclose(E.transpose(2, 1, 0, 3), E.conj(), rtol=0, atol=1e-15)
+
+
+def test_fill_two_electron_symmetries(O):
+    """fillTwoElectronSymmetricHermitianArray (src/mathFunctions.hpp:367-433) as
+    restated in the oracle and in the product (tcscf_amd.api._fill_two_electron):
+    on tensors with the symmetries it assumes (8-fold for real functions;
+    electron swap and Hermitian for complex ones) it reproduces the tensor
+    exactly, so every image is written once; on a tensor without them, the
+    canonical quadruples (a <= b, a <= c, d >= b or d >= a) keep their own
+    values and the two restatements agree entry for entry."""
+    from tcscf_amd.api import _fill_two_electron
+    rng = np.random.default_rng(4)
+    N = 5
+    R = rng.normal(size=(N,) * 4)
+    R = R + R.transpose(1, 0, 3, 2)
+    R = R + R.transpose(2, 3, 0, 1)
+    R = R + R.transpose(0, 3, 2, 1)
+    R = R + R.transpose(2, 1, 0, 3)
+    X = rng.normal(size=(N,) * 4) + 1j * rng.normal(size=(N,) * 4)
+    C = X + X.transpose(1, 0, 3, 2)
+    C = C + C.transpose(2, 3, 0, 1).conj()
+    for fill in (O.fill_two_electron, _fill_two_electron):
+        assert np.array_equal(fill(N, True, lambda a, b, c, d: R[a, b, c, d]), R.astype(complex))
+        assert np.allclose(fill(N, False, lambda a, b, c, d: C[a, b, c, d]), C, rtol=0, atol=1e-13)
+    for real in (True, False):
+        A = O.fill_two_electron(N, real, lambda a, b, c, d: X[a, b, c, d])
+        assert np.array_equal(A, _fill_two_electron(N, real, lambda a, b, c, d: X[a, b, c, d]))
+        for a in range(N):
+            for b in range(a + 1, N):
+                for c in range(a + 1, N):
+                    for d in range(b + 1 if real else a + 1, N):
+                        assert A[a, b, c, d] == X[a, b, c, d]
+
+
 def test_qmc_spherical_points_and_eri_qmc6d(O):
     """The per-element QMC route (src/AtomicBasis.hpp, src/integration/
     qmcWrapper.hpp:146-276): the point maps integrate e^{-2r} to pi (3-D) and
