@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--precision", default="bf16x5", choices=["bf16x5", "bf16x3", "fp32", "fp64"])
     p.add_argument("--fp64-steps", type=int, default=2)
+    p.add_argument("--storage", default="fp64", choices=["fp64", "fp32"],
+                   help="F storage of the mixed-precision RHF leg (the FP64 reference leg stores FP64)")
     p.add_argument("--tol", type=float, default=1e-9)
     p.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU-baseline slice (0 = auto)")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -89,7 +91,8 @@ def main():
         grid.setBasisFunctions(basis, False)
         if ev:
             ev[0].record(stream)
-        F = tc.computeF(grid, grid, precision=precision)
+        F = tc.computeF(grid, grid, precision=precision,
+                        out_dtype=a.storage if precision != "fp64" else "fp64")
         if ev:
             ev[1].record(stream)
         torch.cuda.synchronize()  # the SCF's own sweep timers then see the sweeps only
@@ -118,6 +121,7 @@ def main():
         t64, k64, e64, hf64 = t_mix, k_mix, e_mix, hf_mix
 
     flops = 2.0 * npair * pairs  # algorithmic: one MAC per pair density per grid pair
+    fbytes32 = a.storage == "fp32" and a.precision != "fp64"
     achieved = flops / (k_mix * 1e-3) / 1e12
     peak = PEAK_TFLOPS[a.precision]
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
@@ -176,7 +180,7 @@ def main():
         "data": "synthetic (even-tempered STO basis, deterministic radial x Lebedev grid)",
         "config": {"workload": f"Ne RHF, {nb} STOs (13s,4p,1d even-tempered 0.3*1.8^k), {n}-point grid "
                                f"({nrad} Gauss-Chebyshev/M4 radial x Lebedev-{order}), 1/r12, tol {a.tol:g}",
-                   "precision": a.precision, "grid_points": n, "nbasis": nb, "npair": npair},
+                   "precision": a.precision, "storage": a.storage, "grid_points": n, "nbasis": nb, "npair": npair},
         "scf_wall_ms": t_mix * 1e3,
         "scf_iterations": hf_mix.numberOfConvergenceLoops(),
         "energy": e_mix,
@@ -187,9 +191,11 @@ def main():
         "computeF_ms": {a.precision: k_mix, "fp64": k64},
         "sweep_ms_per_scf": hf_mix.t_sweep_ms,
         "solve_ms_per_scf": hf_mix.t_solve_ms,
-        "sweep_roofline": hbm_roofline(n * npair * 8 + n * nb * 8, hf_mix.t_sweep_ms / max(1, hf_mix.numberOfConvergenceLoops()),
-                                       "hf_sweep_kernel: one read of the packed FP64 F and of A per iteration; time "
-                                       "per iteration as the SCF measures it (D upload, sweep, G download, sync)"),
+        "sweep_roofline": hbm_roofline(n * npair * (4 if fbytes32 else 8) + n * nb * 8,
+                                       hf_mix.t_sweep_ms / max(1, hf_mix.numberOfConvergenceLoops()),
+                                       "hf_sweep_kernel: one read of the packed %s F and of A per iteration; time "
+                                       "per iteration as the SCF measures it (D upload, sweep, G download, sync)"
+                                       % ("FP32" if fbytes32 else "FP64")),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "tc_ne": tc_res,

@@ -58,19 +58,44 @@ __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F,
 #pragma unroll
   for (int t = 0; t < KG; ++t) gacc[t] = 0;
   const int tri_i = i * (i + 1) / 2;
+  // F rows of a wave's step (points pb .. pb + PPW - 1 of chunk c0) travel
+  // through registers: the next step's rows are loaded while this step computes.
+  // (nb <= 32; the NBP = 64 build keeps synchronous staging, which needs no
+  // extra registers)
+  constexpr bool PF = NBP == 32;
+  constexpr int NL = PF ? (PPW * (NBP * (NBP + 1) / 2) + 63) / 64 : 1;
+  const int64_t rowlen = PPW * npair;
+  TF pre[NL];
+  auto load_rows = [&](int64_t cc0, int ccn, int cpb) {
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+      const int64_t e = lane + 64 * t;
+      const int qq = e >= npair ? 1 : 0;
+      if (e < rowlen && cpb + qq < ccn) pre[t] = F[(cc0 + cpb) * npair + e];
+    }
+  };
+  auto chunk_len = [&](int64_t cc0) { return (int)((cc0 + CH < p1) ? CH : p1 - cc0); };
+  if (PF && p0 < p1 && wave * PPW < chunk_len(p0)) load_rows(p0, chunk_len(p0), wave * PPW);
   __syncthreads();
 
   for (int64_t c0 = p0; c0 < p1; c0 += CH) {
-    const int cn = (int)((c0 + CH < p1) ? CH : p1 - c0);
+    const int cn = chunk_len(c0);
     for (int pb = wave * PPW; pb < cn; pb += 4 * PPW) {
-      // stage F rows and A rows of points pb .. pb+PPW-1
+      // F rows of points pb .. pb+PPW-1 from registers into LDS, then the
+      // next step's rows into registers
       double* fr = Frow + (int64_t)wave * PPW * npair;
-      for (int qq = 0; qq < PPW; ++qq) {
-        const int pt = pb + qq;
-        if (pt < cn) {
-          const TF* src = F + (c0 + pt) * npair;
-          for (int64_t p = lane; p < npair; p += 64) fr[qq * npair + p] = (double)src[p];
+      if constexpr (PF) {
+#pragma unroll
+        for (int t = 0; t < NL; ++t) {
+          const int64_t e = lane + 64 * t;
+          const int qq = e >= npair ? 1 : 0;
+          if (e < rowlen && pb + qq < cn) fr[e] = (double)pre[t];
         }
+        if (pb + 4 * PPW < cn) load_rows(c0, cn, pb + 4 * PPW);
+        else if (c0 + CH < p1 && wave * PPW < chunk_len(c0 + CH)) load_rows(c0 + CH, chunk_len(c0 + CH), wave * PPW);
+      } else {
+        for (int64_t e = lane; e < rowlen; e += 64)
+          if (pb + (e >= npair ? 1 : 0) < cn) fr[e] = (double)F[(c0 + pb) * npair + e];
       }
       const int pt = pb + q;
       const bool live = pt < cn && i < nb;
