@@ -27,18 +27,6 @@ double wall_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-struct Pinned {
-  double* p = nullptr;
-  explicit Pinned(size_t n) {
-    if (n && hipHostMalloc((void**)&p, n * sizeof(double), hipHostMallocDefault) != hipSuccess) p = nullptr;
-  }
-  ~Pinned() {
-    if (p) (void)hipHostFree(p);
-  }
-  Pinned(const Pinned&) = delete;
-  Pinned& operator=(const Pinned&) = delete;
-};
-
 struct Instance {
   const tcscf_scf_item* item;
   int nspin = 1;

@@ -66,7 +66,9 @@ __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F,
   constexpr int NL = PF ? (PPW * (NBP * (NBP + 1) / 2) + 63) / 64 : 1;
   const int64_t rowlen = PPW * npair;
   TF pre[NL];
+  double apre = 0;  // this lane's A value of the prefetched step
   auto load_rows = [&](int64_t cc0, int ccn, int cpb) {
+    if (cpb + q < ccn && i < nb) apre = A[(cc0 + cpb + q) * nb + i];
 #pragma unroll
     for (int t = 0; t < NL; ++t) {
       const int64_t e = lane + 64 * t;
@@ -84,7 +86,9 @@ __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F,
       // F rows of points pb .. pb+PPW-1 from registers into LDS, then the
       // next step's rows into registers
       double* fr = Frow + (int64_t)wave * PPW * npair;
+      double acur = 0;
       if constexpr (PF) {
+        acur = apre;
 #pragma unroll
         for (int t = 0; t < NL; ++t) {
           const int64_t e = lane + 64 * t;
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F,
       const bool live = pt < cn && i < nb;
       const int64_t n = c0 + pt;
       double* ab = Abuf + (int64_t)(live ? pt : 0) * nb;
-      if (live) ab[i] = A[n * nb + i];
+      if (live) ab[i] = PF ? acur : A[n * nb + i];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -335,19 +339,29 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
   HfPlan plan;
   if ((rc = hf_prepare(plan, nspin, nb, n1, F, f_dtype == TCSCF_F32, A1, w1, ws))) return rc;
   HfIter iter(nspin, nocc, nb, S, H, o);
-  std::vector<double> G(nspin * nb2);
+  // D up and G down through page-locked buffers (pageable vectors if that
+  // allocation fails)
+  Pinned pin(2 * (size_t)nspin * nb2);
+  std::vector<double> Dv, Gv;
+  if (!pin.p) {
+    Dv.resize(nspin * nb2);
+    Gv.resize(nspin * nb2);
+  }
+  double* Dh = pin.p ? pin.p : Dv.data();
+  double* Gh = pin.p ? pin.p + nspin * nb2 : Gv.data();
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
   res->iterations = 0;
   res->t_sweep_ms = res->t_solve_ms = 0;
   for (int it = 0; it < o->max_iter; ++it) {
     double t0 = now_ms();
-    if ((rc = hf_enqueue(plan, iter.D.data(), G.data(), st))) return rc;
+    std::copy(iter.D.begin(), iter.D.end(), Dh);
+    if ((rc = hf_enqueue(plan, Dh, Gh, st))) return rc;
     TC_HIP(hipStreamSynchronize(st));
     double t1 = now_ms();
     res->t_sweep_ms += t1 - t0;
     bool done = false;
-    if ((rc = iter.step(G.data(), it, res, trace, &done))) return rc;
+    if ((rc = iter.step(Gh, it, res, trace, &done))) return rc;
     res->t_solve_ms += now_ms() - t1;
     if (done) break;
   }

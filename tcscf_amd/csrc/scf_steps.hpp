@@ -16,6 +16,20 @@
 
 namespace tc {
 
+// Page-locked host staging for the per-iteration uploads and downloads
+// (hipHostMalloc); p is null when the allocation failed.
+struct Pinned {
+  double* p = nullptr;
+  explicit Pinned(size_t n) {
+    if (n && hipHostMalloc((void**)&p, n * sizeof(double), hipHostMallocDefault) != hipSuccess) p = nullptr;
+  }
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  Pinned(const Pinned&) = delete;
+  Pinned& operator=(const Pinned&) = delete;
+};
+
 /* ---------------------------------------------------------------- RCS / UOS */
 struct HfPlan {
   int nspin, nb;

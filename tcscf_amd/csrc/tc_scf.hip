@@ -935,7 +935,15 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
   TcPlan plan;
   if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, f_dtype, jp, ws))) return rc;
   TcIter iter(nup, ndown, nb, S, H, o);
-  std::vector<double> out(tc_out_doubles(nb)), G2(2 * nb2), G3(2 * nb2);
+  std::vector<double> G2(2 * nb2), G3(2 * nb2);
+  // D and C up, the result block down, through page-locked buffers (pageable
+  // vectors if that allocation fails)
+  const size_t nD = iter.D.size(), nC = iter.C.size(), nO = tc_out_doubles(nb);
+  Pinned pin(nD + nC + nO);
+  std::vector<double> stage;
+  if (!pin.p) stage.resize(nD + nC + nO);
+  double* Dh = pin.p ? pin.p : stage.data();
+  double *Ch = Dh + nD, *outh = Ch + nC;
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
   res->iterations = 0;
@@ -943,9 +951,11 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
   for (int it = 0; it < o->max_iter; ++it) {
     double t0 = now_ms();
     const bool restricted = tc_restricted(plan, iter.D.data(), iter.C.data());
-    if ((rc = tc_enqueue(plan, iter.D.data(), iter.C.data(), out.data(), restricted, st))) return rc;
+    std::copy(iter.D.begin(), iter.D.end(), Dh);
+    std::copy(iter.C.begin(), iter.C.end(), Ch);
+    if ((rc = tc_enqueue(plan, Dh, Ch, outh, restricted, st))) return rc;
     TC_HIP(hipStreamSynchronize(st));
-    tc_assemble(plan, out.data(), restricted, G2.data(), G3.data());
+    tc_assemble(plan, outh, restricted, G2.data(), G3.data());
     double t1 = now_ms();
     res->t_sweep_ms += t1 - t0;
     bool done = false;
