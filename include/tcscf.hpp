@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <complex>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -515,6 +516,166 @@ std::vector<REAL> integrateAllR1R2(const integration::QMCGrid<REAL, N1>& r1Grid,
         for (int b4 = 0; b4 < nb; ++b4)
           out[((static_cast<std::size_t>(b1) * nb + b2) * nb + b3) * nb + b4] = e[pk(b1, b3) * np + pk(b2, b4)];
   return out;
+}
+
+/* ------------------------------------------- per-element QMC route (AtomicBasis) */
+namespace detail {
+/* fillTwoElectronSymmetricHermitianArray (src/mathFunctions.hpp:367-433): value
+ * on the canonical quadruples only, copied to their images; A (nb^4) accumulates. */
+template <class F>
+void fillTwoElectron(int N, bool real, F value, std::vector<std::complex<double>>& A) {
+  auto at = [&](int a, int b, int c, int d) -> std::complex<double>& {
+    return A[((static_cast<std::size_t>(a) * N + b) * N + c) * N + d];
+  };
+  for (int a = 0; a < N; ++a)
+    for (int b = a; b < N; ++b)
+      for (int c = a; c < N; ++c)
+        for (int d = real ? b : a; d < N; ++d) {
+          const std::complex<double> v = value(a, b, c, d), vc = std::conj(v);
+          const bool sym = a != b, herm = a != c && a != d, rl = real && d != b;
+          at(a, b, c, d) += v;
+          if (sym) at(b, a, d, c) += v;
+          if (herm || (real && a != c)) at(c, d, a, b) += vc;
+          if (herm && sym) at(d, c, b, a) += vc;
+          if (rl) at(a, d, c, b) += v;
+          if (rl && sym) at(d, a, b, c) += v;
+          if (rl && herm) at(c, b, a, d) += v;
+          if (rl && herm && sym) at(b, c, d, a) += v;
+        }
+}
+
+/* Every quadruple of tcscf_eri_qmc6d (ndim 6, real) or tcscf_eri_qmc5d_atomic
+ * (ndim 5, the reference's complex functions) for basis f, as a complex nb^4
+ * array; for ndim 5 missing -m partners are tabulated alongside and dropped. */
+inline std::vector<std::complex<double>> qmcQuadruples(std::vector<tcscf_basis_fn> f, int ndim, IndexType minPoints,
+                                                       int nshifts, std::uint64_t seed, hipStream_t stream) {
+  const int n0 = static_cast<int>(f.size());
+  std::vector<int> partner(f.size());
+  if (ndim == 5) {
+    auto same = [](const tcscf_basis_fn& x, const tcscf_basis_fn& y, int m) {
+      return x.kind == y.kind && x.n == y.n && x.l == y.l && y.m == m && x.alpha == y.alpha;
+    };
+    for (std::size_t a = 0; a < f.size(); ++a) {
+      if (f[a].m == 0) {
+        partner[a] = static_cast<int>(a);
+        continue;
+      }
+      std::size_t q = 0;
+      while (q < f.size() && !same(f[a], f[q], -f[a].m)) ++q;
+      if (q == f.size()) {
+        tcscf_basis_fn g = f[a];
+        g.m = -g.m;
+        f.push_back(g);
+        partner.push_back(0);
+      }
+      partner[a] = static_cast<int>(q);
+    }
+    for (std::size_t a = static_cast<std::size_t>(n0); a < f.size(); ++a) {
+      std::size_t q = 0;
+      while (!same(f[a], f[q], -f[a].m)) ++q;
+      partner[a] = static_cast<int>(q);
+    }
+  }
+  const int nb = static_cast<int>(f.size());
+  const int64_t n = tcscf_qmc_lattice_npts(minPoints, nshifts);
+  if (n < 0) check(static_cast<int>(n));
+  std::vector<double> p1(3 * n), p2(3 * n), w(n);
+  const int64_t got = tcscf_qmc_spherical_points(minPoints, ndim, nshifts, seed, p1.data(), p2.data(), w.data());
+  if (got < 0) check(static_cast<int>(got));
+  DeviceArray<double> P1, P2, W, A1(static_cast<std::size_t>(n * nb)), A2(static_cast<std::size_t>(n * nb));
+  P1.upload(p1.data(), p1.size());
+  P2.upload(p2.data(), p2.size());
+  W.upload(w.data(), w.size());
+  check(tcscf_basis_eval_fns(P1.data(), n, f.data(), nb, A1.data(), nullptr, stream));
+  check(tcscf_basis_eval_fns(P2.data(), n, f.data(), nb, A2.data(), nullptr, stream));
+  std::vector<std::complex<double>> out(static_cast<std::size_t>(n0) * n0 * n0 * n0);
+  auto o = [&](int a, int b, int c, int d) -> std::complex<double>& {
+    return out[((static_cast<std::size_t>(a) * n0 + b) * n0 + c) * n0 + d];
+  };
+  if (ndim == 6) {
+    const IndexType np = static_cast<IndexType>(nb) * (nb + 1) / 2;
+    DeviceArray<double> E(static_cast<std::size_t>(np * np));
+    check(tcscf_eri_qmc6d(P1.data(), P2.data(), W.data(), n, A1.data(), A2.data(), nb, E.data(), stream));
+    const std::vector<double> e = E.download();
+    auto pk = [](int j, int i) { return j <= i ? static_cast<IndexType>(i) * (i + 1) / 2 + j : static_cast<IndexType>(j) * (j + 1) / 2 + i; };
+    for (int a = 0; a < n0; ++a)
+      for (int b = 0; b < n0; ++b)
+        for (int c = 0; c < n0; ++c)
+          for (int d = 0; d < n0; ++d) o(a, b, c, d) = e[pk(a, c) * np + pk(b, d)];
+    return out;
+  }
+  std::vector<int> ms(f.size());
+  for (std::size_t a = 0; a < f.size(); ++a) ms[a] = f[a].m;
+  DeviceArray<double> E(static_cast<std::size_t>(nb) * nb * nb * nb * 2);
+  check(tcscf_eri_qmc5d_atomic(P1.data(), P2.data(), W.data(), n, A1.data(), A2.data(), nb, ms.data(), partner.data(),
+                               E.data(), stream));
+  const std::vector<double> e = E.download();
+  for (int a = 0; a < n0; ++a)
+    for (int b = 0; b < n0; ++b)
+      for (int c = 0; c < n0; ++c)
+        for (int d = 0; d < n0; ++d) {
+          const std::size_t k = ((static_cast<std::size_t>(a) * nb + b) * nb + c) * nb + d;
+          o(a, b, c, d) = {e[2 * k], e[2 * k + 1]};
+        }
+  return out;
+}
+}  // namespace detail
+
+/* fillR12Array (src/AtomicBasis.hpp:167-177) with r12MatrixElement (:30-48):
+ * every (ab|cd) as one 6-D QMC integral over the point set of
+ * tcscf_qmc_spherical_points(ndim = 6), real basis, filled as the reference
+ * fills it (canonical quadruples copied to their images); nb^4 row-major. */
+template <class B>
+std::vector<double> fillR12Array(const std::vector<B>& basis, IndexType minPoints, int nshifts = 2,
+                                 std::uint64_t seed = 0, hipStream_t stream = nullptr) {
+  const auto E = detail::qmcQuadruples(detail::abi(basis), 6, minPoints, nshifts, seed, stream);
+  const int N = static_cast<int>(basis.size());
+  std::vector<std::complex<double>> A(E.size());
+  detail::fillTwoElectron(N, true, [&](int a, int b, int c, int d) {
+    return E[((static_cast<std::size_t>(a) * N + b) * N + c) * N + d]; }, A);
+  std::vector<double> out(A.size());
+  for (std::size_t k = 0; k < A.size(); ++k) out[k] = A[k].real();
+  return out;
+}
+
+/* fillAtomicR12Array (src/AtomicBasis.hpp:197-248): the 5-D Coulomb integrals
+ * ((l, m) of a, c and of b, d equal) and, elsewhere, the exchange integrals
+ * ((l, m) of a, d and of b, c equal) of the reference's complex functions,
+ * 0 otherwise; complex nb^4 row-major. */
+template <class B>
+std::vector<std::complex<double>> fillAtomicR12Array(const std::vector<B>& basis, IndexType minPoints,
+                                                     int nshifts = 2, std::uint64_t seed = 0,
+                                                     hipStream_t stream = nullptr) {
+  const auto E = detail::qmcQuadruples(detail::abi(basis), 5, minPoints, nshifts, seed, stream);
+  const int N = static_cast<int>(basis.size());
+  auto e = [&](int a, int b, int c, int d) { return E[((static_cast<std::size_t>(a) * N + b) * N + c) * N + d]; };
+  auto lm = [&](int a, int b) { return basis[a].l == basis[b].l && basis[a].m == basis[b].m; };
+  std::vector<std::complex<double>> A(E.size());
+  detail::fillTwoElectron(N, true, [&](int a, int b, int c, int d) {
+    return (lm(a, c) && lm(b, d)) ? std::complex<double>(e(a, b, c, d).real()) : std::complex<double>(0.0); }, A);
+  detail::fillTwoElectron(N, false, [&](int a, int b, int c, int d) {
+    return (lm(a, d) && lm(b, c) && !(lm(a, c) && lm(b, d))) ? e(a, b, c, d) : std::complex<double>(0.0); }, A);
+  return A;
+}
+
+/* atomicCoulombOperator (src/AtomicBasis.hpp:59-98); b1, b3 and b2, b4 must share (l, m). */
+template <class B>
+double atomicCoulombOperator(const B& b1, const B& b2, const B& b3, const B& b4, IndexType minPoints,
+                             int nshifts = 2, std::uint64_t seed = 0, hipStream_t stream = nullptr) {
+  if (b1.l != b3.l || b1.m != b3.m || b2.l != b4.l || b2.m != b4.m)
+    throw Error("atomicCoulombOperator: b1, b3 and b2, b4 need equal (l, m)");
+  const auto E = detail::qmcQuadruples({b1.abi(), b2.abi(), b3.abi(), b4.abi()}, 5, minPoints, nshifts, seed, stream);
+  return E[((0 * 4 + 1) * 4 + 2) * 4 + 3].real();
+}
+
+/* atomicExchangeOperator (src/AtomicBasis.hpp:103-140); b1, b4 and b2, b3 must share (l, m). */
+template <class B>
+std::complex<double> atomicExchangeOperator(const B& b1, const B& b2, const B& b3, const B& b4, IndexType minPoints,
+                                            int nshifts = 2, std::uint64_t seed = 0, hipStream_t stream = nullptr) {
+  if (b1.l != b4.l || b1.m != b4.m || b2.l != b3.l || b2.m != b3.m)
+    throw Error("atomicExchangeOperator: b1, b4 and b2, b3 need equal (l, m)");
+  const auto E = detail::qmcQuadruples({b1.abi(), b2.abi(), b3.abi(), b4.abi()}, 5, minPoints, nshifts, seed, stream);
+  return E[((0 * 4 + 1) * 4 + 2) * 4 + 3];
 }
 
 /* threeElectronIntegrals (src/integration/integrateAll.hpp:369-442), same-spin

@@ -168,6 +168,20 @@ static int gpu_runs(const std::string& dir) {
     tcscf::TCHartreeFock<double> tcu(2, 2, static_cast<int>(basis.size()));
     std::printf("be_tc_ochi %.15f\n", tcu.compute(S, H, g1, Fsu, Fou, Vsu, u, true, 1e-10, 200, 1));
   }
+  // the per-element QMC route (AtomicBasis): 5-D atomic and 6-D arrays, hydrogen-like Z = 2
+  {
+    using HL = tcscf::HydrogenLikeBasisFunction<double>;
+    const std::vector<HL> bz = {HL(2, 1, 0, 0), HL(2, 2, 0, 0), HL(2, 2, 1, 0), HL(2, 2, 1, 1)};
+    const auto A5 = tcscf::fillAtomicR12Array(bz, 20000);
+    const auto A6 = tcscf::fillR12Array(bz, 20000);
+    auto ix = [](int a, int b, int c, int d) { return ((a * 4 + b) * 4 + c) * 4 + d; };
+    std::printf("atomic_j_2p0_2p1 %.15e\n", A5[ix(2, 3, 2, 3)].real());
+    std::printf("atomic_k_2p0_2p1_re %.15e\natomic_k_2p0_2p1_im %.15e\n", A5[ix(2, 3, 3, 2)].real(),
+                A5[ix(2, 3, 3, 2)].imag());
+    std::printf("r12_0101 %.15e\n", A6[ix(0, 1, 0, 1)]);
+    std::printf("coulomb_op %.15e\n", tcscf::atomicCoulombOperator(bz[2], bz[3], bz[2], bz[3], 20000));
+    std::printf("exchange_op_re %.15e\n", tcscf::atomicExchangeOperator(bz[2], bz[3], bz[3], bz[2], 20000).real());
+  }
   return 0;
 }
 

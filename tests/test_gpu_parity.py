@@ -715,6 +715,16 @@ def test_cpp_facade_on_device(tc, O, gpu):
     assert abs(out["be_tc_mixed"] - etc) < 1e-6
     # the general Ochi path with S = {(1, 0, 0)}: the same kernel up to rounding
     assert abs(out["be_tc_ochi"] - etc) < 1e-9
+    # the facade's per-element QMC route gives the Python route's numbers (same points, same kernels)
+    H = tc.HydrogenLikeBasisFunction
+    bz = [H(2, 1, 0, 0), H(2, 2, 0, 0), H(2, 2, 1, 0), H(2, 2, 1, 1)]
+    A5, A6 = tc.fillAtomicR12Array(bz, 20000), tc.fillR12Array(bz, 20000)
+    for key, val in [("atomic_j_2p0_2p1", A5[2, 3, 2, 3].real), ("atomic_k_2p0_2p1_re", A5[2, 3, 3, 2].real),
+                     ("r12_0101", A6[0, 1, 0, 1]),
+                     ("coulomb_op", tc.atomicCoulombOperator(bz[2], bz[3], bz[2], bz[3], 20000)),
+                     ("exchange_op_re", tc.atomicExchangeOperator(bz[2], bz[3], bz[3], bz[2], 20000).real)]:
+        assert abs(out[key] - val) < 1e-12 * abs(val), (key, out[key], val)
+    assert abs(out["atomic_k_2p0_2p1_im"] - A5[2, 3, 3, 2].imag) < 1e-14
 
 
 def test_batched_contraction_matches_single_calls(tc, gpu):
