@@ -671,6 +671,9 @@ size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1) {
          (size_t)us[6] * nb2 + (size_t)kBlocks * 4 * npair + tc_out_doubles(nb);
 }
 
+// waves per workgroup of the unstaged point kernel (halved while the LDS does not fit)
+constexpr int kUnstagedWaves = 8;
+
 int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1,
                int64_t n1, const void* Fs, const void* Fo, const void* V, int f_dtype, const tcscf_jastrow* jp,
                double* ws) {
@@ -716,8 +719,9 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
     p.stage[r] = shared + 8 * staged_wave <= 160 * 1024 ? 1 : 0;
     const size_t per_wave = p.stage[r] ? staged_wave : sizeof(double) * point_wave_elems(nb, ns);
     // staged: 8 waves per workgroup, 12 for the one-spin kernel on FP32 rows
-    // (three waves per SIMD fit its registers) when LDS holds them
-    p.wpw[r] = p.stage[r] ? (ns == 1 && p.f32 && shared + 12 * per_wave <= 160 * 1024 ? 12 : 8) : 4;
+    // (three waves per SIMD fit its registers) when LDS holds them; unstaged:
+    // 8, so that one copy of D and C in LDS serves more waves
+    p.wpw[r] = p.stage[r] ? (ns == 1 && p.f32 && shared + 12 * per_wave <= 160 * 1024 ? 12 : 8) : kUnstagedWaves;
     while (p.wpw[r] > 1 && shared + p.wpw[r] * per_wave > 160 * 1024) p.wpw[r] /= 2;
     p.sh1[r] = shared + p.wpw[r] * per_wave;
     TC_CHECK(p.sh1[r] <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
