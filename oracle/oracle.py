@@ -57,6 +57,14 @@ def get_threads():
     return lib().orc_get_threads()
 
 
+def set_plant(p):
+    """Plant a mistake in the TC Fock build (tests only; 0 removes it):
+    1 transposed operand in the Vja term, 2 the same spin's density against
+    V_oppo in VaaSamePlusOppo, 3 wrong sign of the antisymmetrization, 4 Vai
+    from the other spin's density."""
+    lib().orc_set_plant(int(p))
+
+
 def chebyshev_gauss(n):
     x = np.zeros(n)
     w = np.zeros(n)

@@ -23,6 +23,9 @@
  *                      TC(c=0)==UOS), He single-zeta closed form
  *   eigen-solvers   -> residual checks; two independent algorithms
  *   threeElectron/computeQ/assembleG -> cross-checked against each other only
+ *   TC two-body vector term -> the Fock matrix of the operator grad_1 u . grad_1
+ *                      from its four-index integrals over point pairs (grad u
+ *                      from the Jastrow's closed form), direct minus exchange
  */
 #include <algorithm>
 #include <cmath>
@@ -45,6 +48,10 @@ constexpr double kLn2 = 0.693147180559945309417232121458176568;
 #include "../tcscf_amd/data/lebedev_tables.inc"
 
 thread_local std::string g_err;
+
+/* Planted mistakes for the tests that show a check can fail (0 = none).  Set
+ * only by orc_set_plant; read by tc_fock_build. */
+int g_plant = 0;
 
 inline int64_t pidx(int j, int i) { return (int64_t)i * (i + 1) / 2 + j; }  // j <= i
 
@@ -105,6 +112,7 @@ extern "C" {
 const char* orc_last_error() { return g_err.c_str(); }
 
 void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+void orc_set_plant(int p) { g_plant = p; }
 int orc_get_threads() { return omp_get_max_threads(); }
 
 void orc_chebyshev_gauss(int n, double* x, double* w) { chebyshev_gauss(n, x, w); }
@@ -1092,7 +1100,7 @@ static void tc_fock_build(const int ne[2], int nb, const double* A1, const doubl
       for (int j = 0; j < nb; ++j)
         for (int i = 0; i < nb; ++i) {
           faa += Fo[(k * nb + j) * nb + i] * Db[i * nb + j];
-          for (int c = 0; c < 3; ++c) vaa[c] += V3(Vo, k, j, i)[c] * Db[i * nb + j];
+          for (int c = 0; c < 3; ++c) vaa[c] += V3(Vo, k, j, i)[c] * (g_plant == 2 ? Ds : Db)[i * nb + j];
         }
       FaaSPO[s * n1 + k] = faa;
       for (int c = 0; c < 3; ++c) VaaSPO[(s * n1 + k) * 3 + c] = vaa[c];
@@ -1109,8 +1117,9 @@ static void tc_fock_build(const int ne[2], int nb, const double* A1, const doubl
       for (int i = 0; i < nb; ++i) vai[i] = 0;
       for (int j = 0; j < nb; ++j) {
         double ks[3] = {0, 0, 0};
+        const double* Dv = g_plant == 4 ? Db : Ds;
         for (int q = 0; q < nb; ++q)
-          for (int c = 0; c < 3; ++c) ks[c] += Ds[q * nb + j] * dA[q * 3 + c];
+          for (int c = 0; c < 3; ++c) ks[c] += Dv[q * nb + j] * dA[q * 3 + c];
         for (int i = 0; i < nb; ++i) {
           const double* v = V3(Vs, k, j, i);
           vai[i] += v[0] * ks[0] + v[1] * ks[1] + v[2] * ks[2];
@@ -1222,7 +1231,10 @@ static void tc_fock_build(const int ne[2], int nb, const double* A1, const doubl
         double va = A[j] * (dot3(vaa, dA + i * 3) - Vai[(s * n1 + k) * nb + i]);
         va += dot3(vsji, &sg[(s * n1 + k) * 3]);
         va += dot3(voji, &sg[(sb * n1 + k) * 3]);
-        va -= dot3(&Vja[((s * n1 + k) * nb + j) * 3], dA + i * 3);
+        if (g_plant == 1)
+          va -= dot3(&Vja[((s * n1 + k) * nb + i) * 3], dA + j * 3);
+        else
+          va -= dot3(&Vja[((s * n1 + k) * nb + j) * 3], dA + i * 3);
         vector += w1[k] * va;
         const double* vtsi_i = &VTSI[((s * n1 + k) * nb + i) * 3];
         const double* vtsi_j = &VTSI[((s * n1 + k) * nb + j) * 3];
@@ -1247,7 +1259,8 @@ static void tc_fock_build(const int ne[2], int nb, const double* A1, const doubl
       G3[s * nb2 + ji] = scalar3 / 2;
     }
     for (int j = 0; j < nb; ++j)
-      for (int i = 0; i < nb; ++i) G2[s * nb2 + j * nb + i] += VC[s * nb2 + j * nb + i] - VC[s * nb2 + i * nb + j];
+      for (int i = 0; i < nb; ++i)
+        G2[s * nb2 + j * nb + i] += VC[s * nb2 + j * nb + i] - (g_plant == 3 ? -1 : 1) * VC[s * nb2 + i * nb + j];
   }
 }
 

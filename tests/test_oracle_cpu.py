@@ -535,6 +535,109 @@ def test_factorized_G3_equals_explicit_Q(O, nocc):
     assert rel_err(G3x, G3q) > 1e-3
 
 
+def _ochi_grad1_closed_form(terms, a, a12, same, p1, p2):
+    """grad_1 u(r1, r2) for every point pair (n1, n2, 3), by the chain rule on
+    u = sum_t c_t rb12^i rb1^j rb2^k with rb = r / (r + a) (rb12 with a12):
+    du/dr12 (r1 - r2) / r12 + du/dr1 r1 / |r1|."""
+    d = p1.T[:, None, :] - p2.T[None, :, :]
+    r12 = np.linalg.norm(d, axis=-1)
+    r1 = np.linalg.norm(p1.T, axis=-1)[:, None]
+    r2 = np.linalg.norm(p2.T, axis=-1)[None, :]
+    t12, t1, t2 = r12 / (r12 + a12), r1 / (r1 + a), r2 / (r2 + a)
+    du12 = np.zeros_like(r12)
+    du1 = np.zeros_like(r12)
+    for (i, j, k, c_oppo, c_same) in terms:
+        c = c_same if same else c_oppo
+        if i:
+            du12 += c * i * t12 ** (i - 1) * a12 / (r12 + a12) ** 2 * t1 ** j * t2 ** k
+        if j:
+            du1 += c * j * t1 ** (j - 1) * a / (r1 + a) ** 2 * t12 ** i * t2 ** k
+    return du12[..., None] * d / r12[..., None] + (du1 / r1)[..., None] * p1.T[:, None, :]
+
+
+@pytest.mark.parametrize("jastrow", ["r12_ratio2", "general"])
+def test_tc_two_body_vector_term_brute_force(O, jastrow):
+    """The two-body gradient term of the TC Fock build (HartreeFock.cpp:1168-1181,
+    antisymmetrized at :1215-1222) restated without the per-point
+    intermediates: the mean field of the operator grad_1 u(1, 2) . grad_1 over
+    its four-index integrals
+        M_X[a, b, c, d] = sum_{n1, n3} w1 w3 A_a(1) A_b(3) g_X(1, 3) . grad A_c(1) A_d(3)
+    with g_X = grad_1 u from the Jastrow's closed form per point pair (not
+    from V).  With (j, i) on electron 1: direct over both spins' densities
+    (same-spin kernel for D_s, opposite-spin for D_s') minus same-spin
+    exchange; with (j, i) on electron 2 likewise:
+        vec_s(j,i) = sum_kl D_s(l,k) [M_S(j,k,i,l) - M_S(j,k,l,i) + M_S(k,j,l,i) - M_S(k,j,i,l)]
+                   + sum_kl D_s'(l,k) [M_O(j,k,i,l) + M_O(k,j,l,i)]
+    and G2 = vec/2 - vec^T/2 with F = 0.  (The terms with (j, i) on electron 2
+    and the gradient on the density, V(j,i) . scaledGradients at :1172-1174,
+    are symmetric in (j, i) and cancel in that antisymmetrization.)  Cases:
+    the reference's r12 Jastrow
+    (c_oppo = 2 c_same) and a general Ochi Jastrow with r1 / r12 r2 terms, whose
+    V_same and V_oppo are independent.  Each planted mistake in the oracle's
+    factorized build (a transposed operand, a wrong spin, a wrong sign, the
+    wrong density) fails the comparison."""
+    fns = [(1, 0, 0, 1.3), (2, 1, 0, 0.9), (2, 0, 0, 0.7), (2, 1, 1, 1.1)]
+    pts1, w1 = O.product_grid(4, 7)
+    pts3, w3 = O.product_grid(5, 5)
+    A1, dA1 = O.sto_real(pts1, fns, grads=True)
+    A3 = O.sto_real(pts3, fns)
+    if jastrow == "r12_ratio2":
+        a, a12, terms = 1.0, 1.0, [(1, 0, 0, 0.5, 0.25)]
+    else:
+        a, a12, terms = 0.8, 1.3, [(1, 0, 0, 0.5, 0.25), (0, 1, 0, -0.3, 0.2), (1, 0, 1, 0.15, -0.4),
+                                   (2, 2, 0, 0.1, 0.35)]
+    prm = O.ochi_params(a, a12, terms)
+    Vs = O.compute_F(pts1, pts3, w3, A3, kernel=O.K_JASTROW_GRAD, prm=prm, same=True)
+    Vo = O.compute_F(pts1, pts3, w3, A3, kernel=O.K_JASTROW_GRAD, prm=prm, same=False)
+    nb = len(fns)
+    nocc = (2, 1)
+    rng = np.random.default_rng(11)
+    C = np.zeros((2, nb, 2))
+    D = np.zeros((2, nb, nb))
+    for s in range(2):
+        C[s, :, :nocc[s]] = rng.normal(size=(nb, nocc[s])) * 0.5
+        D[s] = C[s, :, :nocc[s]] @ C[s, :, :nocc[s]].T
+
+    M = {}
+    for same in (True, False):
+        g = _ochi_grad1_closed_form(terms, a, a12, same, pts1, pts3)
+        M[same] = np.einsum("pa,qb,pqx,pcx,qd,p,q->abcd", A1, A3, g, dA1, A3, w1, w3, optimize=True)
+    # the closed-form kernel reproduces the oracle's V (so g is the same function)
+    assert rel_err(np.einsum("pjiq,->pjiq", Vs, 1.0),
+                   np.einsum("qj,qi,pqx,q->pjix", A3, A3, _ochi_grad1_closed_form(terms, a, a12, True, pts1, pts3),
+                             w3)) < 1e-13
+    if jastrow == "general":  # V_oppo is not one multiple of V_same
+        ratio = np.sum(Vo * Vs) / np.sum(Vs * Vs)
+        assert rel_err(Vo, ratio * Vs) > 0.05
+    MS, MO = M[True], M[False]
+    G2b = np.zeros((2, nb, nb))
+    for s in range(2):
+        Ds, Db = D[s], D[1 - s]
+        vec = (np.einsum("lk,jkil->ji", Ds, MS) - np.einsum("lk,jkli->ji", Ds, MS)
+               + np.einsum("lk,kjli->ji", Ds, MS) - np.einsum("lk,kjil->ji", Ds, MS)
+               + np.einsum("lk,jkil->ji", Db, MO) + np.einsum("lk,kjli->ji", Db, MO))
+        G2b[s] = vec / 2 - vec.T / 2
+    Z = np.zeros((A1.shape[0], nb, nb))
+    G2, _ = O.tc_fock(*nocc, A1, dA1, w1, Z, Z, Vs, Vo, D, C)
+    assert np.max(np.abs(G2b)) > 1e-3
+    assert rel_err(G2, G2b) < 1e-12
+    # the vector term alone is antisymmetric
+    assert np.max(np.abs(G2 + G2.transpose(0, 2, 1))) < 1e-13 * np.max(np.abs(G2))
+    # planted mistakes in the oracle, and swapped / sign-flipped inputs, fail
+    try:
+        for plant in (1, 2, 3, 4):
+            O.set_plant(plant)
+            G2x, _ = O.tc_fock(*nocc, A1, dA1, w1, Z, Z, Vs, Vo, D, C)
+            assert rel_err(G2x, G2b) > 1e-3, plant
+    finally:
+        O.set_plant(0)
+    G2x, _ = O.tc_fock(*nocc, A1, dA1, w1, Z, Z, -Vs, -Vo, D, C)
+    assert rel_err(G2x, G2b) > 1e-3
+    if jastrow == "general":
+        G2x, _ = O.tc_fock(*nocc, A1, dA1, w1, Z, Z, Vo, Vs, D, C)
+        assert rel_err(G2x, G2b) > 1e-3
+
+
 def test_three_electron_integrals_are_the_unsymmetrized_Q(O):
     """threeElectronIntegrals (src/integration/integrateAll.hpp:369-442) is the
     one-term kernel T(p,q,r,s,t,u) = sum w A_p A_s V(q,t).V(r,u) whose three
