@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -41,6 +42,7 @@ struct Instance {
   bool active = true;
   bool restricted = false;  // TC: closed shell with identical spins, as captured in the graph
   int rc = 0;
+  std::string err;  // the message of a failed step (set on the worker thread that ran it)
 };
 
 // Runs f(k) for k in [0, n) on up to `threads` host threads.
@@ -253,10 +255,19 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
       }
       ++x.it;
       if (done || x.rc || x.it >= o->max_iter) x.active = false;
-      if (x.rc) r.status = x.rc;
+      if (x.rc) {
+        r.status = x.rc;
+        x.err = last_error();  // thread-local: read on the thread that set it
+      }
     });
     t_host += wall_ms() - t1;
   }
+  // the first failed instance's message reaches the caller's tcscf_last_error()
+  for (int k = 0; k < count; ++k)
+    if (inst[k].rc) {
+      set_error("tcscf_batch_scf: item " + std::to_string(k) + ": " + inst[k].err);
+      break;
+    }
   for (int k = 0; k < count; ++k) {
     Instance& x = inst[k];
     results[k].t_sweep_ms = t_dev;

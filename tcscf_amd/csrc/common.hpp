@@ -5,13 +5,18 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
+#include <set>
 #include <string>
+#include <utility>
 
 #include "../../include/tcscf_hip.h"
 
 namespace tc {
 
 void set_error(const std::string& msg);
+// The calling thread's last message (what tcscf_last_error returns).
+const std::string& last_error();
 
 inline int64_t npair_of(int nb) { return (int64_t)nb * (nb + 1) / 2; }
 
@@ -34,6 +39,27 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
       return TCSCF_EINVAL;                                                                 \
     }                                                                                      \
   } while (0)
+
+// LDS of one gfx950 workgroup.
+constexpr int kMaxLdsBytes = 160 * 1024;
+
+// Lets kernel fn launch with up to kMaxLdsBytes of dynamic LDS on the current
+// device.  hipFuncAttributeMaxDynamicSharedMemorySize is a property of the
+// kernel (per device, process-wide), not of one launch, so it is set once, to
+// the hardware maximum: set per plan, a later plan of the same kernel with a
+// smaller footprint would lower the cap below an earlier plan's launch size.
+inline hipError_t lds_opt_in(const void* fn) {
+  static std::mutex mu;
+  static std::set<std::pair<int, const void*>> done;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.count({dev, fn})) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes);
+  if (e == hipSuccess) done.insert({dev, fn});
+  return e;
+}
 
 // Device scratch owned by the library: grown on demand, never shrunk, one per
 // (device, slot).  Calls that share a slot on one device must be ordered on

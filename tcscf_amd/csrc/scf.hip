@@ -236,14 +236,14 @@ int hf_prepare(HfPlan& p, int nspin, int nb, int64_t n1, const void* F, int f32,
   p.nbp = nb <= 32 ? 32 : 64;
   const int ppw = 64 / p.nbp, ch = p.nbp == 64 ? 16 : 32;
   p.shmem = sizeof(double) * (nspin * nb2 + 4 * ppw * p.npair + ch * nb * (1 + nspin) + 4 * ppw * nspin * nb);
-  TC_CHECK(p.shmem <= 160 * 1024, "SCF sweep: nb too large for the LDS staging");
+  TC_CHECK(p.shmem <= (size_t)kMaxLdsBytes, "SCF sweep: nb too large for the LDS staging");
   if (p.shmem > 64 * 1024) {
     // opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU)
     const void* fns[8] = {(const void*)hf_sweep_kernel<float, 1, 32>,  (const void*)hf_sweep_kernel<float, 1, 64>,
                           (const void*)hf_sweep_kernel<double, 1, 32>, (const void*)hf_sweep_kernel<double, 1, 64>,
                           (const void*)hf_sweep_kernel<float, 2, 32>,  (const void*)hf_sweep_kernel<float, 2, 64>,
                           (const void*)hf_sweep_kernel<double, 2, 32>, (const void*)hf_sweep_kernel<double, 2, 64>};
-    for (const void* f : fns) TC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.shmem));
+    for (const void* f : fns) TC_HIP(lds_opt_in(f));
   }
   return 0;
 }

@@ -724,10 +724,8 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
     p.wpw[r] = p.stage[r] ? (ns == 1 && p.f32 && shared + 12 * per_wave <= 160 * 1024 ? 12 : 8) : kUnstagedWaves;
     while (p.wpw[r] > 1 && shared + p.wpw[r] * per_wave > 160 * 1024) p.wpw[r] /= 2;
     p.sh1[r] = shared + p.wpw[r] * per_wave;
-    TC_CHECK(p.sh1[r] <= 160 * 1024, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
-    if (p.sh1[r] > 64 * 1024)
-      TC_HIP(hipFuncSetAttribute(point_kernel(p.f32, p.stage[r], r), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)p.sh1[r]));
+    TC_CHECK(p.sh1[r] <= (size_t)kMaxLdsBytes, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
+    if (p.sh1[r] > 64 * 1024) TC_HIP(lds_opt_in(point_kernel(p.f32, p.stage[r], r)));
     p.pblocks[r] = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + p.wpw[r] - 1) / p.wpw[r]));
   }
   return 0;
