@@ -14,7 +14,12 @@
  *     scalar kernel and the Jastrow gradient of the r12 Ochi term), since a
  *     lambda cannot cross into device code compiled elsewhere;
  *   - grids, basis tables and F / V live in HBM (DeviceArray); F and V are
- *     packed over pairs j <= i, V for c = 1 (V_same = c_same V, V_oppo = c_oppo V);
+ *     packed over pairs j <= i; TCHartreeFock takes VjiSame and VjiOppo as the
+ *     reference does, or one c = 1 tensor V with the r12 Jastrow's scales
+ *     (V_same = c_same V, V_oppo = c_oppo V), which reads one tensor instead of two;
+ *   - fockOperator, eigenvectors and occupiedOrbitals are row-major host
+ *     vectors indexed as the reference's accessors: (spin, j, i), (spin, j, a),
+ *     [spin](j, b);
  *   - the basis is real (real solid harmonics), a unitary change of the
  *     reference's complex Y_lm that leaves every energy unchanged;
  *   - a non-converged RCS solve throws ConvergenceError (the reference aborts).
@@ -709,7 +714,9 @@ class RCSHartreeFock {
   RCSHartreeFock(int nElectrons, int nBasis) : nElectrons_(nElectrons), basisSize(nBasis) {
     if (nElectrons % 2 != 0) throw Error("RCSHartreeFock: the number of electrons must be even");
     density.assign(static_cast<std::size_t>(nBasis) * nBasis, 0);
+    fockOperator.assign(static_cast<std::size_t>(nBasis) * nBasis, 0);
     eigenvalues.assign(nBasis, 0);
+    eigenvectors.assign(static_cast<std::size_t>(nBasis) * nBasis, 0);
   }
   static constexpr bool needsGradients() { return false; }
 
@@ -722,9 +729,10 @@ class RCSHartreeFock {
     tcscf_scf_opts o = detail::opts(tol, maxIter, respectOneElectronSymmetry, 0, diis);
     tcscf_scf_result res{};
     energyTrace.assign(maxIter, 0);
+    tcscf_scf_observables obs{fockOperator.data(), eigenvectors.data(), nullptr};
     const int rc = tcscf_rcs_scf(nElectrons_, basisSize, overlap.data(), oneElectronTerms.data(), r1Grid.basisValues.data(),
                                  r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), Fji.data(), detail::dtype<FT>(), &o, &res,
-                                 energyTrace.data(), density.data(), eigenvalues.data(), r1Grid.stream);
+                                 energyTrace.data(), density.data(), eigenvalues.data(), &obs, r1Grid.stream);
     iter_ = res.iterations;
     energyTrace.resize(iter_ > 0 ? iter_ : 0);
     detail::check(rc);
@@ -733,8 +741,10 @@ class RCSHartreeFock {
   int numberOfConvergenceLoops() const { return iter_; }
   T highestOccupiedOrbitalEnergy() const { return eigenvalues[nElectrons_ / 2 - 1]; }
 
-  std::vector<T> density;      /* nb x nb (one spin) */
-  std::vector<T> eigenvalues;  /* ascending */
+  std::vector<T> density;       /* nb x nb (one spin) */
+  std::vector<T> fockOperator;  /* nb x nb: (j, i), built from the converged density */
+  std::vector<T> eigenvalues;   /* ascending */
+  std::vector<T> eigenvectors;  /* nb x nb: (j, a), column a = eigenvector a */
   std::vector<T> energyTrace;
 
  private:
@@ -753,7 +763,9 @@ class UOSHartreeFock {
  public:
   UOSHartreeFock(int nUp, int nDown, int nBasis) : nElectrons{nUp, nDown}, basisSize(nBasis) {
     density.assign(2 * static_cast<std::size_t>(nBasis) * nBasis, 0);
+    fockOperator.assign(2 * static_cast<std::size_t>(nBasis) * nBasis, 0);
     eigenvalues.assign(2 * static_cast<std::size_t>(nBasis), 0);
+    eigenvectors.assign(2 * static_cast<std::size_t>(nBasis) * nBasis, 0);
   }
   static constexpr bool needsGradients() { return false; }
 
@@ -766,10 +778,11 @@ class UOSHartreeFock {
     tcscf_scf_opts o = detail::opts(tol, maxIter, respectOneElectronSymmetry, 0, diis);
     tcscf_scf_result res{};
     energyTrace.assign(maxIter, 0);
+    tcscf_scf_observables obs{fockOperator.data(), eigenvectors.data(), nullptr};
     const int rc = tcscf_uos_scf(nElectrons[0], nElectrons[1], basisSize, overlap.data(), oneElectronTerms.data(),
                                  r1Grid.basisValues.data(), r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(),
                                  Fji.data(), detail::dtype<FT>(), &o, &res, energyTrace.data(), density.data(),
-                                 eigenvalues.data(), r1Grid.stream);
+                                 eigenvalues.data(), &obs, r1Grid.stream);
     iter_ = res.iterations;
     energyTrace.resize(iter_ > 0 ? iter_ : 0);
     detail::check(rc);
@@ -785,8 +798,10 @@ class UOSHartreeFock {
 
   const int nElectrons[2];
   const int basisSize;
-  std::vector<T> density;      /* 2 x nb x nb */
-  std::vector<T> eigenvalues;  /* 2 x nb, ascending per spin */
+  std::vector<T> density;       /* 2 x nb x nb */
+  std::vector<T> fockOperator;  /* 2 x nb x nb: (spin, j, i) */
+  std::vector<T> eigenvalues;   /* 2 x nb, ascending per spin */
+  std::vector<T> eigenvectors;  /* 2 x nb x nb: (spin, j, a) */
   std::vector<T> energyTrace;
 
  private:
@@ -794,44 +809,47 @@ class UOSHartreeFock {
 };
 
 /* TCHartreeFock (src/HartreeFock.hpp:353-418, src/HartreeFock.cpp:1007-1362).
- * The reference takes VjiSame and VjiOppo; here one c = 1 tensor V and the
- * Jastrow's c_same / c_oppo (V_same = c_same V, V_oppo = c_oppo V). */
+ * compute takes VjiSame and VjiOppo as the reference does; the overload with
+ * one c = 1 tensor V and a Jastrow's c_same / c_oppo (V_same = c_same V,
+ * V_oppo = c_oppo V) is the same Fock build reading one tensor. */
 template <class T>
 class TCHartreeFock {
  public:
   TCHartreeFock(int nUp, int nDown, int nBasis) : nElectrons{nUp, nDown}, basisSize(nBasis) {
     density.assign(2 * static_cast<std::size_t>(nBasis) * nBasis, 0);
+    fockOperator.assign(2 * static_cast<std::size_t>(nBasis) * nBasis, 0);
     occupiedOrbitalPseudoEnergy.assign(static_cast<std::size_t>(nUp + nDown), 0);
+    occupiedOrbitals[0].assign(static_cast<std::size_t>(nBasis) * nUp, 0);
+    occupiedOrbitals[1].assign(static_cast<std::size_t>(nBasis) * nDown, 0);
   }
   static constexpr bool needsGradients() { return true; }
 
-  /* mgsMode 0: the reference's Gram-Schmidt (Euclidean norm after the first
-   * vector, src/orthogonalization.hpp:152); 1: the overlap metric throughout. */
-  /* FT = float: the mixed-precision Fock build (per-point terms in FP32). */
+  /* The reference's signature (src/HartreeFock.hpp:386-394): F_same / F_oppo
+   * and V_same / V_oppo, e.g. from computeF / computeV with an OchiJastrow for
+   * each spin.  mgsMode 0: the reference's Gram-Schmidt (Euclidean norm after
+   * the first vector, src/orthogonalization.hpp:152); 1: the overlap metric
+   * throughout.  FT = float: FP32 storage of F / V (FP64 arithmetic). */
+  template <int N1, class FT>
+  T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
+            const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<FT>& FjiSame, const DeviceArray<FT>& FjiOppo,
+            const DeviceArray<FT>& VjiSame, const DeviceArray<FT>& VjiOppo, bool respectOneElectronSymmetry,
+            double tol = 1e-5, int maxIter = 200, int mgsMode = 0, int diis = 0) {
+    return run(overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo, VjiSame, &VjiOppo, nullptr,
+               respectOneElectronSymmetry, tol, maxIter, mgsMode, diis);
+  }
+  /* One c = 1 tensor Vji (computeV with a Jastrow) and the Jastrow's scales. */
   template <int N1, class FT>
   T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
             const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<FT>& FjiSame, const DeviceArray<FT>& FjiOppo,
             const DeviceArray<FT>& Vji, const Jastrow& jastrow, bool respectOneElectronSymmetry, double tol = 1e-5,
             int maxIter = 200, int mgsMode = 0, int diis = 0) {
-    detail::checkSquare(overlap, basisSize, "overlap");
-    detail::checkSquare(oneElectronTerms, basisSize, "oneElectronTerms");
-    if (!r1Grid.gradientsStored()) throw Error("TCHartreeFock: the r1 grid needs basis gradients");
-    tcscf_scf_opts o = detail::opts(tol, maxIter, respectOneElectronSymmetry, mgsMode, diis);
-    tcscf_scf_result res{};
     const tcscf_jastrow j = jastrow.abi();
-    energyTrace.assign(maxIter, 0);
-    const int rc = tcscf_tc_scf(nElectrons[0], nElectrons[1], basisSize, overlap.data(), oneElectronTerms.data(),
-                                r1Grid.basisValues.data(), r1Grid.basisGradients.data(),
-                                r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), FjiSame.data(), FjiOppo.data(),
-                                Vji.data(), detail::dtype<FT>(), &j, &o, &res, energyTrace.data(), density.data(),
-                                occupiedOrbitalPseudoEnergy.data(), r1Grid.stream);
-    iter_ = res.iterations;
-    energyTrace.resize(iter_ > 0 ? iter_ : 0);
-    detail::check(rc);
-    return res.energy;
+    return run(overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo, Vji, static_cast<const DeviceArray<FT>*>(nullptr), &j,
+               respectOneElectronSymmetry, tol, maxIter, mgsMode, diis);
   }
-  /* With the general Ochi Jastrow: Vji is VjiSame, and VjiOppo = rho VjiSame
-   * (OchiJastrow::tcScales). */
+  /* With a general Ochi Jastrow whose opposite-spin coefficients are one
+   * multiple rho of the same-spin ones: Vji is VjiSame and VjiOppo = rho VjiSame
+   * (OchiJastrow::tcScales); otherwise pass both tensors. */
   template <int N1, class FT>
   T compute(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
             const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<FT>& FjiSame, const DeviceArray<FT>& FjiOppo,
@@ -851,10 +869,46 @@ class TCHartreeFock {
   const int nElectrons[2];
   const int basisSize;
   std::vector<T> density;                      /* 2 x nb x nb */
+  std::vector<T> fockOperator;                 /* 2 x nb x nb: (spin, j, i), H + G2 - G3 */
   std::vector<T> occupiedOrbitalPseudoEnergy;  /* nUp then nDown, ascending Re(lambda) */
+  std::vector<T> occupiedOrbitals[2];          /* per spin nb x nocc_s: (j, b), S-metric Gram-Schmidt */
   std::vector<T> energyTrace;
 
  private:
+  template <int N1, class FT>
+  T run(const std::vector<double>& overlap, const std::vector<double>& oneElectronTerms,
+        const integration::QMCGrid<T, N1>& r1Grid, const DeviceArray<FT>& FjiSame, const DeviceArray<FT>& FjiOppo,
+        const DeviceArray<FT>& V, const DeviceArray<FT>* Vo, const tcscf_jastrow* j, bool respectOneElectronSymmetry,
+        double tol, int maxIter, int mgsMode, int diis) {
+    detail::checkSquare(overlap, basisSize, "overlap");
+    detail::checkSquare(oneElectronTerms, basisSize, "oneElectronTerms");
+    if (!r1Grid.gradientsStored()) throw Error("TCHartreeFock: the r1 grid needs basis gradients");
+    const std::size_t np = static_cast<std::size_t>(basisSize) * (basisSize + 1) / 2, n1 = r1Grid.nGrid();
+    if (FjiSame.size() != n1 * np || FjiOppo.size() != n1 * np || V.size() != 3 * n1 * np ||
+        (Vo && Vo->size() != 3 * n1 * np))
+      throw Error("TCHartreeFock: F must be n1 x npair and V n1 x 3 x npair (packed)");
+    tcscf_scf_opts o = detail::opts(tol, maxIter, respectOneElectronSymmetry, mgsMode, diis);
+    tcscf_scf_result res{};
+    energyTrace.assign(maxIter, 0);
+    const int nmax = std::max(1, std::max(nElectrons[0], nElectrons[1]));
+    std::vector<T> occ(2 * static_cast<std::size_t>(basisSize) * nmax, 0);
+    tcscf_scf_observables obs{fockOperator.data(), nullptr, occ.data()};
+    const int rc = tcscf_tc_scf(nElectrons[0], nElectrons[1], basisSize, overlap.data(), oneElectronTerms.data(),
+                                r1Grid.basisValues.data(), r1Grid.basisGradients.data(),
+                                r1Grid.quadratureGrid.weights.data(), r1Grid.nGrid(), FjiSame.data(), FjiOppo.data(),
+                                V.data(), Vo ? Vo->data() : nullptr, detail::dtype<FT>(), j, &o, &res,
+                                energyTrace.data(), density.data(), occupiedOrbitalPseudoEnergy.data(), &obs,
+                                r1Grid.stream);
+    for (int s = 0; s < 2; ++s)
+      for (int jj = 0; jj < basisSize; ++jj)
+        for (int b = 0; b < nElectrons[s]; ++b)
+          occupiedOrbitals[s][static_cast<std::size_t>(jj) * nElectrons[s] + b] =
+              occ[(static_cast<std::size_t>(s) * basisSize + jj) * nmax + b];
+    iter_ = res.iterations;
+    energyTrace.resize(iter_ > 0 ? iter_ : 0);
+    detail::check(rc);
+    return res.energy;
+  }
   int iter_ = 0;
 };
 

@@ -124,6 +124,22 @@ typedef struct tcscf_scf_result {
   double t_solve_ms;  /* time in the FP64 eigen-solves and density updates */
 } tcscf_scf_result;
 
+/* The solver objects' observable members after compute (host buffers, each
+ * may be NULL).  fock: fockOperator (src/HartreeFock.hpp:286, :342, :405),
+ * nspin x nb x nb, [s][j][i], the Fock operator H + G (TC: H + G2 - G3, after
+ * the mask) of the last iteration: at convergence the one built from the
+ * converged density, as the reference leaves it.  eigenvectors (RCS / UOS,
+ * :288, :344): nspin x nb x nb, [s][j][a] = coefficient of basis function j in
+ * eigenvector a (ascending eigenvalues) of the last eigen-solve, the one that
+ * gave the final density.  occupied (TC, occupiedOrbitals :408): 2 x nb x
+ * max(nup, ndown), [s][j][b], the Gram-Schmidt-orthonormalized occupied
+ * orbitals of the last solve (entries b >= nocc_s zero). */
+typedef struct tcscf_scf_observables {
+  double* fock;
+  double* eigenvectors;
+  double* occupied;
+} tcscf_scf_observables;
+
 /* ---------------------------------------------------------------- misc */
 const char* tcscf_last_error(void);
 int tcscf_version(void);
@@ -331,48 +347,65 @@ int tcscf_three_electron(const double* A1_dev, const double* w1_dev, int64_t n1,
 
 /* ---------------------------------------------------------------- SCF drivers */
 /* RCSHartreeFock::compute (src/HartreeFock.cpp:785-886).  S, H host nb x nb
- * row-major; A1_dev n1 x nb; w1_dev n1; F_dev packed n1 x npair (f_dtype).
- * Synchronous.  energy_trace (max_iter), D_out (nb x nb), evals_out (nb) may be NULL. */
+ * row-major (S symmetric: the reference's column-major overlap view holds the
+ * same numbers); A1_dev n1 x nb; w1_dev n1; F_dev packed n1 x npair (f_dtype).
+ * Synchronous.  energy_trace (max_iter), D_out (nb x nb), evals_out (nb) and
+ * obs (fockOperator / eigenvectors, nspin = 1) may be NULL. */
 int tcscf_rcs_scf(int nelec, int nb, const double* S, const double* H, const double* A1_dev, const double* w1_dev,
                   int64_t n1, const void* F_dev, int f_dtype, const tcscf_scf_opts* opts, tcscf_scf_result* res,
-                  double* energy_trace, double* D_out, double* evals_out, void* stream);
+                  double* energy_trace, double* D_out, double* evals_out, tcscf_scf_observables* obs, void* stream);
 
 /* UOSHartreeFock::compute (src/HartreeFock.cpp:890-1003).  D_out 2 x nb x nb,
- * evals_out 2 x nb. */
+ * evals_out 2 x nb, obs nspin = 2. */
 int tcscf_uos_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1_dev,
                   const double* w1_dev, int64_t n1, const void* F_dev, int f_dtype, const tcscf_scf_opts* opts,
-                  tcscf_scf_result* res, double* energy_trace, double* D_out, double* evals_out, void* stream);
+                  tcscf_scf_result* res, double* energy_trace, double* D_out, double* evals_out,
+                  tcscf_scf_observables* obs, void* stream);
 
-/* TCHartreeFock::compute (src/HartreeFock.cpp:1007-1362), real basis.  Fs_dev,
- * Fo_dev: packed n1 x npair F_same / F_oppo (TCSCF_K_TC_SCALAR); V_dev: packed
- * n1 x 3 x npair (TCSCF_K_JASTROW_GRAD, c = 1); all three of f_dtype:
- * TCSCF_F64 runs the Fock build in FP64, TCSCF_F32 (mixed precision) forms
- * its per-point terms in FP32 and sums them over points in FP64.  A1_dev
- * n1 x nb; dA1_dev n1 x nb x 3; S, H host.  The Fock build runs on the device,
- * the FP64 generalized non-Hermitian eigen-solve (role of LAPACK ggev, per
- * symmetry block), the Re(lambda) sort and the MGS of the occupied orbitals
- * on the host.  Synchronous.  D_out 2 x nb x nb; pseudo_out nup + ndown
- * occupied pseudo-energies. */
+/* TCHartreeFock::compute(overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo,
+ * VjiSame, VjiOppo, respectOneElectronSymmetry) (src/HartreeFock.hpp:386-394,
+ * src/HartreeFock.cpp:1007-1362), real basis.  Fs_dev, Fo_dev: packed
+ * n1 x npair F_same / F_oppo (TCSCF_K_TC_SCALAR).  Vs_dev, Vo_dev: packed
+ * n1 x 3 x npair VjiSame / VjiOppo (TCSCF_K_JASTROW_GRAD or the general Ochi
+ * gradient for each spin), any two tensors; or Vo_dev = NULL, Vs_dev the c = 1
+ * r12 Jastrow gradient and jastrow's c_same / c_oppo its scales (V_same =
+ * c_same V, V_oppo = c_oppo V: one tensor read instead of two; jastrow is
+ * unused when Vo_dev is given).  All of f_dtype: TCSCF_F32 stores them in
+ * FP32, the Fock build's arithmetic is FP64 either way.  A1_dev n1 x nb;
+ * dA1_dev n1 x nb x 3; S, H host.  The Fock build runs on the device, the FP64
+ * generalized non-Hermitian eigen-solve (role of LAPACK ggev, per symmetry
+ * block), the Re(lambda) sort and the MGS of the occupied orbitals on the
+ * host.  Synchronous.  D_out 2 x nb x nb; pseudo_out nup + ndown occupied
+ * pseudo-energies; obs (fockOperator, occupiedOrbitals) may be NULL. */
 int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1_dev,
                  const double* dA1_dev, const double* w1_dev, int64_t n1, const void* Fs_dev, const void* Fo_dev,
-                 const void* V_dev, int f_dtype, const tcscf_jastrow* jastrow, const tcscf_scf_opts* opts,
-                 tcscf_scf_result* res, double* energy_trace, double* D_out, double* pseudo_out, void* stream);
+                 const void* Vs_dev, const void* Vo_dev, int f_dtype, const tcscf_jastrow* jastrow,
+                 const tcscf_scf_opts* opts, tcscf_scf_result* res, double* energy_trace, double* D_out,
+                 double* pseudo_out, tcscf_scf_observables* obs, void* stream);
 
 /* One TC Fock build (the per-iteration sweeps and assembly of
  * src/HartreeFock.cpp:1118-1224) from D_host (2 x nb x nb) and the occupied
  * orbitals C_host (2 x nb x max(nup, ndown), column b of spin s used for
- * b < nocc_s): writes G2 and G3 (2 x nb x nb each, host) before the mask. */
+ * b < nocc_s): writes G2 and G3 (2 x nb x nb each, host) before the mask.
+ * V tensors as in tcscf_tc_scf. */
 int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1_dev, const double* dA1_dev, const double* w1_dev,
-                  int64_t n1, const void* Fs_dev, const void* Fo_dev, const void* V_dev, int f_dtype,
-                  const tcscf_jastrow* jastrow, const double* D_host, const double* C_host, double* G2_host,
-                  double* G3_host, void* stream);
+                  int64_t n1, const void* Fs_dev, const void* Fo_dev, const void* Vs_dev, const void* Vo_dev,
+                  int f_dtype, const tcscf_jastrow* jastrow, const double* D_host, const double* C_host,
+                  double* G2_host, double* G3_host, void* stream);
+
+/* No reference counterpart (a diagnostic): the point-kernel launch the TC
+ * Fock build picks for these sizes, f_dtype and V form (general_v: VjiSame
+ * and VjiOppo as two tensors).  info[0..2]: V rows staged in LDS (0 / 1), waves
+ * per workgroup, LDS bytes of the two-spin build; info[3..5] the same for the
+ * restricted closed-shell build. */
+int tcscf_tc_launch_info(int nup, int ndown, int nb, int f_dtype, int general_v, int* info);
 
 /* ---------------------------------------------------------------- batched SCF */
 enum tcscf_scf_kind { TCSCF_SCF_RCS = 0, TCSCF_SCF_UOS = 1, TCSCF_SCF_TC = 2 };
 
 /* One instance of a batched solve: the arguments of tcscf_rcs_scf /
  * tcscf_uos_scf (kind RCS / UOS: F_dev, f_dtype; RCS uses nup + ndown electrons)
- * or tcscf_tc_scf (kind TC: Fs_dev, Fo_dev, V_dev, f_dtype, dA1_dev, jastrow). */
+ * or tcscf_tc_scf (kind TC: Fs_dev, Fo_dev, V_dev, Vo_dev, f_dtype, dA1_dev, jastrow). */
 typedef struct tcscf_scf_item {
   int kind;
   int nup, ndown, nb;
@@ -387,6 +420,7 @@ typedef struct tcscf_scf_item {
   const void* Fs_dev;
   const void* Fo_dev;
   const void* V_dev;
+  const void* Vo_dev; /* VjiOppo with V_dev = VjiSame, or NULL (V_dev c = 1, scaled by jastrow) */
   tcscf_jastrow jastrow;
   double* energy_trace; /* host, opts->max_iter entries, or NULL */
   double* D_out;        /* host, nspin x nb x nb (RCS 1, else 2), or NULL */

@@ -499,14 +499,28 @@ def computeV(r1Grid, r3Grid, jastrow, precision="fp64", out_dtype="fp64", ordere
     return _compute(r1Grid, r3Grid, _lib.K_JASTROW_GRAD, jastrow, True, precision, out_dtype, ordered)
 
 
-def _tc_dtype(Fs, Fo, V, n1):
-    """The f_dtype of a TC tensor triple: all float64 (FP64 Fock build) or all
-    float32 (mixed-precision Fock build), one row per r1 point."""
+def _tc_dtype(Fs, Fo, V, n1, Vo=None):
+    """The f_dtype of the TC tensors: all float64 or all float32 (FP32
+    storage; the Fock build's arithmetic is FP64 either way), one row per r1
+    point, each V the size of the first."""
     torch = _torch()
-    dts = {T.dtype for T in (Fs, Fo, V)}
-    if len(dts) != 1 or dts.pop() not in (torch.float64, torch.float32) or any(T.shape[0] != n1 for T in (Fs, Fo, V)):
+    ts = (Fs, Fo, V) + ((Vo,) if Vo is not None else ())
+    dts = {T.dtype for T in ts}
+    if len(dts) != 1 or dts.pop() not in (torch.float64, torch.float32) or any(T.shape[0] != n1 for T in ts):
         raise ValueError("TC: F_same, F_oppo and V must all be float64 or all float32, one row per r1 point")
+    if Vo is not None and Vo.numel() != V.numel():
+        raise ValueError("TC: VjiSame and VjiOppo must have the same shape")
     return _lib.F32 if Fs.dtype == torch.float32 else _lib.F64
+
+
+def _observables(fock, eigenvectors, occupied):
+    """tcscf_scf_observables over numpy float64 arrays (each may be None)."""
+    def ptr(a):
+        if a is None:
+            return None
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return a.ctypes.data
+    return _lib.ScfObservables(ptr(fock), ptr(eigenvectors), ptr(occupied))
 
 
 def computeF_batch(specs, precision="bf16x5", out_dtype="fp64"):
@@ -785,6 +799,7 @@ def batch_scf(instances, tol=1e-5, max_iter=200, respectOneElectronSymmetry=True
             it.dA1_dev = g.basisGradients.data_ptr()
             it.f_dtype = _tc_dtype(x["Fs"], x["Fo"], x["V"], g.nGrid())
             it.Fs_dev, it.Fo_dev, it.V_dev = x["Fs"].data_ptr(), x["Fo"].data_ptr(), x["V"].data_ptr()
+            it.Vo_dev = None
             it.jastrow = x["jastrow"]._struct()
         else:
             F = x["F"]
@@ -846,7 +861,9 @@ class RCSHartreeFock(_HFBase):
             raise ValueError("RCS needs an even number of electrons")
         self.nElectrons, self.basisSize = nElectrons, nBasis
         self.density = np.zeros((1, nBasis, nBasis))
+        self.fockOperator = np.zeros((1, nBasis, nBasis))
         self.eigenvalues = np.zeros(nBasis)
+        self.eigenvectors = np.zeros((1, nBasis, nBasis))
         self._iters = 0
 
     @staticmethod
@@ -865,10 +882,12 @@ class RCSHartreeFock(_HFBase):
         trace = np.zeros(max_iter)
         D = np.zeros((nb, nb))
         ev = np.zeros(nb)
+        obs = _observables(self.fockOperator, self.eigenvectors, None)
         rc = lib().tcscf_rcs_scf(self.nElectrons, nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                  _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()), fp, fd,
                                  ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, 0, diis)), ctypes.byref(res),
-                                 trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp), _stream())
+                                 trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp),
+                                 ctypes.byref(obs), _stream())
         self.density[0] = D
         self.eigenvalues[:] = ev
         return self._finish(rc, res, trace)
@@ -884,7 +903,9 @@ class UOSHartreeFock(_HFBase):
         self.nElectrons = (numSpinUp, numSpinDown)
         self.basisSize = nBasis
         self.density = np.zeros((2, nBasis, nBasis))
+        self.fockOperator = np.zeros((2, nBasis, nBasis))
         self.eigenvalues = np.zeros((2, nBasis))
+        self.eigenvectors = np.zeros((2, nBasis, nBasis))
         self._iters = 0
 
     @staticmethod
@@ -903,10 +924,12 @@ class UOSHartreeFock(_HFBase):
         trace = np.zeros(max_iter)
         D = np.zeros((2, nb, nb))
         ev = np.zeros((2, nb))
+        obs = _observables(self.fockOperator, self.eigenvectors, None)
         rc = lib().tcscf_uos_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                  _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()), fp, fd,
                                  ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, 0, diis)), ctypes.byref(res),
-                                 trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp), _stream())
+                                 trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp),
+                                 ctypes.byref(obs), _stream())
         self.density[:] = D
         self.eigenvalues[:] = ev
         return self._finish(rc, res, trace)
@@ -918,69 +941,89 @@ class UOSHartreeFock(_HFBase):
 class TCHartreeFock(_HFBase):
     """TCHartreeFock (src/HartreeFock.hpp:353-418, src/HartreeFock.cpp:1007-1362).
 
-    compute() takes the packed F_same / F_oppo (computeF with kernel='tc') and
-    the packed c = 1 Jastrow-gradient tensor V (computeV); V_same = c_same V and
-    V_oppo = c_oppo V are applied inside the Fock build.  mgs_mode 0 keeps the
-    reference's Gram-Schmidt (Euclidean norm after the first vector), 1 uses
-    the overlap metric throughout."""
+    compute(S, H, r1Grid, FjiSame, FjiOppo, VjiSame, VjiOppo, ...) takes the
+    packed F_same / F_oppo (computeF with kernel='tc', or an OchiJastrow per
+    spin) and V_same / V_oppo as the reference does.  In place of VjiOppo a
+    Jastrow (or an OchiJastrow with c_oppo = rho c_same) may be given: VjiSame is
+    then the c = 1 tensor V of computeV and V_same = c_same V, V_oppo = c_oppo V
+    are applied inside the Fock build, which reads one tensor instead of two.
+    mgs_mode 0 keeps the reference's Gram-Schmidt (Euclidean norm after the
+    first vector), 1 uses the overlap metric throughout.  After compute:
+    density, fockOperator (2, nb, nb), occupiedOrbitals [spin] (nb, nocc_s),
+    occupiedOrbitalPseudoEnergy."""
 
     def __init__(self, numSpinUp, numSpinDown, nBasis):
         self.nElectrons = (numSpinUp, numSpinDown)
         self.basisSize = nBasis
         self.density = np.zeros((2, nBasis, nBasis))
+        self.fockOperator = np.zeros((2, nBasis, nBasis))
         self.occupiedOrbitalPseudoEnergy = [np.zeros(numSpinUp), np.zeros(numSpinDown)]
+        self.occupiedOrbitals = [np.zeros((nBasis, numSpinUp)), np.zeros((nBasis, numSpinDown))]
         self._iters = 0
 
     @staticmethod
     def needsGradients():
         return True
 
-    def compute(self, overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo, Vji, jastrow,
+    @staticmethod
+    def _v_args(VjiSame, VjiOppo, n1):
+        """(Vs, Vo or None, jastrow struct or None) and the f_dtype check."""
+        if isinstance(VjiOppo, (Jastrow, OchiJastrow)):
+            return VjiSame, None, VjiOppo._tc_struct()
+        return VjiSame, VjiOppo, None
+
+    def compute(self, overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo, VjiSame, VjiOppo,
                 respectOneElectronSymmetry=True, tol=1e-5, max_iter=200, mgs_mode=0, diis=0):
-        torch = _torch()
         nb = self.basisSize
         S, Sp = _np(overlap)
         H, Hp = _np(oneElectronTerms)
         if S.shape != (nb, nb) or H.shape != (nb, nb) or r1Grid.nBasis() != nb or not r1Grid.gradientsStored():
             raise ValueError("TC: shape mismatch or basis gradients not stored on the r1 grid")
         n1 = r1Grid.nGrid()
-        fdt = _tc_dtype(FjiSame, FjiOppo, Vji, n1)
+        Vs, Vo, jp = self._v_args(VjiSame, VjiOppo, n1)
+        fdt = _tc_dtype(FjiSame, FjiOppo, Vs, n1, Vo)
         res = _lib.ScfResult()
         trace = np.zeros(max_iter)
         D = np.zeros((2, nb, nb))
         ps = np.zeros(sum(self.nElectrons))
-        jp = jastrow._tc_struct()
+        nmax = max(1, max(self.nElectrons))
+        occ = np.zeros((2, nb, nmax))
+        obs = _observables(self.fockOperator, None, occ)
         rc = lib().tcscf_tc_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                 _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(n1),
-                                _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), fdt, ctypes.byref(jp),
+                                _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vs), _ptr(Vo) if Vo is not None else None, fdt,
+                                ctypes.byref(jp) if jp is not None else None,
                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, mgs_mode, diis)),
                                 ctypes.byref(res), trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp),
-                                ps.ctypes.data_as(_dp), _stream())
+                                ps.ctypes.data_as(_dp), ctypes.byref(obs), _stream())
         self.density[:] = D
         self.occupiedOrbitalPseudoEnergy = [ps[:self.nElectrons[0]].copy(), ps[self.nElectrons[0]:].copy()]
+        self.occupiedOrbitals = [occ[s, :, :self.nElectrons[s]].copy() for s in range(2)]
         return self._finish(rc, res, trace)
 
     def highestOccupiedOrbitalEnergy(self):
         return max(self.occupiedOrbitalPseudoEnergy[0][-1], self.occupiedOrbitalPseudoEnergy[1][-1])
 
-    def fockBuild(self, r1Grid, FjiSame, FjiOppo, Vji, jastrow, density, occupiedOrbitals):
+    def fockBuild(self, r1Grid, FjiSame, FjiOppo, VjiSame, VjiOppo, density, occupiedOrbitals):
         """One Fock build of the TC loop (src/HartreeFock.cpp:1118-1224) at a
         given density (2, nb, nb) and occupied orbitals (2, nb, max nocc):
         returns the two-electron term G2 and the three-electron term G3, each
-        (2, nb, nb), before the one-electron-symmetry mask."""
+        (2, nb, nb), before the one-electron-symmetry mask.  VjiOppo as in
+        compute (a tensor, or a Jastrow with VjiSame the c = 1 tensor)."""
         nb = self.basisSize
         nmax = max(self.nElectrons)
         D = np.ascontiguousarray(density, dtype=np.float64)
         C = np.ascontiguousarray(occupiedOrbitals, dtype=np.float64)
         if D.shape != (2, nb, nb) or C.shape != (2, nb, nmax) or r1Grid.nBasis() != nb:
             raise ValueError("fockBuild: density must be (2, nb, nb) and orbitals (2, nb, max nocc)")
-        fdt = _tc_dtype(FjiSame, FjiOppo, Vji, r1Grid.nGrid())
+        Vs, Vo, jp = self._v_args(VjiSame, VjiOppo, r1Grid.nGrid())
+        fdt = _tc_dtype(FjiSame, FjiOppo, Vs, r1Grid.nGrid(), Vo)
         G2 = np.zeros((2, nb, nb))
         G3 = np.zeros((2, nb, nb))
-        jp = jastrow._tc_struct()
         check(lib().tcscf_tc_fock(self.nElectrons[0], self.nElectrons[1], nb, _ptr(r1Grid.basisValues),
                                   _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights),
-                                  ctypes.c_int64(r1Grid.nGrid()), _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vji), fdt,
-                                  ctypes.byref(jp), D.ctypes.data_as(_dp), C.ctypes.data_as(_dp),
-                                  G2.ctypes.data_as(_dp), G3.ctypes.data_as(_dp), _stream()))
+                                  ctypes.c_int64(r1Grid.nGrid()), _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vs),
+                                  _ptr(Vo) if Vo is not None else None, fdt,
+                                  ctypes.byref(jp) if jp is not None else None, D.ctypes.data_as(_dp),
+                                  C.ctypes.data_as(_dp), G2.ctypes.data_as(_dp), G3.ctypes.data_as(_dp), _stream()))
         return G2, G3

@@ -105,13 +105,13 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
         cleanup();
         TC_CHECK(false, "tcscf_batch_scf: DIIS with TC needs mgs_mode = 1");
       }
-      const size_t n = tc_ws_doubles(it.nup, it.ndown, nb, it.n1);
+      const size_t n = tc_ws_doubles(it.nup, it.ndown, nb, it.n1, it.Vo_dev != nullptr);
       if (hipMalloc((void**)&x.ws, n * sizeof(double)) != hipSuccess) {
         cleanup();
         TC_CHECK(false, "tcscf_batch_scf: out of device memory");
       }
       if ((rc = tc_prepare(x.tcp, it.nup, it.ndown, nb, it.A1_dev, it.dA1_dev, it.w1_dev, it.n1, it.Fs_dev,
-                           it.Fo_dev, it.V_dev, it.f_dtype, &it.jastrow, x.ws))) {
+                           it.Fo_dev, it.V_dev, it.Vo_dev, it.f_dtype, &it.jastrow, x.ws))) {
         cleanup();
         return rc;
       }

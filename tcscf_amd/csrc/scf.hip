@@ -275,7 +275,7 @@ int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_
 HfIter::HfIter(int nspin_, const int* nocc_, int nb_, const double* S_, const double* H_, const tcscf_scf_opts* o_)
     : nspin(nspin_), nb(nb_), nocc{nocc_[0], nspin_ > 1 ? nocc_[1] : 0}, S(S_), H(H_), o(o_),
       D((size_t)nspin_ * nb_ * nb_, 0.0), Fock((size_t)nspin_ * nb_ * nb_), evals((size_t)nspin_ * nb_, 0.0),
-      diis(o_->diis) {}
+      evecs((size_t)nspin_ * nb_ * nb_, 0.0), diis(o_->diis) {}
 
 int HfIter::step(double* G, int it, tcscf_scf_result* res, double* trace, bool* done) {
   const int64_t nb2 = (int64_t)nb * nb;
@@ -319,6 +319,7 @@ int HfIter::step(double* G, int it, tcscf_scf_result* res, double* trace, bool* 
       return TCSCF_ESOLVER;
     }
     std::copy(ev.begin(), ev.end(), evals.begin() + s * nb);
+    std::copy(C.begin(), C.begin() + nb2, evecs.begin() + s * nb2);
     density_from(nb, nocc[s], C, D.data() + s * nb2);
   }
   return 0;
@@ -328,7 +329,7 @@ namespace {
 
 int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H, const double* A1, const double* w1,
            int64_t n1, const void* F, int f_dtype, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
-           double* D_out, double* evals_out, hipStream_t st) {
+           double* D_out, double* evals_out, tcscf_scf_observables* obs, hipStream_t st) {
   TC_CHECK(S && H && A1 && w1 && F && o && res && nb > 0 && n1 > 0, "SCF: bad arguments");
   TC_CHECK(o->max_iter > 0, "SCF: max_iter must be positive");
   TC_CHECK(o->diis >= 0, "SCF: diis depth must be >= 0");
@@ -367,6 +368,10 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
   }
   if (D_out) std::copy(iter.D.begin(), iter.D.end(), D_out);
   if (evals_out) std::copy(iter.evals.begin(), iter.evals.end(), evals_out);
+  if (obs) {
+    if (obs->fock) std::copy(iter.Fock.begin(), iter.Fock.end(), obs->fock);
+    if (obs->eigenvectors) std::copy(iter.evecs.begin(), iter.evecs.end(), obs->eigenvectors);
+  }
   if (res->status != TCSCF_OK) set_error("SCF: did not converge");
   return res->status;
 }
@@ -378,18 +383,20 @@ extern "C" {
 
 int tcscf_rcs_scf(int nelec, int nb, const double* S, const double* H, const double* A1, const double* w1, int64_t n1,
                   const void* F, int f_dtype, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
-                  double* D_out, double* evals_out, void* stream) {
+                  double* D_out, double* evals_out, tcscf_scf_observables* obs, void* stream) {
   TC_CHECK(nelec > 0 && nelec % 2 == 0, "RCS: the number of electrons must be even");
   const int nocc[1] = {nelec / 2};
-  return tc::hf_scf(1, nocc, nb, S, H, A1, w1, n1, F, f_dtype, o, res, trace, D_out, evals_out, (hipStream_t)stream);
+  return tc::hf_scf(1, nocc, nb, S, H, A1, w1, n1, F, f_dtype, o, res, trace, D_out, evals_out, obs,
+                    (hipStream_t)stream);
 }
 
 int tcscf_uos_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1, const double* w1,
                   int64_t n1, const void* F, int f_dtype, const tcscf_scf_opts* o, tcscf_scf_result* res,
-                  double* trace, double* D_out, double* evals_out, void* stream) {
+                  double* trace, double* D_out, double* evals_out, tcscf_scf_observables* obs, void* stream) {
   TC_CHECK(nup >= 0 && ndown >= 0 && nup <= nb && ndown <= nb, "UOS: bad electron counts");
   const int nocc[2] = {nup, ndown};
-  return tc::hf_scf(2, nocc, nb, S, H, A1, w1, n1, F, f_dtype, o, res, trace, D_out, evals_out, (hipStream_t)stream);
+  return tc::hf_scf(2, nocc, nb, S, H, A1, w1, n1, F, f_dtype, o, res, trace, D_out, evals_out, obs,
+                    (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -62,6 +62,7 @@ struct HfIter {
   const double* H;
   const tcscf_scf_opts* o;
   std::vector<double> D, Fock, evals, C, err;
+  std::vector<double> evecs;  // nspin x nb x nb: [s][j][a], eigenvector a of the last solve
   dense::Diis diis;
   double energy = std::numeric_limits<double>::max();
   HfIter(int nspin_, const int* nocc_, int nb_, const double* S_, const double* H_, const tcscf_scf_opts* o_);
@@ -75,9 +76,10 @@ struct TcPlan {
   int nup, ndown, nb, nmax, nvec;
   int64_t n1, npair;
   const double *A1, *dA1, *w1;
-  const void *Fs, *Fo, *V;  // element type: FP32 when f32, else FP64
-  int f32;                  // FP32 Fock build (per-point terms in FP32, sums over points in FP64)
-  double c_same, c_oppo;
+  const void *Fs, *Fo, *V, *Vo;  // element type: FP32 when f32, else FP64
+  int f32;                       // FP32 storage of F / V (per-point arithmetic in FP64)
+  int gv, nwt;                   // V / Vo independent (V_same, V_oppo): gv = 1, nwt = 2 W tensors
+  double c_same, c_oppo;         // scales of one c = 1 tensor V (1, 1 when gv)
   double *dD, *dC, *dVec, *dAl, *dH, *dPG, *dPV, *dOut;
   // point-kernel launch, index 0 for two spins, 1 for the restricted build:
   // waves per workgroup, V rows staged in LDS, LDS bytes, workgroups
@@ -86,11 +88,16 @@ struct TcPlan {
   int64_t pblocks[2];
   int ustart[7];  // rank workgroups per target (prefix offsets)
 };
-size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1);
+size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1, bool general_v);
 size_t tc_out_doubles(int nb);  // the per-build result block copied back to the host
+// Vo non-null: V and Vo are the reference's independent VjiSame / VjiOppo (jp
+// unused); else V is the c = 1 tensor and jp gives c_same / c_oppo.
 int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1,
-               int64_t n1, const void* Fs, const void* Fo, const void* V, int f_dtype, const tcscf_jastrow* jp,
-               double* ws);
+               int64_t n1, const void* Fs, const void* Fo, const void* V, const void* Vo, int f_dtype,
+               const tcscf_jastrow* jp, double* ws);
+// The point kernel's launch (stage, wpw, sh1, pblocks) from nb, nmax, f32, gv
+// and n1; false when the LDS cannot hold one wave.
+bool tc_launch_shape(TcPlan& p);
 // Closed shell whose two spins hold bitwise identical densities and orbitals:
 // the spin-1 Fock terms are then the spin-0 numbers, formed once.
 bool tc_restricted(const TcPlan& p, const double* D, const double* C);

@@ -1,9 +1,12 @@
 // Transcorrelated Fock build and SCF driver (TCHartreeFock::compute,
 // src/HartreeFock.cpp:1007-1362), real basis.
 //
-// Inputs per r1 point n: packed F_same[n][p], F_oppo[n][p] (scalar TC kernel
-// with c_same / c_oppo) and V[n][xyz][p] for c = 1 (V_same = c_same V,
-// V_oppo = c_oppo V); basis values A[n][k] and gradients dA[n][k][xyz].
+// Inputs per r1 point n: packed F_same[n][p], F_oppo[n][p] (scalar TC kernel)
+// and the Jastrow-gradient tensors, either as the reference has them
+// (V_same[n][xyz][p] and V_oppo[n][xyz][p], independent) or, for a Jastrow
+// whose opposite-spin coefficients are one multiple of the same-spin ones, as
+// one c = 1 tensor V with V_same = c_same V, V_oppo = c_oppo V; basis values
+// A[n][k] and gradients dA[n][k][xyz].
 //
 // Every term of the reference's G2 / G3 assembly (:1157-1224) is one of
 //   (a) a rank-one product L_j R_i of per-point vectors      -> "vectors"
@@ -41,14 +44,19 @@ constexpr int kWChunk = 2;  // orbitals of W per LDS round of |U|^2
 // Compact per-point output, per spin s, vectors of nb doubles (lane i = basis
 // index i):
 //   0        u0 = w (faa A - FaiS)                      G2 : A (x) u0
-//   1        u1 = w (VT . dA - c_s Vai1)                VEC: A (x) u1
+//   1        u1 = w (VT . dA - Vai)                     VEC: A (x) u1
 //   2        u2 = w (|VT|^2 - sNS - sNO_other) A + Z    G3 : A (x) u2
 //   3        u3 = Z = 2 w (SVbiVT - VTSI . VT)          G3 : u3 (x) A
 //   4..6     t_c = VTSI_c                               VEC: (-w t_c) (x) dA_c
 //                                                       G3 : (2 w t_c) (x) t_c
-//   7+3b+c   W_bc = sum_j C_s[j][b] V_c(j, .)           G3 : (alpha W_bc) (x) W_bc
-// with alpha_s = -2 w (occ_s c_s^2 + occ_o c_o^2) per (point, spin), and
-// x (x) y the rank-one matrix x_j y_i.  (Vja == VTSI for symmetric V.)
+//   7+3(t nmax+b)+c  W^t_bc = sum_j C_s[j][b] V^t_c(j, .)  G3 : (alpha_t W^t_bc) (x) W^t_bc
+// with t = 0 for V_same and t = 1 for V_oppo (independent tensors only),
+// alpha_0 = -2 w occ_s, alpha_1 = -2 w occ_o per (point, spin); with one
+// proportional tensor t = 0 only and alpha_0 = -2 w (occ_s c_s^2 + occ_o c_o^2).
+// x (x) y is the rank-one matrix x_j y_i.  (Vja == VTSI for symmetric V.)
+// The reference's V(j,i) . scaledGradients terms (:1172-1174) are symmetric in
+// (j, i) and cancel in the antisymmetrization G2 = scalar/2 + vec/2 - vec^T/2,
+// so they are not formed.
 constexpr int kVecFixed = 7;
 
 // Fs, Fo and V hold the kernel's storage type ST (double, or float for the
@@ -56,18 +64,19 @@ constexpr int kVecFixed = 7;
 struct PointArgs {
   const void* Fs;     // [n1][npair]
   const void* Fo;     // [n1][npair]
-  const void* V;      // [n1][3][npair]
+  const void* V;      // [n1][3][npair]: V_same, or the c = 1 tensor
+  const void* Vo;     // [n1][3][npair]: V_oppo (independent tensors), else unused
   const double* A;    // [n1][nb]
   const double* dA;   // [n1][nb][3]
   const double* w;    // [n1]
   const double* D;    // [2][nb][nb]
   const double* C;    // [2][nb][nmax]
-  double c_same, c_oppo;
+  double c_same, c_oppo;  // 1, 1 for independent tensors
   int64_t n1, npair;
   int nb, nmax, nocc0, nocc1, nvec;
   double* vecs;       // [n1][2][nvec][nb]
-  double* alpha;      // [n1][2]
-  double* hvec;       // [n1][2 spins][2 targets (VEC, G3)][3]
+  double* alpha;      // [n1][2 spins][2 tensors]
+  double* hvec;       // [n1][2 spins][2 tensors][3]: the G3 V-linear vectors
 };
 
 // Cross-lane sums in registers: DPP moves within a row of 16 lanes (pairs at
@@ -134,11 +143,11 @@ __device__ inline double part_sum(double v, int nbp) {
   return v;
 }
 
-// Doubles of LDS that the staged V rows ([3][npair] of element type ST) of a
-// wave take, 0 when not staged.
+// Doubles of LDS that the staged V rows ([nt][3][npair] of element type ST,
+// nt tensors) of a wave take, 0 when not staged.
 template <class ST>
-__host__ __device__ inline int64_t staged_doubles(bool stage, int nb) {
-  return stage ? (3 * ((int64_t)nb * (nb + 1) / 2) * (int64_t)sizeof(ST) + 7) / 8 : 0;
+__host__ __device__ inline int64_t staged_doubles(bool stage, int nb, int nt) {
+  return stage ? (3 * nt * ((int64_t)nb * (nb + 1) / 2) * (int64_t)sizeof(ST) + 7) / 8 : 0;
 }
 
 // Pass 1: one wave per r1 point.  Lane = jp * nbp + i: i is the basis index
@@ -148,15 +157,18 @@ __host__ __device__ inline int64_t staged_doubles(bool stage, int nb) {
 // addresses, a gather that touches a separate cache line for almost every
 // lane.  One pass over j forms every F- and V-linear intermediate (F / V
 // against D, X and kS, and W = V C for the first kWMain orbitals), a second
-// pass V . E once E is known.  STAGE copies the V rows (read in both passes)
-// into the wave's LDS area by direct-to-LDS loads first, so those gathers hit
-// LDS; it pays when 8 waves of a workgroup fit in LDS and hide the latency of
-// the dependent passes.  D and C are shared in LDS, the per-point vectors
-// live in a small per-wave LDS area.  NS = 1 (closed shell with identical
-// spin densities and orbitals) forms spin 0 only, the spin-1 terms being the
-// same numbers.
-template <class ST, bool STAGE, int NS>
-__global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void tc_point_kernel(PointArgs a) {
+// pass V . E once E is known.  STAGE copies the V rows (read in several
+// passes) into the wave's LDS area by direct-to-LDS loads first, so those
+// gathers hit LDS; it pays when 8 waves of a workgroup fit in LDS and hide the
+// latency of the dependent passes.  D and C are shared in LDS, the per-point
+// vectors live in a small per-wave LDS area.  NS = 1 (closed shell with
+// identical spin densities and orbitals) forms spin 0 only, the spin-1 terms
+// being the same numbers.  GV: V_same and V_oppo are independent tensors (the
+// reference's VjiSame / VjiOppo); otherwise one c = 1 tensor scaled by c_same
+// and c_oppo.
+template <class ST, bool STAGE, int NS, bool GV>
+__global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4 && !GV) ? 768 : 512, 1) void tc_point_kernel(PointArgs a) {
+  constexpr int NT = GV ? 2 : 1;  // V tensors
   extern __shared__ double smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int nb = a.nb, nmax = a.nmax;
@@ -164,7 +176,7 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
   const int64_t np = a.npair, nb2 = (int64_t)nb * nb;
   double* Ds = smem;                       // [NS][nb][nb]
   double* Cs = Ds + NS * nb2;              // [NS][nb][nmax]
-  double* base = Cs + NS * (int64_t)nb * nmax + wave * (point_wave_elems(nb, NS) + staged_doubles<ST>(STAGE, nb));
+  double* base = Cs + NS * (int64_t)nb * nmax + wave * (point_wave_elems(nb, NS) + staged_doubles<ST>(STAGE, nb, NT));
   double* As = base;                       // [nb]
   double* dAs = As + nb;                   // [nb][3]
   double* Xs = dAs + 3 * nb;               // [NS][nb]
@@ -172,7 +184,7 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
   double* kS = VT + 3 * NS * nb;           // [NS][nb][3]
   double* Es = kS + 3 * NS * nb;           // [NS][nb][3]
   double* Wc = kS;                         // [NS][nb][kWChunk][3], once kS and Es are consumed
-  ST* Sv = reinterpret_cast<ST*>(Es + 3 * NS * nb);  // STAGE: the point's V rows [3][npair]
+  ST* Sv = reinterpret_cast<ST*>(Es + 3 * NS * nb);  // STAGE: the point's V rows [NT][3][npair]
   for (int64_t e = tid; e < NS * nb2; e += blockDim.x) Ds[e] = a.D[e];
   for (int64_t e = tid; e < NS * (int64_t)nb * nmax; e += blockDim.x) Cs[e] = a.C[e];
   __syncthreads();
@@ -184,13 +196,13 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
   const double cs = a.c_same, co = a.c_oppo;
   const int nocc[2] = {a.nocc0, a.nocc1};
   const int64_t nvw = (int64_t)gridDim.x * nwaves;
-  // body(j, f_s, f_o, v0, v1, v2) for every j of the lane
-  auto for_j = [&](const ST* fs, const ST* fo, const ST* vv, auto&& body) {
+  auto pidx = [&](int j) -> int64_t { return j <= i ? tri_i + j : j * (j + 1) / 2 + i; };
+  // body(j, v0, v1, v2) over the lane's j for one V tensor
+  auto for_j = [&](const ST* vv, auto&& body) {
     if (act)
       for (int j = jp; j < nb; j += parts) {
-        const int64_t pj = j <= i ? tri_i + j : j * (j + 1) / 2 + i;
-        body(j, fs ? double(fs[pj]) : 0.0, fo ? double(fo[pj]) : 0.0, double(vv[pj]), double(vv[np + pj]),
-             double(vv[2 * np + pj]));
+        const int64_t pj = pidx(j);
+        body(j, double(vv[pj]), double(vv[np + pj]), double(vv[2 * np + pj]));
       }
   };
 
@@ -214,17 +226,22 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
     const ST* __restrict__ fs = (const ST*)a.Fs + n * np;
     const ST* __restrict__ fo = (const ST*)a.Fo + n * np;
     const ST* __restrict__ vv = (const ST*)a.V + n * 3 * np;
+    const ST* __restrict__ vo = GV ? (const ST*)a.Vo + n * 3 * np : vv;
     if (STAGE) {
       // the V rows copied by direct-to-LDS loads (lane l of a load writes
       // dword k + l of the run), all loads in flight together, one wait
-      const unsigned* g = reinterpret_cast<const unsigned*>(vv);
-      unsigned* l = reinterpret_cast<unsigned*>(Sv);
       const int64_t ndw = 3 * np * (int64_t)(sizeof(ST) / 4);
-      for (int64_t k = 0; k < ndw; k += 64)
-        if (k + lane < ndw)
-          __builtin_amdgcn_global_load_lds(g + k + lane, (__attribute__((address_space(3))) void*)(l + k), 4, 0, 0);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const unsigned* g = reinterpret_cast<const unsigned*>(t ? vo : vv);
+        unsigned* l = reinterpret_cast<unsigned*>(Sv) + t * ndw;
+        for (int64_t k = 0; k < ndw; k += 64)
+          if (k + lane < ndw)
+            __builtin_amdgcn_global_load_lds(g + k + lane, (__attribute__((address_space(3))) void*)(l + k), 4, 0, 0);
+      }
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       vv = Sv;
+      if (GV) vo = Sv + 3 * np;
     }
     if (own) {
       As[i] = a.A[n * nb + i];
@@ -258,98 +275,120 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
     double dAi[3] = {0, 0, 0};
     if (act)
       for (int c = 0; c < 3; ++c) dAi[c] = dAs[i * 3 + c];
-    double occ[2], sg[2][3];
+    double occ[2];
     for (int s = 0; s < NS; ++s) {
       const double x = own ? Xs[s * nb + i] : 0.0;
       occ[s] = uniform(wave_sum(x * Ai));  // calculateOccupiedElectronDensity
-      for (int c = 0; c < 3; ++c) sg[s][c] = uniform(wave_sum(x * dAi[c]));  // calculateScaledGradients
     }
-    // one pass over j: Faa, FaiS, V . D, Vai, Q (= VTSI / c_s) and
-    // W_s[i][b][c] = sum_j C_s[j][b] V_c(j, i) for the first kWMain orbitals
+    // one pass over j: Faa, FaiS, V . D (both tensors), Vai, Q (= VTSI / c_s)
+    // and W_s[i][b][c] = sum_j C_s[j][b] V_c(j, i) for the first kWMain orbitals
     double faa_p[2] = {0, 0}, fai[2] = {0, 0}, vd[2][3] = {{0, 0, 0}, {0, 0, 0}}, vai1[2] = {0, 0};
+    double vdo[2][3] = {{0, 0, 0}, {0, 0, 0}};  // GV: V_oppo . D_s
     double q[2][3] = {{0, 0, 0}, {0, 0, 0}};
     double wreg[2][kWMain][3];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int b = 0; b < kWMain; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
-    for_j(fs, fo, vv, [&](int j, double f_s, double f_o, double v0, double v1, double v2) {
-      w_accumulate(wreg, 0, j, v0, v1, v2);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const double dsj = Ds[s * nb2 + j * nb + i], dbj = Ds[(NS == 1 ? s : 1 - s) * nb2 + j * nb + i];
-        faa_p[s] += f_s * dsj + f_o * dbj;
-        const double xj = Xs[s * nb + j];
-        fai[s] += xj * f_s;
-        vd[s][0] += v0 * dsj;
-        vd[s][1] += v1 * dsj;
-        vd[s][2] += v2 * dsj;
-        const double* ks = kS + (s * nb + j) * 3;
-        vai1[s] += v0 * ks[0] + v1 * ks[1] + v2 * ks[2];
-        q[s][0] += xj * v0;
-        q[s][1] += xj * v1;
-        q[s][2] += xj * v2;
-      }
-    });
-    // W blocks of kWMain orbitals: the first from the main pass, the rest
-    // from passes of their own; each block goes to vout and, kWChunk orbitals
-    // at a time through LDS (over kS and Es, consumed / not yet written), adds
-    // its part of sum_ab |U_s[a][b]|^2, U_s[a][b][c] = sum_l C_s[l][a] W_s[l][b][c]
-    // (calculateVab)
-    double* vout = a.vecs + (int64_t)n * 2 * a.nvec * nb;
-    double nacc[2] = {0, 0};
-    for (int b0 = 0; b0 < nmax; b0 += kWMain) {
-      if (b0 > 0) {
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-#pragma unroll
-          for (int b = 0; b < kWMain; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
-        for_j(nullptr, nullptr, vv, [&](int j, double, double, double v0, double v1, double v2) {
-          w_accumulate(wreg, b0, j, v0, v1, v2);
-        });
-      }
-#pragma unroll
-      for (int c0 = 0; c0 < kWMain; c0 += kWChunk) {
-        if (b0 + c0 >= nmax) break;
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-#pragma unroll
-          for (int b = c0; b < c0 + kWChunk && b < kWMain; ++b)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              const double wv = part_sum(b0 + b < nocc[s] ? wreg[s][b][c] : 0.0, nbp);
-              if (own) {
-                Wc[((s * nb + i) * kWChunk + b - c0) * 3 + c] = wv;
-                if (b0 + b < nmax) vout[((int64_t)s * a.nvec + kVecFixed + 3 * (b0 + b) + c) * nb + i] = wv;
-              }
-            }
-        wave_sync_lds();
-        const int lim = std::min(nmax - b0 - c0, kWMain - c0), nbc = lim < kWChunk ? lim : kWChunk;
+    if (act)
+      for (int j = jp; j < nb; j += parts) {
+        const int64_t pj = pidx(j);
+        const double f_s = double(fs[pj]), f_o = double(fo[pj]);
+        const double v0 = double(vv[pj]), v1 = double(vv[np + pj]), v2 = double(vv[2 * np + pj]);
+        double o0 = 0, o1 = 0, o2 = 0;
+        if (GV) {
+          o0 = double(vo[pj]);
+          o1 = double(vo[np + pj]);
+          o2 = double(vo[2 * np + pj]);
+        }
+        w_accumulate(wreg, 0, j, v0, v1, v2);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-          const int nu = nocc[s] * nbc * 3;  // (a, b in the chunk, c)
-          for (int t = lane; t < nu; t += 64) {
-            const int aa = t / (nbc * 3), rem = t - aa * nbc * 3, bb = rem / 3, c = rem - bb * 3;
-            double u = 0;
-            for (int l = 0; l < nb; ++l)
-              u += Cs[((int64_t)s * nb + l) * nmax + aa] * Wc[((s * nb + l) * kWChunk + bb) * 3 + c];
-            nacc[s] += u * u;
+          const double dsj = Ds[s * nb2 + j * nb + i], dbj = Ds[(NS == 1 ? s : 1 - s) * nb2 + j * nb + i];
+          faa_p[s] += f_s * dsj + f_o * dbj;
+          const double xj = Xs[s * nb + j];
+          fai[s] += xj * f_s;
+          vd[s][0] += v0 * dsj;
+          vd[s][1] += v1 * dsj;
+          vd[s][2] += v2 * dsj;
+          if (GV) {
+            vdo[s][0] += o0 * dsj;
+            vdo[s][1] += o1 * dsj;
+            vdo[s][2] += o2 * dsj;
           }
+          const double* ks = kS + (s * nb + j) * 3;
+          vai1[s] += v0 * ks[0] + v1 * ks[1] + v2 * ks[2];
+          q[s][0] += xj * v0;
+          q[s][1] += xj * v1;
+          q[s][2] += xj * v2;
         }
-        wave_sync_lds();
       }
-    }
+    // W blocks of kWMain orbitals per tensor t: (t = 0, b0 = 0) from the main
+    // pass, the rest from passes of their own; each block goes to vout and,
+    // kWChunk orbitals at a time through LDS (over kS and Es, consumed / not yet
+    // written), adds its part of sum_ab |U^t_s[a][b]|^2,
+    // U^t_s[a][b][c] = sum_l C_s[l][a] W^t_s[l][b][c] (calculateVab)
+    double* vout = a.vecs + (int64_t)n * 2 * a.nvec * nb;
+    double nacc[NT][2];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) nacc[t][0] = nacc[t][1] = 0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      for (int b0 = 0; b0 < nmax; b0 += kWMain) {
+        if (t > 0 || b0 > 0) {
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int b = 0; b < kWMain; ++b) wreg[s][b][0] = wreg[s][b][1] = wreg[s][b][2] = 0;
+          for_j(t ? vo : vv, [&](int j, double v0, double v1, double v2) { w_accumulate(wreg, b0, j, v0, v1, v2); });
+        }
+#pragma unroll
+        for (int c0 = 0; c0 < kWMain; c0 += kWChunk) {
+          if (b0 + c0 >= nmax) break;
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int b = c0; b < c0 + kWChunk && b < kWMain; ++b)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                const double wv = part_sum(b0 + b < nocc[s] ? wreg[s][b][c] : 0.0, nbp);
+                if (own) {
+                  Wc[((s * nb + i) * kWChunk + b - c0) * 3 + c] = wv;
+                  if (b0 + b < nmax)
+                    vout[((int64_t)s * a.nvec + kVecFixed + 3 * (t * nmax + b0 + b) + c) * nb + i] = wv;
+                }
+              }
+          wave_sync_lds();
+          const int lim = std::min(nmax - b0 - c0, kWMain - c0), nbc = lim < kWChunk ? lim : kWChunk;
+#pragma unroll
+          for (int s = 0; s < NS; ++s) {
+            const int nu = nocc[s] * nbc * 3;  // (a, b in the chunk, c)
+            for (int u = lane; u < nu; u += 64) {
+              const int aa = u / (nbc * 3), rem = u - aa * nbc * 3, bb = rem / 3, c = rem - bb * 3;
+              double uv = 0;
+              for (int l = 0; l < nb; ++l)
+                uv += Cs[((int64_t)s * nb + l) * nmax + aa] * Wc[((s * nb + l) * kWChunk + bb) * 3 + c];
+              nacc[t][s] += uv * uv;
+            }
+          }
+          wave_sync_lds();
+        }
+      }
     double faa[2], VTv[2][3];
-    double vdsum[2][3];
+    double vdsum[2][3], vdosum[2][3];
     for (int s = 0; s < NS; ++s) {
       faa[s] = uniform(wave_sum(faa_p[s]));  // FaaSamePlusOppo
       for (int c = 0; c < 3; ++c) vdsum[s][c] = uniform(wave_sum(vd[s][c]));
+      if (GV)
+        for (int c = 0; c < 3; ++c) vdosum[s][c] = uniform(wave_sum(vdo[s][c]));
       fai[s] = part_sum(fai[s], nbp);
       vai1[s] = part_sum(vai1[s], nbp);
       for (int c = 0; c < 3; ++c) q[s][c] = part_sum(q[s][c], nbp);
     }
+    // VaaSamePlusOppo == VTilde: V_same . D_s + V_oppo . D_s'
     for (int s = 0; s < NS; ++s)
-      for (int c = 0; c < 3; ++c) VTv[s][c] = cs * vdsum[s][c] + co * vdsum[NS == 1 ? s : 1 - s][c];  // VaaSamePlusOppo == VTilde
+      for (int c = 0; c < 3; ++c)
+        VTv[s][c] = GV ? vdsum[s][c] + vdosum[NS == 1 ? s : 1 - s][c] : cs * vdsum[s][c] + co * vdsum[NS == 1 ? s : 1 - s][c];
     if (own)
       for (int s = 0; s < NS; ++s)
         for (int c = 0; c < 3; ++c) VT[(s * nb + i) * 3 + c] = cs * q[s][c];  // VTildeSubI (== Vja)
@@ -377,7 +416,7 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
     wave_sync_lds();
     // SumOverb_Vbi_dot_VTildeb_s[i] = c_s sum_k V(k,i) . E_s[k]
     double sv1[2] = {0, 0};
-    for_j(nullptr, nullptr, vv, [&](int k, double, double, double v0, double v1, double v2) {
+    for_j(vv, [&](int k, double v0, double v1, double v2) {
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         const double* e = Es + (s * nb + k) * 3;
@@ -386,11 +425,12 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
     });
     sv1[0] = part_sum(sv1[0], nbp);
     sv1[1] = part_sum(sv1[1], nbp);
-    double scvt[2][3], nrmU[2];
+    double scvt[2][3], nrmU[NT][2];
     for (int s = 0; s < NS; ++s) {
       const double x = own ? Xs[s * nb + i] : 0.0;
       for (int c = 0; c < 3; ++c) scvt[s][c] = uniform(wave_sum(own ? x * VT[(s * nb + i) * 3 + c] : 0.0));  // SumOverb_conjVTildeb_occb
-      nrmU[s] = uniform(wave_sum(nacc[s]));
+#pragma unroll
+      for (int t = 0; t < NT; ++t) nrmU[t][s] = uniform(wave_sum(nacc[t][s]));
     }
     const double wn = a.w[n];
     for (int s = 0; s < NS; ++s) {
@@ -402,7 +442,9 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
       const double vtn = VTv[s][0] * VTv[s][0] + VTv[s][1] * VTv[s][1] + VTv[s][2] * VTv[s][2];
       const double svbi = cs * sv1[s];
       const double Z = 2 * wn * (svbi - vtdot);
-      const double sNS = cs * cs * nrmU[s], sNOb = co * co * nrmU[sb];
+      // sumOfNormsVabSame_s and sumOfNormsVabOppo of the other spin
+      const double sNS = GV ? nrmU[0][s] : cs * cs * nrmU[0][s];
+      const double sNOb = GV ? nrmU[NT - 1][sb] : co * co * nrmU[0][sb];
       if (own) {
         vs[0 * nb + i] = wn * (faa[s] * Ai - fai[s]);
         vs[1 * nb + i] = wn * (VTv[s][0] * dAi[0] + VTv[s][1] * dAi[1] + VTv[s][2] * dAi[2] - cs * vai1[s]);
@@ -412,13 +454,26 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
         vs[5 * nb + i] = vtsi1;
         vs[6 * nb + i] = vtsi2;
       }
-      if (lane == 0) a.alpha[n * 2 + s] = -2 * wn * (occ[s] * cs * cs + occ[sb] * co * co);
+      double* al = a.alpha + (n * 2 + s) * 2;
+      if (lane == 0) {
+        if (GV) {
+          al[0] = -2 * wn * occ[s];
+          al[1] = -2 * wn * occ[sb];
+        } else {
+          al[0] = -2 * wn * (occ[s] * cs * cs + occ[sb] * co * co);
+        }
+      }
+      // G3's V-linear terms: h_same . V_same(j,i) + h_oppo . V_oppo(j,i)
       if (lane < 3) {
         const int c = lane;
         double* h = a.hvec + ((int64_t)n * 2 + s) * 6;
-        h[c] = wn * (cs * sg[s][c] + co * sg[sb][c]);
-        h[3 + c] = wn * (2 * occ[s] * cs * VTv[s][c] - 2 * cs * scvt[s][c] + 2 * occ[sb] * co * VTv[sb][c] -
-                         2 * co * scvt[sb][c]);
+        if (GV) {
+          h[c] = wn * (2 * occ[s] * VTv[s][c] - 2 * scvt[s][c]);
+          h[3 + c] = wn * (2 * occ[sb] * VTv[sb][c] - 2 * scvt[sb][c]);
+        } else {
+          h[c] = wn * (2 * occ[s] * cs * VTv[s][c] - 2 * cs * scvt[s][c] + 2 * occ[sb] * co * VTv[sb][c] -
+                       2 * co * scvt[sb][c]);
+        }
       }
     }
     wave_sync_lds();
@@ -427,20 +482,20 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4) ? 768 : 512, 1) void t
 
 struct RankArgs {
   const double* vecs;   // [n1][2][nvec][nb]
-  const double* alpha;  // [n1][2]
+  const double* alpha;  // [n1][2 spins][2 tensors]
   const double* A;      // [n1][nb]
   const double* dA;     // [n1][nb][3]
   const double* w;      // [n1]
   int64_t n1;
-  int nb, nvec, nocc0, nocc1;
+  int nb, nvec, nocc0, nocc1, nmax, nwt;  // nwt: V tensors with W blocks (1 or 2)
   int ustart[7];        // workgroups of target 3 s + t: [ustart[st], ustart[st + 1])
   double* part;         // [gridDim.x][nb][nb]
 };
 
-// (point, entry) terms per point of target st = 3 s + t: 1, 4, 5 + 3 nocc_s.
-__host__ __device__ inline int rank_entries(int st, int nocc0, int nocc1) {
+// (point, entry) terms per point of target st = 3 s + t: 1, 4, 5 + 3 nwt nocc_s.
+__host__ __device__ inline int rank_entries(int st, int nocc0, int nocc1, int nwt) {
   const int s = st / 3, tg = st - 3 * s;
-  return tg == 0 ? 1 : (tg == 1 ? 4 : 5 + 3 * (s ? nocc1 : nocc0));
+  return tg == 0 ? 1 : (tg == 1 ? 4 : 5 + 3 * nwt * (s ? nocc1 : nocc0));
 }
 
 // Pass 2: G_{s,t}[j][i] = sum over (point, entry) of L[j] R[i] (the rank-one
@@ -459,7 +514,8 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
   while (st < 5 && u >= a.ustart[st + 1]) ++st;
   const int s = st / 3, tg = st - 3 * s;
   const int nb = a.nb;
-  const int E = rank_entries(st, a.nocc0, a.nocc1);
+  const int E = rank_entries(st, a.nocc0, a.nocc1, a.nwt);
+  const int w3 = 3 * (s ? a.nocc1 : a.nocc0);  // W entries per tensor
   // this workgroup's terms: k = kb + [0, K) of the target's n1 E (point, entry) terms
   const int64_t kb = (int64_t)(u - a.ustart[st]) * kRankChunk;
   const int K = (int)(a.n1 * E - kb < kRankChunk ? a.n1 * E - kb : kRankChunk);
@@ -489,8 +545,8 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
     const int64_t n = kin ? p0 + q : p0;
     if (!kin) e = 0;
     // entry e of target tg: L / R source (0 the point's vectors, 1 A, 2 dA),
-    // offset and stride, and the scale
-    int ls = 1, lo = 0, rs = 0, ro = 0, rst = 1, sk = 0;
+    // offset and stride, and the scale (W entries of tensor wt use alpha_wt)
+    int ls = 1, lo = 0, rs = 0, ro = 0, rst = 1, sk = 0, wt = 0;
     if (tg == 1) {
       const bool z = e == 0;
       ls = z ? 1 : 0;
@@ -501,8 +557,10 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
       sk = z ? 0 : 1;
     } else if (tg == 2) {
       const bool z = e == 0, one = e == 1, tc3 = e >= 2 && e < 5;
-      ls = z ? 1 : 0;  // (A, u2) | (u3, A) | (2 w t_c, t_c) | (alpha W_bc, W_bc)
-      lo = z ? 0 : (one ? 3 * nb : (tc3 ? (2 + e) * nb : (kVecFixed + e - 5) * nb));
+      const int ww = e - 5;
+      wt = ww >= w3 ? 1 : 0;
+      ls = z ? 1 : 0;  // (A, u2) | (u3, A) | (2 w t_c, t_c) | (alpha_wt W^wt_bc, W^wt_bc)
+      lo = z ? 0 : (one ? 3 * nb : (tc3 ? (2 + e) * nb : (kVecFixed + (wt ? 3 * a.nmax + ww - w3 : ww)) * nb));
       rs = one ? 1 : 0;
       ro = z ? 2 * nb : (one ? 0 : lo);
       sk = z || one ? 0 : (tc3 ? 2 : 3);
@@ -513,7 +571,7 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
     const double* L = (ls == 0 ? vn : An) + lo;
     const double* R = (rs == 0 ? vn : (rs == 1 ? An : dAn)) + ro;
     o.w = a.w[n];
-    o.al = a.alpha[n * 2 + s];
+    o.al = a.alpha[(n * 2 + s) * 2 + wt];
     o.sk = kin ? sk : 4;
 #pragma unroll
     for (int m = 0; m < NT; ++m) {
@@ -573,11 +631,12 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
   for (int e2 = threadIdx.x; e2 < nb * nb; e2 += 256) out[e2] = red[e2 / nb][e2 % nb];
 }
 
-// Pass 3: Rv[s][target][p] = sum_n sum_c h[n][s][target][c] V[n][c][p], for the
-// NK = 2 x spins (s, target) pairs.
-template <class ST, int NK>
-__global__ __launch_bounds__(256) void tc_vlin_kernel(const ST* __restrict__ V, const double* __restrict__ hvec,
-                                                      int64_t n1, int64_t npair, double* __restrict__ part) {
+// Pass 3: Rv[s][p] = sum_n sum_t sum_c h[n][s][t][c] V_t[n][c][p] for the NK
+// spins formed and the NT tensors (V, or V_same and V_oppo).
+template <class ST, int NK, int NT>
+__global__ __launch_bounds__(256) void tc_vlin_kernel(const ST* __restrict__ V, const ST* __restrict__ Vo,
+                                                      const double* __restrict__ hvec, int64_t n1, int64_t npair,
+                                                      double* __restrict__ part) {
   const int64_t per = (n1 + gridDim.x - 1) / gridDim.x;
   const int64_t p0 = (int64_t)blockIdx.x * per;
   const int64_t p1 = p0 + per < n1 ? p0 + per : n1;
@@ -585,10 +644,17 @@ __global__ __launch_bounds__(256) void tc_vlin_kernel(const ST* __restrict__ V, 
     double acc[NK] = {};
 #pragma unroll 4
     for (int64_t n = p0; n < p1; ++n) {  // unrolled: several points' loads in flight
-      const double v0 = V[(n * 3) * npair + p], v1 = V[(n * 3 + 1) * npair + p], v2 = V[(n * 3 + 2) * npair + p];
       const double* h = hvec + n * 12;
 #pragma unroll
-      for (int k = 0; k < NK; ++k) acc[k] += v0 * h[3 * k] + v1 * h[3 * k + 1] + v2 * h[3 * k + 2];
+      for (int t = 0; t < NT; ++t) {
+        const ST* Vt = t ? Vo : V;
+        const double v0 = Vt[(n * 3) * npair + p], v1 = Vt[(n * 3 + 1) * npair + p], v2 = Vt[(n * 3 + 2) * npair + p];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          const double* hk = h + (k * 2 + t) * 3;
+          acc[k] += v0 * hk[0] + v1 * hk[1] + v2 * hk[2];
+        }
+      }
     }
 #pragma unroll
     for (int k = 0; k < NK; ++k) part[((int64_t)blockIdx.x * NK + k) * npair + p] = acc[k];
@@ -636,14 +702,17 @@ double now_ms() {
 
 inline int64_t pk(int j, int i) { return j <= i ? (int64_t)i * (i + 1) / 2 + j : (int64_t)j * (j + 1) / 2 + i; }
 
-// The point-kernel instance for a storage type, staging choice and spin count.
-template <int NS>
+// The point-kernel instance for a storage type, staging choice, spin count and
+// V form (independent V_same / V_oppo or one proportional tensor).
+template <int NS, bool GV>
 inline const void* point_kernel_ns(int f32, int stage) {
-  if (f32) return stage ? (const void*)tc_point_kernel<float, true, NS> : (const void*)tc_point_kernel<float, false, NS>;
-  return stage ? (const void*)tc_point_kernel<double, true, NS> : (const void*)tc_point_kernel<double, false, NS>;
+  if (f32)
+    return stage ? (const void*)tc_point_kernel<float, true, NS, GV> : (const void*)tc_point_kernel<float, false, NS, GV>;
+  return stage ? (const void*)tc_point_kernel<double, true, NS, GV> : (const void*)tc_point_kernel<double, false, NS, GV>;
 }
-inline const void* point_kernel(int f32, int stage, bool restricted) {
-  return restricted ? point_kernel_ns<1>(f32, stage) : point_kernel_ns<2>(f32, stage);
+inline const void* point_kernel(int f32, int stage, bool restricted, bool gv) {
+  if (gv) return restricted ? point_kernel_ns<1, true>(f32, stage) : point_kernel_ns<2, true>(f32, stage);
+  return restricted ? point_kernel_ns<1, false>(f32, stage) : point_kernel_ns<2, false>(f32, stage);
 }
 
 }  // namespace tcs
@@ -652,32 +721,33 @@ inline const void* point_kernel(int f32, int stage, bool restricted) {
 namespace tc {
 using namespace tcs;
 
-size_t tc_out_doubles(int nb) { return 6 * (size_t)nb * nb + 4 * (size_t)nb * (nb + 1) / 2; }
+size_t tc_out_doubles(int nb) { return 6 * (size_t)nb * nb + 2 * (size_t)nb * (nb + 1) / 2; }
 
 // Rank workgroups per target st = 3 s + t (kRankChunk terms each), as prefix
 // offsets ustart[0..6].
-static void rank_units(int nup, int ndown, int64_t n1, int* ustart) {
+static void rank_units(int nup, int ndown, int nwt, int64_t n1, int* ustart) {
   ustart[0] = 0;
   for (int st = 0; st < 6; ++st)
-    ustart[st + 1] = ustart[st] + (int)((n1 * rank_entries(st, nup, ndown) + kRankChunk - 1) / kRankChunk);
+    ustart[st + 1] = ustart[st] + (int)((n1 * rank_entries(st, nup, ndown, nwt) + kRankChunk - 1) / kRankChunk);
 }
 
-size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1) {
-  const int nmax = std::max(1, std::max(nup, ndown));
-  const size_t nb2 = (size_t)nb * nb, npair = (size_t)nb * (nb + 1) / 2, nvec = kVecFixed + 3 * nmax;
+size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1, bool general_v) {
+  const int nmax = std::max(1, std::max(nup, ndown)), nwt = general_v ? 2 : 1;
+  const size_t nb2 = (size_t)nb * nb, npair = (size_t)nb * (nb + 1) / 2, nvec = kVecFixed + 3 * nmax * nwt;
   int us[7];
-  rank_units(nup, ndown, n1, us);
-  return 2 * nb2 + 2 * (size_t)nb * nmax + (size_t)n1 * 2 * nvec * nb + (size_t)n1 * 2 + (size_t)n1 * 12 +
-         (size_t)us[6] * nb2 + (size_t)kBlocks * 4 * npair + tc_out_doubles(nb);
+  rank_units(nup, ndown, nwt, n1, us);
+  return 2 * nb2 + 2 * (size_t)nb * nmax + (size_t)n1 * 2 * nvec * nb + (size_t)n1 * 4 + (size_t)n1 * 12 +
+         (size_t)us[6] * nb2 + (size_t)kBlocks * 2 * npair + tc_out_doubles(nb);
 }
 
 // waves per workgroup of the unstaged point kernel (halved while the LDS does not fit)
 constexpr int kUnstagedWaves = 8;
 
 int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1,
-               int64_t n1, const void* Fs, const void* Fo, const void* V, int f_dtype, const tcscf_jastrow* jp,
-               double* ws) {
-  TC_CHECK(A1 && dA1 && w1 && Fs && Fo && V && jp && ws, "tcscf_tc_fock: null");
+               int64_t n1, const void* Fs, const void* Fo, const void* V, const void* Vo, int f_dtype,
+               const tcscf_jastrow* jp, double* ws) {
+  TC_CHECK(A1 && dA1 && w1 && Fs && Fo && V && ws, "tcscf_tc_fock: null");
+  TC_CHECK(Vo || jp, "tcscf_tc_fock: one V tensor needs the Jastrow's c_same / c_oppo");
   TC_CHECK(f_dtype == TCSCF_F64 || f_dtype == TCSCF_F32, "tcscf_tc_fock: f_dtype must be TCSCF_F64 or TCSCF_F32");
   p.f32 = f_dtype == TCSCF_F32;
   TC_CHECK(nb > 0 && nb <= 64, "tcscf_tc_fock: 1 <= nb <= 64");
@@ -686,7 +756,9 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   p.ndown = ndown;
   p.nb = nb;
   p.nmax = std::max(1, std::max(nup, ndown));
-  p.nvec = kVecFixed + 3 * p.nmax;
+  p.gv = Vo != nullptr;
+  p.nwt = p.gv ? 2 : 1;
+  p.nvec = kVecFixed + 3 * p.nmax * p.nwt;
   p.n1 = n1;
   p.npair = (int64_t)nb * (nb + 1) / 2;
   p.A1 = A1;
@@ -695,40 +767,52 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   p.Fs = Fs;
   p.Fo = Fo;
   p.V = V;
-  p.c_same = jp->c_same;
-  p.c_oppo = jp->c_oppo;
+  p.Vo = Vo;
+  p.c_same = p.gv ? 1.0 : jp->c_same;
+  p.c_oppo = p.gv ? 1.0 : jp->c_oppo;
   const int64_t nb2 = (int64_t)nb * nb;
   p.dD = ws;
   p.dC = p.dD + 2 * nb2;
   p.dVec = p.dC + 2 * (int64_t)nb * p.nmax;
   p.dAl = p.dVec + n1 * 2 * p.nvec * nb;
-  p.dH = p.dAl + n1 * 2;
+  p.dH = p.dAl + n1 * 4;
   p.dPG = p.dH + n1 * 12;
-  rank_units(nup, ndown, n1, p.ustart);
+  rank_units(nup, ndown, p.nwt, n1, p.ustart);
   p.dPV = p.dPG + (int64_t)p.ustart[6] * nb2;
-  p.dOut = p.dPV + (int64_t)kBlocks * 4 * p.npair;
-  // Point-kernel launch per spin count (index 0: two spins, 1: restricted).
-  // The V rows are staged in LDS when 8 waves of a workgroup hold them (the
-  // gathers then come from LDS and 8 waves hide their latency), else the
-  // gathers go to memory with up to 4 waves per workgroup.
+  p.dOut = p.dPV + (int64_t)kBlocks * 2 * p.npair;
+  TC_CHECK(tc_launch_shape(p), "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
+  for (int r = 0; r < 2; ++r)
+    if (p.sh1[r] > 64 * 1024) TC_HIP(lds_opt_in(point_kernel(p.f32, p.stage[r], r, p.gv)));
+  return 0;
+}
+
+// Point-kernel launch per spin count (index 0: two spins, 1: restricted), from
+// nb, nmax, the storage type and the V form: the V rows are staged in LDS when
+// 8 waves of a workgroup hold them (the gathers then come from LDS and 8 waves
+// hide their latency), else the gathers go to memory with 8 waves per
+// workgroup.  False when even one wave does not fit.
+bool tc_launch_shape(TcPlan& p) {
+  const int nb = p.nb;
+  const int64_t nb2 = (int64_t)nb * nb;
   for (int r = 0; r < 2; ++r) {
     const int ns = r ? 1 : 2;
     const size_t shared = sizeof(double) * ns * (nb2 + (int64_t)nb * p.nmax);  // D and C
     const size_t staged_wave =
-        sizeof(double) * (point_wave_elems(nb, ns) + (p.f32 ? staged_doubles<float>(true, nb) : staged_doubles<double>(true, nb)));
-    p.stage[r] = shared + 8 * staged_wave <= 160 * 1024 ? 1 : 0;
+        sizeof(double) * (point_wave_elems(nb, ns) +
+                          (p.f32 ? staged_doubles<float>(true, nb, p.nwt) : staged_doubles<double>(true, nb, p.nwt)));
+    p.stage[r] = shared + 8 * staged_wave <= (size_t)kMaxLdsBytes ? 1 : 0;
     const size_t per_wave = p.stage[r] ? staged_wave : sizeof(double) * point_wave_elems(nb, ns);
     // staged: 8 waves per workgroup, 12 for the one-spin kernel on FP32 rows
-    // (three waves per SIMD fit its registers) when LDS holds them; unstaged:
-    // 8, so that one copy of D and C in LDS serves more waves
-    p.wpw[r] = p.stage[r] ? (ns == 1 && p.f32 && shared + 12 * per_wave <= 160 * 1024 ? 12 : 8) : kUnstagedWaves;
-    while (p.wpw[r] > 1 && shared + p.wpw[r] * per_wave > 160 * 1024) p.wpw[r] /= 2;
+    // of one V tensor (three waves per SIMD fit its registers) when LDS holds
+    // them; unstaged: 8, so that one copy of D and C in LDS serves more waves
+    p.wpw[r] = p.stage[r] ? (ns == 1 && p.f32 && !p.gv && shared + 12 * per_wave <= (size_t)kMaxLdsBytes ? 12 : 8)
+                          : kUnstagedWaves;
+    while (p.wpw[r] > 1 && shared + p.wpw[r] * per_wave > (size_t)kMaxLdsBytes) p.wpw[r] /= 2;
     p.sh1[r] = shared + p.wpw[r] * per_wave;
-    TC_CHECK(p.sh1[r] <= (size_t)kMaxLdsBytes, "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
-    if (p.sh1[r] > 64 * 1024) TC_HIP(lds_opt_in(point_kernel(p.f32, p.stage[r], r)));
-    p.pblocks[r] = std::max<int64_t>(1, std::min<int64_t>(2048, (n1 + p.wpw[r] - 1) / p.wpw[r]));
+    if (p.sh1[r] > (size_t)kMaxLdsBytes) return false;
+    p.pblocks[r] = std::max<int64_t>(1, std::min<int64_t>(2048, (p.n1 + p.wpw[r] - 1) / p.wpw[r]));
   }
-  return 0;
+  return true;
 }
 
 bool tc_restricted(const TcPlan& p, const double* D, const double* C) {
@@ -743,13 +827,13 @@ int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, doub
   const int nst = restricted ? 3 : 6;  // rank targets (spin, term) formed
   TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * 2 * nb2, hipMemcpyHostToDevice, st));
   TC_HIP(hipMemcpyAsync(p.dC, C_host, sizeof(double) * 2 * p.nb * p.nmax, hipMemcpyHostToDevice, st));
-  PointArgs a{p.Fs, p.Fo, p.V, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb, p.nmax,
-              p.nup, p.ndown, p.nvec, p.dVec, p.dAl, p.dH};
+  PointArgs a{p.Fs, p.Fo, p.V, p.Vo, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb,
+              p.nmax, p.nup, p.ndown, p.nvec, p.dVec, p.dAl, p.dH};
   void* pargs[] = {&a};
   const int r = restricted ? 1 : 0;
-  TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage[r], restricted), dim3((unsigned)p.pblocks[r]), dim3(64 * p.wpw[r]),
-                         pargs, p.sh1[r], st));
-  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, {}, p.dPG};
+  TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage[r], restricted, p.gv), dim3((unsigned)p.pblocks[r]),
+                         dim3(64 * p.wpw[r]), pargs, p.sh1[r], st));
+  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.nmax, p.nwt, {}, p.dPG};
   RankRanges rr{};
   for (int q = 0; q < 7; ++q) ra.ustart[q] = rr.ustart[q] = p.ustart[q];
   const dim3 rgrid((unsigned)p.ustart[nst]);
@@ -760,16 +844,19 @@ int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, doub
   else
     hipLaunchKernelGGL(tc_rank_kernel<4>, rgrid, dim3(256), 0, st, ra);
   TC_HIP(hipGetLastError());
-  const int nk = restricted ? 2 : 4;  // (spin, target) pairs of the V-linear sums
+  const int nk = restricted ? 1 : 2;  // spins of the V-linear sums
   const dim3 vg(kBlocks), vb(256);
-  if (p.f32 && restricted)
-    hipLaunchKernelGGL((tc_vlin_kernel<float, 2>), vg, vb, 0, st, (const float*)p.V, p.dH, p.n1, p.npair, p.dPV);
-  else if (p.f32)
-    hipLaunchKernelGGL((tc_vlin_kernel<float, 4>), vg, vb, 0, st, (const float*)p.V, p.dH, p.n1, p.npair, p.dPV);
-  else if (restricted)
-    hipLaunchKernelGGL((tc_vlin_kernel<double, 2>), vg, vb, 0, st, (const double*)p.V, p.dH, p.n1, p.npair, p.dPV);
-  else
-    hipLaunchKernelGGL((tc_vlin_kernel<double, 4>), vg, vb, 0, st, (const double*)p.V, p.dH, p.n1, p.npair, p.dPV);
+#define TC_VLIN(ST, NK, NT)                                                                                       \
+  hipLaunchKernelGGL((tc_vlin_kernel<ST, NK, NT>), vg, vb, 0, st, (const ST*)p.V, (const ST*)(p.gv ? p.Vo : p.V), \
+                     p.dH, p.n1, p.npair, p.dPV)
+  if (p.f32) {
+    if (p.gv) { if (restricted) TC_VLIN(float, 1, 2); else TC_VLIN(float, 2, 2); }
+    else { if (restricted) TC_VLIN(float, 1, 1); else TC_VLIN(float, 2, 1); }
+  } else {
+    if (p.gv) { if (restricted) TC_VLIN(double, 1, 2); else TC_VLIN(double, 2, 2); }
+    else { if (restricted) TC_VLIN(double, 1, 1); else TC_VLIN(double, 2, 1); }
+  }
+#undef TC_VLIN
   TC_HIP(hipGetLastError());
   hipLaunchKernelGGL(tc_reduce_ranges, dim3((unsigned)(nst * nb2)), dim3(256), 0, st, p.dPG, rr, nb2, p.dOut);
   TC_HIP(hipGetLastError());
@@ -788,13 +875,10 @@ void tc_assemble(const TcPlan& p, const double* out, bool restricted, double* G2
     const double* Sc = out + (so * 3 + 0) * nb2;
     const double* Vc = out + (so * 3 + 1) * nb2;
     const double* S3 = out + (so * 3 + 2) * nb2;
-    const double* Rvec = out + 6 * nb2 + (so * 2 + 0) * p.npair;
-    const double* Rg3 = out + 6 * nb2 + (so * 2 + 1) * p.npair;
+    const double* Rg3 = out + 6 * nb2 + so * p.npair;
     for (int j = 0; j < nb; ++j)
       for (int i = 0; i < nb; ++i) {
-        const double vji = Vc[j * nb + i] + Rvec[pk(j, i)];
-        const double vij = Vc[i * nb + j] + Rvec[pk(i, j)];
-        G2[s * nb2 + j * nb + i] = Sc[j * nb + i] / 2 + (vji / 2 - vij / 2);
+        G2[s * nb2 + j * nb + i] = Sc[j * nb + i] / 2 + (Vc[j * nb + i] / 2 - Vc[i * nb + j] / 2);
         G3[s * nb2 + j * nb + i] = (S3[j * nb + i] + Rg3[pk(j, i)]) / 2;
       }
   }
@@ -901,15 +985,16 @@ extern "C" {
  * orbitals C (2 x nb x nmax, column b of spin s valid for b < nocc_s) to
  * G2 and G3 (each 2 x nb x nb, host), before the one-electron mask. */
 int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1, const double* dA1, const double* w1, int64_t n1,
-                  const void* Fs, const void* Fo, const void* V, int f_dtype, const tcscf_jastrow* jp,
-                  const double* D_host, const double* C_host, double* G2_host, double* G3_host, void* stream) {
+                  const void* Fs, const void* Fo, const void* V, const void* Vo, int f_dtype,
+                  const tcscf_jastrow* jp, const double* D_host, const double* C_host, double* G2_host,
+                  double* G3_host, void* stream) {
   TC_CHECK(D_host && C_host && G2_host && G3_host, "tcscf_tc_fock: null");
   hipStream_t st = (hipStream_t)stream;
   int rc = 0;
-  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1), &rc);
+  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1, Vo != nullptr), &rc);
   if (rc) return rc;
   TcPlan plan;
-  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, f_dtype, jp, ws))) return rc;
+  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, Vo, f_dtype, jp, ws))) return rc;
   std::vector<double> out(tc_out_doubles(nb));
   const bool restricted = tc_restricted(plan, D_host, C_host);
   if ((rc = tc_enqueue(plan, D_host, C_host, out.data(), restricted, st))) return rc;
@@ -918,24 +1003,49 @@ int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1, const double* dA
   return 0;
 }
 
+/* The point-kernel launch the TC Fock build picks: info[0..2] = V rows staged
+ * in LDS, waves per workgroup, LDS bytes for two spins; info[3..5] the same
+ * for the restricted closed-shell build. */
+int tcscf_tc_launch_info(int nup, int ndown, int nb, int f_dtype, int general_v, int* info) {
+  TC_CHECK(info && nb > 0 && nb <= 64 && nup >= 0 && ndown >= 0 && nup <= nb && ndown <= nb,
+           "tcscf_tc_launch_info: bad arguments");
+  TC_CHECK(f_dtype == TCSCF_F64 || f_dtype == TCSCF_F32, "tcscf_tc_launch_info: f_dtype must be TCSCF_F64 or TCSCF_F32");
+  TcPlan p{};
+  p.nb = nb;
+  p.nmax = std::max(1, std::max(nup, ndown));
+  p.f32 = f_dtype == TCSCF_F32;
+  p.gv = general_v ? 1 : 0;
+  p.nwt = p.gv ? 2 : 1;
+  p.n1 = 1;
+  TC_CHECK(tc_launch_shape(p), "tcscf_tc_launch_info: basis / occupied orbitals too large for the LDS staging");
+  for (int r = 0; r < 2; ++r) {
+    info[3 * r] = p.stage[r];
+    info[3 * r + 1] = p.wpw[r];
+    info[3 * r + 2] = (int)p.sh1[r];
+  }
+  return 0;
+}
+
 /* TCHartreeFock::compute (src/HartreeFock.cpp:1007-1362), real basis.
- * Fs, Fo: packed n1 x npair; V: packed n1 x 3 x npair for c = 1 (all three
- * f_dtype: TCSCF_F32 runs the per-point terms of the Fock build in FP32);
- * A1: n1 x nb; dA1: n1 x nb x 3.  D_out 2 x nb x nb, pseudo_out nup + ndown. */
+ * Fs, Fo: packed n1 x npair; Vs, Vo: packed n1 x 3 x npair V_same / V_oppo,
+ * or Vo = NULL and Vs the c = 1 tensor scaled by jp's c_same / c_oppo (all
+ * f_dtype: TCSCF_F32 stores them in FP32, the Fock build's arithmetic is
+ * FP64); A1: n1 x nb; dA1: n1 x nb x 3.  D_out 2 x nb x nb, pseudo_out
+ * nup + ndown, obs: the solver's observable members (may be NULL). */
 int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, const double* A1, const double* dA1,
-                 const double* w1, int64_t n1, const void* Fs, const void* Fo, const void* V, int f_dtype,
-                 const tcscf_jastrow* jp, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
-                 double* D_out, double* pseudo_out, void* stream) {
+                 const double* w1, int64_t n1, const void* Fs, const void* Fo, const void* Vs, const void* Vo,
+                 int f_dtype, const tcscf_jastrow* jp, const tcscf_scf_opts* o, tcscf_scf_result* res, double* trace,
+                 double* D_out, double* pseudo_out, tcscf_scf_observables* obs, void* stream) {
   TC_CHECK(S && H && o && res && o->max_iter > 0, "tcscf_tc_scf: bad arguments");
   TC_CHECK(o->diis >= 0, "tcscf_tc_scf: diis depth must be >= 0");
   TC_CHECK(o->diis == 0 || o->mgs_mode == 1, "tcscf_tc_scf: DIIS needs S-orthonormal orbitals (mgs_mode = 1)");
   hipStream_t st = (hipStream_t)stream;
   const int64_t nb2 = (int64_t)nb * nb;
   int rc = 0;
-  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1), &rc);
+  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1, Vo != nullptr), &rc);
   if (rc) return rc;
   TcPlan plan;
-  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, V, f_dtype, jp, ws))) return rc;
+  if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, Vs, Vo, f_dtype, jp, ws))) return rc;
   TcIter iter(nup, ndown, nb, S, H, o);
   std::vector<double> G2(2 * nb2), G3(2 * nb2);
   // D and C up, the result block down, through page-locked buffers (pageable
@@ -967,6 +1077,10 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
   }
   if (D_out) std::copy(iter.D.begin(), iter.D.end(), D_out);
   if (pseudo_out) std::copy(iter.pseudo.begin(), iter.pseudo.end(), pseudo_out);
+  if (obs) {
+    if (obs->fock) std::copy(iter.Fock.begin(), iter.Fock.end(), obs->fock);
+    if (obs->occupied) std::copy(iter.C.begin(), iter.C.end(), obs->occupied);
+  }
   if (res->status != TCSCF_OK) set_error("TC SCF: did not converge");
   return res->status;
 }

@@ -108,7 +108,7 @@ static void play_model(const Alloc& a, const void* src, void* dst, size_t n) {
   const int nmax = std::max(1, std::max(m.nup, m.ndown));
   const double* C = D + 2 * nb2;
   const size_t npair = (size_t)nb * (nb + 1) / 2;
-  if (n != sizeof(double) * (6 * nb2 + 4 * npair)) { std::printf("TC size mismatch %zu\n", n); std::abort(); }
+  if (n != sizeof(double) * (6 * nb2 + 2 * npair)) { std::printf("TC size mismatch %zu\n", n); std::abort(); }
   std::vector<double> Dt(nb2, 0.0);
   for (int s = 0; s < 2; ++s)
     for (size_t e = 0; e < nb2; ++e) Dt[e] += D[s * nb2 + e];
@@ -131,7 +131,7 @@ static void play_model(const Alloc& a, const void* src, void* dst, size_t n) {
       }
   }
   double* R = out + 6 * nb2;
-  for (size_t e = 0; e < 4 * npair; ++e) R[e] = m.eps * 1e-3 * std::sin(0.37 * e + a.idx);
+  for (size_t e = 0; e < 2 * npair; ++e) R[e] = m.eps * 1e-3 * std::sin(0.37 * e + a.idx);
 }
 
 static void run_op(const Op& o) {

@@ -167,6 +167,40 @@ static int gpu_runs(const std::string& dir) {
     auto Vsu = tcscf::computeV(g1, g3, u, true);
     tcscf::TCHartreeFock<double> tcu(2, 2, static_cast<int>(basis.size()));
     std::printf("be_tc_ochi %.15f\n", tcu.compute(S, H, g1, Fsu, Fou, Vsu, u, true, 1e-10, 200, 1));
+    // the reference's signature, VjiSame and VjiOppo as two tensors: the same Jastrow ...
+    auto Vou = tcscf::computeV(g1, g3, u, false);
+    tcscf::TCHartreeFock<double> tcr(2, 2, static_cast<int>(basis.size()));
+    std::printf("be_tc_ref_signature %.15f\n", tcr.compute(S, H, g1, Fsu, Fou, Vsu, Vou, true, 1e-10, 200, 1));
+    // ... and a general Ochi Jastrow whose spin coefficients are not proportional
+    const tcscf::OchiJastrow ug{1.1, 1.0, {{0.5, 0.25}, {-0.3, 0.2}, {0.15, -0.4}}, {{1, 0, 0}, {2, 1, 0}, {1, 1, 1}}};
+    auto Fsg = tcscf::computeF(g1, g1, ug, true);
+    auto Fog = tcscf::computeF(g1, g1, ug, false);
+    auto Vsg = tcscf::computeV(g1, g3, ug, true);
+    auto Vog = tcscf::computeV(g1, g3, ug, false);
+    tcscf::TCHartreeFock<double> tcg(2, 2, static_cast<int>(basis.size()));
+    std::printf("be_tc_general %.15f\nbe_tc_general_iters %d\n", tcg.compute(S, H, g1, Fsg, Fog, Vsg, Vog, true, 1e-10, 200, 1),
+                tcg.numberOfConvergenceLoops());
+    // observables: occupiedOrbitals (S-orthonormal under the metric Gram-Schmidt),
+    // density = C C^T, fockOperator
+    const int nb = static_cast<int>(basis.size());
+    double orth = 0, dens = 0;
+    for (int s = 0; s < 2; ++s) {
+      const auto& C = tcg.occupiedOrbitals[s];
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          double t = 0;
+          for (int j = 0; j < nb; ++j)
+            for (int k = 0; k < nb; ++k) t += C[j * 2 + a] * S[j * nb + k] * C[k * 2 + b];
+          orth = std::max(orth, std::fabs(t - (a == b ? 1.0 : 0.0)));
+        }
+      for (int j = 0; j < nb; ++j)
+        for (int k = 0; k < nb; ++k)
+          dens = std::max(dens, std::fabs(tcg.density[(s * nb + j) * nb + k] -
+                                          (C[j * 2] * C[k * 2] + C[j * 2 + 1] * C[k * 2 + 1])));
+    }
+    std::printf("be_tc_general_orth_err %.3e\nbe_tc_general_density_err %.3e\n", orth, dens);
+    std::printf("be_tc_general_fock_00 %.15f\nbe_tc_general_fock_1_23 %.15f\n", tcg.fockOperator[0],
+                tcg.fockOperator[(nb + 2) * nb + 3]);
   }
   // the per-element QMC route (AtomicBasis): 5-D atomic and 6-D arrays, hydrogen-like Z = 2
   {

@@ -329,3 +329,25 @@ def test_ochi_jastrow_host_logic(tc):
         kw.update(bad)
         with pytest.raises(ValueError):
             tc.OchiJastrow(**kw)
+
+
+def test_tc_launch_info_table(tc):
+    """The TC point kernel's launch choice (no GPU call): V rows staged in LDS
+    with 12 waves per workgroup for the restricted FP32 Ne build (nb 30,
+    5 + 5 electrons), 8 unstaged waves once the rows do not fit (FP32 two spins
+    at nb 44), the independent-V build staging both tensors at nb 30, at
+    least one wave for every nb <= 64 and an error past it."""
+    import ctypes
+    from tcscf_amd import _lib
+    def info(nup, ndown, nb, f32, gv):
+        out = (ctypes.c_int * 6)()
+        _lib.check(tc.lib().tcscf_tc_launch_info(nup, ndown, nb, _lib.F32 if f32 else _lib.F64, gv, out))
+        return list(out)
+    assert info(5, 5, 30, True, 0)[3:5] == [1, 12]
+    assert info(5, 5, 30, True, 0)[0:2] == [1, 8]
+    assert info(6, 5, 44, True, 0)[0:2] == [0, 8]
+    assert info(5, 5, 30, True, 1)[0:2] == [1, 8]
+    assert all(x[2] <= 160 * 1024 and x[5] <= 160 * 1024 for x in [info(9, 9, 60, True, 1), info(9, 9, 60, False, 0)])
+    assert info(60, 60, 60, False, 0)[1] >= 1
+    with pytest.raises(tc.TcscfError):
+        info(5, 5, 65, False, 0)

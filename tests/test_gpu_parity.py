@@ -255,6 +255,23 @@ def test_rcs_ne_small_grid_end_to_end(tc, O, gpu):
     assert hd.numberOfConvergenceLoops() < hf.numberOfConvergenceLoops()
     eu = tc.UOSHartreeFock(5, 5, 30).compute(S, H, g, F, True, tol=1e-10, diis=6)
     assert abs(eu - ref["energy"]) < 1e-9
+    # observables (src/HartreeFock.hpp:286-288, :342-344): the eigenvalues are
+    # the oracle's; the eigenvectors are S-orthonormal, ordered by eigenvalue and
+    # give the density; the Fock operator is the one the energy was read from
+    # (RCS E = sum D (H + F), the reference's 2 sum D (H + G/2))
+    assert np.max(np.abs(hf.eigenvalues - ref["evals"])) < 1e-8
+    Cv = hf.eigenvectors[0]
+    assert np.max(np.abs(Cv.T @ S @ Cv - np.eye(30))) < 1e-10
+    assert np.max(np.abs(Cv[:, :5] @ Cv[:, :5].T - hf.density[0])) < 1e-12
+    assert abs(np.sum(hf.density[0].T * (H + hf.fockOperator[0])) - e) < 1e-10 * abs(e)
+    assert np.allclose(hf.fockOperator[0], hf.fockOperator[0].T, atol=1e-12)
+    hu = tc.UOSHartreeFock(5, 5, 30)
+    eu = hu.compute(S, H, g, F, True, tol=1e-10)
+    for s_ in range(2):
+        Cv = hu.eigenvectors[s_]
+        assert np.max(np.abs(Cv.T @ S @ Cv - np.eye(30))) < 1e-10
+        assert np.max(np.abs(Cv[:, :5] @ Cv[:, :5].T - hu.density[s_])) < 1e-12
+    assert abs(0.5 * sum(np.sum(hu.density[s_].T * (H + hu.fockOperator[s_])) for s_ in range(2)) - eu) < 1e-10 * abs(eu)
 
 
 def test_rcs_ne_config2_full_size(tc, O, gpu):
@@ -441,9 +458,70 @@ def test_tc_scf_general_ochi_vs_oracle(tc, O, gpu):
     assert ref["status"] == 0
     assert abs(e - ref["energy"]) < 1e-9, (e, ref["energy"])
     assert np.max(np.abs(hf.density - ref["D"])) < 1e-7
+    # the reference's signature with both tensors: the same fixed point
+    e2 = tc.TCHartreeFock(2, 2, nb).compute(S, H, g1, Fs, Fo, Vs, Vo, True, tol=1e-10, max_iter=200, mgs_mode=1)
+    assert abs(e2 - e) < 1e-10
     bad = tc.OchiJastrow(1.1, 1.0, [(0.5, 0.25), (0.3, 0.1)], _OCHI_TERMS[:2])
     with pytest.raises(ValueError):
         tc.TCHartreeFock(2, 2, nb).compute(S, H, g1, Fs, Fo, Vs, bad, True, tol=1e-10)
+
+
+@pytest.mark.parametrize("nup,ndown,mgs_mode,dtype", [(2, 2, 1, "fp64"), (3, 2, 1, "fp64"), (2, 2, 0, "fp64"),
+                                                      (3, 2, 1, "fp32")])
+def test_tc_scf_independent_V_vs_oracle(tc, O, gpu, nup, ndown, mgs_mode, dtype):
+    """TCHartreeFock::compute with the reference's signature
+    (src/HartreeFock.hpp:386-394): F_same, F_oppo, V_same and V_oppo from a
+    general Ochi Jastrow whose spin coefficients are not proportional, so the
+    two V tensors are independent; against the oracle fed the same tensors
+    (energy trace 1e-9 Eh; FP32 storage against the oracle fed the rounded
+    values).  The observables: fockOperator is H + G2 - G3 of the final
+    density (mask applied), the occupied orbitals give the density and are
+    S-orthonormal under the metric Gram-Schmidt."""
+    torch = gpu
+    basis = tc.even_tempered({0: 5, 1: 2}, 0.6, 2.2)
+    nb = len(basis)
+    g1 = _grid(tc, 14, 11, tc.RADIAL_M4)
+    g3 = _grid(tc, 10, 7, tc.RADIAL_M4)
+    for g in (g1, g3):
+        g.setBasisFunctions(basis, True)
+    c = [(0.5, 0.25), (-0.3, 0.2), (0.15, -0.4), (0.1, 0.05)]
+    u = tc.OchiJastrow(1.1, 1.0, c, _OCHI_TERMS)
+    assert u.spin_ratio() is None
+    Fs = tc.computeF(g1, g1, kernel="tc", jastrow=u, same_spin=True)
+    Fo = tc.computeF(g1, g1, kernel="tc", jastrow=u, same_spin=False)
+    Vs = tc.computeV(g1, g3, u, same_spin=True)
+    Vo = tc.computeV(g1, g3, u, same_spin=False)
+    if dtype == "fp32":
+        Fs, Fo, Vs, Vo = (T.float() for T in (Fs, Fo, Vs, Vo))
+    S, H = tc.one_electron_matrices(basis, nup + ndown)
+    hf = tc.TCHartreeFock(nup, ndown, nb)
+    e = hf.compute(S, H, g1, Fs, Fo, Vs, Vo, True, tol=1e-10, max_iter=200, mgs_mode=mgs_mode)
+    A = g1.basisValues.cpu().numpy()
+    dA = g1.basisGradients.cpu().numpy()
+    full = lambda V: np.ascontiguousarray(tc.unpack(V.double().cpu().numpy(), nb).transpose(0, 2, 3, 1))
+    Fso, Foo = tc.unpack(Fs.double().cpu().numpy(), nb), tc.unpack(Fo.double().cpu().numpy(), nb)
+    ref = O.tc_scf(nup, ndown, S, H, A, dA, g1.weights_host, Fso, Foo, full(Vs), full(Vo), tol=1e-10, max_iter=200,
+                   mgs_mode=mgs_mode)
+    assert ref["status"] == 0
+    n = min(len(hf.energy_trace), len(ref["trace"]))
+    assert np.max(np.abs(hf.energy_trace[:n] - ref["trace"][:n])) < 1e-9
+    assert abs(e - ref["energy"]) < 1e-9, (e, ref["energy"])
+    assert np.max(np.abs(hf.density - ref["D"])) < 1e-7
+    # observables
+    nocc = (nup, ndown)
+    for s_ in range(2):
+        C = hf.occupiedOrbitals[s_]
+        assert C.shape == (nb, nocc[s_])
+        assert np.max(np.abs(C @ C.T - hf.density[s_])) < 1e-12
+        if mgs_mode == 1:
+            assert np.max(np.abs(C.T @ S @ C - np.eye(nocc[s_]))) < 1e-10
+    Cpad = np.zeros((2, nb, max(nocc)))
+    for s_ in range(2):
+        Cpad[s_, :, :nocc[s_]] = hf.occupiedOrbitals[s_]
+    G2, G3 = O.tc_fock(nup, ndown, A, dA, g1.weights_host, Fso, Foo, full(Vs), full(Vo), hf.density, Cpad)
+    mask = np.abs(H) > 0
+    fock = H[None] + np.where(mask[None], G2, 0) - np.where(mask[None], G3, 0)
+    assert rel_err(hf.fockOperator, fock) < 1e-9
 
 
 def test_tc_zero_jastrow_equals_uos(tc, gpu):
@@ -462,12 +540,15 @@ def test_tc_zero_jastrow_equals_uos(tc, gpu):
     assert abs(e - eu) < 1e-9
 
 
-@pytest.mark.parametrize("nup,ndown", [(2, 2), (2, 1), (1, 1)])
-def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown):
+@pytest.mark.parametrize("nup,ndown,vform", [(2, 2, "one"), (2, 1, "one"), (1, 1, "one"), (2, 2, "two"),
+                                              (2, 1, "two"), (1, 1, "two")])
+def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown, vform):
     """One device TC Fock build at a given density and orbitals (different per
     spin) against the oracle's factorized restatement (G2 and G3), and G3
     against assembleG over the spin-labelled explicit three-body tensors
-    (src/HartreeFock.cpp:613-750)."""
+    (src/HartreeFock.cpp:613-750).  vform 'one': one c = 1 tensor with the r12
+    Jastrow's scales; 'two': independent V_same / V_oppo of a general Ochi
+    Jastrow (the reference's VjiSame / VjiOppo)."""
     from tcscf_amd import SlaterTypeOrbital as STO
     basis = [STO(1.3, 1, 0, 0), STO(0.9, 2, 1, 0), STO(0.7, 2, 0, 0)]
     nb = len(basis)
@@ -478,11 +559,17 @@ def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown):
     for s, no in enumerate((nup, ndown)):
         C[s, :, :no] = rng.normal(size=(nb, no)) * 0.5
         D[s] = C[s, :, :no] @ C[s, :, :no].T
-    G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, V, jas, D, C)
     A = g1.basisValues.cpu().numpy()
     dA = g1.basisGradients.cpu().numpy()
-    Vf = tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1)
-    Vs, Vo = np.ascontiguousarray(jas.c_same * Vf), np.ascontiguousarray(jas.c_oppo * Vf)
+    if vform == "one":
+        G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, V, jas, D, C)
+        Vf = tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1)
+        Vs, Vo = np.ascontiguousarray(jas.c_same * Vf), np.ascontiguousarray(jas.c_oppo * Vf)
+    else:
+        u = tc.OchiJastrow(1.1, 1.0, [(0.5, 0.25), (-0.3, 0.2), (0.15, -0.4)], _OCHI_TERMS[:3])
+        Vsd, Vod = tc.computeV(g1, g3, u, same_spin=True), tc.computeV(g1, g3, u, same_spin=False)
+        G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, Vsd, Vod, D, C)
+        Vs, Vo = (np.ascontiguousarray(tc.unpack(T.cpu().numpy(), nb).transpose(0, 2, 3, 1)) for T in (Vsd, Vod))
     G2o, G3o = O.tc_fock(nup, ndown, A, dA, g1.weights_host, tc.unpack(Fs.cpu().numpy(), nb),
                          tc.unpack(Fo.cpu().numpy(), nb), Vs, Vo, D, C)
     assert rel_err(G2, G2o) < 1e-12
@@ -491,18 +578,45 @@ def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown):
     assert rel_err(G3, G3q) < 1e-11
 
 
-@pytest.mark.parametrize("nbasis,nup,ndown,same", [(12, 7, 6, False), (20, 6, 7, False), (36, 9, 8, False),
-                                                    (20, 6, 6, True), (36, 7, 7, True)])
-def test_tc_fock_build_many_orbitals_vs_oracle(tc, O, gpu, nbasis, nup, ndown, same):
+@pytest.mark.parametrize("nbasis,nup,ndown,same,dtype,vform,launch",
+                         [(12, 7, 6, False, "fp64", "one", None), (20, 6, 7, False, "fp64", "one", None),
+                          (36, 9, 8, False, "fp64", "one", (0, 8)), (20, 6, 6, True, "fp64", "one", None),
+                          (36, 7, 7, True, "fp64", "one", None),
+                          # FP32 storage, unstaged 8-wave launches: two spins at nb >= 40, restricted at nb 56
+                          (44, 6, 5, False, "fp32", "one", (0, 8)), (56, 7, 7, True, "fp32", "one", (0, 8)),
+                          (20, 6, 6, True, "fp32", "one", (1, 12)),
+                          # independent V_same / V_oppo: W blocks of both tensors
+                          (20, 7, 6, False, "fp64", "two", None), (36, 7, 7, True, "fp32", "two", None),
+                          (40, 6, 5, False, "fp32", "two", (0, 8))])
+def test_tc_fock_build_many_orbitals_vs_oracle(tc, O, gpu, nbasis, nup, ndown, same, dtype, vform, launch):
     """The device TC Fock build with more occupied orbitals than one register
     block of W = V C holds (several blocks and LDS chunks for |U|^2), for each
     lane split of the basis (16, 32 and 64 lanes per basis row), against the
     oracle's factorized restatement.  same: a closed shell with identical spin
     densities and orbitals, which forms spin 0 once; both spin blocks must then
-    be bitwise equal."""
-    basis = (_mixed_basis() * 3)[:nbasis]
+    be bitwise equal.  dtype fp32: F / V stored in FP32, the oracle fed the same
+    rounded values (the build's arithmetic is FP64, so 1e-12 still holds).
+    launch: the (staged, waves per workgroup) the build must pick, from
+    tcscf_tc_launch_info."""
+    import ctypes
+    basis = (_mixed_basis() * 4)[:nbasis]
     nb = len(basis)
     g1, g3, jas, Fs, Fo, V = _tc_case(tc, basis, (5, 7, tc.RADIAL_M4), (6, 7, tc.RADIAL_M4))
+    if vform == "two":
+        u = tc.OchiJastrow(1.1, 1.0, [(0.5, 0.25), (-0.3, 0.2)], _OCHI_TERMS[:2])
+        Vd, Vod = tc.computeV(g1, g3, u, same_spin=True), tc.computeV(g1, g3, u, same_spin=False)
+    else:
+        Vd, Vod = V, None
+    if dtype == "fp32":
+        Fs, Fo, Vd = Fs.float(), Fo.float(), Vd.float()
+        Vod = Vod.float() if Vod is not None else None
+    from tcscf_amd import _lib
+    info = (ctypes.c_int * 6)()
+    _lib.check(tc.lib().tcscf_tc_launch_info(nup, ndown, nb, _lib.F32 if dtype == "fp32" else _lib.F64,
+                                             int(vform == "two"), info))
+    if launch is not None:
+        r = 3 if same else 0
+        assert (info[r], info[r + 1]) == launch, list(info)
     rng = np.random.default_rng(nbasis)
     C = np.zeros((2, nb, max(nup, ndown)))
     D = np.zeros((2, nb, nb))
@@ -510,15 +624,19 @@ def test_tc_fock_build_many_orbitals_vs_oracle(tc, O, gpu, nbasis, nup, ndown, s
     for s, no in enumerate((nup, ndown)):
         C[s, :, :no] = C0[:, :no] if same else rng.normal(size=(nb, no)) * 0.3
         D[s] = C[s, :, :no] @ C[s, :, :no].T
-    G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, V, jas, D, C)
+    hf = tc.TCHartreeFock(nup, ndown, nb)
+    G2, G3 = hf.fockBuild(g1, Fs, Fo, Vd, jas if Vod is None else Vod, D, C)
     if same:
         assert np.array_equal(G2[0], G2[1]) and np.array_equal(G3[0], G3[1])
     A = g1.basisValues.cpu().numpy()
     dA = g1.basisGradients.cpu().numpy()
-    Vf = tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1)
-    Vs, Vo = np.ascontiguousarray(jas.c_same * Vf), np.ascontiguousarray(jas.c_oppo * Vf)
-    G2o, G3o = O.tc_fock(nup, ndown, A, dA, g1.weights_host, tc.unpack(Fs.cpu().numpy(), nb),
-                         tc.unpack(Fo.cpu().numpy(), nb), Vs, Vo, D, C)
+    full = lambda T: tc.unpack(T.double().cpu().numpy(), nb).transpose(0, 2, 3, 1)
+    if Vod is None:
+        Vs, Vo = np.ascontiguousarray(jas.c_same * full(Vd)), np.ascontiguousarray(jas.c_oppo * full(Vd))
+    else:
+        Vs, Vo = np.ascontiguousarray(full(Vd)), np.ascontiguousarray(full(Vod))
+    G2o, G3o = O.tc_fock(nup, ndown, A, dA, g1.weights_host, tc.unpack(Fs.double().cpu().numpy(), nb),
+                         tc.unpack(Fo.double().cpu().numpy(), nb), Vs, Vo, D, C)
     assert rel_err(G2, G2o) < 1e-12
     assert rel_err(G3, G3o) < 1e-12
 
@@ -715,6 +833,20 @@ def test_cpp_facade_on_device(tc, O, gpu):
     assert abs(out["be_tc_mixed"] - etc) < 1e-6
     # the general Ochi path with S = {(1, 0, 0)}: the same kernel up to rounding
     assert abs(out["be_tc_ochi"] - etc) < 1e-9
+    # the reference's signature (VjiSame, VjiOppo as two tensors) on the same Jastrow
+    assert abs(out["be_tc_ref_signature"] - out["be_tc_ochi"]) < 1e-10
+    # ... and on a general Ochi Jastrow with independent V_same / V_oppo, against the oracle
+    ug = tc.OchiJastrow(1.1, 1.0, [(0.5, 0.25), (-0.3, 0.2), (0.15, -0.4)], [(1, 0, 0), (2, 1, 0), (1, 1, 1)])
+    Fsg = tc.computeF(g1, g1, kernel="tc", jastrow=ug, same_spin=True)
+    Fog = tc.computeF(g1, g1, kernel="tc", jastrow=ug, same_spin=False)
+    full = lambda V: np.ascontiguousarray(tc.unpack(V.cpu().numpy(), len(bb)).transpose(0, 2, 3, 1))
+    Vsg, Vog = full(tc.computeV(g1, g3, ug, same_spin=True)), full(tc.computeV(g1, g3, ug, same_spin=False))
+    refg = O.tc_scf(2, 2, Sb, Hb, g1.basisValues.cpu().numpy(), g1.basisGradients.cpu().numpy(), g1.weights_host,
+                    tc.unpack(Fsg.cpu().numpy(), len(bb)), tc.unpack(Fog.cpu().numpy(), len(bb)), Vsg, Vog, tol=1e-10,
+                    max_iter=200, mgs_mode=1)
+    assert refg["status"] == 0
+    assert abs(out["be_tc_general"] - refg["energy"]) < 1e-9
+    assert out["be_tc_general_orth_err"] < 1e-10 and out["be_tc_general_density_err"] < 1e-12
     # the facade's per-element QMC route gives the Python route's numbers (same points, same kernels)
     H = tc.HydrogenLikeBasisFunction
     bz = [H(2, 1, 0, 0), H(2, 2, 0, 0), H(2, 2, 1, 0), H(2, 2, 1, 1)]
