@@ -760,34 +760,6 @@ def test_tc_scf_on_sobol_grids_vs_oracle(tc, O, gpu):
     assert abs(e - ref["energy"]) < 1e-9, (e, ref["energy"])
 
 
-def test_tc_qmc_ne_config4_mixed_vs_fp64(tc, gpu):
-    """The QMC workload at full size (Ne, 40 STOs, 2**16 Sobol points per
-    coordinate, b = 1): the BF16x5 precompute gives the FP64 TC energy to
-    1e-6 Eh (same grids, same SCF).  With this basis the plain Roothaan loop
-    oscillates between two states on these grids (so does plain RHF), so the
-    solve uses DIIS (depth 6)."""
-    from tcscf_amd import configs
-    c = configs.ne_qmc_config4()
-    basis, nb = c["basis"], len(c["basis"])
-    g1 = tc.QMCGrid.sobol(c["npts"], seed=c["seeds"][0])
-    g2 = tc.QMCGrid.sobol(c["npts"], seed=c["seeds"][1])
-    g3 = tc.QMCGrid.sobol(c["npts"], seed=c["seeds"][2])
-    for g in (g1, g2, g3):
-        g.setBasisFunctions(basis, True)
-    S, H = tc.one_electron_matrices(basis, c["Z"])
-    jas = tc.Jastrow.from_b(1.0)
-    E = {}
-    for prec in ("bf16x5", "fp64"):
-        Fs = tc.computeF(g1, g2, kernel="tc", jastrow=jas, same_spin=True, precision=prec)
-        Fo = tc.computeF(g1, g2, kernel="tc", jastrow=jas, same_spin=False, precision=prec)
-        V = tc.computeV(g1, g3, jas, precision=prec)
-        E[prec] = tc.TCHartreeFock(c["nup"], c["ndown"], nb).compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-9,
-                                                                      mgs_mode=1, diis=6)
-        del Fs, Fo, V
-    print(f"QMC Ne TC: E(bf16x5) = {E['bf16x5']:.10f}, E(fp64) = {E['fp64']:.10f}")
-    assert abs(E["bf16x5"] - E["fp64"]) < 1e-6
-
-
 def test_cpp_facade_on_device(tc, O, gpu):
     """The C++ facade (include/tcscf.hpp, driven by tests/cpp/facade_test.cpp)
     against the oracle and the Python mirror: He config 1 RHF/UHF, the grid
