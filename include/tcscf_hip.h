@@ -232,6 +232,24 @@ int tcscf_compute_F(const double* pts1_dev, int64_t n1, const double* pts2_dev, 
                     const int32_t* ord2_dev, int64_t n2, const double* A2_dev, int nb, int kernel,
                     const tcscf_jastrow* jastrow, int same_spin, int precision, double scale, void* F_dev,
                     int out_dtype, void* stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that must not call
+ * hipMalloc): tcscf_compute_F with the same arguments, results and stream
+ * semantics, but its workspace (the packed pair densities and padded
+ * coordinates, tcscf_F_workspace_bytes(n2, nb, precision)) lives in a fixed
+ * 128 MiB device array of the library instead of memory it allocates; it
+ * calls no hipMalloc / hipFree.  A contraction whose workspace exceeds
+ * 128 MiB is rejected with TCSCF_EINVAL (the Ne grid of 30,200 points and 30
+ * functions fits at every precision).  The first call on a device sets the
+ * array up (looks up its device address; the runtime maps it then): make that
+ * call outside graph capture.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams or threads without
+ * ordering) race; calls ordered on one stream do not. */
+int tcscf_compute_F_noalloc(const double* pts1_dev, int64_t n1, const double* pts2_dev, const double* w2_dev,
+                            const int32_t* ord2_dev, int64_t n2, const double* A2_dev, int nb, int kernel,
+                            const tcscf_jastrow* jastrow, int same_spin, int precision, double scale, void* F_dev,
+                            int out_dtype, void* stream);
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision);
 
 /* tcscf_compute_F for the general Ochi Jastrow (the integrand lambdas of

@@ -21,7 +21,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16x5": BF16X5}
 # every symbol include/tcscf_hip.h declares
 EXPORTS = [
     "tcscf_last_error", "tcscf_version", "tcscf_radial_grid", "tcscf_lebedev_npts", "tcscf_lebedev",
-    "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F",
+    "tcscf_product_grid", "tcscf_sto_overlap_core", "tcscf_basis_eval", "tcscf_compute_F", "tcscf_compute_F_noalloc",
     "tcscf_F_workspace_bytes", "tcscf_compute_F_tc_pair", "tcscf_compute_F_ochi", "tcscf_eri_qmc6d", "tcscf_eri_qmc5d_atomic", "tcscf_qmc_spherical_points", "tcscf_basis_eval_fns", "tcscf_basis_one_electron", "tcscf_rcs_scf", "tcscf_uos_scf", "tcscf_tc_scf", "tcscf_tc_fock", "tcscf_tc_launch_info", "tcscf_eri", "tcscf_eri_gradient", "tcscf_three_electron",
     "tcscf_qmc_lattice_npts", "tcscf_qmc_lattice_grid", "tcscf_sobol_points", "tcscf_sobol_grid", "tcscf_load_sto",
     "tcscf_batch_compute_F", "tcscf_batch_tuned_rows", "tcscf_batch_scf",

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtcscf_hip.so")
-SOURCES = ["kernels.hip", "scf.hip", "tc_scf.hip", "host_grids.cpp", "dense.cpp", "batch.cpp"]
+SOURCES = ["kernels.hip", "scf.hip", "tc_scf.hip", "noalloc.hip", "host_grids.cpp", "dense.cpp", "batch.cpp"]
 ARCH = os.environ.get("TCSCF_OFFLOAD_ARCH", "gfx950")
 
 

@@ -61,6 +61,12 @@ inline hipError_t lds_opt_in(const void* fn) {
   return e;
 }
 
+// tcscf_compute_F with the workspace given (tcscf_F_workspace_bytes of it,
+// device memory) or, for ws_fixed = null, from the library's scratch slot.
+int compute_F_into(const double* pts1, int64_t n1, const double* pts2, const double* w2, const int32_t* ord2,
+                   int64_t n2, const double* A2, int nb, int kernel, const tcscf_jastrow* jp, int same_spin,
+                   int precision, double scale, void* F, int out_dtype, void* stream, char* ws_fixed);
+
 // Device scratch owned by the library: grown on demand, never shrunk, one per
 // (device, slot).  Calls that share a slot on one device must be ordered on
 // one stream (every entry point documents the slots it uses).

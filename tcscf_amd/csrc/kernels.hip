@@ -1068,16 +1068,18 @@ __global__ __launch_bounds__(256) void tc_pair_combine(T* __restrict__ Fs, T* __
   }
 }
 
+// ws_fixed: the workspace (tcscf_F_workspace_bytes of it) given by the caller;
+// null takes it from the library's scratch slot.
 template <class Kern>
 int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, const double* w2, const int* ord2,
              int64_t n2, const double* A2, int nb, double scale, void* F, int out_f32, Kern kern, hipStream_t st,
-             void* F2 = nullptr) {
+             void* F2 = nullptr, char* ws_fixed = nullptr) {
   const int64_t npair = npair_of(nb);
   int rc = 0;
   if (precision == TCSCF_BF16X3 || precision == TCSCF_BF16X5) {
     const int nsplit = precision == TCSCF_BF16X3 ? 2 : 3;
     const int64_t n2p = round_up(n2, 32), ncb = round_up(npair, 256) / 16, nkb = n2p / 32;
-    char* ws = (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
+    char* ws = ws_fixed ? ws_fixed : (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
     if (rc) return rc;
     unsigned short* planes = (unsigned short*)ws;
     const int64_t plane = nkb * ncb * 512;  // bf16 per plane
@@ -1102,7 +1104,7 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
     TC_CHECK(false, "combined-output kernels run on the BF16 path only");
   } else {
   const int64_t n2p = round_up(n2, 16), ldp = round_up(npair, 128);
-  char* ws = (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
+  char* ws = ws_fixed ? ws_fixed : (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
   if (rc) return rc;
   const size_t el = precision == TCSCF_FP64 ? 8 : 4;
   double* xd = (double*)(ws + (size_t)n2p * ldp * el);
@@ -1127,6 +1129,43 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
 }
 
 }  // namespace
+
+namespace tc {
+
+int compute_F_into(const double* pts1, int64_t n1, const double* pts2, const double* w2, const int32_t* ord2,
+                   int64_t n2, const double* A2, int nb, int kernel, const tcscf_jastrow* jp, int same_spin,
+                   int precision, double scale, void* F, int out_dtype, void* stream, char* ws_fixed) {
+  TC_CHECK(pts1 && pts2 && w2 && A2 && F && n1 > 0 && n2 > 0 && nb > 0, "tcscf_compute_F: bad arguments");
+  TC_CHECK(precision >= 0 && precision <= 3, "tcscf_compute_F: unknown precision");
+  TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F: unknown out_dtype");
+  TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F: grid too large");
+  hipStream_t st = (hipStream_t)stream;
+  const int of = out_dtype == TCSCF_F32;
+  switch (kernel) {
+    case TCSCF_K_COULOMB:
+      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernCoulomb{}, st, nullptr,
+                      ws_fixed);
+    case TCSCF_K_TC_SCALAR: {
+      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: TC kernel needs jastrow parameters with a12 > 0");
+      const double c = same_spin ? jp->c_same : jp->c_oppo;
+      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of,
+                      KernTCScalar{jp->a12, 2 * c * c * jp->a12 * jp->a12}, st, nullptr, ws_fixed);
+    }
+    case TCSCF_K_JASTROW_GRAD:
+      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: gradient kernel needs jastrow parameters with a12 > 0");
+      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernJastrowGrad{jp->a12}, st,
+                      nullptr, ws_fixed);
+    case TCSCF_K_TC_SMOOTH:
+      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: TC kernel needs jastrow parameters with a12 > 0");
+      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernTCSmooth{jp->a12}, st,
+                      nullptr, ws_fixed);
+    default:
+      TC_CHECK(false, "tcscf_compute_F: unknown kernel");
+  }
+  return TCSCF_EINVAL;
+}
+
+}  // namespace tc
 
 extern "C" {
 
@@ -1192,31 +1231,8 @@ size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision) {
 int tcscf_compute_F(const double* pts1, int64_t n1, const double* pts2, const double* w2, const int32_t* ord2,
                     int64_t n2, const double* A2, int nb, int kernel, const tcscf_jastrow* jp, int same_spin,
                     int precision, double scale, void* F, int out_dtype, void* stream) {
-  TC_CHECK(pts1 && pts2 && w2 && A2 && F && n1 > 0 && n2 > 0 && nb > 0, "tcscf_compute_F: bad arguments");
-  TC_CHECK(precision >= 0 && precision <= 3, "tcscf_compute_F: unknown precision");
-  TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F: unknown out_dtype");
-  TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F: grid too large");
-  hipStream_t st = (hipStream_t)stream;
-  const int of = out_dtype == TCSCF_F32;
-  switch (kernel) {
-    case TCSCF_K_COULOMB:
-      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernCoulomb{}, st);
-    case TCSCF_K_TC_SCALAR: {
-      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: TC kernel needs jastrow parameters with a12 > 0");
-      const double c = same_spin ? jp->c_same : jp->c_oppo;
-      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of,
-                      KernTCScalar{jp->a12, 2 * c * c * jp->a12 * jp->a12}, st);
-    }
-    case TCSCF_K_JASTROW_GRAD:
-      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: gradient kernel needs jastrow parameters with a12 > 0");
-      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernJastrowGrad{jp->a12}, st);
-    case TCSCF_K_TC_SMOOTH:
-      TC_CHECK(jp && jp->a12 > 0, "tcscf_compute_F: TC kernel needs jastrow parameters with a12 > 0");
-      return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, F, of, KernTCSmooth{jp->a12}, st);
-    default:
-      TC_CHECK(false, "tcscf_compute_F: unknown kernel");
-  }
-  return TCSCF_EINVAL;
+  return tc::compute_F_into(pts1, n1, pts2, w2, ord2, n2, A2, nb, kernel, jp, same_spin, precision, scale, F,
+                            out_dtype, stream, nullptr);
 }
 
 int tcscf_compute_F_ochi(const double* pts1, int64_t n1, const double* pts2, const double* w2, const int32_t* ord2,
