@@ -737,15 +737,18 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
     az = ok ? (float)p1[2 * n1 + grow] : (float)-kFar;
   }
 
-  // coordinates of step t -> crd[t & 1] (lanes 0..31 of wave 0, 16 bytes each)
+  // Coordinates of step t: lanes 0..31 of wave 0 load them into a register
+  // (fetch_coords) two steps ahead and store them into crd[t & 1] before the
+  // barrier of the step that issued the load (store_coords).  A plain load, so
+  // the only wait is for that load: the B fragments loaded after it stay in
+  // flight across the barrier.  (A direct-to-LDS load here made every later
+  // LDS access wait for all outstanding loads, B included.)
+  float4 cnext = make_float4(0.f, 0.f, 0.f, 0.f);
   auto fetch_coords = [&](int64_t t) {
-    if (wave == 0 && lane < 32)
-      __builtin_amdgcn_global_load_lds((const void*)(x2 + t * 32 + lane),
-                                       (__attribute__((address_space(3))) void*)&crd[t & 1][0], 16, 0, 0);
+    if (wave == 0 && lane < 32) cnext = x2[t * 32 + lane];
   };
-  // wave 0 waits for its direct-to-LDS loads before the barrier that publishes them
-  auto coords_landed = [&] {
-    if (wave == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  auto store_coords = [&](int64_t t) {
+    if (wave == 0 && lane < 32) crd[t & 1][lane] = cnext;
   };
   // K = hi + lo, both bf16 (round to nearest even, v_cvt_pk_bf16_f32), for the
   // step whose coordinates sit in crd[buf]
@@ -813,8 +816,11 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
   const int64_t nsteps = n2p / 32;
   us8 bqA[NSPLIT][NB], bqB[NSPLIT][NB];  // B fragments, ping-pong between steps
   fetch_coords(0);
-  if (nsteps > 1) fetch_coords(1);
-  coords_landed();
+  store_coords(0);
+  if (nsteps > 1) {
+    fetch_coords(1);
+    store_coords(1);
+  }
   __syncthreads();
   generate(0);
   load_b(bqA, 0);
@@ -823,22 +829,27 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
   // One 32-point step: MFMAs on ktile[s & 1] and cur, B of step s + 1 into nxt,
   // K of step s + 1 into the other tile.  FRESH: the step starts a new FP32
   // partial (zero accumulator operand); the fold to FP64 follows every FOLD
-  // steps.  With several kernel components (ONEBLOCK) the step between the
-  // coordinate fetch and the barrier is one basic block (the B load past the
-  // last step re-reads the last step, and the K generated after the last step
-  // is not used), so the scheduler interleaves each wave's MFMAs with its
-  // generation and fold; with one component the MFMAs issue as one run and
-  // the other wave of the SIMD fills the VALU meanwhile (measured faster for
-  // one component, slower for three).
-  constexpr bool ONEBLOCK = NC > 1;
+  // steps.  The step between the coordinate fetch and the barrier is one
+  // basic block: the loads past the last step re-read the last step and the K
+  // generated after the last step is not used.  With no branch around the
+  // loads, the wait before each MFMA counts only loads older than its operand,
+  // so the B fragments of step s + 1 stay in flight through step s (behind a
+  // branch, the join took the count of the path without them and every MFMA
+  // waited for the loads just issued).  With several kernel components the
+  // scheduler interleaves each wave's MFMAs with its generation and fold;
+  // with one component (RUN) scheduling fences keep the MFMAs one run while
+  // the other wave of the SIMD fills the VALU (measured faster for one
+  // component, slower for three).
+  constexpr bool RUN = NC == 1;
+  auto fence = [] {
+    if constexpr (RUN) __builtin_amdgcn_sched_barrier(0);
+  };
   auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto fresh) {
     constexpr bool FRESH = decltype(fresh)::value;
     const int buf = (int)(s & 1);
-    if (s + 2 < nsteps) fetch_coords(s + 2);  // into crd[s & 1], read last by generate(s) one step ago
-    if (ONEBLOCK)
-      load_b(nxt, s + 1 < nsteps ? s + 1 : s);
-    else if (s + 1 < nsteps)
-      load_b(nxt, s + 1);
+    fetch_coords(s + 2 < nsteps ? s + 2 : nsteps - 1);  // for crd[s & 1], read last by generate(s) one step ago
+    load_b(nxt, s + 1 < nsteps ? s + 1 : s);
+    fence();
     const f4 zero = {0, 0, 0, 0};
     // Product-major: every accumulator of a component takes one product before
     // any takes the next, so consecutive MFMAs are independent and each
@@ -871,9 +882,11 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
         }
       }
     }
-    if (ONEBLOCK || s + 1 < nsteps) generate(buf ^ 1);
+    fence();
+    generate(buf ^ 1);
+    store_coords(s + 2);
+    fence();
     if constexpr (!FRESH || FOLD == 1) fold_all();
-    coords_landed();
     __syncthreads();
   };
   using T = std::integral_constant<bool, true>;
