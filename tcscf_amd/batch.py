@@ -79,7 +79,8 @@ def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None, ou
 def solve_each(items, tol, max_iter, diis):
     """The SCF of every instance with the single-instance drivers, one after
     another (the graph-batched tcscf_batch_scf is switched off).  Returns the
-    per-instance dicts energy / iterations / status / trace / D."""
+    per-instance dicts energy / iterations / status / trace / D (and C, the
+    occupied orbitals, for TC)."""
     out = []
     for x in items:
         nb = x["r1"].nBasis()
@@ -99,5 +100,6 @@ def solve_each(items, tol, max_iter, diis):
         except TcscfError as err:
             status = err.code
         out.append(dict(energy=hf.energy, iterations=hf.numberOfConvergenceLoops(), status=status,
-                        trace=hf.energy_trace, D=hf.density))
+                        trace=hf.energy_trace, D=hf.density,
+                        C=getattr(hf, "occupiedOrbitals", None)))
     return out
