@@ -20,36 +20,35 @@ namespace tc {
 constexpr int kSweepBlocks = 512;
 constexpr int kGPerThread = 32;  // NSPIN * nb * nb <= 256 * 32
 
-__device__ inline int64_t pidx(int j, int i) { return j <= i ? (int64_t)i * (i + 1) / 2 + j : (int64_t)j * (j + 1) / 2 + i; }
-
 // G_s[j][i] = sum_n w_n A[n,j] (c_self A[n,i] Faa_s[n] + c_other A[n,i] Faa_sbar[n] - Fai_s[n,i])
 //   Faa_s[n]  = sum_{j,i} F[n,j,i] D_s[i,j]                   (calculateIaa, HartreeFock.cpp:183-213)
 //   Fai_s[n,i] = sum_j (sum_k D_s[k,j] A[n,k]) F[n,j,i]       (calculateFai, HartreeFock.cpp:144-179)
 // RCS (NSPIN = 1): c_self = 2, c_other = 0   (HartreeFock.cpp:828-842)
 // UOS (NSPIN = 2): c_self = c_other = 1      (HartreeFock.cpp:940-955)
 // D is symmetric (C C^T), so D[i][j] = D[j][i].
-// Lane layout: a wave works on PPW = 64 / NBP points at once, lane = q * NBP + i
-// (point q of the wave, basis index i < nb <= NBP).  The workgroup stages D in
-// LDS once, walks its r1 range in chunks of CH points, stages each point's
-// packed F row and A row, forms Y_s[n,i] = w_n (...) for the chunk, then folds
-// the chunk into register-held partial G with a rank-CH update.
+// This kernel serves 32 < nb <= 64 (NBP = 64: one point per wave, lane i).
+// The workgroup stages D in LDS once, walks its r1 range in chunks of CH
+// points, stages each point's packed F row and A row, forms
+// Y_s[n,i] = w_n (...) for the chunk, then folds the chunk into register-held
+// partial G with a rank-CH update.
 template <class TF, int NSPIN, int NBP>
 __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F, int64_t npair,
                                                        const double* __restrict__ A, const double* __restrict__ w,
                                                        int64_t n1, int nb, const double* __restrict__ D,
                                                        double c_self, double c_other, double* __restrict__ Gpart) {
-  constexpr int PPW = 64 / NBP;  // points per wave at a time
-  constexpr int CH = NBP == 64 ? 16 : 32;  // points per chunk
+  static_assert(NBP == 64, "nb <= 32 runs hf_sweep_col_kernel");
+  constexpr int PPW = 1;   // points per wave at a time
+  constexpr int CH = 16;   // points per chunk
   constexpr int KG = NSPIN * NBP * NBP / 256;  // G elements per thread
   extern __shared__ double smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int q = lane / NBP, i = lane % NBP;
+  const int i = lane;
   const int64_t nb2 = (int64_t)nb * nb;
   double* Ds = smem;                                   // [NSPIN][nb][nb]
-  double* Frow = Ds + NSPIN * nb2;                     // [4][PPW][npair]
+  double* Frow = Ds + NSPIN * nb2;                     // [4][npair]
   double* Abuf = Frow + 4 * PPW * npair;               // [CH][nb]
   double* Ybuf = Abuf + CH * nb;                       // [CH][NSPIN][nb]
-  double* Xb = Ybuf + CH * NSPIN * nb;                 // [4][PPW][NSPIN][nb]
+  double* Xb = Ybuf + CH * NSPIN * nb;                 // [4][NSPIN][nb]
   for (int64_t e = tid; e < NSPIN * nb2; e += 256) Ds[e] = D[e];
   const int64_t per = (n1 + gridDim.x - 1) / gridDim.x;
   const int64_t p0 = (int64_t)blockIdx.x * per;
@@ -58,58 +57,22 @@ __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F,
 #pragma unroll
   for (int t = 0; t < KG; ++t) gacc[t] = 0;
   const int tri_i = i * (i + 1) / 2;
-  // F rows of a wave's step (points pb .. pb + PPW - 1 of chunk c0) travel
-  // through registers: the next step's rows are loaded while this step computes.
-  // (nb <= 32; the NBP = 64 build keeps synchronous staging, which needs no
-  // extra registers)
-  constexpr bool PF = NBP == 32;
-  constexpr int NL = PF ? (PPW * (NBP * (NBP + 1) / 2) + 63) / 64 : 1;
-  const int64_t rowlen = PPW * npair;
-  TF pre[NL];
-  double apre = 0;  // this lane's A value of the prefetched step
-  auto load_rows = [&](int64_t cc0, int ccn, int cpb) {
-    if (cpb + q < ccn && i < nb) apre = A[(cc0 + cpb + q) * nb + i];
-#pragma unroll
-    for (int t = 0; t < NL; ++t) {
-      const int64_t e = lane + 64 * t;
-      const int qq = e >= npair ? 1 : 0;
-      if (e < rowlen && cpb + qq < ccn) pre[t] = F[(cc0 + cpb) * npair + e];
-    }
-  };
   auto chunk_len = [&](int64_t cc0) { return (int)((cc0 + CH < p1) ? CH : p1 - cc0); };
-  if (PF && p0 < p1 && wave * PPW < chunk_len(p0)) load_rows(p0, chunk_len(p0), wave * PPW);
   __syncthreads();
 
   for (int64_t c0 = p0; c0 < p1; c0 += CH) {
     const int cn = chunk_len(c0);
-    for (int pb = wave * PPW; pb < cn; pb += 4 * PPW) {
-      // F rows of points pb .. pb+PPW-1 from registers into LDS, then the
-      // next step's rows into registers
-      double* fr = Frow + (int64_t)wave * PPW * npair;
-      double acur = 0;
-      if constexpr (PF) {
-        acur = apre;
-#pragma unroll
-        for (int t = 0; t < NL; ++t) {
-          const int64_t e = lane + 64 * t;
-          const int qq = e >= npair ? 1 : 0;
-          if (e < rowlen && pb + qq < cn) fr[e] = (double)pre[t];
-        }
-        if (pb + 4 * PPW < cn) load_rows(c0, cn, pb + 4 * PPW);
-        else if (c0 + CH < p1 && wave * PPW < chunk_len(c0 + CH)) load_rows(c0 + CH, chunk_len(c0 + CH), wave * PPW);
-      } else {
-        for (int64_t e = lane; e < rowlen; e += 64)
-          if (pb + (e >= npair ? 1 : 0) < cn) fr[e] = (double)F[(c0 + pb) * npair + e];
-      }
-      const int pt = pb + q;
-      const bool live = pt < cn && i < nb;
+    for (int pt = wave; pt < cn; pt += 4) {
+      double* fr = Frow + (int64_t)wave * npair;
+      for (int64_t e = lane; e < npair; e += 64) fr[e] = (double)F[(c0 + pt) * npair + e];
+      const bool live = i < nb;
       const int64_t n = c0 + pt;
-      double* ab = Abuf + (int64_t)(live ? pt : 0) * nb;
-      if (live) ab[i] = PF ? acur : A[n * nb + i];
+      double* ab = Abuf + (int64_t)pt * nb;
+      if (live) ab[i] = A[n * nb + i];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      double* xb = Xb + ((int64_t)wave * PPW + q) * NSPIN * nb;
+      double* xb = Xb + (int64_t)wave * NSPIN * nb;
       if (live) {
 #pragma unroll
         for (int s = 0; s < NSPIN; ++s) {
@@ -126,7 +89,7 @@ __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F,
 #pragma unroll
       for (int s = 0; s < NSPIN; ++s) faa[s] = fai[s] = 0;
       if (live) {
-        const double* f = fr + q * npair;
+        const double* f = fr;
         for (int j = 0; j < nb; ++j) {
           const double fv = f[j <= i ? tri_i + j : j * (j + 1) / 2 + i];
 #pragma unroll
@@ -175,21 +138,214 @@ __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F,
   }
 }
 
-// Deterministic sum of per-workgroup partials, one workgroup per element:
-// out[e] = sum_b part[b][e] (fixed tree order).
-__global__ __launch_bounds__(256) void reduce_partials(const double* __restrict__ part, int nparts, int64_t len,
-                                                       double* __restrict__ out) {
-  __shared__ double red[256];
-  const int64_t e = blockIdx.x;
-  double s = 0;
-  for (int b = threadIdx.x; b < nparts; b += 256) s += part[(int64_t)b * len + e];
-  red[threadIdx.x] = s;
+// Raw buffer resources (gfx9 word 3: 32-bit data format, no swizzle): loads
+// at byte offsets at or past the view's size return zero.
+constexpr int kBufferWord3 = 0x00020000;
+template <class T>
+__device__ inline T buffer_load(__amdgpu_buffer_rsrc_t r, int voffset, int soffset);
+template <>
+__device__ inline double buffer_load<double>(__amdgpu_buffer_rsrc_t r, int voffset, int soffset) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voffset, soffset, 0));
+}
+template <>
+__device__ inline float buffer_load<float>(__amdgpu_buffer_rsrc_t r, int voffset, int soffset) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voffset, soffset, 0));
+}
+
+template <class TF, int NBP>
+constexpr int hf_col_ld() { return NBP == 32 ? (sizeof(TF) == 8 ? 33 : 34) : 20; }
+
+// The same G for nb <= NBP in {16, 32}, one r1 point per wave step.  Lane
+// h * NBP + i of a wave owns basis column i and the row slice
+// j in [h J, h J + J) (H = 64 / NBP slices of J = NBP / H rows): it keeps that
+// slice of column i of G in registers (RCS: of D too), so the per-point work is
+// register FMAs against LDS reads that are either one contiguous row of the
+// point's F matrix or a broadcast:
+//   X_s[i]   = sum_k D_s[k][i] a[k]                (slice sums added over h)
+//   Faa_s    = sum_{j,i} F[j][i] D_s[j][i]         (added over the 64 lanes)
+//   Fai_s[i] = sum_j X_s[j] F[j][i]                (slice sums added over h)
+//   G_s[j][i] += a[j] w (a[i] (c_self Faa_s + c_other Faa_sbar) - Fai_s[i])
+// Each wave streams its points' packed F rows through registers, one step
+// ahead, and unpacks them into its own NBP x NBP LDS matrix (zero outside
+// nb x nb, so every loop runs to NBP unmasked).  Waves never wait for each
+// other until the end, where the block sums its waves' G in a fixed order into
+// one partial G.
+template <class TF, int NSPIN, int NBP>
+__global__ __launch_bounds__(256, NSPIN == 1 ? 3 : 2) void hf_sweep_col_kernel(const TF* __restrict__ F, int64_t npair,
+                                                              const double* __restrict__ A,
+                                                              const double* __restrict__ w, int64_t n1, int nb,
+                                                              const double* __restrict__ D, double c_self,
+                                                              double c_other, double* __restrict__ Gpart) {
+  constexpr int H = 64 / NBP, J = NBP / H;
+  // padded row stride of the unpacked matrix, so that neither the scattered
+  // stores of a packed row nor the H row slices read at once share LDS banks
+  constexpr int LD = hf_col_ld<TF, NBP>(), FMAT = NBP * LD;
+  constexpr int NL = (NBP * (NBP + 1) / 2 + 63) / 64;  // loads per lane per point at nb = NBP
+  extern __shared__ double smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: point indices and row addresses stay scalar
+  const int h = lane / NBP, i = lane % NBP, j0 = h * J;
+  double* ab = smem + wave * NBP;                  // [4][NBP]: a of the wave's point
+  double* xb = smem + 4 * NBP + wave * NSPIN * NBP;  // [4][NSPIN][NBP]: X of the wave's point
+  TF* fm = reinterpret_cast<TF*>(smem + 4 * NBP * (1 + NSPIN)) + wave * (FMAT + 2);  // [4][NBP x LD + 2]
+
+  // zero the unpacked matrix once: entries outside nb x nb are never written
+  // (element FMAT takes the stores of lanes with no pair)
+  for (int e = lane; e < FMAT + 2; e += 64) fm[e] = TF(0);
+  double dreg[NSPIN][J];
+#pragma unroll
+  for (int s = 0; s < NSPIN; ++s)
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) {
+      const int k = j0 + jj;
+      const double d = D[((int64_t)s * nb + (k < nb ? k : 0)) * nb + (i < nb ? i : 0)];
+      dreg[s][jj] = (k < nb && i < nb) ? d : 0.0;
+    }
+  // where element lane + 64 t of a point's packed row goes: pair (jj <= ii) of
+  // the packed row ii (ii + 1) / 2 + jj, stored at both mirror positions (two
+  // 16-bit offsets; FMAT for lanes past the row)
+  uint32_t dst[NL];
+#pragma unroll
+  for (int t = 0; t < NL; ++t) {
+    const int p = lane + 64 * t;
+    dst[t] = (uint32_t)FMAT | ((uint32_t)FMAT << 16);
+    if (p < npair) {
+      int ii = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+      while (ii * (ii + 1) / 2 > p) --ii;
+      while ((ii + 1) * (ii + 2) / 2 <= p) ++ii;
+      const int jj = p - ii * (ii + 1) / 2;
+      dst[t] = (uint32_t)(jj * LD + ii) | ((uint32_t)(ii * LD + jj) << 16);
+    }
+  }
+  double g[NSPIN][J];
+#pragma unroll
+  for (int s = 0; s < NSPIN; ++s)
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) g[s][jj] = 0;
+
+  const int64_t gw = (int64_t)blockIdx.x * 4 + wave, nw = (int64_t)gridDim.x * 4;
+  TF pre[NL];
+  double apre = 0, wpre = 0;
+  // through a raw buffer view of the row: scalar base, one lane offset, and
+  // elements past the row read as zero (no branches, no clamping)
+  const int ic = i < nb ? i : 0;
+  auto load = [&](int64_t n) {
+    const __amdgpu_buffer_rsrc_t row =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<TF*>(F + n * npair), (short)0, (int)(npair * sizeof(TF)),
+                                          kBufferWord3);
+#pragma unroll
+    for (int t = 0; t < NL; ++t) pre[t] = buffer_load<TF>(row, lane * (int)sizeof(TF), t * 64 * (int)sizeof(TF));
+    apre = A[n * nb + ic];
+    wpre = w[n];
+  };
+  if (gw < n1) load(gw);
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+  for (int64_t n = gw; n < n1; n += nw) {
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+      fm[dst[t] & 0xffff] = pre[t];
+      fm[dst[t] >> 16] = pre[t];
+    }
+    const double a = i < nb ? apre : 0.0, wn = wpre;
+    ab[i] = a;  // the H lanes of column i store the same value
+    load(n + nw < n1 ? n + nw : n);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+    for (int s = 0; s < NSPIN; ++s) {
+      double x = 0;
+#pragma unroll
+      for (int jj = 0; jj < J; ++jj) x += dreg[s][jj] * ab[j0 + jj];
+#pragma unroll
+      for (int off = NBP; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
+      xb[s * NBP + i] = x;  // equal in the H lanes of column i
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    double faa[NSPIN], fai[NSPIN];
+#pragma unroll
+    for (int s = 0; s < NSPIN; ++s) faa[s] = fai[s] = 0;
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) {
+      const double f = (double)fm[(j0 + jj) * LD + i];
+#pragma unroll
+      for (int s = 0; s < NSPIN; ++s) {
+        faa[s] += f * dreg[s][jj];
+        fai[s] += xb[s * NBP + j0 + jj] * f;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NSPIN; ++s) {
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) faa[s] += __shfl_xor(faa[s], off, 64);
+#pragma unroll
+      for (int off = NBP; off < 64; off <<= 1) fai[s] += __shfl_xor(fai[s], off, 64);
+    }
+    double y[NSPIN];
+#pragma unroll
+    for (int s = 0; s < NSPIN; ++s) {
+      const double other = NSPIN == 2 ? faa[1 - s] : 0.0;
+      y[s] = wn * (a * (c_self * faa[s] + c_other * other) - fai[s]);
+    }
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) {
+      const double aj = ab[j0 + jj];
+#pragma unroll
+      for (int s = 0; s < NSPIN; ++s) g[s][jj] += aj * y[s];
+    }
+  }
+  // block sum in wave order, per spin
+  __syncthreads();
+  double* red = smem;  // [4][NBP][NBP] doubles over the block's LDS
+#pragma unroll
+  for (int s = 0; s < NSPIN; ++s) {
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) red[(wave * NBP + j0 + jj) * NBP + i] = g[s][jj];
+    __syncthreads();
+    const int64_t nb2 = (int64_t)nb * nb;
+    for (int e = tid; e < nb2; e += 256) {
+      const int j = (int)(e / nb), c = (int)(e - (int64_t)j * nb);
+      Gpart[((int64_t)blockIdx.x * NSPIN + s) * nb2 + e] =
+          ((red[j * NBP + c] + red[(NBP + j) * NBP + c]) + red[(2 * NBP + j) * NBP + c]) +
+          red[(3 * NBP + j) * NBP + c];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[e] = red[0];
+}
+
+// Deterministic sum of per-workgroup partials: out[e] = sum_b part[b][e].
+// Workgroup of 1024 (16 waves) per 64 consecutive elements; wave v sums the
+// contiguous slice of partials v * per .. (v + 1) * per - 1 for its lanes'
+// elements (eight loads in flight at a time), then the 16 slice sums are added
+// in wave order.
+__global__ __launch_bounds__(1024) void reduce_partials(const double* __restrict__ part, int nparts, int64_t len,
+                                                        double* __restrict__ out) {
+  __shared__ double red[16][64];
+  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+  const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+  const int per = (nparts + 15) / 16;
+  const int b0 = v * per, b1 = b0 + per < nparts ? b0 + per : nparts;
+  double s = 0;
+  if (e < len) {
+    int b = b0;
+    for (; b + 8 <= b1; b += 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = part[(int64_t)(b + u) * len + e];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; b < b1; ++b) s += part[(int64_t)b * len + e];
+  }
+  red[v][lane] = s;
+  __syncthreads();
+  if (v == 0 && e < len) {
+    double t = 0;
+    for (int u = 0; u < 16; ++u) t += red[u][lane];
+    out[e] = t;
+  }
 }
 
 namespace {
@@ -233,42 +389,67 @@ int hf_prepare(HfPlan& p, int nspin, int nb, int64_t n1, const void* F, int f32,
   p.dPart = ws;
   p.dD = p.dPart + (int64_t)kSweepBlocks * nspin * nb2;
   p.dG = p.dD + nspin * nb2;
-  p.nbp = nb <= 32 ? 32 : 64;
-  const int ppw = 64 / p.nbp, ch = p.nbp == 64 ? 16 : 32;
-  p.shmem = sizeof(double) * (nspin * nb2 + 4 * ppw * p.npair + ch * nb * (1 + nspin) + 4 * ppw * nspin * nb);
+  p.nbp = nb <= 16 ? 16 : nb <= 32 ? 32 : 64;
+  if (p.nbp < 64) {
+    // column-layout kernel: a and X of each wave's point, the unpacked F
+    // matrices; the final block sum reuses the LDS as 4 FP64 matrices
+    const int mat = p.nbp * p.nbp;
+    const int ld = p.nbp == 32 ? (f32 ? hf_col_ld<float, 32>() : hf_col_ld<double, 32>()) : hf_col_ld<double, 16>();
+    const size_t fsz = (size_t)4 * (p.nbp * ld + 2) * (f32 ? sizeof(float) : sizeof(double));
+    p.shmem = std::max(sizeof(double) * 4 * p.nbp * (1 + nspin) + fsz, sizeof(double) * 4 * mat);
+    p.blocks = (int)std::max<int64_t>(1, std::min<int64_t>(kSweepBlocks, (n1 + 3) / 4));
+  } else {
+    const int ppw = 64 / p.nbp, ch = 16;
+    p.shmem = sizeof(double) * (nspin * nb2 + 4 * ppw * p.npair + ch * nb * (1 + nspin) + 4 * ppw * nspin * nb);
+    p.blocks = kSweepBlocks;
+  }
   TC_CHECK(p.shmem <= (size_t)kMaxLdsBytes, "SCF sweep: nb too large for the LDS staging");
   if (p.shmem > 64 * 1024) {
     // opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU)
-    const void* fns[8] = {(const void*)hf_sweep_kernel<float, 1, 32>,  (const void*)hf_sweep_kernel<float, 1, 64>,
-                          (const void*)hf_sweep_kernel<double, 1, 32>, (const void*)hf_sweep_kernel<double, 1, 64>,
-                          (const void*)hf_sweep_kernel<float, 2, 32>,  (const void*)hf_sweep_kernel<float, 2, 64>,
-                          (const void*)hf_sweep_kernel<double, 2, 32>, (const void*)hf_sweep_kernel<double, 2, 64>};
+    const void* fns[12] = {
+        (const void*)hf_sweep_kernel<float, 1, 64>,      (const void*)hf_sweep_kernel<double, 1, 64>,
+        (const void*)hf_sweep_kernel<float, 2, 64>,      (const void*)hf_sweep_kernel<double, 2, 64>,
+        (const void*)hf_sweep_col_kernel<float, 1, 32>,  (const void*)hf_sweep_col_kernel<double, 1, 32>,
+        (const void*)hf_sweep_col_kernel<float, 2, 32>,  (const void*)hf_sweep_col_kernel<double, 2, 32>,
+        (const void*)hf_sweep_col_kernel<float, 1, 16>,  (const void*)hf_sweep_col_kernel<double, 1, 16>,
+        (const void*)hf_sweep_col_kernel<float, 2, 16>,  (const void*)hf_sweep_col_kernel<double, 2, 16>};
     for (const void* f : fns) TC_HIP(lds_opt_in(f));
   }
   return 0;
 }
 
-int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st) {
+int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st, double* G_mapped) {
   const int64_t nb2 = (int64_t)p.nb * p.nb;
   TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * p.nspin * nb2, hipMemcpyHostToDevice, st));
   const double cs = p.nspin == 1 ? 2.0 : 1.0, co = p.nspin == 1 ? 0.0 : 1.0;
-  dim3 grid(kSweepBlocks), block(256);
-#define TC_SWEEP(TF, NS, NBP)                                                                                   \
-  hipLaunchKernelGGL((hf_sweep_kernel<TF, NS, NBP>), grid, block, p.shmem, st, (const TF*)p.F, p.npair, p.A, p.w, \
-                     p.n1, p.nb, p.dD, cs, co, p.dPart)
-  if (p.nspin == 1) {
-    if (p.f32) { if (p.nbp == 32) TC_SWEEP(float, 1, 32); else TC_SWEEP(float, 1, 64); }
-    else { if (p.nbp == 32) TC_SWEEP(double, 1, 32); else TC_SWEEP(double, 1, 64); }
-  } else {
-    if (p.f32) { if (p.nbp == 32) TC_SWEEP(float, 2, 32); else TC_SWEEP(float, 2, 64); }
-    else { if (p.nbp == 32) TC_SWEEP(double, 2, 32); else TC_SWEEP(double, 2, 64); }
-  }
+  dim3 grid(p.blocks), block(256);
+#define TC_SWEEP(K, TF, NS, NBP)                                                                               \
+  hipLaunchKernelGGL((K<TF, NS, NBP>), grid, block, p.shmem, st, (const TF*)p.F, p.npair, p.A, p.w, p.n1, p.nb, \
+                     p.dD, cs, co, p.dPart)
+#define TC_SWEEP_NS(K, NS, NBP) \
+  do {                           \
+    if (p.f32)                   \
+      TC_SWEEP(K, float, NS, NBP); \
+    else                         \
+      TC_SWEEP(K, double, NS, NBP); \
+  } while (0)
+#define TC_SWEEP_ALL(K, NBP)                       \
+  do {                                             \
+    if (p.nspin == 1) TC_SWEEP_NS(K, 1, NBP);      \
+    else TC_SWEEP_NS(K, 2, NBP);                   \
+  } while (0)
+  if (p.nbp == 16) TC_SWEEP_ALL(hf_sweep_col_kernel, 16);
+  else if (p.nbp == 32) TC_SWEEP_ALL(hf_sweep_col_kernel, 32);
+  else TC_SWEEP_ALL(hf_sweep_kernel, 64);
+#undef TC_SWEEP_ALL
+#undef TC_SWEEP_NS
 #undef TC_SWEEP
   TC_HIP(hipGetLastError());
   const int64_t len = p.nspin * nb2;
-  hipLaunchKernelGGL(reduce_partials, dim3((unsigned)len), dim3(256), 0, st, p.dPart, kSweepBlocks, len, p.dG);
+  hipLaunchKernelGGL(reduce_partials, dim3((unsigned)((len + 63) / 64)), dim3(1024), 0, st, p.dPart, p.blocks, len,
+                     G_mapped ? G_mapped : p.dG);
   TC_HIP(hipGetLastError());
-  TC_HIP(hipMemcpyAsync(G_host, p.dG, sizeof(double) * len, hipMemcpyDeviceToHost, st));
+  if (!G_mapped) TC_HIP(hipMemcpyAsync(G_host, p.dG, sizeof(double) * len, hipMemcpyDeviceToHost, st));
   return 0;
 }
 
@@ -340,16 +521,18 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
   HfPlan plan;
   if ((rc = hf_prepare(plan, nspin, nb, n1, F, f_dtype == TCSCF_F32, A1, w1, ws))) return rc;
   HfIter iter(nspin, nocc, nb, S, H, o);
-  // D up and G down through page-locked buffers (pageable vectors if that
-  // allocation fails)
-  Pinned pin(2 * (size_t)nspin * nb2);
+  // D up through a page-locked buffer; G written by the partial-sum kernel
+  // straight into coherent page-locked memory the device maps (a copy from
+  // device memory into pageable vectors where either allocation fails)
+  Pinned pin((size_t)nspin * nb2);
+  Pinned gpin((size_t)nspin * nb2, hipHostMallocCoherent | hipHostMallocMapped);
   std::vector<double> Dv, Gv;
-  if (!pin.p) {
-    Dv.resize(nspin * nb2);
-    Gv.resize(nspin * nb2);
-  }
+  if (!pin.p) Dv.resize(nspin * nb2);
+  if (!gpin.p) Gv.resize(nspin * nb2);
   double* Dh = pin.p ? pin.p : Dv.data();
-  double* Gh = pin.p ? pin.p + nspin * nb2 : Gv.data();
+  double* Gh = gpin.p ? gpin.p : Gv.data();
+  double* Gmap = nullptr;
+  if (gpin.p && hipHostGetDevicePointer((void**)&Gmap, gpin.p, 0) != hipSuccess) Gmap = nullptr;
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
   res->iterations = 0;
@@ -357,7 +540,7 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
   for (int it = 0; it < o->max_iter; ++it) {
     double t0 = now_ms();
     std::copy(iter.D.begin(), iter.D.end(), Dh);
-    if ((rc = hf_enqueue(plan, Dh, Gh, st))) return rc;
+    if ((rc = hf_enqueue(plan, Dh, Gh, st, Gmap))) return rc;
     TC_HIP(hipStreamSynchronize(st));
     double t1 = now_ms();
     res->t_sweep_ms += t1 - t0;

@@ -17,11 +17,11 @@
 namespace tc {
 
 // Page-locked host staging for the per-iteration uploads and downloads
-// (hipHostMalloc); p is null when the allocation failed.
+// (hipHostMalloc with the given flags); p is null when the allocation failed.
 struct Pinned {
   double* p = nullptr;
-  explicit Pinned(size_t n) {
-    if (n && hipHostMalloc((void**)&p, n * sizeof(double), hipHostMallocDefault) != hipSuccess) p = nullptr;
+  explicit Pinned(size_t n, unsigned flags = hipHostMallocDefault) {
+    if (n && hipHostMalloc((void**)&p, n * sizeof(double), flags) != hipSuccess) p = nullptr;
   }
   ~Pinned() {
     if (p) (void)hipHostFree(p);
@@ -41,7 +41,8 @@ struct HfPlan {
   double* dD;     // nspin x nb x nb
   double* dPart;  // per-workgroup partial G
   double* dG;     // nspin x nb x nb
-  int nbp;
+  int nbp;     // lane group per point: 16 / 32 (column kernel) or 64
+  int blocks;  // sweep workgroups (= partials)
   size_t shmem;
 };
 size_t hf_ws_doubles(int nspin, int nb);
@@ -49,8 +50,10 @@ size_t hf_ws_doubles(int nspin, int nb);
 // kernels in to their LDS size.
 int hf_prepare(HfPlan& p, int nspin, int nb, int64_t n1, const void* F, int f32, const double* A, const double* w,
                double* ws);
-// D (host, nspin x nb x nb) -> G (host), asynchronous on st.
-int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st);
+// D (host, nspin x nb x nb) -> G (host), asynchronous on st.  With G_mapped
+// (the device address of G_host, coherent page-locked memory) the partial-sum
+// kernel writes G there directly and no copy follows.
+int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st, double* G_mapped = nullptr);
 
 // Host half of one RCS / UOS iteration (src/HartreeFock.cpp:785-1003): mask,
 // Fock, energy from the current density, convergence test, optional DIIS,
