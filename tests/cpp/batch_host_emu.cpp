@@ -161,6 +161,7 @@ hipError_t hipFree(void* p) {
 }
 hipError_t hipHostMalloc(void** p, size_t n, unsigned int) { *p = std::malloc(n); return hipSuccess; }
 hipError_t hipHostFree(void* p) { std::free(p); return hipSuccess; }
+hipError_t hipHostGetDevicePointer(void** d, void* h, unsigned int) { *d = h; return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) { *s = (hipStream_t)new int(1); return hipSuccess; }
 hipError_t hipStreamDestroy(hipStream_t s) { delete (int*)s; return hipSuccess; }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { *e = (hipEvent_t)new int(2); return hipSuccess; }
