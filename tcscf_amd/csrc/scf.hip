@@ -13,6 +13,7 @@
 
 #include "common.hpp"
 #include "dense.hpp"
+#include "device_util.hpp"
 #include "scf_steps.hpp"
 
 namespace tc {
@@ -138,20 +139,6 @@ __global__ __launch_bounds__(256) void hf_sweep_kernel(const TF* __restrict__ F,
   }
 }
 
-// Raw buffer resources (gfx9 word 3: 32-bit data format, no swizzle): loads
-// at byte offsets at or past the view's size return zero.
-constexpr int kBufferWord3 = 0x00020000;
-template <class T>
-__device__ inline T buffer_load(__amdgpu_buffer_rsrc_t r, int voffset, int soffset);
-template <>
-__device__ inline double buffer_load<double>(__amdgpu_buffer_rsrc_t r, int voffset, int soffset) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voffset, soffset, 0));
-}
-template <>
-__device__ inline float buffer_load<float>(__amdgpu_buffer_rsrc_t r, int voffset, int soffset) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voffset, soffset, 0));
-}
-
 template <class TF, int NBP>
 constexpr int hf_col_ld() { return NBP == 32 ? (sizeof(TF) == 8 ? 33 : 34) : 20; }
 
@@ -230,11 +217,9 @@ __global__ __launch_bounds__(256, NSPIN == 1 ? 3 : 2) void hf_sweep_col_kernel(c
   // elements past the row read as zero (no branches, no clamping)
   const int ic = i < nb ? i : 0;
   auto load = [&](int64_t n) {
-    const __amdgpu_buffer_rsrc_t row =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<TF*>(F + n * npair), (short)0, (int)(npair * sizeof(TF)),
-                                          kBufferWord3);
+    const __amdgpu_buffer_rsrc_t row = buffer_view(F + n * npair, npair);
 #pragma unroll
-    for (int t = 0; t < NL; ++t) pre[t] = buffer_load<TF>(row, lane * (int)sizeof(TF), t * 64 * (int)sizeof(TF));
+    for (int t = 0; t < NL; ++t) pre[t] = buffer_load<TF>(row, (lane + 64 * t) * (int)sizeof(TF));
     apre = A[n * nb + ic];
     wpre = w[n];
   };

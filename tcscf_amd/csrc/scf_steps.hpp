@@ -89,6 +89,11 @@ struct TcPlan {
   int wpw[2], stage[2];
   size_t sh1[2];
   int64_t pblocks[2];
+  // column point kernel (nb <= 32, D = C C^T): usable, LDS bytes, column
+  // count of its instance, dot-product entries; dCc the packed orbitals
+  int col[2], col_noc[2], col_nent[2];
+  size_t col_sh[2];
+  double* dCc;
   int ustart[7];  // rank workgroups per target (prefix offsets)
 };
 size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1, bool general_v);

@@ -29,6 +29,7 @@
 
 #include "common.hpp"
 #include "dense.hpp"
+#include "device_util.hpp"
 #include "scf_steps.hpp"
 
 namespace tc {
@@ -480,6 +481,548 @@ __global__ __launch_bounds__((NS == 1 && sizeof(ST) == 4 && !GV) ? 768 : 512, 1)
   }
 }
 
+/* ------------------------------------------------------------------------
+ * Pass 1 in column layout (nb <= 32) for D_s = C_s C_s^T, the density every
+ * SCF iteration forms from its occupied orbitals.  Then every F- and V-linear
+ * intermediate of a point is a product M C of one of the point's matrices
+ * M in {F_same, F_oppo, V_c, Vo_c} with the occupied orbitals (columns of
+ * spin 0, then spin 1):
+ *   X_s = C_s x_s, x_s = C_s^T A          kS_s,c = C_s y_s,c, y_s,c = C_s^T dA_c
+ *   Faa_s = tr(C_s^T Fs C_s) + tr(C_sb^T Fo C_sb)     FaiS_s = (Fs C_s) x_s
+ *   U_s,c = C_s^T V_c C_s:  V_c . D_s = tr(U_s,c),  sum |U|^2 as before
+ *   Q_s,c = (V_c C_s) x_s    Vai_s = sum_c (V_c C_s) y_s,c
+ *   E_s,c = D_s VTSI_s,c = C_s g_s,c with g_s,c = c_s U_s,c x_s, so
+ *   V(k, i) . E_s = sum_c (V_c C_s) g_s,c and SumOverb_conjVTildeb_occb_c = x_s . g_s,c
+ * One pass over j, uniform across the wave with C[j][.] in scalar registers,
+ * accumulates P_M[i][col] = sum_j M[j][i] C[j][col]: lane h * 32 + i holds
+ * column i of the matrices of half h (h = 0: Fs, Fo, V_0 [, Vo_0]; h = 1:
+ * V_2, [Vo_2 |  -], V_1 [, Vo_1]).  The remaining cross-lane sums (U, x, y)
+ * are dot products over l of C[l][a] with the columns of W = V C, A and dA,
+ * one lane per entry, through LDS.  The G3 V-linear terms h . V(j, i) are
+ * added here from the staged V rows, so the build reads V once.
+ * ---------------------------------------------------------------------- */
+struct ColArgs {
+  const void* Fs;
+  const void* Fo;
+  const void* V;
+  const void* Vo;
+  const double* A;
+  const double* dA;
+  const double* w;
+  double c_same, c_oppo;
+  int64_t n1, npair;
+  int nb, nmax, nocc0, nocc1, nvec, nent;
+  double* vecs;   // [n1][2][nvec][nb]
+  double* alpha;  // [n1][2][2]
+  double* vpart;  // [gridDim.x][NS][npair]: V-linear partial sums per workgroup
+};
+
+// Cc[j][col] (ld noc): the occupied orbitals of spin 0 in columns [0, nocc0),
+// of spin 1 (ns = 2) in [nocc0, nocc0 + nocc1), zero elsewhere; Dw[s][p]
+// (ld kRowStride): D_s packed by pair p = i (i + 1) / 2 + j (j <= i), doubled
+// off the diagonal, zero past npair.
+constexpr int kDwStride = 576;
+__global__ __launch_bounds__(256) void tc_pack_cols(const double* __restrict__ C, const double* __restrict__ D, int nb,
+                                                    int nmax, int nocc0, int nocc1, int ns, int noc,
+                                                    double* __restrict__ Cc, double* __restrict__ Dw) {
+  for (int e = threadIdx.x; e < nb * noc; e += blockDim.x) {
+    const int j = e / noc, col = e - j * noc;
+    double v = 0;
+    if (col < nocc0)
+      v = C[(int64_t)j * nmax + col];
+    else if (ns == 2 && col - nocc0 < nocc1)
+      v = C[((int64_t)nb + j) * nmax + col - nocc0];
+    Cc[e] = v;
+  }
+  const int np = nb * (nb + 1) / 2;
+  for (int e = threadIdx.x; e < 2 * kDwStride; e += blockDim.x) {
+    const int s = e / kDwStride, p = e - s * kDwStride;
+    double v = 0;
+    if (p < np) {
+      int ii = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+      while (ii * (ii + 1) / 2 > p) --ii;
+      while ((ii + 1) * (ii + 2) / 2 <= p) ++ii;
+      const int jj = p - ii * (ii + 1) / 2;
+      v = (jj == ii ? 1.0 : 2.0) * D[((int64_t)s * nb + jj) * nb + ii];
+    }
+    Dw[e] = v;
+  }
+}
+
+// LDS layout of the column kernel, in doubles.  Per workgroup: Cb[32][NOC]
+// (the orbitals), the dot-product entry table (nent ints) and the task table
+// of the step after it (ntask ints).  Per wave: the staged rows (NROW rows of
+// kRowStride ST; later in the point reused for the per-lane results
+// Rq[NS][10][32]), Wt[nb][LDW] (per basis index: the W = V C columns, Fs C, A
+// and dA), Res[nent] (dot products), Xg[NS][4][NOC] (x and y by column),
+// Gs[NS][3][NOC] (g by column), Tn[NS][NT][3][2] (trace and sum of squares of
+// U) and Sc[NS][4] (occ, x . g_c).
+static_assert(kDwStride == 576, "");
+constexpr int kRowStride = kDwStride;  // staged row stride (elements): 9 loads of 64 lanes, >= npair at nb = 32
+template <int NOC, bool GV>
+struct ColShape {
+  static constexpr int NT = GV ? 2 : 1;
+  static constexpr int NROW = GV ? 8 : 4;   // Fs, V_0, V_1, V_2 [, Vo_0..2 and one unused row]
+  static constexpr int XF = 3 * NT * NOC;   // Wt column of Fs C
+  static constexpr int XA = XF + NOC;       // A, then dA_0..2
+  static constexpr int LDW = (XA + 4) | 1;  // odd row stride
+};
+__host__ __device__ inline int64_t col_rows_doubles(int nrow, int esz, int ns) {
+  const int64_t rows = ((int64_t)nrow * kRowStride * esz + 7) / 8;
+  return rows > ns * 10 * 32 ? rows : ns * 10 * 32;
+}
+__host__ __device__ inline int64_t col_wave_doubles(int nrow, int esz, int nb, int ldw, int nent, int ns, int nt,
+                                                    int noc) {
+  return col_rows_doubles(nrow, esz, ns) + (int64_t)nb * ldw + nent + ns * 7 * noc + ns * nt * 6 + ns * 4;
+}
+__host__ __device__ inline int col_tasks(int ns, int nt, int nc0, int nc1, int noc) {
+  return 3 * (nc0 + (ns == 2 ? nc1 : 0)) + ns * nt * 3 + ns + ns * 4 * noc;
+}
+__host__ __device__ inline int64_t col_block_doubles(int noc, int nent, int ntask) {
+  return 32 * noc + (nent + ntask + 1) / 2;
+}
+
+// Entries of the dot-product step, in order: for s, t, c, b < nc_s, a <= b:
+// U^t_s,c[a][b] (Wt column (3 t + c) NOC + off_s + b); then per s: x_s[a]
+// (column XA) and y_s,c[a] (column XA + 1 + c), a < nc_s.
+__host__ __device__ inline int col_entries(int ns, int nt, int nc0, int nc1) {
+  const int t0 = nc0 * (nc0 + 1) / 2, t1 = ns == 2 ? nc1 * (nc1 + 1) / 2 : 0;
+  return nt * 3 * (t0 + t1) + 4 * (nc0 + (ns == 2 ? nc1 : 0));
+}
+
+template <class ST, int NS, bool GV, int NOC>
+__global__ __launch_bounds__(256, 2) void tc_point_col_kernel(ColArgs a, const double* __restrict__ Cc,
+                                                              const double* __restrict__ Dw) {
+  using Sh = ColShape<NOC, GV>;
+  constexpr int NT = Sh::NT, NROW = Sh::NROW, XF = Sh::XF, XA = Sh::XA, LDW = Sh::LDW;
+  constexpr int K = GV ? 4 : 2;              // matrix slots per lane
+  constexpr int NSTG = GV ? 7 : 4;           // rows staged per point
+  constexpr int NL = kRowStride / 64;        // loads per lane per row
+  constexpr int RS = kRowStride;
+  extern __shared__ double smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int h = lane >> 5, i = lane & 31;
+  const int nb = a.nb, nent = a.nent;
+  const int64_t np = a.npair;
+  const int nc[2] = {a.nocc0, NS == 2 ? a.nocc1 : a.nocc0};
+  const int off[2] = {0, NS == 2 ? a.nocc0 : 0};
+  const double cs = a.c_same, co = a.c_oppo;
+  const bool act = i < nb;
+  const int ic = act ? i : nb - 1;
+  const int Ti = ic * (ic + 1) / 2;
+  const int tri0 = nc[0] * (nc[0] + 1) / 2, tri1 = NS == 2 ? nc[1] * (nc[1] + 1) / 2 : 0;
+  const int nu0 = NT * 3 * tri0, nu = nu0 + NT * 3 * tri1;
+  const int na = 3 * (nc[0] + (NS == 2 ? nc[1] : 0)), nbt = NS * NT * 3, ntask = na + nbt + NS + NS * 4 * NOC;
+  auto ubase = [&](int s, int t, int c) { return (s ? nu0 : 0) + (3 * t + c) * (s ? tri1 : tri0); };
+  auto xbase = [&](int s) { return nu + (s ? 4 * nc[0] : 0); };
+
+  // LDS carving
+  double* Cb = smem;                                 // [32][NOC]
+  int* ent = reinterpret_cast<int*>(Cb + 32 * NOC);  // [nent]: acol | wcol << 8
+  int* tsk = ent + nent;                             // [ntask]
+  double* wbase = smem + col_block_doubles(NOC, nent, ntask) +
+                  (int64_t)wave * col_wave_doubles(NROW, sizeof(ST), nb, LDW, nent, NS, NT, NOC);
+  ST* rows = reinterpret_cast<ST*>(wbase);  // [NROW][RS]
+  double* Rq = wbase;                       // [NS][10][32], once the rows are consumed
+  double* Wt = wbase + col_rows_doubles(NROW, sizeof(ST), NS);  // [nb][LDW]
+  double* Res = Wt + (int64_t)nb * LDW;     // [nent]
+  double* Xg = Res + nent;                  // [NS][4][NOC]: x, y_0..2 by column
+  double* Gs = Xg + NS * 4 * NOC;           // [NS][3][NOC]
+  double* Tn = Gs + NS * 3 * NOC;           // [NS][NT][3][2]
+  double* Sc = Tn + NS * NT * 6;            // [NS][4]: occ, x . g_c
+
+  // workgroup set-up: C rows, the entry table and the task table
+  for (int e = tid; e < 32 * NOC; e += blockDim.x) Cb[e] = e / NOC < nb ? Cc[e] : 0.0;
+  for (int e = tid; e < nent; e += blockDim.x) {
+    int acol, wcol;
+    if (e < nu) {
+      const int s = e < nu0 ? 0 : 1, tri = s ? tri1 : tri0;
+      const int r0 = s ? e - nu0 : e, grp = r0 / tri, r = r0 - grp * tri;  // grp = 3 t + c
+      int b = (int)((sqrt(8.0 * r + 1.0) - 1.0) * 0.5);
+      while (b * (b + 1) / 2 > r) --b;
+      while ((b + 1) * (b + 2) / 2 <= r) ++b;
+      acol = off[s] + r - b * (b + 1) / 2;
+      wcol = grp * NOC + off[s] + b;
+    } else {
+      const int r0 = e - nu, s = (NS == 2 && r0 >= 4 * nc[0]) ? 1 : 0, r = s ? r0 - 4 * nc[0] : r0;
+      if (r < nc[s]) {
+        acol = off[s] + r;
+        wcol = XA;
+      } else {
+        const int q = r - nc[s];
+        acol = off[s] + q / 3;
+        wcol = XA + 1 + q % 3;
+      }
+    }
+    ent[e] = acol | (wcol << 8);
+  }
+  // tasks: kind 0 g_s,c[b] | 1 trace and squares of U^t_s,c | 2 occ_s | 3 the
+  // x / y copy of column col (q = 0 x, 1 + c y_c); packed as kind | s << 2 |
+  // c << 3 | t << 5 | b (or col) << 6 | q << 11
+  for (int e = tid; e < ntask; e += blockDim.x) {
+    int v;
+    if (e < na) {
+      const int s = (NS == 2 && e >= 3 * nc[0]) ? 1 : 0, r = s ? e - 3 * nc[0] : e;
+      const int c = r / nc[s], b = r - c * nc[s];
+      v = 0 | (s << 2) | (c << 3) | (b << 6);
+    } else if (e < na + nbt) {
+      const int r = e - na, s = r / (NT * 3), t = (r / 3) % NT, c = r % 3;
+      v = 1 | (s << 2) | (c << 3) | (t << 5);
+    } else if (e < na + nbt + NS) {
+      v = 2 | ((e - na - nbt) << 2);
+    } else {
+      const int r = e - na - nbt - NS, s = r / (4 * NOC), q = (r / NOC) % 4, col = r % NOC;
+      v = 3 | (s << 2) | (col << 6) | (q << 11);
+    }
+    tsk[e] = v;
+  }
+
+  // the lane's matrix slots: without V_oppo half 0 Fs, V_0 and half 1 V_1,
+  // V_2 (staged rows 0..3); with it half 0 Fs, V_0, Vo_0 and an unused row,
+  // half 1 V_1, V_2, Vo_1, Vo_2 (rows 4..7)
+  const int rbase = h * K * RS;
+  double vl[NS][NL];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int u = 0; u < NL; ++u) vl[s][u] = 0;
+
+  // the point's rows through registers, one point ahead: the staged rows and
+  // F_oppo (used only in Faa, at staging)
+  const ST* Fs = (const ST*)a.Fs;
+  const ST* Fo = (const ST*)a.Fo;
+  const ST* Vv = (const ST*)a.V;
+  const ST* Vw = (const ST*)(GV ? a.Vo : a.V);
+  ST pre[NSTG][NL], pfo[NL];
+  double pa = 0, pw = 0, pd[3] = {0, 0, 0};
+  auto load = [&](int64_t nn) {
+    const int64_t n = uniform_i64(nn);
+#pragma unroll
+    for (int r = 0; r < NSTG; ++r) {
+      // pre row r: 0 Fs, 1..3 V_0..2, 4..6 Vo_0..2
+      const ST* src = r == 0 ? Fs + n * np : (r < 4 ? Vv + (n * 3 + r - 1) * np : Vw + (n * 3 + r - 4) * np);
+      const __amdgpu_buffer_rsrc_t rv = buffer_view(src, np);
+#pragma unroll
+      for (int t = 0; t < NL; ++t) pre[r][t] = buffer_load<ST>(rv, (lane + 64 * t) * (int)sizeof(ST));
+    }
+    const __amdgpu_buffer_rsrc_t ro = buffer_view(Fo + n * np, np);
+#pragma unroll
+    for (int t = 0; t < NL; ++t) pfo[t] = buffer_load<ST>(ro, (lane + 64 * t) * (int)sizeof(ST));
+    pa = a.A[n * nb + ic];
+    pd[0] = a.dA[(n * nb + ic) * 3];
+    pd[1] = a.dA[(n * nb + ic) * 3 + 1];
+    pd[2] = a.dA[(n * nb + ic) * 3 + 2];
+    pw = a.w[n];
+  };
+  // staged row of pre row r and of V^t_c
+  //   without Vo: Fs 0, V_0..2 1..3;  with Vo: Fs 0, V_0 1, V_1 4, V_2 5, Vo_0 2, Vo_1 6, Vo_2 7
+  auto srow = [](int r) {
+    constexpr int g[7] = {0, 1, 4, 5, 2, 6, 7};
+    return GV ? g[r] : r;
+  };
+  auto vrow = [&](int t, int c) { return srow(t ? 4 + c : 1 + c); };
+  // packed D with doubled off-diagonal (Faa = sum_p Dw_p F_p), [2][kRowStride]
+  // zero past npair
+  const __amdgpu_buffer_rsrc_t dwv = buffer_view(Dw, 2 * kRowStride);
+  const int64_t gw = (int64_t)blockIdx.x * 4 + wave, nw = (int64_t)gridDim.x * 4;  // 4 waves per workgroup
+  if (gw < a.n1) load(gw);
+  __syncthreads();
+
+  for (int64_t n = gw; n < a.n1; n += nw) {
+    // staged rows (each load past npair reads zero into the row's padding);
+    // Faa_s = tr(Fs D_s) + tr(Fo D_sb) from the rows in registers
+    double fa[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) fa[s] = 0;
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+      const double f = (double)pre[0][t], g = (double)pfo[t];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int sb = NS == 1 ? s : 1 - s;
+        const double d = buffer_load<double>(dwv, (s * RS + lane + 64 * t) * 8);
+        const double db = buffer_load<double>(dwv, (sb * RS + lane + 64 * t) * 8);
+        fa[s] += f * d + g * db;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NSTG; ++r)
+#pragma unroll
+      for (int t = 0; t < NL; ++t) rows[srow(r) * RS + lane + 64 * t] = pre[r][t];
+    const double Ai = act ? pa : 0.0, wn = pw;
+    const double dAi[3] = {act ? pd[0] : 0.0, act ? pd[1] : 0.0, act ? pd[2] : 0.0};
+    load(n + nw < a.n1 ? n + nw : n);
+    double faa[2];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) faa[s] = uniform(wave_sum(fa[s]));
+    wave_sync_lds();
+
+    {
+      // P[k][col] = sum_j M_k[j][i] C[j][col]
+      double acc[K][NOC];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int col = 0; col < NOC; ++col) acc[k][col] = 0;
+      const ST* lr = rows + rbase;
+#pragma unroll 2
+      for (int j = 0; j < nb; ++j) {
+        const int pj = j <= ic ? Ti + j : j * (j + 1) / 2 + ic;
+        double m[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) m[k] = (double)lr[k * RS + pj];
+        const double* cj = Cc + (int64_t)j * NOC;
+#pragma unroll
+        for (int col = 0; col < NOC; ++col) {
+          const double c = cj[col];
+#pragma unroll
+          for (int k = 0; k < K; ++k) acc[k][col] += m[k] * c;
+        }
+      }
+      // row i of Wt: the W^t_c columns from the slots that hold them, half 0
+      // also Fs C, A and dA
+      if (act) {
+        double* wr = Wt + (int64_t)i * LDW;
+        if (!h) {
+#pragma unroll
+          for (int col = 0; col < NOC; ++col) {
+            wr[XF + col] = acc[0][col];                  // Fs C
+            wr[col] = acc[1][col];                       // W^0_0
+            if (GV) wr[3 * NOC + col] = acc[2][col];     // W^1_0
+          }
+          wr[XA] = Ai;
+          wr[XA + 1] = dAi[0];
+          wr[XA + 2] = dAi[1];
+          wr[XA + 3] = dAi[2];
+        } else {
+#pragma unroll
+          for (int col = 0; col < NOC; ++col) {
+            wr[NOC + col] = acc[0][col];                 // W^0_1
+            wr[2 * NOC + col] = acc[1][col];             // W^0_2
+            if (GV) {
+              wr[4 * NOC + col] = acc[2][col];           // W^1_1
+              wr[5 * NOC + col] = acc[K - 1][col];       // W^1_2
+            }
+          }
+        }
+      }
+    }
+    wave_sync_lds();
+    // dot products over l: U, x, y
+    for (int e = lane; e < nent; e += 64) {
+      const int en = ent[e], acol = en & 0xff, wcol = en >> 8;
+      const double* cb = Cb + acol;
+      const double* wt = Wt + wcol;
+      double v0 = 0, v1 = 0, v2 = 0, v3 = 0;  // four chains of loads in flight
+      int l = 0;
+      for (; l + 4 <= nb; l += 4) {
+        v0 += cb[l * NOC] * wt[l * LDW];
+        v1 += cb[(l + 1) * NOC] * wt[(l + 1) * LDW];
+        v2 += cb[(l + 2) * NOC] * wt[(l + 2) * LDW];
+        v3 += cb[(l + 3) * NOC] * wt[(l + 3) * LDW];
+      }
+      for (; l < nb; ++l) v0 += cb[l * NOC] * wt[l * LDW];
+      Res[e] = (v0 + v1) + (v2 + v3);
+    }
+    wave_sync_lds();
+    // per spin: g_s,c = c_s U^0_s,c x_s (by column); trace and sum of squares
+    // of U^t_s,c (2 sum over the triangle - sum of the diagonal); occ_s =
+    // |x_s|^2; x and y by column
+    for (int e = lane; e < ntask; e += 64) {
+      const int v = tsk[e], kind = v & 3, s = (v >> 2) & 1, c = (v >> 3) & 3, t = (v >> 5) & 1, b = (v >> 6) & 31;
+      if (kind == 0) {
+        const int ub = ubase(s, 0, c), xb = xbase(s);
+        double g0 = 0, g1 = 0;
+        for (int q = 0; q < nc[s]; ++q) {
+          const int lo = q < b ? q : b, hi = q < b ? b : q;
+          const double u = Res[ub + hi * (hi + 1) / 2 + lo] * Res[xb + q];
+          if (q & 1) g1 += u; else g0 += u;
+        }
+        Gs[(s * 3 + c) * NOC + off[s] + b] = cs * (g0 + g1);
+      } else if (kind == 1) {
+        const int tri = s ? tri1 : tri0, ub = ubase(s, t, c);
+        double sq0 = 0, sq1 = 0, tr = 0, dq = 0;
+        int q = 0;
+        for (; q + 2 <= tri; q += 2) {
+          const double u0 = Res[ub + q], u1 = Res[ub + q + 1];
+          sq0 += u0 * u0;
+          sq1 += u1 * u1;
+        }
+        if (q < tri) sq0 += Res[ub + q] * Res[ub + q];
+        for (int p = 0; p < nc[s]; ++p) {
+          const double d = Res[ub + p * (p + 1) / 2 + p];
+          tr += d;
+          dq += d * d;
+        }
+        Tn[((s * NT + t) * 3 + c) * 2] = tr;
+        Tn[((s * NT + t) * 3 + c) * 2 + 1] = 2 * (sq0 + sq1) - dq;
+      } else if (kind == 2) {
+        const int xb = xbase(s);
+        double v0 = 0, v1 = 0;
+        for (int q = 0; q < nc[s]; ++q) {
+          const double x = Res[xb + q];
+          if (q & 1) v1 += x * x; else v0 += x * x;
+        }
+        Sc[s * 4] = v0 + v1;  // occ_s = |x_s|^2
+      } else {
+        // x_s (q = 0) and y_s,c (q = 1 + c) at their columns, zero elsewhere
+        const int col = b, q = v >> 11, bb = col - off[s];
+        const bool in = bb >= 0 && bb < nc[s];
+        const double xv = in ? Res[xbase(s) + (q == 0 ? bb : nc[s] + 3 * bb + q - 1)] : 0.0;
+        Xg[(s * 4 + q) * NOC + col] = xv;
+        if (q < 3 && !in) Gs[(s * 3 + q) * NOC + col] = 0.0;  // g outside the spin's columns
+      }
+    }
+    wave_sync_lds();
+    // x_s . g_s,c (SumOverb_conjVTildeb_occb) by column, one lane per (s, c)
+    if (lane < 3 * NS) {
+      const int s = lane / 3, c = lane % 3;
+      double v = 0;
+#pragma unroll
+      for (int col = 0; col < NOC; ++col) v += Xg[s * 4 * NOC + col] * Gs[(s * 3 + c) * NOC + col];
+      Sc[s * 4 + 1 + c] = v;
+    }
+    wave_sync_lds();
+    // point-uniform values
+    double vtv[2][3], occ[2], scvt[2][3], nr[NT][2];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      occ[s] = Sc[s * 4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) scvt[s][c] = Sc[s * 4 + 1 + c];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        nr[t][s] = Tn[((s * NT + t) * 3) * 2 + 1] + Tn[((s * NT + t) * 3 + 1) * 2 + 1] + Tn[((s * NT + t) * 3 + 2) * 2 + 1];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int sb = NS == 1 ? s : 1 - s;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double vd = Tn[((s * NT + 0) * 3 + c) * 2], vdb = Tn[((sb * NT + 0) * 3 + c) * 2];
+        vtv[s][c] = GV ? vd + Tn[((sb * NT + 1) * 3 + c) * 2] : cs * vd + co * vdb;  // VaaSamePlusOppo
+      }
+    }
+    // G3's V-linear terms h_same . V_same(j, i) + h_oppo . V_oppo(j, i) over
+    // the staged V rows (lanes past npair add the rows' zero padding)
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int sb = NS == 1 ? s : 1 - s;
+      double hv[NT][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (GV) {
+          hv[0][c] = wn * (2 * occ[s] * vtv[s][c] - 2 * scvt[s][c]);
+          hv[NT - 1][c] = wn * (2 * occ[sb] * vtv[sb][c] - 2 * scvt[sb][c]);
+        } else {
+          hv[0][c] = wn * (2 * occ[s] * cs * vtv[s][c] - 2 * cs * scvt[s][c] + 2 * occ[sb] * co * vtv[sb][c] -
+                           2 * co * scvt[sb][c]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NL; ++u) {
+        const int p = lane + 64 * u;
+        double v = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v += hv[t][c] * (double)rows[vrow(t, c) * RS + p];
+        vl[s][u] += v;
+      }
+    }
+    wave_sync_lds();  // the rows are consumed: Rq may overwrite them
+    // per-lane dot products over the columns, five per lane, the same code in
+    // both halves: half 0 Fs C . x, W_0 . x, W_1 . x, W_0 . y_0, W_0 . g_0;
+    // half 1 W_2 . x, W_1 . y_1, W_2 . y_2, W_1 . g_1, W_2 . g_2
+    if (act) {
+      const double* wr = Wt + (int64_t)i * LDW;
+      const int wsrc[5] = {h ? 2 * NOC : XF, h ? NOC : 0, h ? 2 * NOC : NOC, h ? NOC : 0, h ? 2 * NOC : 0};
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const double* xg = Xg + s * 4 * NOC;
+        const double* gs = Gs + s * 3 * NOC;
+        const double* csrc[5] = {xg, h ? xg + 2 * NOC : xg, h ? xg + 3 * NOC : xg, h ? gs + NOC : xg + NOC,
+                                 h ? gs + 2 * NOC : gs};
+        double r[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int col = 0; col < NOC; ++col)
+#pragma unroll
+          for (int k = 0; k < 5; ++k) r[k] += wr[wsrc[k] + col] * csrc[k][col];
+        double* rq = Rq + s * 10 * 32;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rq[(5 * h + k) * 32 + i] = r[k];
+      }
+    }
+    wave_sync_lds();
+    // the point's vectors: half 0 u0..u4, half 1 u5, u6; the W vectors.  Rq
+    // rows: 0 FaiS, 1 Q_0, 2 Q_1, 3 Vai y_0 part, 4 V.E g_0 part, 5 Q_2,
+    // 6 Vai y_1 part, 7 Vai y_2 part, 8 V.E g_1 part, 9 V.E g_2 part
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int sb = NS == 1 ? s : 1 - s;
+      double* vs = a.vecs + ((int64_t)n * 2 + s) * a.nvec * nb;
+      const double* rq = Rq + s * 10 * 32;
+      if (act) {
+        if (!h) {
+          const double fai = rq[i], vt0 = cs * rq[32 + i], vt1 = cs * rq[64 + i], vt2 = cs * rq[160 + i];
+          const double vai = rq[96 + i] + rq[192 + i] + rq[224 + i];
+          const double sv1 = rq[128 + i] + rq[256 + i] + rq[288 + i];
+          const double vtdot = vtv[s][0] * vt0 + vtv[s][1] * vt1 + vtv[s][2] * vt2;
+          const double vtn = vtv[s][0] * vtv[s][0] + vtv[s][1] * vtv[s][1] + vtv[s][2] * vtv[s][2];
+          const double Z = 2 * wn * (cs * sv1 - vtdot);
+          const double sNS = GV ? nr[0][s] : cs * cs * nr[0][s];
+          const double sNOb = GV ? nr[NT - 1][sb] : co * co * nr[0][sb];
+          vs[0 * nb + i] = wn * (faa[s] * Ai - fai);
+          vs[1 * nb + i] = wn * (vtv[s][0] * dAi[0] + vtv[s][1] * dAi[1] + vtv[s][2] * dAi[2] - cs * vai);
+          vs[2 * nb + i] = wn * (vtn - sNS - sNOb) * Ai + Z;
+          vs[3 * nb + i] = Z;
+          vs[4 * nb + i] = vt0;
+        } else {
+          vs[5 * nb + i] = cs * rq[64 + i];
+          vs[6 * nb + i] = cs * rq[160 + i];
+        }
+        // W^t_s,c[i][b] for b < nmax (zero past nc_s), rows 3 (t nmax + b) + c;
+        // the halves take alternate b
+        const double* wr = Wt + (int64_t)i * LDW;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          for (int b = h; b < a.nmax; b += 2)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              vs[(kVecFixed + 3 * (t * a.nmax + b) + c) * nb + i] =
+                  b < nc[s] ? wr[(3 * t + c) * NOC + off[s] + b] : 0.0;
+      }
+      double* al = a.alpha + (n * 2 + s) * 2;
+      if (lane == 0) {
+        if (GV) {
+          al[0] = -2 * wn * occ[s];
+          al[1] = -2 * wn * occ[sb];
+        } else {
+          al[0] = -2 * wn * (occ[s] * cs * cs + occ[sb] * co * co);
+        }
+      }
+    }
+    wave_sync_lds();
+  }
+  // V-linear partials: the waves' sums added in wave order
+  __syncthreads();
+  double* red = smem;  // [4 waves][NS][npair] over the workgroup's LDS
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int p = lane + 64 * u;
+      if (p < np) red[((int64_t)wave * NS + s) * np + p] = vl[s][u];
+    }
+  __syncthreads();
+  for (int64_t e = tid; e < NS * np; e += blockDim.x) {
+    double v = 0;
+    for (int wv = 0; wv < 4; ++wv) v += red[(int64_t)wv * NS * np + e];
+    a.vpart[(int64_t)blockIdx.x * NS * np + e] = v;
+  }
+}
+
 struct RankArgs {
   const double* vecs;   // [n1][2][nvec][nb]
   const double* alpha;  // [n1][2 spins][2 tensors]
@@ -715,6 +1258,24 @@ inline const void* point_kernel(int f32, int stage, bool restricted, bool gv) {
   return restricted ? point_kernel_ns<1, false>(f32, stage) : point_kernel_ns<2, false>(f32, stage);
 }
 
+// The column-kernel instance for a storage type, spin count, V form and
+// column count noc (4, 5, 8 or 16).
+template <int NS, bool GV, int NOC>
+inline const void* col_kernel_n(int f32) {
+  return f32 ? (const void*)tc_point_col_kernel<float, NS, GV, NOC> : (const void*)tc_point_col_kernel<double, NS, GV, NOC>;
+}
+template <int NS, bool GV>
+inline const void* col_kernel_g(int f32, int noc) {
+  return noc <= 4 ? col_kernel_n<NS, GV, 4>(f32)
+                  : (noc <= 5 ? col_kernel_n<NS, GV, 5>(f32)
+                              : (noc <= 8 ? col_kernel_n<NS, GV, 8>(f32) : col_kernel_n<NS, GV, 16>(f32)));
+}
+inline const void* col_kernel(int f32, bool restricted, bool gv, int noc) {
+  if (gv) return restricted ? col_kernel_g<1, true>(f32, noc) : col_kernel_g<2, true>(f32, noc);
+  return restricted ? col_kernel_g<1, false>(f32, noc) : col_kernel_g<2, false>(f32, noc);
+}
+inline int col_ldw(int noc, int nt) { return (3 * nt * noc + noc + 4) | 1; }
+
 }  // namespace tcs
 }  // namespace tc
 
@@ -737,7 +1298,7 @@ size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1, bool general_v) {
   int us[7];
   rank_units(nup, ndown, nwt, n1, us);
   return 2 * nb2 + 2 * (size_t)nb * nmax + (size_t)n1 * 2 * nvec * nb + (size_t)n1 * 4 + (size_t)n1 * 12 +
-         (size_t)us[6] * nb2 + (size_t)kBlocks * 2 * npair + tc_out_doubles(nb);
+         (size_t)us[6] * nb2 + (size_t)kBlocks * 2 * npair + tc_out_doubles(nb) + 32 * 16 + 2 * kDwStride;
 }
 
 // waves per workgroup of the unstaged point kernel (halved while the LDS does not fit)
@@ -780,9 +1341,12 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   rank_units(nup, ndown, p.nwt, n1, p.ustart);
   p.dPV = p.dPG + (int64_t)p.ustart[6] * nb2;
   p.dOut = p.dPV + (int64_t)kBlocks * 2 * p.npair;
+  p.dCc = p.dOut + tc_out_doubles(nb);
   TC_CHECK(tc_launch_shape(p), "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < 2; ++r) {
     if (p.sh1[r] > 64 * 1024) TC_HIP(lds_opt_in(point_kernel(p.f32, p.stage[r], r, p.gv)));
+    if (p.col[r] && p.col_sh[r] > 64 * 1024) TC_HIP(lds_opt_in(col_kernel(p.f32, r, p.gv, p.col_noc[r])));
+  }
   return 0;
 }
 
@@ -811,6 +1375,18 @@ bool tc_launch_shape(TcPlan& p) {
     p.sh1[r] = shared + p.wpw[r] * per_wave;
     if (p.sh1[r] > (size_t)kMaxLdsBytes) return false;
     p.pblocks[r] = std::max<int64_t>(1, std::min<int64_t>(2048, (p.n1 + p.wpw[r] - 1) / p.wpw[r]));
+    // column kernel: nb <= 32 and at most 16 occupied columns (both spins)
+    const int ncol = ns == 1 ? std::max(p.nup, p.ndown) : p.nup + p.ndown;
+    const int noc = ncol <= 4 ? 4 : (ncol <= 5 ? 5 : (ncol <= 8 ? 8 : 16));
+    const int nt = p.gv ? 2 : 1, nrow = p.gv ? 8 : 4;
+    const int nc0 = ns == 1 ? std::max(p.nup, p.ndown) : p.nup;
+    p.col_noc[r] = noc;
+    p.col_nent[r] = col_entries(ns, nt, nc0, p.ndown);
+    const int ntask = col_tasks(ns, nt, nc0, p.ndown, noc);
+    p.col_sh[r] = sizeof(double) * (col_block_doubles(noc, p.col_nent[r], ntask) +
+                                    4 * col_wave_doubles(nrow, p.f32 ? 4 : 8, nb, col_ldw(noc, nt), p.col_nent[r],
+                                                         ns, nt, noc));
+    p.col[r] = nb <= 32 && ncol <= 16 && (ns == 2 || p.nup == p.ndown) && p.col_sh[r] <= (size_t)kMaxLdsBytes;
   }
   return true;
 }
@@ -821,18 +1397,56 @@ bool tc_restricted(const TcPlan& p, const double* D, const double* C) {
          std::memcmp(C, C + nc, nc * sizeof(double)) == 0;
 }
 
+// D_s = C_s C_s^T for the occupied columns of each spin formed (to rounding:
+// 1e-14 of the largest |D|), the condition of the column point kernel.
+static bool density_from_orbitals(const TcPlan& p, const double* D, const double* C, bool restricted) {
+  const int nb = p.nb;
+  double dmax = 0;
+  for (int64_t q = 0; q < 2 * (int64_t)nb * nb; ++q) dmax = std::max(dmax, std::fabs(D[q]));
+  const double tol = 1e-14 * (1.0 + dmax);
+  for (int s = 0; s < (restricted ? 1 : 2); ++s) {
+    const int no = s ? p.ndown : p.nup;
+    const double* Cs = C + (size_t)s * nb * p.nmax;
+    for (int u = 0; u < nb; ++u)
+      for (int v = 0; v < nb; ++v) {
+        double t = 0;
+        for (int b = 0; b < no; ++b) t += Cs[(size_t)u * p.nmax + b] * Cs[(size_t)v * p.nmax + b];
+        if (!(std::fabs(t - D[((size_t)s * nb + u) * nb + v]) <= tol)) return false;
+      }
+  }
+  return true;
+}
+
 int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, bool restricted,
                hipStream_t st) {
   const int64_t nb2 = (int64_t)p.nb * p.nb;
   const int nst = restricted ? 3 : 6;  // rank targets (spin, term) formed
   TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * 2 * nb2, hipMemcpyHostToDevice, st));
   TC_HIP(hipMemcpyAsync(p.dC, C_host, sizeof(double) * 2 * p.nb * p.nmax, hipMemcpyHostToDevice, st));
-  PointArgs a{p.Fs, p.Fo, p.V, p.Vo, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb,
-              p.nmax, p.nup, p.ndown, p.nvec, p.dVec, p.dAl, p.dH};
-  void* pargs[] = {&a};
   const int r = restricted ? 1 : 0;
-  TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage[r], restricted, p.gv), dim3((unsigned)p.pblocks[r]),
-                         dim3(64 * p.wpw[r]), pargs, p.sh1[r], st));
+  const bool col = p.col[r] && density_from_orbitals(p, D_host, C_host, restricted);
+  if (col) {
+    // orbitals packed by column, then the column point kernel (its V-linear
+    // sums included) over kBlocks workgroups of 4 waves
+    const int ns = restricted ? 1 : 2, noc = p.col_noc[r];
+    double* dw = p.dCc + 32 * 16;
+    hipLaunchKernelGGL(tc_pack_cols, dim3(1), dim3(256), 0, st, p.dC, p.dD, p.nb, p.nmax, p.nup, p.ndown, ns, noc,
+                       p.dCc, dw);
+    TC_HIP(hipGetLastError());
+    ColArgs ca{p.Fs, p.Fo, p.V, p.Vo, p.A1, p.dA1, p.w1, p.c_same, p.c_oppo, p.n1, p.npair, p.nb, p.nmax,
+               p.nup, p.ndown, p.nvec, p.col_nent[r], p.dVec, p.dAl, p.dPV};
+    const double* cc = p.dCc;
+    const double* dwc = dw;
+    void* cargs[] = {&ca, &cc, &dwc};
+    TC_HIP(hipLaunchKernel(col_kernel(p.f32, restricted, p.gv, noc), dim3(kBlocks), dim3(256), cargs, p.col_sh[r],
+                           st));
+  } else {
+    PointArgs a{p.Fs, p.Fo, p.V, p.Vo, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb,
+                p.nmax, p.nup, p.ndown, p.nvec, p.dVec, p.dAl, p.dH};
+    void* pargs[] = {&a};
+    TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage[r], restricted, p.gv), dim3((unsigned)p.pblocks[r]),
+                           dim3(64 * p.wpw[r]), pargs, p.sh1[r], st));
+  }
   RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.nmax, p.nwt, {}, p.dPG};
   RankRanges rr{};
   for (int q = 0; q < 7; ++q) ra.ustart[q] = rr.ustart[q] = p.ustart[q];
@@ -845,19 +1459,21 @@ int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, doub
     hipLaunchKernelGGL(tc_rank_kernel<4>, rgrid, dim3(256), 0, st, ra);
   TC_HIP(hipGetLastError());
   const int nk = restricted ? 1 : 2;  // spins of the V-linear sums
-  const dim3 vg(kBlocks), vb(256);
+  if (!col) {
+    const dim3 vg(kBlocks), vb(256);
 #define TC_VLIN(ST, NK, NT)                                                                                       \
   hipLaunchKernelGGL((tc_vlin_kernel<ST, NK, NT>), vg, vb, 0, st, (const ST*)p.V, (const ST*)(p.gv ? p.Vo : p.V), \
                      p.dH, p.n1, p.npair, p.dPV)
-  if (p.f32) {
-    if (p.gv) { if (restricted) TC_VLIN(float, 1, 2); else TC_VLIN(float, 2, 2); }
-    else { if (restricted) TC_VLIN(float, 1, 1); else TC_VLIN(float, 2, 1); }
-  } else {
-    if (p.gv) { if (restricted) TC_VLIN(double, 1, 2); else TC_VLIN(double, 2, 2); }
-    else { if (restricted) TC_VLIN(double, 1, 1); else TC_VLIN(double, 2, 1); }
-  }
+    if (p.f32) {
+      if (p.gv) { if (restricted) TC_VLIN(float, 1, 2); else TC_VLIN(float, 2, 2); }
+      else { if (restricted) TC_VLIN(float, 1, 1); else TC_VLIN(float, 2, 1); }
+    } else {
+      if (p.gv) { if (restricted) TC_VLIN(double, 1, 2); else TC_VLIN(double, 2, 2); }
+      else { if (restricted) TC_VLIN(double, 1, 1); else TC_VLIN(double, 2, 1); }
+    }
 #undef TC_VLIN
-  TC_HIP(hipGetLastError());
+    TC_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(tc_reduce_ranges, dim3((unsigned)(nst * nb2)), dim3(256), 0, st, p.dPG, rr, nb2, p.dOut);
   TC_HIP(hipGetLastError());
   hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(nk * p.npair)), dim3(256), 0, st, p.dPV, kBlocks, nk * p.npair,

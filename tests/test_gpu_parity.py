@@ -576,7 +576,8 @@ def test_tc_zero_jastrow_equals_uos(tc, gpu):
 
 @pytest.mark.parametrize("nup,ndown,vform", [(2, 2, "one"), (2, 1, "one"), (1, 1, "one"), (2, 2, "two"),
                                               (2, 1, "two"), (1, 1, "two")])
-def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown, vform):
+@pytest.mark.parametrize("d_off", [False, True])
+def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown, vform, d_off):
     """One device TC Fock build at a given density and orbitals (different per
     spin) against the oracle's factorized restatement (G2 and G3), and G3
     against assembleG over the spin-labelled explicit three-body tensors
@@ -593,6 +594,10 @@ def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown, vform):
     for s, no in enumerate((nup, ndown)):
         C[s, :, :no] = rng.normal(size=(nb, no)) * 0.5
         D[s] = C[s, :, :no] @ C[s, :, :no].T
+    if d_off:
+        # a density that is not C C^T: the build then runs the D-based point
+        # kernel instead of the column kernel (which needs D = C C^T)
+        D = D + 0.01 * np.eye(nb)[None]
     A = g1.basisValues.cpu().numpy()
     dA = g1.basisGradients.cpu().numpy()
     if vform == "one":
@@ -608,8 +613,9 @@ def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown, vform):
                          tc.unpack(Fo.cpu().numpy(), nb), Vs, Vo, D, C)
     assert rel_err(G2, G2o) < 1e-12
     assert rel_err(G3, G3o) < 1e-12
-    G3q = O.assemble_G3(*O.compute_Q(A, g1.weights_host, Vs, Vo, labelled=True), D)
-    assert rel_err(G3, G3q) < 1e-11
+    if not d_off:  # the explicit three-body form holds for D = C C^T
+        G3q = O.assemble_G3(*O.compute_Q(A, g1.weights_host, Vs, Vo, labelled=True), D)
+        assert rel_err(G3, G3q) < 1e-11
 
 
 @pytest.mark.parametrize("nbasis,nup,ndown,same,dtype,vform,launch",
