@@ -1258,22 +1258,16 @@ inline const void* point_kernel(int f32, int stage, bool restricted, bool gv) {
   return restricted ? point_kernel_ns<1, false>(f32, stage) : point_kernel_ns<2, false>(f32, stage);
 }
 
-// The column-kernel instance for a storage type, spin count, V form and
+// The column-kernel instance (FP32 rows, one V tensor) for a spin count and
 // column count noc (4, 5, 8 or 16).
-template <int NS, bool GV, int NOC>
-inline const void* col_kernel_n(int f32) {
-  return f32 ? (const void*)tc_point_col_kernel<float, NS, GV, NOC> : (const void*)tc_point_col_kernel<double, NS, GV, NOC>;
+template <int NS>
+inline const void* col_kernel_g(int noc) {
+  return noc <= 4 ? (const void*)tc_point_col_kernel<float, NS, false, 4>
+                  : (noc <= 5 ? (const void*)tc_point_col_kernel<float, NS, false, 5>
+                              : (noc <= 8 ? (const void*)tc_point_col_kernel<float, NS, false, 8>
+                                          : (const void*)tc_point_col_kernel<float, NS, false, 16>));
 }
-template <int NS, bool GV>
-inline const void* col_kernel_g(int f32, int noc) {
-  return noc <= 4 ? col_kernel_n<NS, GV, 4>(f32)
-                  : (noc <= 5 ? col_kernel_n<NS, GV, 5>(f32)
-                              : (noc <= 8 ? col_kernel_n<NS, GV, 8>(f32) : col_kernel_n<NS, GV, 16>(f32)));
-}
-inline const void* col_kernel(int f32, bool restricted, bool gv, int noc) {
-  if (gv) return restricted ? col_kernel_g<1, true>(f32, noc) : col_kernel_g<2, true>(f32, noc);
-  return restricted ? col_kernel_g<1, false>(f32, noc) : col_kernel_g<2, false>(f32, noc);
-}
+inline const void* col_kernel(bool restricted, int noc) { return restricted ? col_kernel_g<1>(noc) : col_kernel_g<2>(noc); }
 inline int col_ldw(int noc, int nt) { return (3 * nt * noc + noc + 4) | 1; }
 
 }  // namespace tcs
@@ -1345,7 +1339,7 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   TC_CHECK(tc_launch_shape(p), "tcscf_tc_fock: basis / occupied orbitals too large for the LDS staging");
   for (int r = 0; r < 2; ++r) {
     if (p.sh1[r] > 64 * 1024) TC_HIP(lds_opt_in(point_kernel(p.f32, p.stage[r], r, p.gv)));
-    if (p.col[r] && p.col_sh[r] > 64 * 1024) TC_HIP(lds_opt_in(col_kernel(p.f32, r, p.gv, p.col_noc[r])));
+    if (p.col[r] && p.col_sh[r] > 64 * 1024) TC_HIP(lds_opt_in(col_kernel(r, p.col_noc[r])));
   }
   return 0;
 }
@@ -1386,7 +1380,10 @@ bool tc_launch_shape(TcPlan& p) {
     p.col_sh[r] = sizeof(double) * (col_block_doubles(noc, p.col_nent[r], ntask) +
                                     4 * col_wave_doubles(nrow, p.f32 ? 4 : 8, nb, col_ldw(noc, nt), p.col_nent[r],
                                                          ns, nt, noc));
-    p.col[r] = nb <= 32 && ncol <= 16 && (ns == 2 || p.nup == p.ndown) && p.col_sh[r] <= (size_t)kMaxLdsBytes;
+    // (FP32 rows of one V tensor: with FP64 rows or independent V_same / V_oppo
+    // its staging registers spill and the D-based kernel is faster)
+    p.col[r] = nb <= 32 && ncol <= 16 && p.f32 && !p.gv && (ns == 2 || p.nup == p.ndown) &&
+               p.col_sh[r] <= (size_t)kMaxLdsBytes;
   }
   return true;
 }
@@ -1438,7 +1435,7 @@ int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, doub
     const double* cc = p.dCc;
     const double* dwc = dw;
     void* cargs[] = {&ca, &cc, &dwc};
-    TC_HIP(hipLaunchKernel(col_kernel(p.f32, restricted, p.gv, noc), dim3(kBlocks), dim3(256), cargs, p.col_sh[r],
+    TC_HIP(hipLaunchKernel(col_kernel(restricted, noc), dim3(kBlocks), dim3(256), cargs, p.col_sh[r],
                            st));
   } else {
     PointArgs a{p.Fs, p.Fo, p.V, p.Vo, p.A1, p.dA1, p.w1, p.dD, p.dC, p.c_same, p.c_oppo, p.n1, p.npair, p.nb,

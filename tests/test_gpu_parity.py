@@ -577,17 +577,22 @@ def test_tc_zero_jastrow_equals_uos(tc, gpu):
 @pytest.mark.parametrize("nup,ndown,vform", [(2, 2, "one"), (2, 1, "one"), (1, 1, "one"), (2, 2, "two"),
                                               (2, 1, "two"), (1, 1, "two")])
 @pytest.mark.parametrize("d_off", [False, True])
-def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown, vform, d_off):
+@pytest.mark.parametrize("f32", [False, True])
+def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown, vform, d_off, f32):
     """One device TC Fock build at a given density and orbitals (different per
     spin) against the oracle's factorized restatement (G2 and G3), and G3
     against assembleG over the spin-labelled explicit three-body tensors
     (src/HartreeFock.cpp:613-750).  vform 'one': one c = 1 tensor with the r12
     Jastrow's scales; 'two': independent V_same / V_oppo of a general Ochi
-    Jastrow (the reference's VjiSame / VjiOppo)."""
+    Jastrow (the reference's VjiSame / VjiOppo).  f32: the tensors stored in
+    FP32 (the oracle fed the same values); with D = C C^T and one V tensor this
+    runs the column point kernel."""
     from tcscf_amd import SlaterTypeOrbital as STO
     basis = [STO(1.3, 1, 0, 0), STO(0.9, 2, 1, 0), STO(0.7, 2, 0, 0)]
     nb = len(basis)
     g1, g3, jas, Fs, Fo, V = _tc_case(tc, basis, (6, 7, tc.RADIAL_M4), (8, 7, tc.RADIAL_M4))
+    if f32:
+        Fs, Fo, V = Fs.float(), Fo.float(), V.float()
     rng = np.random.default_rng(11)
     C = np.zeros((2, nb, max(nup, ndown)))
     D = np.zeros((2, nb, nb))
@@ -602,15 +607,17 @@ def test_tc_fock_build_vs_oracle_and_explicit_Q(tc, O, gpu, nup, ndown, vform, d
     dA = g1.basisGradients.cpu().numpy()
     if vform == "one":
         G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, V, jas, D, C)
-        Vf = tc.unpack(V.cpu().numpy(), nb).transpose(0, 2, 3, 1)
+        Vf = tc.unpack(V.double().cpu().numpy(), nb).transpose(0, 2, 3, 1)
         Vs, Vo = np.ascontiguousarray(jas.c_same * Vf), np.ascontiguousarray(jas.c_oppo * Vf)
     else:
         u = tc.OchiJastrow(1.1, 1.0, [(0.5, 0.25), (-0.3, 0.2), (0.15, -0.4)], _OCHI_TERMS[:3])
         Vsd, Vod = tc.computeV(g1, g3, u, same_spin=True), tc.computeV(g1, g3, u, same_spin=False)
+        if f32:
+            Vsd, Vod = Vsd.float(), Vod.float()
         G2, G3 = tc.TCHartreeFock(nup, ndown, nb).fockBuild(g1, Fs, Fo, Vsd, Vod, D, C)
-        Vs, Vo = (np.ascontiguousarray(tc.unpack(T.cpu().numpy(), nb).transpose(0, 2, 3, 1)) for T in (Vsd, Vod))
-    G2o, G3o = O.tc_fock(nup, ndown, A, dA, g1.weights_host, tc.unpack(Fs.cpu().numpy(), nb),
-                         tc.unpack(Fo.cpu().numpy(), nb), Vs, Vo, D, C)
+        Vs, Vo = (np.ascontiguousarray(tc.unpack(T.double().cpu().numpy(), nb).transpose(0, 2, 3, 1)) for T in (Vsd, Vod))
+    G2o, G3o = O.tc_fock(nup, ndown, A, dA, g1.weights_host, tc.unpack(Fs.double().cpu().numpy(), nb),
+                         tc.unpack(Fo.double().cpu().numpy(), nb), Vs, Vo, D, C)
     assert rel_err(G2, G2o) < 1e-12
     assert rel_err(G3, G3o) < 1e-12
     if not d_off:  # the explicit three-body form holds for D = C C^T
