@@ -701,7 +701,7 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 // ktile: the workgroup's K tile (LDS, [2][NC][MB][2][64] fragments); crd: a
 // two-slot LDS ring of r2 coordinates (32 points per step), filled two steps
 // ahead by direct-to-LDS loads of wave 0 so no register waits on them.
-template <int MB, int NPROD, int FOLD, class Kern>
+template <int MB, int NPROD, int FOLD, class Kern, int NB = 4>
 __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], float4 (*crd)[32],
                                           const double* __restrict__ p1,
                                           int64_t n1, const float4* __restrict__ x2, int64_t n2p,
@@ -710,7 +710,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
                                           int64_t npair, int64_t ld_row, double scale, Kern kern, int64_t row0,
                                           int64_t cbw) {
   constexpr int NC = Kern::ncomp;
-  constexpr int NB = 4;                   // 16-column blocks per wave
+  static_assert(NB == 2 || NB == 4, "16-column blocks per wave: 2 or 4");
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
   constexpr int VALS = MB * 64 * 8;       // K values per component per k-step
   constexpr int EPT = VALS / 256;         // per thread
@@ -948,16 +948,16 @@ constexpr int bf16_min_blocks() {
   return (Kern::ncomp == 1 || NPROD == 3 || Comp5<Kern>::value < Kern::ncomp) ? 2 : 1;
 }
 
-template <int MB, int NPROD, int FOLD, class Kern>
+template <int MB, int NPROD, int FOLD, class Kern, int NB>
 __global__ __launch_bounds__(256, (bf16_min_blocks<Kern, NPROD>())) void fgemm_bf16_kernel(
     const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
     const us8* __restrict__ P0, int64_t plane, int64_t ncb, void* __restrict__ Fout, void* __restrict__ Fout2,
     int out_f32, int64_t npair, double scale, Kern kern) {
   __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
   __shared__ float4 crd[2][32];
-  bf16_tile<MB, NPROD, FOLD, Kern>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32, npair,
-                                   Combines<Kern>::value ? npair : Kern::ncomp * npair, scale, kern,
-                                   (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * 16);
+  bf16_tile<MB, NPROD, FOLD, Kern, NB>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32, npair,
+                                       Combines<Kern>::value ? npair : Kern::ncomp * npair, scale, kern,
+                                       (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * (4 * NB));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1017,29 +1017,30 @@ int launch_f32(const double* p1, int64_t n1, const float4* x2, int64_t n2p, cons
   return 0;
 }
 
-template <int MB, int NPROD, int FOLD, class Kern>
+template <int MB, int NPROD, int FOLD, class Kern, int NB = 4>
 int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane, int64_t ncb,
                 void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st, void* F2) {
-  dim3 grid((unsigned)(ncb / 16), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
-  hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, Kern>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0, plane,
-                     ncb, F, F2, out_f32, npair, scale, kern);
+  dim3 grid((unsigned)(ncb / (4 * NB)), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
+  hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, Kern, NB>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0,
+                     plane, ncb, F, F2, out_f32, npair, scale, kern);
   TC_HIP(hipGetLastError());
   return 0;
 }
 
-// BF16 kernel variant: rows per workgroup 16*MB, FP64 fold every FOLD steps.
-// TCSCF_BF16_CFG="MB,FOLD" selects one for tuning.
+// BF16 kernel variant: rows per workgroup 16*MB, FP64 fold every FOLD steps,
+// NB 16-column blocks per wave.  TCSCF_BF16_CFG="MB,FOLD[,NB]" selects one for
+// tuning.
 struct Bf16Cfg {
-  int mb, fold;
+  int mb, fold, nb;
 };
 Bf16Cfg bf16_cfg(int ncomp) {
   static Bf16Cfg env = [] {
-    Bf16Cfg c{0, 0};
-    if (const char* e = std::getenv("TCSCF_BF16_CFG")) std::sscanf(e, "%d,%d", &c.mb, &c.fold);
+    Bf16Cfg c{0, 0, 4};
+    if (const char* e = std::getenv("TCSCF_BF16_CFG")) std::sscanf(e, "%d,%d,%d", &c.mb, &c.fold, &c.nb);
     return c;
   }();
-  Bf16Cfg c = env.mb ? env : Bf16Cfg{2, 1};
-  if (ncomp != 1) c.mb = 1;
+  Bf16Cfg c = env.mb ? env : Bf16Cfg{2, 1, 4};
+  if (ncomp != 1) c = Bf16Cfg{1, c.fold, 4};
   return c;
 }
 
@@ -1047,6 +1048,11 @@ template <int NPROD, int MB, class Kern>
 int launch_bf16_fold(Bf16Cfg c, const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0,
                      int64_t plane, int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern,
                      hipStream_t st, void* F2) {
+  if (MB == 4 && c.nb == 2) {
+    if (c.fold == 2)
+      return launch_bf16<MB, NPROD, 2, Kern, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+    return launch_bf16<MB, NPROD, 1, Kern, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+  }
   if (c.fold == 2)
     return launch_bf16<MB, NPROD, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
   return launch_bf16<MB, NPROD, 1>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
