@@ -174,3 +174,26 @@ def test_batch64_fixture_contraction(tc, gpu):
         worst = max(worst, err)
         assert err < 3e-6, (k, err)
     print(f"batch64: worst relative tensor error vs FP64 {worst:.2e}")
+
+
+def test_batch_accurate_policy_subset(tc, gpu):
+    """The batch's 'accurate' policy on a subset of the fixture (the three
+    lightest and the three heaviest instances): the light ones stay BF16x5 and
+    are within 1e-6 Eh of FP64, the heavy ones (Z >= FP64_FROM_Z) contract in
+    FP64 and equal the FP64 run bitwise."""
+    import torch
+    from tcscf_amd import batch
+    with open(os.path.join(ROOT, "tests", "golden", "batch64_instances.json")) as f:
+        instances = json.load(f)
+    by_z = sorted(instances, key=lambda x: (x["Z"], x["npts"]))
+    sub = by_z[:3] + by_z[-3:]
+    prep = batch.prepare(sub)
+    res_a, t_a = batch.run(prep, "bf16x5", synchronize=torch.cuda.synchronize, policy="accurate")
+    res_d, _ = batch.run(prep, "fp64", synchronize=torch.cuda.synchronize)
+    assert t_a["fp64_instances"] == sum(x["Z"] >= batch.FP64_FROM_Z for x in sub) == 3
+    for x, a, d in zip(sub, res_a, res_d):
+        assert a["status"] == d["status"] == 0
+        if x["Z"] >= batch.FP64_FROM_Z:
+            assert a["energy"] == d["energy"]
+        else:
+            assert abs(a["energy"] - d["energy"]) < 1e-6, (x["Z"], a["energy"] - d["energy"])
