@@ -30,7 +30,7 @@ PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x5": 5}
 # HBM bytes per launch of the contraction kernel on this workload, from rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --pmc-probe`, each scaled by the
 # reading for a 512 MiB copy (FETCH x2.0, WRITE x1.0); profiles/bf16_fgemm_ne30_traffic.md
-TRAFFIC_BYTES = {"bf16x5": 1.502e9 + 1.163e8, "fp64": 1.016e9 + 1.156e8}
+TRAFFIC_BYTES = {"bf16x5": 1.734e9 + 1.144e8, "fp64": 1.016e9 + 1.156e8}
 
 
 def parse():
@@ -197,9 +197,10 @@ def main():
         "block_errors_vs_fp64": blocks,
         "sweep_roofline": hbm_roofline(n * npair * (4 if fbytes32 else 8) + n * nb * 8,
                                        hf_mix.t_sweep_ms / max(1, hf_mix.numberOfConvergenceLoops()),
-                                       "hf_sweep_kernel: one read of the packed %s F and of A per iteration; time "
-                                       "per iteration as the SCF measures it (D upload, sweep, G download, sync)"
-                                       % ("FP32" if fbytes32 else "FP64")),
+                                       "%s: one read of the packed %s F and of A per iteration; time per "
+                                       "iteration as the SCF measures it (D upload, sweep, reduction, G readback)"
+                                       % ("hf_sweep_col_kernel" if nb <= 32 else "hf_sweep_kernel",
+                                          "FP32" if fbytes32 else "FP64")),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "tc_ne": tc_res,

@@ -405,7 +405,10 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
  * src/HartreeFock.cpp:1118-1224) from D_host (2 x nb x nb) and the occupied
  * orbitals C_host (2 x nb x max(nup, ndown), column b of spin s used for
  * b < nocc_s): writes G2 and G3 (2 x nb x nb each, host) before the mask.
- * V tensors as in tcscf_tc_scf. */
+ * V tensors as in tcscf_tc_scf.  When D is C C^T (as in every SCF
+ * iteration), nb <= 32 and the rows are FP32 of one V tensor, the sweep works
+ * from C alone; any other D takes the D-based sweep.  The two agree to FP64
+ * rounding. */
 int tcscf_tc_fock(int nup, int ndown, int nb, const double* A1_dev, const double* dA1_dev, const double* w1_dev,
                   int64_t n1, const void* Fs_dev, const void* Fo_dev, const void* Vs_dev, const void* Vo_dev,
                   int f_dtype, const tcscf_jastrow* jastrow, const double* D_host, const double* C_host,
