@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "device_util.hpp"
 
 namespace tc {
 
@@ -941,6 +942,270 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Software-pipelined BF16 tile (one kernel component, FP64 fold per step)   */
+/* ------------------------------------------------------------------------ */
+// The tile, products and rounding of bf16_tile for one kernel component, with
+// each 32-point step laid out so that its VALU work issues between MFMAs
+// instead of after them:
+//   - step s folds step s - 1's FP32 partials into the FP64 sums accumulator
+//     by accumulator, each fold just before that accumulator's fresh product
+//     (a register dependency on the fold's result keeps that order);
+//   - the K tile of step s + 1 is generated between step s's other products;
+//   - B fragments come through a raw buffer view whose scalar offset carries
+//     the step and the split plane, so no lane computes an address;
+//   - every wave loads and stores the 32 r2 coordinates of a step (the same
+//     values), so a step has no branch and is one scheduling region.
+// Per accumulator and element the product order (hh, hm, lh, hl, lm) and the
+// fold after every step are those of bf16_tile: the results are bitwise equal.
+template <int MB, int NPROD, class Kern, int NB, int NW>
+__device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (*crd)[32],
+                                               const double* __restrict__ p1, int64_t n1,
+                                               const float4* __restrict__ x2, int64_t n2p,
+                                               const us8* __restrict__ P0, int64_t plane, int64_t ncb,
+                                               void* __restrict__ Fout, int out_f32, int64_t npair, int64_t ld_row,
+                                               double scale, Kern kern, int64_t row0, int64_t cbw) {
+  static_assert(Kern::ncomp == 1 && !Combines<Kern>::value, "one kernel component");
+  constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
+  constexpr int VALS = MB * 64 * 8;     // K values per k-step
+  constexpr int EPT = VALS / (64 * NW);  // per thread
+  static_assert(EPT >= 2 && EPT <= 8 && VALS % (64 * NW) == 0 && EPT % 2 == 0, "K values per thread: 2 .. 8");
+  constexpr int NACC = MB * NB;
+  typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int64_t cb0 = cbw + wave * NB;
+
+  // this thread generates elements ge .. ge+EPT-1 of fragment lane gl of m-block gm
+  const int gi = tid * EPT;
+  const int gm = gi / 512;
+  const int gl = (gi / 8) & 63;
+  const int ge = gi & 7;
+  const int gk = 8 * (gl >> 4) + ge;
+  float ax, ay, az;
+  {
+    const int64_t grow = row0 + 16 * gm + (gl & 15);
+    const bool ok = grow < n1;
+    ax = ok ? (float)p1[grow] : (float)-kFar;
+    ay = ok ? (float)p1[n1 + grow] : (float)-kFar;
+    az = ok ? (float)p1[2 * n1 + grow] : (float)-kFar;
+  }
+
+  // B planes: lane and the wave's first column block in the vector offset,
+  // block n in the instruction offset (n * 1 KiB), step and plane in the
+  // scalar offset.  Every offset lies inside the planes by construction.
+  const __amdgpu_buffer_rsrc_t pv = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<us8*>(P0), (short)0, (int)((int64_t)NSPLIT * plane * 16), kBufferWord3);
+  const int voff = (int)(cb0 * 1024) + lane * 16;
+  const int plane_bytes = (int)(plane * 16);
+  auto load_b = [&](us8 (&dst)[NSPLIT][NB], int64_t s) {
+    const int so = __builtin_amdgcn_readfirstlane((int)(s * ncb * 1024));
+#pragma unroll
+    for (int t = 0; t < NSPLIT; ++t)
+#pragma unroll
+      for (int n = 0; n < NB; ++n)
+        dst[t][n] = __builtin_bit_cast(us8, __builtin_amdgcn_raw_buffer_load_b128(pv, voff + n * 1024,
+                                                                                   so + t * plane_bytes, 0));
+  };
+  // r2 coordinates: lanes l and l + 32 of every wave load point l of a step
+  const __amdgpu_buffer_rsrc_t cv =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(x2), (short)0, (int)(n2p * 16), kBufferWord3);
+  const int coff = (lane & 31) * 16;
+  u4 cnext;
+  auto fetch_coords = [&](int64_t t) {
+    cnext = __builtin_amdgcn_raw_buffer_load_b128(cv, coff, __builtin_amdgcn_readfirstlane((int)(t * 512)), 0);
+  };
+  auto store_coords = [&](int64_t t) { crd[t & 1][lane & 31] = __builtin_bit_cast(float4, cnext); };
+  // K = hi + lo (bf16, round to nearest even) at points qn[e], qn[e + 1]:
+  // the hi and lo words of the pair
+  auto gen_pair = [&](const float4 (&qn)[EPT], int e, unsigned& hw, unsigned& lw) {
+#pragma clang fp contract(off)
+    float v0[1], v1[1];
+    kernel_eval(kern, ax, ay, az, qn[e].x, qn[e].y, qn[e].z, v0);
+    kernel_eval(kern, ax, ay, az, qn[e + 1].x, qn[e + 1].y, qn[e + 1].z, v1);
+    bf2 h;
+    h[0] = (__bf16)v0[0];
+    h[1] = (__bf16)v1[0];
+    const unsigned hb = __builtin_bit_cast(unsigned, h);
+    bf2 l;
+    l[0] = (__bf16)(v0[0] - __uint_as_float(hb << 16));
+    l[1] = (__bf16)(v1[0] - __uint_as_float(hb & 0xffff0000u));
+    hw = hb;
+    lw = __builtin_bit_cast(unsigned, l);
+  };
+  auto store_k = [&](const unsigned (&hw)[EPT / 2], const unsigned (&lw)[EPT / 2], int buf) {
+    unsigned* dh = (unsigned*)((unsigned short*)&ktile[buf][gm][0][gl] + ge);
+    unsigned* dl = (unsigned*)((unsigned short*)&ktile[buf][gm][1][gl] + ge);
+#pragma unroll
+    for (int e = 0; e < EPT / 2; ++e) {
+      dh[e] = hw[e];
+      dl[e] = lw[e];
+    }
+  };
+  auto generate = [&](const float4 (&qn)[EPT], int buf) {
+    unsigned hw[EPT / 2], lw[EPT / 2];
+#pragma unroll
+    for (int e = 0; e < EPT; e += 2) gen_pair(qn, e, hw[e / 2], lw[e / 2]);
+    store_k(hw, lw, buf);
+  };
+
+  double sum[MB][NB][4];
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sum[m][n][i] = 0;
+  f4 acc[MB][NB];
+  const int64_t nsteps = n2p / 32, last = nsteps - 1;
+  us8 bqA[NSPLIT][NB], bqB[NSPLIT][NB];  // B fragments, ping-pong between steps
+  // prologue: coordinates of steps 0 and 1 in the ring, step 2's in cnext,
+  // K(0) in ktile[0], B(0) in bqA
+  fetch_coords(0);
+  store_coords(0);
+  fetch_coords(last < 1 ? last : 1);
+  store_coords(1);
+  fetch_coords(last < 2 ? last : 2);
+  __syncthreads();
+  {
+    float4 q0[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) q0[e] = crd[0][gk + e];
+    generate(q0, 0);
+  }
+  load_b(bqA, 0);
+  __syncthreads();
+
+  // zero accumulator operand of the fresh products after the first step: an
+  // empty asm statement redefines it after each accumulator's fold (it stays
+  // zero), a register dependency that keeps each fold before the product
+  // that overwrites its accumulator
+  f4 zc = {0, 0, 0, 0};
+  auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto first) {
+    constexpr bool FIRST = decltype(first)::value;
+    const int buf = (int)(s & 1);
+    // --- region 1: the ring store, the loads (coordinates first, so waiting
+    // for them one step later leaves the B loads in flight), the LDS reads
+    store_coords(s + 2);  // ring slot s & 1 was last read for K(s), before the previous barrier
+    fetch_coords(s + 3 <= last ? s + 3 : last);
+    __builtin_amdgcn_sched_barrier(0);
+    load_b(nxt, s + 1 <= last ? s + 1 : last);
+    bf8 ah[MB], al[MB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      ah[m] = __builtin_bit_cast(bf8, ktile[buf][m][0][lane]);
+      al[m] = __builtin_bit_cast(bf8, ktile[buf][m][1][lane]);
+    }
+    float4 qn[EPT];  // coordinates of step s + 1
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) qn[e] = crd[buf ^ 1][gk + e];
+    __builtin_amdgcn_sched_barrier(0);
+    // --- region 2: per accumulator, the fold of step s - 1's partial (8 VALU)
+    // and the fresh product
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        if constexpr (FIRST) {
+          const f4 zero = {0, 0, 0, 0};
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], __builtin_bit_cast(bf8, cur[0][n]), zero, 0, 0,
+                                                              0);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) sum[m][n][i] += (double)acc[m][n][i];
+          asm("; fold %1" : "+v"(zc) : "v"(sum[m][n][3]));
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], __builtin_bit_cast(bf8, cur[0][n]), zc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) {
+      if constexpr (!FIRST) __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // --- region 3: the other products with the K generation of step s + 1
+    // two VALU per MFMA, the K tile stores last
+    auto round = [&](const bf8 (&a)[MB], int sp) {
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], __builtin_bit_cast(bf8, cur[sp][n]), acc[m][n],
+                                                              0, 0, 0);
+    };
+    // the K pairs of step s + 1 are spread through the rounds in source
+    // order, which the group solver follows
+    unsigned hw[EPT / 2], lw[EPT / 2];
+    constexpr int NR = NPROD - 1;  // rounds after the fresh one
+    auto gen_after = [&](int rd) {
+#pragma unroll
+      for (int e = 0; e < EPT; e += 2)
+        if ((e / 2) * NR / (EPT / 2) == rd) gen_pair(qn, e, hw[e / 2], lw[e / 2]);
+    };
+    round(ah, 1);
+    gen_after(0);
+    round(al, 0);
+    gen_after(1);
+    if constexpr (NPROD == 5) {
+      round(ah, NSPLIT - 1);
+      gen_after(2);
+      round(al, 1);
+      gen_after(3);
+    }
+    store_k(hw, lw, buf ^ 1);
+#pragma unroll
+    for (int k = 0; k < (NPROD - 1) * NACC; ++k) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+  };
+  using Tt = std::integral_constant<bool, true>;
+  using Ff = std::integral_constant<bool, false>;
+  step(0, bqA, bqB, Tt{});
+  int64_t s = 1;
+  for (; s + 1 < nsteps; s += 2) {
+    step(s, bqB, bqA, Ff{});
+    step(s + 1, bqA, bqB, Ff{});
+  }
+  if (s < nsteps) step(s, bqB, bqA, Ff{});
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sum[m][n][i] += (double)acc[m][n][i];
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t row = row0 + 16 * m + d_row_bf16(g, i), col = 16 * (cb0 + n) + r;
+        if (row < n1 && col < npair) {
+          const int64_t o = row * ld_row + col;
+          const double v = scale * sum[m][n][i];
+          if (out_f32) ((float*)Fout)[o] = (float)v;
+          else ((double*)Fout)[o] = v;
+        }
+      }
+}
+
+// NW waves per workgroup (256 threads: two workgroups per CU; 512: one)
+template <int MB, int NPROD, class Kern, int NB, int NW>
+__global__ __launch_bounds__(64 * NW, 8 / NW) void fgemm_bf16_pipe_kernel(
+    const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
+    const us8* __restrict__ P0, int64_t plane, int64_t ncb, void* __restrict__ Fout, int out_f32, int64_t npair,
+    double scale, Kern kern) {
+  __shared__ us8 ktile[2][MB][2][64];
+  __shared__ float4 crd[2][32];
+  bf16_tile_pipe<MB, NPROD, Kern, NB, NW>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, out_f32, npair, npair,
+                                          scale, kern, (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * (NW * NB));
+}
+
 // Two workgroups per CU (a 256-VGPR budget) where the kernel fits it without
 // spills: one component, the three-product split, or a mixed split.
 template <class Kern, int NPROD>
@@ -1017,10 +1282,45 @@ int launch_f32(const double* p1, int64_t n1, const float4* x2, int64_t n2p, cons
   return 0;
 }
 
+// The software-pipelined kernel serves the Coulomb kernel at the default fold.
+// Shipped: 64 rows x 256 columns per 8-wave workgroup (each B fragment feeds
+// 64 rows, half the vector-memory traffic of 32-row tiles, with the K values
+// per thread of the 4-wave tile).  TCSCF_BF16_PIPE selects another layout for
+// A/B timing: 0 = bf16_tile, 1 = 32 x 256 per 4 waves, 3 = 64 x 128 per 4 waves.
+int bf16_pipe_mode() {
+  static const int mode = [] {
+    const char* e = std::getenv("TCSCF_BF16_PIPE");
+    return e ? std::atoi(e) : 2;
+  }();
+  return mode;
+}
+
+template <int MB, int NB, int NW, int NPROD, class Kern>
+int launch_bf16_pipe(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane,
+                     int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+  dim3 grid((unsigned)(ncb / (NW * NB)), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
+  hipLaunchKernelGGL((fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW>), grid, dim3(64 * NW), 0, st, p1, n1, x2, n2p,
+                     P0, plane, ncb, F, out_f32, npair, scale, kern);
+  TC_HIP(hipGetLastError());
+  return 0;
+}
+
 template <int MB, int NPROD, int FOLD, class Kern, int NB = 4>
 int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane, int64_t ncb,
                 void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st, void* F2) {
   dim3 grid((unsigned)(ncb / (4 * NB)), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
+  constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
+  if constexpr (std::is_same<Kern, KernCoulomb>::value && FOLD == 1 && MB == 2 && NB == 4) {
+    // the pipelined kernel's buffer offsets are 32-bit byte offsets
+    const int mode = bf16_pipe_mode();
+    if (mode && (int64_t)NSPLIT * plane * 16 < ((int64_t)1 << 31) && n2p * 16 < ((int64_t)1 << 31)) {
+      if (mode == 1)
+        return launch_bf16_pipe<2, 4, 4, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      if (mode == 3)
+        return launch_bf16_pipe<4, 2, 4, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+      return launch_bf16_pipe<4, 2, 8, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+    }
+  }
   hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, Kern, NB>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0,
                      plane, ncb, F, F2, out_f32, npair, scale, kern);
   TC_HIP(hipGetLastError());
