@@ -122,6 +122,9 @@ typedef struct tcscf_scf_result {
   int status;       /* TCSCF_OK or TCSCF_ENOTCONV / TCSCF_ESOLVER */
   double t_sweep_ms;  /* time in the per-iteration device sweeps */
   double t_solve_ms;  /* time in the FP64 eigen-solves and density updates */
+  int graph_iterations; /* iterations whose device build replayed a HIP graph (0: enqueued directly) */
+  int graph_captures;   /* captures of that graph during this call: 0 when a graph kept from an earlier
+                           call with the same inputs served, more than 1 when the kernel choice changed */
 } tcscf_scf_result;
 
 /* The solver objects' observable members after compute (host buffers, each

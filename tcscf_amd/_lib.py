@@ -98,7 +98,8 @@ class ScfObservables(ctypes.Structure):
 
 class ScfResult(ctypes.Structure):
     _fields_ = [("energy", ctypes.c_double), ("iterations", ctypes.c_int), ("status", ctypes.c_int),
-                ("t_sweep_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double)]
+                ("t_sweep_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double), ("graph_iterations", ctypes.c_int),
+                ("graph_captures", ctypes.c_int)]
 
 
 _lib = None

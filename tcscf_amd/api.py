@@ -849,6 +849,8 @@ class _HFBase:
         self.energy_trace = trace[:max(res.iterations, 0)].copy()
         self.t_sweep_ms = res.t_sweep_ms
         self.t_solve_ms = res.t_solve_ms
+        self.graph_iterations = res.graph_iterations  # iterations replayed from a HIP graph
+        self.graph_captures = res.graph_captures  # captures of that graph during this call
         check(rc)
         return res.energy
 

@@ -94,6 +94,7 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
     r.energy = std::numeric_limits<double>::max();
     r.iterations = 0;
     r.t_sweep_ms = r.t_solve_ms = 0;
+    r.graph_iterations = r.graph_captures = 0;
     const int nb = it.nb;
     const int64_t nb2 = (int64_t)nb * nb;
     if (!(it.S && it.H && it.A1_dev && it.w1_dev && it.n1 > 0 && nb > 0)) {
@@ -272,6 +273,8 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
     Instance& x = inst[k];
     results[k].t_sweep_ms = t_dev;
     results[k].t_solve_ms = t_host;
+    results[k].graph_iterations = x.it;
+    results[k].graph_captures = builds;
     if (x.item->D_out) {
       const std::vector<double>& D = x.tci ? x.tci->D : x.hfi->D;
       std::copy(D.begin(), D.end(), x.item->D_out);

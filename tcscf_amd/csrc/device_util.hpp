@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdint>
+
 namespace tc {
 
 // Raw buffer resources (gfx9 word 3: 32-bit data format, no swizzle): loads
@@ -37,5 +40,13 @@ template <>
 __device__ inline float buffer_load<float>(__amdgpu_buffer_rsrc_t r, int byte_offset) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_offset, 0, 0));
 }
+
+// dst[i] = src[i] for i < n: the upload of a graph-replayed SCF iteration,
+// src being mapped page-locked host memory (a kernel node, not a copy node).
+static __global__ __launch_bounds__(256) void copy_doubles(const double* __restrict__ src, double* __restrict__ dst,
+                                                           int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+inline dim3 copy_grid(int64_t n) { return dim3((unsigned)std::min<int64_t>(64, (n + 255) / 256)); }
 
 }  // namespace tc

@@ -8,7 +8,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <limits>
+#include <map>
 #include <vector>
 
 #include "common.hpp"
@@ -356,6 +358,17 @@ void density_from(int nb, int nocc, const std::vector<double>& C, double* D) {
 
 }  // namespace
 
+IterContext& iter_context(int kind) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, IterContext*> all;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  IterContext*& c = all[{dev, kind}];
+  if (!c) c = new IterContext();  // kept to the end of the process
+  return *c;
+}
+
 size_t hf_ws_doubles(int nspin, int nb) { return (size_t)nspin * nb * nb * (kSweepBlocks + 2); }
 
 int hf_prepare(HfPlan& p, int nspin, int nb, int64_t n1, const void* F, int f32, const double* A, const double* w,
@@ -403,9 +416,12 @@ int hf_prepare(HfPlan& p, int nspin, int nb, int64_t n1, const void* F, int f32,
   return 0;
 }
 
-int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st, double* G_mapped) {
+namespace {
+
+// The sweep over F and the partial-sum kernel writing G (nspin x nb x nb) to
+// G_dev, from the density already in p.dD.
+int hf_sweep_reduce(const HfPlan& p, hipStream_t st, double* G_dev) {
   const int64_t nb2 = (int64_t)p.nb * p.nb;
-  TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * p.nspin * nb2, hipMemcpyHostToDevice, st));
   const double cs = p.nspin == 1 ? 2.0 : 1.0, co = p.nspin == 1 ? 0.0 : 1.0;
   dim3 grid(p.blocks), block(256);
 #define TC_SWEEP(K, TF, NS, NBP)                                                                               \
@@ -432,10 +448,27 @@ int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_
   TC_HIP(hipGetLastError());
   const int64_t len = p.nspin * nb2;
   hipLaunchKernelGGL(reduce_partials, dim3((unsigned)((len + 63) / 64)), dim3(1024), 0, st, p.dPart, p.blocks, len,
-                     G_mapped ? G_mapped : p.dG);
+                     G_dev);
   TC_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st, double* G_mapped) {
+  const int64_t len = (int64_t)p.nspin * p.nb * p.nb;
+  TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * len, hipMemcpyHostToDevice, st));
+  int rc = hf_sweep_reduce(p, st, G_mapped ? G_mapped : p.dG);
+  if (rc) return rc;
   if (!G_mapped) TC_HIP(hipMemcpyAsync(G_host, p.dG, sizeof(double) * len, hipMemcpyDeviceToHost, st));
   return 0;
+}
+
+int hf_enqueue_mapped(const HfPlan& p, const double* D_mapped, double* G_mapped, hipStream_t st) {
+  const int64_t len = (int64_t)p.nspin * p.nb * p.nb;
+  hipLaunchKernelGGL(copy_doubles, copy_grid(len), dim3(256), 0, st, D_mapped, p.dD, len);
+  TC_HIP(hipGetLastError());
+  return hf_sweep_reduce(p, st, G_mapped);
 }
 
 HfIter::HfIter(int nspin_, const int* nocc_, int nb_, const double* S_, const double* H_, const tcscf_scf_opts* o_)
@@ -504,29 +537,54 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
   double* ws = (double*)scratch(kSlotSweep, sizeof(double) * hf_ws_doubles(nspin, nb), &rc);
   if (rc) return rc;
   HfPlan plan;
+  std::memset(&plan, 0, sizeof plan);  // its bytes key the iteration graph
   if ((rc = hf_prepare(plan, nspin, nb, n1, F, f_dtype == TCSCF_F32, A1, w1, ws))) return rc;
   HfIter iter(nspin, nocc, nb, S, H, o);
-  // D up through a page-locked buffer; G written by the partial-sum kernel
-  // straight into coherent page-locked memory the device maps (a copy from
-  // device memory into pageable vectors where either allocation fails)
-  Pinned pin((size_t)nspin * nb2);
-  Pinned gpin((size_t)nspin * nb2, hipHostMallocCoherent | hipHostMallocMapped);
+  // D up and G down through the device's coherent page-locked buffers the
+  // device maps (pageable vectors where that allocation failed).  Every
+  // iteration is enqueued on the caller's stream (D copy, sweep, partial sums
+  // writing G straight into the mapped buffer); with TCSCF_SCF_GRAPH=1 it
+  // replays one HIP graph instead (D upload kernel, sweep, partial sums).
+  IterContext& cx = iter_context(0);
+  std::lock_guard<std::mutex> lock(cx.mu);
+  const int64_t len = nspin * nb2;
+  if ((rc = cx.prepare(2 * (size_t)len))) return rc;
   std::vector<double> Dv, Gv;
-  if (!pin.p) Dv.resize(nspin * nb2);
-  if (!gpin.p) Gv.resize(nspin * nb2);
-  double* Dh = pin.p ? pin.p : Dv.data();
-  double* Gh = gpin.p ? gpin.p : Gv.data();
-  double* Gmap = nullptr;
-  if (gpin.p && hipHostGetDevicePointer((void**)&Gmap, gpin.p, 0) != hipSuccess) Gmap = nullptr;
+  double *Dh, *Gh, *Dmap = nullptr, *Gmap = nullptr;
+  if (cx.host) {
+    Dh = cx.host;
+    Gh = cx.host + len;
+    Dmap = cx.dev;
+    Gmap = cx.dev + len;
+  } else {
+    Dv.resize(len);
+    Gv.resize(len);
+    Dh = Dv.data();
+    Gh = Gv.data();
+  }
+  const bool use_graph = IterContext::enabled(0) && Dmap;
+  std::vector<unsigned char> key;
+  if (use_graph) {
+    key_append(key, plan);
+    key_append(key, Dmap);
+    TC_HIP(hipStreamSynchronize(st));  // F, A, w come from the caller's stream
+  }
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
   res->iterations = 0;
   res->t_sweep_ms = res->t_solve_ms = 0;
+  res->graph_iterations = res->graph_captures = 0;
   for (int it = 0; it < o->max_iter; ++it) {
     double t0 = now_ms();
     std::copy(iter.D.begin(), iter.D.end(), Dh);
-    if ((rc = hf_enqueue(plan, Dh, Gh, st, Gmap))) return rc;
-    TC_HIP(hipStreamSynchronize(st));
+    if (use_graph) {
+      if ((rc = cx.run(key, [&](hipStream_t s) { return hf_enqueue_mapped(plan, Dmap, Gmap, s); }))) return rc;
+      ++res->graph_iterations;
+      res->graph_captures = cx.captures;
+    } else {
+      if ((rc = hf_enqueue(plan, Dh, Gh, st, Gmap))) return rc;
+      TC_HIP(hipStreamSynchronize(st));
+    }
     double t1 = now_ms();
     res->t_sweep_ms += t1 - t0;
     bool done = false;

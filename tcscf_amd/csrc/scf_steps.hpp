@@ -8,7 +8,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -29,6 +31,99 @@ struct Pinned {
   Pinned(const Pinned&) = delete;
   Pinned& operator=(const Pinned&) = delete;
 };
+
+// The per-iteration device work of one driver kind (0: RCS / UOS, 1: TC) on
+// one device, as a HIP graph replayed every iteration.  Its resources are kept
+// across calls: the capture / replay stream, the coherent page-locked buffers
+// the device maps (inputs in, results out, moved by kernels: the graph holds
+// kernel nodes only), and the last graph with the bytes it was captured from
+// (the plan, the buffer addresses, the kernel choice).  A solve whose key
+// matches replays that graph without capturing; another key captures anew.
+// Created on first use and never destroyed (the runtime may be gone at exit).
+// The mutex serializes the drivers' calls on a device.  Without the graph the
+// drivers enqueue every iteration directly (the same kernels in the same
+// order: the same results).
+struct IterContext {
+  std::mutex mu;
+  hipStream_t cs = nullptr;
+  Pinned* buf = nullptr;  // coherent, mapped
+  double* host = nullptr;
+  double* dev = nullptr;
+  size_t cap = 0;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  std::vector<unsigned char> key;
+  int captures = 0;  // captures during the current call
+
+  // TCSCF_SCF_GRAPH=1: graphs for every driver, 0: none; unset: where the
+  // graph was measured faster than direct launches.  TC (Ne, nb = 30): 298
+  // against 310 us per iteration.  RCS / UOS, whose iteration is three
+  // kernels, run 7 us slower per iteration from a graph and enqueue directly.
+  static bool enabled(int kind) {
+    const char* e = std::getenv("TCSCF_SCF_GRAPH");
+    if (e && *e) return std::atoi(e) != 0;
+    return kind == 1;
+  }
+  // n doubles of mapped buffer (host / dev null when that fails); the stream.
+  int prepare(size_t n) {
+    captures = 0;
+    if (!cs) TC_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    if (n > cap) {
+      drop_graph();
+      delete buf;
+      host = dev = nullptr;
+      cap = 0;
+      buf = new Pinned(n, hipHostMallocCoherent | hipHostMallocMapped);
+      if (buf->p && hipHostGetDevicePointer((void**)&dev, buf->p, 0) == hipSuccess && dev) {
+        host = buf->p;
+        cap = n;
+      } else {
+        dev = nullptr;
+      }
+    }
+    return 0;
+  }
+  // One iteration: captures enqueue(stream) when the key differs from the
+  // last capture's, then replays the graph and waits for it.
+  template <class F>
+  int run(const std::vector<unsigned char>& k, F&& enqueue) {
+    if (!ex || k != key) {
+      drop_graph();
+      TC_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue(cs);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(cs, &graph);
+      if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        TC_HIP(e);
+      }
+      g = graph;
+      TC_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      key = k;
+      ++captures;
+    }
+    TC_HIP(hipGraphLaunch(ex, cs));
+    TC_HIP(hipStreamSynchronize(cs));
+    return 0;
+  }
+  void drop_graph() {
+    if (ex) (void)hipGraphExecDestroy(ex);
+    if (g) (void)hipGraphDestroy(g);
+    ex = nullptr;
+    g = nullptr;
+    key.clear();
+  }
+};
+// The context of driver kind 0 (RCS / UOS) or 1 (TC) on the current device.
+IterContext& iter_context(int kind);
+
+// The bytes of a trivially copyable value, appended to a graph key.
+template <class T>
+inline void key_append(std::vector<unsigned char>& k, const T& v) {
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(&v);
+  k.insert(k.end(), b, b + sizeof(T));
+}
 
 /* ---------------------------------------------------------------- RCS / UOS */
 struct HfPlan {
@@ -54,6 +149,9 @@ int hf_prepare(HfPlan& p, int nspin, int nb, int64_t n1, const void* F, int f32,
 // (the device address of G_host, coherent page-locked memory) the partial-sum
 // kernel writes G there directly and no copy follows.
 int hf_enqueue(const HfPlan& p, const double* D_host, double* G_host, hipStream_t st, double* G_mapped = nullptr);
+// The same build between mapped page-locked buffers (device addresses): D
+// uploaded by a kernel, G written by the partial-sum kernel; kernels only.
+int hf_enqueue_mapped(const HfPlan& p, const double* D_mapped, double* G_mapped, hipStream_t st);
 
 // Host half of one RCS / UOS iteration (src/HartreeFock.cpp:785-1003): mask,
 // Fock, energy from the current density, convergence test, optional DIIS,
@@ -113,6 +211,15 @@ bool tc_restricted(const TcPlan& p, const double* D, const double* C);
 // restricted (tc_restricted of D and C) forms the spin-0 blocks only.
 int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, bool restricted,
                hipStream_t st);
+// Whether the column point kernel serves D and C (host): nb, the storage and
+// the occupied columns allow it, and D = C C^T for every spin formed.
+bool tc_col_ok(const TcPlan& p, const double* D, const double* C, bool restricted);
+// The same build between mapped page-locked buffers (device addresses): D then
+// C (2 nb^2 + 2 nb nmax doubles) uploaded by a kernel, the result block
+// (tc_out_doubles) written by the reductions; kernels only.  col: tc_col_ok
+// of the uploaded D and C.
+int tc_enqueue_mapped(const TcPlan& p, const double* DC_mapped, double* out_mapped, bool restricted, bool col,
+                      hipStream_t st);
 // out -> G2, G3 (2 x nb x nb each, before the mask): G2 = scalar/2 + (vec/2 - vec^T/2),
 // G3 = scalar3/2 (src/HartreeFock.cpp:1209-1222); restricted copies spin 0 to spin 1.
 void tc_assemble(const TcPlan& p, const double* out, bool restricted, double* G2, double* G3);

@@ -1414,14 +1414,16 @@ static bool density_from_orbitals(const TcPlan& p, const double* D, const double
   return true;
 }
 
-int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, bool restricted,
-               hipStream_t st) {
+bool tc_col_ok(const TcPlan& p, const double* D, const double* C, bool restricted) {
+  return p.col[restricted ? 1 : 0] && density_from_orbitals(p, D, C, restricted);
+}
+
+// The build's kernels from D and C already in p.dD / p.dC; the result block
+// (tc_out_doubles) goes to out_dev.
+static int tc_build(const TcPlan& p, bool restricted, bool col, hipStream_t st, double* out_dev) {
   const int64_t nb2 = (int64_t)p.nb * p.nb;
   const int nst = restricted ? 3 : 6;  // rank targets (spin, term) formed
-  TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * 2 * nb2, hipMemcpyHostToDevice, st));
-  TC_HIP(hipMemcpyAsync(p.dC, C_host, sizeof(double) * 2 * p.nb * p.nmax, hipMemcpyHostToDevice, st));
   const int r = restricted ? 1 : 0;
-  const bool col = p.col[r] && density_from_orbitals(p, D_host, C_host, restricted);
   if (col) {
     // orbitals packed by column, then the column point kernel (its V-linear
     // sums included) over kBlocks workgroups of 4 waves
@@ -1471,14 +1473,34 @@ int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, doub
 #undef TC_VLIN
     TC_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(tc_reduce_ranges, dim3((unsigned)(nst * nb2)), dim3(256), 0, st, p.dPG, rr, nb2, p.dOut);
+  hipLaunchKernelGGL(tc_reduce_ranges, dim3((unsigned)(nst * nb2)), dim3(256), 0, st, p.dPG, rr, nb2, out_dev);
   TC_HIP(hipGetLastError());
   hipLaunchKernelGGL(tc_reduce, dim3((unsigned)(nk * p.npair)), dim3(256), 0, st, p.dPV, kBlocks, nk * p.npair,
-                     p.dOut + 6 * nb2);
+                     out_dev + 6 * nb2);
   TC_HIP(hipGetLastError());
+  return 0;
+}
+
+int tc_enqueue(const TcPlan& p, const double* D_host, const double* C_host, double* out_host, bool restricted,
+               hipStream_t st) {
+  const int64_t nb2 = (int64_t)p.nb * p.nb;
+  TC_HIP(hipMemcpyAsync(p.dD, D_host, sizeof(double) * 2 * nb2, hipMemcpyHostToDevice, st));
+  TC_HIP(hipMemcpyAsync(p.dC, C_host, sizeof(double) * 2 * p.nb * p.nmax, hipMemcpyHostToDevice, st));
+  const int rc = tc_build(p, restricted, tc_col_ok(p, D_host, C_host, restricted), st, p.dOut);
+  if (rc) return rc;
   TC_HIP(hipMemcpyAsync(out_host, p.dOut, sizeof(double) * tc_out_doubles(p.nb), hipMemcpyDeviceToHost, st));
   return 0;
 }
+
+int tc_enqueue_mapped(const TcPlan& p, const double* DC_mapped, double* out_mapped, bool restricted, bool col,
+                      hipStream_t st) {
+  // p.dC follows p.dD in the workspace, as C follows D in the mapped buffer
+  const int64_t n = 2 * (int64_t)p.nb * p.nb + 2 * (int64_t)p.nb * p.nmax;
+  hipLaunchKernelGGL(copy_doubles, copy_grid(n), dim3(256), 0, st, DC_mapped, p.dD, n);
+  TC_HIP(hipGetLastError());
+  return tc_build(p, restricted, col, st, out_mapped);
+}
+
 
 void tc_assemble(const TcPlan& p, const double* out, bool restricted, double* G2, double* G3) {
   const int nb = p.nb;
@@ -1658,28 +1680,54 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
   double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1, Vo != nullptr), &rc);
   if (rc) return rc;
   TcPlan plan;
+  std::memset(&plan, 0, sizeof plan);  // its bytes key the iteration graph
   if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, Vs, Vo, f_dtype, jp, ws))) return rc;
   TcIter iter(nup, ndown, nb, S, H, o);
   std::vector<double> G2(2 * nb2), G3(2 * nb2);
-  // D and C up, the result block down, through page-locked buffers (pageable
-  // vectors if that allocation fails)
+  // D and C up, the result block down, through the device's coherent
+  // page-locked buffer the device maps: each iteration replays one HIP graph
+  // (upload kernel, the build's kernels, reductions writing into the mapped
+  // buffer), captured again when the kernel choice changes (restricted build,
+  // column kernel).  Where the mapped buffer cannot be had, or with
+  // TCSCF_SCF_GRAPH=0, every iteration is enqueued on the caller's stream with
+  // copies (pageable vectors if the page-locked allocation failed).
   const size_t nD = iter.D.size(), nC = iter.C.size(), nO = tc_out_doubles(nb);
-  Pinned pin(nD + nC + nO);
+  IterContext& cx = iter_context(1);
+  std::lock_guard<std::mutex> lock(cx.mu);
+  if ((rc = cx.prepare(nD + nC + nO))) return rc;
   std::vector<double> stage;
-  if (!pin.p) stage.resize(nD + nC + nO);
-  double* Dh = pin.p ? pin.p : stage.data();
+  if (!cx.host) stage.resize(nD + nC + nO);
+  double* Dh = cx.host ? cx.host : stage.data();
   double *Ch = Dh + nD, *outh = Ch + nC;
+  const bool use_graph = IterContext::enabled(1) && cx.dev;
+  if (use_graph) TC_HIP(hipStreamSynchronize(st));  // the tensors come from the caller's stream
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();
   res->iterations = 0;
   res->t_sweep_ms = res->t_solve_ms = 0;
+  res->graph_iterations = res->graph_captures = 0;
   for (int it = 0; it < o->max_iter; ++it) {
     double t0 = now_ms();
     const bool restricted = tc_restricted(plan, iter.D.data(), iter.C.data());
     std::copy(iter.D.begin(), iter.D.end(), Dh);
     std::copy(iter.C.begin(), iter.C.end(), Ch);
-    if ((rc = tc_enqueue(plan, Dh, Ch, outh, restricted, st))) return rc;
-    TC_HIP(hipStreamSynchronize(st));
+    if (use_graph) {
+      const bool col = tc_col_ok(plan, Dh, Ch, restricted);
+      std::vector<unsigned char> key;
+      key_append(key, plan);
+      key_append(key, cx.dev);
+      key_append(key, (int)restricted);
+      key_append(key, (int)col);
+      if ((rc = cx.run(key, [&](hipStream_t s) {
+             return tc_enqueue_mapped(plan, cx.dev, cx.dev + nD + nC, restricted, col, s);
+           })))
+        return rc;
+      ++res->graph_iterations;
+      res->graph_captures = cx.captures;
+    } else {
+      if ((rc = tc_enqueue(plan, Dh, Ch, outh, restricted, st))) return rc;
+      TC_HIP(hipStreamSynchronize(st));
+    }
     tc_assemble(plan, outh, restricted, G2.data(), G3.data());
     double t1 = now_ms();
     res->t_sweep_ms += t1 - t0;

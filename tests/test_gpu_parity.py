@@ -421,6 +421,65 @@ def test_tc_scf_vs_oracle(tc, O, gpu, Z, nup, ndown, mgs_mode):
             tc.TCHartreeFock(nup, ndown, nb).compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-10, mgs_mode=0, diis=6)
 
 
+def test_scf_iteration_graph_matches_direct_enqueue(tc, gpu, monkeypatch):
+    """Each SCF iteration's device build replays one HIP graph (kernel nodes
+    only: the D / C upload from mapped page-locked memory, the sweeps, the
+    reductions writing into mapped memory); TCSCF_SCF_GRAPH=0 enqueues the same
+    kernels directly.  Both give bitwise-equal traces, densities and energies
+    for RCS, UOS, restricted TC on FP32 tensors (column kernel), open-shell TC
+    and TC with independent V_same / V_oppo.  Every iteration replays the
+    graph; a solve captures it at most a few times (once per kernel choice),
+    and the same solve again replays the graph kept from the first without a
+    capture."""
+    from tcscf_amd import configs
+
+    def both(run):
+        out = {}
+        for mode in ("1", "0", "1"):  # graphs for every driver, none, every driver again
+            monkeypatch.setenv("TCSCF_SCF_GRAPH", mode)
+            out[mode + str(len(out))] = run()
+        monkeypatch.delenv("TCSCF_SCF_GRAPH")
+        g1, d0, g2 = out["10"], out["01"], out["12"]
+        for a in (g1, g2):
+            assert a[0] == d0[0] and np.array_equal(a[1], d0[1]) and np.array_equal(a[2], d0[2])
+        its = len(d0[1])
+        assert g1[3] == its and 1 <= g1[4] <= 3, g1[3:]  # every iteration replayed; few captures
+        assert d0[3] == 0 and d0[4] == 0, d0[3:]
+        assert g2[3] == its and g2[4] == 0, g2[3:]  # the kept graph served the repeat
+
+    def obs(hf, e):
+        return e, hf.energy_trace, hf.density.copy(), hf.graph_iterations, hf.graph_captures
+
+    c = configs.he_config1()
+    g, S, H = _scf_inputs(tc, c)
+    F = tc.computeF(g, g)
+    both(lambda: (lambda hf: obs(hf, hf.compute(S, H, g, F, False, tol=1e-10)))(tc.RCSHartreeFock(2, 5)))
+    basis = tc.even_tempered({0: 6, 1: 2}, 0.5, 2.0)
+    g2 = _grid(tc, 14, 11, tc.RADIAL_M4)
+    g2.setBasisFunctions(basis, False)
+    S2, H2 = tc.one_electron_matrices(basis, 5)
+    F2 = tc.computeF(g2, g2, precision="bf16x5", out_dtype="fp32")
+    both(lambda: (lambda hf: obs(hf, hf.compute(S2, H2, g2, F2, True, tol=1e-10, diis=4)))(
+        tc.UOSHartreeFock(3, 2, len(basis))))
+    tb = tc.even_tempered({0: 5, 1: 2}, 0.6, 2.2)
+    nb = len(tb)
+    g1, g3, jas, Fs, Fo, V = _tc_case(tc, tb, (14, 11, tc.RADIAL_M4), (10, 7, tc.RADIAL_M4))
+    Fs32, Fo32, V32 = Fs.float(), Fo.float(), V.float()
+    for (nup, ndown, Z, tensors) in ((2, 2, 4, (Fs32, Fo32, V32, jas)), (3, 2, 5, (Fs, Fo, V, jas)),
+                                     (2, 2, 4, (Fs, Fo, jas.c_same * V, jas.c_oppo * V))):
+        St, Ht = tc.one_electron_matrices(tb, Z)
+        both(lambda: (lambda hf: obs(hf, hf.compute(St, Ht, g1, *tensors, True, tol=1e-10, max_iter=100,
+                                                    mgs_mode=1, diis=6)))(tc.TCHartreeFock(nup, ndown, nb)))
+    # default: the TC driver replays its graph, RCS / UOS enqueue directly
+    monkeypatch.delenv("TCSCF_SCF_GRAPH", raising=False)
+    hf = tc.TCHartreeFock(2, 2, nb)
+    hf.compute(St, Ht, g1, Fs32, Fo32, V32, jas, True, tol=1e-10, max_iter=100, mgs_mode=1, diis=6)
+    assert hf.graph_iterations == hf.numberOfConvergenceLoops() > 0
+    hr = tc.RCSHartreeFock(2, 5)
+    hr.compute(S, H, g, F, False, tol=1e-10)
+    assert hr.graph_iterations == 0
+
+
 _OCHI_TERMS = [(1, 0, 0), (2, 1, 0), (0, 2, 1), (1, 1, 1)]
 
 
