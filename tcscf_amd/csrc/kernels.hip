@@ -943,34 +943,40 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
 }
 
 /* ------------------------------------------------------------------------ */
-/* Software-pipelined BF16 tile (one kernel component, FP64 fold per step)   */
+/* Software-pipelined BF16 tile (FP64 fold after every step)                 */
 /* ------------------------------------------------------------------------ */
-// The tile, products and rounding of bf16_tile for one kernel component, with
-// each 32-point step laid out so that its VALU work issues between MFMAs
-// instead of after them:
+// The tile, products and rounding of bf16_tile, with each 32-point step laid
+// out so that its VALU work issues between MFMAs instead of after them:
 //   - step s folds step s - 1's FP32 partials into the FP64 sums accumulator
 //     by accumulator, each fold just before that accumulator's fresh product
 //     (a register dependency on the fold's result keeps that order);
-//   - the K tile of step s + 1 is generated between step s's other products;
+//   - the K tiles of step s + 1 are generated between step s's other
+//     products;
 //   - B fragments come through a raw buffer view whose scalar offset carries
 //     the step and the split plane, so no lane computes an address;
 //   - every wave loads and stores the 32 r2 coordinates of a step (the same
 //     values), so a step has no branch and is one scheduling region.
-// Per accumulator and element the product order (hh, hm, lh, hl, lm) and the
-// fold after every step are those of bf16_tile: the results are bitwise equal.
+// Workgroup: NW waves over 16 MB rows; wave w owns the 16-column blocks
+// cbw + w NB .. + NB - 1 of the pairs.  Per accumulator and element the
+// product order (hh, hm, lh, then hl, lm for the components that take five
+// products) and the fold after every step are those of bf16_tile: the results
+// are bitwise equal.
 template <int MB, int NPROD, class Kern, int NB, int NW>
-__device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (*crd)[32],
+__device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][64], float4 (*crd)[32],
                                                const double* __restrict__ p1, int64_t n1,
                                                const float4* __restrict__ x2, int64_t n2p,
                                                const us8* __restrict__ P0, int64_t plane, int64_t ncb,
-                                               void* __restrict__ Fout, int out_f32, int64_t npair, int64_t ld_row,
-                                               double scale, Kern kern, int64_t row0, int64_t cbw) {
-  static_assert(Kern::ncomp == 1 && !Combines<Kern>::value, "one kernel component");
+                                               void* __restrict__ Fout, void* __restrict__ Fout2, int out_f32,
+                                               int64_t npair, int64_t ld_row, double scale, Kern kern, int64_t row0,
+                                               int64_t cbw) {
+  constexpr int NC = Kern::ncomp;
+  constexpr int C5 = NPROD == 5 ? Comp5<Kern>::value : 0;  // components with five products
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
-  constexpr int VALS = MB * 64 * 8;     // K values per k-step
+  constexpr int VALS = MB * 64 * 8;     // K values per component per k-step
   constexpr int EPT = VALS / (64 * NW);  // per thread
   static_assert(EPT >= 2 && EPT <= 8 && VALS % (64 * NW) == 0 && EPT % 2 == 0, "K values per thread: 2 .. 8");
-  constexpr int NACC = MB * NB;
+  constexpr int NACC = NC * MB * NB;
+  constexpr int NROUND = NC * 2 + C5 * 2;  // product rounds after the fresh one
   typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
   typedef unsigned u4 __attribute__((ext_vector_type(4)));
   const int tid = threadIdx.x;
@@ -1018,47 +1024,49 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (
     cnext = __builtin_amdgcn_raw_buffer_load_b128(cv, coff, __builtin_amdgcn_readfirstlane((int)(t * 512)), 0);
   };
   auto store_coords = [&](int64_t t) { crd[t & 1][lane & 31] = __builtin_bit_cast(float4, cnext); };
-  // K = hi + lo (bf16, round to nearest even) at points qn[e], qn[e + 1]:
-  // the hi and lo words of the pair
-  auto gen_pair = [&](const float4 (&qn)[EPT], int e, unsigned& hw, unsigned& lw) {
+  // K = hi + lo (bf16, round to nearest even) of every component at points
+  // qn[e], qn[e + 1]: the hi and lo words of the pair
+  auto gen_pair = [&](const float4 (&qn)[EPT], int e, unsigned (&hw)[NC], unsigned (&lw)[NC]) {
 #pragma clang fp contract(off)
-    float v0[1], v1[1];
+    float v0[NC], v1[NC];
     kernel_eval(kern, ax, ay, az, qn[e].x, qn[e].y, qn[e].z, v0);
     kernel_eval(kern, ax, ay, az, qn[e + 1].x, qn[e + 1].y, qn[e + 1].z, v1);
-    bf2 h;
-    h[0] = (__bf16)v0[0];
-    h[1] = (__bf16)v1[0];
-    const unsigned hb = __builtin_bit_cast(unsigned, h);
-    bf2 l;
-    l[0] = (__bf16)(v0[0] - __uint_as_float(hb << 16));
-    l[1] = (__bf16)(v1[0] - __uint_as_float(hb & 0xffff0000u));
-    hw = hb;
-    lw = __builtin_bit_cast(unsigned, l);
-  };
-  auto store_k = [&](const unsigned (&hw)[EPT / 2], const unsigned (&lw)[EPT / 2], int buf) {
-    unsigned* dh = (unsigned*)((unsigned short*)&ktile[buf][gm][0][gl] + ge);
-    unsigned* dl = (unsigned*)((unsigned short*)&ktile[buf][gm][1][gl] + ge);
 #pragma unroll
-    for (int e = 0; e < EPT / 2; ++e) {
-      dh[e] = hw[e];
-      dl[e] = lw[e];
+    for (int c = 0; c < NC; ++c) {
+      bf2 h;
+      h[0] = (__bf16)v0[c];
+      h[1] = (__bf16)v1[c];
+      const unsigned hb = __builtin_bit_cast(unsigned, h);
+      bf2 l;
+      l[0] = (__bf16)(v0[c] - __uint_as_float(hb << 16));
+      l[1] = (__bf16)(v1[c] - __uint_as_float(hb & 0xffff0000u));
+      hw[c] = hb;
+      lw[c] = __builtin_bit_cast(unsigned, l);
     }
   };
-  auto generate = [&](const float4 (&qn)[EPT], int buf) {
-    unsigned hw[EPT / 2], lw[EPT / 2];
+  auto store_k = [&](const unsigned (&hw)[EPT / 2][NC], const unsigned (&lw)[EPT / 2][NC], int buf) {
 #pragma unroll
-    for (int e = 0; e < EPT; e += 2) gen_pair(qn, e, hw[e / 2], lw[e / 2]);
-    store_k(hw, lw, buf);
+    for (int c = 0; c < NC; ++c) {
+      unsigned* dh = (unsigned*)((unsigned short*)&ktile[buf][c][gm][0][gl] + ge);
+      unsigned* dl = (unsigned*)((unsigned short*)&ktile[buf][c][gm][1][gl] + ge);
+#pragma unroll
+      for (int e = 0; e < EPT / 2; ++e) {
+        dh[e] = hw[e][c];
+        dl[e] = lw[e][c];
+      }
+    }
   };
 
-  double sum[MB][NB][4];
+  double sum[NC][MB][NB][4];
 #pragma unroll
-  for (int m = 0; m < MB; ++m)
+  for (int c = 0; c < NC; ++c)
 #pragma unroll
-    for (int n = 0; n < NB; ++n)
+    for (int m = 0; m < MB; ++m)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) sum[m][n][i] = 0;
-  f4 acc[MB][NB];
+      for (int n = 0; n < NB; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sum[c][m][n][i] = 0;
+  f4 acc[NC][MB][NB];
   const int64_t nsteps = n2p / 32, last = nsteps - 1;
   us8 bqA[NSPLIT][NB], bqB[NSPLIT][NB];  // B fragments, ping-pong between steps
   // prologue: coordinates of steps 0 and 1 in the ring, step 2's in cnext,
@@ -1073,7 +1081,10 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (
     float4 q0[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) q0[e] = crd[0][gk + e];
-    generate(q0, 0);
+    unsigned hw[EPT / 2][NC], lw[EPT / 2][NC];
+#pragma unroll
+    for (int e = 0; e < EPT; e += 2) gen_pair(q0, e, hw[e / 2], lw[e / 2]);
+    store_k(hw, lw, 0);
   }
   load_b(bqA, 0);
   __syncthreads();
@@ -1092,12 +1103,14 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (
     fetch_coords(s + 3 <= last ? s + 3 : last);
     __builtin_amdgcn_sched_barrier(0);
     load_b(nxt, s + 1 <= last ? s + 1 : last);
-    bf8 ah[MB], al[MB];
+    bf8 ah[NC][MB], al[NC][MB];
 #pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      ah[m] = __builtin_bit_cast(bf8, ktile[buf][m][0][lane]);
-      al[m] = __builtin_bit_cast(bf8, ktile[buf][m][1][lane]);
-    }
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        ah[c][m] = __builtin_bit_cast(bf8, ktile[buf][c][m][0][lane]);
+        al[c][m] = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
+      }
     float4 qn[EPT];  // coordinates of step s + 1
 #pragma unroll
     for (int e = 0; e < EPT; ++e) qn[e] = crd[buf ^ 1][gk + e];
@@ -1105,58 +1118,63 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (
     // --- region 2: per accumulator, the fold of step s - 1's partial (8 VALU)
     // and the fresh product
 #pragma unroll
-    for (int m = 0; m < MB; ++m)
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int n = 0; n < NB; ++n) {
-        if constexpr (FIRST) {
-          const f4 zero = {0, 0, 0, 0};
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], __builtin_bit_cast(bf8, cur[0][n]), zero, 0, 0,
-                                                              0);
-        } else {
+      for (int m = 0; m < MB; ++m)
 #pragma unroll
-          for (int i = 0; i < 4; ++i) sum[m][n][i] += (double)acc[m][n][i];
-          asm("; fold %1" : "+v"(zc) : "v"(sum[m][n][3]));
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], __builtin_bit_cast(bf8, cur[0][n]), zc, 0, 0, 0);
+        for (int n = 0; n < NB; ++n) {
+          if constexpr (FIRST) {
+            const f4 zero = {0, 0, 0, 0};
+            acc[c][m][n] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c][m], __builtin_bit_cast(bf8, cur[0][n]), zero, 0, 0, 0);
+          } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
+            asm("; fold %1" : "+v"(zc) : "v"(sum[c][m][n][3]));
+            acc[c][m][n] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c][m], __builtin_bit_cast(bf8, cur[0][n]), zc, 0, 0, 0);
+          }
         }
-      }
 #pragma unroll
     for (int k = 0; k < NACC; ++k) {
       if constexpr (!FIRST) __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
     }
     __builtin_amdgcn_sched_barrier(0);
-    // --- region 3: the other products with the K generation of step s + 1
-    // two VALU per MFMA, the K tile stores last
-    auto round = [&](const bf8 (&a)[MB], int sp) {
+    // --- region 3: the other products, component by component, with the K
+    // pairs of step s + 1 spread through them in source order (which the
+    // group solver follows) two VALU per MFMA, the K tile stores last
+    auto round = [&](int c, const bf8 (&a)[MB], int sp) {
 #pragma unroll
       for (int m = 0; m < MB; ++m)
 #pragma unroll
         for (int n = 0; n < NB; ++n)
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], __builtin_bit_cast(bf8, cur[sp][n]), acc[m][n],
-                                                              0, 0, 0);
+          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], __builtin_bit_cast(bf8, cur[sp][n]),
+                                                                 acc[c][m][n], 0, 0, 0);
     };
-    // the K pairs of step s + 1 are spread through the rounds in source
-    // order, which the group solver follows
-    unsigned hw[EPT / 2], lw[EPT / 2];
-    constexpr int NR = NPROD - 1;  // rounds after the fresh one
+    unsigned hw[EPT / 2][NC], lw[EPT / 2][NC];
     auto gen_after = [&](int rd) {
 #pragma unroll
       for (int e = 0; e < EPT; e += 2)
-        if ((e / 2) * NR / (EPT / 2) == rd) gen_pair(qn, e, hw[e / 2], lw[e / 2]);
+        if ((e / 2) * NROUND / (EPT / 2) == rd) gen_pair(qn, e, hw[e / 2], lw[e / 2]);
     };
-    round(ah, 1);
-    gen_after(0);
-    round(al, 0);
-    gen_after(1);
-    if constexpr (NPROD == 5) {
-      round(ah, NSPLIT - 1);
-      gen_after(2);
-      round(al, 1);
-      gen_after(3);
+    int rd = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      round(c, ah[c], 1);
+      gen_after(rd++);
+      round(c, al[c], 0);
+      gen_after(rd++);
+      if (c < C5) {  // folded at compile time (c is unrolled)
+        round(c, ah[c], NSPLIT - 1);
+        gen_after(rd++);
+        round(c, al[c], 1);
+        gen_after(rd++);
+      }
     }
     store_k(hw, lw, buf ^ 1);
 #pragma unroll
-    for (int k = 0; k < (NPROD - 1) * NACC; ++k) {
+    for (int k = 0; k < NROUND * MB * NB; ++k) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
       __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
     }
@@ -1173,11 +1191,13 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (
   }
   if (s < nsteps) step(s, bqB, bqA, Ff{});
 #pragma unroll
-  for (int m = 0; m < MB; ++m)
+  for (int c = 0; c < NC; ++c)
 #pragma unroll
-    for (int n = 0; n < NB; ++n)
+    for (int m = 0; m < MB; ++m)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) sum[m][n][i] += (double)acc[m][n][i];
+      for (int n = 0; n < NB; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
 #pragma unroll
   for (int m = 0; m < MB; ++m)
 #pragma unroll
@@ -1186,10 +1206,30 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (
       for (int i = 0; i < 4; ++i) {
         const int64_t row = row0 + 16 * m + d_row_bf16(g, i), col = 16 * (cb0 + n) + r;
         if (row < n1 && col < npair) {
-          const int64_t o = row * ld_row + col;
-          const double v = scale * sum[m][n][i];
-          if (out_f32) ((float*)Fout)[o] = (float)v;
-          else ((double*)Fout)[o] = v;
+          if constexpr (Combines<Kern>::value) {
+            // outputs o = 0, 1 of kern.finish to Fout / Fout2, row-major n1 x npair each
+            static_assert(Kern::nout == 2, "two outputs");
+            double v[NC], o[2];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] = scale * sum[c][m][n][i];
+            kern.finish(v, o);
+            const int64_t e = row * ld_row + col;
+            if (out_f32) {
+              ((float*)Fout)[e] = (float)o[0];
+              ((float*)Fout2)[e] = (float)o[1];
+            } else {
+              ((double*)Fout)[e] = o[0];
+              ((double*)Fout2)[e] = o[1];
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              const int64_t o = row * ld_row + c * npair + col;
+              const double v = scale * sum[c][m][n][i];
+              if (out_f32) ((float*)Fout)[o] = (float)v;
+              else ((double*)Fout)[o] = v;
+            }
+          }
         }
       }
 }
@@ -1198,12 +1238,13 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[MB][2][64], float4 (
 template <int MB, int NPROD, class Kern, int NB, int NW>
 __global__ __launch_bounds__(64 * NW, 8 / NW) void fgemm_bf16_pipe_kernel(
     const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
-    const us8* __restrict__ P0, int64_t plane, int64_t ncb, void* __restrict__ Fout, int out_f32, int64_t npair,
-    double scale, Kern kern) {
-  __shared__ us8 ktile[2][MB][2][64];
+    const us8* __restrict__ P0, int64_t plane, int64_t ncb, void* __restrict__ Fout, void* __restrict__ Fout2,
+    int out_f32, int64_t npair, double scale, Kern kern) {
+  __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
   __shared__ float4 crd[2][32];
-  bf16_tile_pipe<MB, NPROD, Kern, NB, NW>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, out_f32, npair, npair,
-                                          scale, kern, (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * (NW * NB));
+  bf16_tile_pipe<MB, NPROD, Kern, NB, NW>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32, npair,
+                                          Combines<Kern>::value ? npair : Kern::ncomp * npair, scale, kern,
+                                          (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * (NW * NB));
 }
 
 // Two workgroups per CU (a 256-VGPR budget) where the kernel fits it without
@@ -1282,43 +1323,64 @@ int launch_f32(const double* p1, int64_t n1, const float4* x2, int64_t n2p, cons
   return 0;
 }
 
-// The software-pipelined kernel serves the Coulomb kernel at the default fold.
-// Shipped: 64 rows x 256 columns per 8-wave workgroup (each B fragment feeds
-// 64 rows, half the vector-memory traffic of 32-row tiles, with the K values
-// per thread of the 4-wave tile).  TCSCF_BF16_PIPE selects another layout for
-// A/B timing: 0 = bf16_tile, 1 = 32 x 256 per 4 waves, 3 = 64 x 128 per 4 waves.
-int bf16_pipe_mode() {
-  static const int mode = [] {
+// Layout of the software-pipelined kernel per kernel functor (TCSCF_BF16_PIPE
+// selects another for A/B timing; 0 = bf16_tile).  One component (the
+// Coulomb kernel): 64 rows x 256 columns per 8-wave workgroup (mode 2, each B
+// fragment feeds 64 rows: half the vector-memory traffic of 32-row tiles at
+// the K values per thread of the 4-wave tile; Ne 5.07 -> 4.53 ms); 1 = 32 x
+// 256 per 4 waves, 3 = 64 x 128 per 4 waves.  Two components (the F pair):
+// 32 x 256 per 8 waves (mode 1; TC-Ne 9.45 -> 7.80 ms), 2 = 64 x 128 per 8
+// waves.  Three (V): bf16_tile, which measured faster (3.27 ms) than the one
+// spill-free pipelined layout, 32 x 128 per 8 waves (3.95 ms, mode 1).
+// Kernels whose functor evaluates behind a branch keep bf16_tile.
+int bf16_pipe_mode(int ncomp) {
+  static const int env = [] {
     const char* e = std::getenv("TCSCF_BF16_PIPE");
-    return e ? std::atoi(e) : 2;
+    return e ? std::atoi(e) : -1;
   }();
-  return mode;
+  if (env >= 0) return env;
+  return ncomp == 1 ? 2 : (ncomp == 2 ? 1 : 0);
 }
 
 template <int MB, int NB, int NW, int NPROD, class Kern>
 int launch_bf16_pipe(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane,
-                     int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st) {
+                     int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st,
+                     void* F2) {
   dim3 grid((unsigned)(ncb / (NW * NB)), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
   hipLaunchKernelGGL((fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW>), grid, dim3(64 * NW), 0, st, p1, n1, x2, n2p,
-                     P0, plane, ncb, F, out_f32, npair, scale, kern);
+                     P0, plane, ncb, F, F2, out_f32, npair, scale, kern);
   TC_HIP(hipGetLastError());
   return 0;
 }
+
+template <class Kern>
+struct PipeServed : std::integral_constant<bool, std::is_same<Kern, KernCoulomb>::value ||
+                                                     std::is_same<Kern, KernTCPair>::value ||
+                                                     std::is_same<Kern, KernJastrowGrad>::value> {};
 
 template <int MB, int NPROD, int FOLD, class Kern, int NB = 4>
 int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane, int64_t ncb,
                 void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st, void* F2) {
   dim3 grid((unsigned)(ncb / (4 * NB)), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
-  if constexpr (std::is_same<Kern, KernCoulomb>::value && FOLD == 1 && MB == 2 && NB == 4) {
+  constexpr int NC = Kern::ncomp;
+  if constexpr (PipeServed<Kern>::value && FOLD == 1 && MB == (NC == 1 ? 2 : 1) && NB == 4) {
     // the pipelined kernel's buffer offsets are 32-bit byte offsets
-    const int mode = bf16_pipe_mode();
+    const int mode = bf16_pipe_mode(NC);
     if (mode && (int64_t)NSPLIT * plane * 16 < ((int64_t)1 << 31) && n2p * 16 < ((int64_t)1 << 31)) {
-      if (mode == 1)
-        return launch_bf16_pipe<2, 4, 4, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
-      if (mode == 3)
-        return launch_bf16_pipe<4, 2, 4, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
-      return launch_bf16_pipe<4, 2, 8, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st);
+#define TC_PIPE(M, B, W) \
+  return launch_bf16_pipe<M, B, W, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2)
+      if constexpr (NC == 1) {
+        if (mode == 1) TC_PIPE(2, 4, 4);
+        if (mode == 3) TC_PIPE(4, 2, 4);
+        TC_PIPE(4, 2, 8);
+      } else if constexpr (NC == 2) {
+        if (mode == 2) TC_PIPE(4, 1, 8);
+        TC_PIPE(2, 2, 8);
+      } else {
+        TC_PIPE(2, 1, 8);
+      }
+#undef TC_PIPE
     }
   }
   hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, Kern, NB>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0,
