@@ -961,7 +961,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
 // product order (hh, hm, lh, then hl, lm for the components that take five
 // products) and the fold after every step are those of bf16_tile: the results
 // are bitwise equal.
-template <int MB, int NPROD, class Kern, int NB, int NW>
+template <int MB, int NPROD, class Kern, int NB, int NW, int FOLD = 1, bool REV = false>
 __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][64], float4 (*crd)[32],
                                                const double* __restrict__ p1, int64_t n1,
                                                const float4* __restrict__ x2, int64_t n2p,
@@ -1089,13 +1089,28 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
   load_b(bqA, 0);
   __syncthreads();
 
+  // Product j of a component with np products: the K part (0 hi, 1 lo) and
+  // the P plane.  The base order is hh, hm, lh, hl, lm; REV issues it
+  // backwards, the small corrections first into a fresh partial and hh last,
+  // so the corrections round at their own magnitude.
+  auto kpart = [](int np, int j) {
+    const int b = REV ? np - 1 - j : j;
+    return (b == 2 || b == 4) ? 1 : 0;
+  };
+  auto plane_of = [](int np, int j) {
+    const int b = REV ? np - 1 - j : j;
+    return b == 0 || b == 2 ? 0 : (b == 3 ? NSPLIT - 1 : 1);
+  };
   // zero accumulator operand of the fresh products after the first step: an
   // empty asm statement redefines it after each accumulator's fold (it stays
   // zero), a register dependency that keeps each fold before the product
   // that overwrites its accumulator
   f4 zc = {0, 0, 0, 0};
-  auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto first) {
-    constexpr bool FIRST = decltype(first)::value;
+  // FRESH: the step starts a new FP32 partial (every FOLD steps); FOLDP: the
+  // previous partial is folded first (a fresh step other than the first)
+  auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto fresh, auto foldp) {
+    constexpr bool FRESH = decltype(fresh)::value;
+    constexpr bool FOLDP = decltype(foldp)::value;
     const int buf = (int)(s & 1);
     // --- region 1: the ring store, the loads (coordinates first, so waiting
     // for them one step later leaves the B loads in flight), the LDS reads
@@ -1103,55 +1118,51 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
     fetch_coords(s + 3 <= last ? s + 3 : last);
     __builtin_amdgcn_sched_barrier(0);
     load_b(nxt, s + 1 <= last ? s + 1 : last);
-    bf8 ah[NC][MB], al[NC][MB];
+    bf8 ak[NC][2][MB];  // K hi / lo fragments
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int m = 0; m < MB; ++m) {
-        ah[c][m] = __builtin_bit_cast(bf8, ktile[buf][c][m][0][lane]);
-        al[c][m] = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
+        ak[c][0][m] = __builtin_bit_cast(bf8, ktile[buf][c][m][0][lane]);
+        ak[c][1][m] = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
       }
     float4 qn[EPT];  // coordinates of step s + 1
 #pragma unroll
     for (int e = 0; e < EPT; ++e) qn[e] = crd[buf ^ 1][gk + e];
     __builtin_amdgcn_sched_barrier(0);
-    // --- region 2: per accumulator, the fold of step s - 1's partial (8 VALU)
-    // and the fresh product
+    // --- region 2: per accumulator the first product of the step, after the
+    // fold of the previous partial (8 VALU) in a step that starts a new one
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
+    for (int c = 0; c < NC; ++c) {
+      const int np = c < C5 ? 5 : (NPROD == 5 ? 3 : NPROD);
+      const int kp = kpart(np, 0), sp = plane_of(np, 0);
 #pragma unroll
       for (int m = 0; m < MB; ++m)
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
-          if constexpr (FIRST) {
+          const bf8 b = __builtin_bit_cast(bf8, cur[sp][n]);
+          if constexpr (!FRESH) {
+            acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ak[c][kp][m], b, acc[c][m][n], 0, 0, 0);
+          } else if constexpr (!FOLDP) {
             const f4 zero = {0, 0, 0, 0};
-            acc[c][m][n] =
-                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c][m], __builtin_bit_cast(bf8, cur[0][n]), zero, 0, 0, 0);
+            acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ak[c][kp][m], b, zero, 0, 0, 0);
           } else {
 #pragma unroll
             for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
             asm("; fold %1" : "+v"(zc) : "v"(sum[c][m][n][3]));
-            acc[c][m][n] =
-                __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c][m], __builtin_bit_cast(bf8, cur[0][n]), zc, 0, 0, 0);
+            acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ak[c][kp][m], b, zc, 0, 0, 0);
           }
         }
+    }
 #pragma unroll
     for (int k = 0; k < NACC; ++k) {
-      if constexpr (!FIRST) __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
+      if constexpr (FOLDP) __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
     }
     __builtin_amdgcn_sched_barrier(0);
     // --- region 3: the other products, component by component, with the K
     // pairs of step s + 1 spread through them in source order (which the
     // group solver follows) two VALU per MFMA, the K tile stores last
-    auto round = [&](int c, const bf8 (&a)[MB], int sp) {
-#pragma unroll
-      for (int m = 0; m < MB; ++m)
-#pragma unroll
-        for (int n = 0; n < NB; ++n)
-          acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], __builtin_bit_cast(bf8, cur[sp][n]),
-                                                                 acc[c][m][n], 0, 0, 0);
-    };
     unsigned hw[EPT / 2][NC], lw[EPT / 2][NC];
     auto gen_after = [&](int rd) {
 #pragma unroll
@@ -1161,16 +1172,19 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
     int rd = 0;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      round(c, ah[c], 1);
-      gen_after(rd++);
-      round(c, al[c], 0);
-      gen_after(rd++);
-      if (c < C5) {  // folded at compile time (c is unrolled)
-        round(c, ah[c], NSPLIT - 1);
-        gen_after(rd++);
-        round(c, al[c], 1);
-        gen_after(rd++);
-      }
+      const int np = c < C5 ? 5 : (NPROD == 5 ? 3 : NPROD);
+#pragma unroll
+      for (int j = 1; j < 5; ++j)
+        if (j < np) {  // folded at compile time (c and j are unrolled)
+          const int kp = kpart(np, j), sp = plane_of(np, j);
+#pragma unroll
+          for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int n = 0; n < NB; ++n)
+              acc[c][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  ak[c][kp][m], __builtin_bit_cast(bf8, cur[sp][n]), acc[c][m][n], 0, 0, 0);
+          gen_after(rd++);
+        }
     }
     store_k(hw, lw, buf ^ 1);
 #pragma unroll
@@ -1183,13 +1197,14 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
   };
   using Tt = std::integral_constant<bool, true>;
   using Ff = std::integral_constant<bool, false>;
-  step(0, bqA, bqB, Tt{});
+  using Odd = std::integral_constant<bool, FOLD == 1>;  // odd steps start a partial only at FOLD 1
+  step(0, bqA, bqB, Tt{}, Ff{});
   int64_t s = 1;
   for (; s + 1 < nsteps; s += 2) {
-    step(s, bqB, bqA, Ff{});
-    step(s + 1, bqA, bqB, Ff{});
+    step(s, bqB, bqA, Odd{}, Odd{});
+    step(s + 1, bqA, bqB, Tt{}, Tt{});
   }
-  if (s < nsteps) step(s, bqB, bqA, Ff{});
+  if (s < nsteps) step(s, bqB, bqA, Odd{}, Odd{});
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -1235,14 +1250,14 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
 }
 
 // NW waves per workgroup (256 threads: two workgroups per CU; 512: one)
-template <int MB, int NPROD, class Kern, int NB, int NW>
+template <int MB, int NPROD, class Kern, int NB, int NW, int FOLD = 1, bool REV = false>
 __global__ __launch_bounds__(64 * NW, 8 / NW) void fgemm_bf16_pipe_kernel(
     const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
     const us8* __restrict__ P0, int64_t plane, int64_t ncb, void* __restrict__ Fout, void* __restrict__ Fout2,
     int out_f32, int64_t npair, double scale, Kern kern) {
   __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
   __shared__ float4 crd[2][32];
-  bf16_tile_pipe<MB, NPROD, Kern, NB, NW>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32, npair,
+  bf16_tile_pipe<MB, NPROD, Kern, NB, NW, FOLD, REV>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32, npair,
                                           Combines<Kern>::value ? npair : Kern::ncomp * npair, scale, kern,
                                           (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * (NW * NB));
 }
@@ -1342,15 +1357,25 @@ int bf16_pipe_mode(int ncomp) {
   return ncomp == 1 ? 2 : (ncomp == 2 ? 1 : 0);
 }
 
-template <int MB, int NB, int NW, int NPROD, class Kern>
+template <int MB, int NB, int NW, int NPROD, class Kern, int FOLD = 1, bool REV = false>
 int launch_bf16_pipe(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane,
                      int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st,
                      void* F2) {
   dim3 grid((unsigned)(ncb / (NW * NB)), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
-  hipLaunchKernelGGL((fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW>), grid, dim3(64 * NW), 0, st, p1, n1, x2, n2p,
-                     P0, plane, ncb, F, F2, out_f32, npair, scale, kern);
+  hipLaunchKernelGGL((fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW, FOLD, REV>), grid, dim3(64 * NW), 0, st, p1, n1,
+                     x2, n2p, P0, plane, ncb, F, F2, out_f32, npair, scale, kern);
   TC_HIP(hipGetLastError());
   return 0;
+}
+// Accumulation scheme of the Coulomb kernel (A/B): 0 = FP64 fold every step,
+// products hh first; 1 = every step, corrections first; 2 = every second step,
+// corrections first; 3 = every second step, hh first.
+int bf16_scheme() {
+  static const int v = [] {
+    const char* e = std::getenv("TCSCF_BF16_SCHEME");
+    return e ? std::atoi(e) : 0;
+  }();
+  return v;
 }
 
 template <class Kern>
@@ -1373,6 +1398,16 @@ int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, con
       if constexpr (NC == 1) {
         if (mode == 1) TC_PIPE(2, 4, 4);
         if (mode == 3) TC_PIPE(4, 2, 4);
+        const int sch = bf16_scheme();
+        if (sch == 1)
+          return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 1, true>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
+                                                                 scale, kern, st, F2);
+        if (sch == 2)
+          return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 2, true>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
+                                                                 scale, kern, st, F2);
+        if (sch == 3)
+          return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 2, false>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
+                                                                  scale, kern, st, F2);
         TC_PIPE(4, 2, 8);
       } else if constexpr (NC == 2) {
         if (mode == 2) TC_PIPE(4, 1, 8);
