@@ -30,7 +30,7 @@ PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x5": 5}
 # HBM bytes per launch of the contraction kernel on this workload, from rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --pmc-probe`, each scaled by the
 # reading for a 512 MiB copy (FETCH x2.0, WRITE x1.0); profiles/bf16_fgemm_ne30_traffic.md
-TRAFFIC_BYTES = {"bf16x5": 1.734e9 + 1.144e8, "fp64": 1.016e9 + 1.156e8}
+TRAFFIC_BYTES = {"bf16x5": 1.502e9 + 1.140e8, "fp64": 1.016e9 + 1.156e8}
 
 
 def parse():
@@ -332,6 +332,8 @@ def run_tc(a, tc, configs, torch):
                      "precompute_ms": float(np.mean([r["pre"] for r in runs])),
                      "scf_ms": float(np.mean([r["scf"] for r in runs])),
                      "F_ms": F_ms, "V_ms": V_ms, "iterations": hf.numberOfConvergenceLoops(), "fock_build_ms": fb_ms,
+                     "scf_iteration_graph": {"iterations_replayed": hf.graph_iterations,
+                                             "captures_in_last_solve": hf.graph_captures},
                      "precompute_roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": peak,
                                              "unit": "TFLOP/s", "frac": round(ach / peak, 4),
                                              "algorithmic_flop": fl_F + fl_V,
@@ -413,16 +415,24 @@ def run_batch(a, tc, configs, torch):
 
 def pmc_probe(tc, cfg, torch):
     """Kernels to read hardware counters on: the contraction at both
-    precisions, one RHF solve (the HBM-bound sweep) and a plain 512 MiB copy of
-    known traffic (read 512 MiB, write 512 MiB) to scale FETCH_SIZE/WRITE_SIZE."""
+    precisions, one RHF solve (the HBM-bound sweep), the TC-Ne mixed
+    contractions (F pair at BF16x5 / BF16x3, V at BF16x3, FP32 storage) and a
+    plain 512 MiB copy of known traffic (read 512 MiB, write 512 MiB) to scale
+    FETCH_SIZE/WRITE_SIZE."""
     g = tc.QMCGrid.radial_lebedev(*cfg["grid"], 1.0, 1.0)
-    g.setBasisFunctions(cfg["basis"], False)
+    g.setBasisFunctions(cfg["basis"], True)
     F = tc.computeF(g, g, precision="bf16x5")
     torch.cuda.synchronize()
     F64 = tc.computeF(g, g, precision="fp64")
     torch.cuda.synchronize()
     S, H = tc.one_electron_matrices(cfg["basis"], cfg["Z"])
     e = tc.RCSHartreeFock(10, len(cfg["basis"])).compute(S, H, g, F, True, tol=1e-9)
+    g3 = tc.QMCGrid.radial_lebedev(50, 23, cfg["grid"][2], 1.0, 1.0)
+    g3.setBasisFunctions(cfg["basis"], True)
+    jas = tc.Jastrow.from_b(1.0)
+    tc.computeF_tc_pair(g, g, jas, precision="bf16x5", smooth_precision="bf16x3", out_dtype="fp32")
+    tc.computeV(g, g3, jas, precision="bf16x3", out_dtype="fp32")
+    torch.cuda.synchronize()
     x = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda").fill_(1.0)
     y = torch.empty_like(x)
     y.copy_(x)
