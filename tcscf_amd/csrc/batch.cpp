@@ -41,6 +41,7 @@ struct Instance {
   int it = 0;
   bool active = true;
   bool restricted = false;  // TC: closed shell with identical spins, as captured in the graph
+  bool col = false;         // TC: the column point kernel, as captured in the graph
   int rc = 0;
   std::string err;  // the message of a failed step (set on the worker thread that ran it)
 };
@@ -188,15 +189,29 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
     for (int k = 0; k < count; ++k)
       if (inst[k].active) active.push_back(k);
     if (active.empty()) break;
-    // TC instances whose spins became (or stopped being) identical change kernels
+    // TC instances whose spins became (or stopped being) identical, or whose
+    // density stopped (or started) being C C^T, change kernels
     bool recapture = active != built;
     for (int k : active) {
       Instance& x = inst[k];
       if (!x.tci) continue;
       const bool r = tc_restricted(x.tcp, x.tci->D.data(), x.tci->C.data());
-      if (r != x.restricted) {
+      const bool c = tc_col_ok(x.tcp, x.tci->D.data(), x.tci->C.data(), r);
+      if (r != x.restricted || c != x.col) {
         x.restricted = r;
+        x.col = c;
         recapture = true;
+      }
+    }
+    // current densities (and TC orbitals) into the pinned upload buffers, before
+    // a capture: tc_enqueue reads them there to choose the point kernel
+    for (int k : active) {
+      Instance& x = inst[k];
+      if (x.tci) {
+        std::memcpy(x.up->p, x.tci->D.data(), x.tci->D.size() * sizeof(double));
+        std::memcpy(x.up->p + x.tci->D.size(), x.tci->C.data(), x.tci->C.size() * sizeof(double));
+      } else {
+        std::memcpy(x.up->p, x.hfi->D.data(), x.hfi->D.size() * sizeof(double));
       }
     }
     if (recapture) {  // capture the Fock builds of the active set
@@ -228,16 +243,6 @@ extern "C" int tcscf_batch_scf(const tcscf_scf_item* items, int count, const tcs
       TC_BHIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
       built = active;
       ++builds;
-    }
-    // current densities (and TC orbitals) into the pinned upload buffers
-    for (int k : active) {
-      Instance& x = inst[k];
-      if (x.tci) {
-        std::memcpy(x.up->p, x.tci->D.data(), x.tci->D.size() * sizeof(double));
-        std::memcpy(x.up->p + x.tci->D.size(), x.tci->C.data(), x.tci->C.size() * sizeof(double));
-      } else {
-        std::memcpy(x.up->p, x.hfi->D.data(), x.hfi->D.size() * sizeof(double));
-      }
     }
     const double t0 = wall_ms();
     TC_BHIP(hipGraphLaunch(exec, cs));
