@@ -80,7 +80,8 @@ enum ScratchSlot {
   kSlotSmall = 4,
   kSlotBatchMeta = 5,  // + 0 / 1: batch descriptors for the two tile heights
   kSlotQmc = 7,        // tcscf_eri_qmc6d: the per-point pair densities / r12 and the contraction
-  kNumSlots = 8
+  kSlotBatchMetaV = 8, // batch descriptors of the three-component (V) tiles
+  kNumSlots = 9
 };
 
 }  // namespace tc

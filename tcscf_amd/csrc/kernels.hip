@@ -2194,6 +2194,39 @@ __global__ __launch_bounds__(256, 2) void bf16_batch_kernel(const BatchDesc* __r
   }
 }
 
+// The V tensors of a batch: one tile is (instance, 16-row block, 256-column
+// block) with all three components of the Jastrow gradient, which share r12,
+// its rsqrt, the gradient scale and the B fragments -- the tile of the
+// single-instance V kernel, so results match it bit for bit.
+template <int NPROD>
+__global__ __launch_bounds__(256, (bf16_min_blocks<KernJastrowGrad, NPROD>())) void bf16_batch_v_kernel(
+    const BatchDesc* __restrict__ desc, const int64_t* __restrict__ start, int ndesc, int64_t t_end,
+    unsigned long long* __restrict__ counter) {
+  __shared__ us8 ktile[2][3][1][2][64];
+  __shared__ float4 crd[2][32];
+  __shared__ int64_t cur;
+  for (;;) {
+    if (threadIdx.x == 0) cur = (int64_t)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const int64_t t = cur;
+    __syncthreads();  // every thread has read cur before it is replaced
+    if (t >= t_end) break;
+    int lo = 0, hi = ndesc - 1;  // last instance with start <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchDesc& d = desc[lo];
+    const int64_t local = t - start[lo];
+    const int64_t cb = local % d.cblocks, rb = local / d.cblocks;
+    KernJastrowGrad kern;
+    kern.a12 = d.a12;
+    bf16_tile<1, NPROD, 1, KernJastrowGrad, 4>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, d.F, nullptr,
+                                               d.out_f32, d.npair, 3 * d.npair, d.scale, kern, rb * 16, cb * 16);
+  }
+}
+
 __global__ void zero_counter(unsigned long long* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
 }
@@ -2235,7 +2268,7 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
   int rc = 0;
   char* ws = (char*)scratch(kSlotPairs, off[count] + 256, &rc);
   if (rc) return rc;
-  unsigned long long* counter = (unsigned long long*)(ws + off[count]);
+  unsigned long long* counter = (unsigned long long*)(ws + off[count]);  // [0] F queue, [1] V queue
   std::vector<BatchDesc> desc(count);
   std::vector<int> order(count);
   std::iota(order.begin(), order.end(), 0);
@@ -2276,50 +2309,78 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
     const double c = it.same_spin ? it.jastrow.c_same : it.jastrow.c_oppo;
     d.c2a2 = 2 * c * c * it.jastrow.a12 * it.jastrow.a12;
   }
-  // largest r2 first, so the tail of the queue holds the short tiles
+  // largest r2 first, so the tail of a queue holds the short tiles; the V
+  // tensors (three components per tile) form a queue of their own
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return desc[a].n2p > desc[b].n2p; });
-  std::vector<BatchDesc> sorted(count);
-  for (int k = 0; k < count; ++k) sorted[k] = desc[order[k]];
+  std::vector<BatchDesc> sorted, vdesc;
+  for (int k = 0; k < count; ++k) (desc[order[k]].ncomp == 3 ? vdesc : sorted).push_back(desc[order[k]]);
+  const int nf = (int)sorted.size(), nv = (int)vdesc.size();
 
   int dev = 0, ncu = 256;
   TC_HIP(hipGetDevice(&dev));
   TC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  // descriptors for one row-block height (16 MB rows); uploaded behind the counter
-  auto upload = [&](int mb, int64_t* total, const BatchDesc** d_desc, const int64_t** d_start) -> int {
-    std::vector<int64_t> start(count);
+  // descriptors of a queue for one row-block height (16 MB rows; per tile ncomp
+  // components, or all of them in one tile for the V queue), in a scratch slot
+  auto upload_list = [&](std::vector<BatchDesc>& list, int mb, bool all_comp, int slot, int64_t* total,
+                         const BatchDesc** d_desc, const int64_t** d_start) -> int {
+    const int n = (int)list.size();
+    std::vector<int64_t> start(n);
     int64_t t = 0;
-    for (int k = 0; k < count; ++k) {
-      sorted[k].rblocks = (sorted[k].n1 + 16 * mb - 1) / (16 * mb);
-      sorted[k].tiles = sorted[k].rblocks * sorted[k].cblocks * sorted[k].ncomp;
+    for (int k = 0; k < n; ++k) {
+      list[k].rblocks = (list[k].n1 + 16 * mb - 1) / (16 * mb);
+      list[k].tiles = list[k].rblocks * list[k].cblocks * (all_comp ? 1 : list[k].ncomp);
       start[k] = t;
-      t += sorted[k].tiles;
+      t += list[k].tiles;
     }
     *total = t;
     int r = 0;
-    char* meta = (char*)scratch(kSlotBatchMeta + (mb - 1), sizeof(BatchDesc) * count + sizeof(int64_t) * count + 256, &r);
+    char* meta = (char*)scratch(slot, sizeof(BatchDesc) * n + sizeof(int64_t) * n + 256, &r);
     if (r) return r;
-    TC_HIP(hipMemcpyAsync(meta, sorted.data(), sizeof(BatchDesc) * count, hipMemcpyHostToDevice, st));
-    TC_HIP(hipMemcpyAsync(meta + round_up(sizeof(BatchDesc) * count, 256), start.data(), sizeof(int64_t) * count,
+    TC_HIP(hipMemcpyAsync(meta, list.data(), sizeof(BatchDesc) * n, hipMemcpyHostToDevice, st));
+    TC_HIP(hipMemcpyAsync(meta + round_up(sizeof(BatchDesc) * n, 256), start.data(), sizeof(int64_t) * n,
                           hipMemcpyHostToDevice, st));
     TC_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
     *d_desc = (const BatchDesc*)meta;
-    *d_start = (const int64_t*)(meta + round_up(sizeof(BatchDesc) * count, 256));
+    *d_start = (const int64_t*)(meta + round_up(sizeof(BatchDesc) * n, 256));
     return 0;
+  };
+  auto upload = [&](int mb, int64_t* total, const BatchDesc** d_desc, const int64_t** d_start) -> int {
+    return upload_list(sorted, mb, false, kSlotBatchMeta + (mb - 1), total, d_desc, d_start);
   };
   auto launch = [&](int mb, const BatchDesc* dd, const int64_t* ds, int64_t t0, int64_t t1) -> int {
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, st, counter);
     TC_HIP(hipGetLastError());
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(t1 - t0, (int64_t)ncu * 2));
     if (nsplit == 3) {
-      if (mb == 1) hipLaunchKernelGGL((bf16_batch_kernel<1, 5>), dim3(grid), dim3(256), 0, st, dd, ds, count, t0, t1, counter);
-      else hipLaunchKernelGGL((bf16_batch_kernel<2, 5>), dim3(grid), dim3(256), 0, st, dd, ds, count, t0, t1, counter);
+      if (mb == 1) hipLaunchKernelGGL((bf16_batch_kernel<1, 5>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter);
+      else hipLaunchKernelGGL((bf16_batch_kernel<2, 5>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter);
     } else {
-      if (mb == 1) hipLaunchKernelGGL((bf16_batch_kernel<1, 3>), dim3(grid), dim3(256), 0, st, dd, ds, count, t0, t1, counter);
-      else hipLaunchKernelGGL((bf16_batch_kernel<2, 3>), dim3(grid), dim3(256), 0, st, dd, ds, count, t0, t1, counter);
+      if (mb == 1) hipLaunchKernelGGL((bf16_batch_kernel<1, 3>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter);
+      else hipLaunchKernelGGL((bf16_batch_kernel<2, 3>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter);
     }
     TC_HIP(hipGetLastError());
     return 0;
   };
+  // the V queue: one persistent launch over its three-component tiles
+  if (nv) {
+    int64_t vtotal;
+    const BatchDesc* vd;
+    const int64_t* vs;
+    if ((rc = upload_list(vdesc, 1, true, kSlotBatchMetaV, &vtotal, &vd, &vs))) return rc;
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, st, counter + 1);
+    TC_HIP(hipGetLastError());
+    const int per_cu = nsplit == 3 ? bf16_min_blocks<KernJastrowGrad, 5>() : bf16_min_blocks<KernJastrowGrad, 3>();
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(vtotal, (int64_t)ncu * per_cu));
+    if (nsplit == 3)
+      hipLaunchKernelGGL((bf16_batch_v_kernel<5>), dim3(grid), dim3(256), 0, st, vd, vs, nv, vtotal, counter + 1);
+    else
+      hipLaunchKernelGGL((bf16_batch_v_kernel<3>), dim3(grid), dim3(256), 0, st, vd, vs, nv, vtotal, counter + 1);
+    TC_HIP(hipGetLastError());
+  }
+  if (!nf) {
+    TC_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
 
   // Autotuner: the row-block height (16 or 32 rows per tile) is timed on the
   // head of the queue (the largest tiles) the first time a precision is seen
