@@ -114,6 +114,9 @@ typedef struct tcscf_scf_opts {
                      last k Fock operators (FP64, host), from the second iteration on.
                      No reference counterpart (the benchmark workloads ask for it);
                      TC needs mgs_mode = 1 with it. */
+  int graph;      /* per-iteration HIP graph: 0 the driver's default (TC: a graph; RCS / UOS:
+                     direct launches; TCSCF_SCF_GRAPH=1 / 0 overrides it), 1 a graph, 2 direct
+                     launches (e.g. for solves running concurrently on several streams) */
 } tcscf_scf_opts;
 
 typedef struct tcscf_scf_result {

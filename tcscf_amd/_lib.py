@@ -79,7 +79,7 @@ class BatchItem(ctypes.Structure):
 
 class ScfOpts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("mask", ctypes.c_int),
-                ("mgs_mode", ctypes.c_int), ("diis", ctypes.c_int)]
+                ("mgs_mode", ctypes.c_int), ("diis", ctypes.c_int), ("graph", ctypes.c_int)]
 
 
 class ScfItem(ctypes.Structure):

@@ -826,8 +826,11 @@ def eri_full(E, nb):
 
 
 # --------------------------------------------------------------------------- SCF
-def _opts(tol, max_iter, mask, mgs_mode=0, diis=0):
-    return _lib.ScfOpts(float(tol), int(max_iter), int(bool(mask)), int(mgs_mode), int(diis))
+def _opts(tol, max_iter, mask, mgs_mode=0, diis=0, graph=None):
+    """graph: None the driver's default, True a per-iteration HIP graph, False
+    direct launches (tcscf_scf_opts.graph 0 / 1 / 2)."""
+    g = 0 if graph is None else (1 if graph else 2)
+    return _lib.ScfOpts(float(tol), int(max_iter), int(bool(mask)), int(mgs_mode), int(diis), g)
 
 
 def _f_args(r1Grid, F):
@@ -873,7 +876,7 @@ class RCSHartreeFock(_HFBase):
         return False
 
     def compute(self, overlap, oneElectronTerms, r1Grid, Fji, respectOneElectronSymmetry=True, tol=1e-5,
-                max_iter=50, diis=0):
+                max_iter=50, diis=0, graph=None):
         nb = self.basisSize
         S, Sp = _np(overlap)
         H, Hp = _np(oneElectronTerms)
@@ -887,7 +890,7 @@ class RCSHartreeFock(_HFBase):
         obs = _observables(self.fockOperator, self.eigenvectors, None)
         rc = lib().tcscf_rcs_scf(self.nElectrons, nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                  _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()), fp, fd,
-                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, 0, diis)), ctypes.byref(res),
+                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, 0, diis, graph)), ctypes.byref(res),
                                  trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp),
                                  ctypes.byref(obs), _stream())
         self.density[0] = D
@@ -915,7 +918,7 @@ class UOSHartreeFock(_HFBase):
         return False
 
     def compute(self, overlap, oneElectronTerms, r1Grid, Fji, respectOneElectronSymmetry=True, tol=1e-5,
-                max_iter=100, diis=0):
+                max_iter=100, diis=0, graph=None):
         nb = self.basisSize
         S, Sp = _np(overlap)
         H, Hp = _np(oneElectronTerms)
@@ -929,7 +932,7 @@ class UOSHartreeFock(_HFBase):
         obs = _observables(self.fockOperator, self.eigenvectors, None)
         rc = lib().tcscf_uos_scf(self.nElectrons[0], self.nElectrons[1], nb, Sp, Hp, _ptr(r1Grid.basisValues),
                                  _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(r1Grid.nGrid()), fp, fd,
-                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, 0, diis)), ctypes.byref(res),
+                                 ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, 0, diis, graph)), ctypes.byref(res),
                                  trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp), ev.ctypes.data_as(_dp),
                                  ctypes.byref(obs), _stream())
         self.density[:] = D
@@ -975,7 +978,7 @@ class TCHartreeFock(_HFBase):
         return VjiSame, VjiOppo, None
 
     def compute(self, overlap, oneElectronTerms, r1Grid, FjiSame, FjiOppo, VjiSame, VjiOppo,
-                respectOneElectronSymmetry=True, tol=1e-5, max_iter=200, mgs_mode=0, diis=0):
+                respectOneElectronSymmetry=True, tol=1e-5, max_iter=200, mgs_mode=0, diis=0, graph=None):
         nb = self.basisSize
         S, Sp = _np(overlap)
         H, Hp = _np(oneElectronTerms)
@@ -995,7 +998,7 @@ class TCHartreeFock(_HFBase):
                                 _ptr(r1Grid.basisGradients), _ptr(r1Grid.quadratureGrid.weights), ctypes.c_int64(n1),
                                 _ptr(FjiSame), _ptr(FjiOppo), _ptr(Vs), _ptr(Vo) if Vo is not None else None, fdt,
                                 ctypes.byref(jp) if jp is not None else None,
-                                ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, mgs_mode, diis)),
+                                ctypes.byref(_opts(tol, max_iter, respectOneElectronSymmetry, mgs_mode, diis, graph)),
                                 ctypes.byref(res), trace.ctypes.data_as(_dp), D.ctypes.data_as(_dp),
                                 ps.ctypes.data_as(_dp), ctypes.byref(obs), _stream())
         self.density[:] = D

@@ -96,34 +96,90 @@ def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None, ou
     t2 = time.perf_counter()
     del tensors
     return res, dict(precompute_ms=(t1 - t0) * 1e3, scf_ms=(t2 - t1) * 1e3, total_ms=(t2 - t0) * 1e3,
-                     scf="one instance after another (the single-instance drivers)",
+                     scf="the single-instance drivers, four host threads on streams of their own, largest first",
                      fp64_instances=sum(instance_precision(p["inst"], precision, policy) == "fp64" for p in prepared))
 
 
-def solve_each(items, tol, max_iter, diis):
-    """The SCF of every instance with the single-instance drivers, one after
-    another (the graph-batched tcscf_batch_scf is switched off).  Returns the
-    per-instance dicts energy / iterations / status / trace / D (and C, the
-    occupied orbitals, for TC)."""
-    out = []
-    for x in items:
+_STREAMS = {}
+
+
+def _worker_streams(n):
+    """n streams of the current device (default priority), kept for the process:
+    the drivers keep their per-call resources per stream, so a fixed set bounds
+    them."""
+    import torch
+    dev = torch.cuda.current_device()
+    have = _STREAMS.setdefault(dev, [])
+    while len(have) < n:
+        have.append(torch.cuda.Stream())
+    return have[:n]
+
+
+def _cost(x):
+    """Relative cost of an instance's SCF: points x basis pairs, five tensors'
+    worth for TC."""
+    nb = x["r1"].nBasis()
+    return x["r1"].nGrid() * nb * nb * (5 if x["kind"] == "tc" else 1)
+
+
+def solve_each(items, tol, max_iter, diis, workers=4):
+    """The SCF of every instance with the single-instance drivers.  `workers`
+    host threads, each on a stream of its own, take the instances largest first
+    and launch each iteration's kernels directly (graph=False); the drivers'
+    per-stream resources let the solves run side by side, and every solve's
+    arithmetic is that of a solve alone.  (The graph-batched tcscf_batch_scf is
+    switched off.)  Returns the per-instance dicts energy / iterations / status
+    / trace / D (and C, the occupied orbitals, for TC)."""
+    import itertools
+    import threading
+
+    import torch
+
+    out = [None] * len(items)
+    order = sorted(range(len(items)), key=lambda k: -_cost(items[k]))
+    nxt = itertools.count()
+    failures = []
+
+    def solve(x):
         nb = x["r1"].nBasis()
         if x["kind"] == "tc":
             hf = api.TCHartreeFock(x["nup"], x["ndown"], nb)
             call = lambda: hf.compute(x["S"], x["H"], x["r1"], x["Fs"], x["Fo"], x["V"], x["jastrow"], True, tol=tol,
-                                      max_iter=max_iter, mgs_mode=1, diis=diis)
+                                      max_iter=max_iter, mgs_mode=1, diis=diis, graph=False)
         elif x["kind"] == "rcs":
             hf = api.RCSHartreeFock(x["nup"] + x["ndown"], nb)
-            call = lambda: hf.compute(x["S"], x["H"], x["r1"], x["F"], True, tol=tol, max_iter=max_iter, diis=diis)
+            call = lambda: hf.compute(x["S"], x["H"], x["r1"], x["F"], True, tol=tol, max_iter=max_iter, diis=diis,
+                                      graph=False)
         else:
             hf = api.UOSHartreeFock(x["nup"], x["ndown"], nb)
-            call = lambda: hf.compute(x["S"], x["H"], x["r1"], x["F"], True, tol=tol, max_iter=max_iter, diis=diis)
+            call = lambda: hf.compute(x["S"], x["H"], x["r1"], x["F"], True, tol=tol, max_iter=max_iter, diis=diis,
+                                      graph=False)
         try:
             call()
             status = 0
         except TcscfError as err:
             status = err.code
-        out.append(dict(energy=hf.energy, iterations=hf.numberOfConvergenceLoops(), status=status,
-                        trace=hf.energy_trace, D=hf.density,
-                        C=getattr(hf, "occupiedOrbitals", None)))
+        return dict(energy=hf.energy, iterations=hf.numberOfConvergenceLoops(), status=status,
+                    trace=hf.energy_trace, D=hf.density, C=getattr(hf, "occupiedOrbitals", None))
+
+    def worker(stream):
+        try:
+            with torch.cuda.stream(stream):
+                while True:
+                    q = next(nxt)  # atomic under the interpreter lock
+                    if q >= len(order):
+                        return
+                    out[order[q]] = solve(items[order[q]])
+        except BaseException as err:  # surfaces in the caller
+            failures.append(err)
+
+    torch.cuda.synchronize()  # the tensors come from the contraction's stream
+    streams = _worker_streams(max(1, workers))
+    threads = [threading.Thread(target=worker, args=(s,)) for s in streams]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if failures:
+        raise failures[0]
     return out

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -358,13 +359,13 @@ void density_from(int nb, int nocc, const std::vector<double>& C, double* D) {
 
 }  // namespace
 
-IterContext& iter_context(int kind) {
+IterContext& iter_context(int kind, hipStream_t st) {
   static std::mutex mu;
-  static std::map<std::pair<int, int>, IterContext*> all;
+  static std::map<std::tuple<int, int, hipStream_t>, IterContext*> all;
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lock(mu);
-  IterContext*& c = all[{dev, kind}];
+  IterContext*& c = all[std::make_tuple(dev, kind, st)];
   if (!c) c = new IterContext();  // kept to the end of the process
   return *c;
 }
@@ -532,10 +533,14 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
   TC_CHECK(S && H && A1 && w1 && F && o && res && nb > 0 && n1 > 0, "SCF: bad arguments");
   TC_CHECK(o->max_iter > 0, "SCF: max_iter must be positive");
   TC_CHECK(o->diis >= 0, "SCF: diis depth must be >= 0");
+  TC_CHECK(o->graph >= 0 && o->graph <= 2, "SCF: graph must be 0 (default), 1 (graph) or 2 (direct launches)");
   const int64_t nb2 = (int64_t)nb * nb;
   int rc = 0;
-  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * hf_ws_doubles(nspin, nb), &rc);
-  if (rc) return rc;
+  IterContext& cx = iter_context(0, st);
+  std::lock_guard<std::mutex> lock(cx.mu);
+  const int64_t len = nspin * nb2;
+  if ((rc = cx.prepare(2 * (size_t)len, sizeof(double) * hf_ws_doubles(nspin, nb)))) return rc;
+  double* ws = (double*)cx.ws;
   HfPlan plan;
   std::memset(&plan, 0, sizeof plan);  // its bytes key the iteration graph
   if ((rc = hf_prepare(plan, nspin, nb, n1, F, f_dtype == TCSCF_F32, A1, w1, ws))) return rc;
@@ -545,10 +550,6 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
   // iteration is enqueued on the caller's stream (D copy, sweep, partial sums
   // writing G straight into the mapped buffer); with TCSCF_SCF_GRAPH=1 it
   // replays one HIP graph instead (D upload kernel, sweep, partial sums).
-  IterContext& cx = iter_context(0);
-  std::lock_guard<std::mutex> lock(cx.mu);
-  const int64_t len = nspin * nb2;
-  if ((rc = cx.prepare(2 * (size_t)len))) return rc;
   std::vector<double> Dv, Gv;
   double *Dh, *Gh, *Dmap = nullptr, *Gmap = nullptr;
   if (cx.host) {
@@ -562,7 +563,7 @@ int hf_scf(int nspin, const int* nocc, int nb, const double* S, const double* H,
     Dh = Dv.data();
     Gh = Gv.data();
   }
-  const bool use_graph = IterContext::enabled(0) && Dmap;
+  const bool use_graph = IterContext::enabled(0, o->graph) && Dmap;
   std::vector<unsigned char> key;
   if (use_graph) {
     key_append(key, plan);

@@ -7,10 +7,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <limits>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -32,17 +34,19 @@ struct Pinned {
   Pinned& operator=(const Pinned&) = delete;
 };
 
-// The per-iteration device work of one driver kind (0: RCS / UOS, 1: TC) on
-// one device, as a HIP graph replayed every iteration.  Its resources are kept
-// across calls: the capture / replay stream, the coherent page-locked buffers
-// the device maps (inputs in, results out, moved by kernels: the graph holds
-// kernel nodes only), and the last graph with the bytes it was captured from
-// (the plan, the buffer addresses, the kernel choice).  A solve whose key
-// matches replays that graph without capturing; another key captures anew.
-// Created on first use and never destroyed (the runtime may be gone at exit).
-// The mutex serializes the drivers' calls on a device.  Without the graph the
-// drivers enqueue every iteration directly (the same kernels in the same
-// order: the same results).
+// The per-call resources of one driver kind (0: RCS / UOS, 1: TC) on one
+// device and caller stream: the sweep workspace, the coherent page-locked
+// buffers the device maps (inputs in, results out), the capture / replay
+// stream, and the last per-iteration HIP graph with the bytes it was captured
+// from (the plan, the buffer address, the kernel choice).  Every iteration's
+// device build replays that graph (kernel nodes only: the data move through
+// the mapped buffers by kernels); a solve whose key matches replays it without
+// capturing, another key captures anew.  Kept across calls, created on first
+// use and never destroyed (the runtime may be gone at exit).  The mutex
+// serializes the drivers' calls on one (device, stream); calls on different
+// streams have separate resources and may run concurrently.  Without the
+// graph the drivers enqueue every iteration directly (the same kernels in the
+// same order: the same results).
 struct IterContext {
   std::mutex mu;
   hipStream_t cs = nullptr;
@@ -50,22 +54,28 @@ struct IterContext {
   double* host = nullptr;
   double* dev = nullptr;
   size_t cap = 0;
+  void* ws = nullptr;  // sweep workspace (device)
+  size_t ws_bytes = 0;
   hipGraph_t g = nullptr;
   hipGraphExec_t ex = nullptr;
   std::vector<unsigned char> key;
   int captures = 0;  // captures during the current call
 
-  // TCSCF_SCF_GRAPH=1: graphs for every driver, 0: none; unset: where the
-  // graph was measured faster than direct launches.  TC (Ne, nb = 30): 298
-  // against 310 us per iteration.  RCS / UOS, whose iteration is three
-  // kernels, run 7 us slower per iteration from a graph and enqueue directly.
-  static bool enabled(int kind) {
+  // opt: tcscf_scf_opts.graph.  TCSCF_SCF_GRAPH=1: graphs for every driver,
+  // 0: none; unset: where the graph was measured faster than direct launches.
+  // TC (Ne, nb = 30): 298 against 310 us per iteration.  RCS / UOS, whose
+  // iteration is three kernels, run 7 us slower per iteration from a graph
+  // and enqueue directly.
+  static bool enabled(int kind, int opt) {
+    if (opt == 1) return true;
+    if (opt == 2) return false;
     const char* e = std::getenv("TCSCF_SCF_GRAPH");
     if (e && *e) return std::atoi(e) != 0;
     return kind == 1;
   }
-  // n doubles of mapped buffer (host / dev null when that fails); the stream.
-  int prepare(size_t n) {
+  // n doubles of mapped buffer (host / dev null when that fails), a workspace
+  // of ws_need bytes (null with *rc set when that fails), the stream.
+  int prepare(size_t n, size_t ws_need) {
     captures = 0;
     if (!cs) TC_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     if (n > cap) {
@@ -80,6 +90,22 @@ struct IterContext {
       } else {
         dev = nullptr;
       }
+    }
+    if (ws_need > ws_bytes) {
+      drop_graph();
+      if (ws) {
+        TC_HIP(hipStreamSynchronize(cs));
+        TC_HIP(hipFree(ws));
+        ws = nullptr;
+        ws_bytes = 0;
+      }
+      const size_t want = std::max(ws_need, (size_t)1 << 20);
+      if (hipMalloc(&ws, want) != hipSuccess) {
+        ws = nullptr;
+        set_error("SCF: hipMalloc of " + std::to_string(want) + " bytes of workspace failed");
+        return TCSCF_ENOMEM;
+      }
+      ws_bytes = want;
     }
     return 0;
   }
@@ -115,8 +141,9 @@ struct IterContext {
     key.clear();
   }
 };
-// The context of driver kind 0 (RCS / UOS) or 1 (TC) on the current device.
-IterContext& iter_context(int kind);
+// The context of driver kind 0 (RCS / UOS) or 1 (TC) on the current device
+// for calls on stream st.
+IterContext& iter_context(int kind, hipStream_t st);
 
 // The bytes of a trivially copyable value, appended to a graph key.
 template <class T>

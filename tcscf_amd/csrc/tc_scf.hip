@@ -1673,12 +1673,19 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
                  double* D_out, double* pseudo_out, tcscf_scf_observables* obs, void* stream) {
   TC_CHECK(S && H && o && res && o->max_iter > 0, "tcscf_tc_scf: bad arguments");
   TC_CHECK(o->diis >= 0, "tcscf_tc_scf: diis depth must be >= 0");
+  TC_CHECK(o->graph >= 0 && o->graph <= 2, "tcscf_tc_scf: graph must be 0 (default), 1 (graph) or 2 (direct)");
   TC_CHECK(o->diis == 0 || o->mgs_mode == 1, "tcscf_tc_scf: DIIS needs S-orthonormal orbitals (mgs_mode = 1)");
   hipStream_t st = (hipStream_t)stream;
   const int64_t nb2 = (int64_t)nb * nb;
   int rc = 0;
-  double* ws = (double*)scratch(kSlotSweep, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1, Vo != nullptr), &rc);
-  if (rc) return rc;
+  IterContext& cx = iter_context(1, st);
+  std::lock_guard<std::mutex> lock(cx.mu);
+  {
+    const size_t nmx = std::max(1, std::max(nup, ndown));
+    const size_t ndc = 2 * (size_t)nb2 + 2 * (size_t)nb * nmx + tc_out_doubles(nb);  // D, C, result block
+    if ((rc = cx.prepare(ndc, sizeof(double) * tc_ws_doubles(nup, ndown, nb, n1, Vo != nullptr)))) return rc;
+  }
+  double* ws = (double*)cx.ws;
   TcPlan plan;
   std::memset(&plan, 0, sizeof plan);  // its bytes key the iteration graph
   if ((rc = tc_prepare(plan, nup, ndown, nb, A1, dA1, w1, n1, Fs, Fo, Vs, Vo, f_dtype, jp, ws))) return rc;
@@ -1692,14 +1699,11 @@ int tcscf_tc_scf(int nup, int ndown, int nb, const double* S, const double* H, c
   // TCSCF_SCF_GRAPH=0, every iteration is enqueued on the caller's stream with
   // copies (pageable vectors if the page-locked allocation failed).
   const size_t nD = iter.D.size(), nC = iter.C.size(), nO = tc_out_doubles(nb);
-  IterContext& cx = iter_context(1);
-  std::lock_guard<std::mutex> lock(cx.mu);
-  if ((rc = cx.prepare(nD + nC + nO))) return rc;
   std::vector<double> stage;
   if (!cx.host) stage.resize(nD + nC + nO);
   double* Dh = cx.host ? cx.host : stage.data();
   double *Ch = Dh + nD, *outh = Ch + nC;
-  const bool use_graph = IterContext::enabled(1) && cx.dev;
+  const bool use_graph = IterContext::enabled(1, o->graph) && cx.dev;
   if (use_graph) TC_HIP(hipStreamSynchronize(st));  // the tensors come from the caller's stream
   res->status = TCSCF_ENOTCONV;
   res->energy = std::numeric_limits<double>::max();

@@ -197,3 +197,45 @@ def test_batch_accurate_policy_subset(tc, gpu):
             assert a["energy"] == d["energy"]
         else:
             assert abs(a["energy"] - d["energy"]) < 1e-6, (x["Z"], a["energy"] - d["energy"])
+
+
+def test_batch_concurrent_solves_match_one_at_a_time(tc, gpu):
+    """The batch's SCF half on four host threads, each on a stream of its own
+    (the drivers' per-stream resources, direct launches), gives every instance
+    bitwise the result of the same solves one after another on one stream, and
+    of a solve alone with the default per-iteration graph.  Twelve instances of
+    the fixture: RCS, UOS and TC, open and closed shells, nb from 7 to 53."""
+    import torch
+    from tcscf_amd import batch
+    with open(os.path.join(ROOT, "tests", "golden", "batch64_instances.json")) as f:
+        instances = json.load(f)
+    sub = [x for x in instances if x["npts"] <= 5500][:12]
+    assert {x["method"] for x in sub} >= {"tc", "hf"} and max(x["nb"] for x in sub) > 32
+    prep = batch.prepare(sub)
+    specs = batch.contraction_specs(prep)
+    tensors = tc.computeF_batch(specs, precision="bf16x5", out_dtype="fp32")
+    torch.cuda.synchronize()
+    items, k = [], 0
+    for p in prep:
+        x = p["inst"]
+        d = dict(nup=x["nup"], ndown=x["ndown"], S=p["S"], H=p["H"], r1=p["grid"])
+        if x["method"] == "tc":
+            d.update(kind="tc", Fs=tensors[k], Fo=tensors[k + 1], V=tensors[k + 2], jastrow=p["jastrow"])
+            k += 3
+        else:
+            d.update(kind="rcs" if x["nup"] == x["ndown"] else "uos", F=tensors[k])
+            k += 1
+        items.append(d)
+    par = batch.solve_each(items, tol=1e-8, max_iter=60, diis=6, workers=4)
+    seq = batch.solve_each(items, tol=1e-8, max_iter=60, diis=6, workers=1)
+    for x, a, b in zip(items, par, seq):
+        assert a["status"] == b["status"] and a["iterations"] == b["iterations"]
+        assert a["energy"] == b["energy"] and np.array_equal(a["trace"], b["trace"]) and np.array_equal(a["D"], b["D"])
+    # the default (graph) path of a TC solve alone gives the same bits
+    q = next(i for i, x in enumerate(items) if x["kind"] == "tc")
+    x = items[q]
+    hf = tc.TCHartreeFock(x["nup"], x["ndown"], x["r1"].nBasis())
+    e = hf.compute(x["S"], x["H"], x["r1"], x["Fs"], x["Fo"], x["V"], x["jastrow"], True, tol=1e-8, max_iter=60,
+                   mgs_mode=1, diis=6)
+    assert hf.graph_iterations == hf.numberOfConvergenceLoops()
+    assert e == par[q]["energy"] and np.array_equal(hf.energy_trace, par[q]["trace"])
