@@ -62,12 +62,14 @@ def instance_precision(inst, precision, policy):
     return precision
 
 
-def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None, out_dtype=None, policy=None):
+def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None, out_dtype=None, policy=None,
+        workers=4):
     """Precompute (one batched call per contraction precision) + batched SCF.
     policy None: every instance at `precision`; 'accurate': FP64 for the
     instances instance_precision sends there.  F / V storage: FP32 for the BF16
     contractions (the Fock builds keep FP64 arithmetic), FP64 otherwise,
-    unless out_dtype says.  Returns (results, timings dict)."""
+    unless out_dtype says.  workers: host threads (streams) of the SCF half.
+    Returns (results, timings dict)."""
     specs = contraction_specs(prepared)
     owner = [k for k, p in enumerate(prepared) for _ in range(3 if p["inst"]["method"] == "tc" else 1)]
     precs = [instance_precision(prepared[k]["inst"], precision, policy) for k in owner]
@@ -92,11 +94,11 @@ def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None, ou
             d.update(kind="rcs" if x["nup"] == x["ndown"] else "uos", F=tensors[k])
             k += 1
         items.append(d)
-    res = solve_each(items, tol=tol, max_iter=max_iter, diis=diis)
+    res = solve_each(items, tol=tol, max_iter=max_iter, diis=diis, workers=workers)
     t2 = time.perf_counter()
     del tensors
     return res, dict(precompute_ms=(t1 - t0) * 1e3, scf_ms=(t2 - t1) * 1e3, total_ms=(t2 - t0) * 1e3,
-                     scf="the single-instance drivers, four host threads on streams of their own, largest first",
+                     scf=f"the single-instance drivers, {workers} host threads on streams of their own, largest first",
                      fp64_instances=sum(instance_precision(p["inst"], precision, policy) == "fp64" for p in prepared))
 
 
