@@ -1208,16 +1208,17 @@ __global__ __launch_bounds__(256) void tc_vlin_kernel(const ST* __restrict__ V, 
 struct RankRanges {
   int ustart[7];
 };
-// Sum over the partials b in [b_begin, b_end) of part[b][e] for 64 consecutive
-// elements e (one per lane) of a workgroup of 16 waves: wave v sums a
-// contiguous slice of the partials (eight coalesced loads in flight), then the
-// 16 slice sums are added in wave order.
-__device__ inline double sum_partials_1024(const double* __restrict__ part, int b_begin, int b_end, int64_t len,
-                                           int64_t e) {
-  __shared__ double red[16][64];
-  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
-  const int per = (b_end - b_begin + 15) / 16;
-  const int b0 = b_begin + v * per, b1 = b0 + per < b_end ? b0 + per : b_end;
+// Sum over the partials b in [b_begin, b_end) of part[b][e] for 16 consecutive
+// elements e of a workgroup of 1024 threads: the 64 groups of 16 lanes (lane
+// = element) each sum a contiguous slice of the partials (eight loads of one
+// 128-byte line in flight), then the 64 slice sums are added in group order.
+// Returns the sum in the threads of group 0.
+__device__ inline double sum_partials_16(const double* __restrict__ part, int b_begin, int b_end, int64_t len,
+                                         int64_t e) {
+  __shared__ double red[64][16];
+  const int el = threadIdx.x & 15, grp = threadIdx.x >> 4;
+  const int per = (b_end - b_begin + 63) / 64;
+  const int b0 = b_begin + grp * per, b1 = b0 + per < b_end ? b0 + per : b_end;
   double s = 0;
   if (e < len) {
     int b = b0;
@@ -1230,31 +1231,31 @@ __device__ inline double sum_partials_1024(const double* __restrict__ part, int 
     }
     for (; b < b1; ++b) s += part[(int64_t)b * len + e];
   }
-  red[v][lane] = s;
+  red[grp][el] = s;
   __syncthreads();
   double t = 0;
-  if (v == 0)
-    for (int u = 0; u < 16; ++u) t += red[u][lane];
+  if (grp == 0)
+    for (int u = 0; u < 64; ++u) t += red[u][el];
   return t;
 }
 
 // out[st * len + e] = sum over the rank workgroups u of target st of part[u][e]:
-// one workgroup of 1024 per (target, 64 consecutive elements).
+// one workgroup of 1024 per (target, 16 consecutive elements).
 __global__ __launch_bounds__(1024) void tc_reduce_ranges(const double* __restrict__ part, RankRanges r, int64_t len,
                                                          double* __restrict__ out) {
-  const int64_t nblk = (len + 63) / 64;
+  const int64_t nblk = (len + 15) / 16;
   const int st = (int)(blockIdx.x / nblk);
-  const int64_t e = (blockIdx.x - (int64_t)st * nblk) * 64 + (threadIdx.x & 63);
-  const double t = sum_partials_1024(part, r.ustart[st], r.ustart[st + 1], len, e);
-  if ((threadIdx.x >> 6) == 0 && e < len) out[(int64_t)st * len + e] = t;
+  const int64_t e = (blockIdx.x - (int64_t)st * nblk) * 16 + (threadIdx.x & 15);
+  const double t = sum_partials_16(part, r.ustart[st], r.ustart[st + 1], len, e);
+  if ((threadIdx.x >> 4) == 0 && e < len) out[(int64_t)st * len + e] = t;
 }
 
-// out[e] = sum_b part[b][e], b < nparts: one workgroup of 1024 per 64 elements.
+// out[e] = sum_b part[b][e], b < nparts: one workgroup of 1024 per 16 elements.
 __global__ __launch_bounds__(1024) void tc_reduce(const double* __restrict__ part, int nparts, int64_t len,
                                                   double* __restrict__ out) {
-  const int64_t e = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-  const double t = sum_partials_1024(part, 0, nparts, len, e);
-  if ((threadIdx.x >> 6) == 0 && e < len) out[e] = t;
+  const int64_t e = (int64_t)blockIdx.x * 16 + (threadIdx.x & 15);
+  const double t = sum_partials_16(part, 0, nparts, len, e);
+  if ((threadIdx.x >> 4) == 0 && e < len) out[e] = t;
 }
 
 double now_ms() {
@@ -1491,10 +1492,10 @@ static int tc_build(const TcPlan& p, bool restricted, bool col, hipStream_t st, 
 #undef TC_VLIN
     TC_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(tc_reduce_ranges, dim3((unsigned)(nst * ((nb2 + 63) / 64))), dim3(1024), 0, st, p.dPG, rr, nb2,
+  hipLaunchKernelGGL(tc_reduce_ranges, dim3((unsigned)(nst * ((nb2 + 15) / 16))), dim3(1024), 0, st, p.dPG, rr, nb2,
                      out_dev);
   TC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)((nk * p.npair + 63) / 64)), dim3(1024), 0, st, p.dPV, kBlocks,
+  hipLaunchKernelGGL(tc_reduce, dim3((unsigned)((nk * p.npair + 15) / 16)), dim3(1024), 0, st, p.dPV, kBlocks,
                      nk * p.npair, out_dev + 6 * nb2);
   TC_HIP(hipGetLastError());
   return 0;
