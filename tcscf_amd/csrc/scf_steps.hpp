@@ -220,6 +220,7 @@ struct TcPlan {
   size_t col_sh[2];
   double* dCc;
   int ustart[7];  // rank workgroups per target (prefix offsets)
+  int rchunk;     // (point, entry) terms per rank workgroup
 };
 size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1, bool general_v);
 size_t tc_out_doubles(int nb);  // the per-build result block copied back to the host

@@ -24,6 +24,8 @@
 #include <complex>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -38,7 +40,7 @@ namespace tcs {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlocks = 512;
-constexpr int kRankChunk = 512;  // (point, entry) terms per rank workgroup
+
 constexpr int kWMain = 5;   // occupied orbitals of W = V C per pass over j (the first in the main pass)
 constexpr int kWChunk = 2;  // orbitals of W per LDS round of |U|^2
 
@@ -1033,6 +1035,7 @@ struct RankArgs {
   int nb, nvec, nocc0, nocc1, nmax, nwt;  // nwt: V tensors with W blocks (1 or 2)
   int ustart[7];        // workgroups of target 3 s + t: [ustart[st], ustart[st + 1])
   double* part;         // [gridDim.x][nb][nb]
+  int chunk;            // (point, entry) terms per workgroup
 };
 
 // (point, entry) terms per point of target st = 3 s + t: 1, 4, 5 + 3 nwt nocc_s.
@@ -1060,8 +1063,8 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
   const int E = rank_entries(st, a.nocc0, a.nocc1, a.nwt);
   const int w3 = 3 * (s ? a.nocc1 : a.nocc0);  // W entries per tensor
   // this workgroup's terms: k = kb + [0, K) of the target's n1 E (point, entry) terms
-  const int64_t kb = (int64_t)(u - a.ustart[st]) * kRankChunk;
-  const int K = (int)(a.n1 * E - kb < kRankChunk ? a.n1 * E - kb : kRankChunk);
+  const int64_t kb = (int64_t)(u - a.ustart[st]) * a.chunk;
+  const int K = (int)(a.n1 * E - kb < a.chunk ? a.n1 * E - kb : a.chunk);
   const int64_t p0 = kb / E;
   const int e0 = (int)(kb - p0 * E);
   const float invE = 1.0f / (float)E;
@@ -1297,19 +1300,63 @@ using namespace tcs;
 
 size_t tc_out_doubles(int nb) { return 6 * (size_t)nb * nb + 2 * (size_t)nb * (nb + 1) / 2; }
 
-// Rank workgroups per target st = 3 s + t (kRankChunk terms each), as prefix
+// (point, entry) terms per rank workgroup.  The workgroups of one build run in
+// rounds of the device's resident slots (CUs x workgroups per CU); a chunk
+// that leaves the last round mostly idle costs that round in full (Ne: 512
+// terms gave 1.44 rounds of 1024 slots, 76 us; whole rounds measured as fast
+// at twice the chunk).  So the chunk is the smallest multiple of 16 whose
+// workgroups fill whole rounds: ceil(terms / (rounds x slots)), with as many
+// rounds as keep it at or below 1024 terms.  TCSCF_RANK_CHUNK fixes it for
+// A/B timing.
+static int rank_chunk(int nup, int ndown, int nwt, int nb, int64_t n1) {
+  if (const char* e = std::getenv("TCSCF_RANK_CHUNK")) {
+    const int v = std::atoi(e);
+    if (v >= 16 && v <= 65536) return v / 16 * 16;
+  }
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, int> slots_of;  // (device, NT) -> resident workgroups
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int nt = nb <= 32 ? 2 : (nb <= 48 ? 3 : 4);
+  int slots = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto f = slots_of.find({dev, nt});
+    if (f != slots_of.end()) {
+      slots = f->second;
+    } else {
+      int ncu = 0, per = 0;
+      const void* fn = nt == 2 ? (const void*)tc_rank_kernel<2>
+                               : (nt == 3 ? (const void*)tc_rank_kernel<3> : (const void*)tc_rank_kernel<4>);
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, 0) != hipSuccess || ncu <= 0 || per <= 0) {
+        ncu = 256;
+        per = 1;
+      }
+      slots = slots_of[{dev, nt}] = ncu * per;
+    }
+  }
+  int64_t terms = 0;
+  for (int st = 0; st < 6; ++st) terms += n1 * rank_entries(st, nup, ndown, nwt);
+  for (int64_t rounds = 1;; ++rounds) {
+    const int64_t c = (terms + rounds * slots - 1) / (rounds * slots);
+    if (c <= 1024 || rounds >= 64) return (int)std::max<int64_t>(64, (c + 15) / 16 * 16);
+  }
+}
+
+// Rank workgroups per target st = 3 s + t (chunk terms each), as prefix
 // offsets ustart[0..6].
-static void rank_units(int nup, int ndown, int nwt, int64_t n1, int* ustart) {
+static void rank_units(int nup, int ndown, int nwt, int64_t n1, int chunk, int* ustart) {
   ustart[0] = 0;
   for (int st = 0; st < 6; ++st)
-    ustart[st + 1] = ustart[st] + (int)((n1 * rank_entries(st, nup, ndown, nwt) + kRankChunk - 1) / kRankChunk);
+    ustart[st + 1] = ustart[st] + (int)((n1 * rank_entries(st, nup, ndown, nwt) + chunk - 1) / chunk);
 }
 
 size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1, bool general_v) {
   const int nmax = std::max(1, std::max(nup, ndown)), nwt = general_v ? 2 : 1;
   const size_t nb2 = (size_t)nb * nb, npair = (size_t)nb * (nb + 1) / 2, nvec = kVecFixed + 3 * nmax * nwt;
   int us[7];
-  rank_units(nup, ndown, nwt, n1, us);
+  rank_units(nup, ndown, nwt, n1, rank_chunk(nup, ndown, nwt, nb, n1), us);
   return 2 * nb2 + 2 * (size_t)nb * nmax + (size_t)n1 * 2 * nvec * nb + (size_t)n1 * 4 + (size_t)n1 * 12 +
          (size_t)us[6] * nb2 + (size_t)kBlocks * 2 * npair + tc_out_doubles(nb) + 32 * 16 + 2 * kDwStride;
 }
@@ -1351,7 +1398,8 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   p.dAl = p.dVec + n1 * 2 * p.nvec * nb;
   p.dH = p.dAl + n1 * 4;
   p.dPG = p.dH + n1 * 12;
-  rank_units(nup, ndown, p.nwt, n1, p.ustart);
+  p.rchunk = rank_chunk(nup, ndown, p.nwt, nb, n1);
+  rank_units(nup, ndown, p.nwt, n1, p.rchunk, p.ustart);
   p.dPV = p.dPG + (int64_t)p.ustart[6] * nb2;
   p.dOut = p.dPV + (int64_t)kBlocks * 2 * p.npair;
   p.dCc = p.dOut + tc_out_doubles(nb);
@@ -1465,7 +1513,8 @@ static int tc_build(const TcPlan& p, bool restricted, bool col, hipStream_t st, 
     TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage[r], restricted, p.gv), dim3((unsigned)p.pblocks[r]),
                            dim3(64 * p.wpw[r]), pargs, p.sh1[r], st));
   }
-  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.nmax, p.nwt, {}, p.dPG};
+  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.nmax, p.nwt, {}, p.dPG,
+              p.rchunk};
   RankRanges rr{};
   for (int q = 0; q < 7; ++q) ra.ustart[q] = rr.ustart[q] = p.ustart[q];
   const dim3 rgrid((unsigned)p.ustart[nst]);
