@@ -4,7 +4,7 @@ One step = one full SCF solve on inputs already resident in HBM: STO basis
 tabulation, the fused kernel-generation + contraction F = Kw . P (every
 (r1, r2) grid pair, 1/r12 kernel), and the RHF iterations to convergence.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x5|bf16x3|fp32|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x4|bf16x5|bf16x3|fp32|fp64]
 
 Prints one JSON line.  `value` is grid-pair contractions per second over the
 whole step (pairs x kernel components / step time); the FP64 drop-in path is
@@ -25,12 +25,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "SCF wall time + grid-pair contractions/s: mixed-prec MFMA vs FP64 GPU @1e-6 Eh"
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x5": 2500.0}
-PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x5": 5}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x4": 2500.0, "bf16x5": 2500.0}
+PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x4": 4, "bf16x5": 5}
 # HBM bytes per launch of the contraction kernel on this workload, from rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --pmc-probe`, each scaled by the
 # reading for a 512 MiB copy (FETCH x2.0, WRITE x1.0); profiles/bf16_fgemm_ne30_traffic.md
-TRAFFIC_BYTES = {"bf16x5": 1.502e9 + 1.140e8, "fp64": 1.016e9 + 1.156e8}
+TRAFFIC_BYTES = {"bf16x4": 1.502e9 + 1.139e8, "bf16x5": 1.502e9 + 1.140e8, "fp64": 1.016e9 + 1.156e8}
 
 
 def parse():
@@ -38,7 +38,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--precision", default="bf16x5", choices=["bf16x5", "bf16x3", "fp32", "fp64"])
+    p.add_argument("--precision", default="bf16x4", choices=["bf16x4", "bf16x5", "bf16x3", "fp32", "fp64"])
     p.add_argument("--fp64-steps", type=int, default=2)
     p.add_argument("--storage", default="fp64", choices=["fp64", "fp32"],
                    help="F storage of the mixed-precision RHF leg (the FP64 reference leg stores FP64)")
@@ -49,14 +49,18 @@ def parse():
     p.add_argument("--tc-steps", type=int, default=5, help="timed TC-HF Ne solves per precision")
     p.add_argument("--tc-mgs", type=int, default=1, help="TC Gram-Schmidt: 0 reference, 1 metric")
     p.add_argument("--tc-diis", type=int, default=6, help="TC DIIS depth (the TC workload asks for 6; 0 = plain)")
-    p.add_argument("--tc-smooth-precision", default="bf16x3", choices=["bf16x5", "bf16x3", "fp32", "fp64"],
+    p.add_argument("--tc-smooth-precision", default="bf16x3", choices=["bf16x4", "bf16x5", "bf16x3", "fp32", "fp64"],
                    help="mixed TC: precision of the smooth 1/(r12 + a12)^4 part of F_same / F_oppo")
-    p.add_argument("--tc-v-precision", default="bf16x3", choices=["bf16x5", "bf16x3", "fp32", "fp64"],
+    p.add_argument("--tc-v-precision", default="bf16x3", choices=["bf16x4", "bf16x5", "bf16x3", "fp32", "fp64"],
                    help="mixed TC: precision of the Jastrow-gradient tensor V")
     p.add_argument("--no-batch", action="store_true", help="skip the batched 64-instance side measurement")
+    p.add_argument("--batch-precision", default="bf16x5", choices=["bf16x4", "bf16x5", "bf16x3", "fp32"],
+                   help="the batch's mixed precision (BF16x5: its small grids average the dropped Kl*Pm term "
+                        "less, and its tile does not gain from four products)")
     p.add_argument("--pmc-probe", action="store_true",
-                   help="for rocprofv3 --pmc runs: one BF16x5 and one FP64 computeF on the Ne grid, one RHF "
-                        "solve, then a 512 MiB device copy whose counter reading calibrates the byte counters")
+                   help="for rocprofv3 --pmc runs: BF16x5, BF16x4 and FP64 computeF on the Ne grid, one RHF "
+                        "solve, the TC-Ne mixed contractions, then a 512 MiB device copy whose counter reading "
+                        "calibrates the byte counters")
     return p.parse_args()
 
 
@@ -177,7 +181,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16x5": "bf16 split (K 2 terms, P 3 terms, 5 products), f32 MFMA per 32-pt step, f64 sums + f64 SCF",
+        "dtype": {"bf16x4": "bf16 split (K 2 terms, P 3 terms, 4 products: Kl*Pm dropped), f32 MFMA per two 32-pt steps, "
+                            "f64 sums + f64 SCF",
+                  "bf16x5": "bf16 split (K 2 terms, P 3 terms, 5 products), f32 MFMA per 32-pt step, f64 sums + f64 SCF",
                   "bf16x3": "bf16 split (K 2 terms, P 2 terms, 3 products), f32 MFMA per 32-pt step, f64 sums + f64 SCF",
                   "fp32": "f32 MFMA, f64 cross-chunk + f64 SCF", "fp64": "f64"}[a.precision],
         "data": "synthetic (even-tempered STO basis, deterministic radial x Lebedev grid)",
@@ -382,10 +388,11 @@ def run_batch(a, tc, configs, torch):
     inst = configs.batch_instances()
     prep = batch.prepare(inst)
     torch.cuda.synchronize()
-    batch.run(prep, a.precision, synchronize=torch.cuda.synchronize)  # warm-up (and the tile autotuner)
-    res_m, t_m = batch.run(prep, a.precision, synchronize=torch.cuda.synchronize)
+    prec = a.batch_precision
+    batch.run(prep, prec, synchronize=torch.cuda.synchronize)  # warm-up (and the tile autotuner)
+    res_m, t_m = batch.run(prep, prec, synchronize=torch.cuda.synchronize)
     res_d, t_d = batch.run(prep, "fp64", synchronize=torch.cuda.synchronize)
-    res_a, t_a = batch.run(prep, a.precision, synchronize=torch.cuda.synchronize, policy="accurate")
+    res_a, t_a = batch.run(prep, prec, synchronize=torch.cuda.synchronize, policy="accurate")
 
     def errors(res):
         both = [(m, d) for m, d in zip(res, res_d) if m["status"] == 0 and d["status"] == 0]
@@ -401,27 +408,29 @@ def run_batch(a, tc, configs, torch):
                         f"{sum(x['method'] == 'tc' for x in inst)} TC",
             "pair_units": units,
             "value": units / (t_m["total_ms"] * 1e-3), "unit": "grid-pair contractions/s",
-            a.precision: t_m, "fp64": t_d, "speedup_vs_fp64": t_d["total_ms"] / t_m["total_ms"],
-            "converged": {a.precision: sum(r["status"] == 0 for r in res_m), "fp64": sum(r["status"] == 0 for r in res_d)},
+            "precision": prec, prec: t_m, "fp64": t_d, "speedup_vs_fp64": t_d["total_ms"] / t_m["total_ms"],
+            "converged": {prec: sum(r["status"] == 0 for r in res_m), "fp64": sum(r["status"] == 0 for r in res_d)},
             "energy_err_vs_fp64": errors(res_m),
             # every instance within 1e-6 Eh: FP64 contractions from Z = batch.FP64_FROM_Z up
             "accurate_policy": {"fp64_from_Z": batch.FP64_FROM_Z, "fp64_instances": t_a["fp64_instances"],
                                 "total_ms": t_a["total_ms"], "value": units / (t_a["total_ms"] * 1e-3),
                                 "speedup_vs_fp64": t_d["total_ms"] / t_a["total_ms"],
                                 "energy_err_vs_fp64": errors(res_a)},
-            "tile_rows_autotuned": tc.lib().tcscf_batch_tuned_rows(tc._lib.PRECISIONS[a.precision])
-            if a.precision in ("bf16x5", "bf16x3") else None}
+            "tile_rows_autotuned": tc.lib().tcscf_batch_tuned_rows(tc._lib.PRECISIONS[prec])
+            if prec.startswith("bf16") else None}
 
 
 def pmc_probe(tc, cfg, torch):
-    """Kernels to read hardware counters on: the contraction at both
-    precisions, one RHF solve (the HBM-bound sweep), the TC-Ne mixed
-    contractions (F pair at BF16x5 / BF16x3, V at BF16x3, FP32 storage) and a
+    """Kernels to read hardware counters on: the contraction at BF16x4,
+    BF16x5 and FP64, one RHF solve (the HBM-bound sweep), the TC-Ne mixed
+    contractions (F pair at BF16x4 / BF16x3, V at BF16x3, FP32 storage) and a
     plain 512 MiB copy of known traffic (read 512 MiB, write 512 MiB) to scale
     FETCH_SIZE/WRITE_SIZE."""
     g = tc.QMCGrid.radial_lebedev(*cfg["grid"], 1.0, 1.0)
     g.setBasisFunctions(cfg["basis"], True)
     F = tc.computeF(g, g, precision="bf16x5")
+    torch.cuda.synchronize()
+    F = tc.computeF(g, g, precision="bf16x4")
     torch.cuda.synchronize()
     F64 = tc.computeF(g, g, precision="fp64")
     torch.cuda.synchronize()
@@ -430,7 +439,7 @@ def pmc_probe(tc, cfg, torch):
     g3 = tc.QMCGrid.radial_lebedev(50, 23, cfg["grid"][2], 1.0, 1.0)
     g3.setBasisFunctions(cfg["basis"], True)
     jas = tc.Jastrow.from_b(1.0)
-    tc.computeF_tc_pair(g, g, jas, precision="bf16x5", smooth_precision="bf16x3", out_dtype="fp32")
+    tc.computeF_tc_pair(g, g, jas, precision="bf16x4", smooth_precision="bf16x3", out_dtype="fp32")
     tc.computeV(g, g3, jas, precision="bf16x3", out_dtype="fp32")
     torch.cuda.synchronize()
     x = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda").fill_(1.0)

@@ -395,7 +395,13 @@ struct OchiJastrow {
   }
 };
 
-enum class Precision : int { FP64 = TCSCF_FP64, FP32 = TCSCF_FP32, BF16X3 = TCSCF_BF16X3, BF16X5 = TCSCF_BF16X5 };
+enum class Precision : int {
+  FP64 = TCSCF_FP64,
+  FP32 = TCSCF_FP32,
+  BF16X3 = TCSCF_BF16X3,
+  BF16X4 = TCSCF_BF16X4,
+  BF16X5 = TCSCF_BF16X5
+};
 
 namespace detail {
 /* tcscf_dtype of a device array's element type (FP64, or FP32 storage of the

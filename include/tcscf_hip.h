@@ -51,7 +51,9 @@ enum tcscf_precision {
   TCSCF_FP32 = 1,   /* v_mfma_f32_16x16x4_f32, FP64 cross-chunk sums */
   TCSCF_BF16X3 = 2, /* v_mfma_f32_16x16x32_bf16: K = Kh + Kl, P = Ph + Pm;
                        Kh Ph + Kh Pm + Kl Ph, FP64 sums per 32-point step */
-  TCSCF_BF16X5 = 3  /* as BF16X3 with P = Ph + Pm + Pl and + Kh Pl + Kl Pm */
+  TCSCF_BF16X5 = 3, /* as BF16X3 with P = Ph + Pm + Pl and + Kh Pl + Kl Pm */
+  TCSCF_BF16X4 = 4  /* as BF16X5 without Kl Pm; FP64 sums per 64-point pair of
+                       steps, each step's products issued smallest first */
 };
 
 enum tcscf_dtype { TCSCF_F64 = 0, TCSCF_F32 = 1 };
@@ -304,7 +306,7 @@ typedef struct tcscf_batch_item {
 
 /* No single reference counterpart: computeF / computeV (src/HartreeFock.hpp:
  * 88-229) for many instances of uneven size (atoms, bases, grids, kernels) at
- * once.  BF16X3 / BF16X5: one persistent launch over a work queue of
+ * once.  BF16X3 / BF16X4 / BF16X5: one persistent launch over a work queue of
  * (instance, row block, 256-pair block, component) tiles, largest instances
  * first; the rows per tile (16 or 32) come from an online autotuner that times
  * both on the head of the queue the first time a precision is used in the

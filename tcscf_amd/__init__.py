@@ -4,7 +4,7 @@ The compute lives in libtcscf_hip.so (hand-written HIP, C ABI in
 include/tcscf_hip.h); this package is the host-side mirror of the reference's
 QMCGrid / computeF / computeV / *HartreeFock call surface.
 """
-from ._lib import (BF16X3, BF16X5, FP32, FP64, K_COULOMB, K_JASTROW_GRAD, K_TC_SCALAR, RADIAL_BECKE, RADIAL_M4,
+from ._lib import (BF16X3, BF16X4, BF16X5, FP32, FP64, K_COULOMB, K_JASTROW_GRAD, K_TC_SCALAR, RADIAL_BECKE, RADIAL_M4,
                    RADIAL_XOVER1MX, ConvergenceError, SolverError, TcscfError, lib)
 from .api import (HydrogenLikeBasisFunction, Jastrow, OchiBasisFunction, OchiJastrow, batch_scf, QMCGrid, QuadratureGrid, RCSHartreeFock,
                   SlaterTypeOrbital, TCHartreeFock, UOSHartreeFock,
@@ -14,7 +14,7 @@ from .api import (HydrogenLikeBasisFunction, Jastrow, OchiBasisFunction, OchiJas
                   threeElectronIntegrals, unpack)
 
 __all__ = [
-    "BF16X3", "BF16X5", "FP32", "FP64", "K_COULOMB", "K_JASTROW_GRAD", "K_TC_SCALAR", "RADIAL_BECKE", "RADIAL_M4",
+    "BF16X3", "BF16X4", "BF16X5", "FP32", "FP64", "K_COULOMB", "K_JASTROW_GRAD", "K_TC_SCALAR", "RADIAL_BECKE", "RADIAL_M4",
     "RADIAL_XOVER1MX", "ConvergenceError", "SolverError", "TcscfError", "lib", "HydrogenLikeBasisFunction", "Jastrow",
     "OchiBasisFunction", "OchiJastrow", "QMCGrid",
     "QuadratureGrid", "batch_scf", "RCSHartreeFock", "SlaterTypeOrbital", "TCHartreeFock", "UOSHartreeFock", "coreMatrixElement",

@@ -12,11 +12,11 @@ LIB_PATH = os.path.join(HERE, "libtcscf_hip.so")
 # enums (include/tcscf_hip.h)
 OK, EINVAL, EHIP, ENOTCONV, ESOLVER, ENOMEM = 0, -1, -2, -3, -4, -5
 K_COULOMB, K_TC_SCALAR, K_JASTROW_GRAD, K_TC_SMOOTH = 0, 1, 2, 3
-FP64, FP32, BF16X3, BF16X5 = 0, 1, 2, 3
+FP64, FP32, BF16X3, BF16X5, BF16X4 = 0, 1, 2, 3, 4
 F64, F32 = 0, 1
 RADIAL_M4, RADIAL_BECKE, RADIAL_XOVER1MX = 0, 1, 2
 
-PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16x5": BF16X5}
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16x4": BF16X4, "bf16x5": BF16X5}
 
 # every symbol include/tcscf_hip.h declares
 EXPORTS = [

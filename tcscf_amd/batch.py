@@ -77,7 +77,7 @@ def run(prepared, precision, tol=1e-8, max_iter=60, diis=6, synchronize=None, ou
     tensors = [None] * len(specs)
     for prec in dict.fromkeys(precs):  # each distinct precision once, in first-use order
         idx = [q for q, pr in enumerate(precs) if pr == prec]
-        od = out_dtype or ("fp32" if prec in ("bf16x3", "bf16x5") else "fp64")
+        od = out_dtype or ("fp32" if prec.startswith("bf16") else "fp64")
         for q, T in zip(idx, api.computeF_batch([specs[q] for q in idx], precision=prec, out_dtype=od)):
             tensors[q] = T
     if synchronize:

@@ -14,6 +14,11 @@
 
 namespace tc {
 
+// The BF16 split precisions (MFMA on bf16 pieces of K and of the pair densities).
+inline bool is_bf16(int precision) {
+  return precision == TCSCF_BF16X3 || precision == TCSCF_BF16X4 || precision == TCSCF_BF16X5;
+}
+
 void set_error(const std::string& msg);
 // The calling thread's last message (what tcscf_last_error returns).
 const std::string& last_error();

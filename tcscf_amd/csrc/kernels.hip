@@ -686,10 +686,14 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 /* holds B[8g .. 8g+7][r]; D rows d_row_bf16(g, i).                          */
 /* K ~ Kh + Kl (bf16), P ~ Ph + Pm (+ Pl) (bf16, pre-split in fragment      */
 /* order).  NPROD = 3: Kh Ph + Kh Pm + Kl Ph (P split in two);               */
-/* NPROD = 5: + Kh Pl + Kl Pm (P split in three).  Products are exact in     */
-/* FP32; the FP32 accumulator covers one 32-point k-step and is folded into  */
-/* FP64 sums after every step (long FP32 runs lose the small terms of the    */
-/* large-dynamic-range pair densities).                                      */
+/* NPROD = 5: + Kh Pl + Kl Pm (P split in three); NPROD = 4: + Kh Pl only    */
+/* (Kl Pm is a K rounding residual times a P one: independent per pair, it   */
+/* averages out, where the P residuals, shared by every r1 row, do not).     */
+/* Products are exact in FP32; the FP32 accumulator covers FOLD 32-point     */
+/* k-steps and is then folded into FP64 sums (long FP32 runs lose the small  */
+/* terms of the large-dynamic-range pair densities).  NPROD = 4 takes FOLD = */
+/* 2 and issues each step's products smallest first (Kh Pl, Kl Ph, Kh Pm,   */
+/* Kh Ph), so the corrections add at their own magnitude.                    */
 /* Workgroup = 4 waves sharing 16*MB rows; wave w owns columns               */
 /* [64 w, 64 w + 64) of the workgroup's 256.  Each k-step the workgroup      */
 /* generates its K tile once, split hi/lo, into LDS in A-fragment order      */
@@ -717,6 +721,8 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
   constexpr int EPT = VALS / 256;         // per thread
   static_assert(EPT >= 2 && EPT <= 8 && VALS % 256 == 0 && EPT % 2 == 0, "MB must be 1, 2 or 4");
   static_assert(FOLD == 1 || FOLD == 2, "FP64 fold every 1 or 2 steps");
+  static_assert(NPROD >= 3 && NPROD <= 5, "3, 4 or 5 products");
+  constexpr bool REV = NPROD == 4;
   typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -873,15 +879,17 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
         ah[m] = __builtin_bit_cast(bf8, ktile[buf][c][m][0][lane]);
         al[m] = __builtin_bit_cast(bf8, ktile[buf][c][m][1][lane]);
       }
-      mfma_all(c, ah, 0, FRESH);
-      mfma_all(c, ah, 1, false);
-      mfma_all(c, al, 0, false);
-      if constexpr (NPROD == 5) {
-        if (c < Comp5<Kern>::value) {  // folded at compile time (c is unrolled)
-          mfma_all(c, ah, NSPLIT - 1, false);
-          mfma_all(c, al, 1, false);
+      // products b = 0 hh, 1 hm, 2 lh, 3 hl, 4 lm; np of them (folded at
+      // compile time: c and j are unrolled), smallest first for NPROD = 4
+      const int np = (NPROD >= 4 && c < Comp5<Kern>::value) ? NPROD : 3;
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        if (j < np) {
+          const int b = REV ? np - 1 - j : j;
+          const int sp = (b == 0 || b == 2) ? 0 : (b == 3 ? NSPLIT - 1 : 1);
+          if (b == 2 || b == 4) mfma_all(c, al, sp, FRESH && j == 0);
+          else mfma_all(c, ah, sp, FRESH && j == 0);
         }
-      }
     }
     fence();
     generate(buf ^ 1);
@@ -970,13 +978,13 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
                                                int64_t npair, int64_t ld_row, double scale, Kern kern, int64_t row0,
                                                int64_t cbw) {
   constexpr int NC = Kern::ncomp;
-  constexpr int C5 = NPROD == 5 ? Comp5<Kern>::value : 0;  // components with five products
+  constexpr int C5 = NPROD >= 4 ? Comp5<Kern>::value : 0;  // components with NPROD (4 or 5) products
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
   constexpr int VALS = MB * 64 * 8;     // K values per component per k-step
   constexpr int EPT = VALS / (64 * NW);  // per thread
   static_assert(EPT >= 2 && EPT <= 8 && VALS % (64 * NW) == 0 && EPT % 2 == 0, "K values per thread: 2 .. 8");
   constexpr int NACC = NC * MB * NB;
-  constexpr int NROUND = NC * 2 + C5 * 2;  // product rounds after the fresh one
+  constexpr int NROUND = NC * 2 + C5 * (NPROD - 3);  // product rounds after the fresh one
   typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
   typedef unsigned u4 __attribute__((ext_vector_type(4)));
   const int tid = threadIdx.x;
@@ -1134,7 +1142,7 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
     // fold of the previous partial (8 VALU) in a step that starts a new one
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const int np = c < C5 ? 5 : (NPROD == 5 ? 3 : NPROD);
+      const int np = c < C5 ? NPROD : (NPROD >= 4 ? 3 : NPROD);
       const int kp = kpart(np, 0), sp = plane_of(np, 0);
 #pragma unroll
       for (int m = 0; m < MB; ++m)
@@ -1172,7 +1180,7 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
     int rd = 0;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const int np = c < C5 ? 5 : (NPROD == 5 ? 3 : NPROD);
+      const int np = c < C5 ? NPROD : (NPROD >= 4 ? 3 : NPROD);
 #pragma unroll
       for (int j = 1; j < 5; ++j)
         if (j < np) {  // folded at compile time (c and j are unrolled)
@@ -1389,25 +1397,28 @@ int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, con
   dim3 grid((unsigned)(ncb / (4 * NB)), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
   constexpr int NC = Kern::ncomp;
-  if constexpr (PipeServed<Kern>::value && FOLD == 1 && MB == (NC == 1 ? 2 : 1) && NB == 4) {
+  if constexpr (PipeServed<Kern>::value && FOLD == (NPROD == 4 ? 2 : 1) && MB == (NC == 1 ? 2 : 1) && NB == 4) {
     // the pipelined kernel's buffer offsets are 32-bit byte offsets
     const int mode = bf16_pipe_mode(NC);
     if (mode && (int64_t)NSPLIT * plane * 16 < ((int64_t)1 << 31) && n2p * 16 < ((int64_t)1 << 31)) {
 #define TC_PIPE(M, B, W) \
-  return launch_bf16_pipe<M, B, W, NPROD>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2)
+  return launch_bf16_pipe<M, B, W, NPROD, Kern, FOLD, NPROD == 4>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, \
+                                                                  scale, kern, st, F2)
       if constexpr (NC == 1) {
         if (mode == 1) TC_PIPE(2, 4, 4);
         if (mode == 3) TC_PIPE(4, 2, 4);
-        const int sch = bf16_scheme();
-        if (sch == 1)
-          return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 1, true>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
-                                                                 scale, kern, st, F2);
-        if (sch == 2)
-          return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 2, true>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
-                                                                 scale, kern, st, F2);
-        if (sch == 3)
-          return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 2, false>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
-                                                                  scale, kern, st, F2);
+        if constexpr (NPROD == 5) {
+          const int sch = bf16_scheme();
+          if (sch == 1)
+            return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 1, true>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
+                                                                   scale, kern, st, F2);
+          if (sch == 2)
+            return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 2, true>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
+                                                                   scale, kern, st, F2);
+          if (sch == 3)
+            return launch_bf16_pipe<4, 2, 8, NPROD, Kern, 2, false>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair,
+                                                                    scale, kern, st, F2);
+        }
         TC_PIPE(4, 2, 8);
       } else if constexpr (NC == 2) {
         if (mode == 2) TC_PIPE(4, 1, 8);
@@ -1445,14 +1456,21 @@ template <int NPROD, int MB, class Kern>
 int launch_bf16_fold(Bf16Cfg c, const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0,
                      int64_t plane, int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern,
                      hipStream_t st, void* F2) {
-  if (MB == 4 && c.nb == 2) {
-    if (c.fold == 2)
+  if constexpr (NPROD == 4) {  // four products: FP64 fold every second step
+    if (MB == 4 && c.nb == 2)
       return launch_bf16<MB, NPROD, 2, Kern, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
-    return launch_bf16<MB, NPROD, 1, Kern, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
-  }
-  if (c.fold == 2)
     return launch_bf16<MB, NPROD, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
-  return launch_bf16<MB, NPROD, 1>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+  } else {
+    if (MB == 4 && c.nb == 2) {
+      if (c.fold == 2)
+        return launch_bf16<MB, NPROD, 2, Kern, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st,
+                                                  F2);
+      return launch_bf16<MB, NPROD, 1, Kern, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+    }
+    if (c.fold == 2)
+      return launch_bf16<MB, NPROD, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+    return launch_bf16<MB, NPROD, 1>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+  }
 }
 
 template <int NPROD, class Kern>
@@ -1492,7 +1510,7 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
              void* F2 = nullptr, char* ws_fixed = nullptr) {
   const int64_t npair = npair_of(nb);
   int rc = 0;
-  if (precision == TCSCF_BF16X3 || precision == TCSCF_BF16X5) {
+  if (is_bf16(precision)) {
     const int nsplit = precision == TCSCF_BF16X3 ? 2 : 3;
     const int64_t n2p = round_up(n2, 32), ncb = round_up(npair, 256) / 16, nkb = n2p / 32;
     char* ws = ws_fixed ? ws_fixed : (char*)scratch(kSlotPairs, tcscf_F_workspace_bytes(n2, nb, precision), &rc);
@@ -1514,6 +1532,8 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
     const int64_t plane8 = plane / 8;
     if (precision == TCSCF_BF16X3)
       return launch_bf16_mb<3>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st, F2);
+    if (precision == TCSCF_BF16X4)
+      return launch_bf16_mb<4>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st, F2);
     return launch_bf16_mb<5>(pts1, n1, x4, n2p, P0, plane8, ncb, F, out_f32, npair, scale, kern, st, F2);
   }
   if constexpr (Combines<Kern>::value) {
@@ -1552,7 +1572,7 @@ int compute_F_into(const double* pts1, int64_t n1, const double* pts2, const dou
                    int64_t n2, const double* A2, int nb, int kernel, const tcscf_jastrow* jp, int same_spin,
                    int precision, double scale, void* F, int out_dtype, void* stream, char* ws_fixed) {
   TC_CHECK(pts1 && pts2 && w2 && A2 && F && n1 > 0 && n2 > 0 && nb > 0, "tcscf_compute_F: bad arguments");
-  TC_CHECK(precision >= 0 && precision <= 3, "tcscf_compute_F: unknown precision");
+  TC_CHECK(precision >= 0 && precision <= 4, "tcscf_compute_F: unknown precision");
   TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F: unknown out_dtype");
   TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F: grid too large");
   hipStream_t st = (hipStream_t)stream;
@@ -1633,7 +1653,7 @@ int tcscf_basis_eval(const double* pts, int64_t n, const tcscf_sto* fns, int nb,
 
 size_t tcscf_F_workspace_bytes(int64_t n2, int nb, int precision) {
   const int64_t npair = npair_of(nb);
-  if (precision == TCSCF_BF16X3 || precision == TCSCF_BF16X5) {
+  if (tc::is_bf16(precision)) {
     const int64_t n2p = round_up(n2, 32), ncb = round_up(npair, 256) / 16;
     const size_t nsplit = precision == TCSCF_BF16X3 ? 2 : 3;
     return (size_t)(n2p / 32) * ncb * 512 * 2 * nsplit + (size_t)n2p * (3 * 8 + 16) + 4096;
@@ -1655,7 +1675,7 @@ int tcscf_compute_F_ochi(const double* pts1, int64_t n1, const double* pts2, con
                          int64_t n2, const double* A2, int nb, int kernel, const tcscf_ochi_jastrow* u, int same_spin,
                          int precision, double scale, void* F, int out_dtype, void* stream) {
   TC_CHECK(pts1 && pts2 && w2 && A2 && F && n1 > 0 && n2 > 0 && nb > 0 && u, "tcscf_compute_F_ochi: bad arguments");
-  TC_CHECK(precision >= 0 && precision <= 3, "tcscf_compute_F_ochi: unknown precision");
+  TC_CHECK(precision >= 0 && precision <= 4, "tcscf_compute_F_ochi: unknown precision");
   TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F_ochi: unknown out_dtype");
   TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F_ochi: grid too large");
   TC_CHECK(u->a > 0 && u->a12 > 0, "tcscf_compute_F_ochi: need a > 0 and a12 > 0");
@@ -1691,13 +1711,13 @@ int tcscf_compute_F_tc_pair(const double* pts1, int64_t n1, const double* pts2, 
   TC_CHECK(Fs && Fo && Fs != Fo, "tcscf_compute_F_tc_pair: two distinct outputs needed");
   const double a2 = jp->a12 * jp->a12;
   const double cs = 2 * jp->c_same * jp->c_same * a2, co = 2 * jp->c_oppo * jp->c_oppo * a2;
-  if (precision == TCSCF_BF16X5 && smooth_precision == TCSCF_BF16X3) {
-    // one launch: 1/r12 at five products and the smooth part at three share
-    // the three-way split pair densities and r12; the epilogue combines
+  if ((precision == TCSCF_BF16X5 || precision == TCSCF_BF16X4) && smooth_precision == TCSCF_BF16X3) {
+    // one launch: 1/r12 at five (four) products and the smooth part at three
+    // share the three-way split pair densities and r12; the epilogue combines
     TC_CHECK(pts1 && pts2 && w2 && A2 && n1 > 0 && n2 > 0 && nb > 0, "tcscf_compute_F_tc_pair: bad arguments");
     TC_CHECK(out_dtype == TCSCF_F64 || out_dtype == TCSCF_F32, "tcscf_compute_F_tc_pair: unknown out_dtype");
     TC_CHECK(n1 < ((int64_t)1 << 31) && n2 < ((int64_t)1 << 31), "tcscf_compute_F_tc_pair: grid too large");
-    return dispatch(TCSCF_BF16X5, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, Fs, out_dtype == TCSCF_F32,
+    return dispatch(precision, pts1, n1, pts2, w2, ord2, n2, A2, nb, scale, Fs, out_dtype == TCSCF_F32,
                     KernTCPair{jp->a12, cs, co}, (hipStream_t)stream, Fo);
   }
   int rc = tcscf_compute_F(pts1, n1, pts2, w2, ord2, n2, A2, nb, TCSCF_K_COULOMB, nullptr, 1, precision, scale, Fs,
@@ -2189,8 +2209,9 @@ __global__ __launch_bounds__(256, 2) void bf16_batch_kernel(const BatchDesc* __r
     const int64_t cb = local % d.cblocks, rb = local / d.cblocks;
     const KernAny kern{d.kind, comp, d.a12, d.c2a2};
     void* Fc = d.out_f32 ? (void*)((float*)d.F + comp * d.npair) : (void*)((double*)d.F + comp * d.npair);
-    bf16_tile<MB, NPROD, 1, KernAny>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, Fc, nullptr, d.out_f32,
-                                     d.npair, d.ncomp * d.npair, d.scale, kern, rb * (16 * MB), cb * 16);
+    bf16_tile<MB, NPROD, NPROD == 4 ? 2 : 1, KernAny>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, Fc,
+                                                      nullptr, d.out_f32, d.npair, d.ncomp * d.npair, d.scale, kern,
+                                                      rb * (16 * MB), cb * 16);
   }
 }
 
@@ -2222,8 +2243,9 @@ __global__ __launch_bounds__(256, (bf16_min_blocks<KernJastrowGrad, NPROD>())) v
     const int64_t cb = local % d.cblocks, rb = local / d.cblocks;
     KernJastrowGrad kern;
     kern.a12 = d.a12;
-    bf16_tile<1, NPROD, 1, KernJastrowGrad, 4>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane, d.ncb, d.F, nullptr,
-                                               d.out_f32, d.npair, 3 * d.npair, d.scale, kern, rb * 16, cb * 16);
+    bf16_tile<1, NPROD, NPROD == 4 ? 2 : 1, KernJastrowGrad, 4>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane,
+                                                                d.ncb, d.F, nullptr, d.out_f32, d.npair, 3 * d.npair,
+                                                                d.scale, kern, rb * 16, cb * 16);
   }
 }
 
@@ -2231,7 +2253,7 @@ __global__ void zero_counter(unsigned long long* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
 }
 
-// Autotuned row-block height per product count (3 or 5), process-wide.
+// Autotuned row-block height per product count (3, 4 or 5), process-wide.
 std::mutex g_tune_mu;
 std::map<int, int> g_tuned_mb;
 
@@ -2244,7 +2266,7 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
     TC_CHECK(items[k].out_dtype == TCSCF_F64 || items[k].out_dtype == TCSCF_F32,
              "tcscf_batch_compute_F: item " + std::to_string(k) + ": unknown out_dtype");
   hipStream_t st = (hipStream_t)stream;
-  if (precision != TCSCF_BF16X3 && precision != TCSCF_BF16X5) {  // FP64 / FP32: one launch per instance
+  if (!is_bf16(precision)) {  // FP64 / FP32: one launch per instance
     for (int k = 0; k < count; ++k) {
       const tcscf_batch_item& it = items[k];
       const int rc = tcscf_compute_F(it.pts1_dev, it.n1, it.pts2_dev, it.w2_dev, it.ord2_dev, it.n2, it.A2_dev, it.nb,
@@ -2255,6 +2277,7 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
     return 0;
   }
   const int nsplit = precision == TCSCF_BF16X3 ? 2 : 3;
+  const int nprod = precision == TCSCF_BF16X3 ? 3 : (precision == TCSCF_BF16X4 ? 4 : 5);
   // workspace: per instance pair-density planes + padded coordinates, then the counter
   std::vector<size_t> off(count + 1, 0);
   for (int k = 0; k < count; ++k) {
@@ -2351,13 +2374,19 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, st, counter);
     TC_HIP(hipGetLastError());
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(t1 - t0, (int64_t)ncu * 2));
-    if (nsplit == 3) {
-      if (mb == 1) hipLaunchKernelGGL((bf16_batch_kernel<1, 5>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter);
-      else hipLaunchKernelGGL((bf16_batch_kernel<2, 5>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter);
+#define TC_BATCH(M, P) \
+  hipLaunchKernelGGL((bf16_batch_kernel<M, P>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter)
+    if (nprod == 5) {
+      if (mb == 1) TC_BATCH(1, 5);
+      else TC_BATCH(2, 5);
+    } else if (nprod == 4) {
+      if (mb == 1) TC_BATCH(1, 4);
+      else TC_BATCH(2, 4);
     } else {
-      if (mb == 1) hipLaunchKernelGGL((bf16_batch_kernel<1, 3>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter);
-      else hipLaunchKernelGGL((bf16_batch_kernel<2, 3>), dim3(grid), dim3(256), 0, st, dd, ds, nf, t0, t1, counter);
+      if (mb == 1) TC_BATCH(1, 3);
+      else TC_BATCH(2, 3);
     }
+#undef TC_BATCH
     TC_HIP(hipGetLastError());
     return 0;
   };
@@ -2369,10 +2398,14 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
     if ((rc = upload_list(vdesc, 1, true, kSlotBatchMetaV, &vtotal, &vd, &vs))) return rc;
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, st, counter + 1);
     TC_HIP(hipGetLastError());
-    const int per_cu = nsplit == 3 ? bf16_min_blocks<KernJastrowGrad, 5>() : bf16_min_blocks<KernJastrowGrad, 3>();
+    const int per_cu = nprod == 5   ? bf16_min_blocks<KernJastrowGrad, 5>()
+                       : nprod == 4 ? bf16_min_blocks<KernJastrowGrad, 4>()
+                                    : bf16_min_blocks<KernJastrowGrad, 3>();
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(vtotal, (int64_t)ncu * per_cu));
-    if (nsplit == 3)
+    if (nprod == 5)
       hipLaunchKernelGGL((bf16_batch_v_kernel<5>), dim3(grid), dim3(256), 0, st, vd, vs, nv, vtotal, counter + 1);
+    else if (nprod == 4)
+      hipLaunchKernelGGL((bf16_batch_v_kernel<4>), dim3(grid), dim3(256), 0, st, vd, vs, nv, vtotal, counter + 1);
     else
       hipLaunchKernelGGL((bf16_batch_v_kernel<3>), dim3(grid), dim3(256), 0, st, vd, vs, nv, vtotal, counter + 1);
     TC_HIP(hipGetLastError());
@@ -2385,7 +2418,7 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
   // Autotuner: the row-block height (16 or 32 rows per tile) is timed on the
   // head of the queue (the largest tiles) the first time a precision is seen
   // in this process, and the faster one is kept.
-  const int key = nsplit == 3 ? 5 : 3;
+  const int key = nprod;
   int mb_best = 0;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -2438,6 +2471,6 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
 extern "C" int tcscf_batch_tuned_rows(int precision) {
   using namespace tc;
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto f = g_tuned_mb.find(precision == TCSCF_BF16X3 ? 3 : 5);
+  auto f = g_tuned_mb.find(precision == TCSCF_BF16X3 ? 3 : (precision == TCSCF_BF16X4 ? 4 : 5));
   return f == g_tuned_mb.end() ? 0 : 16 * f->second;
 }

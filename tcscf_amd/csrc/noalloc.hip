@@ -48,7 +48,7 @@ int tcscf_compute_F_noalloc(const double* pts1, int64_t n1, const double* pts2, 
                             const int32_t* ord2, int64_t n2, const double* A2, int nb, int kernel,
                             const tcscf_jastrow* jastrow, int same_spin, int precision, double scale, void* F,
                             int out_dtype, void* stream) {
-  TC_CHECK(n2 > 0 && nb > 0 && precision >= 0 && precision <= 3, "tcscf_compute_F_noalloc: bad arguments");
+  TC_CHECK(n2 > 0 && nb > 0 && precision >= 0 && precision <= 4, "tcscf_compute_F_noalloc: bad arguments");
   const size_t need = tcscf_F_workspace_bytes(n2, nb, precision);
   if (need > kScratchBytes) {
     tc::set_error("tcscf_compute_F_noalloc: the workspace of this contraction (" + std::to_string(need) +

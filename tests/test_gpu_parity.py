@@ -157,13 +157,14 @@ def test_F_tc_pair_vs_oracle(tc, O, gpu):
             prm = O.ochi_params(1.0, jas.a12, [(1, 0, 0, jas.c_oppo, jas.c_same)])
             ref = _F_oracle(O, g1, g1, kernel=O.K_TC_SCALAR, prm=prm, same=same)
             assert rel_err(tc.unpack(F.cpu().numpy(), 9), ref) < 1e-12
-        Ms, Mo = tc.computeF_tc_pair(g1, g1, jas, precision="bf16x5", smooth_precision="bf16x3", out_dtype="fp32")
-        assert Ms.dtype == gpu.float32
-        for M, F in ((Ms, Fs), (Mo, Fo)):
-            assert rel_err(M.double().cpu().numpy(), F.cpu().numpy()) < 3e-6
+        for prec in ("bf16x5", "bf16x4"):
+            Ms, Mo = tc.computeF_tc_pair(g1, g1, jas, precision=prec, smooth_precision="bf16x3", out_dtype="fp32")
+            assert Ms.dtype == gpu.float32
+            for M, F in ((Ms, Fs), (Mo, Fo)):
+                assert rel_err(M.double().cpu().numpy(), F.cpu().numpy()) < 3e-6, prec
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("bf16x3", 3e-5), ("bf16x5", 3e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("bf16x3", 3e-5), ("bf16x4", 3e-6), ("bf16x5", 3e-6)])
 def test_F_mixed_precision_error(tc, gpu, precision, tol):
     """Mixed-precision F against the FP64 drop-in on the Ne basis (nb = 30):
     max element error relative to max |F| (tolerance in the parameters)."""
@@ -189,17 +190,18 @@ def test_F_summation_order_is_a_permutation(tc, gpu):
 
 
 def test_mixed_precision_energy_ne(tc, gpu):
-    """Ne RHF on a 50 x 194-point grid: the BF16x5 and FP32 energies lie
-    within 1e-6 Eh of the FP64 drop-in (BF16x3 is reported, not held)."""
+    """Ne RHF on a 50 x 194-point grid: the BF16x5, BF16x4 and FP32 energies
+    lie within 1e-6 Eh of the FP64 drop-in (BF16x3 is reported, not held)."""
     from tcscf_amd import configs
     c = configs.ne_config2()
     g, S, H = _scf_inputs(tc, c, (50, 23, tc.RADIAL_M4))
     e = {}
-    for prec in ("fp64", "fp32", "bf16x5", "bf16x3"):
+    for prec in ("fp64", "fp32", "bf16x5", "bf16x4", "bf16x3"):
         F = tc.computeF(g, g, precision=prec)
         e[prec] = tc.RCSHartreeFock(10, 30).compute(S, H, g, F, True, tol=1e-11, max_iter=100)
     print({k: v - e["fp64"] for k, v in e.items()})
     assert abs(e["bf16x5"] - e["fp64"]) < 1e-6
+    assert abs(e["bf16x4"] - e["fp64"]) < 1e-6
     assert abs(e["fp32"] - e["fp64"]) < 1e-6
 
 
@@ -961,7 +963,7 @@ def test_batched_contraction_matches_single_calls(tc, gpu):
     for g in (g2d, gq):
         g.setBasisFunctions(b2, False)
     specs.append(dict(r1=gq, r2=g2d, kernel="coulomb"))
-    for prec in ("bf16x5", "bf16x3"):
+    for prec in ("bf16x5", "bf16x4", "bf16x3"):
         outs = tc.computeF_batch(specs, precision=prec)
         for sp, F in zip(specs, outs):
             if sp["kernel"] == "grad":
@@ -970,7 +972,7 @@ def test_batched_contraction_matches_single_calls(tc, gpu):
                 ref = tc.computeF(sp["r1"], sp["r2"], kernel=sp["kernel"], jastrow=sp.get("jastrow"),
                                   same_spin=sp.get("same_spin", True), precision=prec)
             assert torch.equal(F, ref), (prec, sp["kernel"])
-        assert tc.lib().tcscf_batch_tuned_rows(tc.BF16X5 if prec == "bf16x5" else tc.BF16X3) in (16, 32)
+        assert tc.lib().tcscf_batch_tuned_rows(tc._lib.PRECISIONS[prec]) in (16, 32)
     # FP32 storage: the same rounding of the same sums as the single calls
     outs = tc.computeF_batch(specs, precision="bf16x5", out_dtype="fp32")
     for sp, F in zip(specs, outs):

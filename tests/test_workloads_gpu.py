@@ -107,19 +107,21 @@ def test_tc_config3_full_size(tc, O, gpu, atom):
     assert rel_err(hf.fockOperator, H[None] + np.where(mask, G2o, 0) - np.where(mask, G3o, 0)) < 1e-12
     del Fs, Fo, V
     torch.cuda.empty_cache()
-    Fs32, Fo32 = tc.computeF_tc_pair(g1, g1, jas, precision="bf16x5", smooth_precision="bf16x3", out_dtype="fp32")
     V32 = tc.computeV(g1, g3, jas, precision="bf16x3", out_dtype="fp32")
-    em = tc.TCHartreeFock(c["nup"], c["ndown"], nb).compute(S, H, g1, Fs32, Fo32, V32, jas, True, tol=1e-9,
-                                                            max_iter=200, mgs_mode=1, diis=6)
-    print(f"TC {atom}: E(fp64) = {e64:.10f}, E(mixed) = {em:.10f}, dE = {em - e64:.2e}")
-    assert abs(em - e64) < 1e-6
+    for prec in ("bf16x5", "bf16x4"):
+        Fs32, Fo32 = tc.computeF_tc_pair(g1, g1, jas, precision=prec, smooth_precision="bf16x3", out_dtype="fp32")
+        em = tc.TCHartreeFock(c["nup"], c["ndown"], nb).compute(S, H, g1, Fs32, Fo32, V32, jas, True, tol=1e-9,
+                                                                max_iter=200, mgs_mode=1, diis=6)
+        del Fs32, Fo32
+        print(f"TC {atom} ({prec} F pair): E(fp64) = {e64:.10f}, E(mixed) = {em:.10f}, dE = {em - e64:.2e}")
+        assert abs(em - e64) < 1e-6, prec
 
 
 @pytest.mark.parametrize("b", [0.5, 1.0, 2.0])
 def test_tc_qmc_ne_config4_mixed_vs_fp64(tc, gpu, b):
     """The QMC workload at full size (Ne, 40 STOs, 2**16 scrambled-Sobol points
-    per coordinate) for each Jastrow b: the BF16x5 precompute gives the FP64 TC
-    energy to 1e-6 Eh (same grids, same SCF).  With this basis the plain
+    per coordinate) for each Jastrow b: the BF16x5 and BF16x4 precomputes give
+    the FP64 TC energy to 1e-6 Eh (same grids, same SCF).  With this basis the plain
     Roothaan loop oscillates between two states on these grids (so does plain
     RHF), so the solve uses DIIS (depth 6)."""
     from tcscf_amd import configs
@@ -134,20 +136,23 @@ def test_tc_qmc_ne_config4_mixed_vs_fp64(tc, gpu, b):
     S, H = tc.one_electron_matrices(basis, c["Z"])
     jas = tc.Jastrow.from_b(b)
     E = {}
-    for prec in ("bf16x5", "fp64"):
+    for prec in ("bf16x5", "bf16x4", "fp64"):
         Fs = tc.computeF(g1, g2, kernel="tc", jastrow=jas, same_spin=True, precision=prec)
         Fo = tc.computeF(g1, g2, kernel="tc", jastrow=jas, same_spin=False, precision=prec)
         V = tc.computeV(g1, g3, jas, precision=prec)
         E[prec] = tc.TCHartreeFock(c["nup"], c["ndown"], nb).compute(S, H, g1, Fs, Fo, V, jas, True, tol=1e-9,
                                                                       mgs_mode=1, diis=6)
         del Fs, Fo, V
-    print(f"QMC Ne TC b = {b}: E(bf16x5) = {E['bf16x5']:.10f}, E(fp64) = {E['fp64']:.10f}")
+    print(f"QMC Ne TC b = {b}: E(bf16x5) = {E['bf16x5']:.10f}, E(bf16x4) = {E['bf16x4']:.10f}, "
+          f"E(fp64) = {E['fp64']:.10f}")
     assert abs(E["bf16x5"] - E["fp64"]) < 1e-6
+    assert abs(E["bf16x4"] - E["fp64"]) < 1e-6
 
 
-def test_batch64_fixture_contraction(tc, gpu):
+@pytest.mark.parametrize("prec", ["bf16x5", "bf16x4"])
+def test_batch64_fixture_contraction(tc, gpu, prec):
     """The 64-instance serving workload from its committed fixture
-    (tests/golden/batch64_instances.json): the one persistent BF16x5 launch over
+    (tests/golden/batch64_instances.json): the one persistent BF16 launch over
     every F / V of every instance equals the single-instance calls bit for bit
     (FP32 storage, as the workload stores them), and each tensor is within
     3e-6 relative of the FP64 contraction."""
@@ -159,26 +164,27 @@ def test_batch64_fixture_contraction(tc, gpu):
     prepared = batch.prepare(instances)
     specs = batch.contraction_specs(prepared)
     assert len(specs) == 32 + 32 * 3
-    outs = tc.computeF_batch(specs, precision="bf16x5", out_dtype="fp32")
+    outs = tc.computeF_batch(specs, precision=prec, out_dtype="fp32")
     worst = 0.0
     for k, (sp, F) in enumerate(zip(specs, outs)):
         if sp["kernel"] == "grad":
-            single = tc.computeV(sp["r1"], sp["r2"], sp["jastrow"], precision="bf16x5", out_dtype="fp32")
+            single = tc.computeV(sp["r1"], sp["r2"], sp["jastrow"], precision=prec, out_dtype="fp32")
             ref = tc.computeV(sp["r1"], sp["r2"], sp["jastrow"])
         else:
             kw = dict(kernel=sp["kernel"], jastrow=sp.get("jastrow"), same_spin=sp.get("same_spin", True))
-            single = tc.computeF(sp["r1"], sp["r2"], precision="bf16x5", out_dtype="fp32", **kw)
+            single = tc.computeF(sp["r1"], sp["r2"], precision=prec, out_dtype="fp32", **kw)
             ref = tc.computeF(sp["r1"], sp["r2"], **kw)
         assert torch.equal(F, single), k
         err = float((F.double() - ref).abs().max() / ref.abs().max())
         worst = max(worst, err)
         assert err < 3e-6, (k, err)
-    print(f"batch64: worst relative tensor error vs FP64 {worst:.2e}")
+    print(f"batch64 ({prec}): worst relative tensor error vs FP64 {worst:.2e}")
 
 
-def test_batch_accurate_policy_subset(tc, gpu):
+@pytest.mark.parametrize("prec", ["bf16x5", "bf16x4"])
+def test_batch_accurate_policy_subset(tc, gpu, prec):
     """The batch's 'accurate' policy on a subset of the fixture (the three
-    lightest and the three heaviest instances): the light ones stay BF16x5 and
+    lightest and the three heaviest instances): the light ones stay BF16 and
     are within 1e-6 Eh of FP64, the heavy ones (Z >= FP64_FROM_Z) contract in
     FP64 and equal the FP64 run bitwise."""
     import torch
@@ -188,7 +194,7 @@ def test_batch_accurate_policy_subset(tc, gpu):
     by_z = sorted(instances, key=lambda x: (x["Z"], x["npts"]))
     sub = by_z[:3] + by_z[-3:]
     prep = batch.prepare(sub)
-    res_a, t_a = batch.run(prep, "bf16x5", synchronize=torch.cuda.synchronize, policy="accurate")
+    res_a, t_a = batch.run(prep, prec, synchronize=torch.cuda.synchronize, policy="accurate")
     res_d, _ = batch.run(prep, "fp64", synchronize=torch.cuda.synchronize)
     assert t_a["fp64_instances"] == sum(x["Z"] >= batch.FP64_FROM_Z for x in sub) == 3
     for x, a, d in zip(sub, res_a, res_d):

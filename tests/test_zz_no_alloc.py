@@ -41,7 +41,8 @@ def test_compute_F_noalloc_matches_compute_F(tc, gpu):
     small.setBasisFunctions(basis_s, False)
     jas = tc.Jastrow.from_b(1.0)
     cases = [("bf16x5", "fp64", _lib.K_COULOMB, None, True), ("fp64", "fp64", _lib.K_COULOMB, None, True),
-             ("bf16x3", "fp32", _lib.K_TC_SCALAR, jas, False), ("fp32", "fp64", _lib.K_JASTROW_GRAD, jas, True)]
+             ("bf16x3", "fp32", _lib.K_TC_SCALAR, jas, False), ("fp32", "fp64", _lib.K_JASTROW_GRAD, jas, True),
+             ("bf16x4", "fp32", _lib.K_COULOMB, None, True)]
     kname = {_lib.K_COULOMB: "coulomb", _lib.K_TC_SCALAR: "tc", _lib.K_JASTROW_GRAD: "grad"}
     # set-up: the first call on the device looks the array up (the runtime maps it)
     warm = torch.empty((small.nGrid(), tc.npair(small.nBasis())), dtype=torch.float64, device="cuda")
