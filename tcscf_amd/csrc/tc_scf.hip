@@ -561,6 +561,7 @@ __global__ __launch_bounds__(256) void tc_pack_cols(const double* __restrict__ C
 // U) and Sc[NS][4] (occ, x . g_c).
 static_assert(kDwStride == 576, "");
 constexpr int kRowStride = kDwStride;  // staged row stride (elements): 9 loads of 64 lanes, >= npair at nb = 32
+static_assert(31 * 32 / 2 + 31 < kRowStride, "packed (i, j) for i, j < 32 lies inside a staged row");
 template <int NOC, bool GV>
 struct ColShape {
   static constexpr int NT = GV ? 2 : 1;
@@ -768,19 +769,34 @@ __global__ __launch_bounds__(256, 2) void tc_point_col_kernel(ColArgs a, const d
 #pragma unroll
         for (int col = 0; col < NOC; ++col) acc[k][col] = 0;
       const ST* lr = rows + rbase;
-#pragma unroll 2
-      for (int j = 0; j < nb; ++j) {
-        const int pj = j <= ic ? Ti + j : j * (j + 1) / 2 + ic;
-        double m[K];
+      // JR j per round: the lane's row elements and the C rows (LDS, the C
+      // rows as broadcast reads: only LDS reads in flight, so each FMA waits
+      // for its own operands) of a round issued before its FMAs.  A round
+      // never passes j = 31 (nb <= 32); j past nb takes the zero rows of Cb
+      // (its row elements, packed index <= 527, lie inside the staged rows)
+      constexpr int JR = NOC <= 5 ? 4 : (NOC <= 8 ? 2 : 1);
+#pragma unroll 1
+      for (int j0 = 0; j0 < nb; j0 += JR) {
+        const int jb = __builtin_amdgcn_readfirstlane(j0);
+        double m[JR][K], c[JR][NOC];
 #pragma unroll
-        for (int k = 0; k < K; ++k) m[k] = (double)lr[k * RS + pj];
-        const double* cj = Cc + (int64_t)j * NOC;
+        for (int u = 0; u < JR; ++u) {
+          const int j = jb + u;
+          // packed (i, j): Ti + j for j <= i, else T(j) + i (a select, no branch)
+          const int below = -(int)(j <= ic);
+          const int pj = ((Ti + j) & below) | ((((j * (j + 1)) >> 1) + ic) & ~below);
 #pragma unroll
-        for (int col = 0; col < NOC; ++col) {
-          const double c = cj[col];
+          for (int k = 0; k < K; ++k) m[u][k] = (double)lr[k * RS + pj];
+          const double* cj = Cb + j * NOC;
 #pragma unroll
-          for (int k = 0; k < K; ++k) acc[k][col] += m[k] * c;
+          for (int col = 0; col < NOC; ++col) c[u][col] = cj[col];
         }
+#pragma unroll
+        for (int u = 0; u < JR; ++u)
+#pragma unroll
+          for (int col = 0; col < NOC; ++col)
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[k][col] += m[u][k] * c[u][col];
       }
       // row i of Wt: the W^t_c columns from the slots that hold them, half 0
       // also Fs C, A and dA
