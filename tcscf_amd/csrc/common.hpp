@@ -86,7 +86,8 @@ enum ScratchSlot {
   kSlotBatchMeta = 5,  // + 0 / 1: batch descriptors for the two tile heights
   kSlotQmc = 7,        // tcscf_eri_qmc6d: the per-point pair densities / r12 and the contraction
   kSlotBatchMetaV = 8, // batch descriptors of the three-component (V) tiles
-  kNumSlots = 9
+  kSlotSplit = 9,      // partial sums of the pipelined BF16 kernel's split last round
+  kNumSlots = 10
 };
 
 }  // namespace tc

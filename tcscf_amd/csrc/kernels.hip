@@ -976,7 +976,7 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
                                                const us8* __restrict__ P0, int64_t plane, int64_t ncb,
                                                void* __restrict__ Fout, void* __restrict__ Fout2, int out_f32,
                                                int64_t npair, int64_t ld_row, double scale, Kern kern, int64_t row0,
-                                               int64_t cbw) {
+                                               int64_t cbw, int64_t sb, int64_t ns, double* __restrict__ part) {
   constexpr int NC = Kern::ncomp;
   constexpr int C5 = NPROD >= 4 ? Comp5<Kern>::value : 0;  // components with NPROD (4 or 5) products
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
@@ -1015,7 +1015,7 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
   const int voff = (int)(cb0 * 1024) + lane * 16;
   const int plane_bytes = (int)(plane * 16);
   auto load_b = [&](us8 (&dst)[NSPLIT][NB], int64_t s) {
-    const int so = __builtin_amdgcn_readfirstlane((int)(s * ncb * 1024));
+    const int so = __builtin_amdgcn_readfirstlane((int)((sb + s) * ncb * 1024));
 #pragma unroll
     for (int t = 0; t < NSPLIT; ++t)
 #pragma unroll
@@ -1029,7 +1029,7 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
   const int coff = (lane & 31) * 16;
   u4 cnext;
   auto fetch_coords = [&](int64_t t) {
-    cnext = __builtin_amdgcn_raw_buffer_load_b128(cv, coff, __builtin_amdgcn_readfirstlane((int)(t * 512)), 0);
+    cnext = __builtin_amdgcn_raw_buffer_load_b128(cv, coff, __builtin_amdgcn_readfirstlane((int)((sb + t) * 512)), 0);
   };
   auto store_coords = [&](int64_t t) { crd[t & 1][lane & 31] = __builtin_bit_cast(float4, cnext); };
   // K = hi + lo (bf16, round to nearest even) of every component at points
@@ -1075,7 +1075,7 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
 #pragma unroll
         for (int i = 0; i < 4; ++i) sum[c][m][n][i] = 0;
   f4 acc[NC][MB][NB];
-  const int64_t nsteps = n2p / 32, last = nsteps - 1;
+  const int64_t nsteps = ns, last = nsteps - 1;  // steps sb .. sb + ns - 1 of r2
   us8 bqA[NSPLIT][NB], bqB[NSPLIT][NB];  // B fragments, ping-pong between steps
   // prologue: coordinates of steps 0 and 1 in the ring, step 2's in cnext,
   // K(0) in ktile[0], B(0) in bqA
@@ -1221,6 +1221,20 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
       for (int n = 0; n < NB; ++n)
 #pragma unroll
         for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
+  if (part) {
+    // a piece of a split tile: the raw FP64 sums, [c][16 MB rows][16 NW NB cols]
+    constexpr int TR = 16 * MB, TC = 16 * NW * NB;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            part[((int64_t)c * TR + 16 * m + d_row_bf16(g, i)) * TC + 16 * (wave * NB + n) + r] = sum[c][m][n][i];
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < MB; ++m)
 #pragma unroll
@@ -1258,16 +1272,107 @@ __device__ __forceinline__ void bf16_tile_pipe(us8 (*ktile)[Kern::ncomp][MB][2][
 }
 
 // NW waves per workgroup (256 threads: two workgroups per CU; 512: one)
+// Workgroup t = blockIdx.x is tile t: column group t % gx, row block t / gx.
 template <int MB, int NPROD, class Kern, int NB, int NW, int FOLD = 1, bool REV = false>
 __global__ __launch_bounds__(64 * NW, 8 / NW) void fgemm_bf16_pipe_kernel(
     const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
     const us8* __restrict__ P0, int64_t plane, int64_t ncb, void* __restrict__ Fout, void* __restrict__ Fout2,
-    int out_f32, int64_t npair, double scale, Kern kern) {
+    int out_f32, int64_t npair, double scale, Kern kern, int64_t gx) {
   __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
   __shared__ float4 crd[2][32];
-  bf16_tile_pipe<MB, NPROD, Kern, NB, NW, FOLD, REV>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32, npair,
-                                          Combines<Kern>::value ? npair : Kern::ncomp * npair, scale, kern,
-                                          (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * (NW * NB));
+  const int64_t t = blockIdx.x;
+  bf16_tile_pipe<MB, NPROD, Kern, NB, NW, FOLD, REV>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32,
+                                                     npair, Combines<Kern>::value ? npair : Kern::ncomp * npair,
+                                                     scale, kern, (t / gx) * (16 * MB), (t % gx) * (NW * NB), 0,
+                                                     n2p / 32, nullptr);
+}
+
+// Split tail: the tiles t0 .. t0 + W / nst - 1 (the last, partly filled round of
+// a launch) cut into gridDim.x runs of r2 steps, one per workgroup, over the
+// tail's steps in tile order.  Run w covers tail steps [b(w), b(w + 1)), b(w)
+// = w W / gridDim.x moved down to an even step of its tile, so each piece's
+// two-step FP32 partials pair the steps as the unsplit tile does.  A run is
+// at most one tile's worth, so at most two pieces (the end of one tile, the
+// start of the next); piece k of run w leaves its raw FP64 sums in part slot
+// 2 w + k for bf16_pipe_fixup.
+__host__ __device__ inline int64_t split_bound(int64_t w, int64_t W, int64_t S, int64_t nst) {
+  const int64_t b = w * W / S, t0 = b / nst * nst;
+  return t0 + (b - t0) / 2 * 2;
+}
+
+template <int MB, int NPROD, class Kern, int NB, int NW, int FOLD = 1, bool REV = false>
+__global__ __launch_bounds__(64 * NW, 8 / NW) void fgemm_bf16_pipe_tail_kernel(
+    const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
+    const us8* __restrict__ P0, int64_t plane, int64_t ncb, int64_t npair, Kern kern, int64_t gx, int64_t t0,
+    int64_t W, double* __restrict__ part) {
+  __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
+  __shared__ float4 crd[2][32];
+  constexpr int64_t TE = (int64_t)Kern::ncomp * 16 * MB * 16 * NW * NB;  // doubles per piece
+  const int64_t nst = n2p / 32, S = gridDim.x, w = blockIdx.x;
+  const int64_t g1 = split_bound(w + 1, W, S, nst);
+  int k = 0;
+  for (int64_t g = split_bound(w, W, S, nst); g < g1 && k < 2; ++k) {
+    const int64_t tr = g / nst, s0 = g - tr * nst, s1 = g1 - tr * nst < nst ? g1 - tr * nst : nst;
+    const int64_t t = t0 + tr;
+    if (k) __syncthreads();  // the previous piece's LDS reads are done
+    bf16_tile_pipe<MB, NPROD, Kern, NB, NW, FOLD, REV>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, nullptr, nullptr,
+                                                       0, npair, 0, 1.0, kern, (t / gx) * (16 * MB),
+                                                       (t % gx) * (NW * NB), s0, s1 - s0, part + (2 * w + k) * TE);
+    g = tr * nst + s1;
+  }
+}
+
+// The split tiles' outputs: per element the pieces' FP64 sums added in step
+// order, then the epilogue of bf16_tile_pipe (scale, combine, storage type).
+// Grid: (element chunks of 256, tail tiles).
+template <class Kern, int TR, int TC>
+__global__ __launch_bounds__(256) void bf16_pipe_fixup(const double* __restrict__ part, int64_t n1, int64_t npair,
+                                                       int64_t gx, int64_t t0, int64_t nst, int64_t W, int64_t S,
+                                                       void* __restrict__ Fout, void* __restrict__ Fout2, int out_f32,
+                                                       double scale, Kern kern) {
+  constexpr int NC = Kern::ncomp;
+  constexpr int64_t TE = (int64_t)NC * TR * TC;
+  const int64_t tr = blockIdx.y, t = t0 + tr;
+  const int e = blockIdx.x * 256 + threadIdx.x;  // element (row_local, col_local)
+  if (e >= TR * TC) return;
+  const int rl = e / TC, cl = e - rl * TC;
+  const int64_t row = (t / gx) * TR + rl, col = (t % gx) * TC + cl;
+  if (row >= n1 || col >= npair) return;
+  // the runs that cover tile tr: from the first w with b(w + 1) > tr nst
+  const int64_t lo = tr * nst, hi = lo + nst;
+  int64_t w = lo * S / W;
+  while (w > 0 && split_bound(w, W, S, nst) > lo) --w;
+  while (split_bound(w + 1, W, S, nst) <= lo) ++w;
+  double v[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) v[c] = 0;
+  for (; w < S && split_bound(w, W, S, nst) < hi; ++w) {
+    const int64_t k = tr - split_bound(w, W, S, nst) / nst;  // piece of run w that lies in tile tr
+    const double* pc = part + (2 * w + k) * TE;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] += pc[((int64_t)c * TR + rl) * TC + cl];
+  }
+  if constexpr (Combines<Kern>::value) {
+    double x[NC], o[2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = scale * v[c];
+    kern.finish(x, o);
+    const int64_t e2 = row * npair + col;
+    if (out_f32) {
+      ((float*)Fout)[e2] = (float)o[0];
+      ((float*)Fout2)[e2] = (float)o[1];
+    } else {
+      ((double*)Fout)[e2] = o[0];
+      ((double*)Fout2)[e2] = o[1];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int64_t o = row * (NC * npair) + c * npair + col;
+      if (out_f32) ((float*)Fout)[o] = (float)(scale * v[c]);
+      else ((double*)Fout)[o] = scale * v[c];
+    }
+  }
 }
 
 // Two workgroups per CU (a 256-VGPR budget) where the kernel fits it without
@@ -1365,14 +1470,68 @@ int bf16_pipe_mode(int ncomp) {
   return ncomp == 1 ? 2 : (ncomp == 2 ? 1 : 0);
 }
 
+// Set while a caller-provided workspace is in use (the no-allocation entry
+// point): the split tail's partial sums would need library scratch.
+thread_local bool t_fixed_ws = false;
+
+// Resident workgroups of kernel fn on the current device (CUs x workgroups per
+// CU), kept per (device, kernel).
+int resident_slots(const void* fn, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int> slots;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  auto f = slots.find({dev, fn});
+  if (f != slots.end()) return f->second;
+  int ncu = 0, per = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, 0) != hipSuccess)
+    return 0;
+  return slots[{dev, fn}] = ncu * per;
+}
+
+// The launch's tiles run in rounds of the resident slots; a last round of r
+// tiles (r < slots) leaves slots - r of them idle for a whole tile.  Unless
+// TCSCF_BF16_SPLIT=0, that round is split: the full rounds run as usual, the
+// last r tiles are cut along r2 into one equal run per slot (at most two
+// pieces each, partial sums in library scratch) and a fix-up pass adds each
+// tile's pieces in step order.  The split tiles' FP64 sums then associate
+// differently from an unsplit tile's (last-bit differences); the tiles of the
+// full rounds are unchanged.
 template <int MB, int NB, int NW, int NPROD, class Kern, int FOLD = 1, bool REV = false>
 int launch_bf16_pipe(const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0, int64_t plane,
                      int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern, hipStream_t st,
                      void* F2) {
-  dim3 grid((unsigned)(ncb / (NW * NB)), (unsigned)((n1 + 16 * MB - 1) / (16 * MB)));
-  hipLaunchKernelGGL((fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW, FOLD, REV>), grid, dim3(64 * NW), 0, st, p1, n1,
-                     x2, n2p, P0, plane, ncb, F, F2, out_f32, npair, scale, kern);
-  TC_HIP(hipGetLastError());
+  constexpr int NC = Kern::ncomp, TR = 16 * MB, TC = 16 * NW * NB;
+  const int64_t gx = ncb / (NW * NB), gy = (n1 + TR - 1) / TR, tiles = gx * gy, nst = n2p / 32;
+  const void* fn = (const void*)fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW, FOLD, REV>;
+  const char* env = std::getenv("TCSCF_BF16_SPLIT");
+  const int64_t S = (env && std::atoi(env) == 0) || t_fixed_ws ? 0 : resident_slots(fn, 64 * NW);
+  const int64_t r = S ? tiles % S : 0, full = tiles - r;
+  // split when the last round is at most 90% full and has at least four steps per slot
+  const bool split = S && tiles > S && r > 0 && r * 10 <= S * 9 && r * nst >= 4 * S;
+  double* part = nullptr;
+  if (split) {
+    int rc = 0;
+    part = (double*)scratch(kSlotSplit, (size_t)(2 * S) * NC * TR * TC * sizeof(double), &rc);
+    if (rc) return rc;
+  }
+  const int64_t main_tiles = split ? full : tiles;
+  if (main_tiles > 0) {
+    hipLaunchKernelGGL((fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW, FOLD, REV>), dim3((unsigned)main_tiles),
+                       dim3(64 * NW), 0, st, p1, n1, x2, n2p, P0, plane, ncb, F, F2, out_f32, npair, scale, kern, gx);
+    TC_HIP(hipGetLastError());
+  }
+  if (split) {
+    const int64_t W = r * nst;
+    hipLaunchKernelGGL((fgemm_bf16_pipe_tail_kernel<MB, NPROD, Kern, NB, NW, FOLD, REV>), dim3((unsigned)S),
+                       dim3(64 * NW), 0, st, p1, n1, x2, n2p, P0, plane, ncb, npair, kern, gx, full, W, part);
+    TC_HIP(hipGetLastError());
+    hipLaunchKernelGGL((bf16_pipe_fixup<Kern, TR, TC>), dim3((unsigned)((TR * TC + 255) / 256), (unsigned)r),
+                       dim3(256), 0, st, part, n1, npair, gx, full, nst, W, S, F, F2, out_f32, scale, kern);
+    TC_HIP(hipGetLastError());
+  }
   return 0;
 }
 // Accumulation scheme of the Coulomb kernel (A/B): 0 = FP64 fold every step,
@@ -1510,6 +1669,11 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
              void* F2 = nullptr, char* ws_fixed = nullptr) {
   const int64_t npair = npair_of(nb);
   int rc = 0;
+  struct FixedWs {  // a caller-provided workspace: no other scratch either
+    bool prev;
+    explicit FixedWs(bool on) : prev(t_fixed_ws) { t_fixed_ws = on; }
+    ~FixedWs() { t_fixed_ws = prev; }
+  } fixed_guard(ws_fixed != nullptr);
   if (is_bf16(precision)) {
     const int nsplit = precision == TCSCF_BF16X3 ? 2 : 3;
     const int64_t n2p = round_up(n2, 32), ncb = round_up(npair, 256) / 16, nkb = n2p / 32;

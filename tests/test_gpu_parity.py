@@ -178,6 +178,49 @@ def test_F_mixed_precision_error(tc, gpu, precision, tol):
     assert err < tol, err
 
 
+def test_F_split_last_round_matches_unsplit(tc, gpu, monkeypatch):
+    """The pipelined BF16 kernel splits the last, partly filled round of its
+    tiles along r2 (one run of steps per resident workgroup, pieces added by
+    a fix-up pass).  On the full Ne grid (944 tiles of 64 rows: 3 full rounds
+    and 176 split tiles at 256 resident workgroups) the results equal the
+    unsplit launch up to the FP64 association of the pieces' sums: the split
+    pieces start on even steps, so the two-step FP32 partials of BF16x4 pair
+    the steps exactly as the unsplit tiles do.  Tolerances in the asserts:
+    1e-13 of max |F| for FP64 storage, one FP32 rounding for FP32 storage;
+    and the RHF energy to 1e-10 Eh."""
+    import torch
+    from tcscf_amd import configs
+    c = configs.ne_config2()
+    nrad, order, kind = c["grid"]
+    g = tc.QMCGrid.radial_lebedev(nrad, order, kind, 1.0, 1.0)
+    g.setBasisFunctions(c["basis"], False)
+    jas = tc.Jastrow.from_b(1.0)
+    cases = [("bf16x4", "fp64"), ("bf16x5", "fp64"), ("bf16x4", "fp32")]
+    for prec, odt in cases:
+        monkeypatch.setenv("TCSCF_BF16_SPLIT", "0")
+        a = tc.computeF(g, g, precision=prec, out_dtype=odt)
+        monkeypatch.setenv("TCSCF_BF16_SPLIT", "1")
+        b = tc.computeF(g, g, precision=prec, out_dtype=odt)
+        torch.cuda.synchronize()
+        scale = a.double().abs().max().item()
+        d = (a.double() - b.double()).abs().max().item() / scale
+        print(f"{prec} {odt}: max diff {d:.2e} of max |F|, {(a != b).sum().item()} elements differ")
+        assert d < (1e-13 if odt == "fp64" else 6e-8), (prec, odt, d)
+    monkeypatch.setenv("TCSCF_BF16_SPLIT", "0")
+    Pa = tc.computeF_tc_pair(g, g, jas, precision="bf16x4", smooth_precision="bf16x3")
+    monkeypatch.setenv("TCSCF_BF16_SPLIT", "1")
+    Pb = tc.computeF_tc_pair(g, g, jas, precision="bf16x4", smooth_precision="bf16x3")
+    for x, y in zip(Pa, Pb):
+        assert ((x - y).abs().max() / x.abs().max()).item() < 1e-13
+    S, H = tc.one_electron_matrices(c["basis"], c["Z"])
+    e = []
+    for sp in ("0", "1"):
+        monkeypatch.setenv("TCSCF_BF16_SPLIT", sp)
+        F = tc.computeF(g, g, precision="bf16x4")
+        e.append(tc.RCSHartreeFock(10, 30).compute(S, H, g, F, True, tol=1e-10, max_iter=100))
+    assert abs(e[0] - e[1]) < 1e-10, e
+
+
 def test_F_summation_order_is_a_permutation(tc, gpu):
     """Summing r2 in radius order or in grid order gives the same FP64 F."""
     from tcscf_amd import configs
