@@ -221,6 +221,7 @@ struct TcPlan {
   double* dCc;
   int ustart[7];  // rank workgroups per target (prefix offsets)
   int rchunk;     // (point, entry) terms per rank workgroup
+  int rpts[6];    // points per rank workgroup of each target
 };
 size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1, bool general_v);
 size_t tc_out_doubles(int nb);  // the per-build result block copied back to the host

@@ -1051,7 +1051,7 @@ struct RankArgs {
   int nb, nvec, nocc0, nocc1, nmax, nwt;  // nwt: V tensors with W blocks (1 or 2)
   int ustart[7];        // workgroups of target 3 s + t: [ustart[st], ustart[st + 1])
   double* part;         // [gridDim.x][nb][nb]
-  int chunk;            // (point, entry) terms per workgroup
+  int pts[6];           // points per workgroup of target st (a multiple of 4)
 };
 
 // (point, entry) terms per point of target st = 3 s + t: 1, 4, 5 + 3 nwt nocc_s.
@@ -1061,15 +1061,18 @@ __host__ __device__ inline int rank_entries(int st, int nocc0, int nocc1, int nw
 }
 
 // Pass 2: G_{s,t}[j][i] = sum over (point, entry) of L[j] R[i] (the rank-one
-// terms listed above) as an FP64 MFMA product, blockIdx.y = 3 s + t.  K runs
-// over (point, entry) pairs of the block's points; the 4 waves take
-// interleaved 4-wide k-steps over all NT x NT output tiles (16x16 each) and
-// are summed in a fixed order through LDS.  v_mfma_f64_16x16x4: A[j][k] from
-// lane 16 g + r = L_{k0+g}[16 m + r], B[k][i] = R_{k0+g}[16 t + r].
+// terms listed above) as an FP64 MFMA product, blockIdx.y = 3 s + t.  A
+// workgroup takes pts[st] consecutive points of its target; one k-step is
+// one entry of four consecutive points (lane group g: point 4 q + g), so the
+// entry -- which vectors, offsets and scale -- is uniform across the wave and
+// decoded in scalar registers.  The 4 waves take interleaved k-steps (entry
+// fastest) over all NT x NT output tiles (16x16 each) and are summed in a
+// fixed order through LDS.  v_mfma_f64_16x16x4: A[j][k] from lane 16 g + r =
+// L_{k0+g}[16 m + r], B[k][i] = R_{k0+g}[16 t + r].
 template <int NT>
 __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
   __shared__ double red[NT * 16][NT * 16];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane >> 4, r = lane & 15;
   const int u = blockIdx.x;
   int st = 0;
@@ -1078,36 +1081,34 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
   const int nb = a.nb;
   const int E = rank_entries(st, a.nocc0, a.nocc1, a.nwt);
   const int w3 = 3 * (s ? a.nocc1 : a.nocc0);  // W entries per tensor
-  // this workgroup's terms: k = kb + [0, K) of the target's n1 E (point, entry) terms
-  const int64_t kb = (int64_t)(u - a.ustart[st]) * a.chunk;
-  const int K = (int)(a.n1 * E - kb < a.chunk ? a.n1 * E - kb : a.chunk);
-  const int64_t p0 = kb / E;
-  const int e0 = (int)(kb - p0 * E);
-  const float invE = 1.0f / (float)E;
-  // Operands of the k-step at k0 (lane (g, r): L_{k0+g}[16 m + r] and
-  // R_{k0+g}[16 m + r]) in two halves: fetch issues the raw loads into a ring
-  // slot (no arithmetic on loaded values, so nothing waits for them there);
-  // use scales / masks them just before the MFMAs, three steps later.
+  // this workgroup's points pb .. pb + np - 1, as nq quads
+  const int P = a.pts[st];
+  const int64_t pb = (int64_t)(u - a.ustart[st]) * P;
+  const int np = (int)(a.n1 - pb < P ? a.n1 - pb : P);
+  const int nq = (np + 3) / 4;
+  const double* vs = a.vecs + s * a.nvec * nb;  // spin s of point 0
+  // Operands of a k-step (lane (g, r): L[16 m + r] and R[16 m + r] of the
+  // step's entry at point 4 q + g) in two halves: fetch issues the raw loads
+  // into a ring slot (no arithmetic on loaded values, so nothing waits for
+  // them there); use scales / masks them just before the MFMAs, three steps
+  // later.  The fetch cursor (fq, fe) walks this wave's k-steps wave + 4 j.
   struct Slot {
     double l[NT], r[NT];
     double w, al;
-    int sk;  // scale: 0 none, 1 -w, 2 2w, 3 alpha, 4 zero (past K)
+    int sk;  // scale: 0 none, 1 -w, 2 2w, 3 alpha, 4 zero (past the points)
   };
-  auto fetch = [&](int k0, Slot& o) {
-    const int k = k0 + g, kk = e0 + k;
-    const bool kin = k < K;
-    int q = (int)(((float)kk + 0.5f) * invE), e = kk - q * E;  // kk = q E + e
-    if (e < 0) {
-      --q;
-      e += E;
-    } else if (e >= E) {
-      ++q;
-      e -= E;
+  int fq = wave / E, fe = wave - (wave / E) * E;
+  auto fetch = [&](Slot& o) {
+    const int q = fq, e = fe;
+    fe += 4;
+    while (fe >= E) {
+      fe -= E;
+      ++fq;
     }
-    const int64_t n = kin ? p0 + q : p0;
-    if (!kin) e = 0;
-    // entry e of target tg: L / R source (0 the point's vectors, 1 A, 2 dA),
-    // offset and stride, and the scale (W entries of tensor wt use alpha_wt)
+    const bool in = 4 * q + g < np;
+    const int64_t n = pb + (in ? 4 * q + g : 0);
+    // entry e of target tg (uniform): L / R source (0 the point's vectors, 1 A,
+    // 2 dA), offset and stride, and the scale (W entries of tensor wt use alpha_wt)
     int ls = 1, lo = 0, rs = 0, ro = 0, rst = 1, sk = 0, wt = 0;
     if (tg == 1) {
       const bool z = e == 0;
@@ -1127,14 +1128,11 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
       ro = z ? 2 * nb : (one ? 0 : lo);
       sk = z || one ? 0 : (tc3 ? 2 : 3);
     }  // tg == 0: (A, u0)
-    const double* vn = a.vecs + (n * 2 + s) * a.nvec * nb;
-    const double* An = a.A + n * nb;
-    const double* dAn = a.dA + n * nb * 3;
-    const double* L = (ls == 0 ? vn : An) + lo;
-    const double* R = (rs == 0 ? vn : (rs == 1 ? An : dAn)) + ro;
+    const double* L = (ls == 0 ? vs + n * 2 * a.nvec * nb : a.A + n * nb) + lo;
+    const double* R = (rs == 0 ? vs + n * 2 * a.nvec * nb : (rs == 1 ? a.A + n * nb : a.dA + n * nb * 3)) + ro;
     o.w = a.w[n];
     o.al = a.alpha[(n * 2 + s) * 2 + wt];
-    o.sk = kin ? sk : 4;
+    o.sk = in && q < nq ? sk : 4;
 #pragma unroll
     for (int m = 0; m < NT; ++m) {
       const int row = 16 * m + r < nb ? 16 * m + r : nb - 1;  // in range; masked in use
@@ -1162,17 +1160,17 @@ __global__ __launch_bounds__(256) void tc_rank_kernel(RankArgs a) {
       for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[t], acc[m][t], 0, 0, 0);
   };
   // a ring of three slots: loads run three k-steps ahead of their MFMAs (steps
-  // past K load in-range data that use() zeroes)
+  // past the points load in-range data that use() zeroes)
   Slot s0, s1, s2;
-  const int kw = 4 * wave;
-  fetch(kw, s0);
-  fetch(kw + 16, s1);
-  for (int k0 = kw; k0 < K; k0 += 48) {
-    fetch(k0 + 32, s2);
+  const int KS = nq * E;  // k-steps of the workgroup
+  fetch(s0);
+  fetch(s1);
+  for (int k0 = wave; k0 < KS; k0 += 12) {
+    fetch(s2);
     use(s0);
-    fetch(k0 + 48, s0);
+    fetch(s0);
     use(s1);
-    fetch(k0 + 64, s1);
+    fetch(s1);
     use(s2);
   }
   // fixed-order sum of the 4 waves' tiles
@@ -1360,19 +1358,24 @@ static int rank_chunk(int nup, int ndown, int nwt, int nb, int64_t n1) {
   }
 }
 
-// Rank workgroups per target st = 3 s + t (chunk terms each), as prefix
-// offsets ustart[0..6].
-static void rank_units(int nup, int ndown, int nwt, int64_t n1, int chunk, int* ustart) {
+// Rank workgroups per target st = 3 s + t, as prefix offsets ustart[0..6],
+// and the points per workgroup of each target (about chunk terms, a
+// multiple of 4 points).
+static void rank_units(int nup, int ndown, int nwt, int64_t n1, int chunk, int* ustart, int* pts) {
   ustart[0] = 0;
-  for (int st = 0; st < 6; ++st)
-    ustart[st + 1] = ustart[st] + (int)((n1 * rank_entries(st, nup, ndown, nwt) + chunk - 1) / chunk);
+  for (int st = 0; st < 6; ++st) {
+    const int E = rank_entries(st, nup, ndown, nwt);
+    const int P = std::max(4, (chunk / E + 3) / 4 * 4);
+    if (pts) pts[st] = P;
+    ustart[st + 1] = ustart[st] + (int)((n1 + P - 1) / P);
+  }
 }
 
 size_t tc_ws_doubles(int nup, int ndown, int nb, int64_t n1, bool general_v) {
   const int nmax = std::max(1, std::max(nup, ndown)), nwt = general_v ? 2 : 1;
   const size_t nb2 = (size_t)nb * nb, npair = (size_t)nb * (nb + 1) / 2, nvec = kVecFixed + 3 * nmax * nwt;
   int us[7];
-  rank_units(nup, ndown, nwt, n1, rank_chunk(nup, ndown, nwt, nb, n1), us);
+  rank_units(nup, ndown, nwt, n1, rank_chunk(nup, ndown, nwt, nb, n1), us, nullptr);
   return 2 * nb2 + 2 * (size_t)nb * nmax + (size_t)n1 * 2 * nvec * nb + (size_t)n1 * 4 + (size_t)n1 * 12 +
          (size_t)us[6] * nb2 + (size_t)kBlocks * 2 * npair + tc_out_doubles(nb) + 32 * 16 + 2 * kDwStride;
 }
@@ -1415,7 +1418,7 @@ int tc_prepare(TcPlan& p, int nup, int ndown, int nb, const double* A1, const do
   p.dH = p.dAl + n1 * 4;
   p.dPG = p.dH + n1 * 12;
   p.rchunk = rank_chunk(nup, ndown, p.nwt, nb, n1);
-  rank_units(nup, ndown, p.nwt, n1, p.rchunk, p.ustart);
+  rank_units(nup, ndown, p.nwt, n1, p.rchunk, p.ustart, p.rpts);
   p.dPV = p.dPG + (int64_t)p.ustart[6] * nb2;
   p.dOut = p.dPV + (int64_t)kBlocks * 2 * p.npair;
   p.dCc = p.dOut + tc_out_doubles(nb);
@@ -1529,10 +1532,10 @@ static int tc_build(const TcPlan& p, bool restricted, bool col, hipStream_t st, 
     TC_HIP(hipLaunchKernel(point_kernel(p.f32, p.stage[r], restricted, p.gv), dim3((unsigned)p.pblocks[r]),
                            dim3(64 * p.wpw[r]), pargs, p.sh1[r], st));
   }
-  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.nmax, p.nwt, {}, p.dPG,
-              p.rchunk};
+  RankArgs ra{p.dVec, p.dAl, p.A1, p.dA1, p.w1, p.n1, p.nb, p.nvec, p.nup, p.ndown, p.nmax, p.nwt, {}, p.dPG, {}};
   RankRanges rr{};
   for (int q = 0; q < 7; ++q) ra.ustart[q] = rr.ustart[q] = p.ustart[q];
+  for (int q = 0; q < 6; ++q) ra.pts[q] = p.rpts[q];
   const dim3 rgrid((unsigned)p.ustart[nst]);
   if (p.nb <= 32)
     hipLaunchKernelGGL(tc_rank_kernel<2>, rgrid, dim3(256), 0, st, ra);
