@@ -713,7 +713,8 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
                                           const us8* __restrict__ P0, int64_t plane, int64_t ncb,
                                           void* __restrict__ Fout, void* __restrict__ Fout2, int out_f32,
                                           int64_t npair, int64_t ld_row, double scale, Kern kern, int64_t row0,
-                                          int64_t cbw) {
+                                          int64_t cbw, int64_t sb = 0, int64_t ns = -1,
+                                          double* __restrict__ part = nullptr) {
   constexpr int NC = Kern::ncomp;
   static_assert(NB == 2 || NB == 4, "16-column blocks per wave: 2 or 4");
   constexpr int NSPLIT = NPROD == 3 ? 2 : 3;
@@ -752,7 +753,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
   // LDS access wait for all outstanding loads, B included.)
   float4 cnext = make_float4(0.f, 0.f, 0.f, 0.f);
   auto fetch_coords = [&](int64_t t) {
-    if (wave == 0 && lane < 32) cnext = x2[t * 32 + lane];
+    if (wave == 0 && lane < 32) cnext = x2[(sb + t) * 32 + lane];
   };
   auto store_coords = [&](int64_t t) {
     if (wave == 0 && lane < 32) crd[t & 1][lane] = cnext;
@@ -796,7 +797,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
 #pragma unroll
     for (int t = 0; t < NSPLIT; ++t)
 #pragma unroll
-      for (int n = 0; n < NB; ++n) dst[t][n] = P0[t * plane + (s * ncb + cb0 + n) * 64 + lane];
+      for (int n = 0; n < NB; ++n) dst[t][n] = P0[t * plane + ((sb + s) * ncb + cb0 + n) * 64 + lane];
   };
 
   double sum[NC][MB][NB][4];
@@ -820,7 +821,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
           for (int i = 0; i < 4; ++i) sum[c][m][n][i] += (double)acc[c][m][n][i];
   };
 
-  const int64_t nsteps = n2p / 32;
+  const int64_t nsteps = ns < 0 ? n2p / 32 : ns;  // steps sb .. sb + nsteps - 1 of r2
   us8 bqA[NSPLIT][NB], bqB[NSPLIT][NB];  // B fragments, ping-pong between steps
   fetch_coords(0);
   store_coords(0);
@@ -905,6 +906,20 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
     if (s + 1 < nsteps) step(s + 1, bqB, bqA, Fz{});
   }
   if (FOLD == 2 && (nsteps & 1)) fold_all();  // the last, unpaired step
+  if (part) {
+    // a piece of a split tile: the raw FP64 sums, [c][16 MB rows][64 NB cols]
+    constexpr int TR = 16 * MB, TC = 64 * NB;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            part[((int64_t)c * TR + 16 * m + d_row_bf16(g, i)) * TC + 16 * (wave * NB + n) + r] = sum[c][m][n][i];
+    return;
+  }
   if constexpr (Combines<Kern>::value) {
     // outputs o = 0, 1 of kern.finish to Fout / Fout2, row-major n1 x npair each
     static_assert(Kern::nout == 2, "two outputs");
@@ -1382,16 +1397,42 @@ constexpr int bf16_min_blocks() {
   return (Kern::ncomp == 1 || NPROD == 3 || Comp5<Kern>::value < Kern::ncomp) ? 2 : 1;
 }
 
+// Workgroup t = blockIdx.x is tile t: column group t % gx, row block t / gx.
 template <int MB, int NPROD, int FOLD, class Kern, int NB>
 __global__ __launch_bounds__(256, (bf16_min_blocks<Kern, NPROD>())) void fgemm_bf16_kernel(
     const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
     const us8* __restrict__ P0, int64_t plane, int64_t ncb, void* __restrict__ Fout, void* __restrict__ Fout2,
-    int out_f32, int64_t npair, double scale, Kern kern) {
+    int out_f32, int64_t npair, double scale, Kern kern, int64_t gx) {
   __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
   __shared__ float4 crd[2][32];
+  const int64_t t = blockIdx.x;
   bf16_tile<MB, NPROD, FOLD, Kern, NB>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, Fout, Fout2, out_f32, npair,
                                        Combines<Kern>::value ? npair : Kern::ncomp * npair, scale, kern,
-                                       (int64_t)blockIdx.y * (16 * MB), (int64_t)blockIdx.x * (4 * NB));
+                                       (t / gx) * (16 * MB), (t % gx) * (4 * NB));
+}
+
+// The split tail of fgemm_bf16_kernel (runs and pieces as for the pipelined
+// kernel's fgemm_bf16_pipe_tail_kernel).
+template <int MB, int NPROD, int FOLD, class Kern, int NB>
+__global__ __launch_bounds__(256, (bf16_min_blocks<Kern, NPROD>())) void fgemm_bf16_tail_kernel(
+    const double* __restrict__ p1, int64_t n1, const float4* __restrict__ x2, int64_t n2p,
+    const us8* __restrict__ P0, int64_t plane, int64_t ncb, int64_t npair, Kern kern, int64_t gx, int64_t t0,
+    int64_t W, double* __restrict__ part) {
+  __shared__ us8 ktile[2][Kern::ncomp][MB][2][64];
+  __shared__ float4 crd[2][32];
+  constexpr int64_t TE = (int64_t)Kern::ncomp * 16 * MB * 64 * NB;  // doubles per piece
+  const int64_t nst = n2p / 32, S = gridDim.x, w = blockIdx.x;
+  const int64_t g1 = split_bound(w + 1, W, S, nst);
+  int k = 0;
+  for (int64_t g = split_bound(w, W, S, nst); g < g1 && k < 2; ++k) {
+    const int64_t tr = g / nst, s0 = g - tr * nst, s1 = g1 - tr * nst < nst ? g1 - tr * nst : nst;
+    const int64_t t = t0 + tr;
+    if (k) __syncthreads();  // the previous piece's LDS reads are done
+    bf16_tile<MB, NPROD, FOLD, Kern, NB>(ktile, crd, p1, n1, x2, n2p, P0, plane, ncb, nullptr, nullptr, 0, npair, 0,
+                                         1.0, kern, (t / gx) * (16 * MB), (t % gx) * (4 * NB), s0, s1 - s0,
+                                         part + (2 * w + k) * TE);
+    g = tr * nst + s1;
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1491,6 +1532,18 @@ int resident_slots(const void* fn, int block) {
   return slots[{dev, fn}] = ncu * per;
 }
 
+// The resident slots to split the last round of a launch of kernel fn over,
+// or 0 for no split: the split is off (TCSCF_BF16_SPLIT=0, a caller-provided
+// workspace), the launch fits one round, or its last round is over 90% full
+// or has under four r2 steps per slot.
+int64_t split_slots(const void* fn, int block, int64_t tiles, int64_t nst) {
+  const char* env = std::getenv("TCSCF_BF16_SPLIT");
+  if ((env && std::atoi(env) == 0) || t_fixed_ws) return 0;
+  const int64_t S = resident_slots(fn, block);
+  const int64_t r = S ? tiles % S : 0;
+  return S && tiles > S && r > 0 && r * 10 <= S * 9 && r * nst >= 4 * S ? S : 0;
+}
+
 // The launch's tiles run in rounds of the resident slots; a last round of r
 // tiles (r < slots) leaves slots - r of them idle for a whole tile.  Unless
 // TCSCF_BF16_SPLIT=0, that round is split: the full rounds run as usual, the
@@ -1505,12 +1558,10 @@ int launch_bf16_pipe(const double* p1, int64_t n1, const float4* x2, int64_t n2p
                      void* F2) {
   constexpr int NC = Kern::ncomp, TR = 16 * MB, TC = 16 * NW * NB;
   const int64_t gx = ncb / (NW * NB), gy = (n1 + TR - 1) / TR, tiles = gx * gy, nst = n2p / 32;
-  const void* fn = (const void*)fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW, FOLD, REV>;
-  const char* env = std::getenv("TCSCF_BF16_SPLIT");
-  const int64_t S = (env && std::atoi(env) == 0) || t_fixed_ws ? 0 : resident_slots(fn, 64 * NW);
+  const int64_t S = split_slots((const void*)fgemm_bf16_pipe_kernel<MB, NPROD, Kern, NB, NW, FOLD, REV>, 64 * NW,
+                                tiles, nst);
   const int64_t r = S ? tiles % S : 0, full = tiles - r;
-  // split when the last round is at most 90% full and has at least four steps per slot
-  const bool split = S && tiles > S && r > 0 && r * 10 <= S * 9 && r * nst >= 4 * S;
+  const bool split = S > 0;
   double* part = nullptr;
   if (split) {
     int rc = 0;
@@ -1588,9 +1639,31 @@ int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, con
 #undef TC_PIPE
     }
   }
-  hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, Kern, NB>), grid, dim3(256), 0, st, p1, n1, x2, n2p, P0,
-                     plane, ncb, F, F2, out_f32, npair, scale, kern);
-  TC_HIP(hipGetLastError());
+  // the plain tile, with the last round split as for the pipelined kernel
+  constexpr int TR = 16 * MB, TC = 64 * NB;
+  const int64_t gx = grid.x, tiles = gx * grid.y, nst = n2p / 32;
+  const int64_t S = split_slots((const void*)fgemm_bf16_kernel<MB, NPROD, FOLD, Kern, NB>, 256, tiles, nst);
+  const int64_t r = S ? tiles % S : 0, full = tiles - r;
+  double* part = nullptr;
+  if (S) {
+    int rc = 0;
+    part = (double*)scratch(kSlotSplit, (size_t)(2 * S) * NC * TR * TC * sizeof(double), &rc);
+    if (rc) return rc;
+  }
+  if (full > 0) {
+    hipLaunchKernelGGL((fgemm_bf16_kernel<MB, NPROD, FOLD, Kern, NB>), dim3((unsigned)full), dim3(256), 0, st, p1, n1,
+                       x2, n2p, P0, plane, ncb, F, F2, out_f32, npair, scale, kern, gx);
+    TC_HIP(hipGetLastError());
+  }
+  if (S) {
+    const int64_t W = r * nst;
+    hipLaunchKernelGGL((fgemm_bf16_tail_kernel<MB, NPROD, FOLD, Kern, NB>), dim3((unsigned)S), dim3(256), 0, st, p1,
+                       n1, x2, n2p, P0, plane, ncb, npair, kern, gx, full, W, part);
+    TC_HIP(hipGetLastError());
+    hipLaunchKernelGGL((bf16_pipe_fixup<Kern, TR, TC>), dim3((unsigned)((TR * TC + 255) / 256), (unsigned)r),
+                       dim3(256), 0, st, part, n1, npair, gx, full, nst, W, S, F, F2, out_f32, scale, kern);
+    TC_HIP(hipGetLastError());
+  }
   return 0;
 }
 

@@ -187,7 +187,8 @@ def test_F_split_last_round_matches_unsplit(tc, gpu, monkeypatch):
     pieces start on even steps, so the two-step FP32 partials of BF16x4 pair
     the steps exactly as the unsplit tiles do.  Tolerances in the asserts:
     1e-13 of max |F| for FP64 storage, one FP32 rounding for FP32 storage;
-    and the RHF energy to 1e-10 Eh."""
+    and the RHF energy to 1e-10 Eh.  The plain tile (V, the TC scalar
+    kernel) splits its last round the same way."""
     import torch
     from tcscf_amd import configs
     c = configs.ne_config2()
@@ -211,6 +212,18 @@ def test_F_split_last_round_matches_unsplit(tc, gpu, monkeypatch):
     monkeypatch.setenv("TCSCF_BF16_SPLIT", "1")
     Pb = tc.computeF_tc_pair(g, g, jas, precision="bf16x4", smooth_precision="bf16x3")
     for x, y in zip(Pa, Pb):
+        assert ((x - y).abs().max() / x.abs().max()).item() < 1e-13
+    # the plain tile: V (three components, 16-row tiles over the 9,700-point
+    # r3 grid of the TC workload) and the TC scalar kernel
+    g.setBasisFunctions(c["basis"], True)
+    g3 = tc.QMCGrid.radial_lebedev(50, 23, kind, 1.0, 1.0)
+    g3.setBasisFunctions(c["basis"], True)
+    for fn in (lambda: tc.computeV(g, g3, jas, precision="bf16x3"),
+               lambda: tc.computeF(g, g3, kernel="tc", jastrow=jas, precision="bf16x4")):
+        monkeypatch.setenv("TCSCF_BF16_SPLIT", "0")
+        x = fn()
+        monkeypatch.setenv("TCSCF_BF16_SPLIT", "1")
+        y = fn()
         assert ((x - y).abs().max() / x.abs().max()).item() < 1e-13
     S, H = tc.one_electron_matrices(c["basis"], c["Z"])
     e = []
