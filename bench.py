@@ -27,10 +27,12 @@ sys.path.insert(0, ROOT)
 METRIC = "SCF wall time + grid-pair contractions/s: mixed-prec MFMA vs FP64 GPU @1e-6 Eh"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x4": 2500.0, "bf16x5": 2500.0}
 PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x4": 4, "bf16x5": 5}
-# HBM bytes per launch of the contraction kernel on this workload, from rocprofv3
-# --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --pmc-probe`, each scaled by the
-# reading for a 512 MiB copy (FETCH x2.0, WRITE x1.0); profiles/bf16_fgemm_ne30_traffic.md
-TRAFFIC_BYTES = {"bf16x4": 1.502e9 + 1.139e8, "bf16x5": 1.502e9 + 1.140e8, "fp64": 1.016e9 + 1.156e8}
+# HBM bytes per computeF of the contraction kernels on this workload (BF16: the
+# full-round launch, the split last round and its fix-up together), from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --pmc-probe`, each
+# scaled by the reading for a 512 MiB copy (FETCH x2.0, WRITE x1.0);
+# profiles/bf16_fgemm_ne30_traffic.md
+TRAFFIC_BYTES = {"bf16x4": 2.205e9 + 1.682e8, "bf16x5": 2.207e9 + 1.694e8, "fp64": 1.016e9 + 1.156e8}
 
 
 def parse():
@@ -131,9 +133,10 @@ def main():
     peak = PEAK_TFLOPS[a.precision]
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get(a.precision),
-                "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                "traffic_unit": "bytes per computeF (HBM, rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                 "issued_frac": round(achieved * PRODUCTS[a.precision] / peak, 4),
-                "kernel": "fgemm (fused 1/r12 generation + contraction), computeF wall incl. packing",
+                "kernel": "fgemm_bf16_pipe (fused 1/r12 generation + contraction: full rounds, split last round, "
+                          "fix-up), computeF wall incl. packing",
                 "kernel_ms": round(k_mix, 4)}
 
     blocks = block_errors(tc, grid, S, H, cfg, a, hf_mix, hf64, e_mix, e64) if a.precision != "fp64" else None
