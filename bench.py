@@ -56,6 +56,9 @@ def parse():
     p.add_argument("--tc-v-precision", default="bf16x3", choices=["bf16x4", "bf16x5", "bf16x3", "fp32", "fp64"],
                    help="mixed TC: precision of the Jastrow-gradient tensor V")
     p.add_argument("--no-batch", action="store_true", help="skip the batched 64-instance side measurement")
+    p.add_argument("--no-qmc", action="store_true", help="skip the QMC-grid TC-HF Ne side measurement")
+    p.add_argument("--qmc-repeats", type=int, default=5,
+                   help="QMC workload: independently scrambled grid sets per Jastrow b (the error bar)")
     p.add_argument("--batch-precision", default="bf16x5", choices=["bf16x4", "bf16x5", "bf16x3", "fp32"],
                    help="the batch's mixed precision (BF16x5: its small grids average the dropped Kl*Pm term "
                         "less, and its tile does not gain from four products)")
@@ -147,6 +150,9 @@ def main():
     batch_res = None
     if not a.no_batch:
         batch_res = run_batch(a, tc, configs, torch)
+    qmc_res = None
+    if not a.no_qmc:
+        qmc_res = run_qmc(a, tc, configs, torch)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -214,6 +220,7 @@ def main():
         "cpu_baseline": cpu,
         "tc_ne": tc_res,
         "batch64": batch_res,
+        "qmc_ne": qmc_res,
     }
     print(json.dumps(res))
 
@@ -378,6 +385,98 @@ def run_tc(a, tc, configs, torch):
                     "fp64_pair_F_ms": pair64,
                     "speedup_vs_fp64_with_pair_F": (f64["total_ms"] - f64["F_ms"] + pair64) / mix["total_ms"]})
     return res
+
+
+def run_qmc(a, tc, configs, torch):
+    """The QMC workload: TC-HF Ne (40 STOs) on scrambled-Sobol grids of 2**16
+    points per electron coordinate, Jastrow b in {0.5, 1, 2}.  Per b,
+    --qmc-repeats independently scrambled grid sets (r1, r2, r3 seeds 3k + 1,
+    3k + 2, 3k + 3): the mixed precompute (as the TC-Ne leg: F pair at
+    --precision / --tc-smooth-precision, V at --tc-v-precision, FP32 storage)
+    and the SCF (DIIS 6, metric MGS), whose energies give the QMC error bar
+    (mean +- standard deviation over the grid sets, as the reference's driver
+    reports it); and on the first grid set the FP64 precompute and SCF, whose
+    energy gives the FP64 deviation.  A grid set whose SCF does not converge
+    is drawn again (the next seeds), at most three times in a row."""
+    import numpy as np
+    cfg = configs.ne_qmc_config4()
+    basis, nb = cfg["basis"], len(cfg["basis"])
+    S, H = tc.one_electron_matrices(basis, cfg["Z"])
+
+    def solve(jas, k, prec):
+        g = [tc.QMCGrid.sobol(cfg["npts"], seed=3 * k + q) for q in (1, 2, 3)]
+        for x in g:
+            x.setBasisFunctions(basis, True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if prec == "fp64":
+            Fs = tc.computeF(g[0], g[1], kernel="tc", jastrow=jas, same_spin=True, precision="fp64")
+            Fo = tc.computeF(g[0], g[1], kernel="tc", jastrow=jas, same_spin=False, precision="fp64")
+            V = tc.computeV(g[0], g[2], jas, precision="fp64")
+        else:
+            Fs, Fo = tc.computeF_tc_pair(g[0], g[1], jas, precision=prec, smooth_precision=a.tc_smooth_precision,
+                                         out_dtype="fp32")
+            V = tc.computeV(g[0], g[2], jas, precision=a.tc_v_precision, out_dtype="fp32")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hf = tc.TCHartreeFock(cfg["nup"], cfg["ndown"], nb)
+        try:
+            e = hf.compute(S, H, g[0], Fs, Fo, V, jas, True, tol=a.tol, max_iter=200, mgs_mode=1, diis=6)
+        except tc.ConvergenceError:
+            e = None
+        t2 = time.perf_counter()
+        del Fs, Fo, V
+        return e, (t1 - t0) * 1e3, (t2 - t1) * 1e3, hf.numberOfConvergenceLoops()
+
+    out = {"workload": f"TC-HF Ne, {nb} STOs, scrambled-Sobol grids of {cfg['npts']} points per coordinate "
+                       f"(r1, r2, r3), DIIS 6, metric MGS",
+           "mixed_tensors": {"F_coulomb": a.precision, "F_smooth": a.tc_smooth_precision, "V": a.tc_v_precision,
+                             "storage": "fp32"}}
+    solve(tc.Jastrow.from_b(1.0), 0, a.precision)  # warm-up (workspaces, code objects)
+    for b in cfg["b_values"]:
+        jas = tc.Jastrow.from_b(b)
+        energies, pre, scf, its, k, fails, k0, sets = [], [], [], [], 0, 0, None, []
+        while len(energies) < a.qmc_repeats and fails < 3:
+            e, tp, ts, n_it = solve(jas, k, a.precision)
+            k += 1
+            if e is None:
+                fails += 1
+                continue
+            fails = 0
+            if k0 is None:
+                k0 = k - 1
+            sets.append(k - 1)
+            energies.append(e)
+            pre.append(tp)
+            scf.append(ts)
+            its.append(n_it)
+        e64, tp64, ts64, _ = solve(jas, k0 or 0, "fp64")  # the first grid set that converged
+        E = np.array(energies)
+        # the grid set farthest from the median, also in FP64: whether its energy
+        # is the grid's (an SCF state of that quadrature) or the precision's
+        far = None
+        if len(E) > 2:
+            j = int(np.argmax(np.abs(E - np.median(E))))
+            if j > 0:
+                ef, _, _, _ = solve(jas, sets[j], "fp64")
+                far = {"grid_set": sets[j], "energy": float(E[j]), "fp64_energy": ef,
+                       "energy_dev_vs_fp64": (float(E[j]) - ef) if ef is not None else None}
+        out[f"b={b:g}"] = {
+            "energies": [float(x) for x in E],
+            "energy_mean": float(E.mean()) if len(E) else None,
+            "energy_median": float(np.median(E)) if len(E) else None,
+            "qmc_error_bar_std": float(E.std(ddof=1)) if len(E) > 1 else None,
+            "grid_sets": len(E), "grid_sets_redrawn": k - len(E),
+            "mixed_ms_per_solve": {"precompute": float(np.mean(pre)) if pre else None,
+                                   "scf": float(np.mean(scf)) if scf else None,
+                                   "iterations": float(np.mean(its)) if its else None},
+            "fp64_ms_per_solve": {"precompute": tp64, "scf": ts64},
+            "fp64_energy_first_grid_set": e64,
+            "energy_dev_vs_fp64_first_grid_set": (energies[0] - e64) if energies and e64 is not None else None,
+            "speedup_vs_fp64": (tp64 + ts64) / (pre[0] + scf[0]) if pre else None,
+            "farthest_from_median": far,
+        }
+    return out
 
 
 def run_batch(a, tc, configs, torch):
