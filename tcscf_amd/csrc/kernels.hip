@@ -256,27 +256,41 @@ template <int NSPLIT>
 __global__ __launch_bounds__(256) void pack_pairs_bf16(const double* __restrict__ A, const double* __restrict__ w,
                                                        const int* __restrict__ ord, int64_t n2, int nb, int64_t npair,
                                                        int64_t nkb, int64_t ncb, unsigned short* __restrict__ planes) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one (fragment, lane, e)
-  const int64_t total = nkb * ncb * 512;
-  if (idx >= total) return;
-  const int64_t frag = idx >> 9;
-  const int within = (int)(idx & 511);
-  const int lane = within >> 3, e = within & 7;
+  // one fragment lane: its 8 bf16 of every plane (points 32 kb + 8 g .. + 7,
+  // pair 16 cb + r) as one 16-byte store per plane
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t lanes = nkb * ncb * 64, total = lanes * 8;
+  if (idx >= lanes) return;
+  const int64_t frag = idx >> 6;
+  const int lane = (int)(idx & 63);
   const int64_t kb = frag / ncb, cb = frag - kb * ncb;
   const int g = lane >> 4, r = lane & 15;
-  const int64_t k = 32 * kb + 8 * g + e, p = 16 * cb + r;
-  double v = 0;
-  if (k < n2 && p < npair) {
-    const int64_t ks = ord ? ord[k] : k;
-    int j, i;
-    pair_of(p, &j, &i);
-    v = w[ks] * A[ks * nb + j] * A[ks * nb + i];
+  const int64_t k0 = 32 * kb + 8 * g, p = 16 * cb + r;
+  int j = 0, i = 0;
+  if (p < npair) pair_of(p, &j, &i);
+  unsigned short h[NSPLIT][8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t k = k0 + e;
+    double v = 0;
+    if (k < n2 && p < npair) {
+      const int64_t ks = ord ? ord[k] : k;
+      v = w[ks] * A[ks * nb + j] * A[ks * nb + i];
+    }
+#pragma unroll
+    for (int t = 0; t < NSPLIT; ++t) {
+      h[t][e] = bf16_bits_rne((float)v);
+      v -= (double)bf16_to_float(h[t][e]);
+    }
   }
 #pragma unroll
   for (int t = 0; t < NSPLIT; ++t) {
-    const unsigned short h = bf16_bits_rne((float)v);
-    planes[t * total + idx] = h;
-    v -= (double)bf16_to_float(h);
+    uint4 q;
+    q.x = h[t][0] | ((unsigned)h[t][1] << 16);
+    q.y = h[t][2] | ((unsigned)h[t][3] << 16);
+    q.z = h[t][4] | ((unsigned)h[t][5] << 16);
+    q.w = h[t][6] | ((unsigned)h[t][7] << 16);
+    *(uint4*)(planes + t * total + idx * 8) = q;
   }
 }
 
@@ -1756,7 +1770,7 @@ int dispatch(int precision, const double* pts1, int64_t n1, const double* pts2, 
     const int64_t plane = nkb * ncb * 512;  // bf16 per plane
     double* xd = (double*)(planes + nsplit * plane);
     float4* x4 = (float4*)(xd + 3 * n2p);
-    const unsigned gp = (unsigned)((plane + 255) / 256);
+    const unsigned gp = (unsigned)((plane / 8 + 255) / 256);  // one thread per fragment lane
     if (nsplit == 2)
       hipLaunchKernelGGL(pack_pairs_bf16<2>, dim3(gp), dim3(256), 0, st, A2, w2, ord2, n2, nb, npair, nkb, ncb, planes);
     else
@@ -2539,7 +2553,7 @@ extern "C" int tcscf_batch_compute_F(const tcscf_batch_item* items, int count, i
     unsigned short* planes = (unsigned short*)(ws + off[k]);
     double* xd = (double*)(planes + nsplit * plane);
     float4* x4 = (float4*)(xd + 3 * n2p);
-    const unsigned gp = (unsigned)((plane + 255) / 256);
+    const unsigned gp = (unsigned)((plane / 8 + 255) / 256);  // one thread per fragment lane
     if (nsplit == 2)
       hipLaunchKernelGGL(pack_pairs_bf16<2>, dim3(gp), dim3(256), 0, st, it.A2_dev, it.w2_dev, it.ord2_dev, it.n2,
                          it.nb, npair, nkb, ncb, planes);
