@@ -33,11 +33,16 @@ def build(verbose=False, jobs=None):
     # code object v5: loadable by the HIP runtime torch bundles as well as /opt/rocm's
     hip_flags = [f"--offload-arch={ARCH}", "-mcode-object-version=5", "-Xclang", "-target-feature", "-Xclang",
                  "-packed-fp32-ops", "-munsafe-fp-atomics"]
+    # the dense FP64 solvers: complex products and quotients inline (the
+    # textbook product, Smith's quotient) instead of the C99 runtime routines
+    # with their inf / NaN recovery, which the finite SCF matrices never need
+    per_file = {"dense.cpp": ["-fcomplex-arithmetic=improved"]}
     objs, procs = [], []
     for s in srcs:
         obj = os.path.join(objdir, os.path.basename(s) + ".o")
         objs.append(obj)
-        cmd = [hipcc()] + common + (hip_flags + ["-x", "hip"] if s.endswith(".hip") else []) + ["-c", s, "-o", obj]
+        cmd = ([hipcc()] + common + per_file.get(os.path.basename(s), []) +
+               (hip_flags + ["-x", "hip"] if s.endswith(".hip") else []) + ["-c", s, "-o", obj])
         if verbose:
             print(" ".join(cmd))
         procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 namespace tc::dense {
@@ -409,22 +410,31 @@ void mgs(int k, int n, std::vector<cplx>& v, const double* S, bool euclid_after_
 }
 
 void Diis::extrapolate(std::vector<double>& F, const std::vector<double>& e) {
+  // the new error's inner products with the stored ones (and itself); the
+  // older entries of B are kept from earlier calls
+  std::deque<double> row;
+  for (size_t j = 0; j <= e_.size(); ++j) {
+    const std::vector<double>& ej = j < e_.size() ? e_[j] : e;
+    double t = 0;
+    for (size_t q = 0; q < e.size(); ++q) t += e[q] * ej[q];
+    row.push_back(t);
+  }
   f_.push_back(F);
   e_.push_back(e);
-  while ((int)f_.size() > depth_) {
+  b_.push_back(std::move(row));
+  auto drop_oldest = [&] {
     f_.pop_front();
     e_.pop_front();
-  }
+    b_.pop_front();
+    for (auto& r : b_) r.pop_front();
+  };
+  while ((int)f_.size() > depth_) drop_oldest();
   while (f_.size() >= 2) {
     const int m = (int)f_.size(), n = m + 1;
     // [B 1; 1^T 0] [c; lambda] = [0; 1], B_ij = <e_i, e_j>
     std::vector<double> M((size_t)n * n, 0.0), x(n, 0.0);
     for (int i = 0; i < m; ++i)
-      for (int j = 0; j <= i; ++j) {
-        double t = 0;
-        for (size_t q = 0; q < e_[i].size(); ++q) t += e_[i][q] * e_[j][q];
-        M[i * n + j] = M[j * n + i] = t;
-      }
+      for (int j = 0; j <= i; ++j) M[i * n + j] = M[j * n + i] = b_[i][j];
     for (int i = 0; i < m; ++i) M[i * n + m] = M[m * n + i] = 1.0;
     x[m] = 1.0;
     // Gaussian elimination with partial pivoting, relative pivot threshold
@@ -460,20 +470,31 @@ void Diis::extrapolate(std::vector<double>& F, const std::vector<double>& e) {
         for (size_t q = 0; q < F.size(); ++q) F[q] += x[i] * f_[i][q];
       return;
     }
-    f_.pop_front();  // singular subspace: drop the oldest pair and retry
-    e_.pop_front();
+    drop_oldest();  // singular subspace: drop the oldest pair and retry
   }
 }
 
 namespace {
-// C = A B for n x n row-major
+// C = A B for n x n row-major.  Row i of C accumulates A[i][k] B[k][:] in k
+// order (each element's sum in the order of a dot product over k, so the
+// result is that of the i-j-k loop), with the inner loop over contiguous j.
 void matmul(int n, const double* A, const double* B, double* C) {
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      double t = 0;
-      for (int k = 0; k < n; ++k) t += A[i * n + k] * B[k * n + j];
-      C[i * n + j] = t;
+  for (int i = 0; i < n; ++i) {
+    double* c = C + (int64_t)i * n;
+    for (int j = 0; j < n; ++j) c[j] = 0;
+    for (int k = 0; k < n; ++k) {
+      const double a = A[(int64_t)i * n + k];
+      const double* b = B + (int64_t)k * n;
+      for (int j = 0; j < n; ++j) c[j] += a * b[j];
     }
+  }
+}
+
+// Spin s's F and D equal spin 0's bit for bit (a closed shell whose two spin
+// operators coincide): its error vector is spin 0's.
+bool same_as_spin0(int s, int64_t nb2, const double* F, const double* D) {
+  return s > 0 && std::memcmp(F, F + s * nb2, sizeof(double) * nb2) == 0 &&
+         std::memcmp(D, D + s * nb2, sizeof(double) * nb2) == 0;
 }
 }  // namespace
 
@@ -482,6 +503,10 @@ void commutator_error(int nspin, int nb, const double* F, const double* D, const
   e.assign(nspin * nb2, 0.0);
   std::vector<double> FD(nb2), FDS(nb2), SD(nb2), SDF(nb2);
   for (int s = 0; s < nspin; ++s) {
+    if (same_as_spin0(s, nb2, F, D)) {
+      std::copy(e.begin(), e.begin() + nb2, e.begin() + s * nb2);
+      continue;
+    }
     matmul(nb, F + s * nb2, D + s * nb2, FD.data());
     matmul(nb, FD.data(), S, FDS.data());
     matmul(nb, S, D + s * nb2, SD.data());
@@ -495,6 +520,10 @@ void right_eigen_error(int nspin, int nb, const double* F, const double* D, cons
   e.assign(nspin * nb2, 0.0);
   std::vector<double> FD(nb2), SD(nb2), SDFD(nb2);
   for (int s = 0; s < nspin; ++s) {
+    if (same_as_spin0(s, nb2, F, D)) {
+      std::copy(e.begin(), e.begin() + nb2, e.begin() + s * nb2);
+      continue;
+    }
     matmul(nb, F + s * nb2, D + s * nb2, FD.data());
     matmul(nb, S, D + s * nb2, SD.data());
     matmul(nb, SD.data(), FD.data(), SDFD.data());

@@ -40,6 +40,7 @@ class Diis {
  private:
   int depth_;
   std::deque<std::vector<double>> f_, e_;
+  std::deque<std::deque<double>> b_;  // b_[i][j] = <e_i, e_j> for j <= i (the stored pairs)
 };
 
 // DIIS error of a Hermitian Fock operator, per spin: F D S - S D F.
