@@ -42,8 +42,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--precision", default="bf16x4", choices=["bf16x4", "bf16x5", "bf16x3", "fp32", "fp64"])
     p.add_argument("--fp64-steps", type=int, default=2)
-    p.add_argument("--storage", default="fp64", choices=["fp64", "fp32"],
-                   help="F storage of the mixed-precision RHF leg (the FP64 reference leg stores FP64)")
+    p.add_argument("--storage", default="fp32", choices=["fp64", "fp32"],
+                   help="F storage of the mixed-precision RHF leg (the FP64 reference leg stores FP64); FP32 "
+                        "halves the sweep's bytes, the SCF arithmetic stays FP64")
     p.add_argument("--tol", type=float, default=1e-9)
     p.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU-baseline slice (0 = auto)")
     p.add_argument("--no-cpu-baseline", action="store_true")
