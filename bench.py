@@ -28,11 +28,12 @@ METRIC = "SCF wall time + grid-pair contractions/s: mixed-prec MFMA vs FP64 GPU 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x4": 2500.0, "bf16x5": 2500.0}
 PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x4": 4, "bf16x5": 5}
 # HBM bytes per computeF of the contraction kernels on this workload (BF16: the
-# full-round launch, the split last round and its fix-up together), from
+# full-round launch, the split last round and its fix-up together; BF16x4 with
+# the FP32 F storage of the mixed RHF leg, BF16x5 and FP64 with FP64 F), from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of `bench.py --pmc-probe`, each
 # scaled by the reading for a 512 MiB copy (FETCH x2.0, WRITE x1.0);
 # profiles/bf16_fgemm_ne30_traffic.md
-TRAFFIC_BYTES = {"bf16x4": 2.205e9 + 1.682e8, "bf16x5": 2.207e9 + 1.694e8, "fp64": 1.016e9 + 1.156e8}
+TRAFFIC_BYTES = {"bf16x4": 2.205e9 + 1.122e8, "bf16x5": 2.207e9 + 1.694e8, "fp64": 1.016e9 + 1.156e8}
 
 
 def parse():
@@ -533,7 +534,7 @@ def pmc_probe(tc, cfg, torch):
     g.setBasisFunctions(cfg["basis"], True)
     F = tc.computeF(g, g, precision="bf16x5")
     torch.cuda.synchronize()
-    F = tc.computeF(g, g, precision="bf16x4")
+    F = tc.computeF(g, g, precision="bf16x4", out_dtype="fp32")  # as the bench's mixed RHF leg stores it
     torch.cuda.synchronize()
     F64 = tc.computeF(g, g, precision="fp64")
     torch.cuda.synchronize()
