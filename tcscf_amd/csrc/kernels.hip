@@ -29,6 +29,7 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 constexpr double kFar = 1.0e6;  // padding coordinate: far from every grid point
+constexpr int kVFold = 4;       // FP64 fold period (32-point steps) of the BF16 V tiles
 constexpr int kOchiMaxTerms = TCSCF_OCHI_MAX_TERMS;
 
 // Output layouts of the 16x16 MFMA accumulators: element i of lane l = 16 g + r
@@ -707,7 +708,10 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 /* k-steps and is then folded into FP64 sums (long FP32 runs lose the small  */
 /* terms of the large-dynamic-range pair densities).  NPROD = 4 takes FOLD = */
 /* 2 and issues each step's products smallest first (Kh Pl, Kl Ph, Kh Pm,   */
-/* Kh Ph), so the corrections add at their own magnitude.                    */
+/* Kh Ph), so the corrections add at their own magnitude.  The three-       */
+/* component V tiles (Jastrow gradient) take FOLD = 4 at every NPROD: their  */
+/* fold is a third of their VALU work at FOLD = 1, and V's error against    */
+/* FP64 does not move with it.                                               */
 /* Workgroup = 4 waves sharing 16*MB rows; wave w owns columns               */
 /* [64 w, 64 w + 64) of the workgroup's 256.  Each k-step the workgroup      */
 /* generates its K tile once, split hi/lo, into LDS in A-fragment order      */
@@ -735,7 +739,7 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
   constexpr int VALS = MB * 64 * 8;       // K values per component per k-step
   constexpr int EPT = VALS / 256;         // per thread
   static_assert(EPT >= 2 && EPT <= 8 && VALS % 256 == 0 && EPT % 2 == 0, "MB must be 1, 2 or 4");
-  static_assert(FOLD == 1 || FOLD == 2, "FP64 fold every 1 or 2 steps");
+  static_assert(FOLD == 1 || FOLD == 2 || FOLD == 4, "FP64 fold every 1, 2 or 4 steps");
   static_assert(NPROD >= 3 && NPROD <= 5, "3, 4 or 5 products");
   constexpr bool REV = NPROD == 4;
   typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
@@ -866,8 +870,9 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
   auto fence = [] {
     if constexpr (RUN) __builtin_amdgcn_sched_barrier(0);
   };
-  auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto fresh) {
+  auto step = [&](int64_t s, us8 (&cur)[NSPLIT][NB], us8 (&nxt)[NSPLIT][NB], auto fresh, auto foldend) {
     constexpr bool FRESH = decltype(fresh)::value;
+    constexpr bool FOLDEND = decltype(foldend)::value;
     const int buf = (int)(s & 1);
     fetch_coords(s + 2 < nsteps ? s + 2 : nsteps - 1);  // for crd[s & 1], read last by generate(s) one step ago
     load_b(nxt, s + 1 < nsteps ? s + 1 : s);
@@ -910,16 +915,27 @@ __device__ __forceinline__ void bf16_tile(us8 (*ktile)[Kern::ncomp][MB][2][64], 
     generate(buf ^ 1);
     store_coords(s + 2);
     fence();
-    if constexpr (!FRESH || FOLD == 1) fold_all();
+    if constexpr (FOLDEND) fold_all();
     __syncthreads();
   };
   using T = std::integral_constant<bool, true>;
+  using Fb = std::integral_constant<bool, false>;
   using Fz = std::integral_constant<bool, FOLD == 1>;
-  for (int64_t s = 0; s < nsteps; s += 2) {
-    step(s, bqA, bqB, T{});
-    if (s + 1 < nsteps) step(s + 1, bqB, bqA, Fz{});
+  if constexpr (FOLD == 4) {
+    for (int64_t s = 0; s < nsteps; s += 4) {
+      step(s, bqA, bqB, T{}, Fb{});
+      if (s + 1 < nsteps) step(s + 1, bqB, bqA, Fb{}, Fb{});
+      if (s + 2 < nsteps) step(s + 2, bqA, bqB, Fb{}, Fb{});
+      if (s + 3 < nsteps) step(s + 3, bqB, bqA, Fb{}, T{});
+    }
+    if (nsteps & 3) fold_all();  // the last, incomplete group of four
+  } else {
+    for (int64_t s = 0; s < nsteps; s += 2) {
+      step(s, bqA, bqB, T{}, Fz{});
+      if (s + 1 < nsteps) step(s + 1, bqB, bqA, Fz{}, T{});
+    }
+    if (FOLD == 2 && (nsteps & 1)) fold_all();  // the last, unpaired step
   }
-  if (FOLD == 2 && (nsteps & 1)) fold_all();  // the last, unpaired step
   if (part) {
     // a piece of a split tile: the raw FP64 sums, [c][16 MB rows][64 NB cols]
     constexpr int TR = 16 * MB, TC = 64 * NB;
@@ -1319,14 +1335,15 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void fgemm_bf16_pipe_kernel(
 // Split tail: the tiles t0 .. t0 + W / nst - 1 (the last, partly filled round of
 // a launch) cut into gridDim.x runs of r2 steps, one per workgroup, over the
 // tail's steps in tile order.  Run w covers tail steps [b(w), b(w + 1)), b(w)
-// = w W / gridDim.x moved down to an even step of its tile, so each piece's
-// two-step FP32 partials pair the steps as the unsplit tile does.  A run is
+// = w W / gridDim.x moved down to a multiple of four steps of its tile, so
+// each piece's FP32 partials (two or four steps each) group the steps as the
+// unsplit tile does.  A run is
 // at most one tile's worth, so at most two pieces (the end of one tile, the
 // start of the next); piece k of run w leaves its raw FP64 sums in part slot
 // 2 w + k for bf16_pipe_fixup.
 __host__ __device__ inline int64_t split_bound(int64_t w, int64_t W, int64_t S, int64_t nst) {
   const int64_t b = w * W / S, t0 = b / nst * nst;
-  return t0 + (b - t0) / 2 * 2;
+  return t0 + (b - t0) / 4 * 4;
 }
 
 template <int MB, int NPROD, class Kern, int NB, int NW, int FOLD = 1, bool REV = false>
@@ -1683,7 +1700,8 @@ int launch_bf16(const double* p1, int64_t n1, const float4* x2, int64_t n2p, con
 
 // BF16 kernel variant: rows per workgroup 16*MB, FP64 fold every FOLD steps,
 // NB 16-column blocks per wave.  TCSCF_BF16_CFG="MB,FOLD[,NB]" selects one for
-// tuning.
+// tuning (its FOLD applies to the one- and two-component kernels at three or
+// five products; V always folds every kVFold steps).
 struct Bf16Cfg {
   int mb, fold, nb;
 };
@@ -1702,7 +1720,9 @@ template <int NPROD, int MB, class Kern>
 int launch_bf16_fold(Bf16Cfg c, const double* p1, int64_t n1, const float4* x2, int64_t n2p, const us8* P0,
                      int64_t plane, int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern,
                      hipStream_t st, void* F2) {
-  if constexpr (NPROD == 4) {  // four products: FP64 fold every second step
+  if constexpr (Kern::ncomp == 3) {  // V (three components): FP64 fold every kVFold steps
+    return launch_bf16<MB, NPROD, kVFold>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
+  } else if constexpr (NPROD == 4) {  // four products: FP64 fold every second step
     if (MB == 4 && c.nb == 2)
       return launch_bf16<MB, NPROD, 2, Kern, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
     return launch_bf16<MB, NPROD, 2>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
@@ -2494,7 +2514,7 @@ __global__ __launch_bounds__(256, (bf16_min_blocks<KernJastrowGrad, NPROD>())) v
     const int64_t cb = local % d.cblocks, rb = local / d.cblocks;
     KernJastrowGrad kern;
     kern.a12 = d.a12;
-    bf16_tile<1, NPROD, NPROD == 4 ? 2 : 1, KernJastrowGrad, 4>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane,
+    bf16_tile<1, NPROD, kVFold, KernJastrowGrad, 4>(ktile, crd, d.p1, d.n1, d.x2, d.n2p, d.P0, d.plane,
                                                                 d.ncb, d.F, nullptr, d.out_f32, d.npair, 3 * d.npair,
                                                                 d.scale, kern, rb * 16, cb * 16);
   }
