@@ -310,8 +310,10 @@ typedef struct tcscf_batch_item {
  * (instance, row block, 256-pair block, component) tiles, largest instances
  * first; the rows per tile (16 or 32) come from an online autotuner that times
  * both on the head of the queue the first time a precision is used in the
- * process.  Results equal tcscf_compute_F's bit for bit.  FP64 / FP32: one
- * launch per instance.  Synchronizes the stream before returning. */
+ * process.  Results equal tcscf_compute_F's bit for bit (for a V item, when
+ * its r1 points x r3 points x pairs stay below 2e12, above which the single
+ * call folds its FP32 partials every step).  FP64 / FP32: one launch per
+ * instance.  Synchronizes the stream before returning. */
 int tcscf_batch_compute_F(const tcscf_batch_item* items_host, int count, int precision, void* stream);
 /* Rows per tile the autotuner chose (16 or 32; 0 before the first batch). */
 int tcscf_batch_tuned_rows(int precision);

@@ -30,6 +30,11 @@ typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 constexpr double kFar = 1.0e6;  // padding coordinate: far from every grid point
 constexpr int kVFold = 4;       // FP64 fold period (32-point steps) of the BF16 V tiles
+// V launches above this many (r1 point, r3 point, pair) units fold every step:
+// on a long launch the four-step fold loses its speed (measured: 65,536 x
+// 65,536 points, 820 pairs, 3.5e12 units: 98-105 ms against 87-88 ms folding
+// every step; 30,200 x 9,700 points, 465 pairs: 2.6 ms against 2.9 ms)
+constexpr double kVLongWork = 2e12;
 constexpr int kOchiMaxTerms = TCSCF_OCHI_MAX_TERMS;
 
 // Output layouts of the 16x16 MFMA accumulators: element i of lane l = 16 g + r
@@ -709,9 +714,9 @@ __global__ __launch_bounds__(256) void fgemm_f32_kernel(const double* __restrict
 /* terms of the large-dynamic-range pair densities).  NPROD = 4 takes FOLD = */
 /* 2 and issues each step's products smallest first (Kh Pl, Kl Ph, Kh Pm,   */
 /* Kh Ph), so the corrections add at their own magnitude.  The three-       */
-/* component V tiles (Jastrow gradient) take FOLD = 4 at every NPROD: their  */
-/* fold is a third of their VALU work at FOLD = 1, and V's error against    */
-/* FP64 does not move with it.                                               */
+/* component V tiles (Jastrow gradient) take FOLD = 4 at every NPROD (FOLD  */
+/* = 1 on launches above kVLongWork units): their fold is a third of their  */
+/* VALU work at FOLD = 1, and V's error against FP64 does not move with it. */
 /* Workgroup = 4 waves sharing 16*MB rows; wave w owns columns               */
 /* [64 w, 64 w + 64) of the workgroup's 256.  Each k-step the workgroup      */
 /* generates its K tile once, split hi/lo, into LDS in A-fragment order      */
@@ -1721,6 +1726,8 @@ int launch_bf16_fold(Bf16Cfg c, const double* p1, int64_t n1, const float4* x2, 
                      int64_t plane, int64_t ncb, void* F, int out_f32, int64_t npair, double scale, Kern kern,
                      hipStream_t st, void* F2) {
   if constexpr (Kern::ncomp == 3) {  // V (three components): FP64 fold every kVFold steps
+    if ((double)n1 * (double)n2p * (double)npair > kVLongWork)
+      return launch_bf16<MB, NPROD, 1>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
     return launch_bf16<MB, NPROD, kVFold>(p1, n1, x2, n2p, P0, plane, ncb, F, out_f32, npair, scale, kern, st, F2);
   } else if constexpr (NPROD == 4) {  // four products: FP64 fold every second step
     if (MB == 4 && c.nb == 2)
@@ -2489,7 +2496,9 @@ __global__ __launch_bounds__(256, 2) void bf16_batch_kernel(const BatchDesc* __r
 // The V tensors of a batch: one tile is (instance, 16-row block, 256-column
 // block) with all three components of the Jastrow gradient, which share r12,
 // its rsqrt, the gradient scale and the B fragments -- the tile of the
-// single-instance V kernel, so results match it bit for bit.
+// single-instance V kernel, so results match it bit for bit (below
+// kVLongWork units per instance, where the single call folds every kVFold
+// steps as this kernel does).
 template <int NPROD>
 __global__ __launch_bounds__(256, (bf16_min_blocks<KernJastrowGrad, NPROD>())) void bf16_batch_v_kernel(
     const BatchDesc* __restrict__ desc, const int64_t* __restrict__ start, int ndesc, int64_t t_end,
