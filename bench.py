@@ -39,8 +39,8 @@ TRAFFIC_BYTES = {"bf16x4": 2.205e9 + 1.122e8, "bf16x5": 2.207e9 + 1.694e8, "fp64
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--precision", default="bf16x4", choices=["bf16x4", "bf16x5", "bf16x3", "fp32", "fp64"])
     p.add_argument("--fp64-steps", type=int, default=2)
     p.add_argument("--storage", default="fp32", choices=["fp64", "fp32"],
