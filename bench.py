@@ -74,8 +74,8 @@ def parse():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the multi-GPU design is described in DESIGN.md "
-                                   "(r1-row sharding, one all-reduce of nb^2 values per SCF iteration)",
+        print(json.dumps({"error": "this bench measures one GPU; a multi-GPU run would shard the r1 rows "
+                                   "(one all-reduce of nb^2 values per SCF iteration), which is not built",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     import torch
